@@ -1,0 +1,217 @@
+"""Benchmark: MLUPS and HBM-roofline fraction of the fused D3Q19 step.
+
+Workload: periodic Taylor-Green vortex on 256^3 cells (LBM(254,254,254),
+every cell TYPE_F), u0 = 0.04, Re = 1600, D3Q19 BGK.  `value` is FP32-storage
+(f - w_i) MLUPS with the lattice resident in HBM; the FP16-shifted storage
+rate and both storages' relative L2 errors against an FP64-storage run of
+the same steps are reported beside it.
+
+One JSON line on stdout:
+  value      MLUPS = fluid cells * K / (event-timed seconds of K fused launches)
+  roofline   achieved = 153 B/cell * cells / mean launch time (FP32 A/B pull:
+             19 loads + 19 stores of 4 B + 1 B mask) against 8 TB/s
+  cpu_baseline  the CPU oracle (the reference's OpenMP step restated: AoS
+             368-B cells, three sweeps) timed on this host on a few steps of
+             the same 256^3 workload
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "lattice-boltzmann-method---graduate-fsi-_amd")
+sys.path.insert(0, PKG)
+
+import cases  # noqa: E402
+import lbm_amd  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "MLUPS and % HBM roofline, 256³ D3Q19, FP32 vs FP16-storage at stated L2 err"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_rate(case, storage, warmup, steps):
+    lat = lbm_amd.Lattice(case.Nx, case.Ny, case.Nz, case.tau, storage=storage)
+    lat.upload(**case.fields())
+    lat.init()
+    if warmup:
+        lat.step(warmup)
+    code = lat.sync()
+    if code:
+        raise RuntimeError(f"device status {code} during warmup")
+    t0 = time.perf_counter()
+    ms = lat.time_steps(steps)
+    code = lat.sync()
+    wall = time.perf_counter() - t0
+    if code:
+        raise RuntimeError(f"device status {code} during timed steps")
+    n = lat.nfluid
+    bpc = lat.bytes_per_cell
+    lat.close()
+    return dict(ms=ms, wall_s=wall, nfluid=n, bytes_per_cell=bpc,
+                mlups=n * steps / (ms / 1e3) / 1e6, gbs=bpc * n * steps / (ms / 1e3) / 1e9)
+
+
+def precision_study(case, nsteps):
+    """Relative L2 of u and of rho-1 over TYPE_F cells after nsteps, FP32 and
+    FP16S storage against FP64 storage (GPU FP64 is itself checked against
+    the CPU oracle to 1e-12 in the test suite)."""
+    out = {}
+    fields = {}
+    for st in ("f64", "f32", "f16s"):
+        lat = lbm_amd.Lattice(case.Nx, case.Ny, case.Nz, case.tau, storage=st)
+        lat.upload(**case.fields())
+        lat.init()
+        ke0 = lat.reduce(lbm_amd.REDUCE_KINETIC)
+        lat.step(nsteps)
+        code = lat.sync()
+        rho, ux, uy, uz = lat.macro()
+        ke1 = lat.reduce(lbm_amd.REDUCE_KINETIC)
+        lat.close()
+        fields[st] = (rho, ux, uy, uz)
+        out[st] = dict(status=code, ke_ratio=ke1 / ke0)
+    r = fields["f64"]
+    num_u = lambda a: sum(float(((x - y) ** 2).sum()) for x, y in zip(a[1:], r[1:]))  # noqa: E731
+    den_u = sum(float((y ** 2).sum()) for y in r[1:])
+    for st in ("f32", "f16s"):
+        a = fields[st]
+        out[st]["l2_u"] = float(np.sqrt(num_u(a) / den_u))
+        out[st]["l2_rho"] = float(np.sqrt(((a[0] - r[0]) ** 2).sum() / (r[0] ** 2).sum()))
+        out[st]["l2_rho_minus_1"] = float(
+            np.sqrt(((a[0] - r[0]) ** 2).sum() / ((r[0] - 1.0) ** 2).sum()))
+    return out
+
+
+def cpu_baseline(case, budget_s, max_steps):
+    """The CPU oracle on a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # the CPU checker/baseline
+
+    threads = os.cpu_count() or 1
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    oracle.set_threads(threads)
+    o = oracle.Oracle(case.Nx, case.Ny, case.Nz, case.tau)
+    o.set_cells(**case.fields())
+    o.init()
+    o.run(1)  # first-touch / warm-up
+    n = 0
+    t0 = time.perf_counter()
+    while n < max_steps and (time.perf_counter() - t0) < budget_s:
+        e = o.run(1)
+        if e:
+            raise RuntimeError(f"oracle status {e}")
+        n += 1
+    dt = time.perf_counter() - t0
+    o.close()
+    nfl = case.nfluid
+    return dict(value=nfl * n / dt / 1e6, unit="MLUPS", cores=threads, kind="port",
+                sample=f"TGV 256^3 FP64, {n} steps after 1 warm-up step ({dt:.1f} s); "
+                       "oracle/lbm_oracle.c -O3 -fopenmp, AoS 368-B cells, 3 sweeps/step")
+
+
+def load_traffic(storage):
+    p = os.path.join(ROOT, "profiles", f"stream_collide_tgv256_{storage}_traffic.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--precision-steps", type=int, default=5000)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-max-steps", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--storage", default="f32", choices=["f64", "f32", "f16s"])
+    a = ap.parse_args()
+
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the multi-GPU path (z-slab halo exchange) is a "
+              "design, not a build; run with --gpus 1", file=sys.stderr)
+        return 2
+
+    case = cases.taylor_green(a.size, u0=0.04, Re=1600.0)
+    log(f"bench: TGV {a.size}^3, tau={case.tau:.6f}, warmup={a.warmup}, steps={a.steps}")
+    main_r = run_rate(case, a.storage, a.warmup, a.steps)
+    log(f"bench: {a.storage} {main_r['mlups']:.0f} MLUPS, {main_r['gbs']:.0f} GB/s")
+    other = "f16s" if a.storage != "f16s" else "f32"
+    oth_r = run_rate(case, other, a.warmup, a.steps)
+    log(f"bench: {other} {oth_r['mlups']:.0f} MLUPS, {oth_r['gbs']:.0f} GB/s")
+
+    prec = None
+    if not a.no_precision:
+        log(f"bench: precision study, {a.precision_steps} steps x 3 storages")
+        prec = precision_study(case, a.precision_steps)
+        log(f"bench: precision {json.dumps(prec)}")
+
+    cpu = None
+    if not a.no_cpu:
+        log("bench: CPU baseline (oracle, OpenMP)")
+        cpu = cpu_baseline(case, a.cpu_seconds, a.cpu_max_steps)
+        log(f"bench: cpu {cpu['value']:.2f} MLUPS on {cpu['cores']} threads")
+
+    ms_step = main_r["ms"] / a.steps
+    achieved = main_r["bytes_per_cell"] * main_r["nfluid"] / (ms_step / 1e3) / 1e9
+    traffic = load_traffic(a.storage)
+    out = {
+        "metric": METRIC,
+        "value": round(main_r["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"f64": "f64", "f32": "f32", "f16s": "f32"}[a.storage],
+        "data": "synthetic",
+        "config": {
+            "workload": f"periodic Taylor-Green vortex {a.size}^3 D3Q19 BGK, u0=0.04, Re=1600",
+            "storage": {"f64": "FP64", "f32": "FP32 (f - w_i)", "f16s": "FP16 (f - w_i)*2^12"}[a.storage],
+            "cells": main_r["nfluid"],
+            "tau": case.tau,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "bytes_per_cell": main_r["bytes_per_cell"],
+        },
+        other: {
+            "mlups": round(oth_r["mlups"], 1),
+            "achieved_gbs": round(oth_r["gbs"], 1),
+            "frac": round(oth_r["gbs"] / HBM_PEAK_GBS, 4),
+            "bytes_per_cell": oth_r["bytes_per_cell"],
+            "speedup_vs_main": round(oth_r["mlups"] / main_r["mlups"], 3),
+        },
+        "precision": prec,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

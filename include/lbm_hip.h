@@ -1,0 +1,138 @@
+/*
+ * lbm_hip.h -- C-ABI of the MI355X (gfx950) lattice Boltzmann step.
+ *
+ * The reference exposes its step as the C++ class LBM (LBM.h:620-664) plus
+ * the free function IBLBM (LBM.h:687).  It has no FFI of its own; this header
+ * is the boundary that class calls through.  Every entry point below names
+ * the reference member or function whose work it takes over.  Plain C types
+ * only: an opaque context, pointers and sizes.
+ *
+ * Host-array convention: cell (i,j,k) of the padded (Nx+2)(Ny+2)(Nz+2)
+ * lattice is element (i*(Ny+2)+j)*(Nz+2)+k (k fastest), the order of the
+ * reference's LBM::fluid[i][j][k] (lbm.cpp:17-28).  Populations of a cell are
+ * 19 consecutive doubles in the reference's D3Q19 direction order
+ * (LBM.h:546-552).  All host arrays are caller-owned and copied synchronously.
+ *
+ * Return codes: 0 = OK; 1/2/3 = the reference's three std::terminate() sites
+ * (negative equilibrium lbm.cpp:114-117, density outside [0,100]
+ * lbm.cpp:772-775, temperature outside [0,100] lbm.cpp:794-797); negative =
+ * argument, state or HIP errors.  A context is used by one host thread; all
+ * of its device work runs in order on the context's own HIP stream.
+ */
+#ifndef LBM_HIP_H
+#define LBM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_OK 0
+#define LBM_ERR_FEQ_NEGATIVE 1
+#define LBM_ERR_DENSITY_RANGE 2
+#define LBM_ERR_TEMPERATURE_RANGE 3
+#define LBM_ERR_ARG (-1)
+#define LBM_ERR_STATE (-2)
+#define LBM_ERR_HIP (-3)
+#define LBM_ERR_NOMEM (-4)
+#define LBM_ERR_UNSUPPORTED (-5)
+
+/* cell-type codes, LBM.h:505-515 */
+#define LBM_TYPE_F 0
+#define LBM_TYPE_S 1
+#define LBM_TYPE_OUT 2
+#define LBM_TYPE_IN 3
+#define LBM_TYPE_P 4
+#define LBM_TYPE_L 5
+#define LBM_TYPE_T 6
+#define LBM_TYPE_Q 7
+
+/* population storage precision (collision arithmetic is FP64 for F64 and
+ * FP32 otherwise) */
+#define LBM_STORE_F64 0  /* f as double                                   */
+#define LBM_STORE_F32 1  /* f - w_i as float                              */
+#define LBM_STORE_F16S 2 /* (f - w_i) * 2^LBM_F16S_SCALE_LOG2 as IEEE half */
+#define LBM_F16S_SCALE_LOG2 12
+
+/* lbm_reduce_fp64 quantities */
+#define LBM_REDUCE_MASS 0     /* sum of rho over TYPE_F cells           */
+#define LBM_REDUCE_MOMENTUM 1 /* out[0..2] = sum of rho*u over TYPE_F   */
+#define LBM_REDUCE_KINETIC 2  /* sum of rho*|u|^2/2 over TYPE_F         */
+
+typedef struct lbm_ctx lbm_ctx;
+
+typedef struct lbm_desc {
+  int32_t nx, ny, nz; /* interior size, as LBM(Nx,Ny,Nz,tau) (lbm.cpp:17)   */
+  int32_t lattice;    /* 19 = D3Q19 (the only lattice of this library)      */
+  int32_t storage;    /* LBM_STORE_*                                        */
+  int32_t device;     /* HIP device ordinal                                 */
+  double tau;         /* relaxation time; omega = dt/tau (lbm.cpp:76)       */
+  int32_t reserved[8];
+} lbm_desc;
+
+/* LBM::LBM(Nx,Ny,Nz,tau), lbm.cpp:17-28: allocates the device lattice. */
+int lbm_create(const lbm_desc *desc, lbm_ctx **out);
+/* The reference never frees its lattice; this releases everything. */
+int lbm_destroy(lbm_ctx *ctx);
+
+/* The setup writes to lb.fluid[i][j][k].{type,density,ux,uy,uz,F}
+ * (e.g. lbm.cpp:1833-1852).  Any pointer may be NULL: type defaults to
+ * TYPE_F, rho to 1, u and F to 0.  Takes effect at the next lbm_init. */
+int lbm_upload_cells(lbm_ctx *ctx, const int16_t *type, const double *rho,
+                     const double *ux, const double *uy, const double *uz,
+                     const double *Fx, const double *Fy, const double *Fz);
+
+/* LBM::Init, lbm.cpp:31-57: f = f_star = feq(rho,u) in every cell. */
+int lbm_init(lbm_ctx *ctx);
+/* LBM::Collision + ConvLBM, lbm.cpp:60-179 (TYPE_F cells, BGK + Guo). */
+int lbm_collide(lbm_ctx *ctx);
+/* LBM::Streaming, lbm.cpp:182-295 (pull form, half-way bounce-back with
+ * the per-cell wall velocity of lbm.cpp:941-962, frozen links). */
+int lbm_stream(lbm_ctx *ctx);
+/* LBM::MacroProp, lbm.cpp:657-812. */
+int lbm_macro(lbm_ctx *ctx);
+/* main.cpp:212-221 repeated nsteps times (Collision, Streaming, MacroProp)
+ * as one fused stream-collide launch per step.  emit_macro != 0 writes the
+ * last step's rho,u to the device macro arrays (lbm_download_macro). */
+int lbm_step(lbm_ctx *ctx, int32_t nsteps, int32_t emit_macro);
+/* Waits for the context's stream; returns the sticky status (see codes). */
+int lbm_sync(lbm_ctx *ctx);
+int lbm_status(lbm_ctx *ctx, int64_t *first_bad_cell);
+
+/* Reads of lb.fluid[i][j][k].{density,ux,uy,uz} as the reference holds them
+ * after MacroProp (lbm.cpp:672-675).  Any pointer may be NULL. */
+int lbm_download_macro(lbm_ctx *ctx, double *rho, double *ux, double *uy,
+                       double *uz);
+/* Reads of lb.fluid[i][j][k].f (post-streaming populations), ncells*19. */
+int lbm_download_f(lbm_ctx *ctx, double *f);
+/* Reads of lb.fluid[i][j][k].F (the force used by the next collision). */
+int lbm_download_force(lbm_ctx *ctx, double *Fx, double *Fy, double *Fz);
+
+/* FP64 deterministic reductions over TYPE_F cells (no reference
+ * counterpart; the reference never sums mass). */
+int lbm_reduce_fp64(lbm_ctx *ctx, int32_t what, double *out);
+
+/* The streaming tables the device kernels use, for bit-exact comparison with
+ * lbm.cpp:186-295: per cell and direction m, src = linear index of the
+ * cell x - c_m (wrapped modulo N+2 per axis, lbm.cpp:197-199) and
+ * kind = 0 stream, 1 bounce-back, 2 frozen; non-fluid cells give src = -1,
+ * kind = 255.  type receives the cell-type mask.  Arrays: ncells*19. */
+int lbm_export_links(lbm_ctx *ctx, int16_t *type, int32_t *src,
+                     uint8_t *kind);
+
+/* sizes and identification */
+int64_t lbm_ncells(const lbm_ctx *ctx);
+int64_t lbm_nfluid(const lbm_ctx *ctx);
+int32_t lbm_storage(const lbm_ctx *ctx);
+/* Timed run of nsteps fused launches bracketed by HIP events on the
+ * context's stream; *ms receives the elapsed milliseconds. */
+int lbm_time_steps(lbm_ctx *ctx, int32_t nsteps, double *ms);
+/* Bytes one fused launch moves per fluid cell (algorithmic). */
+double lbm_bytes_per_cell(const lbm_ctx *ctx);
+const char *lbm_error_string(int code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,93 @@
+"""Build the gfx950 library liblbm_hip.so in-tree with hipcc.
+
+Usage: python build.py [--jobs N]
+
+Every HIP source is compiled for gfx950 only, with packed-FP32 instruction
+selection disabled, into _build/, then linked into liblbm_hip.so next to this
+file.  The C++ drop-in host layer (host/) is compiled into the same library.
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "liblbm_hip.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+COMMON = [
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-Wall",
+    "-Wno-unused-function",
+    f"-I{os.path.join(ROOT, 'include')}",
+    f"-I{os.path.join(HERE, 'csrc')}",
+    f"-I{os.path.join(HERE, 'host')}",
+]
+# gfx950 code objects; packed FP32 ops (v_pk_*_f32) are not generated
+DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+SOURCES = [
+    os.path.join(HERE, "csrc", "lbm_kernels.hip"),
+    os.path.join(HERE, "csrc", "lbm_capi.hip"),
+]
+HOST_SOURCES = [
+    os.path.join(HERE, "host", "LBM.cpp"),
+]
+
+
+def _run(cmd: list[str]) -> None:
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(
+        ln for ln in r.stdout.splitlines() if "packed-fp32-ops" not in ln and ln.strip()
+    )
+    if r.returncode != 0:
+        raise RuntimeError(f"command failed ({r.returncode}): {' '.join(cmd)}\n{out}")
+    if out and os.environ.get("LBM_BUILD_VERBOSE"):
+        print(out)
+
+
+def _obj(src: str) -> str:
+    return os.path.join(BUILD, os.path.basename(src) + ".o")
+
+
+def build(jobs: int = 4) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    cmds = []
+    for s in SOURCES:
+        cmds.append([HIPCC, *COMMON, *DEVICE, "-c", s, "-o", _obj(s)])
+    for s in HOST_SOURCES:
+        if os.path.exists(s):
+            cmds.append([HIPCC, *COMMON, "-x", "c++", "-c", s, "-o", _obj(s)])
+    with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        list(ex.map(_run, cmds))
+    objs = [_obj(s) for s in SOURCES] + [_obj(s) for s in HOST_SOURCES if os.path.exists(s)]
+    tmp = LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle() -> None:
+    """The CPU oracle (test infrastructure) under oracle/."""
+    _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--jobs", type=int, default=4)
+    a = ap.parse_args()
+    print(build(a.jobs))
+    build_oracle()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,246 @@
+"""Runtime case set-ups: the role of the reference's main_setup_* functions.
+
+The reference hard-codes its cases behind compile-time #defines
+(LBM.h:8-72, lbm.cpp:1429-2031).  Here each case is a function returning the
+fields a setup writes into lb.fluid[i][j][k] -- type, density, ux, uy, uz and
+F -- as numpy arrays of shape (Nx+2, Ny+2, Nz+2), plus the lattice size and
+tau.  The same arrays feed the GPU library and the CPU oracle, so both start
+from identical (seeded) inputs.
+
+Cell-type precedence follows the reference's setup loops: the later `if`
+wins (e.g. lbm.cpp:1838-1841).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+TYPE_F, TYPE_S, TYPE_OUT, TYPE_IN = 0, 1, 2, 3
+# LBM.h:74 and lbm.cpp:1859 use this literal for pi
+PI_REF = 3.14159265
+
+_MASK64 = (1 << 64) - 1
+_GOLDEN = 0x9E3779B97F4A7C15
+
+
+def splitmix64(x):
+    """splitmix64 finaliser (Steele, Lea & Flood 2014) on uint64 numpy arrays."""
+    x = np.asarray(x, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = x + np.uint64(_GOLDEN)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def counter_uniform(seed: int, counter) -> np.ndarray:
+    """Counter-based uniform draws in [-1, 1): u = splitmix64(seed*G + counter)."""
+    c = np.asarray(counter, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        base = np.uint64((seed * _GOLDEN) & _MASK64)
+        z = splitmix64(base + c)
+    return (z >> np.uint64(11)).astype(np.float64) * (2.0 ** -53) * 2.0 - 1.0
+
+
+@dataclass
+class Case:
+    name: str
+    Nx: int
+    Ny: int
+    Nz: int
+    tau: float
+    type: np.ndarray
+    rho: np.ndarray
+    ux: np.ndarray
+    uy: np.ndarray
+    uz: np.ndarray
+    Fx: np.ndarray | None = None
+    Fy: np.ndarray | None = None
+    Fz: np.ndarray | None = None
+    params: dict = field(default_factory=dict)
+
+    @property
+    def shape(self):
+        return (self.Nx + 2, self.Ny + 2, self.Nz + 2)
+
+    def fields(self):
+        return dict(type=self.type, rho=self.rho, ux=self.ux, uy=self.uy, uz=self.uz,
+                    Fx=self.Fx, Fy=self.Fy, Fz=self.Fz)
+
+    @property
+    def nfluid(self) -> int:
+        return int((self.type == TYPE_F).sum())
+
+
+def _blank(Nx, Ny, Nz):
+    shape = (Nx + 2, Ny + 2, Nz + 2)
+    return (np.zeros(shape, np.int16), np.ones(shape), np.zeros(shape), np.zeros(shape),
+            np.zeros(shape))
+
+
+def cavity(N: int = 64, u_lid: float = 0.05, Re: float = 100.0, Ny: int | None = None,
+           Nz: int | None = None) -> Case:
+    """Lid-driven cavity LBM(N,N,N): all six ghost faces TYPE_S, the whole
+    j = Ny+1 face moves with ux = u_lid (the moving-wall pattern of the
+    Couette setup, lbm.cpp:1536-1584), rho = 1 and u = 0 elsewhere.
+    nu = u_lid*N/Re, tau = 0.5 + nu/cs2 (LBM.h:100-101)."""
+    Nx = N
+    Ny = N if Ny is None else Ny
+    Nz = N if Nz is None else Nz
+    t, rho, ux, uy, uz = _blank(Nx, Ny, Nz)
+    t[0, :, :] = TYPE_S
+    t[-1, :, :] = TYPE_S
+    t[:, 0, :] = TYPE_S
+    t[:, -1, :] = TYPE_S
+    t[:, :, 0] = TYPE_S
+    t[:, :, -1] = TYPE_S
+    ux[:, -1, :] = u_lid
+    nu = u_lid * N / Re
+    tau = 0.5 + 3.0 * nu
+    return Case("cavity", Nx, Ny, Nz, tau, t, rho, ux, uy, uz,
+                params=dict(u_lid=u_lid, Re=Re, nu=nu))
+
+
+def taylor_green(Np: int = 256, u0: float = 0.04, Re: float = 1600.0) -> Case:
+    """Periodic Taylor-Green vortex on a Np^3 periodic box.  The reference
+    wraps modulo N+2 over its padded array (lbm.cpp:197-199), so the box is
+    LBM(Np-2, Np-2, Np-2) with every cell TYPE_F.  k = 2 pi / Np,
+    u = u0 sin kx cos ky cos kz, v = -u0 cos kx sin ky cos kz, w = 0,
+    p = rho0 u0^2/16 (cos 2kx + cos 2ky)(cos 2kz + 2), rho = 1 + p/cs2,
+    nu = u0 Np / (2 pi) / Re."""
+    n = Np - 2
+    k = 2.0 * math.pi / Np
+    x = np.arange(Np, dtype=np.float64)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    ux = u0 * np.sin(k * X) * np.cos(k * Y) * np.cos(k * Z)
+    uy = -u0 * np.cos(k * X) * np.sin(k * Y) * np.cos(k * Z)
+    uz = np.zeros_like(ux)
+    p = u0 * u0 / 16.0 * (np.cos(2 * k * X) + np.cos(2 * k * Y)) * (np.cos(2 * k * Z) + 2.0)
+    rho = 1.0 + 3.0 * p
+    t = np.zeros(ux.shape, np.int16)
+    nu = u0 * Np / (2.0 * math.pi) / Re
+    tau = 0.5 + 3.0 * nu
+    return Case("taylor_green", n, n, n, tau, t, rho, ux, uy, uz,
+                params=dict(u0=u0, Re=Re, nu=nu, k=k))
+
+
+def channel(Nx: int = 256, Ny: int = 128, Nz: int = 64, u_in: float = 0.04, Re: float = 100.0,
+            D: float = 20.0, seed: int = 1, perturb: float = 0.01) -> Case:
+    """Channel in the pattern of main_setup_Cylinder_IBM (lbm.cpp:1828-1855):
+    i = 0 TYPE_IN, i = Nx+1 TYPE_OUT, then the j and k ghost faces TYPE_S
+    (walls override inlet/outlet on the edges).  F/IN/OUT cells start at
+    rho = 1, u = u_in; TYPE_F cells carry a seeded +-perturb relative
+    perturbation: u = u_in(1 + p xi_x), v = p u_in xi_y, w = p u_in xi_z with
+    xi = counter_uniform(seed, 3*cell + component)."""
+    t, rho, ux, uy, uz = _blank(Nx, Ny, Nz)
+    t[0, :, :] = TYPE_IN
+    t[-1, :, :] = TYPE_OUT
+    t[:, 0, :] = TYPE_S
+    t[:, -1, :] = TYPE_S
+    t[:, :, 0] = TYPE_S
+    t[:, :, -1] = TYPE_S
+    live = t != TYPE_S
+    ux[live] = u_in
+    if perturb:
+        n = t.size
+        cell = np.arange(n, dtype=np.uint64).reshape(t.shape)
+        xi = [counter_uniform(seed, 3 * cell + np.uint64(c)) for c in range(3)]
+        f = t == TYPE_F
+        ux[f] = u_in * (1.0 + perturb * xi[0][f])
+        uy[f] = perturb * u_in * xi[1][f]
+        uz[f] = perturb * u_in * xi[2][f]
+    nu = u_in * D / Re
+    tau = 0.5 + 3.0 * nu
+    return Case("channel", Nx, Ny, Nz, tau, t, rho, ux, uy, uz,
+                params=dict(u_in=u_in, Re=Re, D=D, nu=nu, seed=seed))
+
+
+def couette(Nx: int = 2, Ny: int = 30, Nz: int = 2, tau: float = 0.9, u_wall: float = 0.01) -> Case:
+    """main_setup_Couette (lbm.cpp:1536-1584): x and z periodic through TYPE_F
+    ghosts, j = 0 and j = Ny+1 TYPE_S, top wall ux = u_wall."""
+    t, rho, ux, uy, uz = _blank(Nx, Ny, Nz)
+    t[:, 0, :] = TYPE_S
+    t[:, -1, :] = TYPE_S
+    ux[:, -1, :] = u_wall
+    return Case("couette", Nx, Ny, Nz, tau, t, rho, ux, uy, uz, params=dict(u_wall=u_wall))
+
+
+def poiseuille(Nx: int = 2, Ny: int = 30, Nz: int = 2, tau: float = 0.9, g: float = 1e-6) -> Case:
+    """Channel between TYPE_S walls at j = 0, Ny+1 (main_setup_Poiseuille,
+    lbm.cpp:1495-1530) driven by a uniform body force Fx = g in every cell."""
+    t, rho, ux, uy, uz = _blank(Nx, Ny, Nz)
+    t[:, 0, :] = TYPE_S
+    t[:, -1, :] = TYPE_S
+    Fx = np.full(t.shape, g)
+    return Case("poiseuille", Nx, Ny, Nz, tau, t, rho, ux, uy, uz, Fx=Fx,
+                Fy=np.zeros(t.shape), Fz=np.zeros(t.shape), params=dict(g=g))
+
+
+def random_mask_case(Nx: int, Ny: int, Nz: int, seed: int = 7, p_solid: float = 0.15,
+                     p_other: float = 0.05, u_amp: float = 0.03, tau: float = 0.8) -> Case:
+    """A lattice with random solid / inlet-outlet / fluid cells, random moving
+    solid velocities and random fluid velocities.  Exercises every streaming
+    branch (stream, bounce-back with averaged wall velocity, frozen link) and
+    the periodic wrap of the ghost shell."""
+    rng = np.random.default_rng(seed)
+    shape = (Nx + 2, Ny + 2, Nz + 2)
+    r = rng.random(shape)
+    t = np.zeros(shape, np.int16)
+    t[r < p_solid] = TYPE_S
+    t[(r >= p_solid) & (r < p_solid + p_other)] = np.int16(TYPE_IN)
+    rho = 1.0 + 0.01 * rng.standard_normal(shape)
+    ux = u_amp * rng.standard_normal(shape)
+    uy = u_amp * rng.standard_normal(shape)
+    uz = u_amp * rng.standard_normal(shape)
+    # half of the solids are stationary
+    still = (t == TYPE_S) & (rng.random(shape) < 0.5)
+    ux[still] = uy[still] = uz[still] = 0.0
+    return Case("random_mask", Nx, Ny, Nz, tau, t, rho, ux, uy, uz, params=dict(seed=seed))
+
+
+def cylinder_markers(center_x: float, center_y: float, D: float, n_theta: int,
+                     z_levels) -> dict:
+    """Markers on a cylinder of diameter D along z, in the pattern of
+    Generate_Cylinder_Marker (lbm.cpp:1857-1877): x = r cos(2 pi n/N) +
+    center_x + 0.5, y = r sin(2 pi n/N) + center_y + 0.5 (PI = 3.14159265),
+    counter-clockwise, one ring per z level.  Normals follow
+    MARKER::Normal_Marker (lbm.cpp:1048-1069): t1 = unit chord to the next
+    marker of the ring, n = t1 x z_hat.  ds = chord length to the next
+    marker (main.cpp:123-131) times the axial spacing (1 for a single 2D
+    ring)."""
+    z_levels = np.atleast_1d(np.asarray(z_levels, dtype=np.float64))
+    r = D / 2.0
+    a = 2 * PI_REF / n_theta * np.arange(n_theta)
+    x = r * np.cos(a) + center_x + 0.5
+    y = r * np.sin(a) + center_y + 0.5
+    x2, y2 = np.roll(x, -1), np.roll(y, -1)
+    dx_, dy_ = x2 - x, y2 - y
+    norm = np.sqrt(dx_ ** 2 + dy_ ** 2)
+    t1x, t1y = dx_ / norm, dy_ / norm
+    # (t1x, t1y, 0) x (0, 0, 1) = (t1y, -t1x, 0)
+    nxr, nyr = t1y, -t1x
+    chord = np.sqrt((x - x2) ** 2 + (y - y2) ** 2)
+    dz = 1.0 if z_levels.size < 2 else float(z_levels[1] - z_levels[0])
+    nz_ = z_levels.size
+    return dict(
+        x=np.tile(x, nz_), y=np.tile(y, nz_), z=np.repeat(z_levels, n_theta),
+        nx=np.tile(nxr, nz_), ny=np.tile(nyr, nz_), nz=np.zeros(n_theta * nz_),
+        ds=np.tile(chord * dz, nz_),
+    )
+
+
+def batch_cavities(count: int = 96, seed: int = 1, edges=(32, 48, 64, 96, 128),
+                   u_range=(0.02, 0.08), re_range=(50.0, 1000.0)):
+    """The seeded cavity ensemble: edge from `edges`, lid velocity and Re
+    uniform in their ranges, all drawn with counter_uniform(seed, .)."""
+    out = []
+    for b in range(count):
+        xi = (counter_uniform(seed, np.arange(3, dtype=np.uint64) + np.uint64(3 * b)) + 1.0) / 2.0
+        e = edges[min(int(xi[0] * len(edges)), len(edges) - 1)]
+        u = u_range[0] + (u_range[1] - u_range[0]) * xi[1]
+        re = re_range[0] + (re_range[1] - re_range[0]) * xi[2]
+        out.append(dict(N=int(e), u_lid=float(u), Re=float(re)))
+    return out

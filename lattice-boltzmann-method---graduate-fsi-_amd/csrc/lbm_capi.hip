@@ -1,0 +1,556 @@
+// lbm_capi.hip -- the C-ABI of include/lbm_hip.h.
+//
+// A context owns one device lattice: two population buffers (A/B), the cell
+// types, the kernel mask, the bounce-back link words, the wall velocities,
+// the macroscopic arrays and the force.  Host arrays at the boundary are in
+// the reference's [i][j][k] order (k fastest); device arrays are x fastest.
+//
+// State machine (mirrors where the reference's data is between its calls):
+//   ST_F     the current buffer holds f (after Init, Streaming or a step)
+//   ST_FSTAR the current buffer holds f* (after Collision)
+// LBM::Collision needs ST_F, LBM::Streaming needs ST_FSTAR.  lbm_step from
+// ST_F runs one collision, then nsteps-1 fused stream+macro+collide launches,
+// then one stream+macro launch, and ends in ST_F with rho,u of the last step
+// in the macro arrays -- the same data the reference holds after MacroProp.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "../../include/lbm_hip.h"
+#include "lbm_launch.h"
+
+using namespace lbm;
+
+namespace {
+
+enum { ST_NONE = 0, ST_F = 1, ST_FSTAR = 2 };
+
+}  // namespace
+
+struct lbm_ctx {
+  lbm_desc d{};
+  Geo g{};
+  int storage = 0;
+  size_t esize = 8;  // bytes per stored population
+  size_t csize = 8;  // bytes per macro/force value
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  void *cur = nullptr, *oth = nullptr;
+  int16_t *type = nullptr;
+  uint8_t *mask = nullptr;
+  uint32_t *links = nullptr;
+  void *uw[3] = {nullptr, nullptr, nullptr};
+  void *mac[4] = {nullptr, nullptr, nullptr, nullptr};  // rho, ux, uy, uz
+  void *F[3] = {nullptr, nullptr, nullptr};
+  int force_mode = 0;
+  double Fu[3] = {0, 0, 0};
+  int *status = nullptr;
+  unsigned long long *first_bad = nullptr;
+  double *red_partial = nullptr, *red_out = nullptr;
+  int state = ST_NONE;
+  int uploaded = 0;
+  // The reference's first Streaming reads the solids' setup velocities; from
+  // the first MacroProp on (lbm.cpp:672-675 runs over solids too) it reads
+  // their moments.  Set while the wall tables still hold the setup values.
+  int walls_from_setup = 0;
+  int64_t nfluid = 0;
+  int block = 256;
+  std::vector<int16_t> h_type;  // device order
+};
+
+#define HIPCHK(x)                              \
+  do {                                         \
+    hipError_t e_ = (x);                       \
+    if (e_ != hipSuccess) return LBM_ERR_HIP;  \
+  } while (0)
+
+static LaunchCtx launch_ctx(const lbm_ctx *c) {
+  LaunchCtx L;
+  memset(&L, 0, sizeof L);
+  L.g = c->g;
+  L.storage = c->storage;
+  L.force_mode = c->force_mode;
+  L.block = c->block;
+  L.omega = 1.0 / c->d.tau;  // omega = dt/tau, lbm.cpp:76
+  L.Fu[0] = c->Fu[0];
+  L.Fu[1] = c->Fu[1];
+  L.Fu[2] = c->Fu[2];
+  L.Fx = c->F[0];
+  L.Fy = c->F[1];
+  L.Fz = c->F[2];
+  L.links = c->links;
+  L.uwx = c->uw[0];
+  L.uwy = c->uw[1];
+  L.uwz = c->uw[2];
+  L.rho = c->mac[0];
+  L.ux = c->mac[1];
+  L.uy = c->mac[2];
+  L.uz = c->mac[3];
+  L.mask = c->mask;
+  L.macro_sel = 1 | 16;  // TYPE_F and TYPE_S cells
+  L.status = c->status;
+  L.first_bad = c->first_bad;
+  L.stream = c->stream;
+  return L;
+}
+
+// reference order r = (i*ny + j)*nz + k  <->  device order d = (k*ny + j)*nx + i
+template <class F>
+static void for_cells(const lbm_ctx *c, F &&f) {
+  const uint32_t nx = c->g.nx, ny = c->g.ny, nz = c->g.nz;
+  for (uint32_t i = 0; i < nx; i++)
+    for (uint32_t j = 0; j < ny; j++)
+      for (uint32_t k = 0; k < nz; k++) {
+        const size_t r = ((size_t)i * ny + j) * nz + k;
+        const size_t d = ((size_t)k * ny + j) * nx + i;
+        f(r, d);
+      }
+}
+
+static int upload_real(lbm_ctx *c, void *dst, const double *src, double dflt) {
+  const size_t n = c->g.ncell;
+  if (c->csize == 8) {
+    std::vector<double> h(n, dflt);
+    if (src) for_cells(c, [&](size_t r, size_t d) { h[d] = src[r]; });
+    HIPCHK(hipMemcpy(dst, h.data(), n * 8, hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> h(n, (float)dflt);
+    if (src) for_cells(c, [&](size_t r, size_t d) { h[d] = (float)src[r]; });
+    HIPCHK(hipMemcpy(dst, h.data(), n * 4, hipMemcpyHostToDevice));
+  }
+  return LBM_OK;
+}
+
+static int download_real(lbm_ctx *c, double *dst, const void *src) {
+  const size_t n = c->g.ncell;
+  if (c->csize == 8) {
+    std::vector<double> h(n);
+    HIPCHK(hipMemcpy(h.data(), src, n * 8, hipMemcpyDeviceToHost));
+    for_cells(c, [&](size_t r, size_t d) { dst[r] = h[d]; });
+  } else {
+    std::vector<float> h(n);
+    HIPCHK(hipMemcpy(h.data(), src, n * 4, hipMemcpyDeviceToHost));
+    for_cells(c, [&](size_t r, size_t d) { dst[r] = (double)h[d]; });
+  }
+  return LBM_OK;
+}
+
+static void free_all(lbm_ctx *c) {
+  void *ptrs[] = {c->cur, c->oth, c->type, c->mask, c->links, c->uw[0], c->uw[1], c->uw[2],
+                  c->mac[0], c->mac[1], c->mac[2], c->mac[3], c->F[0], c->F[1], c->F[2],
+                  c->status, c->first_bad, c->red_partial, c->red_out};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+extern "C" {
+
+const char *lbm_error_string(int code) {
+  switch (code) {
+    case LBM_OK: return "ok";
+    case LBM_ERR_FEQ_NEGATIVE: return "negative equilibrium in collision (lbm.cpp:114-117)";
+    case LBM_ERR_DENSITY_RANGE: return "density outside [0,100] (lbm.cpp:772-775)";
+    case LBM_ERR_TEMPERATURE_RANGE: return "temperature outside [0,100] (lbm.cpp:794-797)";
+    case LBM_ERR_ARG: return "invalid argument";
+    case LBM_ERR_STATE: return "call out of order for the lattice state";
+    case LBM_ERR_HIP: return "HIP runtime error";
+    case LBM_ERR_NOMEM: return "device allocation failed";
+    case LBM_ERR_UNSUPPORTED: return "unsupported configuration";
+  }
+  return "unknown error";
+}
+
+int lbm_create(const lbm_desc *desc, lbm_ctx **out) {
+  if (!desc || !out) return LBM_ERR_ARG;
+  *out = nullptr;
+  if (desc->nx < 1 || desc->ny < 1 || desc->nz < 1) return LBM_ERR_ARG;
+  if (desc->lattice != 19) return LBM_ERR_UNSUPPORTED;
+  if (desc->storage < LBM_STORE_F64 || desc->storage > LBM_STORE_F16S) return LBM_ERR_ARG;
+  if (!(desc->tau > 0.5) || !std::isfinite(desc->tau)) return LBM_ERR_ARG;
+  const uint64_t nx = desc->nx + 2ull, ny = desc->ny + 2ull, nz = desc->nz + 2ull;
+  const uint64_t ncell = nx * ny * nz;
+  const uint64_t ps = (ncell + 255) / 256 * 256;
+  // 32-bit element offsets: 19 planes must stay below 2^32 elements
+  if (ps * 19ull >= (1ull << 32)) return LBM_ERR_UNSUPPORTED;
+
+  lbm_ctx *c = new lbm_ctx();
+  c->d = *desc;
+  c->g.nx = (uint32_t)nx;
+  c->g.ny = (uint32_t)ny;
+  c->g.nz = (uint32_t)nz;
+  c->g.plane = (uint32_t)(nx * ny);
+  c->g.ncell = (uint32_t)ncell;
+  c->g.ps = (uint32_t)ps;
+  c->storage = desc->storage;
+  c->esize = desc->storage == LBM_STORE_F64 ? 8 : (desc->storage == LBM_STORE_F32 ? 4 : 2);
+  c->csize = desc->storage == LBM_STORE_F64 ? 8 : 4;
+  c->dev = desc->device;
+
+  auto fail = [&](int code) {
+    free_all(c);
+    delete c;
+    return code;
+  };
+  if (hipSetDevice(c->dev) != hipSuccess) return fail(LBM_ERR_HIP);
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(LBM_ERR_HIP);
+  const size_t popbytes = c->esize * ps * 19;
+  if (hipMalloc(&c->cur, popbytes) != hipSuccess) return fail(LBM_ERR_NOMEM);
+  if (hipMalloc(&c->oth, popbytes) != hipSuccess) return fail(LBM_ERR_NOMEM);
+  if (hipMalloc((void **)&c->type, ncell * 2) != hipSuccess) return fail(LBM_ERR_NOMEM);
+  if (hipMalloc((void **)&c->mask, ncell) != hipSuccess) return fail(LBM_ERR_NOMEM);
+  for (int q = 0; q < 4; q++)
+    if (hipMalloc(&c->mac[q], ncell * c->csize) != hipSuccess) return fail(LBM_ERR_NOMEM);
+  if (hipMalloc((void **)&c->status, sizeof(int)) != hipSuccess) return fail(LBM_ERR_NOMEM);
+  if (hipMalloc((void **)&c->first_bad, sizeof(unsigned long long)) != hipSuccess)
+    return fail(LBM_ERR_NOMEM);
+  const int nb = reduce_blocks((uint32_t)ncell);
+  if (hipMalloc((void **)&c->red_partial, sizeof(double) * 3 * nb) != hipSuccess) return fail(LBM_ERR_NOMEM);
+  if (hipMalloc((void **)&c->red_out, sizeof(double) * 4) != hipSuccess) return fail(LBM_ERR_NOMEM);
+  int zero = 0;
+  unsigned long long big = ~0ull;
+  if (hipMemcpy(c->status, &zero, sizeof zero, hipMemcpyHostToDevice) != hipSuccess) return fail(LBM_ERR_HIP);
+  if (hipMemcpy(c->first_bad, &big, sizeof big, hipMemcpyHostToDevice) != hipSuccess) return fail(LBM_ERR_HIP);
+  // defaults of LATTICE (LBM.h:560-566): TYPE_F, rho 1, u 0, F 0
+  if (lbm_upload_cells(c, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr) != LBM_OK)
+    return fail(LBM_ERR_HIP);
+  *out = c;
+  return LBM_OK;
+}
+
+int lbm_destroy(lbm_ctx *c) {
+  if (!c) return LBM_ERR_ARG;
+  (void)hipSetDevice(c->dev);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_all(c);
+  delete c;
+  return LBM_OK;
+}
+
+int lbm_upload_cells(lbm_ctx *c, const int16_t *type, const double *rho, const double *ux,
+                     const double *uy, const double *uz, const double *Fx, const double *Fy,
+                     const double *Fz) {
+  if (!c) return LBM_ERR_ARG;
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const size_t n = c->g.ncell;
+  c->h_type.assign(n, (int16_t)LBM_TYPE_F);
+  if (type) {
+    for (size_t r = 0; r < n; r++)
+      if (type[r] < 0 || type[r] > 7) return LBM_ERR_ARG;
+    for_cells(c, [&](size_t r, size_t d) { c->h_type[d] = type[r]; });
+  }
+  c->nfluid = 0;
+  for (size_t d = 0; d < n; d++) c->nfluid += c->h_type[d] == LBM_TYPE_F;
+  HIPCHK(hipMemcpy(c->type, c->h_type.data(), n * 2, hipMemcpyHostToDevice));
+  int e;
+  if ((e = upload_real(c, c->mac[0], rho, 1.0)) != LBM_OK) return e;
+  if ((e = upload_real(c, c->mac[1], ux, 0.0)) != LBM_OK) return e;
+  if ((e = upload_real(c, c->mac[2], uy, 0.0)) != LBM_OK) return e;
+  if ((e = upload_real(c, c->mac[3], uz, 0.0)) != LBM_OK) return e;
+
+  // force: none, uniform (same F in every cell) or a per-cell field
+  const double *Fs[3] = {Fx, Fy, Fz};
+  bool any = false, uniform = true;
+  double F0[3] = {0, 0, 0};
+  for (int q = 0; q < 3; q++) {
+    if (!Fs[q]) continue;
+    F0[q] = Fs[q][0];
+    for (size_t r = 0; r < n; r++) {
+      if (Fs[q][r] != 0.0) any = true;
+      if (Fs[q][r] != F0[q]) uniform = false;
+    }
+  }
+  for (int q = 0; q < 3; q++)
+    if (c->F[q]) {
+      (void)hipFree(c->F[q]);
+      c->F[q] = nullptr;
+    }
+  if (!any) {
+    c->force_mode = 0;
+    c->Fu[0] = c->Fu[1] = c->Fu[2] = 0;
+  } else if (uniform) {
+    c->force_mode = 2;
+    for (int q = 0; q < 3; q++) c->Fu[q] = F0[q];
+  } else {
+    c->force_mode = 1;
+    for (int q = 0; q < 3; q++) {
+      if (hipMalloc(&c->F[q], n * c->csize) != hipSuccess) return LBM_ERR_NOMEM;
+      if ((e = upload_real(c, c->F[q], Fs[q], 0.0)) != LBM_OK) return e;
+    }
+  }
+  c->uploaded = 1;
+  c->state = ST_NONE;
+  return LBM_OK;
+}
+
+int lbm_init(lbm_ctx *c) {
+  if (!c || !c->uploaded) return LBM_ERR_STATE;
+  HIPCHK(hipSetDevice(c->dev));
+  const size_t n = c->g.ncell;
+  // wall tables: allocated when some fluid cell borders a solid
+  bool any_solid = false;
+  for (size_t d = 0; d < n; d++)
+    if (c->h_type[d] == LBM_TYPE_S) {
+      any_solid = true;
+      break;
+    }
+  if (any_solid && !c->links) {
+    HIPCHK(hipMalloc((void **)&c->links, n * 4));
+    for (int q = 0; q < 3; q++) HIPCHK(hipMalloc(&c->uw[q], n * c->csize));
+  }
+  LaunchCtx L = launch_ctx(c);
+  // 1. kernel mask, bounce-back links and wall velocities from the setup
+  //    velocities of the solids (what the first Streaming reads)
+  HIPCHK(launch_tables(L, c->mask, c->links, c->uw[0], c->uw[1], c->uw[2], c->type));
+  // 2. f = f* = feq (LBM::Init)
+  HIPCHK(launch_init(L, c->cur, c->oth));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->walls_from_setup = any_solid ? 1 : 0;
+  c->state = ST_F;
+  return LBM_OK;
+}
+
+// After the first MacroProp: the solids' rho,u become the moments of their
+// (never changing) Init populations, and the wall velocities of every later
+// Streaming are averaged from those (lbm.cpp:941-962).
+static int walls_from_moments(lbm_ctx *c) {
+  if (!c->walls_from_setup) return LBM_OK;
+  LaunchCtx L = launch_ctx(c);
+  L.macro_sel = 16;  // TYPE_S only
+  HIPCHK(launch_macro(L, c->cur));
+  HIPCHK(launch_tables(L, c->mask, c->links, c->uw[0], c->uw[1], c->uw[2], c->type));
+  c->walls_from_setup = 0;
+  return LBM_OK;
+}
+
+int lbm_collide(lbm_ctx *c) {
+  if (!c) return LBM_ERR_ARG;
+  if (c->state != ST_F) return LBM_ERR_STATE;
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(launch_collide(launch_ctx(c), c->cur));
+  c->state = ST_FSTAR;
+  return LBM_OK;
+}
+
+int lbm_stream(lbm_ctx *c) {
+  if (!c) return LBM_ERR_ARG;
+  if (c->state != ST_FSTAR) return LBM_ERR_STATE;
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(launch_stream(launch_ctx(c), c->cur, c->oth, STREAM_ONLY));
+  std::swap(c->cur, c->oth);
+  c->state = ST_F;
+  return LBM_OK;
+}
+
+int lbm_macro(lbm_ctx *c) {
+  if (!c) return LBM_ERR_ARG;
+  if (c->state != ST_F) return LBM_ERR_STATE;
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(launch_macro(launch_ctx(c), c->cur));
+  return walls_from_moments(c);
+}
+
+int lbm_step(lbm_ctx *c, int32_t nsteps, int32_t emit_macro) {
+  (void)emit_macro;  // the last launch always emits rho,u (the state after MacroProp)
+  if (!c || nsteps < 0) return LBM_ERR_ARG;
+  if (c->state == ST_NONE) return LBM_ERR_STATE;
+  if (nsteps == 0) return LBM_OK;
+  HIPCHK(hipSetDevice(c->dev));
+  LaunchCtx L = launch_ctx(c);
+  if (c->state == ST_F) HIPCHK(launch_collide(L, c->cur));
+  if (c->walls_from_setup) {
+    // first step with the setup wall velocities, then switch the tables
+    HIPCHK(launch_stream(L, c->cur, c->oth, STREAM_MACRO));
+    std::swap(c->cur, c->oth);
+    c->state = ST_F;
+    int e = walls_from_moments(c);
+    if (e != LBM_OK) return e;
+    if (--nsteps == 0) return LBM_OK;
+    L = launch_ctx(c);
+    HIPCHK(launch_collide(L, c->cur));
+  }
+  for (int s = 0; s < nsteps - 1; s++) {
+    HIPCHK(launch_stream(L, c->cur, c->oth, STREAM_COLLIDE));
+    std::swap(c->cur, c->oth);
+  }
+  HIPCHK(launch_stream(L, c->cur, c->oth, STREAM_MACRO));
+  std::swap(c->cur, c->oth);
+  c->state = ST_F;
+  return LBM_OK;
+}
+
+int lbm_time_steps(lbm_ctx *c, int32_t nsteps, double *ms) {
+  if (!c || nsteps < 1 || !ms) return LBM_ERR_ARG;
+  if (c->state == ST_NONE) return LBM_ERR_STATE;
+  HIPCHK(hipSetDevice(c->dev));
+  if (c->walls_from_setup) {
+    int e = lbm_step(c, 1, 1);
+    if (e != LBM_OK) return e;
+  }
+  const LaunchCtx L = launch_ctx(c);
+  if (c->state == ST_F) HIPCHK(launch_collide(L, c->cur));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, c->stream));
+  for (int s = 0; s < nsteps; s++) {
+    HIPCHK(launch_stream(L, c->cur, c->oth, STREAM_COLLIDE));
+    std::swap(c->cur, c->oth);
+  }
+  HIPCHK(hipEventRecord(e1, c->stream));
+  HIPCHK(launch_stream(L, c->cur, c->oth, STREAM_MACRO));
+  std::swap(c->cur, c->oth);
+  c->state = ST_F;
+  HIPCHK(hipEventSynchronize(e1));
+  float t = 0.f;
+  HIPCHK(hipEventElapsedTime(&t, e0, e1));
+  *ms = (double)t;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return LBM_OK;
+}
+
+int lbm_sync(lbm_ctx *c) {
+  if (!c) return LBM_ERR_ARG;
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return lbm_status(c, nullptr);
+}
+
+int lbm_status(lbm_ctx *c, int64_t *first_bad_cell) {
+  if (!c) return LBM_ERR_ARG;
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int st = 0;
+  unsigned long long fb = 0;
+  HIPCHK(hipMemcpy(&st, c->status, sizeof st, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(&fb, c->first_bad, sizeof fb, hipMemcpyDeviceToHost));
+  if (first_bad_cell) {
+    if (st == 0) {
+      *first_bad_cell = -1;
+    } else {
+      // device index -> reference linear index
+      const uint64_t i = fb % c->g.nx, t = fb / c->g.nx, j = t % c->g.ny, k = t / c->g.ny;
+      *first_bad_cell = (int64_t)((i * c->g.ny + j) * c->g.nz + k);
+    }
+  }
+  if (st & 2) return LBM_ERR_DENSITY_RANGE;
+  if (st & 1) return LBM_ERR_FEQ_NEGATIVE;
+  return LBM_OK;
+}
+
+int lbm_download_macro(lbm_ctx *c, double *rho, double *ux, double *uy, double *uz) {
+  if (!c) return LBM_ERR_ARG;
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double *outs[4] = {rho, ux, uy, uz};
+  for (int q = 0; q < 4; q++)
+    if (outs[q]) {
+      int e = download_real(c, outs[q], c->mac[q]);
+      if (e != LBM_OK) return e;
+    }
+  return LBM_OK;
+}
+
+int lbm_download_force(lbm_ctx *c, double *Fx, double *Fy, double *Fz) {
+  if (!c) return LBM_ERR_ARG;
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double *outs[3] = {Fx, Fy, Fz};
+  const size_t n = c->g.ncell;
+  for (int q = 0; q < 3; q++) {
+    if (!outs[q]) continue;
+    if (c->force_mode == 1) {
+      int e = download_real(c, outs[q], c->F[q]);
+      if (e != LBM_OK) return e;
+    } else {
+      for (size_t r = 0; r < n; r++) outs[q][r] = c->force_mode == 2 ? c->Fu[q] : 0.0;
+    }
+  }
+  return LBM_OK;
+}
+
+int lbm_download_f(lbm_ctx *c, double *f) {
+  if (!c || !f) return LBM_ERR_ARG;
+  if (c->state == ST_NONE) return LBM_ERR_STATE;
+  HIPCHK(hipSetDevice(c->dev));
+  const size_t n = c->g.ncell;
+  double *d = nullptr;
+  HIPCHK(hipMalloc((void **)&d, n * 19 * sizeof(double)));
+  hipError_t e = launch_f_out(launch_ctx(c), c->cur, d);
+  std::vector<double> h;
+  if (e == hipSuccess) {
+    h.resize(n * 19);
+    e = hipMemcpyAsync(h.data(), d, n * 19 * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  }
+  (void)hipFree(d);
+  if (e != hipSuccess) return LBM_ERR_HIP;
+  for_cells(c, [&](size_t r, size_t dd) { memcpy(f + r * 19, h.data() + dd * 19, 19 * sizeof(double)); });
+  return LBM_OK;
+}
+
+int lbm_export_links(lbm_ctx *c, int16_t *type, int32_t *src, uint8_t *kind) {
+  if (!c) return LBM_ERR_ARG;
+  if (c->state == ST_NONE) return LBM_ERR_STATE;
+  HIPCHK(hipSetDevice(c->dev));
+  const size_t n = c->g.ncell;
+  if (type) for_cells(c, [&](size_t r, size_t d) { type[r] = c->h_type[d]; });
+  if (!src && !kind) return LBM_OK;
+  int32_t *dsrc = nullptr;
+  uint8_t *dkind = nullptr;
+  HIPCHK(hipMalloc((void **)&dsrc, n * 19 * 4));
+  if (hipMalloc((void **)&dkind, n * 19) != hipSuccess) {
+    (void)hipFree(dsrc);
+    return LBM_ERR_NOMEM;
+  }
+  std::vector<int32_t> hs(n * 19);
+  std::vector<uint8_t> hk(n * 19);
+  hipError_t e = launch_links(launch_ctx(c), dsrc, dkind);
+  if (e == hipSuccess) e = hipMemcpyAsync(hs.data(), dsrc, n * 19 * 4, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(hk.data(), dkind, n * 19, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(dsrc);
+  (void)hipFree(dkind);
+  if (e != hipSuccess) return LBM_ERR_HIP;
+  const uint32_t nx = c->g.nx, ny = c->g.ny, nz = c->g.nz;
+  for_cells(c, [&](size_t r, size_t d) {
+    for (int m = 0; m < 19; m++) {
+      int32_t s = hs[d * 19 + m];
+      if (s >= 0) {  // device index -> reference index
+        const uint32_t i = (uint32_t)s % nx, t = (uint32_t)s / nx, j = t % ny, k = t / ny;
+        s = (int32_t)(((size_t)i * ny + j) * nz + k);
+      }
+      if (src) src[r * 19 + m] = s;
+      if (kind) kind[r * 19 + m] = hk[d * 19 + m];
+    }
+  });
+  return LBM_OK;
+}
+
+int lbm_reduce_fp64(lbm_ctx *c, int32_t what, double *out) {
+  if (!c || !out || what < 0 || what > 2) return LBM_ERR_ARG;
+  if (c->state == ST_NONE) return LBM_ERR_STATE;
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(launch_reduce(launch_ctx(c), what, c->red_partial, c->red_out));
+  const int nc = what == 1 ? 3 : 1;
+  HIPCHK(hipMemcpyAsync(out, c->red_out, sizeof(double) * nc, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return LBM_OK;
+}
+
+int64_t lbm_ncells(const lbm_ctx *c) { return c ? (int64_t)c->g.ncell : -1; }
+int64_t lbm_nfluid(const lbm_ctx *c) { return c ? c->nfluid : -1; }
+int32_t lbm_storage(const lbm_ctx *c) { return c ? c->storage : -1; }
+
+double lbm_bytes_per_cell(const lbm_ctx *c) {
+  if (!c) return 0.0;
+  // A/B pull: 19 populations read + 19 written + the 1-byte mask
+  return 2.0 * 19.0 * (double)c->esize + 1.0;
+}
+
+}  // extern "C"

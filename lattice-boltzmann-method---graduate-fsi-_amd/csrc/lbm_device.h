@@ -1,0 +1,127 @@
+// lbm_device.h -- device-side definitions shared by the gfx950 kernels.
+//
+// Lattice: D3Q19 in the reference's direction order (LBM.h:546-552).
+// Device layout: structure of arrays, one plane of `ps` elements per
+// direction, cell index idx = (z*ny + y)*nx + x with x fastest over the
+// padded (Nx+2)(Ny+2)(Nz+2) lattice.  Two such population buffers (A/B)
+// alternate every step.
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <utility>
+
+#include "lbm_launch.h"
+
+namespace lbm {
+
+constexpr int Q = 19;
+// LBM.h:546-552
+constexpr int CX[Q] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0};
+constexpr int CY[Q] = {0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1};
+constexpr int CZ[Q] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1};
+constexpr int OPP[Q] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
+constexpr double W[Q] = {1.0 / 3,  1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 18,
+                         1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36,
+                         1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36};
+
+// per-cell kernel mask bits (derived from the cell types at init)
+constexpr uint8_t M_FLUID = 1;   // TYPE_F: collides and pulls
+constexpr uint8_t M_WALL = 2;    // some x - c_m is TYPE_S: read links[idx]
+constexpr uint8_t M_WRAP = 4;    // on the padded shell: neighbours wrap
+constexpr uint8_t M_MOVING = 8;  // nonzero wall velocity: read uw*[idx]
+constexpr uint8_t M_SOLID = 16;  // TYPE_S
+constexpr uint8_t M_OTHER = 32;  // any other type (IN, OUT, P, L, T, Q)
+
+// sticky status bits
+constexpr int ST_FEQ_NEG = 1;
+constexpr int ST_RHO_RANGE = 2;
+
+template <int... Ms, class F>
+__device__ __forceinline__ void for_dirs_impl(std::integer_sequence<int, Ms...>, F &&f) {
+  (f(std::integral_constant<int, Ms>{}), ...);
+}
+// Calls f(std::integral_constant<int, m>) for m = 0..18, fully unrolled, so
+// that c_m and w_m are compile-time constants inside f.
+template <class F>
+__device__ __forceinline__ void for_dirs(F &&f) {
+  for_dirs_impl(std::make_integer_sequence<int, Q>{}, f);
+}
+
+// Linear index of x - c (the pulling source of direction c), each axis
+// wrapped modulo its padded extent as the reference does
+// ((i + cx + (Nx+2)) % (Nx+2), lbm.cpp:197-199).
+__device__ __forceinline__ uint32_t wrap_src(const Geo &g, uint32_t x, uint32_t y, uint32_t z,
+                                             int cx, int cy, int cz) {
+  uint32_t xs = cx > 0 ? (x == 0 ? g.nx - 1 : x - 1) : (cx < 0 ? (x == g.nx - 1 ? 0 : x + 1) : x);
+  uint32_t ys = cy > 0 ? (y == 0 ? g.ny - 1 : y - 1) : (cy < 0 ? (y == g.ny - 1 ? 0 : y + 1) : y);
+  uint32_t zs = cz > 0 ? (z == 0 ? g.nz - 1 : z - 1) : (cz < 0 ? (z == g.nz - 1 ? 0 : z + 1) : z);
+  return (zs * g.ny + ys) * g.nx + xs;
+}
+// x + c, wrapped (the pushing target of direction c)
+__device__ __forceinline__ uint32_t wrap_dst(const Geo &g, uint32_t x, uint32_t y, uint32_t z,
+                                             int cx, int cy, int cz) {
+  return wrap_src(g, x, y, z, -cx, -cy, -cz);
+}
+__device__ __forceinline__ void coords(const Geo &g, uint32_t idx, uint32_t &x, uint32_t &y,
+                                       uint32_t &z) {
+  x = idx % g.nx;
+  uint32_t t = idx / g.nx;
+  y = t % g.ny;
+  z = t / g.ny;
+}
+
+// ---- population storage -------------------------------------------------
+// Stored value s and compute value v:
+//   double: v = s = f                      (unshifted, FP64 arithmetic)
+//   float : v = s = f - w                  (shifted, FP32 arithmetic)
+//   half  : v = f - w = float(s) * 2^-12   (shifted and scaled, FP32 arith.)
+template <class S>
+struct Store;
+
+template <>
+struct Store<double> {
+  using C = double;
+  static constexpr bool shifted = false;
+  static __device__ __forceinline__ double ld(const double *p) { return *p; }
+  static __device__ __forceinline__ void st(double *p, double v) { __builtin_nontemporal_store(v, p); }
+  static __device__ __forceinline__ double enc(double v) { return v; }
+};
+
+template <>
+struct Store<float> {
+  using C = float;
+  static constexpr bool shifted = true;
+  static __device__ __forceinline__ float ld(const float *p) { return *p; }
+  static __device__ __forceinline__ void st(float *p, float v) { __builtin_nontemporal_store(v, p); }
+  static __device__ __forceinline__ float enc(float v) { return v; }
+};
+
+constexpr float F16S_SCALE = 4096.0f;  // 2^LBM_F16S_SCALE_LOG2
+constexpr float F16S_INV = 1.0f / 4096.0f;
+
+template <>
+struct Store<__half> {
+  using C = float;
+  static constexpr bool shifted = true;
+  static __device__ __forceinline__ float ld(const __half *p) {
+    return __half2float(*p) * F16S_INV;
+  }
+  static __device__ __forceinline__ void st(__half *p, float v) {
+    __half h = __float2half_rn(v * F16S_SCALE);
+    __builtin_nontemporal_store(*reinterpret_cast<unsigned short *>(&h),
+                                reinterpret_cast<unsigned short *>(p));
+  }
+  static __device__ __forceinline__ __half enc(float v) { return __float2half_rn(v * F16S_SCALE); }
+};
+
+// Cubic equilibrium of LBM::Calc_feq (lbm.cpp:628-633) with cs2 = 1/3:
+//   feq = rho w (1 + A),  A = 3cu + 4.5cu^2 - 1.5uu + 4.5cu^3 - 4.5 cu uu.
+template <class C>
+__device__ __forceinline__ C eq_A(C cu, C uu) {
+  return cu * (C(3) + C(4.5) * cu * cu - C(4.5) * uu) + C(4.5) * cu * cu - C(1.5) * uu;
+}
+
+}  // namespace lbm

@@ -1,0 +1,664 @@
+// lbm_kernels.hip -- gfx950 kernels of the D3Q19 BGK step.
+//
+// One thread per cell of the padded lattice, 1D launch over the x-fastest
+// linear index (consecutive lanes = consecutive x, so every population load
+// and store of a wave is one contiguous 64-element run).  The step is
+// bandwidth-bound: per fluid cell it reads 19 populations and a 1-byte mask
+// and writes 19 populations; nothing else is touched on bulk cells.
+//
+//  k_init           LBM::Init (lbm.cpp:31-57), f = feq in every cell
+//  k_tables         mask bits, bounce-back link words and wall velocities
+//                   (lbm.cpp:186-295, 941-962) from the cell types
+//  k_collide        LBM::Collision/ConvLBM (lbm.cpp:60-118), in place
+//  k_stream<...>    LBM::Streaming (lbm.cpp:182-295) in pull form, optionally
+//                   fused with MacroProp (lbm.cpp:657-776) and the next
+//                   step's collision -- the hot kernel
+//  k_macro          LBM::MacroProp (lbm.cpp:657-776)
+//  k_links          the streaming tables as the kernels use them
+//  k_f_out          populations to FP64 for download
+//  k_reduce_*       deterministic FP64 sums (mass, momentum, kinetic energy)
+#include "lbm_device.h"
+#include "lbm_launch.h"
+
+namespace lbm {
+
+template <class C>
+struct KArgs {
+  Geo g;
+  C omega;
+  C Fx, Fy, Fz;                  // uniform force (force mode 2)
+  const C *Fxa, *Fya, *Fza;      // per-cell force (force mode 1)
+  const uint32_t *links;         // bit m: x - c_m is TYPE_S
+  const C *uwx, *uwy, *uwz;      // per-cell wall velocity (lbm.cpp:941-962)
+  C *rho, *ux, *uy, *uz;         // macroscopic arrays
+  int *status;
+  unsigned long long *first_bad;
+};
+
+template <class C, int FORCE>
+__device__ __forceinline__ void load_force(const KArgs<C> &a, uint32_t idx, C &Fx, C &Fy, C &Fz) {
+  if constexpr (FORCE == 1) {
+    Fx = a.Fxa[idx];
+    Fy = a.Fya[idx];
+    Fz = a.Fza[idx];
+  } else if constexpr (FORCE == 2) {
+    Fx = a.Fx;
+    Fy = a.Fy;
+    Fz = a.Fz;
+  } else {
+    Fx = C(0);
+    Fy = C(0);
+    Fz = C(0);
+  }
+}
+
+__device__ __forceinline__ void flag(int *status, unsigned long long *first_bad, int bits,
+                                     uint32_t idx) {
+  atomicOr(status, bits);
+  atomicMin(first_bad, (unsigned long long)idx);
+}
+
+// BGK collision with the Guo source (ConvLBM, lbm.cpp:75-118) of one cell.
+// f holds compute values (shifted for FP32/FP16 storage); rho is the full
+// density, drho = rho - 1.  Returns true when some feq < 0 (lbm.cpp:114).
+template <class S, int FORCE>
+__device__ __forceinline__ bool collide_store(typename Store<S>::C (&f)[Q],
+                                              typename Store<S>::C rho,
+                                              typename Store<S>::C drho,
+                                              typename Store<S>::C ux, typename Store<S>::C uy,
+                                              typename Store<S>::C uz, typename Store<S>::C Fx,
+                                              typename Store<S>::C Fy, typename Store<S>::C Fz,
+                                              typename Store<S>::C omega, S *__restrict__ out,
+                                              uint32_t ps, uint32_t idx) {
+  using C = typename Store<S>::C;
+  const C uu = ux * ux + uy * uy + uz * uz;
+  const C om1 = C(1) - omega;
+  const C fpre = C(1) - omega * C(0.5);
+  C Fu = C(0);
+  if constexpr (FORCE != 0) Fu = Fx * ux + Fy * uy + Fz * uz;
+  bool neg = false;
+  for_dirs([&](auto M) {
+    constexpr int m = decltype(M)::value;
+    constexpr C w = C(W[m]);
+    const C cu = C(CX[m]) * ux + C(CY[m]) * uy + C(CZ[m]) * uz;
+    const C A = eq_A(cu, uu);
+    const C feq_full = rho * w * (C(1) + A);
+    neg |= feq_full < C(0);
+    C feq;
+    if constexpr (Store<S>::shifted) feq = w * (drho + rho * A);
+    else feq = feq_full;
+    C fs = f[m] * om1 + omega * feq;
+    if constexpr (FORCE != 0) {
+      const C Fc = C(CX[m]) * Fx + C(CY[m]) * Fy + C(CZ[m]) * Fz;
+      fs += fpre * w * (C(3) * Fc + C(9) * Fc * cu - C(3) * Fu);
+    }
+    Store<S>::st(out + (uint32_t)m * ps + idx, fs);
+  });
+  return neg;
+}
+
+// ---------------------------------------------------------------------------
+// Pull streaming (+ moments, + collision).  The value arriving at fluid cell
+// x in direction m is, as in the push loop of lbm.cpp:252-292:
+//   x - c_m is TYPE_F      : f*(x - c_m, m)
+//   x - c_m is TYPE_S      : f*(x, opp m) + 2 w_m (u_w . c_m) / cs2
+//   x - c_m any other type : f(x, m) untouched since Init; that value is kept
+//                            in slot m of the source cell, which never
+//                            collides, so the same load reads it.
+// COLLIDE: relax and store f*; otherwise store f (post-streaming).
+// EMIT   : write rho, u (u = (sum f c + F/2)/rho, lbm.cpp:672-675).
+// ---------------------------------------------------------------------------
+template <class S, int FORCE, bool COLLIDE, bool EMIT>
+__global__ __launch_bounds__(256) void k_stream(const S *__restrict__ fin, S *__restrict__ fout,
+                                                const uint8_t *__restrict__ mask,
+                                                KArgs<typename Store<S>::C> a) {
+  using C = typename Store<S>::C;
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.g.ncell) return;
+  const uint8_t mk = mask[idx];
+  if (!(mk & M_FLUID)) return;
+  const uint32_t ps = a.g.ps, nx = a.g.nx, plane = a.g.plane;
+
+  C f[Q];
+  if (!(mk & (M_WALL | M_WRAP))) {
+    for_dirs([&](auto M) {
+      constexpr int m = decltype(M)::value;
+      const uint32_t src = idx - (uint32_t)(CX[m] + CY[m] * (int)nx + CZ[m] * (int)plane);
+      f[m] = Store<S>::ld(fin + (uint32_t)m * ps + src);
+    });
+  } else {
+    uint32_t x, y, z;
+    coords(a.g, idx, x, y, z);
+    const uint32_t lk = (mk & M_WALL) ? a.links[idx] : 0u;
+    C uwx = C(0), uwy = C(0), uwz = C(0);
+    if (mk & M_MOVING) {
+      uwx = a.uwx[idx];
+      uwy = a.uwy[idx];
+      uwz = a.uwz[idx];
+    }
+    for_dirs([&](auto M) {
+      constexpr int m = decltype(M)::value;
+      if (lk & (1u << m)) {
+        const C ucw = C(CX[m]) * uwx + C(CY[m]) * uwy + C(CZ[m]) * uwz;
+        f[m] = Store<S>::ld(fin + (uint32_t)OPP[m] * ps + idx) + C(6.0 * W[m]) * ucw;
+      } else {
+        const uint32_t src =
+            (mk & M_WRAP) ? wrap_src(a.g, x, y, z, CX[m], CY[m], CZ[m])
+                          : idx - (uint32_t)(CX[m] + CY[m] * (int)nx + CZ[m] * (int)plane);
+        f[m] = Store<S>::ld(fin + (uint32_t)m * ps + src);
+      }
+    });
+  }
+
+  if constexpr (!COLLIDE && !EMIT) {
+    for_dirs([&](auto M) {
+      constexpr int m = decltype(M)::value;
+      Store<S>::st(fout + (uint32_t)m * ps + idx, f[m]);
+    });
+    return;
+  } else {
+    // moments (MacroProp, lbm.cpp:668-675)
+    C s0 = C(0);
+    for_dirs([&](auto M) { s0 += f[decltype(M)::value]; });
+    const C jx = ((f[1] - f[2]) + (f[7] - f[8])) + ((f[9] - f[10]) + (f[13] - f[14])) + (f[15] - f[16]);
+    const C jy = ((f[3] - f[4]) + (f[7] - f[8])) + ((f[11] - f[12]) + (f[14] - f[13])) + (f[17] - f[18]);
+    const C jz = ((f[5] - f[6]) + (f[9] - f[10])) + ((f[11] - f[12]) + (f[16] - f[15])) + (f[18] - f[17]);
+    C rho, drho;
+    if constexpr (Store<S>::shifted) {
+      drho = s0;
+      rho = C(1) + s0;
+    } else {
+      rho = s0;
+      drho = s0 - C(1);
+    }
+    C Fx, Fy, Fz;
+    load_force<C, FORCE>(a, idx, Fx, Fy, Fz);
+    C ux, uy, uz;
+    if constexpr (FORCE != 0) {
+      ux = (jx + C(0.5) * Fx) / rho;
+      uy = (jy + C(0.5) * Fy) / rho;
+      uz = (jz + C(0.5) * Fz) / rho;
+    } else {
+      ux = jx / rho;
+      uy = jy / rho;
+      uz = jz / rho;
+    }
+    int bad = (rho < C(0) || rho > C(100)) ? ST_RHO_RANGE : 0;
+    if constexpr (EMIT) {
+      a.rho[idx] = rho;
+      a.ux[idx] = ux;
+      a.uy[idx] = uy;
+      a.uz[idx] = uz;
+    }
+    if constexpr (COLLIDE) {
+      if (collide_store<S, FORCE>(f, rho, drho, ux, uy, uz, Fx, Fy, Fz, a.omega, fout, ps, idx))
+        bad |= ST_FEQ_NEG;
+    } else {
+      for_dirs([&](auto M) {
+        constexpr int m = decltype(M)::value;
+        Store<S>::st(fout + (uint32_t)m * ps + idx, f[m]);
+      });
+    }
+    if (bad) flag(a.status, a.first_bad, bad, idx);
+  }
+}
+
+// LBM::Collision (lbm.cpp:60-73) in place, with rho, u and F as stored.
+template <class S, int FORCE>
+__global__ __launch_bounds__(256) void k_collide(S *__restrict__ buf, const uint8_t *__restrict__ mask,
+                                                 KArgs<typename Store<S>::C> a) {
+  using C = typename Store<S>::C;
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.g.ncell) return;
+  if (!(mask[idx] & M_FLUID)) return;
+  const uint32_t ps = a.g.ps;
+  C f[Q];
+  for_dirs([&](auto M) {
+    constexpr int m = decltype(M)::value;
+    f[m] = Store<S>::ld(buf + (uint32_t)m * ps + idx);
+  });
+  const C rho = a.rho[idx];
+  const C ux = a.ux[idx], uy = a.uy[idx], uz = a.uz[idx];
+  C Fx, Fy, Fz;
+  load_force<C, FORCE>(a, idx, Fx, Fy, Fz);
+  if (collide_store<S, FORCE>(f, rho, rho - C(1), ux, uy, uz, Fx, Fy, Fz, a.omega, buf, ps, idx))
+    flag(a.status, a.first_bad, ST_FEQ_NEG, idx);
+}
+
+// LBM::MacroProp (lbm.cpp:657-776) on TYPE_F and TYPE_S cells.  Cells of
+// other types keep their stored rho,u: their population slots carry the
+// frozen-link values of their fluid neighbours.
+template <class S, int FORCE>
+__global__ __launch_bounds__(256) void k_macro(const S *__restrict__ buf,
+                                               const uint8_t *__restrict__ mask, uint8_t sel,
+                                               KArgs<typename Store<S>::C> a) {
+  using C = typename Store<S>::C;
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.g.ncell) return;
+  if (!(mask[idx] & sel)) return;
+  const uint32_t ps = a.g.ps;
+  C s0 = C(0), jx = C(0), jy = C(0), jz = C(0);
+  for_dirs([&](auto M) {
+    constexpr int m = decltype(M)::value;
+    const C v = Store<S>::ld(buf + (uint32_t)m * ps + idx);
+    s0 += v;
+    if constexpr (CX[m] > 0) jx += v;
+    if constexpr (CX[m] < 0) jx -= v;
+    if constexpr (CY[m] > 0) jy += v;
+    if constexpr (CY[m] < 0) jy -= v;
+    if constexpr (CZ[m] > 0) jz += v;
+    if constexpr (CZ[m] < 0) jz -= v;
+  });
+  const C rho = Store<S>::shifted ? C(1) + s0 : s0;
+  C Fx, Fy, Fz;
+  load_force<C, FORCE>(a, idx, Fx, Fy, Fz);
+  a.rho[idx] = rho;
+  a.ux[idx] = (jx + C(0.5) * Fx) / rho;
+  a.uy[idx] = (jy + C(0.5) * Fy) / rho;
+  a.uz[idx] = (jz + C(0.5) * Fz) / rho;
+  if (rho < C(0) || rho > C(100)) flag(a.status, a.first_bad, ST_RHO_RANGE, idx);
+}
+
+// LBM::Init (lbm.cpp:31-57): f = feq(rho,u) in every cell, written to both
+// buffers.  A cell that never collides (not TYPE_F or TYPE_S) instead stores
+// in slot m the Init value of its fluid neighbour x + c_m in direction m:
+// the reference never rewrites that population (frozen link).
+template <class S>
+__global__ __launch_bounds__(256) void k_init(S *__restrict__ A, S *__restrict__ B,
+                                              const uint8_t *__restrict__ mask,
+                                              KArgs<typename Store<S>::C> a) {
+  using C = typename Store<S>::C;
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.g.ncell) return;
+  const uint8_t mk = mask[idx];
+  uint32_t x, y, z;
+  coords(a.g, idx, x, y, z);
+  const uint32_t ps = a.g.ps;
+  for_dirs([&](auto M) {
+    constexpr int m = decltype(M)::value;
+    uint32_t c = idx;
+    if (mk & M_OTHER) {
+      const uint32_t d = wrap_dst(a.g, x, y, z, CX[m], CY[m], CZ[m]);
+      if (mask[d] & M_FLUID) c = d;
+    }
+    const C rho = a.rho[c], ux = a.ux[c], uy = a.uy[c], uz = a.uz[c];
+    const C uu = ux * ux + uy * uy + uz * uz;
+    const C cu = C(CX[m]) * ux + C(CY[m]) * uy + C(CZ[m]) * uz;
+    const C A_ = eq_A(cu, uu);
+    C v;
+    if constexpr (Store<S>::shifted) v = C(W[m]) * ((rho - C(1)) + rho * A_);
+    else v = rho * C(W[m]) * (C(1) + A_);
+    const S s = Store<S>::enc(v);
+    A[(uint32_t)m * ps + idx] = s;
+    B[(uint32_t)m * ps + idx] = s;
+  });
+}
+
+// Per-cell kernel mask, bounce-back link word and wall velocity from the
+// cell types (device order) and the solids' current velocities.
+template <class C>
+__global__ __launch_bounds__(256) void k_tables(const int16_t *__restrict__ type,
+                                                uint8_t *__restrict__ mask,
+                                                uint32_t *__restrict__ links, C *uwx, C *uwy,
+                                                C *uwz, Geo g, const C *__restrict__ ux,
+                                                const C *__restrict__ uy,
+                                                const C *__restrict__ uz) {
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= g.ncell) return;
+  uint32_t x, y, z;
+  coords(g, idx, x, y, z);
+  const int16_t t = type[idx];
+  uint8_t mk = 0;
+  if (t == 0) mk |= M_FLUID;
+  else if (t == 1) mk |= M_SOLID;
+  else mk |= M_OTHER;
+  if (x == 0 || x == g.nx - 1 || y == 0 || y == g.ny - 1 || z == 0 || z == g.nz - 1) mk |= M_WRAP;
+  uint32_t lk = 0;
+  C wx = C(0), wy = C(0), wz = C(0);
+  if (t == 0) {
+    // bounce-back links: source x - c_m is solid
+    for_dirs([&](auto M) {
+      constexpr int m = decltype(M)::value;
+      if (type[wrap_src(g, x, y, z, CX[m], CY[m], CZ[m])] == 1) lk |= 1u << m;
+    });
+    // Determine_Wall_Velocity, lbm.cpp:941-962: neighbours x + c_l in l order
+    int num = 0;
+    for_dirs([&](auto M) {
+      constexpr int l = decltype(M)::value;
+      const uint32_t n = wrap_dst(g, x, y, z, CX[l], CY[l], CZ[l]);
+      if (type[n] == 1 && (ux[n] != C(0) || uy[n] != C(0) || uz[n] != C(0))) {
+        wx += ux[n];
+        wy += uy[n];
+        wz += uz[n];
+        num++;
+      }
+    });
+    if (num != 0) {
+      wx /= C(num);
+      wy /= C(num);
+      wz /= C(num);
+    }
+    if (lk) mk |= M_WALL;
+    if (lk && (wx != C(0) || wy != C(0) || wz != C(0))) mk |= M_MOVING;
+  }
+  mask[idx] = mk;
+  if (links) links[idx] = lk;
+  if (uwx) {
+    uwx[idx] = wx;
+    uwy[idx] = wy;
+    uwz[idx] = wz;
+  }
+}
+
+// The streaming tables exactly as k_stream resolves them.
+__global__ __launch_bounds__(256) void k_links(const uint8_t *__restrict__ mask,
+                                               const uint32_t *__restrict__ links, Geo g,
+                                               int32_t *__restrict__ src_out,
+                                               uint8_t *__restrict__ kind_out) {
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= g.ncell) return;
+  const uint8_t mk = mask[idx];
+  const uint32_t nx = g.nx, plane = g.plane;
+  if (!(mk & M_FLUID)) {
+    for (int m = 0; m < Q; m++) {
+      src_out[(size_t)idx * Q + m] = -1;
+      kind_out[(size_t)idx * Q + m] = 255;
+    }
+    return;
+  }
+  uint32_t x, y, z;
+  coords(g, idx, x, y, z);
+  const uint32_t lk = (mk & M_WALL) ? links[idx] : 0u;
+  for_dirs([&](auto M) {
+    constexpr int m = decltype(M)::value;
+    const uint32_t src = (mk & M_WRAP)
+                             ? wrap_src(g, x, y, z, CX[m], CY[m], CZ[m])
+                             : idx - (uint32_t)(CX[m] + CY[m] * (int)nx + CZ[m] * (int)plane);
+    uint8_t kind;
+    if (lk & (1u << m)) kind = 1;
+    else kind = (mask[src] & M_FLUID) ? 0 : 2;
+    src_out[(size_t)idx * Q + m] = (int32_t)src;
+    kind_out[(size_t)idx * Q + m] = kind;
+  });
+}
+
+// Populations as FP64 f (unshifted), [idx][m].  TYPE_F and TYPE_S cells come
+// from the buffer; other cells report feq of their stored rho,u (their Init
+// value; their slots hold their neighbours' frozen values).
+template <class S>
+__global__ __launch_bounds__(256) void k_f_out(const S *__restrict__ buf,
+                                               const uint8_t *__restrict__ mask,
+                                               KArgs<typename Store<S>::C> a,
+                                               double *__restrict__ out) {
+  using C = typename Store<S>::C;
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.g.ncell) return;
+  const uint8_t mk = mask[idx];
+  const uint32_t ps = a.g.ps;
+  const C rho = a.rho[idx], ux = a.ux[idx], uy = a.uy[idx], uz = a.uz[idx];
+  const C uu = ux * ux + uy * uy + uz * uz;
+  for_dirs([&](auto M) {
+    constexpr int m = decltype(M)::value;
+    double v;
+    if (mk & (M_FLUID | M_SOLID)) {
+      v = (double)Store<S>::ld(buf + (uint32_t)m * ps + idx);
+      if constexpr (Store<S>::shifted) v += W[m];
+    } else {
+      const C cu = C(CX[m]) * ux + C(CY[m]) * uy + C(CZ[m]) * uz;
+      v = (double)(rho * C(W[m]) * (C(1) + eq_A(cu, uu)));
+    }
+    out[(size_t)idx * Q + m] = v;
+  });
+}
+
+// ---- deterministic FP64 reductions over TYPE_F cells ----------------------
+// Pass 1: block b sums cells [b*chunk, (b+1)*chunk) in a fixed tree order.
+// Pass 2: one block sums the partials in a fixed tree order.
+constexpr int RED_THREADS = 256;
+
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int s = RED_THREADS / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] += sh[t + s];
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+template <class C>
+__global__ __launch_bounds__(RED_THREADS) void k_reduce_1(const uint8_t *__restrict__ mask,
+                                                          const C *__restrict__ rho,
+                                                          const C *__restrict__ ux,
+                                                          const C *__restrict__ uy,
+                                                          const C *__restrict__ uz, uint32_t n,
+                                                          uint32_t chunk, int what,
+                                                          double *__restrict__ partial) {
+  __shared__ double sh[RED_THREADS];
+  const uint32_t b0 = blockIdx.x * chunk;
+  const uint32_t b1 = min(n, b0 + chunk);
+  double s[3] = {0.0, 0.0, 0.0};
+  for (uint32_t i = b0 + threadIdx.x; i < b1; i += RED_THREADS) {
+    if (!(mask[i] & M_FLUID)) continue;
+    const double r = (double)rho[i];
+    if (what == 0) {
+      s[0] += r;
+    } else if (what == 1) {
+      s[0] += r * (double)ux[i];
+      s[1] += r * (double)uy[i];
+      s[2] += r * (double)uz[i];
+    } else {
+      const double x = ux[i], y = uy[i], z = uz[i];
+      s[0] += 0.5 * r * (x * x + y * y + z * z);
+    }
+  }
+  const int nc = (what == 1) ? 3 : 1;
+  for (int c = 0; c < nc; c++) {
+    const double t = block_sum(s[c], sh);
+    if (threadIdx.x == 0) partial[(size_t)c * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_reduce_2(const double *__restrict__ partial,
+                                                          int nblocks, int ncomp,
+                                                          double *__restrict__ out) {
+  __shared__ double sh[RED_THREADS];
+  for (int c = 0; c < ncomp; c++) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nblocks; i += RED_THREADS) s += partial[(size_t)c * nblocks + i];
+    const double t = block_sum(s, sh);
+    if (threadIdx.x == 0) out[c] = t;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline dim3 grid_for(uint32_t n, int bs) { return dim3((n + bs - 1) / bs); }
+
+template <class S>
+static KArgs<typename Store<S>::C> make_args(const LaunchCtx &L) {
+  using C = typename Store<S>::C;
+  KArgs<C> a;
+  a.g = L.g;
+  a.omega = (C)L.omega;
+  a.Fx = (C)L.Fu[0];
+  a.Fy = (C)L.Fu[1];
+  a.Fz = (C)L.Fu[2];
+  a.Fxa = (const C *)L.Fx;
+  a.Fya = (const C *)L.Fy;
+  a.Fza = (const C *)L.Fz;
+  a.links = L.links;
+  a.uwx = (const C *)L.uwx;
+  a.uwy = (const C *)L.uwy;
+  a.uwz = (const C *)L.uwz;
+  a.rho = (C *)L.rho;
+  a.ux = (C *)L.ux;
+  a.uy = (C *)L.uy;
+  a.uz = (C *)L.uz;
+  a.status = L.status;
+  a.first_bad = L.first_bad;
+  return a;
+}
+
+template <class S, int FORCE>
+static hipError_t stream_dispatch(const LaunchCtx &L, const void *fin, void *fout, int mode) {
+  auto a = make_args<S>(L);
+  const dim3 grid = grid_for(L.g.ncell, L.block);
+  const dim3 blk(L.block);
+  const S *in = (const S *)fin;
+  S *out = (S *)fout;
+  switch (mode) {
+    case STREAM_ONLY:
+      hipLaunchKernelGGL((k_stream<S, FORCE, false, false>), grid, blk, 0, L.stream, in, out, L.mask, a);
+      break;
+    case STREAM_MACRO:
+      hipLaunchKernelGGL((k_stream<S, FORCE, false, true>), grid, blk, 0, L.stream, in, out, L.mask, a);
+      break;
+    case STREAM_COLLIDE:
+      hipLaunchKernelGGL((k_stream<S, FORCE, true, false>), grid, blk, 0, L.stream, in, out, L.mask, a);
+      break;
+    case STREAM_COLLIDE_MACRO:
+      hipLaunchKernelGGL((k_stream<S, FORCE, true, true>), grid, blk, 0, L.stream, in, out, L.mask, a);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <class S>
+static hipError_t stream_s(const LaunchCtx &L, const void *fin, void *fout, int mode) {
+  switch (L.force_mode) {
+    case 0: return stream_dispatch<S, 0>(L, fin, fout, mode);
+    case 1: return stream_dispatch<S, 1>(L, fin, fout, mode);
+    case 2: return stream_dispatch<S, 2>(L, fin, fout, mode);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_stream(const LaunchCtx &L, const void *fin, void *fout, int mode) {
+  switch (L.storage) {
+    case 0: return stream_s<double>(L, fin, fout, mode);
+    case 1: return stream_s<float>(L, fin, fout, mode);
+    case 2: return stream_s<__half>(L, fin, fout, mode);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <class S, int FORCE>
+static hipError_t collide_dispatch(const LaunchCtx &L, void *buf, bool macro_only) {
+  auto a = make_args<S>(L);
+  const dim3 grid = grid_for(L.g.ncell, L.block);
+  if (macro_only)
+    hipLaunchKernelGGL((k_macro<S, FORCE>), grid, dim3(L.block), 0, L.stream, (const S *)buf, L.mask, L.macro_sel, a);
+  else
+    hipLaunchKernelGGL((k_collide<S, FORCE>), grid, dim3(L.block), 0, L.stream, (S *)buf, L.mask, a);
+  return hipGetLastError();
+}
+
+template <class S>
+static hipError_t collide_s(const LaunchCtx &L, void *buf, bool macro_only) {
+  switch (L.force_mode) {
+    case 0: return collide_dispatch<S, 0>(L, buf, macro_only);
+    case 1: return collide_dispatch<S, 1>(L, buf, macro_only);
+    case 2: return collide_dispatch<S, 2>(L, buf, macro_only);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_collide(const LaunchCtx &L, void *buf) {
+  switch (L.storage) {
+    case 0: return collide_s<double>(L, buf, false);
+    case 1: return collide_s<float>(L, buf, false);
+    case 2: return collide_s<__half>(L, buf, false);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_macro(const LaunchCtx &L, const void *buf) {
+  switch (L.storage) {
+    case 0: return collide_s<double>(L, (void *)buf, true);
+    case 1: return collide_s<float>(L, (void *)buf, true);
+    case 2: return collide_s<__half>(L, (void *)buf, true);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_init(const LaunchCtx &L, void *A, void *B) {
+  const dim3 grid = grid_for(L.g.ncell, L.block);
+  switch (L.storage) {
+    case 0:
+      hipLaunchKernelGGL(k_init<double>, grid, dim3(L.block), 0, L.stream, (double *)A, (double *)B, L.mask, make_args<double>(L));
+      break;
+    case 1:
+      hipLaunchKernelGGL(k_init<float>, grid, dim3(L.block), 0, L.stream, (float *)A, (float *)B, L.mask, make_args<float>(L));
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_init<__half>, grid, dim3(L.block), 0, L.stream, (__half *)A, (__half *)B, L.mask, make_args<__half>(L));
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_tables(const LaunchCtx &L, uint8_t *mask, uint32_t *links, void *uwx,
+                         void *uwy, void *uwz, const int16_t *type) {
+  const dim3 grid = grid_for(L.g.ncell, L.block);
+  if (L.storage == 0)
+    hipLaunchKernelGGL(k_tables<double>, grid, dim3(L.block), 0, L.stream, type, mask, links,
+                       (double *)uwx, (double *)uwy, (double *)uwz, L.g, (const double *)L.ux,
+                       (const double *)L.uy, (const double *)L.uz);
+  else
+    hipLaunchKernelGGL(k_tables<float>, grid, dim3(L.block), 0, L.stream, type, mask, links,
+                       (float *)uwx, (float *)uwy, (float *)uwz, L.g, (const float *)L.ux,
+                       (const float *)L.uy, (const float *)L.uz);
+  return hipGetLastError();
+}
+
+hipError_t launch_links(const LaunchCtx &L, int32_t *src, uint8_t *kind) {
+  hipLaunchKernelGGL(k_links, grid_for(L.g.ncell, L.block), dim3(L.block), 0, L.stream, L.mask, L.links, L.g, src, kind);
+  return hipGetLastError();
+}
+
+hipError_t launch_f_out(const LaunchCtx &L, const void *buf, double *out) {
+  const dim3 grid = grid_for(L.g.ncell, L.block);
+  switch (L.storage) {
+    case 0:
+      hipLaunchKernelGGL(k_f_out<double>, grid, dim3(L.block), 0, L.stream, (const double *)buf, L.mask, make_args<double>(L), out);
+      break;
+    case 1:
+      hipLaunchKernelGGL(k_f_out<float>, grid, dim3(L.block), 0, L.stream, (const float *)buf, L.mask, make_args<float>(L), out);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_f_out<__half>, grid, dim3(L.block), 0, L.stream, (const __half *)buf, L.mask, make_args<__half>(L), out);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+int reduce_blocks(uint32_t n) {
+  // fixed partition independent of the device: 4096 cells per thread block
+  return (int)((n + 4095) / 4096);
+}
+
+hipError_t launch_reduce(const LaunchCtx &L, int what, double *partial, double *out) {
+  const uint32_t n = L.g.ncell;
+  const int nb = reduce_blocks(n);
+  if (L.storage == 0)
+    hipLaunchKernelGGL(k_reduce_1<double>, dim3(nb), dim3(RED_THREADS), 0, L.stream, L.mask, (const double *)L.rho,
+                       (const double *)L.ux, (const double *)L.uy, (const double *)L.uz, n, 4096u, what, partial);
+  else
+    hipLaunchKernelGGL(k_reduce_1<float>, dim3(nb), dim3(RED_THREADS), 0, L.stream, L.mask, (const float *)L.rho,
+                       (const float *)L.ux, (const float *)L.uy, (const float *)L.uz, n, 4096u, what, partial);
+  hipLaunchKernelGGL(k_reduce_2, dim3(1), dim3(RED_THREADS), 0, L.stream, partial, nb, what == 1 ? 3 : 1, out);
+  return hipGetLastError();
+}
+
+}  // namespace lbm

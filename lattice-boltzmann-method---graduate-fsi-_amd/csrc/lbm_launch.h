@@ -1,0 +1,54 @@
+// lbm_launch.h -- host-side launch interface between the C-ABI layer and the
+// kernel translation unit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lbm {
+
+struct Geo {
+  uint32_t nx, ny, nz;  // padded extents Nx+2, Ny+2, Nz+2
+  uint32_t plane;       // nx*ny
+  uint32_t ncell;       // nx*ny*nz
+  uint32_t ps;          // population plane stride (elements), >= ncell
+};
+
+// stream kernel variants
+enum StreamMode {
+  STREAM_ONLY = 0,          // LBM::Streaming alone (store f)
+  STREAM_MACRO = 1,         // Streaming + MacroProp (store f, emit rho,u)
+  STREAM_COLLIDE = 2,       // Streaming + MacroProp + next Collision (store f*)
+  STREAM_COLLIDE_MACRO = 3  // as STREAM_COLLIDE, and emit rho,u
+};
+
+struct LaunchCtx {
+  Geo g;
+  int storage;     // LBM_STORE_*
+  int force_mode;  // 0 none, 1 per-cell field, 2 uniform
+  int block;       // threads per block
+  double omega;
+  double Fu[3];
+  const void *Fx, *Fy, *Fz;
+  const uint32_t *links;
+  const void *uwx, *uwy, *uwz;
+  void *rho, *ux, *uy, *uz;
+  const uint8_t *mask;
+  uint8_t macro_sel;  // k_macro: cells whose mask has any of these bits
+  int *status;
+  unsigned long long *first_bad;
+  hipStream_t stream;
+};
+
+hipError_t launch_stream(const LaunchCtx &L, const void *fin, void *fout, int mode);
+hipError_t launch_collide(const LaunchCtx &L, void *buf);
+hipError_t launch_macro(const LaunchCtx &L, const void *buf);
+hipError_t launch_init(const LaunchCtx &L, void *A, void *B);
+hipError_t launch_tables(const LaunchCtx &L, uint8_t *mask, uint32_t *links, void *uwx,
+                         void *uwy, void *uwz, const int16_t *type);
+hipError_t launch_links(const LaunchCtx &L, int32_t *src, uint8_t *kind);
+hipError_t launch_f_out(const LaunchCtx &L, const void *buf, double *out);
+int reduce_blocks(uint32_t n);
+hipError_t launch_reduce(const LaunchCtx &L, int what, double *partial, double *out);
+
+}  // namespace lbm

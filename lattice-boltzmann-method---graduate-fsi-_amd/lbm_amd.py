@@ -1,0 +1,251 @@
+"""ctypes binding of liblbm_hip.so (the C-ABI declared in include/lbm_hip.h).
+
+This is plumbing for Python callers (tests, the benchmark): every call goes
+straight to the gfx950 library.  There is no fallback: if the library is not
+built or cannot be loaded, importing this module's Lattice raises.
+
+Array convention: numpy arrays of shape (Nx+2, Ny+2, Nz+2) in C order, i.e.
+the reference's LBM::fluid[i][j][k] indexing with k fastest (lbm.cpp:17-28).
+Populations are (Nx+2, Ny+2, Nz+2, 19) in the reference's D3Q19 order.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblbm_hip.so")
+
+LBM_OK = 0
+LBM_ERR_FEQ_NEGATIVE = 1
+LBM_ERR_DENSITY_RANGE = 2
+LBM_ERR_TEMPERATURE_RANGE = 3
+LBM_ERR_ARG = -1
+LBM_ERR_STATE = -2
+LBM_ERR_HIP = -3
+LBM_ERR_NOMEM = -4
+LBM_ERR_UNSUPPORTED = -5
+
+TYPE_F, TYPE_S, TYPE_OUT, TYPE_IN, TYPE_P, TYPE_L, TYPE_T, TYPE_Q = range(8)
+
+STORE_F64, STORE_F32, STORE_F16S = 0, 1, 2
+STORAGE_NAMES = {"f64": STORE_F64, "f32": STORE_F32, "f16s": STORE_F16S}
+
+REDUCE_MASS, REDUCE_MOMENTUM, REDUCE_KINETIC = 0, 1, 2
+
+
+class LBMError(RuntimeError):
+    def __init__(self, code: int, what: str, msg: str):
+        super().__init__(f"{what}: {msg} (code {code})")
+        self.code = code
+
+
+class _Desc(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32),
+        ("ny", C.c_int32),
+        ("nz", C.c_int32),
+        ("lattice", C.c_int32),
+        ("storage", C.c_int32),
+        ("device", C.c_int32),
+        ("tau", C.c_double),
+        ("reserved", C.c_int32 * 8),
+    ]
+
+
+# name -> (restype, argtypes) for every entry point of include/lbm_hip.h
+_P = C.c_void_p
+_D = C.POINTER(C.c_double)
+_SIGNATURES = {
+    "lbm_create": (C.c_int, [C.POINTER(_Desc), C.POINTER(_P)]),
+    "lbm_destroy": (C.c_int, [_P]),
+    "lbm_upload_cells": (C.c_int, [_P, C.POINTER(C.c_int16), _D, _D, _D, _D, _D, _D, _D]),
+    "lbm_init": (C.c_int, [_P]),
+    "lbm_collide": (C.c_int, [_P]),
+    "lbm_stream": (C.c_int, [_P]),
+    "lbm_macro": (C.c_int, [_P]),
+    "lbm_step": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "lbm_sync": (C.c_int, [_P]),
+    "lbm_status": (C.c_int, [_P, C.POINTER(C.c_int64)]),
+    "lbm_download_macro": (C.c_int, [_P, _D, _D, _D, _D]),
+    "lbm_download_f": (C.c_int, [_P, _D]),
+    "lbm_download_force": (C.c_int, [_P, _D, _D, _D]),
+    "lbm_reduce_fp64": (C.c_int, [_P, C.c_int32, _D]),
+    "lbm_export_links": (
+        C.c_int,
+        [_P, C.POINTER(C.c_int16), C.POINTER(C.c_int32), C.POINTER(C.c_uint8)],
+    ),
+    "lbm_ncells": (C.c_int64, [_P]),
+    "lbm_nfluid": (C.c_int64, [_P]),
+    "lbm_storage": (C.c_int32, [_P]),
+    "lbm_time_steps": (C.c_int, [_P, C.c_int32, _D]),
+    "lbm_bytes_per_cell": (C.c_double, [_P]),
+    "lbm_error_string": (C.c_char_p, [C.c_int]),
+}
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Load liblbm_hip.so and bind every entry point.  Raises if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is not built: run __graft_entry__.build() (hipcc, gfx950) first"
+        )
+    lib = C.CDLL(path)
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGNATURES)
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(_D)
+
+
+def _check(code: int, what: str) -> int:
+    if code != LBM_OK:
+        msg = load_library().lbm_error_string(code).decode()
+        raise LBMError(code, what, msg)
+    return code
+
+
+class Lattice:
+    """One device lattice, the counterpart of the reference's LBM object.
+
+    Method names follow LBM (LBM.h:620-664): init / collision / streaming /
+    macroprop, plus step(n) (the fused fast path) and downloads.
+    """
+
+    def __init__(self, Nx: int, Ny: int, Nz: int, tau: float, storage="f64", device=0):
+        lib = load_library()
+        if isinstance(storage, str):
+            storage = STORAGE_NAMES[storage]
+        d = _Desc(Nx, Ny, Nz, 19, storage, device, float(tau))
+        h = _P()
+        _check(lib.lbm_create(C.byref(d), C.byref(h)), "lbm_create")
+        self._lib = lib
+        self._h = h
+        self.Nx, self.Ny, self.Nz, self.tau = Nx, Ny, Nz, float(tau)
+        self.storage = storage
+        self.shape = (Nx + 2, Ny + 2, Nz + 2)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.lbm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- set-up ---------------------------------------------------------------
+    def _f64(self, a):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.float64), self.shape))
+        return a
+
+    def upload(self, type=None, rho=None, ux=None, uy=None, uz=None, Fx=None, Fy=None, Fz=None):
+        t = None
+        if type is not None:
+            t = np.ascontiguousarray(np.broadcast_to(np.asarray(type, dtype=np.int16), self.shape))
+        arrs = [self._f64(a) for a in (rho, ux, uy, uz, Fx, Fy, Fz)]
+        tp = None if t is None else t.ctypes.data_as(C.POINTER(C.c_int16))
+        _check(self._lib.lbm_upload_cells(self._h, tp, *[_dp(a) for a in arrs]), "lbm_upload_cells")
+
+    # -- the reference's methods ------------------------------------------------
+    def init(self):
+        _check(self._lib.lbm_init(self._h), "lbm_init")
+
+    def collision(self):
+        _check(self._lib.lbm_collide(self._h), "lbm_collide")
+
+    def streaming(self):
+        _check(self._lib.lbm_stream(self._h), "lbm_stream")
+
+    def macroprop(self):
+        _check(self._lib.lbm_macro(self._h), "lbm_macro")
+
+    def step(self, n: int = 1):
+        _check(self._lib.lbm_step(self._h, int(n), 1), "lbm_step")
+
+    def sync(self) -> int:
+        return self._lib.lbm_sync(self._h)
+
+    def status(self):
+        fb = C.c_int64(-1)
+        code = self._lib.lbm_status(self._h, C.byref(fb))
+        return code, fb.value
+
+    def time_steps(self, n: int) -> float:
+        ms = C.c_double(0.0)
+        _check(self._lib.lbm_time_steps(self._h, int(n), C.byref(ms)), "lbm_time_steps")
+        return ms.value
+
+    # -- downloads -------------------------------------------------------------------
+    def macro(self):
+        out = [np.empty(self.shape, np.float64) for _ in range(4)]
+        _check(self._lib.lbm_download_macro(self._h, *[_dp(a) for a in out]), "lbm_download_macro")
+        return tuple(out)
+
+    def force(self):
+        out = [np.empty(self.shape, np.float64) for _ in range(3)]
+        _check(self._lib.lbm_download_force(self._h, *[_dp(a) for a in out]), "lbm_download_force")
+        return tuple(out)
+
+    def populations(self):
+        f = np.empty(self.shape + (19,), np.float64)
+        _check(self._lib.lbm_download_f(self._h, _dp(f)), "lbm_download_f")
+        return f
+
+    def links(self):
+        t = np.empty(self.shape, np.int16)
+        src = np.empty(self.shape + (19,), np.int32)
+        kind = np.empty(self.shape + (19,), np.uint8)
+        _check(
+            self._lib.lbm_export_links(
+                self._h,
+                t.ctypes.data_as(C.POINTER(C.c_int16)),
+                src.ctypes.data_as(C.POINTER(C.c_int32)),
+                kind.ctypes.data_as(C.POINTER(C.c_uint8)),
+            ),
+            "lbm_export_links",
+        )
+        return t, src, kind
+
+    def reduce(self, what: int):
+        out = np.zeros(3, np.float64)
+        _check(self._lib.lbm_reduce_fp64(self._h, int(what), _dp(out)), "lbm_reduce_fp64")
+        return out[:3] if what == REDUCE_MOMENTUM else float(out[0])
+
+    @property
+    def ncells(self) -> int:
+        return int(self._lib.lbm_ncells(self._h))
+
+    @property
+    def nfluid(self) -> int:
+        return int(self._lib.lbm_nfluid(self._h))
+
+    @property
+    def bytes_per_cell(self) -> float:
+        return float(self._lib.lbm_bytes_per_cell(self._h))

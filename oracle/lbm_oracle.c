@@ -1,0 +1,654 @@
+/*
+ * lbm_oracle.c -- CPU FP64 restatement of the reference lattice Boltzmann step.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path and the timed CPU
+ * baseline.  Nothing in the product library links or calls this file.
+ *
+ * Every function cites the reference lines (lbm.cpp / LBM.h / main.cpp of
+ * DestinyFlow01/Lattice-Boltzmann-Method---Graduate-FSI-) it restates.  The
+ * arithmetic is written in the reference's own expression order (including
+ * its pow() calls) so that a gcc build without -ffast-math reproduces the
+ * reference's FP64 rounding wherever the reference itself is deterministic.
+ *
+ * Deliberate differences from the reference, none of which changes a value:
+ *  - std::terminate() becomes an error code (ORC_ERR_*); the sweep finishes.
+ *  - Streaming builds no per-cell std::vector lists (lbm.cpp:207-249): they
+ *    feed only the thermal Neumann wall, which this file does not restate.
+ *    The wall velocity (lbm.cpp:941-962) is computed once per wall cell
+ *    instead of once per wall link; it is the same value each time.
+ *  - Compile-time case flags become runtime fields.
+ */
+#include "lbm_oracle.h"
+
+#include <math.h>
+#include <omp.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* LBM.h:74-87: dt = dx = dy = dz = 1, cs2 = pow(dx,2)/(3*pow(dt,2)) */
+#define ORC_DT 1.0
+#define ORC_DX 1.0
+#define ORC_DY 1.0
+#define ORC_DZ 1.0
+
+/* The reference's LATTICE for its D3Q19 build without IBM/ENERGY
+ * (LBM.h:558-597): 2 + 6 (pad) + 4*8 + 19*8 + 19*8 + 3*8 = 368 bytes. */
+typedef struct orc_cell {
+  short type;
+  double density, ux, uy, uz;
+  double f[19], f_star[19];
+  double F[3];
+} orc_cell;
+
+struct orc_lbm {
+  int Nx, Ny, Nz;
+  double tau;
+  int q;
+  const int *cx, *cy, *cz, *opp;
+  const double *w;
+  double cs2;
+  orc_cell ***fluid; /* [Nx+2][Ny+2][Nz+2], lbm.cpp:17-28 */
+  double *Fm;        /* implicit-IBM Fm[3] per cell (LBM.h:571-573) */
+};
+
+/* D3Q19, LBM.h:542-552 */
+static const int d3q19_cx[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0};
+static const int d3q19_cy[19] = {0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1};
+static const int d3q19_cz[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1};
+static const double d3q19_w[19] = {1.0 / 3, 1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 18,
+                                   1.0 / 18, 1.0 / 18, 1.0 / 36, 1.0 / 36, 1.0 / 36,
+                                   1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36,
+                                   1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36};
+static const int d3q19_opp[19] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
+
+/* D2Q9, LBM.h:530-540 */
+static const int d2q9_cx[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+static const int d2q9_cy[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+static const int d2q9_cz[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+static const double d2q9_w[9] = {4.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 9,
+                                 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36};
+static const int d2q9_opp[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
+
+static int g_threads = 0;
+
+void orc_set_threads(int nthreads) {
+  g_threads = nthreads;
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+}
+
+int orc_get_threads(void) { return omp_get_max_threads(); }
+
+int orc_sizeof_cell(void) { return (int)sizeof(orc_cell); }
+
+/* LBM::LBM, lbm.cpp:17-28: one allocation per (i,j) row of Nz+2 cells.  Cell
+ * defaults follow LATTICE's member initialisers (LBM.h:560-566). */
+orc_lbm *orc_create(int Nx, int Ny, int Nz, double tau, int q) {
+  if (Nx < 1 || Ny < 1 || Nz < 1 || (q != 19 && q != 9)) return NULL;
+  orc_lbm *lb = (orc_lbm *)calloc(1, sizeof(orc_lbm));
+  lb->Nx = Nx;
+  lb->Ny = Ny;
+  lb->Nz = Nz;
+  lb->tau = tau;
+  lb->q = q;
+  if (q == 19) {
+    lb->cx = d3q19_cx; lb->cy = d3q19_cy; lb->cz = d3q19_cz;
+    lb->opp = d3q19_opp; lb->w = d3q19_w;
+  } else {
+    lb->cx = d2q9_cx; lb->cy = d2q9_cy; lb->cz = d2q9_cz;
+    lb->opp = d2q9_opp; lb->w = d2q9_w;
+  }
+  lb->cs2 = pow(ORC_DX, 2) / (3 * pow(ORC_DT, 2)); /* LBM.h:86 */
+  lb->fluid = (orc_cell ***)malloc(sizeof(orc_cell **) * (Nx + 2));
+  for (int i = 0; i < Nx + 2; i++) {
+    lb->fluid[i] = (orc_cell **)malloc(sizeof(orc_cell *) * (Ny + 2));
+    for (int j = 0; j < Ny + 2; j++) {
+      orc_cell *row = (orc_cell *)malloc(sizeof(orc_cell) * (Nz + 2));
+      for (int k = 0; k < Nz + 2; k++) {
+        memset(&row[k], 0, sizeof(orc_cell));
+        row[k].type = ORC_TYPE_F;
+        row[k].density = 1.0;
+      }
+      lb->fluid[i][j] = row;
+    }
+  }
+  lb->Fm = (double *)calloc((size_t)3 * (Nx + 2) * (Ny + 2) * (Nz + 2), sizeof(double));
+  return lb;
+}
+
+void orc_destroy(orc_lbm *lb) {
+  if (!lb) return;
+  for (int i = 0; i < lb->Nx + 2; i++) {
+    for (int j = 0; j < lb->Ny + 2; j++) free(lb->fluid[i][j]);
+    free(lb->fluid[i]);
+  }
+  free(lb->fluid);
+  free(lb->Fm);
+  free(lb);
+}
+
+int64_t orc_ncells(const orc_lbm *lb) {
+  return (int64_t)(lb->Nx + 2) * (lb->Ny + 2) * (lb->Nz + 2);
+}
+
+int orc_q(const orc_lbm *lb) { return lb->q; }
+
+void orc_tables(const orc_lbm *lb, int *cx, int *cy, int *cz, int *opp, double *w) {
+  for (int l = 0; l < lb->q; l++) {
+    cx[l] = lb->cx[l]; cy[l] = lb->cy[l]; cz[l] = lb->cz[l];
+    opp[l] = lb->opp[l]; w[l] = lb->w[l];
+  }
+}
+
+#define LIN(lb, i, j, k) ((((int64_t)(i)) * ((lb)->Ny + 2) + (j)) * ((lb)->Nz + 2) + (k))
+
+void orc_set_cells(orc_lbm *lb, const int16_t *type, const double *rho,
+                   const double *ux, const double *uy, const double *uz,
+                   const double *Fx, const double *Fy, const double *Fz) {
+  for (int i = 0; i < lb->Nx + 2; i++)
+    for (int j = 0; j < lb->Ny + 2; j++)
+      for (int k = 0; k < lb->Nz + 2; k++) {
+        int64_t n = LIN(lb, i, j, k);
+        orc_cell *c = &lb->fluid[i][j][k];
+        if (type) c->type = type[n];
+        if (rho) c->density = rho[n];
+        if (ux) c->ux = ux[n];
+        if (uy) c->uy = uy[n];
+        if (uz) c->uz = uz[n];
+        if (Fx) c->F[0] = Fx[n];
+        if (Fy) c->F[1] = Fy[n];
+        if (Fz) c->F[2] = Fz[n];
+      }
+}
+
+void orc_get_macro(const orc_lbm *lb, double *rho, double *ux, double *uy, double *uz) {
+  for (int i = 0; i < lb->Nx + 2; i++)
+    for (int j = 0; j < lb->Ny + 2; j++)
+      for (int k = 0; k < lb->Nz + 2; k++) {
+        int64_t n = LIN(lb, i, j, k);
+        const orc_cell *c = &lb->fluid[i][j][k];
+        if (rho) rho[n] = c->density;
+        if (ux) ux[n] = c->ux;
+        if (uy) uy[n] = c->uy;
+        if (uz) uz[n] = c->uz;
+      }
+}
+
+void orc_get_force(const orc_lbm *lb, double *Fx, double *Fy, double *Fz) {
+  for (int i = 0; i < lb->Nx + 2; i++)
+    for (int j = 0; j < lb->Ny + 2; j++)
+      for (int k = 0; k < lb->Nz + 2; k++) {
+        int64_t n = LIN(lb, i, j, k);
+        const orc_cell *c = &lb->fluid[i][j][k];
+        if (Fx) Fx[n] = c->F[0];
+        if (Fy) Fy[n] = c->F[1];
+        if (Fz) Fz[n] = c->F[2];
+      }
+}
+
+void orc_get_fm(const orc_lbm *lb, double *Fmx, double *Fmy, double *Fmz) {
+  int64_t nc = orc_ncells(lb);
+  for (int64_t n = 0; n < nc; n++) {
+    if (Fmx) Fmx[n] = lb->Fm[3 * n + 0];
+    if (Fmy) Fmy[n] = lb->Fm[3 * n + 1];
+    if (Fmz) Fmz[n] = lb->Fm[3 * n + 2];
+  }
+}
+
+static void copy_pops(orc_lbm *lb, double *dst, const double *src, int to_host, int star) {
+  int q = lb->q;
+  for (int i = 0; i < lb->Nx + 2; i++)
+    for (int j = 0; j < lb->Ny + 2; j++)
+      for (int k = 0; k < lb->Nz + 2; k++) {
+        int64_t n = LIN(lb, i, j, k);
+        orc_cell *c = &lb->fluid[i][j][k];
+        double *p = star ? c->f_star : c->f;
+        for (int l = 0; l < q; l++) {
+          if (to_host) dst[n * q + l] = p[l];
+          else p[l] = src[n * q + l];
+        }
+      }
+}
+
+void orc_get_f(const orc_lbm *lb, double *f) { copy_pops((orc_lbm *)lb, f, NULL, 1, 0); }
+void orc_get_fstar(const orc_lbm *lb, double *fs) { copy_pops((orc_lbm *)lb, fs, NULL, 1, 1); }
+void orc_set_f(orc_lbm *lb, const double *f) { copy_pops(lb, NULL, f, 0, 0); }
+void orc_set_fstar(orc_lbm *lb, const double *fs) { copy_pops(lb, NULL, fs, 0, 1); }
+
+/* LBM::Calc_feq, lbm.cpp:628-633: the cubic equilibrium, written in the
+ * reference's expression order. */
+static inline double calc_feq(const orc_lbm *lb, const orc_cell *c, int l) {
+  const double cs2 = lb->cs2;
+  double u_dot_u = pow(c->ux, 2) + pow(c->uy, 2) + pow(c->uz, 2);
+  double c_dot_u = c->ux * lb->cx[l] + c->uy * lb->cy[l] + c->uz * lb->cz[l];
+  double f_eq = c->density * lb->w[l] *
+                (1 + c_dot_u / cs2 + pow(c_dot_u, 2) / (2 * cs2 * cs2) - u_dot_u / (2 * cs2) +
+                 pow(c_dot_u, 3) / (6 * pow(cs2, 3)) - c_dot_u * u_dot_u / (2 * cs2 * cs2));
+  return f_eq;
+}
+
+double orc_feq(const orc_lbm *lb, double rho, double ux, double uy, double uz, int l) {
+  orc_cell c;
+  memset(&c, 0, sizeof c);
+  c.density = rho; c.ux = ux; c.uy = uy; c.uz = uz;
+  return calc_feq(lb, &c, l);
+}
+
+/* LBM::Init, lbm.cpp:31-57: f = f_star = feq for every cell, ghosts and
+ * solids included (the type filter is commented out, lbm.cpp:37). */
+void orc_init(orc_lbm *lb) {
+  const int Nx = lb->Nx, Ny = lb->Ny, Nz = lb->Nz, q = lb->q;
+#pragma omp parallel for
+  for (int i = 0; i <= Nx + 1; i++)
+    for (int j = 0; j <= Ny + 1; j++)
+      for (int k = 0; k <= Nz + 1; k++)
+        for (int l = 0; l < q; l++) {
+          double f_eq = calc_feq(lb, &lb->fluid[i][j][k], l);
+          lb->fluid[i][j][k].f[l] = f_eq;
+          lb->fluid[i][j][k].f_star[l] = f_eq;
+        }
+}
+
+/* LBM::ConvLBM, lbm.cpp:75-118: BGK with the Guo source.  Returns 1 when an
+ * equilibrium is negative (the reference terminates, lbm.cpp:114-117). */
+static inline int conv_lbm(const orc_lbm *lb, orc_cell *c) {
+  const double cs2 = lb->cs2;
+  const double omega = ORC_DT / lb->tau;
+  double ux = c->ux, uy = c->uy, uz = c->uz;
+  double Fx = c->F[0], Fy = c->F[1], Fz = c->F[2];
+  double F_dot_u = Fx * ux + Fy * uy + Fz * uz;
+  int bad = 0;
+  for (int l = 0; l < lb->q; l++) {
+    double c_dot_u = ux * lb->cx[l] + uy * lb->cy[l] + uz * lb->cz[l];
+    double f_eq = calc_feq(lb, c, l);
+    double F_dot_c = Fx * lb->cx[l] + Fy * lb->cy[l] + Fz * lb->cz[l];
+    double Source = (1 - omega * 0.5) * lb->w[l] *
+                    (F_dot_c / cs2 + (F_dot_c) * (c_dot_u) / (cs2 * cs2) - F_dot_u / cs2);
+    c->f_star[l] = c->f[l] * (1 - omega) + omega * f_eq + Source * ORC_DT;
+    if (f_eq < 0) bad = 1;
+  }
+  return bad;
+}
+
+/* LBM::Collision, lbm.cpp:60-73: TYPE_F cells only. */
+int orc_collision(orc_lbm *lb) {
+  const int Nx = lb->Nx, Ny = lb->Ny, Nz = lb->Nz;
+  int bad = 0;
+#pragma omp parallel for reduction(| : bad)
+  for (int i = 0; i <= Nx + 1; i++)
+    for (int j = 0; j <= Ny + 1; j++)
+      for (int k = 0; k <= Nz + 1; k++)
+        if (lb->fluid[i][j][k].type == ORC_TYPE_F) bad |= conv_lbm(lb, &lb->fluid[i][j][k]);
+  return bad ? ORC_ERR_FEQ_NEGATIVE : ORC_OK;
+}
+
+#define WRAP(a, n) (((a) + (n)) % (n))
+
+/* LBM::Determine_Wall_Velocity, lbm.cpp:941-962: the mean velocity of the
+ * cell's solid neighbours whose velocity is nonzero, summed in l order. */
+void orc_wall_velocity(const orc_lbm *lb, int i, int j, int k, double uw[3]) {
+  int num = 0;
+  uw[0] = uw[1] = uw[2] = 0.0;
+  for (int l = 0; l < lb->q; l++) {
+    int i_nb = WRAP(i + lb->cx[l], lb->Nx + 2);
+    int j_nb = WRAP(j + lb->cy[l], lb->Ny + 2);
+    int k_nb = WRAP(k + lb->cz[l], lb->Nz + 2);
+    const orc_cell *nb = &lb->fluid[i_nb][j_nb][k_nb];
+    if (nb->type == ORC_TYPE_S) {
+      if (nb->ux != 0 || nb->uy != 0 || nb->uz != 0) {
+        uw[0] += nb->ux;
+        uw[1] += nb->uy;
+        uw[2] += nb->uz;
+        num++;
+      }
+    }
+  }
+  if (num != 0) {
+    uw[0] /= num; uw[1] /= num; uw[2] /= num;
+  }
+}
+
+/* second-order inflow equilibrium of the face loops (lbm.cpp:337 etc.) */
+static inline void inflow_feq(const orc_lbm *lb, orc_cell *dst, const orc_cell *src) {
+  const double cs2 = lb->cs2;
+  double rho = src->density, ux = src->ux, uy = src->uy, uz = src->uz;
+  double u_dot_u = pow(ux, 2) + pow(uy, 2) + pow(uz, 2);
+  for (int l = 0; l < lb->q; l++) {
+    double c_dot_u = ux * lb->cx[l] + uy * lb->cy[l] + uz * lb->cz[l];
+    double f_eq = rho * lb->w[l] *
+                  (1 + c_dot_u / cs2 + pow(c_dot_u, 2) / (2 * cs2 * cs2) - u_dot_u / (2 * cs2));
+    dst->f[l] = f_eq;
+  }
+}
+
+static inline void copy3(orc_cell *dst, const orc_cell *src, int a, int b, int c) {
+  dst->f[a] = src->f[a];
+  dst->f[b] = src->f[b];
+  dst->f[c] = src->f[c];
+}
+
+/* LBM::Streaming, lbm.cpp:182-626. */
+void orc_streaming(orc_lbm *lb) {
+  const int Nx = lb->Nx, Ny = lb->Ny, Nz = lb->Nz, q = lb->q;
+  const double cs2 = lb->cs2;
+  orc_cell ***fl = lb->fluid;
+
+  /* fluid part, lbm.cpp:183-295: push from TYPE_F cells */
+#pragma omp parallel for
+  for (int i = 0; i <= Nx + 1; i++)
+    for (int j = 0; j <= Ny + 1; j++)
+      for (int k = 0; k <= Nz + 1; k++) {
+        if (fl[i][j][k].type != ORC_TYPE_F) continue;
+        int have_uw = 0;
+        double uwall[3] = {0.0, 0.0, 0.0};
+        for (int l = 0; l < q; l++) {
+          int i_nb = WRAP(i + lb->cx[l], Nx + 2);
+          int j_nb = WRAP(j + lb->cy[l], Ny + 2);
+          int k_nb = WRAP(k + lb->cz[l], Nz + 2);
+          orc_cell *nb = &fl[i_nb][j_nb][k_nb];
+          if (nb->type == ORC_TYPE_F) {
+            nb->f[l] = fl[i][j][k].f_star[l];
+          } else if (nb->type == ORC_TYPE_S) {
+            if (!have_uw) {
+              orc_wall_velocity(lb, i, j, k, uwall);
+              have_uw = 1;
+            }
+            double uwall_dot_c = uwall[0] * lb->cx[l] + uwall[1] * lb->cy[l] + uwall[2] * lb->cz[l];
+            fl[i][j][k].f[lb->opp[l]] = fl[i][j][k].f_star[l] - 2 * lb->w[l] * uwall_dot_c / cs2;
+          }
+          /* any other type: nothing is written (frozen link) */
+        }
+      }
+
+  /* Face loops, lbm.cpp:299-625, serial.  They use D2Q9 direction indices
+   * even in D3Q19 (the reference's own code) and the back face writes its
+   * inflow into k = Nz+1 (lbm.cpp:610).  None of them writes a TYPE_F cell. */
+  /* Left (i = 0), lbm.cpp:300-352 */
+  for (int j = 0; j <= Ny + 1; j++)
+    for (int k = 0; k <= Nz + 1; k++) {
+      orc_cell *c = &fl[0][j][k];
+      if (c->type == ORC_TYPE_OUT) copy3(c, &fl[1][j][k], 1, 5, 8);
+      else if (c->type == ORC_TYPE_IN) inflow_feq(lb, c, c);
+    }
+  /* Right (i = Nx+1), lbm.cpp:354-407 */
+  for (int j = 0; j <= Ny + 1; j++)
+    for (int k = 0; k <= Nz + 1; k++) {
+      orc_cell *c = &fl[Nx + 1][j][k];
+      if (c->type == ORC_TYPE_OUT) copy3(c, &fl[Nx][j][k], 3, 6, 7);
+      else if (c->type == ORC_TYPE_IN) inflow_feq(lb, c, c);
+    }
+  /* Top (j = Ny+1), lbm.cpp:409-462 */
+  for (int i = 0; i <= Nx + 1; i++)
+    for (int k = 0; k <= Nz + 1; k++) {
+      orc_cell *c = &fl[i][Ny + 1][k];
+      if (c->type == ORC_TYPE_OUT) copy3(c, &fl[i][Ny][k], 4, 7, 8);
+      else if (c->type == ORC_TYPE_IN) inflow_feq(lb, c, c);
+    }
+  /* Bottom (j = 0), lbm.cpp:464-516 */
+  for (int i = 0; i <= Nx + 1; i++)
+    for (int k = 0; k <= Nz + 1; k++) {
+      orc_cell *c = &fl[i][0][k];
+      if (c->type == ORC_TYPE_OUT) copy3(c, &fl[i][1][k], 2, 5, 6);
+      else if (c->type == ORC_TYPE_IN) inflow_feq(lb, c, c);
+    }
+  /* Front (k = Nz+1), lbm.cpp:518-570 */
+  for (int i = 0; i <= Nx + 1; i++)
+    for (int j = 0; j <= Ny + 1; j++) {
+      orc_cell *c = &fl[i][j][Nz + 1];
+      if (c->type == ORC_TYPE_OUT) copy3(c, &fl[i][j][Nz], 4, 7, 8);
+      else if (c->type == ORC_TYPE_IN) inflow_feq(lb, c, c);
+    }
+  /* Back (k = 0), lbm.cpp:572-625: reads cell k = 0, writes k = Nz+1 */
+  for (int i = 0; i <= Nx + 1; i++)
+    for (int j = 0; j <= Ny + 1; j++) {
+      orc_cell *c = &fl[i][j][0];
+      if (c->type == ORC_TYPE_OUT) copy3(c, &fl[i][j][1], 4, 7, 8);
+      else if (c->type == ORC_TYPE_IN) inflow_feq(lb, &fl[i][j][Nz + 1], c);
+    }
+}
+
+/* LBM::MacroProp, lbm.cpp:657-812, every cell (the type filter is commented
+ * out, lbm.cpp:667). */
+int orc_macroprop(orc_lbm *lb) {
+  const int Nx = lb->Nx, Ny = lb->Ny, Nz = lb->Nz, q = lb->q;
+  int bad = 0;
+#pragma omp parallel for reduction(| : bad)
+  for (int i = 0; i <= Nx + 1; i++)
+    for (int j = 0; j <= Ny + 1; j++)
+      for (int k = 0; k <= Nz + 1; k++) {
+        orc_cell *c = &lb->fluid[i][j][k];
+        double rho = 0, rho_ux = 0, rho_uy = 0, rho_uz = 0;
+        for (int l = 0; l < q; l++) {
+          rho += c->f[l];
+          rho_ux += c->f[l] * lb->cx[l];
+          rho_uy += c->f[l] * lb->cy[l];
+          rho_uz += c->f[l] * lb->cz[l];
+        }
+        c->density = rho;
+        c->ux = (rho_ux + 0.5 * ORC_DT * c->F[0]) / rho;
+        c->uy = (rho_uy + 0.5 * ORC_DT * c->F[1]) / rho;
+        c->uz = (rho_uz + 0.5 * ORC_DT * c->F[2]) / rho;
+        if (c->density < 0 || c->density > 100) bad = 1;
+      }
+  return bad ? ORC_ERR_DENSITY_RANGE : ORC_OK;
+}
+
+/* main.cpp:212-221 */
+int orc_run_plain(orc_lbm *lb, int nsteps) {
+  for (int s = 0; s < nsteps; s++) {
+    int e = orc_collision(lb);
+    if (e) return e;
+    orc_streaming(lb);
+    e = orc_macroprop(lb);
+    if (e) return e;
+  }
+  return ORC_OK;
+}
+
+void orc_export_links(const orc_lbm *lb, int32_t *src, uint8_t *kind) {
+  const int Nx = lb->Nx, Ny = lb->Ny, Nz = lb->Nz, q = lb->q;
+  for (int i = 0; i <= Nx + 1; i++)
+    for (int j = 0; j <= Ny + 1; j++)
+      for (int k = 0; k <= Nz + 1; k++) {
+        int64_t n = LIN(lb, i, j, k);
+        int fluid = lb->fluid[i][j][k].type == ORC_TYPE_F;
+        for (int m = 0; m < q; m++) {
+          if (!fluid) {
+            src[n * q + m] = -1;
+            kind[n * q + m] = 255;
+            continue;
+          }
+          /* the pushing cell is the one whose neighbour in direction m is
+           * this cell: (i - cx[m], ...) wrapped as in lbm.cpp:197-199 */
+          int is = WRAP(i - lb->cx[m], Nx + 2);
+          int js = WRAP(j - lb->cy[m], Ny + 2);
+          int ks = WRAP(k - lb->cz[m], Nz + 2);
+          short t = lb->fluid[is][js][ks].type;
+          src[n * q + m] = (int32_t)LIN(lb, is, js, ks);
+          kind[n * q + m] = t == ORC_TYPE_F ? 0 : (t == ORC_TYPE_S ? 1 : 2);
+        }
+      }
+}
+
+/* ---------------------------------------------------------------------------
+ * Implicit diffuse-interface IBM (lbm.cpp:1048-1417)
+ * ------------------------------------------------------------------------- */
+
+/* kernel(), lbm.cpp:1095-1102 */
+double orc_ibm_kernel(double r) {
+  double d;
+  if (fabs(r) >= 0 && fabs(r) <= 1)
+    d = (3 - 2 * fabs(r) + sqrt(1 + 4 * fabs(r) - 4 * pow(r, 2))) / 8;
+  else if (fabs(r) >= 1 && fabs(r) <= 2)
+    d = (5 - 2 * fabs(r) - sqrt(-7 + 12 * fabs(r) - 4 * pow(r, 2))) / 8;
+  else
+    d = 0;
+  return d;
+}
+
+/* NearestLatticeCoordinate, lbm.cpp:1074-1092: the box point with the
+ * largest positive projection on the marker normal (last one wins a tie). */
+void orc_ibm_nearest(int xmin, int xmax, int ymin, int ymax, int zmin, int zmax,
+                     double mx, double my, double mz, double nx, double ny, double nz,
+                     int *i, int *j, int *k) {
+  double dot = 0;
+  for (int x = xmin; x <= xmax; x++)
+    for (int y = ymin; y <= ymax; y++)
+      for (int z = zmin; z <= zmax; z++) {
+        double v0 = x - mx, v1 = y - my, v2 = z - mz;
+        double dot_trial = v0 * nx + v1 * ny + v2 * nz;
+        if (dot_trial >= dot && dot_trial > 0) {
+          *i = x; *j = y; *k = z;
+          dot = dot_trial;
+        }
+      }
+}
+
+static inline int ibm_cell(short t) {
+  return t == ORC_TYPE_F || t == ORC_TYPE_IN || t == ORC_TYPE_OUT;
+}
+
+static void ibm_interpolate(orc_lbm *lb, int nm, const double *mx, const double *my,
+                            const double *mz, double *ub, int xmin, int xmax, int ymin,
+                            int ymax, int zmin, int zmax) {
+#pragma omp parallel for
+  for (int n = 0; n < nm; n++) {
+    ub[3 * n + 0] = 0; ub[3 * n + 1] = 0; ub[3 * n + 2] = 0;
+    for (int i = xmin; i <= xmax; i++)
+      for (int j = ymin; j <= ymax; j++)
+        for (int k = zmin; k <= zmax; k++) {
+          const orc_cell *c = &lb->fluid[i][j][k];
+          if (!ibm_cell(c->type)) continue;
+          double rx = (i * ORC_DX - mx[n]) / ORC_DX, ry = (j * ORC_DY - my[n]) / ORC_DY,
+                 rz = (k * ORC_DZ - mz[n]) / ORC_DZ;
+          ub[3 * n + 0] += c->ux * orc_ibm_kernel(rx) * orc_ibm_kernel(ry) * orc_ibm_kernel(rz);
+          ub[3 * n + 1] += c->uy * orc_ibm_kernel(rx) * orc_ibm_kernel(ry) * orc_ibm_kernel(rz);
+          ub[3 * n + 2] += c->uz * orc_ibm_kernel(rx) * orc_ibm_kernel(ry) * orc_ibm_kernel(rz);
+        }
+  }
+}
+
+/* IBLBM, implicit branch, lbm.cpp:1135-1167 and 1260-1417. */
+int orc_ibm_step(orc_lbm *lb, int nm, double *mx, double *my, double *mz,
+                 const double *nx, const double *ny, const double *nz,
+                 double *ub, double *Fb, const double *ds,
+                 const double vcm[3], const double vcm_before[3],
+                 int xmin, int xmax, int ymin, int ymax, int zmin, int zmax,
+                 int m_max, double vsolid, double *out) {
+  const int Nx = lb->Nx, Ny = lb->Ny, Nz = lb->Nz, q = lb->q;
+  double Drag = 0, Lift = 0;
+  double *udes = (double *)malloc(sizeof(double) * 3 * (nm > 0 ? nm : 1));
+  int err = ORC_OK;
+
+  /* marker update, lbm.cpp:1154-1161 */
+  for (int n = 0; n < nm; n++) {
+    mx[n] += (vcm[0] + vcm_before[0]) * ORC_DT / 2;
+    my[n] += (vcm[1] + vcm_before[1]) * ORC_DT / 2;
+    mz[n] += (vcm[2] + vcm_before[2]) * ORC_DT / 2;
+    udes[3 * n + 0] = vcm[0];
+    udes[3 * n + 1] = vcm[1];
+    udes[3 * n + 2] = vcm[2];
+  }
+
+  /* unforced moments on F/IN/OUT cells of the whole lattice, F = 0,
+   * lbm.cpp:1262-1285 */
+#pragma omp parallel for
+  for (int i = 0; i <= Nx + 1; i++)
+    for (int j = 0; j <= Ny + 1; j++)
+      for (int k = 0; k <= Nz + 1; k++) {
+        orc_cell *c = &lb->fluid[i][j][k];
+        if (!ibm_cell(c->type)) continue;
+        double rho = 0, rho_u[3] = {0, 0, 0};
+        for (int l = 0; l < q; l++) {
+          rho += c->f[l];
+          rho_u[0] += c->f[l] * lb->cx[l];
+          rho_u[1] += c->f[l] * lb->cy[l];
+          rho_u[2] += c->f[l] * lb->cz[l];
+        }
+        c->density = rho;
+        c->ux = rho_u[0] / rho;
+        c->uy = rho_u[1] / rho;
+        c->uz = rho_u[2] / rho;
+        c->F[0] = 0; c->F[1] = 0; c->F[2] = 0;
+      }
+
+  /* unforced interpolation, lbm.cpp:1288-1309 */
+  ibm_interpolate(lb, nm, mx, my, mz, ub, xmin, xmax, ymin, ymax, zmin, zmax);
+
+  for (int m = 1; m <= m_max; m++) {
+    /* boundary force, serial, lbm.cpp:1314-1329 */
+    for (int n = 0; n < nm; n++) {
+      int i = -1, j = -1, k = 1;
+      orc_ibm_nearest(xmin, xmax, ymin, ymax, zmin, zmax, mx[n], my[n], mz[n], nx[n], ny[n],
+                      nz[n], &i, &j, &k);
+      if (i < 0 || j < 0) { /* the reference leaves i,j uninitialised here */
+        err = -1;
+        i = xmin; j = ymin;
+      }
+      double rho = lb->fluid[i][j][k].density;
+      Fb[3 * n + 0] = 2 * rho * (udes[3 * n + 0] - ub[3 * n + 0]) / ORC_DT;
+      Fb[3 * n + 1] = 2 * rho * (udes[3 * n + 1] - ub[3 * n + 1]) / ORC_DT;
+      Fb[3 * n + 2] = 2 * rho * (udes[3 * n + 2] - ub[3 * n + 2]) / ORC_DT;
+      Drag += -Fb[3 * n + 0] * ORC_DX * ds[n];
+      Lift += -Fb[3 * n + 1] * ORC_DY * ds[n];
+    }
+
+    /* spreading and velocity correction, lbm.cpp:1332-1360 */
+#pragma omp parallel for
+    for (int i = xmin; i <= xmax; i++)
+      for (int j = ymin; j <= ymax; j++)
+        for (int k = zmin; k <= zmax; k++) {
+          orc_cell *c = &lb->fluid[i][j][k];
+          if (!ibm_cell(c->type)) continue;
+          double *Fm = &lb->Fm[3 * LIN(lb, i, j, k)];
+          Fm[0] = 0; Fm[1] = 0; Fm[2] = 0;
+          for (int n = 0; n < nm; n++) {
+            double rx = (i * ORC_DX - mx[n]) / ORC_DX, ry = (j * ORC_DY - my[n]) / ORC_DY,
+                   rz = (k * ORC_DZ - mz[n]) / ORC_DZ;
+            Fm[0] += Fb[3 * n + 0] * ds[n] * orc_ibm_kernel(rx) * orc_ibm_kernel(ry) * orc_ibm_kernel(rz);
+            Fm[1] += Fb[3 * n + 1] * ds[n] * orc_ibm_kernel(rx) * orc_ibm_kernel(ry) * orc_ibm_kernel(rz);
+            Fm[2] += Fb[3 * n + 2] * ds[n] * orc_ibm_kernel(rx) * orc_ibm_kernel(ry) * orc_ibm_kernel(rz);
+          }
+          c->F[0] += -Fm[0];
+          c->F[1] += -Fm[1];
+          c->F[2] += -Fm[2];
+          c->ux += ORC_DT * Fm[0] / (2 * c->density);
+          c->uy += ORC_DT * Fm[1] / (2 * c->density);
+          c->uz += ORC_DT * Fm[2] / (2 * c->density);
+        }
+
+    /* re-interpolation, lbm.cpp:1363-1387 */
+    ibm_interpolate(lb, nm, mx, my, mz, ub, xmin, xmax, ymin, ymax, zmin, zmax);
+  }
+
+  /* Calc_AverageDensity, lbm.cpp:1105-1132: cells of the box (any type)
+   * behind every marker's normal */
+  double rho_avg = 0;
+  int number = 0;
+  for (int i = xmin; i <= xmax; i++)
+    for (int j = ymin; j <= ymax; j++)
+      for (int k = zmin; k <= zmax; k++) {
+        int is_dot_negative = 1;
+        for (int n = 0; n < nm; n++) {
+          double v0 = i * ORC_DX - mx[n], v1 = j * ORC_DY - my[n], v2 = k * ORC_DZ - mz[n];
+          double dot_product = v0 * nx[n] + v1 * ny[n] + v2 * nz[n];
+          if (dot_product > 0) {
+            is_dot_negative = 0;
+            break;
+          }
+        }
+        if (is_dot_negative) {
+          rho_avg += lb->fluid[i][j][k].density;
+          number++;
+        }
+      }
+  rho_avg /= number; /* NaN when no cell qualifies, as in the reference */
+
+  /* added mass, lbm.cpp:1392-1398 */
+  Drag += rho_avg * vsolid * (vcm[0] - vcm_before[0]) / ORC_DT;
+  Lift += rho_avg * vsolid * (vcm[1] - vcm_before[1]) / ORC_DT;
+  out[0] = Drag;
+  out[1] = Lift;
+  out[2] = rho_avg;
+  free(udes);
+  return err;
+}

@@ -1,0 +1,137 @@
+/*
+ * lbm_oracle.h -- CPU FP64 restatement of the reference lattice Boltzmann step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path and
+ * the timed CPU baseline; it is never linked into, called by or shipped with
+ * the product library.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load it.
+ *
+ * It follows the reference's own code (lbm.cpp / LBM.h of
+ * DestinyFlow01/Lattice-Boltzmann-Method---Graduate-FSI-) operation by
+ * operation: the same AoS 368-byte cell, the same three OpenMP sweeps
+ * (Collision, Streaming + face loops, MacroProp), the same cubic equilibrium
+ * and Guo source, the same push streaming with half-way bounce-back and the
+ * same per-cell averaged wall velocity.  Compile-time #defines of the
+ * reference become runtime fields of orc_lbm.
+ *
+ * Host array convention at this interface: cell (i,j,k) of the padded
+ * (Nx+2)(Ny+2)(Nz+2) lattice sits at linear index (i*(Ny+2)+j)*(Nz+2)+k,
+ * k fastest -- the order of LBM::fluid[i][j][k] (lbm.cpp:17-28).
+ */
+#ifndef LBM_ORACLE_H
+#define LBM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* cell-type codes, LBM.h:505-515 */
+#define ORC_TYPE_F 0
+#define ORC_TYPE_S 1
+#define ORC_TYPE_OUT 2
+#define ORC_TYPE_IN 3
+#define ORC_TYPE_P 4
+#define ORC_TYPE_L 5
+#define ORC_TYPE_T 6
+#define ORC_TYPE_Q 7
+
+/* error codes: the reference's three std::terminate() sites */
+#define ORC_OK 0
+#define ORC_ERR_FEQ_NEGATIVE 1      /* lbm.cpp:114-117 */
+#define ORC_ERR_DENSITY_RANGE 2     /* lbm.cpp:772-775 */
+#define ORC_ERR_TEMPERATURE_RANGE 3 /* lbm.cpp:794-797 */
+
+typedef struct orc_lbm orc_lbm;
+
+/* LBM::LBM(Nx,Ny,Nz,tau), lbm.cpp:17-28.  q = 19 (D3Q19) or 9 (D2Q9). */
+orc_lbm *orc_create(int Nx, int Ny, int Nz, double tau, int q);
+void orc_destroy(orc_lbm *lb);
+int64_t orc_ncells(const orc_lbm *lb);
+int orc_sizeof_cell(void);
+
+/* velocity-set tables (LBM.h:529-555) for the lattice in use */
+int orc_q(const orc_lbm *lb);
+void orc_tables(const orc_lbm *lb, int *cx, int *cy, int *cz, int *opp, double *w);
+
+/* writes of lb.fluid[i][j][k].{type,density,ux,uy,uz,F} as the setups do */
+void orc_set_cells(orc_lbm *lb, const int16_t *type, const double *rho,
+                   const double *ux, const double *uy, const double *uz,
+                   const double *Fx, const double *Fy, const double *Fz);
+void orc_get_macro(const orc_lbm *lb, double *rho, double *ux, double *uy,
+                   double *uz);
+void orc_get_force(const orc_lbm *lb, double *Fx, double *Fy, double *Fz);
+/* f and f_star as [cell][l] (l fastest, q entries per cell) */
+void orc_get_f(const orc_lbm *lb, double *f);
+void orc_get_fstar(const orc_lbm *lb, double *fstar);
+void orc_set_f(orc_lbm *lb, const double *f);
+void orc_set_fstar(orc_lbm *lb, const double *fstar);
+
+/* LBM::Calc_feq for one (rho,u) and direction, lbm.cpp:628-633 */
+double orc_feq(const orc_lbm *lb, double rho, double ux, double uy, double uz,
+               int l);
+
+/* the reference step's methods */
+void orc_init(orc_lbm *lb);      /* LBM::Init, lbm.cpp:31-57 */
+int orc_collision(orc_lbm *lb);  /* LBM::Collision + ConvLBM, lbm.cpp:60-179 */
+void orc_streaming(orc_lbm *lb); /* LBM::Streaming, lbm.cpp:182-626 */
+int orc_macroprop(orc_lbm *lb);  /* LBM::MacroProp, lbm.cpp:657-812 */
+/* main.cpp:212-221: Collision -> Streaming -> MacroProp, n times */
+int orc_run_plain(orc_lbm *lb, int nsteps);
+
+/* Determine_Wall_Velocity, lbm.cpp:941-962 */
+void orc_wall_velocity(const orc_lbm *lb, int i, int j, int k, double uw[3]);
+
+/*
+ * Pull form of the streaming pass (restated from lbm.cpp:186-295): for fluid
+ * cell x and direction m, src = x - c_m wrapped modulo (N+2) per axis
+ * (lbm.cpp:197-199) and kind = 0 (src is TYPE_F: f(x,m) = f*(src,m)),
+ * 1 (src is TYPE_S: bounce-back, f(x,m) = f*(x,opp m) + 2 w_m (u_w.c_m)/cs2)
+ * or 2 (any other type: f(x,m) is never rewritten).  For non-fluid cells
+ * src = -1, kind = 255.  src is a linear index in this interface's order.
+ */
+void orc_export_links(const orc_lbm *lb, int32_t *src, uint8_t *kind);
+
+/* number of OpenMP threads the sweeps use (0 = leave the runtime default) */
+void orc_set_threads(int nthreads);
+int orc_get_threads(void);
+
+/* ---- implicit diffuse-interface IBM, lbm.cpp:1048-1417 ----------------- */
+/* Peskin 4-point kernel(r), lbm.cpp:1095-1102 */
+double orc_ibm_kernel(double r);
+
+/*
+ * One IBLBM() call of the implicit scheme (lbm.cpp:1135-1417, branch
+ * Implicit_Diffuse_Interface_Scheme), generalised from the reference's
+ * 2D slab (zmin = zmax = 1) to a box [xmin,xmax]x[ymin,ymax]x[zmin,zmax].
+ * Marker state is caller-owned and updated in place:
+ *   mx,my,mz      marker positions (advanced by the trapezoid rule,
+ *                 lbm.cpp:1154-1163, with vcm and vcm_before)
+ *   nx,ny,nz      marker normals (Normal_Marker, lbm.cpp:1048-1069)
+ *   ub            interpolated marker velocity [3*n] (out)
+ *   Fb            marker force [3*n] (out)
+ *   ds            marker weights (main.cpp:123-131)
+ * Box limits are passed by the caller (lbm.cpp:1165-1167 computes them from
+ * the body centre).  vsolid is the added-mass volume; the average density of
+ * the interior cells is multiplied by it (lbm.cpp:1392-1398).
+ * out[0] = Drag, out[1] = Lift (raw sums, before the C_D/C_L normalisation),
+ * out[2] = the average interior density used for added mass.
+ */
+int orc_ibm_step(orc_lbm *lb, int nm, double *mx, double *my, double *mz,
+                 const double *nx, const double *ny, const double *nz,
+                 double *ub, double *Fb, const double *ds,
+                 const double vcm[3], const double vcm_before[3],
+                 int xmin, int xmax, int ymin, int ymax, int zmin, int zmax,
+                 int m_max, double vsolid, double *out);
+/* main.cpp:195-211: IBLBM -> Collision -> Streaming (no MacroProp) */
+void orc_get_fm(const orc_lbm *lb, double *Fmx, double *Fmy, double *Fmz);
+/* NearestLatticeCoordinate, lbm.cpp:1074-1092 */
+void orc_ibm_nearest(int xmin, int xmax, int ymin, int ymax, int zmin,
+                     int zmax, double mx, double my, double mz, double nx,
+                     double ny, double nz, int *i, int *j, int *k);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

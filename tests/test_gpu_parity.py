@@ -1,0 +1,211 @@
+"""GPU parity of the gfx950 step against the CPU oracle, through the C-ABI.
+
+Bars (stated per test):
+  - cell-type masks and streaming tables: bit-exact;
+  - FP64 storage: relative L2 of u <= 1e-12, max |rho - rho_ref| <= 1e-12;
+  - FP32 storage (f - w_i shifted, FP32 arithmetic): relative L2 <= 1e-5;
+  - FP16 shifted storage: the bound stated in each test.
+Relative L2 is the reference's own error form (lbm.cpp:3219-3231) over
+TYPE_F cells.
+"""
+import numpy as np
+import pytest
+
+import cases
+import lbm_amd
+from conftest import rel_l2, rel_l2_u
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu(case, storage="f64"):
+    lat = lbm_amd.Lattice(case.Nx, case.Ny, case.Nz, case.tau, storage=storage)
+    lat.upload(**case.fields())
+    lat.init()
+    return lat
+
+
+def _cmp(lat, ref, case, tol_u, tol_rho, cells=None):
+    rho, ux, uy, uz = lat.macro()
+    r_rho, r_ux, r_uy, r_uz = ref.macro()
+    m = (case.type == 0) if cells is None else cells
+    eu = rel_l2_u((ux, uy, uz), (r_ux, r_uy, r_uz), m)
+    er = float(np.abs(rho - r_rho)[m].max())
+    assert eu <= tol_u, f"rel L2(u) = {eu:.3e} > {tol_u}"
+    assert er <= tol_rho, f"max|drho| = {er:.3e} > {tol_rho}"
+    return eu, er
+
+
+LINK_CASES = {
+    "cavity": lambda: cases.cavity(8, u_lid=0.05, Re=100.0),
+    "random": lambda: cases.random_mask_case(6, 5, 4, seed=11),
+    "tgv": lambda: cases.taylor_green(12),
+    "channel": lambda: cases.channel(10, 6, 4, perturb=0.01),
+}
+
+
+@pytest.mark.parametrize("name", sorted(LINK_CASES))
+def test_masks_and_streaming_tables_bit_exact(oracle_mod, name):
+    case = LINK_CASES[name]()
+    lat = _gpu(case)
+    t, src, kind = lat.links()
+    ref = oracle_mod.run_case(case, 0)
+    r_src, r_kind = ref.links()
+    assert np.array_equal(t, case.type)
+    assert np.array_equal(src, r_src)
+    assert np.array_equal(kind, r_kind)
+
+
+FIELD_CASES = {
+    "cavity16": (lambda: cases.cavity(16, u_lid=0.05, Re=100.0), 200),
+    "random": (lambda: cases.random_mask_case(10, 9, 8, seed=3, tau=0.7, u_amp=0.02), 60),
+    "tgv16": (lambda: cases.taylor_green(16, u0=0.04, Re=100.0), 100),
+    "couette": (lambda: cases.couette(2, 20, 2, tau=0.9, u_wall=0.02), 300),
+    "poiseuille": (lambda: cases.poiseuille(2, 14, 2, tau=0.8, g=2e-6), 300),
+    "channel": (lambda: cases.channel(20, 12, 8, perturb=0.01), 80),
+}
+
+
+@pytest.mark.parametrize("name", sorted(FIELD_CASES))
+def test_fused_step_f64_matches_oracle(oracle_mod, name):
+    mk, n = FIELD_CASES[name]
+    case = mk()
+    lat = _gpu(case, "f64")
+    lat.step(n)
+    assert lat.status()[0] == 0
+    ref = oracle_mod.run_case(case, n)
+    fl = case.type == 0
+    _cmp(lat, ref, case, 1e-12, 1e-12)
+    # solid cells: MacroProp over every cell (lbm.cpp:667) -- constant moments
+    so = case.type == 1
+    if so.any():
+        rho, ux, _, _ = lat.macro()
+        r_rho, r_ux, _, _ = ref.macro()
+        assert np.abs(rho - r_rho)[so].max() < 1e-13 and np.abs(ux - r_ux)[so].max() < 1e-15
+    # post-streaming populations of the fluid cells
+    f = lat.populations()
+    assert np.abs(f - ref.f())[fl].max() < 1e-13
+
+
+@pytest.mark.parametrize("name", ["random", "cavity16", "poiseuille"])
+def test_unfused_methods_match_oracle(oracle_mod, name):
+    """LBM::Collision, LBM::Streaming, LBM::MacroProp as separate calls."""
+    mk, _ = FIELD_CASES[name]
+    case = mk()
+    lat = _gpu(case, "f64")
+    ref = oracle_mod.run_case(case, 0)
+    for s in range(15):
+        lat.collision()
+        lat.streaming()
+        lat.macroprop()
+        assert ref.collision() == 0
+        ref.streaming()
+        assert ref.macroprop() == 0
+    _cmp(lat, ref, case, 1e-12, 1e-12)
+    fl = case.type == 0
+    assert np.abs(lat.populations() - ref.f())[fl].max() < 1e-13
+
+
+def test_out_of_order_calls_are_rejected():
+    case = cases.cavity(6)
+    lat = _gpu(case)
+    with pytest.raises(lbm_amd.LBMError) as e:
+        lat.streaming()  # Streaming before Collision
+    assert e.value.code == lbm_amd.LBM_ERR_STATE
+    lat.collision()
+    with pytest.raises(lbm_amd.LBMError):
+        lat.collision()
+
+
+@pytest.mark.parametrize("storage,tol", [("f32", 1e-5), ("f16s", 3e-3)])
+def test_reduced_storage_cavity16(oracle_mod, storage, tol):
+    case = cases.cavity(16, u_lid=0.05, Re=100.0)
+    n = 300
+    lat = _gpu(case, storage)
+    lat.step(n)
+    assert lat.status()[0] == 0
+    ref = oracle_mod.run_case(case, n)
+    rho, ux, uy, uz = lat.macro()
+    r_rho, r_ux, r_uy, r_uz = ref.macro()
+    fl = case.type == 0
+    eu = rel_l2_u((ux, uy, uz), (r_ux, r_uy, r_uz), fl)
+    er = rel_l2(rho - 1.0, r_rho - 1.0, fl)
+    print(f"{storage}: rel L2 u {eu:.3e}, rel L2 (rho-1) {er:.3e}")
+    assert eu <= tol
+
+
+@pytest.fixture(scope="module")
+def cavity64_oracle(oracle_mod):
+    case = cases.cavity(64, u_lid=0.05, Re=100.0)
+    ref = oracle_mod.run_case(case, 2000)
+    return case, ref.macro()
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("storage,tol_u", [("f64", 1e-12), ("f32", 1e-5), ("f16s", 5e-3)])
+def test_lid_driven_cavity_64_2000_steps(cavity64_oracle, storage, tol_u):
+    """The CPU-runnable configuration: D3Q19 lid-driven cavity 64^3, lid
+    u = 0.05, Re = 100, 2000 steps, full fields."""
+    case, (r_rho, r_ux, r_uy, r_uz) = cavity64_oracle
+    lat = _gpu(case, storage)
+    lat.step(2000)
+    assert lat.status()[0] == 0
+    rho, ux, uy, uz = lat.macro()
+    fl = case.type == 0
+    eu = rel_l2_u((ux, uy, uz), (r_ux, r_uy, r_uz), fl)
+    er = rel_l2(rho, r_rho, fl)
+    er1 = rel_l2(rho - 1, r_rho - 1, fl)
+    print(f"cavity64 {storage}: rel L2 u {eu:.3e}, rho {er:.3e}, rho-1 {er1:.3e}")
+    assert eu <= tol_u
+    assert er <= (1e-13 if storage == "f64" else 1e-5)
+
+
+def test_density_range_error_reports_cell():
+    case = cases.cavity(8)
+    case.rho[3, 4, 5] = 200.0  # MacroProp terminates on rho > 100 (lbm.cpp:772-775)
+    lat = _gpu(case)
+    lat.step(1)
+    code, cell = lat.status()
+    assert code == lbm_amd.LBM_ERR_DENSITY_RANGE
+    assert 0 <= cell < case.type.size
+
+
+def test_negative_equilibrium_error():
+    case = cases.cavity(8)
+    case.ux[4, 4, 4] = 1.5  # feq < 0 terminates the collision (lbm.cpp:114-117)
+    lat = _gpu(case)
+    lat.collision()
+    code, cell = lat.status()
+    assert code == lbm_amd.LBM_ERR_FEQ_NEGATIVE
+    assert cell == np.ravel_multi_index((4, 4, 4), case.shape)
+
+
+def test_fp64_reductions_match_host_sums():
+    case = cases.taylor_green(24, u0=0.04, Re=200.0)
+    lat = _gpu(case, "f64")
+    lat.step(20)
+    rho, ux, uy, uz = lat.macro()
+    fl = case.type == 0
+    assert lat.reduce(lbm_amd.REDUCE_MASS) == pytest.approx(rho[fl].sum(), rel=1e-14)
+    mom = lat.reduce(lbm_amd.REDUCE_MOMENTUM)
+    want = [(rho * c)[fl].sum() for c in (ux, uy, uz)]
+    assert np.allclose(mom, want, atol=1e-15)
+    ke = lat.reduce(lbm_amd.REDUCE_KINETIC)
+    assert ke == pytest.approx((0.5 * rho * (ux ** 2 + uy ** 2 + uz ** 2))[fl].sum(), rel=1e-13)
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("storage", ["f64", "f32", "f16s"])
+def test_tgv_256_conservation(storage):
+    """Full-size periodic Taylor-Green 256^3: mass and momentum are conserved
+    by periodic BGK streaming + collision (size-independent properties)."""
+    case = cases.taylor_green(256, u0=0.04, Re=1600.0)
+    lat = _gpu(case, storage)
+    m0 = lat.reduce(lbm_amd.REDUCE_MASS)
+    lat.step(200)
+    assert lat.status()[0] == 0
+    m1 = lat.reduce(lbm_amd.REDUCE_MASS)
+    p = lat.reduce(lbm_amd.REDUCE_MOMENTUM)
+    tol = {"f64": 1e-12, "f32": 1e-6, "f16s": 1e-4}[storage]
+    assert abs(m1 - m0) / m0 < tol
+    assert np.abs(p).max() / m0 < tol
