@@ -72,6 +72,12 @@ def build(jobs: int = 4) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
+    # the reference-style driver on the drop-in class (host code only)
+    ex = os.path.join(HERE, "host", "example_cavity.cpp")
+    exe = os.path.join(HERE, "host", "lbm_example_cavity")
+    _run(["g++", "-O2", "-std=c++17", f"-I{os.path.join(ROOT, 'include')}",
+          f"-I{os.path.join(HERE, 'host')}", ex, "-o", exe, f"-L{HERE}", "-llbm_hip",
+          "-Wl,-rpath,$ORIGIN/.."])
     return LIB
 
 

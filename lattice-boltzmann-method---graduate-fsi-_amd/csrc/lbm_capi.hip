@@ -56,6 +56,9 @@ struct lbm_ctx {
   // the first MacroProp on (lbm.cpp:672-675 runs over solids too) it reads
   // their moments.  Set while the wall tables still hold the setup values.
   int walls_from_setup = 0;
+  // TYPE_S cells at k = Nz+1 under a TYPE_IN cell at k = 0 take feq2 of that
+  // cell at the first Streaming (the back-face inflow loop, lbm.cpp:599-611)
+  int backeq_solid_pending = 0;
   int64_t nfluid = 0;
   int block = 256;
   std::vector<int16_t> h_type;  // device order
@@ -310,7 +313,21 @@ int lbm_init(lbm_ctx *c) {
   HIPCHK(launch_init(L, c->cur, c->oth));
   HIPCHK(hipStreamSynchronize(c->stream));
   c->walls_from_setup = any_solid ? 1 : 0;
+  c->backeq_solid_pending = 0;
+  {
+    const uint32_t nx = c->g.nx, plane = c->g.plane, top = (c->g.nz - 1) * plane;
+    for (uint32_t d = 0; d < plane; d++)
+      if (c->h_type[d] == LBM_TYPE_IN && c->h_type[top + d] == LBM_TYPE_S) c->backeq_solid_pending = 1;
+    (void)nx;
+  }
   c->state = ST_F;
+  return LBM_OK;
+}
+
+static int after_stream(lbm_ctx *c) {
+  if (!c->backeq_solid_pending) return LBM_OK;
+  HIPCHK(launch_backeq_solid(launch_ctx(c), c->cur, c->oth));
+  c->backeq_solid_pending = 0;
   return LBM_OK;
 }
 
@@ -343,7 +360,7 @@ int lbm_stream(lbm_ctx *c) {
   HIPCHK(launch_stream(launch_ctx(c), c->cur, c->oth, STREAM_ONLY));
   std::swap(c->cur, c->oth);
   c->state = ST_F;
-  return LBM_OK;
+  return after_stream(c);
 }
 
 int lbm_macro(lbm_ctx *c) {
@@ -362,12 +379,13 @@ int lbm_step(lbm_ctx *c, int32_t nsteps, int32_t emit_macro) {
   HIPCHK(hipSetDevice(c->dev));
   LaunchCtx L = launch_ctx(c);
   if (c->state == ST_F) HIPCHK(launch_collide(L, c->cur));
-  if (c->walls_from_setup) {
+  if (c->walls_from_setup || c->backeq_solid_pending) {
     // first step with the setup wall velocities, then switch the tables
     HIPCHK(launch_stream(L, c->cur, c->oth, STREAM_MACRO));
     std::swap(c->cur, c->oth);
     c->state = ST_F;
-    int e = walls_from_moments(c);
+    int e = after_stream(c);
+    if (e == LBM_OK) e = walls_from_moments(c);
     if (e != LBM_OK) return e;
     if (--nsteps == 0) return LBM_OK;
     L = launch_ctx(c);
@@ -387,7 +405,7 @@ int lbm_time_steps(lbm_ctx *c, int32_t nsteps, double *ms) {
   if (!c || nsteps < 1 || !ms) return LBM_ERR_ARG;
   if (c->state == ST_NONE) return LBM_ERR_STATE;
   HIPCHK(hipSetDevice(c->dev));
-  if (c->walls_from_setup) {
+  if (c->walls_from_setup || c->backeq_solid_pending) {
     int e = lbm_step(c, 1, 1);
     if (e != LBM_OK) return e;
   }

@@ -34,6 +34,10 @@ constexpr uint8_t M_WRAP = 4;    // on the padded shell: neighbours wrap
 constexpr uint8_t M_MOVING = 8;  // nonzero wall velocity: read uw*[idx]
 constexpr uint8_t M_SOLID = 16;  // TYPE_S
 constexpr uint8_t M_OTHER = 32;  // any other type (IN, OUT, P, L, T, Q)
+// cell (i,j,Nz+1) whose (i,j,0) is TYPE_IN: the back-face inflow loop of the
+// reference writes its 2nd-order feq there after every Streaming
+// (lbm.cpp:599-611 reads k = 0 and writes k = Nz+1)
+constexpr uint8_t M_BACKEQ = 64;
 
 // sticky status bits
 constexpr int ST_FEQ_NEG = 1;
@@ -119,6 +123,16 @@ struct Store<__half> {
 
 // Cubic equilibrium of LBM::Calc_feq (lbm.cpp:628-633) with cs2 = 1/3:
 //   feq = rho w (1 + A),  A = 3cu + 4.5cu^2 - 1.5uu + 4.5cu^3 - 4.5 cu uu.
+// Second-order inflow equilibrium of the face loops (lbm.cpp:337 etc.):
+//   feq2 = rho w (1 + 3cu + 4.5cu^2 - 1.5uu); returned shifted (minus w)
+//   when the storage is shifted.
+template <class C, bool SHIFTED>
+__device__ __forceinline__ C feq2_value(C rho, C cu, C uu, C w) {
+  const C B = C(3) * cu + C(4.5) * cu * cu - C(1.5) * uu;
+  if constexpr (SHIFTED) return w * ((rho - C(1)) + rho * B);
+  else return rho * w * (C(1) + B);
+}
+
 template <class C>
 __device__ __forceinline__ C eq_A(C cu, C uu) {
   return cu * (C(3) + C(4.5) * cu * cu - C(4.5) * uu) + C(4.5) * cu * cu - C(1.5) * uu;

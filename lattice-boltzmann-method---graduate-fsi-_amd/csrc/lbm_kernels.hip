@@ -148,6 +148,18 @@ __global__ __launch_bounds__(256) void k_stream(const S *__restrict__ fin, S *__
         f[m] = Store<S>::ld(fin + (uint32_t)m * ps + src);
       }
     });
+    if (mk & M_BACKEQ) {
+      // after the push, the back-face inflow loop overwrites this cell with
+      // feq2 of the TYPE_IN cell (i,j,0) (lbm.cpp:599-611)
+      const uint32_t s = idx - (a.g.nz - 1) * plane;
+      const C r = a.rho[s], vx = a.ux[s], vy = a.uy[s], vz = a.uz[s];
+      const C uu = vx * vx + vy * vy + vz * vz;
+      for_dirs([&](auto M) {
+        constexpr int m = decltype(M)::value;
+        const C cu = C(CX[m]) * vx + C(CY[m]) * vy + C(CZ[m]) * vz;
+        f[m] = feq2_value<C, Store<S>::shifted>(r, cu, uu, C(W[m]));
+      });
+    }
   }
 
   if constexpr (!COLLIDE && !EMIT) {
@@ -313,6 +325,7 @@ __global__ __launch_bounds__(256) void k_tables(const int16_t *__restrict__ type
   else if (t == 1) mk |= M_SOLID;
   else mk |= M_OTHER;
   if (x == 0 || x == g.nx - 1 || y == 0 || y == g.ny - 1 || z == 0 || z == g.nz - 1) mk |= M_WRAP;
+  if (z == g.nz - 1 && type[idx - (g.nz - 1) * g.plane] == 3) mk |= M_BACKEQ;
   uint32_t lk = 0;
   C wx = C(0), wy = C(0), wz = C(0);
   if (t == 0) {
@@ -348,6 +361,30 @@ __global__ __launch_bounds__(256) void k_tables(const int16_t *__restrict__ type
     uwy[idx] = wy;
     uwz[idx] = wz;
   }
+}
+
+// TYPE_S cells hit by the back-face inflow write (M_BACKEQ): from the first
+// Streaming on they hold feq2 of their TYPE_IN partner (lbm.cpp:599-611).
+template <class S>
+__global__ __launch_bounds__(256) void k_backeq_solid(S *__restrict__ A, S *__restrict__ B,
+                                                      const uint8_t *__restrict__ mask,
+                                                      KArgs<typename Store<S>::C> a) {
+  using C = typename Store<S>::C;
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.g.ncell) return;
+  const uint8_t mk = mask[idx];
+  if ((mk & (M_SOLID | M_BACKEQ)) != (M_SOLID | M_BACKEQ)) return;
+  const uint32_t s = idx - (a.g.nz - 1) * a.g.plane;
+  const C r = a.rho[s], vx = a.ux[s], vy = a.uy[s], vz = a.uz[s];
+  const C uu = vx * vx + vy * vy + vz * vz;
+  const uint32_t ps = a.g.ps;
+  for_dirs([&](auto M) {
+    constexpr int m = decltype(M)::value;
+    const C cu = C(CX[m]) * vx + C(CY[m]) * vy + C(CZ[m]) * vz;
+    const S v = Store<S>::enc(feq2_value<C, Store<S>::shifted>(r, cu, uu, C(W[m])));
+    A[(uint32_t)m * ps + idx] = v;
+    B[(uint32_t)m * ps + idx] = v;
+  });
 }
 
 // The streaming tables exactly as k_stream resolves them.
@@ -617,6 +654,24 @@ hipError_t launch_tables(const LaunchCtx &L, uint8_t *mask, uint32_t *links, voi
     hipLaunchKernelGGL(k_tables<float>, grid, dim3(L.block), 0, L.stream, type, mask, links,
                        (float *)uwx, (float *)uwy, (float *)uwz, L.g, (const float *)L.ux,
                        (const float *)L.uy, (const float *)L.uz);
+  return hipGetLastError();
+}
+
+hipError_t launch_backeq_solid(const LaunchCtx &L, void *A, void *B) {
+  const dim3 grid = grid_for(L.g.ncell, L.block);
+  switch (L.storage) {
+    case 0:
+      hipLaunchKernelGGL(k_backeq_solid<double>, grid, dim3(L.block), 0, L.stream, (double *)A, (double *)B, L.mask, make_args<double>(L));
+      break;
+    case 1:
+      hipLaunchKernelGGL(k_backeq_solid<float>, grid, dim3(L.block), 0, L.stream, (float *)A, (float *)B, L.mask, make_args<float>(L));
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_backeq_solid<__half>, grid, dim3(L.block), 0, L.stream, (__half *)A, (__half *)B, L.mask, make_args<__half>(L));
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

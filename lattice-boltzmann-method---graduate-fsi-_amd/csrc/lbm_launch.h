@@ -46,6 +46,7 @@ hipError_t launch_macro(const LaunchCtx &L, const void *buf);
 hipError_t launch_init(const LaunchCtx &L, void *A, void *B);
 hipError_t launch_tables(const LaunchCtx &L, uint8_t *mask, uint32_t *links, void *uwx,
                          void *uwy, void *uwz, const int16_t *type);
+hipError_t launch_backeq_solid(const LaunchCtx &L, void *A, void *B);
 hipError_t launch_links(const LaunchCtx &L, int32_t *src, uint8_t *kind);
 hipError_t launch_f_out(const LaunchCtx &L, const void *buf, double *out);
 int reduce_blocks(uint32_t n);

@@ -1,0 +1,85 @@
+// LBM.h -- drop-in C++ host class with the reference's interface
+// (LBM.h:558-664 of the reference), backed by the gfx950 library through the
+// C-ABI of lbm_hip.h.
+//
+// What stays the same for a caller of the reference:
+//   - class LATTICE with public type/density/ux/uy/uz/f/f_star/F members;
+//   - class LBM with LBM(Nx,Ny,Nz,tau), Init(), Collision(), Streaming(),
+//     MacroProp(), getNx/Ny/Nz/TAU() and the public LATTICE ***fluid indexed
+//     fluid[i][j][k] over the padded (Nx+2)(Ny+2)(Nz+2) lattice;
+//   - setups write fluid[i][j][k].{type,density,ux,uy,uz,F} before Init();
+//   - a negative equilibrium or an out-of-range density prints the
+//     reference's message and calls std::terminate() (lbm.cpp:114-117,
+//     772-775).
+// What is different: `fluid` is a host mirror.  Init() uploads it and fills
+// f/f_star; MacroProp() and Step() refresh density/ux/uy/uz; SyncMirror()
+// also brings back the populations.  The lattice itself lives in HBM.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "lbm_hip.h"
+
+// cell-type codes, LBM.h:505-515
+#ifndef TYPE_F
+#define TYPE_F 0
+#define TYPE_S 1
+#define TYPE_OUT 2
+#define TYPE_IN 3
+#define TYPE_P 4
+#define TYPE_L 5
+#define TYPE_T 6
+#define TYPE_Q 7
+#endif
+
+namespace lbm_host {
+constexpr int npop = 19;  // D3Q19, LBM.h:544
+}
+
+// LBM.h:558-597 (the D3Q19 build without IBM/ENERGY)
+class LATTICE {
+ public:
+  short type = TYPE_F;
+  double density = 1.0;
+  double ux = 0.0;
+  double uy = 0.0;
+  double uz = 0.0;
+  double f[lbm_host::npop], f_star[lbm_host::npop];
+  double F[3] = {0, 0, 0};
+};
+
+class LBM {
+ private:
+  int Nx, Ny, Nz = 1;
+  double tau;
+  struct Impl;
+  std::shared_ptr<Impl> impl_;  // copies share the lattice, as the reference's
+                                // by-value LBM shares its fluid pointer
+
+ public:
+  LATTICE ***fluid;  // host mirror, fluid[i][j][k]
+
+  LBM(int Nx, int Ny, int Nz, double tau);
+  // storage: LBM_STORE_F64 (default, the reference's precision),
+  // LBM_STORE_F32 or LBM_STORE_F16S; device: HIP ordinal
+  LBM(int Nx, int Ny, int Nz, double tau, int storage, int device = 0);
+
+  void Init();       // lbm.cpp:31-57
+  void Collision();  // lbm.cpp:60-179
+  void Streaming();  // lbm.cpp:182-626
+  void MacroProp();  // lbm.cpp:657-863
+
+  int getNx() const;
+  int getNy() const;
+  int getNz() const;
+  double getTAU() const;
+
+  // -- additions --------------------------------------------------------
+  // n reference steps (Collision, Streaming, MacroProp) as fused launches;
+  // refreshes density/ux/uy/uz of the mirror.
+  void Step(int n);
+  // Copies density/ux/uy/uz (and f, F when asked) from the device.
+  void SyncMirror(bool populations = false);
+  lbm_ctx *ctx() const;
+};

@@ -235,6 +235,33 @@ def test_streaming_walls_frozen_links_brute_force(oracle_mod):
     assert (kind[~fl] == 255).all() and (src[~fl] == -1).all()
 
 
+def test_back_face_inflow_writes_the_top_layer(oracle_mod):
+    """lbm.cpp:599-611: for a TYPE_IN cell at k = 0 the back-face loop
+    computes feq2 from that cell's rho,u and stores it in cell (i,j,Nz+1),
+    which may be fluid or solid.  Re-derived here from the 2nd-order
+    equilibrium rho w (1 + 3cu + 4.5cu^2 - 1.5uu)."""
+    shape = (5, 4, 5)
+    t = np.zeros(shape, np.int16)
+    t[2, 1, 0] = 3       # inlet cell on the back face
+    t[1, 2, 0] = 3
+    t[1, 2, -1] = 1      # its top partner is solid
+    rng = np.random.default_rng(2)
+    ux, uy, uz = (0.02 * rng.standard_normal(shape) for _ in range(3))
+    rho = 1 + 0.01 * rng.standard_normal(shape)
+    o = oracle_mod.Oracle(3, 2, 3, 0.7)
+    o.set_cells(type=t, rho=rho, ux=ux, uy=uy, uz=uz)
+    o.init()
+    cx, cy, cz, _, w = o.tables()
+    o.set_fstar(rng.random(shape + (19,)))
+    o.streaming()
+    f = o.f()
+    for (i, j) in ((2, 1), (1, 2)):
+        u = np.array([ux[i, j, 0], uy[i, j, 0], uz[i, j, 0]])
+        cu = cx * u[0] + cy * u[1] + cz * u[2]
+        want = rho[i, j, 0] * w * (1 + 3 * cu + 4.5 * cu ** 2 - 1.5 * (u @ u))
+        assert np.allclose(f[i, j, -1], want, rtol=1e-14, atol=0)
+
+
 def test_stationary_walls_conserve_mass(oracle_mod):
     case = cases.cavity(8, u_lid=0.0, Re=100.0)
     rng = np.random.default_rng(4)
