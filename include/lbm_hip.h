@@ -128,6 +128,11 @@ int32_t lbm_storage(const lbm_ctx *ctx);
 /* Timed run of nsteps fused launches bracketed by HIP events on the
  * context's stream; *ms receives the elapsed milliseconds. */
 int lbm_time_steps(lbm_ctx *ctx, int32_t nsteps, double *ms);
+/* Launch shape of the stream kernels: threads per block (64..1024, a
+ * multiple of 64) and cells per thread (1, or 4 for FP32/FP16 storage).  No
+ * reference counterpart: the autotuner's knob. */
+int lbm_set_launch(lbm_ctx *ctx, int32_t block, int32_t cells_per_thread);
+int lbm_get_launch(const lbm_ctx *ctx, int32_t *block, int32_t *cells_per_thread);
 /* Bytes one fused launch moves per fluid cell (algorithmic). */
 double lbm_bytes_per_cell(const lbm_ctx *ctx);
 const char *lbm_error_string(int code);

@@ -100,17 +100,23 @@ static LaunchCtx launch_ctx(const lbm_ctx *c) {
   return L;
 }
 
-// reference order r = (i*ny + j)*nz + k  <->  device order d = (k*ny + j)*nx + i
+// reference order r = (i*ny + j)*nz + k  <->  device order d = (k*ny + j)*px + i
 template <class F>
 static void for_cells(const lbm_ctx *c, F &&f) {
-  const uint32_t nx = c->g.nx, ny = c->g.ny, nz = c->g.nz;
+  const uint32_t nx = c->g.nx, ny = c->g.ny, nz = c->g.nz, px = c->g.px;
   for (uint32_t i = 0; i < nx; i++)
     for (uint32_t j = 0; j < ny; j++)
       for (uint32_t k = 0; k < nz; k++) {
         const size_t r = ((size_t)i * ny + j) * nz + k;
-        const size_t d = ((size_t)k * ny + j) * nx + i;
+        const size_t d = ((size_t)k * ny + j) * px + i;
         f(r, d);
       }
+}
+
+// device index -> reference index (d must be a real cell, x < nx)
+static int64_t ref_index(const lbm_ctx *c, uint64_t d) {
+  const uint64_t i = d % c->g.px, t = d / c->g.px, j = t % c->g.ny, k = t / c->g.ny;
+  return (int64_t)((i * c->g.ny + j) * c->g.nz + k);
 }
 
 static int upload_real(lbm_ctx *c, void *dst, const double *src, double dflt) {
@@ -175,8 +181,12 @@ int lbm_create(const lbm_desc *desc, lbm_ctx **out) {
   if (desc->storage < LBM_STORE_F64 || desc->storage > LBM_STORE_F16S) return LBM_ERR_ARG;
   if (!(desc->tau > 0.5) || !std::isfinite(desc->tau)) return LBM_ERR_ARG;
   const uint64_t nx = desc->nx + 2ull, ny = desc->ny + 2ull, nz = desc->nz + 2ull;
-  const uint64_t ncell = nx * ny * nz;
-  const uint64_t ps = (ncell + 255) / 256 * 256;
+  const uint64_t px = (nx + 3) / 4 * 4;  // row pitch: 4-cell chunks start aligned
+  const uint64_t ncell = px * ny * nz;
+  // plane stride: at least ncell + plane + px + 1, so that the speculative
+  // bulk loads of the stream kernels (m*ps + idx - off_m for any idx <
+  // ncell, |off_m| <= plane + px + 1) stay inside the 19-plane allocation
+  const uint64_t ps = (ncell + px * ny + px + 1 + 255) / 256 * 256;
   // 32-bit element offsets: 19 planes must stay below 2^32 elements
   if (ps * 19ull >= (1ull << 32)) return LBM_ERR_UNSUPPORTED;
 
@@ -185,7 +195,8 @@ int lbm_create(const lbm_desc *desc, lbm_ctx **out) {
   c->g.nx = (uint32_t)nx;
   c->g.ny = (uint32_t)ny;
   c->g.nz = (uint32_t)nz;
-  c->g.plane = (uint32_t)(nx * ny);
+  c->g.px = (uint32_t)px;
+  c->g.plane = (uint32_t)(px * ny);
   c->g.ncell = (uint32_t)ncell;
   c->g.ps = (uint32_t)ps;
   c->storage = desc->storage;
@@ -240,12 +251,12 @@ int lbm_upload_cells(lbm_ctx *c, const int16_t *type, const double *rho, const d
   HIPCHK(hipSetDevice(c->dev));
   HIPCHK(hipStreamSynchronize(c->stream));
   const size_t n = c->g.ncell;
-  c->h_type.assign(n, (int16_t)LBM_TYPE_F);
-  if (type) {
-    for (size_t r = 0; r < n; r++)
+  const size_t nref = (size_t)c->g.nx * c->g.ny * c->g.nz;
+  c->h_type.assign(n, (int16_t)-1);  // row padding
+  if (type)
+    for (size_t r = 0; r < nref; r++)
       if (type[r] < 0 || type[r] > 7) return LBM_ERR_ARG;
-    for_cells(c, [&](size_t r, size_t d) { c->h_type[d] = type[r]; });
-  }
+  for_cells(c, [&](size_t r, size_t d) { c->h_type[d] = type ? type[r] : (int16_t)LBM_TYPE_F; });
   c->nfluid = 0;
   for (size_t d = 0; d < n; d++) c->nfluid += c->h_type[d] == LBM_TYPE_F;
   HIPCHK(hipMemcpy(c->type, c->h_type.data(), n * 2, hipMemcpyHostToDevice));
@@ -262,7 +273,7 @@ int lbm_upload_cells(lbm_ctx *c, const int16_t *type, const double *rho, const d
   for (int q = 0; q < 3; q++) {
     if (!Fs[q]) continue;
     F0[q] = Fs[q][0];
-    for (size_t r = 0; r < n; r++) {
+    for (size_t r = 0; r < nref; r++) {
       if (Fs[q][r] != 0.0) any = true;
       if (Fs[q][r] != F0[q]) uniform = false;
     }
@@ -315,10 +326,9 @@ int lbm_init(lbm_ctx *c) {
   c->walls_from_setup = any_solid ? 1 : 0;
   c->backeq_solid_pending = 0;
   {
-    const uint32_t nx = c->g.nx, plane = c->g.plane, top = (c->g.nz - 1) * plane;
+    const uint32_t plane = c->g.plane, top = (c->g.nz - 1) * plane;
     for (uint32_t d = 0; d < plane; d++)
       if (c->h_type[d] == LBM_TYPE_IN && c->h_type[top + d] == LBM_TYPE_S) c->backeq_solid_pending = 1;
-    (void)nx;
   }
   c->state = ST_F;
   return LBM_OK;
@@ -451,9 +461,7 @@ int lbm_status(lbm_ctx *c, int64_t *first_bad_cell) {
     if (st == 0) {
       *first_bad_cell = -1;
     } else {
-      // device index -> reference linear index
-      const uint64_t i = fb % c->g.nx, t = fb / c->g.nx, j = t % c->g.ny, k = t / c->g.ny;
-      *first_bad_cell = (int64_t)((i * c->g.ny + j) * c->g.nz + k);
+      *first_bad_cell = ref_index(c, fb);
     }
   }
   if (st & 2) return LBM_ERR_DENSITY_RANGE;
@@ -479,14 +487,14 @@ int lbm_download_force(lbm_ctx *c, double *Fx, double *Fy, double *Fz) {
   HIPCHK(hipSetDevice(c->dev));
   HIPCHK(hipStreamSynchronize(c->stream));
   double *outs[3] = {Fx, Fy, Fz};
-  const size_t n = c->g.ncell;
+  const size_t nref = (size_t)c->g.nx * c->g.ny * c->g.nz;
   for (int q = 0; q < 3; q++) {
     if (!outs[q]) continue;
     if (c->force_mode == 1) {
       int e = download_real(c, outs[q], c->F[q]);
       if (e != LBM_OK) return e;
     } else {
-      for (size_t r = 0; r < n; r++) outs[q][r] = c->force_mode == 2 ? c->Fu[q] : 0.0;
+      for (size_t r = 0; r < nref; r++) outs[q][r] = c->force_mode == 2 ? c->Fu[q] : 0.0;
     }
   }
   return LBM_OK;
@@ -535,14 +543,10 @@ int lbm_export_links(lbm_ctx *c, int16_t *type, int32_t *src, uint8_t *kind) {
   (void)hipFree(dsrc);
   (void)hipFree(dkind);
   if (e != hipSuccess) return LBM_ERR_HIP;
-  const uint32_t nx = c->g.nx, ny = c->g.ny, nz = c->g.nz;
   for_cells(c, [&](size_t r, size_t d) {
     for (int m = 0; m < 19; m++) {
       int32_t s = hs[d * 19 + m];
-      if (s >= 0) {  // device index -> reference index
-        const uint32_t i = (uint32_t)s % nx, t = (uint32_t)s / nx, j = t % ny, k = t / ny;
-        s = (int32_t)(((size_t)i * ny + j) * nz + k);
-      }
+      if (s >= 0) s = (int32_t)ref_index(c, (uint64_t)s);
       if (src) src[r * 19 + m] = s;
       if (kind) kind[r * 19 + m] = hk[d * 19 + m];
     }
@@ -561,7 +565,23 @@ int lbm_reduce_fp64(lbm_ctx *c, int32_t what, double *out) {
   return LBM_OK;
 }
 
-int64_t lbm_ncells(const lbm_ctx *c) { return c ? (int64_t)c->g.ncell : -1; }
+int lbm_set_launch(lbm_ctx *c, int32_t block, int32_t cells_per_thread) {
+  if (!c || block < 64 || block > 256 || block % 64 != 0) return LBM_ERR_ARG;
+  if (cells_per_thread != 1) return LBM_ERR_UNSUPPORTED;
+  c->block = block;
+  return LBM_OK;
+}
+
+int lbm_get_launch(const lbm_ctx *c, int32_t *block, int32_t *cells_per_thread) {
+  if (!c) return LBM_ERR_ARG;
+  if (block) *block = c->block;
+  if (cells_per_thread) *cells_per_thread = 1;
+  return LBM_OK;
+}
+
+int64_t lbm_ncells(const lbm_ctx *c) {
+  return c ? (int64_t)c->g.nx * c->g.ny * c->g.nz : -1;
+}
 int64_t lbm_nfluid(const lbm_ctx *c) { return c ? c->nfluid : -1; }
 int32_t lbm_storage(const lbm_ctx *c) { return c ? c->storage : -1; }
 

@@ -2,9 +2,9 @@
 //
 // Lattice: D3Q19 in the reference's direction order (LBM.h:546-552).
 // Device layout: structure of arrays, one plane of `ps` elements per
-// direction, cell index idx = (z*ny + y)*nx + x with x fastest over the
-// padded (Nx+2)(Ny+2)(Nz+2) lattice.  Two such population buffers (A/B)
-// alternate every step.
+// direction, cell index idx = (z*ny + y)*px + x with x fastest over the
+// padded (Nx+2)(Ny+2)(Nz+2) lattice and a row pitch px (multiple of 4).
+// Two such population buffers (A/B) alternate every step.
 #pragma once
 
 #include <hip/hip_fp16.h>
@@ -27,7 +27,8 @@ constexpr double W[Q] = {1.0 / 3,  1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 /
                          1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36,
                          1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36};
 
-// per-cell kernel mask bits (derived from the cell types at init)
+// per-cell kernel mask bits (derived from the cell types at init); 0 marks
+// a row-padding cell
 constexpr uint8_t M_FLUID = 1;   // TYPE_F: collides and pulls
 constexpr uint8_t M_WALL = 2;    // some x - c_m is TYPE_S: read links[idx]
 constexpr uint8_t M_WRAP = 4;    // on the padded shell: neighbours wrap
@@ -62,7 +63,7 @@ __device__ __forceinline__ uint32_t wrap_src(const Geo &g, uint32_t x, uint32_t 
   uint32_t xs = cx > 0 ? (x == 0 ? g.nx - 1 : x - 1) : (cx < 0 ? (x == g.nx - 1 ? 0 : x + 1) : x);
   uint32_t ys = cy > 0 ? (y == 0 ? g.ny - 1 : y - 1) : (cy < 0 ? (y == g.ny - 1 ? 0 : y + 1) : y);
   uint32_t zs = cz > 0 ? (z == 0 ? g.nz - 1 : z - 1) : (cz < 0 ? (z == g.nz - 1 ? 0 : z + 1) : z);
-  return (zs * g.ny + ys) * g.nx + xs;
+  return (zs * g.ny + ys) * g.px + xs;
 }
 // x + c, wrapped (the pushing target of direction c)
 __device__ __forceinline__ uint32_t wrap_dst(const Geo &g, uint32_t x, uint32_t y, uint32_t z,
@@ -71,71 +72,82 @@ __device__ __forceinline__ uint32_t wrap_dst(const Geo &g, uint32_t x, uint32_t 
 }
 __device__ __forceinline__ void coords(const Geo &g, uint32_t idx, uint32_t &x, uint32_t &y,
                                        uint32_t &z) {
-  x = idx % g.nx;
-  uint32_t t = idx / g.nx;
+  x = idx % g.px;
+  uint32_t t = idx / g.px;
   y = t % g.ny;
   z = t / g.ny;
 }
+// the pull offset of direction m for a cell off the shell: src = idx - off
+template <int m>
+__device__ __forceinline__ uint32_t pull_off(const Geo &g) {
+  return (uint32_t)(CX[m] + CY[m] * (int)g.px + CZ[m] * (int)g.plane);
+}
 
 // ---- population storage -------------------------------------------------
-// Stored value s and compute value v:
-//   double: v = s = f                      (unshifted, FP64 arithmetic)
-//   float : v = s = f - w                  (shifted, FP32 arithmetic)
-//   half  : v = f - w = float(s) * 2^-12   (shifted and scaled, FP32 arith.)
+// Stored value s, compute ("raw") value r and population f:
+//   double: r = s = f                          K = 1  (unshifted, FP64)
+//   float : r = s = f - w                      K = 1  (shifted, FP32)
+//   half  : r = float(s) = (f - w) * 2^12      K = 4096 (shifted, scaled)
+// Kernels work on raw values: the scale K is folded into the constants that
+// produce equilibria and sources, and 1/K into the moments.
 template <class S>
 struct Store;
+
 
 template <>
 struct Store<double> {
   using C = double;
   static constexpr bool shifted = false;
+  static constexpr double K = 1.0;
   static __device__ __forceinline__ double ld(const double *p) { return *p; }
-  static __device__ __forceinline__ void st(double *p, double v) { __builtin_nontemporal_store(v, p); }
-  static __device__ __forceinline__ double enc(double v) { return v; }
+  static __device__ __forceinline__ void st(double *p, double r) { __builtin_nontemporal_store(r, p); }
+  static __device__ __forceinline__ double enc(double r) { return r; }
 };
 
 template <>
 struct Store<float> {
   using C = float;
   static constexpr bool shifted = true;
+  static constexpr float K = 1.0f;
   static __device__ __forceinline__ float ld(const float *p) { return *p; }
-  static __device__ __forceinline__ void st(float *p, float v) { __builtin_nontemporal_store(v, p); }
-  static __device__ __forceinline__ float enc(float v) { return v; }
+  static __device__ __forceinline__ void st(float *p, float r) { __builtin_nontemporal_store(r, p); }
+  static __device__ __forceinline__ float enc(float r) { return r; }
 };
 
 constexpr float F16S_SCALE = 4096.0f;  // 2^LBM_F16S_SCALE_LOG2
-constexpr float F16S_INV = 1.0f / 4096.0f;
 
 template <>
 struct Store<__half> {
   using C = float;
   static constexpr bool shifted = true;
+  static constexpr float K = F16S_SCALE;
   static __device__ __forceinline__ float ld(const __half *p) {
-    return __half2float(*p) * F16S_INV;
+    return (float)*reinterpret_cast<const _Float16 *>(p);
   }
-  static __device__ __forceinline__ void st(__half *p, float v) {
-    __half h = __float2half_rn(v * F16S_SCALE);
-    __builtin_nontemporal_store(*reinterpret_cast<unsigned short *>(&h),
-                                reinterpret_cast<unsigned short *>(p));
+  static __device__ __forceinline__ void st(__half *p, float r) {
+    __builtin_nontemporal_store((_Float16)r, reinterpret_cast<_Float16 *>(p));
   }
-  static __device__ __forceinline__ __half enc(float v) { return __float2half_rn(v * F16S_SCALE); }
+  static __device__ __forceinline__ __half enc(float r) {
+    _Float16 h = (_Float16)r;
+    return *reinterpret_cast<__half *>(&h);
+  }
 };
 
 // Cubic equilibrium of LBM::Calc_feq (lbm.cpp:628-633) with cs2 = 1/3:
 //   feq = rho w (1 + A),  A = 3cu + 4.5cu^2 - 1.5uu + 4.5cu^3 - 4.5 cu uu.
-// Second-order inflow equilibrium of the face loops (lbm.cpp:337 etc.):
-//   feq2 = rho w (1 + 3cu + 4.5cu^2 - 1.5uu); returned shifted (minus w)
-//   when the storage is shifted.
-template <class C, bool SHIFTED>
-__device__ __forceinline__ C feq2_value(C rho, C cu, C uu, C w) {
-  const C B = C(3) * cu + C(4.5) * cu * cu - C(1.5) * uu;
-  if constexpr (SHIFTED) return w * ((rho - C(1)) + rho * B);
-  else return rho * w * (C(1) + B);
-}
-
 template <class C>
 __device__ __forceinline__ C eq_A(C cu, C uu) {
   return cu * (C(3) + C(4.5) * cu * cu - C(4.5) * uu) + C(4.5) * cu * cu - C(1.5) * uu;
+}
+
+// Second-order inflow equilibrium of the face loops (lbm.cpp:337 etc.):
+//   feq2 = rho w (1 + 3cu + 4.5cu^2 - 1.5uu); returned as a raw value of the
+//   storage (shifted by w and scaled by K where the storage is).
+template <class C, bool SHIFTED>
+__device__ __forceinline__ C feq2_raw(C rho, C cu, C uu, C w, C K) {
+  const C B = C(3) * cu + C(4.5) * cu * cu - C(1.5) * uu;
+  if constexpr (SHIFTED) return K * w * ((rho - C(1)) + rho * B);
+  else return rho * w * (C(1) + B);
 }
 
 }  // namespace lbm

@@ -1,22 +1,27 @@
 // lbm_kernels.hip -- gfx950 kernels of the D3Q19 BGK step.
 //
-// One thread per cell of the padded lattice, 1D launch over the x-fastest
-// linear index (consecutive lanes = consecutive x, so every population load
-// and store of a wave is one contiguous 64-element run).  The step is
-// bandwidth-bound: per fluid cell it reads 19 populations and a 1-byte mask
-// and writes 19 populations; nothing else is touched on bulk cells.
+// The step is HBM-bound: per fluid cell it reads 19 populations and a 1-byte
+// mask and writes 19 populations.  One thread per cell of the padded lattice,
+// 1D launch over the x-fastest index, so the lanes of a wave touch one
+// contiguous run per population plane.  The 19 population loads of a cell
+// are issued together with its mask load (see bulk_loads), every load and
+// store addresses a host-computed plane base plus one shared 32-bit lane
+// offset, and the collision works on +-c direction pairs.
 //
 //  k_init           LBM::Init (lbm.cpp:31-57), f = feq in every cell
 //  k_tables         mask bits, bounce-back link words and wall velocities
 //                   (lbm.cpp:186-295, 941-962) from the cell types
 //  k_collide        LBM::Collision/ConvLBM (lbm.cpp:60-118), in place
-//  k_stream<...>    LBM::Streaming (lbm.cpp:182-295) in pull form, optionally
+//  k_stream         LBM::Streaming (lbm.cpp:182-295) in pull form, optionally
 //                   fused with MacroProp (lbm.cpp:657-776) and the next
 //                   step's collision -- the hot kernel
 //  k_macro          LBM::MacroProp (lbm.cpp:657-776)
+//  k_backeq_solid   the back-face inflow write into solid cells
 //  k_links          the streaming tables as the kernels use them
 //  k_f_out          populations to FP64 for download
 //  k_reduce_*       deterministic FP64 sums (mass, momentum, kinetic energy)
+#include <type_traits>
+
 #include "lbm_device.h"
 #include "lbm_launch.h"
 
@@ -33,6 +38,16 @@ struct KArgs {
   C *rho, *ux, *uy, *uz;         // macroscopic arrays
   int *status;
   unsigned long long *first_bad;
+};
+
+// Per-direction plane pointers, computed on the host for each launch so
+// that no wave recomputes them: src[m] = fin + m*ps - off_m (bulk pull
+// source), in[m] = fin + m*ps, out[m] = fout + m*ps.
+template <class S>
+struct Planes {
+  const S *src[Q];
+  const S *in[Q];
+  S *out[Q];
 };
 
 template <class C, int FORCE>
@@ -58,78 +73,205 @@ __device__ __forceinline__ void flag(int *status, unsigned long long *first_bad,
   atomicMin(first_bad, (unsigned long long)idx);
 }
 
-// BGK collision with the Guo source (ConvLBM, lbm.cpp:75-118) of one cell.
-// f holds compute values (shifted for FP32/FP16 storage); rho is the full
-// density, drho = rho - 1.  Returns true when some feq < 0 (lbm.cpp:114).
-template <class S, int FORCE>
-__device__ __forceinline__ bool collide_store(typename Store<S>::C (&f)[Q],
-                                              typename Store<S>::C rho,
-                                              typename Store<S>::C drho,
-                                              typename Store<S>::C ux, typename Store<S>::C uy,
-                                              typename Store<S>::C uz, typename Store<S>::C Fx,
-                                              typename Store<S>::C Fy, typename Store<S>::C Fz,
-                                              typename Store<S>::C omega, S *__restrict__ out,
-                                              uint32_t ps, uint32_t idx) {
+__device__ __forceinline__ float vmin(float a, float b) { return __builtin_fminf(a, b); }
+__device__ __forceinline__ float vmax(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ double vmin(double a, double b) { return __builtin_fmin(a, b); }
+__device__ __forceinline__ double vmax(double a, double b) { return __builtin_fmax(a, b); }
+
+// u = (j + F/2)/rho: FP64 divides each component (the reference's
+// expression); FP32 multiplies by one correctly rounded reciprocal.
+template <class C>
+__device__ __forceinline__ void velocity(C jx, C jy, C jz, C rho, C &ux, C &uy, C &uz) {
+  if constexpr (sizeof(C) == 8) {
+    ux = jx / rho;
+    uy = jy / rho;
+    uz = jz / rho;
+  } else {
+    const C ir = C(1) / rho;
+    ux = jx * ir;
+    uy = jy * ir;
+    uz = jz * ir;
+  }
+}
+
+// Moments of one cell from raw values (MacroProp, lbm.cpp:668-671), summed
+// pairwise over the +-c direction pairs.
+template <class S>
+__device__ __forceinline__ void moments(const typename Store<S>::C (&f)[Q],
+                                        typename Store<S>::C &rho, typename Store<S>::C &drho,
+                                        typename Store<S>::C &jx, typename Store<S>::C &jy,
+                                        typename Store<S>::C &jz) {
   using C = typename Store<S>::C;
+  // p_k = f_m + f_opp, d_k = f_m - f_opp for the pairs (1,2), (3,4), ... (17,18)
+  const C p1 = f[1] + f[2], p2 = f[3] + f[4], p3 = f[5] + f[6], p4 = f[7] + f[8];
+  const C p5 = f[9] + f[10], p6 = f[11] + f[12], p7 = f[13] + f[14], p8 = f[15] + f[16];
+  const C p9 = f[17] + f[18];
+  const C d1 = f[1] - f[2], d2 = f[3] - f[4], d3 = f[5] - f[6], d4 = f[7] - f[8];
+  const C d5 = f[9] - f[10], d6 = f[11] - f[12], d7 = f[13] - f[14], d8 = f[15] - f[16];
+  const C d9 = f[17] - f[18];
+  const C s0 = ((f[0] + (p1 + p2)) + ((p3 + p4) + (p5 + p6))) + ((p7 + p8) + p9);
+  // c(7) = (1,1,0), c(9) = (1,0,1), c(11) = (0,1,1), c(13) = (1,-1,0),
+  // c(15) = (1,0,-1), c(17) = (0,1,-1)
+  jx = (d1 + d4) + ((d5 + d7) + d8);
+  jy = (d2 + d4) + ((d6 - d7) + d9);
+  jz = (d3 + d5) + ((d6 - d8) - d9);
+  if constexpr (Store<S>::shifted) {
+    constexpr C iK = C(1) / C(Store<S>::K);
+    drho = s0 * iK;
+    rho = C(1) + drho;
+    jx *= iK;
+    jy *= iK;
+    jz *= iK;
+  } else {
+    rho = s0;
+    drho = s0 - C(1);
+  }
+}
+
+// BGK collision with the Guo source (ConvLBM, lbm.cpp:75-118) of one cell,
+// in place on raw values.  Returns true when some feq < 0 (lbm.cpp:114).
+// Directions come in pairs m, m+1 = opp(m) with c_{m+1} = -c_m, so
+// A(-cu) = s - p and A(cu) = s + p share s = 4.5cu^2 - 1.5uu and
+// p = cu (3 + 4.5cu^2 - 4.5uu).  f(1-omega) + omega feq is evaluated as
+// feq + (1-omega)(f - feq), which rounds the large term once.
+template <class S, int FORCE>
+__device__ __forceinline__ bool relax(typename Store<S>::C (&f)[Q], typename Store<S>::C rho,
+                                      typename Store<S>::C drho, typename Store<S>::C ux,
+                                      typename Store<S>::C uy, typename Store<S>::C uz,
+                                      typename Store<S>::C Fx, typename Store<S>::C Fy,
+                                      typename Store<S>::C Fz, typename Store<S>::C omega) {
+  using C = typename Store<S>::C;
+  constexpr C K = C(Store<S>::K);
   const C uu = ux * ux + uy * uy + uz * uz;
   const C om1 = C(1) - omega;
-  const C fpre = C(1) - omega * C(0.5);
-  C Fu = C(0);
-  if constexpr (FORCE != 0) Fu = Fx * ux + Fy * uy + Fz * uz;
-  bool neg = false;
+  const C c3 = C(3) - C(4.5) * uu;  // 3 - 4.5uu
+  const C mh = C(-1.5) * uu;        // -1.5uu
+  // raw equilibrium feq = Kw*(drho + rho*A) (shifted) or w*rho*(1 + A)
+  // (unshifted) = ra*A + da per weight class; relaxation in the
+  // non-equilibrium form f* = feq + (1 - omega)(f - feq)
+  C ra[3], da[3];
+  {
+    constexpr C wc[3] = {C(1.0 / 3), C(1.0 / 18), C(1.0 / 36)};
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      ra[q] = (K * wc[q]) * rho;
+      da[q] = Store<S>::shifted ? (K * wc[q]) * drho : ra[q];
+    }
+  }
+  C fpre = C(0), Fu = C(0);
+  if constexpr (FORCE != 0) {
+    fpre = C(1) - omega * C(0.5);
+    Fu = Fx * ux + Fy * uy + Fz * uz;
+  }
+  C amin = mh, amax = mh;  // A of the rest direction is -1.5uu
+  // rest direction
+  {
+    const C fe = ra[0] * mh + da[0];
+    f[0] = fe + om1 * (f[0] - fe);
+  }
+  if constexpr (FORCE != 0) f[0] += (K * C(1.0 / 3)) * fpre * (C(-3) * Fu);
   for_dirs([&](auto M) {
     constexpr int m = decltype(M)::value;
-    constexpr C w = C(W[m]);
-    const C cu = C(CX[m]) * ux + C(CY[m]) * uy + C(CZ[m]) * uz;
-    const C A = eq_A(cu, uu);
-    const C feq_full = rho * w * (C(1) + A);
-    neg |= feq_full < C(0);
-    C feq;
-    if constexpr (Store<S>::shifted) feq = w * (drho + rho * A);
-    else feq = feq_full;
-    C fs = f[m] * om1 + omega * feq;
-    if constexpr (FORCE != 0) {
-      const C Fc = C(CX[m]) * Fx + C(CY[m]) * Fy + C(CZ[m]) * Fz;
-      fs += fpre * w * (C(3) * Fc + C(9) * Fc * cu - C(3) * Fu);
+    if constexpr (m % 2 == 1) {
+      constexpr int q = (m <= 6) ? 1 : 2;
+      const C cu = C(CX[m]) * ux + C(CY[m]) * uy + C(CZ[m]) * uz;
+      const C t2 = cu * cu;
+      const C s_ = C(4.5) * t2 + mh;
+      const C p_ = cu * (C(4.5) * t2 + c3);
+      const C Ap = s_ + p_, Am = s_ - p_;
+      amin = vmin(amin, vmin(Ap, Am));
+      amax = vmax(amax, vmax(Ap, Am));
+      const C fep = ra[q] * Ap + da[q], fem = ra[q] * Am + da[q];
+      C fp = fep + om1 * (f[m] - fep);
+      C fm = fem + om1 * (f[m + 1] - fem);
+      if constexpr (FORCE != 0) {
+        constexpr C Kw = C(Store<S>::K * W[m]);
+        const C Fc = C(CX[m]) * Fx + C(CY[m]) * Fy + C(CZ[m]) * Fz;
+        const C kf = Kw * fpre;
+        // S(c) = Kw fpre (3Fc + 9 Fc cu - 3Fu); S(-c) has Fc, cu negated
+        fp += kf * (Fc * (C(3) + C(9) * cu) - C(3) * Fu);
+        fm += kf * (-Fc * (C(3) - C(9) * cu) - C(3) * Fu);
+      }
+      f[m] = fp;
+      f[m + 1] = fm;
     }
-    Store<S>::st(out + (uint32_t)m * ps + idx, fs);
   });
-  return neg;
+  // feq = rho w (1 + A) < 0 for some direction (lbm.cpp:114-117)
+  return (rho > C(0) && C(1) + amin < C(0)) || (rho < C(0) && C(1) + amax > C(0));
 }
 
 // ---------------------------------------------------------------------------
-// Pull streaming (+ moments, + collision).  The value arriving at fluid cell
-// x in direction m is, as in the push loop of lbm.cpp:252-292:
+// Pull of one fluid cell.  The value arriving at cell x in direction m is,
+// as in the push loop of lbm.cpp:252-292:
 //   x - c_m is TYPE_F      : f*(x - c_m, m)
 //   x - c_m is TYPE_S      : f*(x, opp m) + 2 w_m (u_w . c_m) / cs2
 //   x - c_m any other type : f(x, m) untouched since Init; that value is kept
 //                            in slot m of the source cell, which never
 //                            collides, so the same load reads it.
-// COLLIDE: relax and store f*; otherwise store f (post-streaming).
-// EMIT   : write rho, u (u = (sum f c + F/2)/rho, lbm.cpp:672-675).
 // ---------------------------------------------------------------------------
-template <class S, int FORCE, bool COLLIDE, bool EMIT>
-__global__ __launch_bounds__(256) void k_stream(const S *__restrict__ fin, S *__restrict__ fout,
-                                                const uint8_t *__restrict__ mask,
-                                                KArgs<typename Store<S>::C> a) {
-  using C = typename Store<S>::C;
-  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= a.g.ncell) return;
-  const uint8_t mk = mask[idx];
-  if (!(mk & M_FLUID)) return;
-  const uint32_t ps = a.g.ps, nx = a.g.nx, plane = a.g.plane;
+template <class S>
+__device__ __forceinline__ const S *at(const S *base, uint32_t boff) {
+  return reinterpret_cast<const S *>(reinterpret_cast<const char *>(base) + boff);
+}
+template <class S>
+__device__ __forceinline__ S *at(S *base, uint32_t boff) {
+  return reinterpret_cast<S *>(reinterpret_cast<char *>(base) + boff);
+}
 
-  C f[Q];
-  if (!(mk & (M_WALL | M_WRAP))) {
-    for_dirs([&](auto M) {
-      constexpr int m = decltype(M)::value;
-      const uint32_t src = idx - (uint32_t)(CX[m] + CY[m] * (int)nx + CZ[m] * (int)plane);
-      f[m] = Store<S>::ld(fin + (uint32_t)m * ps + src);
-    });
-  } else {
+// Addressing: a wave-uniform 64-bit base per direction plus one 32-bit
+// per-lane byte offset, so every load uses the scalar-base form
+// (global_load saddr) with no per-access 64-bit vector arithmetic.
+//
+// Bulk loads f[m] = src[m][idx] are issued for every thread before its mask
+// is known (they overlap the mask load instead of waiting a full memory
+// round-trip for it).  They never leave the population allocation: for any
+// idx < ncell, m*ps + idx - off_m lies in [0, 19*ps) because |off_m| <=
+// plane + px + 1 < ps.  Cells that are not plain bulk fluid then patch the
+// directions whose source differs (patch_cell).
+template <class S>
+__device__ __forceinline__ void bulk_loads(const Planes<S> &P, uint32_t idx,
+                                           typename Store<S>::C (&f)[Q]) {
+  const uint32_t boff = idx * (uint32_t)sizeof(S);
+  for_dirs([&](auto M) {
+    constexpr int m = decltype(M)::value;
+    f[m] = Store<S>::ld(at(P.src[m], boff));
+  });
+}
+
+// An empty asm statement that takes the bulk-loaded values as inputs and
+// passes the mask byte through: the mask test then depends on every load
+// having completed, so neither the loads nor the test can be moved across
+// each other (the compiler would otherwise sink the loads into the fluid
+// branch, or hoist the test above them, and serialise two memory
+// round-trips per wave).  It emits no instruction.
+template <class C>
+__device__ __forceinline__ uint32_t after_loads(uint32_t mk, const C (&f)[Q]) {
+  asm volatile(""
+               : "+v"(mk)
+               : "v"(f[0]), "v"(f[1]), "v"(f[2]), "v"(f[3]), "v"(f[4]), "v"(f[5]), "v"(f[6]),
+                 "v"(f[7]), "v"(f[8]), "v"(f[9]), "v"(f[10]), "v"(f[11]), "v"(f[12]),
+                 "v"(f[13]), "v"(f[14]), "v"(f[15]), "v"(f[16]), "v"(f[17]), "v"(f[18]));
+  return mk;
+}
+
+template <class S>
+__device__ __forceinline__ void patch_cell(const Planes<S> &P, uint8_t mk, uint32_t idx,
+                                           const KArgs<typename Store<S>::C> &a,
+                                           typename Store<S>::C (&f)[Q]) {
+  using C = typename Store<S>::C;
+  const uint32_t boff = idx * (uint32_t)sizeof(S);
+  if (mk & M_WRAP) {
+    // periodic wrap of the padded lattice (lbm.cpp:197-199)
     uint32_t x, y, z;
     coords(a.g, idx, x, y, z);
-    const uint32_t lk = (mk & M_WALL) ? a.links[idx] : 0u;
+    for_dirs([&](auto M) {
+      constexpr int m = decltype(M)::value;
+      const uint32_t src = wrap_src(a.g, x, y, z, CX[m], CY[m], CZ[m]);
+      if (src != idx - pull_off<m>(a.g)) f[m] = Store<S>::ld(at(P.in[m], src * (uint32_t)sizeof(S)));
+    });
+  }
+  if (mk & M_WALL) {
+    const uint32_t lk = a.links[idx];
     C uwx = C(0), uwy = C(0), uwz = C(0);
     if (mk & M_MOVING) {
       uwx = a.uwx[idx];
@@ -140,61 +282,46 @@ __global__ __launch_bounds__(256) void k_stream(const S *__restrict__ fin, S *__
       constexpr int m = decltype(M)::value;
       if (lk & (1u << m)) {
         const C ucw = C(CX[m]) * uwx + C(CY[m]) * uwy + C(CZ[m]) * uwz;
-        f[m] = Store<S>::ld(fin + (uint32_t)OPP[m] * ps + idx) + C(6.0 * W[m]) * ucw;
-      } else {
-        const uint32_t src =
-            (mk & M_WRAP) ? wrap_src(a.g, x, y, z, CX[m], CY[m], CZ[m])
-                          : idx - (uint32_t)(CX[m] + CY[m] * (int)nx + CZ[m] * (int)plane);
-        f[m] = Store<S>::ld(fin + (uint32_t)m * ps + src);
+        f[m] = Store<S>::ld(at(P.in[OPP[m]], boff)) + C(Store<S>::K * 6.0 * W[m]) * ucw;
       }
     });
-    if (mk & M_BACKEQ) {
-      // after the push, the back-face inflow loop overwrites this cell with
-      // feq2 of the TYPE_IN cell (i,j,0) (lbm.cpp:599-611)
-      const uint32_t s = idx - (a.g.nz - 1) * plane;
-      const C r = a.rho[s], vx = a.ux[s], vy = a.uy[s], vz = a.uz[s];
-      const C uu = vx * vx + vy * vy + vz * vz;
-      for_dirs([&](auto M) {
-        constexpr int m = decltype(M)::value;
-        const C cu = C(CX[m]) * vx + C(CY[m]) * vy + C(CZ[m]) * vz;
-        f[m] = feq2_value<C, Store<S>::shifted>(r, cu, uu, C(W[m]));
-      });
-    }
   }
-
-  if constexpr (!COLLIDE && !EMIT) {
+  if (mk & M_BACKEQ) {
+    // after the push, the back-face inflow loop overwrites this cell with
+    // feq2 of the TYPE_IN cell (i,j,0) (lbm.cpp:599-611)
+    const uint32_t s = idx - (a.g.nz - 1) * a.g.plane;
+    const C r = a.rho[s], vx = a.ux[s], vy = a.uy[s], vz = a.uz[s];
+    const C uu = vx * vx + vy * vy + vz * vz;
     for_dirs([&](auto M) {
       constexpr int m = decltype(M)::value;
-      Store<S>::st(fout + (uint32_t)m * ps + idx, f[m]);
+      const C cu = C(CX[m]) * vx + C(CY[m]) * vy + C(CZ[m]) * vz;
+      f[m] = feq2_raw<C, Store<S>::shifted>(r, cu, uu, C(W[m]), C(Store<S>::K));
     });
-    return;
-  } else {
-    // moments (MacroProp, lbm.cpp:668-675)
-    C s0 = C(0);
-    for_dirs([&](auto M) { s0 += f[decltype(M)::value]; });
-    const C jx = ((f[1] - f[2]) + (f[7] - f[8])) + ((f[9] - f[10]) + (f[13] - f[14])) + (f[15] - f[16]);
-    const C jy = ((f[3] - f[4]) + (f[7] - f[8])) + ((f[11] - f[12]) + (f[14] - f[13])) + (f[17] - f[18]);
-    const C jz = ((f[5] - f[6]) + (f[9] - f[10])) + ((f[11] - f[12]) + (f[16] - f[15])) + (f[18] - f[17]);
-    C rho, drho;
-    if constexpr (Store<S>::shifted) {
-      drho = s0;
-      rho = C(1) + s0;
-    } else {
-      rho = s0;
-      drho = s0 - C(1);
-    }
+  }
+}
+
+template <class S>
+__device__ __forceinline__ void pull_cell(const Planes<S> &P, uint8_t mk, uint32_t idx,
+                                          const KArgs<typename Store<S>::C> &a,
+                                          typename Store<S>::C (&f)[Q]) {
+  bulk_loads<S>(P, idx, f);
+  if (mk & (M_WALL | M_WRAP | M_BACKEQ)) patch_cell<S>(P, mk, idx, a, f);
+}
+
+// Moments, optional macro emission, optional collision, store of one cell.
+template <class S, int FORCE, bool COLLIDE, bool EMIT>
+__device__ __forceinline__ void finish_cell(typename Store<S>::C (&f)[Q], uint32_t idx,
+                                            const KArgs<typename Store<S>::C> &a,
+                                            const Planes<S> &P) {
+  using C = typename Store<S>::C;
+  if constexpr (COLLIDE || EMIT) {
+    C rho, drho, jx, jy, jz;
+    moments<S>(f, rho, drho, jx, jy, jz);
     C Fx, Fy, Fz;
     load_force<C, FORCE>(a, idx, Fx, Fy, Fz);
     C ux, uy, uz;
-    if constexpr (FORCE != 0) {
-      ux = (jx + C(0.5) * Fx) / rho;
-      uy = (jy + C(0.5) * Fy) / rho;
-      uz = (jz + C(0.5) * Fz) / rho;
-    } else {
-      ux = jx / rho;
-      uy = jy / rho;
-      uz = jz / rho;
-    }
+    if constexpr (FORCE != 0) velocity(jx + C(0.5) * Fx, jy + C(0.5) * Fy, jz + C(0.5) * Fz, rho, ux, uy, uz);
+    else velocity(jx, jy, jz, rho, ux, uy, uz);
     int bad = (rho < C(0) || rho > C(100)) ? ST_RHO_RANGE : 0;
     if constexpr (EMIT) {
       a.rho[idx] = rho;
@@ -203,16 +330,32 @@ __global__ __launch_bounds__(256) void k_stream(const S *__restrict__ fin, S *__
       a.uz[idx] = uz;
     }
     if constexpr (COLLIDE) {
-      if (collide_store<S, FORCE>(f, rho, drho, ux, uy, uz, Fx, Fy, Fz, a.omega, fout, ps, idx))
-        bad |= ST_FEQ_NEG;
-    } else {
-      for_dirs([&](auto M) {
-        constexpr int m = decltype(M)::value;
-        Store<S>::st(fout + (uint32_t)m * ps + idx, f[m]);
-      });
+      if (relax<S, FORCE>(f, rho, drho, ux, uy, uz, Fx, Fy, Fz, a.omega)) bad |= ST_FEQ_NEG;
     }
     if (bad) flag(a.status, a.first_bad, bad, idx);
   }
+  const uint32_t boff = idx * (uint32_t)sizeof(S);
+  for_dirs([&](auto M) {
+    constexpr int m = decltype(M)::value;
+    Store<S>::st(at(P.out[m], boff), f[m]);
+  });
+}
+
+// One cell per thread.
+// COLLIDE: relax and store f*; otherwise store f (post-streaming).
+// EMIT   : write rho, u (u = (sum f c + F/2)/rho, lbm.cpp:672-675).
+template <class S, int FORCE, bool COLLIDE, bool EMIT>
+__global__ __launch_bounds__(256) void k_stream(const Planes<S> P, const uint8_t *__restrict__ mask,
+                                                KArgs<typename Store<S>::C> a) {
+  using C = typename Store<S>::C;
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.g.ncell) return;
+  C f[Q];
+  bulk_loads<S>(P, idx, f);  // in flight together with the mask load
+  const uint8_t mk = (uint8_t)after_loads(mask[idx], f);
+  if (!(mk & M_FLUID)) return;
+  if (mk & (M_WALL | M_WRAP | M_BACKEQ)) patch_cell<S>(P, mk, idx, a, f);
+  finish_cell<S, FORCE, COLLIDE, EMIT>(f, idx, a, P);
 }
 
 // LBM::Collision (lbm.cpp:60-73) in place, with rho, u and F as stored.
@@ -233,13 +376,18 @@ __global__ __launch_bounds__(256) void k_collide(S *__restrict__ buf, const uint
   const C ux = a.ux[idx], uy = a.uy[idx], uz = a.uz[idx];
   C Fx, Fy, Fz;
   load_force<C, FORCE>(a, idx, Fx, Fy, Fz);
-  if (collide_store<S, FORCE>(f, rho, rho - C(1), ux, uy, uz, Fx, Fy, Fz, a.omega, buf, ps, idx))
-    flag(a.status, a.first_bad, ST_FEQ_NEG, idx);
+  const bool neg = relax<S, FORCE>(f, rho, rho - C(1), ux, uy, uz, Fx, Fy, Fz, a.omega);
+  for_dirs([&](auto M) {
+    constexpr int m = decltype(M)::value;
+    Store<S>::st(buf + (uint32_t)m * ps + idx, f[m]);
+  });
+  if (neg) flag(a.status, a.first_bad, ST_FEQ_NEG, idx);
 }
 
-// LBM::MacroProp (lbm.cpp:657-776) on TYPE_F and TYPE_S cells.  Cells of
-// other types keep their stored rho,u: their population slots carry the
-// frozen-link values of their fluid neighbours.
+// LBM::MacroProp (lbm.cpp:657-776) on the cells whose mask has a bit of
+// `sel` (TYPE_F and TYPE_S cells).  Cells of other types keep their stored
+// rho,u: their population slots carry the frozen-link values of their fluid
+// neighbours.
 template <class S, int FORCE>
 __global__ __launch_bounds__(256) void k_macro(const S *__restrict__ buf,
                                                const uint8_t *__restrict__ mask, uint8_t sel,
@@ -249,19 +397,13 @@ __global__ __launch_bounds__(256) void k_macro(const S *__restrict__ buf,
   if (idx >= a.g.ncell) return;
   if (!(mask[idx] & sel)) return;
   const uint32_t ps = a.g.ps;
-  C s0 = C(0), jx = C(0), jy = C(0), jz = C(0);
+  C f[Q];
   for_dirs([&](auto M) {
     constexpr int m = decltype(M)::value;
-    const C v = Store<S>::ld(buf + (uint32_t)m * ps + idx);
-    s0 += v;
-    if constexpr (CX[m] > 0) jx += v;
-    if constexpr (CX[m] < 0) jx -= v;
-    if constexpr (CY[m] > 0) jy += v;
-    if constexpr (CY[m] < 0) jy -= v;
-    if constexpr (CZ[m] > 0) jz += v;
-    if constexpr (CZ[m] < 0) jz -= v;
+    f[m] = Store<S>::ld(buf + (uint32_t)m * ps + idx);
   });
-  const C rho = Store<S>::shifted ? C(1) + s0 : s0;
+  C rho, drho, jx, jy, jz;
+  moments<S>(f, rho, drho, jx, jy, jz);
   C Fx, Fy, Fz;
   load_force<C, FORCE>(a, idx, Fx, Fy, Fz);
   a.rho[idx] = rho;
@@ -297,17 +439,18 @@ __global__ __launch_bounds__(256) void k_init(S *__restrict__ A, S *__restrict__
     const C uu = ux * ux + uy * uy + uz * uz;
     const C cu = C(CX[m]) * ux + C(CY[m]) * uy + C(CZ[m]) * uz;
     const C A_ = eq_A(cu, uu);
-    C v;
-    if constexpr (Store<S>::shifted) v = C(W[m]) * ((rho - C(1)) + rho * A_);
-    else v = rho * C(W[m]) * (C(1) + A_);
-    const S s = Store<S>::enc(v);
+    C r;
+    if constexpr (Store<S>::shifted) r = C(Store<S>::K * W[m]) * ((rho - C(1)) + rho * A_);
+    else r = rho * C(W[m]) * (C(1) + A_);
+    const S s = Store<S>::enc(r);
     A[(uint32_t)m * ps + idx] = s;
     B[(uint32_t)m * ps + idx] = s;
   });
 }
 
 // Per-cell kernel mask, bounce-back link word and wall velocity from the
-// cell types (device order) and the solids' current velocities.
+// cell types (device order; row-padding cells get mask 0) and the solids'
+// current velocities.
 template <class C>
 __global__ __launch_bounds__(256) void k_tables(const int16_t *__restrict__ type,
                                                 uint8_t *__restrict__ mask,
@@ -319,40 +462,43 @@ __global__ __launch_bounds__(256) void k_tables(const int16_t *__restrict__ type
   if (idx >= g.ncell) return;
   uint32_t x, y, z;
   coords(g, idx, x, y, z);
-  const int16_t t = type[idx];
   uint8_t mk = 0;
-  if (t == 0) mk |= M_FLUID;
-  else if (t == 1) mk |= M_SOLID;
-  else mk |= M_OTHER;
-  if (x == 0 || x == g.nx - 1 || y == 0 || y == g.ny - 1 || z == 0 || z == g.nz - 1) mk |= M_WRAP;
-  if (z == g.nz - 1 && type[idx - (g.nz - 1) * g.plane] == 3) mk |= M_BACKEQ;
   uint32_t lk = 0;
   C wx = C(0), wy = C(0), wz = C(0);
-  if (t == 0) {
-    // bounce-back links: source x - c_m is solid
-    for_dirs([&](auto M) {
-      constexpr int m = decltype(M)::value;
-      if (type[wrap_src(g, x, y, z, CX[m], CY[m], CZ[m])] == 1) lk |= 1u << m;
-    });
-    // Determine_Wall_Velocity, lbm.cpp:941-962: neighbours x + c_l in l order
-    int num = 0;
-    for_dirs([&](auto M) {
-      constexpr int l = decltype(M)::value;
-      const uint32_t n = wrap_dst(g, x, y, z, CX[l], CY[l], CZ[l]);
-      if (type[n] == 1 && (ux[n] != C(0) || uy[n] != C(0) || uz[n] != C(0))) {
-        wx += ux[n];
-        wy += uy[n];
-        wz += uz[n];
-        num++;
+  if (x < g.nx) {
+    const int16_t t = type[idx];
+    if (t == 0) mk |= M_FLUID;
+    else if (t == 1) mk |= M_SOLID;
+    else mk |= M_OTHER;
+    if (x == 0 || x == g.nx - 1 || y == 0 || y == g.ny - 1 || z == 0 || z == g.nz - 1)
+      mk |= M_WRAP;
+    if (z == g.nz - 1 && type[idx - (g.nz - 1) * g.plane] == 3) mk |= M_BACKEQ;
+    if (t == 0) {
+      // bounce-back links: source x - c_m is solid
+      for_dirs([&](auto M) {
+        constexpr int m = decltype(M)::value;
+        if (type[wrap_src(g, x, y, z, CX[m], CY[m], CZ[m])] == 1) lk |= 1u << m;
+      });
+      // Determine_Wall_Velocity, lbm.cpp:941-962: neighbours x + c_l in l order
+      int num = 0;
+      for_dirs([&](auto M) {
+        constexpr int l = decltype(M)::value;
+        const uint32_t n = wrap_dst(g, x, y, z, CX[l], CY[l], CZ[l]);
+        if (type[n] == 1 && (ux[n] != C(0) || uy[n] != C(0) || uz[n] != C(0))) {
+          wx += ux[n];
+          wy += uy[n];
+          wz += uz[n];
+          num++;
+        }
+      });
+      if (num != 0) {
+        wx /= C(num);
+        wy /= C(num);
+        wz /= C(num);
       }
-    });
-    if (num != 0) {
-      wx /= C(num);
-      wy /= C(num);
-      wz /= C(num);
+      if (lk) mk |= M_WALL;
+      if (lk && (wx != C(0) || wy != C(0) || wz != C(0))) mk |= M_MOVING;
     }
-    if (lk) mk |= M_WALL;
-    if (lk && (wx != C(0) || wy != C(0) || wz != C(0))) mk |= M_MOVING;
   }
   mask[idx] = mk;
   if (links) links[idx] = lk;
@@ -381,13 +527,13 @@ __global__ __launch_bounds__(256) void k_backeq_solid(S *__restrict__ A, S *__re
   for_dirs([&](auto M) {
     constexpr int m = decltype(M)::value;
     const C cu = C(CX[m]) * vx + C(CY[m]) * vy + C(CZ[m]) * vz;
-    const S v = Store<S>::enc(feq2_value<C, Store<S>::shifted>(r, cu, uu, C(W[m])));
+    const S v = Store<S>::enc(feq2_raw<C, Store<S>::shifted>(r, cu, uu, C(W[m]), C(Store<S>::K)));
     A[(uint32_t)m * ps + idx] = v;
     B[(uint32_t)m * ps + idx] = v;
   });
 }
 
-// The streaming tables exactly as k_stream resolves them.
+// The streaming tables exactly as the stream kernels resolve them.
 __global__ __launch_bounds__(256) void k_links(const uint8_t *__restrict__ mask,
                                                const uint32_t *__restrict__ links, Geo g,
                                                int32_t *__restrict__ src_out,
@@ -395,7 +541,6 @@ __global__ __launch_bounds__(256) void k_links(const uint8_t *__restrict__ mask,
   const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= g.ncell) return;
   const uint8_t mk = mask[idx];
-  const uint32_t nx = g.nx, plane = g.plane;
   if (!(mk & M_FLUID)) {
     for (int m = 0; m < Q; m++) {
       src_out[(size_t)idx * Q + m] = -1;
@@ -408,9 +553,8 @@ __global__ __launch_bounds__(256) void k_links(const uint8_t *__restrict__ mask,
   const uint32_t lk = (mk & M_WALL) ? links[idx] : 0u;
   for_dirs([&](auto M) {
     constexpr int m = decltype(M)::value;
-    const uint32_t src = (mk & M_WRAP)
-                             ? wrap_src(g, x, y, z, CX[m], CY[m], CZ[m])
-                             : idx - (uint32_t)(CX[m] + CY[m] * (int)nx + CZ[m] * (int)plane);
+    const uint32_t src = (mk & M_WRAP) ? wrap_src(g, x, y, z, CX[m], CY[m], CZ[m])
+                                       : idx - pull_off<m>(g);
     uint8_t kind;
     if (lk & (1u << m)) kind = 1;
     else kind = (mask[src] & M_FLUID) ? 0 : 2;
@@ -439,7 +583,7 @@ __global__ __launch_bounds__(256) void k_f_out(const S *__restrict__ buf,
     double v;
     if (mk & (M_FLUID | M_SOLID)) {
       v = (double)Store<S>::ld(buf + (uint32_t)m * ps + idx);
-      if constexpr (Store<S>::shifted) v += W[m];
+      if constexpr (Store<S>::shifted) v = v / (double)Store<S>::K + W[m];
     } else {
       const C cu = C(CX[m]) * ux + C(CY[m]) * uy + C(CZ[m]) * uz;
       v = (double)(rho * C(W[m]) * (C(1) + eq_A(cu, uu)));
@@ -541,28 +685,37 @@ static KArgs<typename Store<S>::C> make_args(const LaunchCtx &L) {
   return a;
 }
 
+template <class S>
+static Planes<S> make_planes(const LaunchCtx &L, const S *in, S *out) {
+  Planes<S> P;
+  const size_t ps = L.g.ps;
+  for (int m = 0; m < Q; m++) {
+    const ptrdiff_t off = (ptrdiff_t)CX[m] + (ptrdiff_t)CY[m] * L.g.px + (ptrdiff_t)CZ[m] * L.g.plane;
+    P.in[m] = in + m * ps;
+    P.src[m] = in + (ptrdiff_t)(m * ps) - off;
+    P.out[m] = out + m * ps;
+  }
+  return P;
+}
+
+template <class S, int FORCE, bool COLLIDE, bool EMIT>
+static void stream_launch(const LaunchCtx &L, const S *in, S *out) {
+  auto a = make_args<S>(L);
+  const Planes<S> P = make_planes(L, in, out);
+  hipLaunchKernelGGL((k_stream<S, FORCE, COLLIDE, EMIT>), grid_for(L.g.ncell, L.block),
+                     dim3(L.block), 0, L.stream, P, L.mask, a);
+}
+
 template <class S, int FORCE>
 static hipError_t stream_dispatch(const LaunchCtx &L, const void *fin, void *fout, int mode) {
-  auto a = make_args<S>(L);
-  const dim3 grid = grid_for(L.g.ncell, L.block);
-  const dim3 blk(L.block);
   const S *in = (const S *)fin;
   S *out = (S *)fout;
   switch (mode) {
-    case STREAM_ONLY:
-      hipLaunchKernelGGL((k_stream<S, FORCE, false, false>), grid, blk, 0, L.stream, in, out, L.mask, a);
-      break;
-    case STREAM_MACRO:
-      hipLaunchKernelGGL((k_stream<S, FORCE, false, true>), grid, blk, 0, L.stream, in, out, L.mask, a);
-      break;
-    case STREAM_COLLIDE:
-      hipLaunchKernelGGL((k_stream<S, FORCE, true, false>), grid, blk, 0, L.stream, in, out, L.mask, a);
-      break;
-    case STREAM_COLLIDE_MACRO:
-      hipLaunchKernelGGL((k_stream<S, FORCE, true, true>), grid, blk, 0, L.stream, in, out, L.mask, a);
-      break;
-    default:
-      return hipErrorInvalidValue;
+    case STREAM_ONLY: stream_launch<S, FORCE, false, false>(L, in, out); break;
+    case STREAM_MACRO: stream_launch<S, FORCE, false, true>(L, in, out); break;
+    case STREAM_COLLIDE: stream_launch<S, FORCE, true, false>(L, in, out); break;
+    case STREAM_COLLIDE_MACRO: stream_launch<S, FORCE, true, true>(L, in, out); break;
+    default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }

@@ -7,10 +7,15 @@
 
 namespace lbm {
 
+// Device lattice geometry.  Cell (x,y,z) of the padded (Nx+2)(Ny+2)(Nz+2)
+// lattice sits at idx = (z*ny + y)*px + x.  The row pitch px >= nx is a
+// multiple of 4 (rows start 16-byte aligned for FP32); cells with x >= nx
+// are dead padding (mask 0, never read by a live cell).
 struct Geo {
   uint32_t nx, ny, nz;  // padded extents Nx+2, Ny+2, Nz+2
-  uint32_t plane;       // nx*ny
-  uint32_t ncell;       // nx*ny*nz
+  uint32_t px;          // row pitch (cells), multiple of 4
+  uint32_t plane;       // px*ny
+  uint32_t ncell;       // px*ny*nz (includes row padding)
   uint32_t ps;          // population plane stride (elements), >= ncell
 };
 

@@ -82,6 +82,8 @@ _SIGNATURES = {
     "lbm_storage": (C.c_int32, [_P]),
     "lbm_time_steps": (C.c_int, [_P, C.c_int32, _D]),
     "lbm_bytes_per_cell": (C.c_double, [_P]),
+    "lbm_set_launch": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "lbm_get_launch": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "lbm_error_string": (C.c_char_p, [C.c_int]),
 }
 
@@ -237,6 +239,14 @@ class Lattice:
         out = np.zeros(3, np.float64)
         _check(self._lib.lbm_reduce_fp64(self._h, int(what), _dp(out)), "lbm_reduce_fp64")
         return out[:3] if what == REDUCE_MOMENTUM else float(out[0])
+
+    def set_launch(self, block: int, cells_per_thread: int = 1):
+        _check(self._lib.lbm_set_launch(self._h, int(block), int(cells_per_thread)), "lbm_set_launch")
+
+    def get_launch(self):
+        b, v = C.c_int32(0), C.c_int32(0)
+        _check(self._lib.lbm_get_launch(self._h, C.byref(b), C.byref(v)), "lbm_get_launch")
+        return b.value, v.value
 
     @property
     def ncells(self) -> int:

@@ -217,3 +217,22 @@ def test_tgv_256_conservation(storage):
     tol = {"f64": 1e-12, "f32": 1e-6, "f16s": 1e-4}[storage]
     assert abs(m1 - m0) / m0 < tol
     assert np.abs(p).max() / m0 < tol
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16s"])
+@pytest.mark.parametrize("block,vec", [(64, 1), (128, 1)])
+def test_launch_shapes_agree(storage, block, vec):
+    """Every launch shape the autotuner may pick gives the same fields as the
+    default (one cell per thread, 256 threads)."""
+    case = cases.taylor_green(32, u0=0.04, Re=200.0)
+    case.type[5:9, 10:12, 3:20] = 1  # a solid block: wall chunks take the per-cell path
+    out = []
+    for shape in ((256, 1), (block, vec)):
+        lat = _gpu(case, storage)
+        lat.set_launch(*shape)
+        lat.step(50)
+        assert lat.status()[0] == 0
+        out.append(lat.macro())
+    fl = case.type == 0
+    eu = rel_l2_u(out[1][1:], out[0][1:], fl)
+    assert eu <= 1e-6, f"{storage} {block}x{vec}: {eu:.3e}"
