@@ -122,11 +122,16 @@ def cpu_baseline(case, budget_s, max_steps):
 
 
 def load_traffic(storage):
-    p = os.path.join(ROOT, "profiles", f"stream_collide_tgv256_{storage}_traffic.json")
+    """HBM bytes per launch of the fused kernel from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (scaled by the copy-kernel reference)."""
+    p = os.path.join(ROOT, "profiles", f"stream_collide_tgv256_{storage}_traffic.txt")
     if not os.path.exists(p):
         return None
     with open(p) as f:
-        return json.load(f).get("bytes_per_launch")
+        for line in f:
+            if line.startswith("bytes_per_launch:"):
+                return int(line.split(":")[1])
+    return None
 
 
 def main():
@@ -141,7 +146,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--storage", default="f32", choices=["f64", "f32", "f16s"])
+    ap.add_argument("--copy-probe-only", action="store_true",
+                    help="run only the 1 GiB device copy (counter scaling reference)")
     a = ap.parse_args()
+    if a.copy_probe_only:
+        g = lbm_amd.copy_probe(1 << 30, 20)
+        print(json.dumps({"copy_probe_gbs": round(g, 1), "bytes_per_launch": 2 * (1 << 30)}))
+        return 0
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the multi-GPU path (z-slab halo exchange) is a "
@@ -168,6 +179,8 @@ def main():
         cpu = cpu_baseline(case, a.cpu_seconds, a.cpu_max_steps)
         log(f"bench: cpu {cpu['value']:.2f} MLUPS on {cpu['cores']} threads")
 
+    copy_gbs = lbm_amd.copy_probe(1 << 30, 20)
+    log(f"bench: 1 GiB device copy {copy_gbs:.0f} GB/s")
     ms_step = main_r["ms"] / a.steps
     achieved = main_r["bytes_per_cell"] * main_r["nfluid"] / (ms_step / 1e3) / 1e9
     traffic = load_traffic(a.storage)
@@ -198,6 +211,8 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "bytes_per_cell": main_r["bytes_per_cell"],
+            "copy_ceiling_gbs": round(copy_gbs, 1),
+            "frac_of_copy": round(achieved / copy_gbs, 4),
         },
         other: {
             "mlups": round(oth_r["mlups"], 1),

@@ -136,6 +136,10 @@ int lbm_get_launch(const lbm_ctx *ctx, int32_t *block, int32_t *cells_per_thread
 /* Bytes one fused launch moves per fluid cell (algorithmic). */
 double lbm_bytes_per_cell(const lbm_ctx *ctx);
 const char *lbm_error_string(int code);
+/* A plain 16-byte-per-lane device copy of `bytes` bytes repeated `reps`
+ * times; *gbs = (read + written bytes) / time.  No reference counterpart: the
+ * practical HBM ceiling and the reference point for hardware byte counters. */
+int lbm_copy_probe(int32_t device, int64_t bytes, int32_t reps, double *gbs);
 
 #ifdef __cplusplus
 }

@@ -37,6 +37,7 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-p
 SOURCES = [
     os.path.join(HERE, "csrc", "lbm_kernels.hip"),
     os.path.join(HERE, "csrc", "lbm_capi.hip"),
+    os.path.join(HERE, "csrc", "lbm_probe.hip"),
 ]
 HOST_SOURCES = [
     os.path.join(HERE, "host", "LBM.cpp"),

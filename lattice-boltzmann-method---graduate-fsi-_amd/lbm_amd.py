@@ -85,6 +85,7 @@ _SIGNATURES = {
     "lbm_set_launch": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "lbm_get_launch": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "lbm_error_string": (C.c_char_p, [C.c_int]),
+    "lbm_copy_probe": (C.c_int, [C.c_int32, C.c_int64, C.c_int32, _D]),
 }
 
 _lib = None
@@ -121,6 +122,13 @@ def _check(code: int, what: str) -> int:
         msg = load_library().lbm_error_string(code).decode()
         raise LBMError(code, what, msg)
     return code
+
+
+def copy_probe(nbytes: int = 1 << 30, reps: int = 20, device: int = 0) -> float:
+    """GB/s of a plain 16-byte-per-lane device copy of nbytes (read + write)."""
+    g = C.c_double(0.0)
+    _check(load_library().lbm_copy_probe(device, int(nbytes), int(reps), C.byref(g)), "lbm_copy_probe")
+    return g.value
 
 
 class Lattice:
