@@ -121,6 +121,42 @@ int lbm_reduce_fp64(lbm_ctx *ctx, int32_t what, double *out);
 int lbm_export_links(lbm_ctx *ctx, int16_t *type, int32_t *src,
                      uint8_t *kind);
 
+/* ---- implicit diffuse-interface IBM (IBLBM, lbm.cpp:1135-1417) ---------- */
+typedef struct lbm_ibm_desc {
+  int32_t nmarkers; /* Lagrangian markers (number_nodes, LBM.h:340)           */
+  int32_t m_max;    /* implicit iterations (LBM.h:352)                       */
+  int32_t box[6];   /* xmin,xmax,ymin,ymax,zmin,zmax, inclusive lattice      *
+                     * indices; the reference's box is floor/ceil(centre-+D) *
+                     * with z = 1..1 (lbm.cpp:1165-1167)                     */
+  int32_t moving;   /* recompute nearest points, interior cells and the    *
+                     * spreading tiles every step (they are recomputed      *
+                     * anyway in every step whose vcm moves the markers)    */
+  int32_t reserved;
+  double vsolid;    /* added-mass volume (Vsolid, LBM.h:355)                 */
+} lbm_ibm_desc;
+
+/* Markers of the body (Generate_Cylinder_Marker, lbm.cpp:1857-1877), their
+ * normals (MARKER::Normal_Marker, lbm.cpp:1048-1069) and weights ds
+ * (main.cpp:123-131).  Requires lbm_init; the box may hold TYPE_F and TYPE_S
+ * cells only.  From here on the context runs the IBM step (no MacroProp). */
+int lbm_ibm_setup(lbm_ctx *ctx, const lbm_ibm_desc *desc, const double *x,
+                  const double *y, const double *z, const double *nx,
+                  const double *ny, const double *nz, const double *ds);
+/* IBLBM(lb, marker, ds, Cl, Cd, V_CM, t) (lbm.cpp:1135-1417): moves the
+ * markers with the trapezoid rule, runs the implicit multi-direct forcing
+ * and returns out = {Drag, Lift, interior density}; Drag and Lift include
+ * the added-mass term and are not yet divided by 1/2 rho0 u^2 D.  The next
+ * lbm_collide uses its result. */
+int lbm_ibm_apply(lbm_ctx *ctx, const double vcm[3], const double vcm_before[3],
+                  double out[3]);
+/* nsteps of main.cpp:199-205 (IBLBM, Collision, Streaming) as fused
+ * launches.  vcm: (nsteps+1) x 3 centre-of-mass velocities (before step 1,
+ * then after each step) or NULL for a body at rest.  out: nsteps x 3. */
+int lbm_ibm_run(lbm_ctx *ctx, int32_t nsteps, const double *vcm, double *out);
+/* marker positions (nmarkers each), velocities and forces (3 x nmarkers) */
+int lbm_ibm_markers(lbm_ctx *ctx, double *x, double *y, double *z, double *ub,
+                    double *Fb);
+
 /* sizes and identification */
 int64_t lbm_ncells(const lbm_ctx *ctx);
 int64_t lbm_nfluid(const lbm_ctx *ctx);
@@ -128,9 +164,9 @@ int32_t lbm_storage(const lbm_ctx *ctx);
 /* Timed run of nsteps fused launches bracketed by HIP events on the
  * context's stream; *ms receives the elapsed milliseconds. */
 int lbm_time_steps(lbm_ctx *ctx, int32_t nsteps, double *ms);
-/* Launch shape of the stream kernels: threads per block (64..1024, a
- * multiple of 64) and cells per thread (1, or 4 for FP32/FP16 storage).  No
- * reference counterpart: the autotuner's knob. */
+/* Launch shape of the stream kernels: threads per block (64..256, a
+ * multiple of 64); cells per thread must be 1.  No reference counterpart:
+ * the launch-shape knob. */
 int lbm_set_launch(lbm_ctx *ctx, int32_t block, int32_t cells_per_thread);
 int lbm_get_launch(const lbm_ctx *ctx, int32_t *block, int32_t *cells_per_thread);
 /* Bytes one fused launch moves per fluid cell (algorithmic). */

@@ -38,6 +38,7 @@ SOURCES = [
     os.path.join(HERE, "csrc", "lbm_kernels.hip"),
     os.path.join(HERE, "csrc", "lbm_capi.hip"),
     os.path.join(HERE, "csrc", "lbm_probe.hip"),
+    os.path.join(HERE, "csrc", "lbm_ibm.hip"),
 ]
 HOST_SOURCES = [
     os.path.join(HERE, "host", "LBM.cpp"),

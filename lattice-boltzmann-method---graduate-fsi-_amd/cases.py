@@ -232,6 +232,43 @@ def cylinder_markers(center_x: float, center_y: float, D: float, n_theta: int,
     )
 
 
+def ibm_cylinder(Nx: int = 60, Ny: int = 40, Nz: int = 1, D: float = 10.0, u_max: float = 0.04,
+                 Re: float = 100.0, center=None, n_theta: int | None = None, seed: int = 1,
+                 perturb: float = 0.0):
+    """main_setup_Cylinder_IBM (lbm.cpp:1828-1855) with its markers: i = 0
+    TYPE_IN, i = Nx+1 and j = 0, Ny+1 TYPE_OUT (slip faces), the k ghost
+    layers TYPE_F (periodic); every cell starts at rho = 1, u = (u_max, 0, 0),
+    TYPE_F cells optionally with the seeded perturbation of channel().  A
+    cylinder of diameter D along z (one marker ring per lattice layer
+    k = 1..Nz) and the reference's box [floor(c - D), ceil(c + D)] in x and
+    y (lbm.cpp:1165-1166), k = 1..Nz.  Returns (case, markers, box)."""
+    t, rho, ux, uy, uz = _blank(Nx, Ny, Nz)
+    t[0, :, :] = TYPE_IN
+    t[-1, :, :] = TYPE_OUT
+    t[:, 0, :] = TYPE_OUT
+    t[:, -1, :] = TYPE_OUT
+    ux[:] = u_max
+    if perturb:
+        cell = np.arange(t.size, dtype=np.uint64).reshape(t.shape)
+        xi = [counter_uniform(seed, 3 * cell + np.uint64(c)) for c in range(3)]
+        f = t == TYPE_F
+        ux[f] = u_max * (1.0 + perturb * xi[0][f])
+        uy[f] = perturb * u_max * xi[1][f]
+        uz[f] = perturb * u_max * xi[2][f]
+    cx, cy = center if center is not None else (Nx / 4.0, Ny / 2.0)
+    if n_theta is None:
+        n_theta = int(round(PI_REF * D))
+    mk = cylinder_markers(cx, cy, D, n_theta, np.arange(1, Nz + 1, dtype=np.float64))
+    box = (int(math.floor(cx - D)), int(math.ceil(cx + D)), int(math.floor(cy - D)),
+           int(math.ceil(cy + D)), 1, Nz)
+    nu = u_max * D / Re
+    tau = 0.5 + 3.0 * nu
+    case = Case("ibm_cylinder", Nx, Ny, Nz, tau, t, rho, ux, uy, uz,
+                params=dict(u_max=u_max, Re=Re, D=D, nu=nu, center=(cx, cy), n_theta=n_theta,
+                            vsolid=PI_REF * D * D / 4.0 * Nz))
+    return case, mk, box
+
+
 def batch_cavities(count: int = 96, seed: int = 1, edges=(32, 48, 64, 96, 128),
                    u_range=(0.02, 0.08), re_range=(50.0, 1000.0)):
     """The seeded cavity ensemble: edge from `edges`, lid velocity and Re

@@ -20,57 +20,12 @@
 #include <vector>
 
 #include "../../include/lbm_hip.h"
+#include "lbm_ctx.h"
 #include "lbm_launch.h"
 
 using namespace lbm;
 
-namespace {
-
-enum { ST_NONE = 0, ST_F = 1, ST_FSTAR = 2 };
-
-}  // namespace
-
-struct lbm_ctx {
-  lbm_desc d{};
-  Geo g{};
-  int storage = 0;
-  size_t esize = 8;  // bytes per stored population
-  size_t csize = 8;  // bytes per macro/force value
-  int dev = 0;
-  hipStream_t stream = nullptr;
-  void *cur = nullptr, *oth = nullptr;
-  int16_t *type = nullptr;
-  uint8_t *mask = nullptr;
-  uint32_t *links = nullptr;
-  void *uw[3] = {nullptr, nullptr, nullptr};
-  void *mac[4] = {nullptr, nullptr, nullptr, nullptr};  // rho, ux, uy, uz
-  void *F[3] = {nullptr, nullptr, nullptr};
-  int force_mode = 0;
-  double Fu[3] = {0, 0, 0};
-  int *status = nullptr;
-  unsigned long long *first_bad = nullptr;
-  double *red_partial = nullptr, *red_out = nullptr;
-  int state = ST_NONE;
-  int uploaded = 0;
-  // The reference's first Streaming reads the solids' setup velocities; from
-  // the first MacroProp on (lbm.cpp:672-675 runs over solids too) it reads
-  // their moments.  Set while the wall tables still hold the setup values.
-  int walls_from_setup = 0;
-  // TYPE_S cells at k = Nz+1 under a TYPE_IN cell at k = 0 take feq2 of that
-  // cell at the first Streaming (the back-face inflow loop, lbm.cpp:599-611)
-  int backeq_solid_pending = 0;
-  int64_t nfluid = 0;
-  int block = 256;
-  std::vector<int16_t> h_type;  // device order
-};
-
-#define HIPCHK(x)                              \
-  do {                                         \
-    hipError_t e_ = (x);                       \
-    if (e_ != hipSuccess) return LBM_ERR_HIP;  \
-  } while (0)
-
-static LaunchCtx launch_ctx(const lbm_ctx *c) {
+LaunchCtx launch_ctx(const lbm_ctx *c) {
   LaunchCtx L;
   memset(&L, 0, sizeof L);
   L.g = c->g;
@@ -97,6 +52,7 @@ static LaunchCtx launch_ctx(const lbm_ctx *c) {
   L.status = c->status;
   L.first_bad = c->first_bad;
   L.stream = c->stream;
+  L.box = c->box;
   return L;
 }
 
@@ -148,12 +104,20 @@ static int download_real(lbm_ctx *c, double *dst, const void *src) {
 }
 
 static void free_all(lbm_ctx *c) {
+  ibm_free(c);
   void *ptrs[] = {c->cur, c->oth, c->type, c->mask, c->links, c->uw[0], c->uw[1], c->uw[2],
                   c->mac[0], c->mac[1], c->mac[2], c->mac[3], c->F[0], c->F[1], c->F[2],
                   c->status, c->first_bad, c->red_partial, c->red_out};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+int after_stream(lbm_ctx *c) {
+  if (!c->backeq_solid_pending) return LBM_OK;
+  HIPCHK(launch_backeq_solid(launch_ctx(c), c->cur, c->oth));
+  c->backeq_solid_pending = 0;
+  return LBM_OK;
 }
 
 extern "C" {
@@ -250,6 +214,10 @@ int lbm_upload_cells(lbm_ctx *c, const int16_t *type, const double *rho, const d
   if (!c) return LBM_ERR_ARG;
   HIPCHK(hipSetDevice(c->dev));
   HIPCHK(hipStreamSynchronize(c->stream));
+  // new cells: a previous IBM set-up no longer applies (lbm_init rebuilds
+  // the mask it marked)
+  ibm_free(c);
+  c->box = BoxArgs{};
   const size_t n = c->g.ncell;
   const size_t nref = (size_t)c->g.nx * c->g.ny * c->g.nz;
   c->h_type.assign(n, (int16_t)-1);  // row padding
@@ -322,6 +290,7 @@ int lbm_init(lbm_ctx *c) {
   HIPCHK(launch_tables(L, c->mask, c->links, c->uw[0], c->uw[1], c->uw[2], c->type));
   // 2. f = f* = feq (LBM::Init)
   HIPCHK(launch_init(L, c->cur, c->oth));
+  if (int e = ibm_remark(c)) return e;
   HIPCHK(hipStreamSynchronize(c->stream));
   c->walls_from_setup = any_solid ? 1 : 0;
   c->backeq_solid_pending = 0;
@@ -334,12 +303,6 @@ int lbm_init(lbm_ctx *c) {
   return LBM_OK;
 }
 
-static int after_stream(lbm_ctx *c) {
-  if (!c->backeq_solid_pending) return LBM_OK;
-  HIPCHK(launch_backeq_solid(launch_ctx(c), c->cur, c->oth));
-  c->backeq_solid_pending = 0;
-  return LBM_OK;
-}
 
 // After the first MacroProp: the solids' rho,u become the moments of their
 // (never changing) Init populations, and the wall velocities of every later
@@ -358,7 +321,15 @@ int lbm_collide(lbm_ctx *c) {
   if (!c) return LBM_ERR_ARG;
   if (c->state != ST_F) return LBM_ERR_STATE;
   HIPCHK(hipSetDevice(c->dev));
-  HIPCHK(launch_collide(launch_ctx(c), c->cur));
+  LaunchCtx L = launch_ctx(c);
+  if (ibm_active(c)) {
+    // an IBM run collides with the result of the preceding IBLBM
+    // (lbm_ibm_apply): force mode 3
+    if (!ibm_ready(c)) return LBM_ERR_STATE;
+    L.force_mode = 3;
+    ibm_consume(c);
+  }
+  HIPCHK(launch_collide(L, c->cur));
   c->state = ST_FSTAR;
   return LBM_OK;
 }
@@ -375,7 +346,7 @@ int lbm_stream(lbm_ctx *c) {
 
 int lbm_macro(lbm_ctx *c) {
   if (!c) return LBM_ERR_ARG;
-  if (c->state != ST_F) return LBM_ERR_STATE;
+  if (c->state != ST_F || ibm_active(c)) return LBM_ERR_STATE;
   HIPCHK(hipSetDevice(c->dev));
   HIPCHK(launch_macro(launch_ctx(c), c->cur));
   return walls_from_moments(c);
@@ -384,7 +355,7 @@ int lbm_macro(lbm_ctx *c) {
 int lbm_step(lbm_ctx *c, int32_t nsteps, int32_t emit_macro) {
   (void)emit_macro;  // the last launch always emits rho,u (the state after MacroProp)
   if (!c || nsteps < 0) return LBM_ERR_ARG;
-  if (c->state == ST_NONE) return LBM_ERR_STATE;
+  if (c->state == ST_NONE || ibm_active(c)) return LBM_ERR_STATE;
   if (nsteps == 0) return LBM_OK;
   HIPCHK(hipSetDevice(c->dev));
   LaunchCtx L = launch_ctx(c);
@@ -413,7 +384,7 @@ int lbm_step(lbm_ctx *c, int32_t nsteps, int32_t emit_macro) {
 
 int lbm_time_steps(lbm_ctx *c, int32_t nsteps, double *ms) {
   if (!c || nsteps < 1 || !ms) return LBM_ERR_ARG;
-  if (c->state == ST_NONE) return LBM_ERR_STATE;
+  if (c->state == ST_NONE || ibm_active(c)) return LBM_ERR_STATE;
   HIPCHK(hipSetDevice(c->dev));
   if (c->walls_from_setup || c->backeq_solid_pending) {
     int e = lbm_step(c, 1, 1);

@@ -1,0 +1,71 @@
+// lbm_ctx.h -- the context behind the opaque lbm_ctx* of the C-ABI, shared
+// by the C-ABI translation units (lbm_capi.hip, lbm_ibm.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "../../include/lbm_hip.h"
+#include "lbm_launch.h"
+
+enum { ST_NONE = 0, ST_F = 1, ST_FSTAR = 2 };
+
+struct lbm_ibm;  // immersed-boundary state (lbm_ibm.hip)
+
+struct lbm_ctx {
+  lbm_desc d{};
+  lbm::Geo g{};
+  int storage = 0;
+  size_t esize = 8;  // bytes per stored population
+  size_t csize = 8;  // bytes per macro/force value
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  void *cur = nullptr, *oth = nullptr;
+  int16_t *type = nullptr;
+  uint8_t *mask = nullptr;
+  uint32_t *links = nullptr;
+  void *uw[3] = {nullptr, nullptr, nullptr};
+  void *mac[4] = {nullptr, nullptr, nullptr, nullptr};  // rho, ux, uy, uz
+  void *F[3] = {nullptr, nullptr, nullptr};
+  int force_mode = 0;
+  double Fu[3] = {0, 0, 0};
+  int *status = nullptr;
+  unsigned long long *first_bad = nullptr;
+  double *red_partial = nullptr, *red_out = nullptr;
+  int state = ST_NONE;
+  int uploaded = 0;
+  // The reference's first Streaming reads the solids' setup velocities; from
+  // the first MacroProp on (lbm.cpp:672-675 runs over solids too) it reads
+  // their moments.  Set while the wall tables still hold the setup values.
+  int walls_from_setup = 0;
+  // TYPE_S cells at k = Nz+1 under a TYPE_IN cell at k = 0 take feq2 of that
+  // cell at the first Streaming (the back-face inflow loop, lbm.cpp:599-611)
+  int backeq_solid_pending = 0;
+  int64_t nfluid = 0;
+  int block = 256;
+  std::vector<int16_t> h_type;  // device order
+  lbm_ibm *ibm = nullptr;       // set by lbm_ibm_setup
+  lbm::BoxArgs box{};           // the IBM box as force mode 3 reads it
+};
+
+#define HIPCHK(x)                              \
+  do {                                         \
+    hipError_t e_ = (x);                       \
+    if (e_ != hipSuccess) return LBM_ERR_HIP;  \
+  } while (0)
+
+// launch parameters of the context's current state (lbm_capi.hip)
+lbm::LaunchCtx launch_ctx(const lbm_ctx *c);
+// work the reference does right after a Streaming call (lbm_capi.hip)
+int after_stream(lbm_ctx *c);
+// the IBM state (lbm_ibm.hip): release; set up at all; IBLBM done and
+// waiting for its collision; mark that collision done
+void ibm_free(lbm_ctx *c);
+int ibm_remark(lbm_ctx *c);
+bool ibm_active(const lbm_ctx *c);
+bool ibm_ready(const lbm_ctx *c);
+void ibm_consume(lbm_ctx *c);
+
+

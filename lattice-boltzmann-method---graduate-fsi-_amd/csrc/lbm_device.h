@@ -39,6 +39,8 @@ constexpr uint8_t M_OTHER = 32;  // any other type (IN, OUT, P, L, T, Q)
 // reference writes its 2nd-order feq there after every Streaming
 // (lbm.cpp:599-611 reads k = 0 and writes k = Nz+1)
 constexpr uint8_t M_BACKEQ = 64;
+// inside the immersed-boundary box (force mode 3 reads the box arrays)
+constexpr uint8_t M_BOX = 128;
 
 // sticky status bits
 constexpr int ST_FEQ_NEG = 1;

@@ -27,6 +27,14 @@ enum StreamMode {
   STREAM_COLLIDE_MACRO = 3  // as STREAM_COLLIDE, and emit rho,u
 };
 
+// IBM box data of force mode 3 (the immersed-boundary step): cells inside
+// [x0, x0+nx) x [y0, y0+ny) x [z0, z0+nz) (mask bit M_BOX) take rho, u and
+// F from these FP64 arrays, indexed (x-x0) + nx*((y-y0) + ny*(z-z0)).
+struct BoxArgs {
+  int32_t x0, y0, z0, nx, ny, nz;
+  const double *rho, *ux, *uy, *uz, *Fx, *Fy, *Fz;
+};
+
 struct LaunchCtx {
   Geo g;
   int storage;     // LBM_STORE_*
@@ -39,6 +47,7 @@ struct LaunchCtx {
   const void *uwx, *uwy, *uwz;
   void *rho, *ux, *uy, *uz;
   const uint8_t *mask;
+  BoxArgs box;        // force mode 3
   uint8_t macro_sel;  // k_macro: cells whose mask has any of these bits
   int *status;
   unsigned long long *first_bad;

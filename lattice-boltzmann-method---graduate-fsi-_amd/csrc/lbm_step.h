@@ -1,0 +1,369 @@
+// lbm_step.h -- device helpers of the D3Q19 step shared by the kernel
+// translation units: kernel arguments, moments, the BGK/Guo relaxation, the
+// pull of one cell and its store.
+#pragma once
+
+#include "lbm_device.h"
+#include "lbm_launch.h"
+
+namespace lbm {
+
+
+// FORCE: 0 none, 1 per-cell field, 2 uniform, 3 immersed-boundary step
+// (u = sum f c / rho without the half force, F = 0, box cells overridden).
+template <class C>
+struct KArgs {
+  Geo g;
+  BoxArgs box;
+  C omega;
+  C Fx, Fy, Fz;                  // uniform force (force mode 2)
+  const C *Fxa, *Fya, *Fza;      // per-cell force (force mode 1)
+  const uint32_t *links;         // bit m: x - c_m is TYPE_S
+  const C *uwx, *uwy, *uwz;      // per-cell wall velocity (lbm.cpp:941-962)
+  C *rho, *ux, *uy, *uz;         // macroscopic arrays
+  int *status;
+  unsigned long long *first_bad;
+};
+
+// Per-direction plane pointers, computed on the host for each launch so
+// that no wave recomputes them: src[m] = fin + m*ps - off_m (bulk pull
+// source), in[m] = fin + m*ps, out[m] = fout + m*ps.
+template <class S>
+struct Planes {
+  const S *src[Q];
+  const S *in[Q];
+  S *out[Q];
+};
+
+template <class C, int FORCE>
+__device__ __forceinline__ void load_force(const KArgs<C> &a, uint32_t idx, C &Fx, C &Fy, C &Fz) {
+  if constexpr (FORCE == 1) {
+    Fx = a.Fxa[idx];
+    Fy = a.Fya[idx];
+    Fz = a.Fza[idx];
+  } else if constexpr (FORCE == 2) {
+    Fx = a.Fx;
+    Fy = a.Fy;
+    Fz = a.Fz;
+  } else {
+    Fx = C(0);
+    Fy = C(0);
+    Fz = C(0);
+  }
+}
+
+__device__ __forceinline__ void flag(int *status, unsigned long long *first_bad, int bits,
+                                     uint32_t idx) {
+  atomicOr(status, bits);
+  atomicMin(first_bad, (unsigned long long)idx);
+}
+
+__device__ __forceinline__ float vmin(float a, float b) { return __builtin_fminf(a, b); }
+__device__ __forceinline__ float vmax(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ double vmin(double a, double b) { return __builtin_fmin(a, b); }
+__device__ __forceinline__ double vmax(double a, double b) { return __builtin_fmax(a, b); }
+
+// u = (j + F/2)/rho: FP64 divides each component (the reference's
+// expression); FP32 multiplies by one correctly rounded reciprocal.
+template <class C>
+__device__ __forceinline__ void velocity(C jx, C jy, C jz, C rho, C &ux, C &uy, C &uz) {
+  if constexpr (sizeof(C) == 8) {
+    ux = jx / rho;
+    uy = jy / rho;
+    uz = jz / rho;
+  } else {
+    const C ir = C(1) / rho;
+    ux = jx * ir;
+    uy = jy * ir;
+    uz = jz * ir;
+  }
+}
+
+// Moments of one cell from raw values (MacroProp, lbm.cpp:668-671), summed
+// pairwise over the +-c direction pairs.
+template <class S>
+__device__ __forceinline__ void moments(const typename Store<S>::C (&f)[Q],
+                                        typename Store<S>::C &rho, typename Store<S>::C &drho,
+                                        typename Store<S>::C &jx, typename Store<S>::C &jy,
+                                        typename Store<S>::C &jz) {
+  using C = typename Store<S>::C;
+  // p_k = f_m + f_opp, d_k = f_m - f_opp for the pairs (1,2), (3,4), ... (17,18)
+  const C p1 = f[1] + f[2], p2 = f[3] + f[4], p3 = f[5] + f[6], p4 = f[7] + f[8];
+  const C p5 = f[9] + f[10], p6 = f[11] + f[12], p7 = f[13] + f[14], p8 = f[15] + f[16];
+  const C p9 = f[17] + f[18];
+  const C d1 = f[1] - f[2], d2 = f[3] - f[4], d3 = f[5] - f[6], d4 = f[7] - f[8];
+  const C d5 = f[9] - f[10], d6 = f[11] - f[12], d7 = f[13] - f[14], d8 = f[15] - f[16];
+  const C d9 = f[17] - f[18];
+  const C s0 = ((f[0] + (p1 + p2)) + ((p3 + p4) + (p5 + p6))) + ((p7 + p8) + p9);
+  // c(7) = (1,1,0), c(9) = (1,0,1), c(11) = (0,1,1), c(13) = (1,-1,0),
+  // c(15) = (1,0,-1), c(17) = (0,1,-1)
+  jx = (d1 + d4) + ((d5 + d7) + d8);
+  jy = (d2 + d4) + ((d6 - d7) + d9);
+  jz = (d3 + d5) + ((d6 - d8) - d9);
+  if constexpr (Store<S>::shifted) {
+    constexpr C iK = C(1) / C(Store<S>::K);
+    drho = s0 * iK;
+    rho = C(1) + drho;
+    jx *= iK;
+    jy *= iK;
+    jz *= iK;
+  } else {
+    rho = s0;
+    drho = s0 - C(1);
+  }
+}
+
+// BGK collision with the Guo source (ConvLBM, lbm.cpp:75-118) of one cell,
+// in place on raw values.  Returns true when some feq < 0 (lbm.cpp:114).
+// Directions come in pairs m, m+1 = opp(m) with c_{m+1} = -c_m, so
+// A(-cu) = s - p and A(cu) = s + p share s = 4.5cu^2 - 1.5uu and
+// p = cu (3 + 4.5cu^2 - 4.5uu).  f(1-omega) + omega feq is evaluated as
+// feq + (1-omega)(f - feq), which rounds the large term once.
+template <class S, int FORCE>
+__device__ __forceinline__ bool relax(typename Store<S>::C (&f)[Q], typename Store<S>::C rho,
+                                      typename Store<S>::C drho, typename Store<S>::C ux,
+                                      typename Store<S>::C uy, typename Store<S>::C uz,
+                                      typename Store<S>::C Fx, typename Store<S>::C Fy,
+                                      typename Store<S>::C Fz, typename Store<S>::C omega) {
+  using C = typename Store<S>::C;
+  constexpr C K = C(Store<S>::K);
+  const C uu = ux * ux + uy * uy + uz * uz;
+  const C om1 = C(1) - omega;
+  const C c3 = C(3) - C(4.5) * uu;  // 3 - 4.5uu
+  const C mh = C(-1.5) * uu;        // -1.5uu
+  // raw equilibrium feq = Kw*(drho + rho*A) (shifted) or w*rho*(1 + A)
+  // (unshifted) = ra*A + da per weight class; relaxation in the
+  // non-equilibrium form f* = feq + (1 - omega)(f - feq)
+  C ra[3], da[3];
+  {
+    constexpr C wc[3] = {C(1.0 / 3), C(1.0 / 18), C(1.0 / 36)};
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      ra[q] = (K * wc[q]) * rho;
+      da[q] = Store<S>::shifted ? (K * wc[q]) * drho : ra[q];
+    }
+  }
+  C fpre = C(0), Fu = C(0);
+  if constexpr (FORCE != 0) {
+    fpre = C(1) - omega * C(0.5);
+    Fu = Fx * ux + Fy * uy + Fz * uz;
+  }
+  C amin = mh, amax = mh;  // A of the rest direction is -1.5uu
+  // rest direction
+  {
+    const C fe = ra[0] * mh + da[0];
+    f[0] = fe + om1 * (f[0] - fe);
+  }
+  if constexpr (FORCE != 0) f[0] += (K * C(1.0 / 3)) * fpre * (C(-3) * Fu);
+  for_dirs([&](auto M) {
+    constexpr int m = decltype(M)::value;
+    if constexpr (m % 2 == 1) {
+      constexpr int q = (m <= 6) ? 1 : 2;
+      const C cu = C(CX[m]) * ux + C(CY[m]) * uy + C(CZ[m]) * uz;
+      const C t2 = cu * cu;
+      const C s_ = C(4.5) * t2 + mh;
+      const C p_ = cu * (C(4.5) * t2 + c3);
+      const C Ap = s_ + p_, Am = s_ - p_;
+      amin = vmin(amin, vmin(Ap, Am));
+      amax = vmax(amax, vmax(Ap, Am));
+      const C fep = ra[q] * Ap + da[q], fem = ra[q] * Am + da[q];
+      C fp = fep + om1 * (f[m] - fep);
+      C fm = fem + om1 * (f[m + 1] - fem);
+      if constexpr (FORCE != 0) {
+        constexpr C Kw = C(Store<S>::K * W[m]);
+        const C Fc = C(CX[m]) * Fx + C(CY[m]) * Fy + C(CZ[m]) * Fz;
+        const C kf = Kw * fpre;
+        // S(c) = Kw fpre (3Fc + 9 Fc cu - 3Fu); S(-c) has Fc, cu negated
+        fp += kf * (Fc * (C(3) + C(9) * cu) - C(3) * Fu);
+        fm += kf * (-Fc * (C(3) - C(9) * cu) - C(3) * Fu);
+      }
+      f[m] = fp;
+      f[m + 1] = fm;
+    }
+  });
+  // feq = rho w (1 + A) < 0 for some direction (lbm.cpp:114-117)
+  return (rho > C(0) && C(1) + amin < C(0)) || (rho < C(0) && C(1) + amax > C(0));
+}
+
+// ---------------------------------------------------------------------------
+// Pull of one fluid cell.  The value arriving at cell x in direction m is,
+// as in the push loop of lbm.cpp:252-292:
+//   x - c_m is TYPE_F      : f*(x - c_m, m)
+//   x - c_m is TYPE_S      : f*(x, opp m) + 2 w_m (u_w . c_m) / cs2
+//   x - c_m any other type : f(x, m) untouched since Init; that value is kept
+//                            in slot m of the source cell, which never
+//                            collides, so the same load reads it.
+// ---------------------------------------------------------------------------
+template <class S>
+__device__ __forceinline__ const S *at(const S *base, uint32_t boff) {
+  return reinterpret_cast<const S *>(reinterpret_cast<const char *>(base) + boff);
+}
+template <class S>
+__device__ __forceinline__ S *at(S *base, uint32_t boff) {
+  return reinterpret_cast<S *>(reinterpret_cast<char *>(base) + boff);
+}
+
+// Addressing: a wave-uniform 64-bit base per direction plus one 32-bit
+// per-lane byte offset, so every load uses the scalar-base form
+// (global_load saddr) with no per-access 64-bit vector arithmetic.
+//
+// Bulk loads f[m] = src[m][idx] are issued for every thread before its mask
+// is known (they overlap the mask load instead of waiting a full memory
+// round-trip for it).  They never leave the population allocation: for any
+// idx < ncell, m*ps + idx - off_m lies in [0, 19*ps) because |off_m| <=
+// plane + px + 1 < ps.  Cells that are not plain bulk fluid then patch the
+// directions whose source differs (patch_cell).
+template <class S>
+__device__ __forceinline__ void bulk_loads(const Planes<S> &P, uint32_t idx,
+                                           typename Store<S>::C (&f)[Q]) {
+  const uint32_t boff = idx * (uint32_t)sizeof(S);
+  for_dirs([&](auto M) {
+    constexpr int m = decltype(M)::value;
+    f[m] = Store<S>::ld(at(P.src[m], boff));
+  });
+}
+
+// An empty asm statement that takes the bulk-loaded values as inputs and
+// passes the mask byte through: the mask test then depends on every load
+// having completed, so neither the loads nor the test can be moved across
+// each other (the compiler would otherwise sink the loads into the fluid
+// branch, or hoist the test above them, and serialise two memory
+// round-trips per wave).  It emits no instruction.
+template <class C>
+__device__ __forceinline__ uint32_t after_loads(uint32_t mk, const C (&f)[Q]) {
+  asm volatile(""
+               : "+v"(mk)
+               : "v"(f[0]), "v"(f[1]), "v"(f[2]), "v"(f[3]), "v"(f[4]), "v"(f[5]), "v"(f[6]),
+                 "v"(f[7]), "v"(f[8]), "v"(f[9]), "v"(f[10]), "v"(f[11]), "v"(f[12]),
+                 "v"(f[13]), "v"(f[14]), "v"(f[15]), "v"(f[16]), "v"(f[17]), "v"(f[18]));
+  return mk;
+}
+
+template <class S>
+__device__ __forceinline__ void patch_cell(const Planes<S> &P, uint8_t mk, uint32_t idx,
+                                           const KArgs<typename Store<S>::C> &a,
+                                           typename Store<S>::C (&f)[Q]) {
+  using C = typename Store<S>::C;
+  const uint32_t boff = idx * (uint32_t)sizeof(S);
+  if (mk & M_WRAP) {
+    // periodic wrap of the padded lattice (lbm.cpp:197-199)
+    uint32_t x, y, z;
+    coords(a.g, idx, x, y, z);
+    for_dirs([&](auto M) {
+      constexpr int m = decltype(M)::value;
+      const uint32_t src = wrap_src(a.g, x, y, z, CX[m], CY[m], CZ[m]);
+      if (src != idx - pull_off<m>(a.g)) f[m] = Store<S>::ld(at(P.in[m], src * (uint32_t)sizeof(S)));
+    });
+  }
+  if (mk & M_WALL) {
+    const uint32_t lk = a.links[idx];
+    C uwx = C(0), uwy = C(0), uwz = C(0);
+    if (mk & M_MOVING) {
+      uwx = a.uwx[idx];
+      uwy = a.uwy[idx];
+      uwz = a.uwz[idx];
+    }
+    for_dirs([&](auto M) {
+      constexpr int m = decltype(M)::value;
+      if (lk & (1u << m)) {
+        const C ucw = C(CX[m]) * uwx + C(CY[m]) * uwy + C(CZ[m]) * uwz;
+        f[m] = Store<S>::ld(at(P.in[OPP[m]], boff)) + C(Store<S>::K * 6.0 * W[m]) * ucw;
+      }
+    });
+  }
+  if (mk & M_BACKEQ) {
+    // after the push, the back-face inflow loop overwrites this cell with
+    // feq2 of the TYPE_IN cell (i,j,0) (lbm.cpp:599-611)
+    const uint32_t s = idx - (a.g.nz - 1) * a.g.plane;
+    const C r = a.rho[s], vx = a.ux[s], vy = a.uy[s], vz = a.uz[s];
+    const C uu = vx * vx + vy * vy + vz * vz;
+    for_dirs([&](auto M) {
+      constexpr int m = decltype(M)::value;
+      const C cu = C(CX[m]) * vx + C(CY[m]) * vy + C(CZ[m]) * vz;
+      f[m] = feq2_raw<C, Store<S>::shifted>(r, cu, uu, C(W[m]), C(Store<S>::K));
+    });
+  }
+}
+
+template <class S>
+__device__ __forceinline__ void pull_cell(const Planes<S> &P, uint8_t mk, uint32_t idx,
+                                          const KArgs<typename Store<S>::C> &a,
+                                          typename Store<S>::C (&f)[Q]) {
+  bulk_loads<S>(P, idx, f);
+  if (mk & (M_WALL | M_WRAP | M_BACKEQ)) patch_cell<S>(P, mk, idx, a, f);
+}
+
+// Box index of a cell inside the IBM box (mask bit M_BOX).
+__device__ __forceinline__ uint32_t box_index(const Geo &g, const BoxArgs &b, uint32_t idx) {
+  uint32_t x, y, z;
+  coords(g, idx, x, y, z);
+  return (uint32_t)((int)x - b.x0) + (uint32_t)b.nx * ((uint32_t)((int)y - b.y0) + (uint32_t)b.ny * (uint32_t)((int)z - b.z0));
+}
+
+// rho, u and F a cell collides with.  Modes 0-2: MacroProp's velocity
+// u = (sum f c + F/2)/rho (lbm.cpp:672-675).  Mode 3, the immersed-boundary
+// step: the unforced velocity sum f c / rho and F = 0 of lbm.cpp:1262-1285
+// (MacroProp is not called in IBM runs, main.cpp:197-211), replaced inside
+// the box by the IBM result (lbm.cpp:1347-1352).
+template <class S, int FORCE>
+__device__ __forceinline__ void cell_state(const typename Store<S>::C (&f)[Q], uint8_t mk,
+                                           uint32_t idx, const KArgs<typename Store<S>::C> &a,
+                                           typename Store<S>::C &rho, typename Store<S>::C &drho,
+                                           typename Store<S>::C &ux, typename Store<S>::C &uy,
+                                           typename Store<S>::C &uz, typename Store<S>::C &Fx,
+                                           typename Store<S>::C &Fy, typename Store<S>::C &Fz) {
+  using C = typename Store<S>::C;
+  C jx, jy, jz;
+  moments<S>(f, rho, drho, jx, jy, jz);
+  if constexpr (FORCE == 3) {
+    if (mk & M_BOX) {
+      const uint32_t b = box_index(a.g, a.box, idx);
+      rho = (C)a.box.rho[b];
+      drho = rho - C(1);
+      ux = (C)a.box.ux[b];
+      uy = (C)a.box.uy[b];
+      uz = (C)a.box.uz[b];
+      Fx = (C)a.box.Fx[b];
+      Fy = (C)a.box.Fy[b];
+      Fz = (C)a.box.Fz[b];
+    } else {
+      velocity(jx, jy, jz, rho, ux, uy, uz);
+      Fx = Fy = Fz = C(0);
+    }
+  } else {
+    load_force<C, FORCE>(a, idx, Fx, Fy, Fz);
+    if constexpr (FORCE != 0) velocity(jx + C(0.5) * Fx, jy + C(0.5) * Fy, jz + C(0.5) * Fz, rho, ux, uy, uz);
+    else velocity(jx, jy, jz, rho, ux, uy, uz);
+  }
+}
+
+// Moments, optional macro emission, optional collision, store of one cell.
+template <class S, int FORCE, bool COLLIDE, bool EMIT>
+__device__ __forceinline__ void finish_cell(typename Store<S>::C (&f)[Q], uint8_t mk, uint32_t idx,
+                                            const KArgs<typename Store<S>::C> &a,
+                                            const Planes<S> &P) {
+  using C = typename Store<S>::C;
+  if constexpr (COLLIDE || EMIT) {
+    C rho, drho, ux, uy, uz, Fx, Fy, Fz;
+    cell_state<S, FORCE>(f, mk, idx, a, rho, drho, ux, uy, uz, Fx, Fy, Fz);
+    int bad = (rho < C(0) || rho > C(100)) ? ST_RHO_RANGE : 0;
+    if constexpr (FORCE == 3) bad = 0;  // no MacroProp (and no density check) in IBM runs
+    if constexpr (EMIT) {
+      a.rho[idx] = rho;
+      a.ux[idx] = ux;
+      a.uy[idx] = uy;
+      a.uz[idx] = uz;
+    }
+    if constexpr (COLLIDE) {
+      if (relax<S, FORCE>(f, rho, drho, ux, uy, uz, Fx, Fy, Fz, a.omega)) bad |= ST_FEQ_NEG;
+    }
+    if (bad) flag(a.status, a.first_bad, bad, idx);
+  }
+  const uint32_t boff = idx * (uint32_t)sizeof(S);
+  for_dirs([&](auto M) {
+    constexpr int m = decltype(M)::value;
+    Store<S>::st(at(P.out[m], boff), f[m]);
+  });
+}
+
+}  // namespace lbm

@@ -55,6 +55,17 @@ class _Desc(C.Structure):
     ]
 
 
+class _IbmDesc(C.Structure):
+    _fields_ = [
+        ("nmarkers", C.c_int32),
+        ("m_max", C.c_int32),
+        ("box", C.c_int32 * 6),
+        ("moving", C.c_int32),
+        ("reserved", C.c_int32),
+        ("vsolid", C.c_double),
+    ]
+
+
 # name -> (restype, argtypes) for every entry point of include/lbm_hip.h
 _P = C.c_void_p
 _D = C.POINTER(C.c_double)
@@ -86,6 +97,10 @@ _SIGNATURES = {
     "lbm_get_launch": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "lbm_error_string": (C.c_char_p, [C.c_int]),
     "lbm_copy_probe": (C.c_int, [C.c_int32, C.c_int64, C.c_int32, _D]),
+    "lbm_ibm_setup": (C.c_int, [_P, C.POINTER(_IbmDesc), _D, _D, _D, _D, _D, _D, _D]),
+    "lbm_ibm_apply": (C.c_int, [_P, _D, _D, _D]),
+    "lbm_ibm_run": (C.c_int, [_P, C.c_int32, _D, _D]),
+    "lbm_ibm_markers": (C.c_int, [_P, _D, _D, _D, _D, _D]),
 }
 
 _lib = None
@@ -247,6 +262,41 @@ class Lattice:
         out = np.zeros(3, np.float64)
         _check(self._lib.lbm_reduce_fp64(self._h, int(what), _dp(out)), "lbm_reduce_fp64")
         return out[:3] if what == REDUCE_MOMENTUM else float(out[0])
+
+    # -- immersed boundary (IBLBM, lbm.cpp:1135-1417) ---------------------------
+    def ibm_setup(self, markers: dict, box, m_max: int = 20, vsolid: float = 0.0, moving=False):
+        """markers: dict of float64 arrays x, y, z, nx, ny, nz, ds; box:
+        (xmin, xmax, ymin, ymax, zmin, zmax) inclusive lattice indices."""
+        n = int(markers["x"].size)
+        d = _IbmDesc(n, int(m_max), (C.c_int32 * 6)(*[int(b) for b in box]), int(bool(moving)), 0,
+                     float(vsolid))
+        arrs = [np.ascontiguousarray(markers[k], dtype=np.float64) for k in
+                ("x", "y", "z", "nx", "ny", "nz", "ds")]
+        self._ibm_n = n
+        _check(self._lib.lbm_ibm_setup(self._h, C.byref(d), *[_dp(a) for a in arrs]), "lbm_ibm_setup")
+
+    def ibm_apply(self, vcm=(0.0, 0.0, 0.0), vcm_before=(0.0, 0.0, 0.0)):
+        out = np.zeros(3)
+        v = np.asarray(vcm, np.float64)
+        b = np.asarray(vcm_before, np.float64)
+        _check(self._lib.lbm_ibm_apply(self._h, _dp(v), _dp(b), _dp(out)), "lbm_ibm_apply")
+        return out
+
+    def ibm_run(self, nsteps: int, vcm=None):
+        """vcm: None (body at rest) or array (nsteps+1, 3).  Returns (nsteps, 3):
+        Drag, Lift, interior density per step."""
+        out = np.zeros((int(nsteps), 3))
+        v = None if vcm is None else np.ascontiguousarray(vcm, np.float64)
+        _check(self._lib.lbm_ibm_run(self._h, int(nsteps), _dp(v), _dp(out)), "lbm_ibm_run")
+        return out
+
+    def ibm_markers(self):
+        n = self._ibm_n
+        x, y, z = np.empty(n), np.empty(n), np.empty(n)
+        ub, Fb = np.empty(3 * n), np.empty(3 * n)
+        _check(self._lib.lbm_ibm_markers(self._h, _dp(x), _dp(y), _dp(z), _dp(ub), _dp(Fb)),
+               "lbm_ibm_markers")
+        return dict(x=x, y=y, z=z, ub=ub, Fb=Fb)
 
     def set_launch(self, block: int, cells_per_thread: int = 1):
         _check(self._lib.lbm_set_launch(self._h, int(block), int(cells_per_thread)), "lbm_set_launch")
