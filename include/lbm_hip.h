@@ -126,13 +126,16 @@ typedef struct lbm_ibm_desc {
   int32_t nmarkers; /* Lagrangian markers (number_nodes, LBM.h:340)           */
   int32_t m_max;    /* implicit iterations (LBM.h:352)                       */
   int32_t box[6];   /* xmin,xmax,ymin,ymax,zmin,zmax, inclusive lattice      *
-                     * indices; the reference's box is floor/ceil(centre-+D) *
-                     * with z = 1..1 (lbm.cpp:1165-1167)                     */
+                     * indices (with follow: only zmin, zmax are read)       */
   int32_t moving;   /* recompute nearest points, interior cells and the    *
                      * spreading tiles every step (they are recomputed      *
                      * anyway in every step whose vcm moves the markers)    */
-  int32_t reserved;
+  int32_t follow;   /* 1: the box is the reference's [floor(c - half),      *
+                     * ceil(c + half)] in x and y around the body centre c, *
+                     * which moves with the markers (lbm.cpp:1162-1166)     */
   double vsolid;    /* added-mass volume (Vsolid, LBM.h:355)                 */
+  double center[2]; /* body centre (center_x, center_y of LBM.h) at set-up  */
+  double half;      /* box half-width (D in lbm.cpp:1165)                   */
 } lbm_ibm_desc;
 
 /* Markers of the body (Generate_Cylinder_Marker, lbm.cpp:1857-1877), their
@@ -156,6 +159,10 @@ int lbm_ibm_run(lbm_ctx *ctx, int32_t nsteps, const double *vcm, double *out);
 /* marker positions (nmarkers each), velocities and forces (3 x nmarkers) */
 int lbm_ibm_markers(lbm_ctx *ctx, double *x, double *y, double *z, double *ub,
                     double *Fb);
+
+/* The current box and, with follow, the body centre (center_x, center_y
+ * after the last IBLBM). */
+int lbm_ibm_box(lbm_ctx *ctx, int32_t box[6], double center[2]);
 
 /* sizes and identification */
 int64_t lbm_ncells(const lbm_ctx *ctx);

@@ -487,21 +487,24 @@ inline dim3 blocks(int n, int bs = 256) { return dim3((unsigned)((n + bs - 1) / 
 
 // ---------------------------------------------------------------------------
 struct lbm_ibm {
-  int nm = 0, m_max = 0, moving = 0;
+  int nm = 0, m_max = 0, moving = 0, follow = 0;
   double vsolid = 0;
+  double cx = 0, cy = 0, half = 0;  // body centre and box half-width (follow)
   Box B{};
+  bool placed = false;
   double *mx = nullptr, *my = nullptr, *mz = nullptr, *nx = nullptr, *ny = nullptr, *nz = nullptr;
   double *ds = nullptr, *ub = nullptr, *Fb = nullptr, *dl = nullptr;
   int32_t *near = nullptr;
+  // box arrays (capacity box_cap cells) and tile arrays (tile_cap tiles)
   double *brho = nullptr, *bux = nullptr, *buy = nullptr, *buz = nullptr;
   double *bFx = nullptr, *bFy = nullptr, *bFz = nullptr;
   uint8_t *live = nullptr, *inside = nullptr;
   int32_t *tcount = nullptr, *tstart = nullptr, *tcursor = nullptr, *tlist = nullptr;
-  int tlist_cap = 0;
+  size_t box_cap = 0, tile_cap = 0, tlist_cap = 0;
   double *out = nullptr;  // 3 per step of a run
   int out_cap = 0;
   int ready = 0;          // IBLBM done, the next collision uses force mode 3
-  std::vector<void *> allocs;
+  std::vector<void *> allocs;  // marker arrays, freed with the state
 };
 
 static void *dalloc(lbm_ibm *m, size_t bytes) {
@@ -511,11 +514,33 @@ static void *dalloc(lbm_ibm *m, size_t bytes) {
   return p;
 }
 
+template <class T>
+static void dfree(T *&p) {
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+
+static void free_box_arrays(lbm_ibm *m) {
+  dfree(m->brho); dfree(m->bux); dfree(m->buy); dfree(m->buz);
+  dfree(m->bFx); dfree(m->bFy); dfree(m->bFz);
+  dfree(m->live); dfree(m->inside);
+  m->box_cap = 0;
+}
+
+static void free_tile_arrays(lbm_ibm *m) {
+  dfree(m->tcount); dfree(m->tcursor); dfree(m->tstart);
+  m->tile_cap = 0;
+}
+
 void ibm_free(lbm_ctx *c) {
   if (!c || !c->ibm) return;
-  for (void *p : c->ibm->allocs) (void)hipFree(p);
-  if (c->ibm->out) (void)hipFree(c->ibm->out);
-  delete c->ibm;
+  lbm_ibm *m = c->ibm;
+  for (void *p : m->allocs) (void)hipFree(p);
+  free_box_arrays(m);
+  free_tile_arrays(m);
+  dfree(m->tlist);
+  dfree(m->out);
+  delete m;
   c->ibm = nullptr;
 }
 
@@ -576,6 +601,77 @@ static hipError_t box_moments(lbm_ctx *c, bool pull) {
   }
 }
 
+// The box of the reference's IBLBM around a centre: [floor(c - half),
+// ceil(c + half)] in x and y (lbm.cpp:1165-1166).
+static void follow_box(const lbm_ibm *m, int box[6]) {
+  box[0] = (int)floor(m->cx - m->half);
+  box[1] = (int)ceil(m->cx + m->half);
+  box[2] = (int)floor(m->cy - m->half);
+  box[3] = (int)ceil(m->cy + m->half);
+  box[4] = m->B.z0;
+  box[5] = m->B.z1;
+}
+
+// Checks a box against the lattice: inside it, and only TYPE_F or TYPE_S
+// cells (inlet/outlet populations are not evolved on the device).
+static int check_box(const lbm_ctx *c, const int b[6]) {
+  if (b[0] < 0 || b[2] < 0 || b[4] < 0 || b[1] >= (int)c->g.nx || b[3] >= (int)c->g.ny ||
+      b[5] >= (int)c->g.nz || b[0] > b[1] || b[2] > b[3] || b[4] > b[5])
+    return LBM_ERR_ARG;
+  for (int k = b[4]; k <= b[5]; k++)
+    for (int j = b[2]; j <= b[3]; j++)
+      for (int i = b[0]; i <= b[1]; i++) {
+        const int16_t t = c->h_type[((size_t)k * c->g.ny + j) * c->g.px + i];
+        if (t != LBM_TYPE_F && t != LBM_TYPE_S) return LBM_ERR_UNSUPPORTED;
+      }
+  return LBM_OK;
+}
+
+// Puts the IBM box at b: clears the old box's mask bits, grows the box and
+// tile arrays when needed, marks the new box and points the collision's
+// box arguments at it.
+static int place_box(lbm_ctx *c, const int b[6]) {
+  lbm_ibm *m = c->ibm;
+  int e = check_box(c, b);
+  if (e != LBM_OK) return e;
+  if (m->placed) {
+    hipLaunchKernelGGL(k_ibm_mark, blocks(m->B.ncell()), dim3(256), 0, c->stream, c->mask, c->g, m->B, m->live, 0);
+    HIPCHK(hipGetLastError());
+    m->placed = false;
+  }
+  Box &B = m->B;
+  B.x0 = b[0]; B.x1 = b[1]; B.y0 = b[2]; B.y1 = b[3]; B.z0 = b[4]; B.z1 = b[5];
+  B.nx = b[1] - b[0] + 1; B.ny = b[3] - b[2] + 1; B.nz = b[5] - b[4] + 1;
+  B.tnx = (B.nx + TX - 1) / TX; B.tny = (B.ny + TY - 1) / TY; B.tnz = (B.nz + TZ - 1) / TZ;
+  const size_t nb = (size_t)B.ncell(), nt = (size_t)B.ntile();
+  if (nb > m->box_cap) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    free_box_arrays(m);
+    double **bp[7] = {&m->brho, &m->bux, &m->buy, &m->buz, &m->bFx, &m->bFy, &m->bFz};
+    for (int q = 0; q < 7; q++)
+      if (hipMalloc((void **)bp[q], 8 * nb) != hipSuccess) return LBM_ERR_NOMEM;
+    if (hipMalloc((void **)&m->live, nb) != hipSuccess) return LBM_ERR_NOMEM;
+    if (hipMalloc((void **)&m->inside, nb) != hipSuccess) return LBM_ERR_NOMEM;
+    for (int q = 0; q < 7; q++) HIPCHK(hipMemsetAsync(*bp[q], 0, 8 * nb, c->stream));
+    m->box_cap = nb;
+  }
+  if (nt > m->tile_cap) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    free_tile_arrays(m);
+    if (hipMalloc((void **)&m->tcount, 4 * nt) != hipSuccess ||
+        hipMalloc((void **)&m->tcursor, 4 * nt) != hipSuccess ||
+        hipMalloc((void **)&m->tstart, 4 * (nt + 1)) != hipSuccess)
+      return LBM_ERR_NOMEM;
+    m->tile_cap = nt;
+  }
+  hipLaunchKernelGGL(k_ibm_mark, blocks((int)nb), dim3(256), 0, c->stream, c->mask, c->g, B, m->live, 1);
+  HIPCHK(hipGetLastError());
+  m->placed = true;
+  // the collision of IBM steps reads the box (force mode 3)
+  c->box = BoxArgs{B.x0, B.y0, B.z0, B.nx, B.ny, B.nz, m->brho, m->bux, m->buy, m->buz, m->bFx, m->bFy, m->bFz};
+  return LBM_OK;
+}
+
 // nearest points, interior cells, spreading tiles for the current markers
 static int ibm_geometry(lbm_ctx *c) {
   lbm_ibm *m = c->ibm;
@@ -592,16 +688,16 @@ static int ibm_geometry(lbm_ctx *c) {
   std::vector<int32_t> cnt(nt);
   HIPCHK(hipMemcpyAsync(cnt.data(), m->tcount, sizeof(int32_t) * nt, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  long total = 0;
+  size_t total = 0;
   for (int t = 0; t < nt; t++) {
     if (cnt[t] > TILE_MAX) return LBM_ERR_UNSUPPORTED;
-    total += cnt[t];
+    total += (size_t)cnt[t];
   }
   if (total > m->tlist_cap) {
-    void *p = dalloc(m, sizeof(int32_t) * (size_t)std::max<long>(total, 1));
-    if (!p) return LBM_ERR_NOMEM;
-    m->tlist = (int32_t *)p;
-    m->tlist_cap = (int)std::max<long>(total, 1);
+    dfree(m->tlist);
+    const size_t cap = std::max<size_t>(total, 1);
+    if (hipMalloc((void **)&m->tlist, sizeof(int32_t) * cap) != hipSuccess) return LBM_ERR_NOMEM;
+    m->tlist_cap = cap;
   }
   hipLaunchKernelGGL(k_ibm_tile_fill, blocks(m->nm), dim3(256), 0, c->stream, B, m->nm, m->mx, m->my, m->mz, m->tstart, m->tcursor, m->tlist);
   hipLaunchKernelGGL(k_ibm_tile_sort, dim3(nt), dim3(256), 0, c->stream, m->tstart, m->tlist);
@@ -629,53 +725,63 @@ static int ibm_iterate(lbm_ctx *c, const double vcm[3], const double vb[3], doub
   return LBM_OK;
 }
 
+// Marker motion of one IBLBM call (lbm.cpp:1154-1166): the trapezoid rule
+// for the markers and, when the box follows the body, for its centre; then
+// the box and the geometry that depends on the marker positions.
 static int ibm_move(lbm_ctx *c, const double vcm[3], const double vb[3]) {
   lbm_ibm *m = c->ibm;
   hipLaunchKernelGGL(k_ibm_move, blocks(m->nm), dim3(256), 0, c->stream, m->nm, m->mx, m->my, m->mz, vcm[0], vcm[1], vcm[2], vb[0], vb[1], vb[2]);
   HIPCHK(hipGetLastError());
-  // the nearest points, interior cells and tiles follow a body that moves
   const bool moved = vcm[0] + vb[0] != 0 || vcm[1] + vb[1] != 0 || vcm[2] + vb[2] != 0;
+  if (m->follow && moved) {
+    const double dt = 1.0;
+    m->cx += (vcm[0] + vb[0]) * dt / 2;
+    m->cy += (vcm[1] + vb[1]) * dt / 2;
+    int b[6];
+    follow_box(m, b);
+    const Box &B = m->B;
+    if (b[0] != B.x0 || b[1] != B.x1 || b[2] != B.y0 || b[3] != B.y1) {
+      int e = place_box(c, b);
+      if (e != LBM_OK) return e;
+    }
+  }
+  // the nearest points, interior cells and tiles follow a body that moves
   if (m->moving || moved) return ibm_geometry(c);
   return LBM_OK;
 }
 
-extern "C" {
-
-int lbm_ibm_setup(lbm_ctx *c, const lbm_ibm_desc *d, const double *x, const double *y,
-                  const double *z, const double *nx, const double *ny, const double *nz,
-                  const double *ds) {
-  if (!c || !d || !x || !y || !z || !nx || !ny || !nz || !ds) return LBM_ERR_ARG;
-  if (c->state == ST_NONE) return LBM_ERR_STATE;
-  if (d->nmarkers < 1 || d->m_max < 0) return LBM_ERR_ARG;
-  const int x0 = d->box[0], x1 = d->box[1], y0 = d->box[2], y1 = d->box[3], z0 = d->box[4], z1 = d->box[5];
-  if (x0 < 0 || y0 < 0 || z0 < 0 || x1 >= (int)c->g.nx || y1 >= (int)c->g.ny || z1 >= (int)c->g.nz ||
-      x0 > x1 || y0 > y1 || z0 > z1)
-    return LBM_ERR_ARG;
-  // inlet/outlet cells inside the box would need the face-loop populations
-  for (int k = z0; k <= z1; k++)
-    for (int j = y0; j <= y1; j++)
-      for (int i = x0; i <= x1; i++) {
-        const int16_t t = c->h_type[((size_t)k * c->g.ny + j) * c->g.px + i];
-        if (t != LBM_TYPE_F && t != LBM_TYPE_S) return LBM_ERR_UNSUPPORTED;
-      }
+static int ibm_setup(lbm_ctx *c, const lbm_ibm_desc *d, const double *x, const double *y,
+                     const double *z, const double *nx, const double *ny, const double *nz,
+                     const double *ds) {
   HIPCHK(hipSetDevice(c->dev));
   HIPCHK(hipStreamSynchronize(c->stream));
   if (c->ibm) {
-    hipLaunchKernelGGL(k_ibm_mark, blocks(c->ibm->B.ncell()), dim3(256), 0, c->stream, c->mask, c->g, c->ibm->B, c->ibm->live, 0);
-    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->ibm->placed) {
+      hipLaunchKernelGGL(k_ibm_mark, blocks(c->ibm->B.ncell()), dim3(256), 0, c->stream, c->mask, c->g, c->ibm->B, c->ibm->live, 0);
+      HIPCHK(hipStreamSynchronize(c->stream));
+    }
     ibm_free(c);
+    c->box = BoxArgs{};
   }
   lbm_ibm *m = new lbm_ibm();
   c->ibm = m;
   m->nm = d->nmarkers;
   m->m_max = d->m_max;
   m->moving = d->moving;
+  m->follow = d->follow;
   m->vsolid = d->vsolid;
-  Box &B = m->B;
-  B.x0 = x0; B.x1 = x1; B.y0 = y0; B.y1 = y1; B.z0 = z0; B.z1 = z1;
-  B.nx = x1 - x0 + 1; B.ny = y1 - y0 + 1; B.nz = z1 - z0 + 1;
-  B.tnx = (B.nx + TX - 1) / TX; B.tny = (B.ny + TY - 1) / TY; B.tnz = (B.nz + TZ - 1) / TZ;
-  const size_t nm = (size_t)m->nm, nb = (size_t)B.ncell(), nt = (size_t)B.ntile();
+  m->cx = d->center[0];
+  m->cy = d->center[1];
+  m->half = d->half;
+  int b[6] = {d->box[0], d->box[1], d->box[2], d->box[3], d->box[4], d->box[5]};
+  if (m->follow) {
+    m->B.z0 = b[4];
+    m->B.z1 = b[5];
+    follow_box(m, b);
+  }
+  int e = check_box(c, b);
+  if (e != LBM_OK) return e;
+  const size_t nm = (size_t)m->nm;
   double **mp[7] = {&m->mx, &m->my, &m->mz, &m->nx, &m->ny, &m->nz, &m->ds};
   const double *src[7] = {x, y, z, nx, ny, nz, ds};
   for (int q = 0; q < 7; q++) {
@@ -687,27 +793,38 @@ int lbm_ibm_setup(lbm_ctx *c, const lbm_ibm_desc *d, const double *x, const doub
   m->Fb = (double *)dalloc(m, 24 * nm);
   m->dl = (double *)dalloc(m, 16 * nm * std::max(1, m->m_max));
   m->near = (int32_t *)dalloc(m, 4 * nm);
-  double **bp[7] = {&m->brho, &m->bux, &m->buy, &m->buz, &m->bFx, &m->bFy, &m->bFz};
-  for (int q = 0; q < 7; q++) *bp[q] = (double *)dalloc(m, 8 * nb);
-  m->live = (uint8_t *)dalloc(m, nb);
-  m->inside = (uint8_t *)dalloc(m, nb);
-  m->tcount = (int32_t *)dalloc(m, 4 * nt);
-  m->tcursor = (int32_t *)dalloc(m, 4 * nt);
-  m->tstart = (int32_t *)dalloc(m, 4 * (nt + 1));
-  if (!m->ub || !m->Fb || !m->dl || !m->near || !m->brho || !m->bFz || !m->live || !m->inside ||
-      !m->tcount || !m->tcursor || !m->tstart)
-    return LBM_ERR_NOMEM;
+  if (!m->ub || !m->Fb || !m->dl || !m->near) return LBM_ERR_NOMEM;
   HIPCHK(hipMemsetAsync(m->ub, 0, 24 * nm, c->stream));
   HIPCHK(hipMemsetAsync(m->Fb, 0, 24 * nm, c->stream));
-  for (int q = 0; q < 7; q++) HIPCHK(hipMemsetAsync(*bp[q], 0, 8 * nb, c->stream));
-  hipLaunchKernelGGL(k_ibm_mark, blocks((int)nb), dim3(256), 0, c->stream, c->mask, c->g, B, m->live, 1);
-  HIPCHK(hipGetLastError());
-  int e = ibm_geometry(c);
+  e = place_box(c, b);
   if (e != LBM_OK) return e;
-  // the collision of IBM steps reads the box (force mode 3)
-  c->box = BoxArgs{B.x0, B.y0, B.z0, B.nx, B.ny, B.nz, m->brho, m->bux, m->buy, m->buz, m->bFx, m->bFy, m->bFz};
+  e = ibm_geometry(c);
+  if (e != LBM_OK) return e;
   HIPCHK(hipStreamSynchronize(c->stream));
   return LBM_OK;
+}
+
+extern "C" {
+
+int lbm_ibm_setup(lbm_ctx *c, const lbm_ibm_desc *d, const double *x, const double *y,
+                  const double *z, const double *nx, const double *ny, const double *nz,
+                  const double *ds) {
+  if (!c || !d || !x || !y || !z || !nx || !ny || !nz || !ds) return LBM_ERR_ARG;
+  if (c->state == ST_NONE) return LBM_ERR_STATE;
+  if (d->nmarkers < 1 || d->m_max < 0) return LBM_ERR_ARG;
+  if (d->follow && !(d->half >= 0)) return LBM_ERR_ARG;
+  const int e = ibm_setup(c, d, x, y, z, nx, ny, nz, ds);
+  if (e != LBM_OK && c->ibm) {
+    // leave no half-built body behind: clear its marks, drop the state
+    if (c->ibm->placed) {
+      const Box &B = c->ibm->B;
+      hipLaunchKernelGGL(k_ibm_mark, blocks(B.ncell()), dim3(256), 0, c->stream, c->mask, c->g, B, c->ibm->live, 0);
+      (void)hipStreamSynchronize(c->stream);
+    }
+    ibm_free(c);
+    c->box = BoxArgs{};
+  }
+  return e;
 }
 
 // IBLBM (lbm.cpp:1135-1417) on the lattice as it stands after Init or
@@ -719,8 +836,7 @@ int lbm_ibm_apply(lbm_ctx *c, const double vcm[3], const double vcm_before[3], d
   HIPCHK(hipSetDevice(c->dev));
   lbm_ibm *m = c->ibm;
   if (m->out_cap < 1) {
-    if (m->out) (void)hipFree(m->out);
-    m->out = nullptr;
+    dfree(m->out);
     HIPCHK(hipMalloc((void **)&m->out, 3 * sizeof(double)));
     m->out_cap = 1;
   }
@@ -749,8 +865,8 @@ int lbm_ibm_run(lbm_ctx *c, int32_t nsteps, const double *vcm, double *out) {
   HIPCHK(hipSetDevice(c->dev));
   lbm_ibm *m = c->ibm;
   if (m->out_cap < nsteps) {
-    if (m->out) (void)hipFree(m->out);
-    m->out = nullptr;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    dfree(m->out);
     HIPCHK(hipMalloc((void **)&m->out, 3 * sizeof(double) * nsteps));
     m->out_cap = nsteps;
   }
@@ -810,11 +926,25 @@ int lbm_ibm_markers(lbm_ctx *c, double *x, double *y, double *z, double *ub, dou
   return LBM_OK;
 }
 
+int lbm_ibm_box(lbm_ctx *c, int32_t box[6], double center[2]) {
+  if (!c) return LBM_ERR_ARG;
+  if (!c->ibm) return LBM_ERR_STATE;
+  const Box &B = c->ibm->B;
+  if (box) {
+    box[0] = B.x0; box[1] = B.x1; box[2] = B.y0; box[3] = B.y1; box[4] = B.z0; box[5] = B.z1;
+  }
+  if (center) {
+    center[0] = c->ibm->cx;
+    center[1] = c->ibm->cy;
+  }
+  return LBM_OK;
+}
+
 }  // extern "C"
 
 // lbm_init rebuilds the kernel mask: mark the box again
 int ibm_remark(lbm_ctx *c) {
-  if (!c || !c->ibm) return LBM_OK;
+  if (!c || !c->ibm || !c->ibm->placed) return LBM_OK;
   const Box &B = c->ibm->B;
   hipLaunchKernelGGL(k_ibm_mark, blocks(B.ncell()), dim3(256), 0, c->stream, c->mask, c->g, B, c->ibm->live, 1);
   HIPCHK(hipGetLastError());

@@ -61,8 +61,10 @@ class _IbmDesc(C.Structure):
         ("m_max", C.c_int32),
         ("box", C.c_int32 * 6),
         ("moving", C.c_int32),
-        ("reserved", C.c_int32),
+        ("follow", C.c_int32),
         ("vsolid", C.c_double),
+        ("center", C.c_double * 2),
+        ("half", C.c_double),
     ]
 
 
@@ -101,6 +103,7 @@ _SIGNATURES = {
     "lbm_ibm_apply": (C.c_int, [_P, _D, _D, _D]),
     "lbm_ibm_run": (C.c_int, [_P, C.c_int32, _D, _D]),
     "lbm_ibm_markers": (C.c_int, [_P, _D, _D, _D, _D, _D]),
+    "lbm_ibm_box": (C.c_int, [_P, C.POINTER(C.c_int32), _D]),
 }
 
 _lib = None
@@ -264,16 +267,27 @@ class Lattice:
         return out[:3] if what == REDUCE_MOMENTUM else float(out[0])
 
     # -- immersed boundary (IBLBM, lbm.cpp:1135-1417) ---------------------------
-    def ibm_setup(self, markers: dict, box, m_max: int = 20, vsolid: float = 0.0, moving=False):
+    def ibm_setup(self, markers: dict, box, m_max: int = 20, vsolid: float = 0.0, moving=False,
+                  follow=None):
         """markers: dict of float64 arrays x, y, z, nx, ny, nz, ds; box:
-        (xmin, xmax, ymin, ymax, zmin, zmax) inclusive lattice indices."""
+        (xmin, xmax, ymin, ymax, zmin, zmax) inclusive lattice indices.
+        follow: None, or (center_x, center_y, half) for a box that follows
+        the body as the reference's does (then only zmin, zmax of box are
+        used)."""
         n = int(markers["x"].size)
-        d = _IbmDesc(n, int(m_max), (C.c_int32 * 6)(*[int(b) for b in box]), int(bool(moving)), 0,
-                     float(vsolid))
+        fc = (0.0, 0.0, 0.0) if follow is None else tuple(float(v) for v in follow)
+        d = _IbmDesc(n, int(m_max), (C.c_int32 * 6)(*[int(b) for b in box]), int(bool(moving)),
+                     int(follow is not None), float(vsolid), (C.c_double * 2)(fc[0], fc[1]), fc[2])
         arrs = [np.ascontiguousarray(markers[k], dtype=np.float64) for k in
                 ("x", "y", "z", "nx", "ny", "nz", "ds")]
         self._ibm_n = n
         _check(self._lib.lbm_ibm_setup(self._h, C.byref(d), *[_dp(a) for a in arrs]), "lbm_ibm_setup")
+
+    def ibm_box(self):
+        b = (C.c_int32 * 6)()
+        cen = np.zeros(2)
+        _check(self._lib.lbm_ibm_box(self._h, b, _dp(cen)), "lbm_ibm_box")
+        return tuple(b[:]), cen
 
     def ibm_apply(self, vcm=(0.0, 0.0, 0.0), vcm_before=(0.0, 0.0, 0.0)):
         out = np.zeros(3)

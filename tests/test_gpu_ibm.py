@@ -164,6 +164,51 @@ def test_ibm_moving_body_added_mass_fp64(oracle_mod):
     lat.close()
 
 
+def test_ibm_box_follows_moving_body_fp64(oracle_mod):
+    """Cylinder_Inline_Oscillating motion, V_CM = (-u cos(2 pi f t), 0, 0)
+    (lbm.cpp:1143-1144), with the reference's box that follows the centre,
+    [floor(c - D), ceil(c + D)] (lbm.cpp:1162-1166): the box shifts and
+    grows during the run."""
+    D = 8.0
+    case, mk, box0 = cases.ibm_cylinder(60, 40, 1, D=D, Re=20.0, center=(30.0, 20.0))
+    nsteps = 16
+    u, fr = 0.1, 0.01
+    vcm = np.zeros((nsteps + 1, 3))
+    t = np.arange(1, nsteps + 1, dtype=np.float64)
+    vcm[1:, 0] = -u * np.cos(2 * cases.PI_REF * fr * t)
+    vs = case.params["vsolid"]
+    o = _oracle(oracle_mod, case)
+    omk = {k: v.copy() for k, v in mk.items()}
+    cx, cy = 30.0, 20.0
+    ref, boxes = [], []
+    for s in range(nsteps):
+        cx += (vcm[s + 1, 0] + vcm[s, 0]) * 1.0 / 2
+        cy += (vcm[s + 1, 1] + vcm[s, 1]) * 1.0 / 2
+        box = (math.floor(cx - D), math.ceil(cx + D), math.floor(cy - D), math.ceil(cy + D), 1, 1)
+        boxes.append(box)
+        e, out = o.ibm_step(omk, vcm[s + 1], vcm[s], box, m_max=20, vsolid=vs)
+        assert e == 0 and o.collision() == 0
+        o.streaming()
+        ref.append(out.copy())
+    ref = np.array(ref)
+    assert len(set(boxes)) > 1 and boxes[-1] != box0  # the box did move
+    for fused in (True, False):
+        lat = _setup(case, mk, box0, vsolid=vs)
+        lat.ibm_setup(mk, box0, m_max=20, vsolid=vs, follow=(30.0, 20.0, D))
+        if fused:
+            got = lat.ibm_run(nsteps, vcm)
+        else:
+            got = np.array([_apply_step(lat, vcm[s + 1], vcm[s]) for s in range(nsteps)])
+        assert lat.sync() == 0
+        b, cen = lat.ibm_box()
+        assert b == boxes[-1]
+        assert np.allclose(cen, (cx, cy), rtol=0, atol=1e-13)
+        _cmp_series(got, ref, 1e-11, f"follow fused={fused}")
+        _cmp_fields(lat, o, case, 1e-11, 1e-12)
+        _cmp_markers(lat, omk, 1e-10)
+        lat.close()
+
+
 def _apply_step(lat, v, vb):
     out = lat.ibm_apply(v, vb)
     lat.collision()
