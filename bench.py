@@ -91,6 +91,79 @@ def precision_study(case, nsteps):
     return out
 
 
+def ibm_channel(Nx=256, Ny=128, Nz=64, D=20.0, u_in=0.04, Re=100.0, n_theta=64):
+    """Rigid IBM cylinder in a channel: LBM(256,128,64), inlet/outlet faces,
+    walls at y and z, u_in = 0.04, Re = 100 on D = 20, +-1% seeded
+    perturbation; the cylinder along z at (Nx/5 + 0.5, Ny/2 + 0.5) carries
+    64 azimuthal x 64 axial = 4096 markers, box [floor(c - D), ceil(c + D)]
+    x k = 1..Nz, implicit scheme with m_max = 20."""
+    import math
+    case = cases.channel(Nx, Ny, Nz, u_in=u_in, Re=Re, D=D, perturb=0.01, seed=1)
+    cx, cy = Nx / 5.0, Ny / 2.0
+    mk = cases.cylinder_markers(cx, cy, D, n_theta, np.arange(1, Nz + 1, dtype=np.float64))
+    box = (math.floor(cx - D), math.ceil(cx + D), math.floor(cy - D), math.ceil(cy + D), 1, Nz)
+    return case, mk, box
+
+
+def ibm_leg(steps, cmp_steps, warmup=20):
+    """The IBM channel: FP32 ms/step of the full IBM step (lbm_ibm_run)
+    beside the plain fused step on the same lattice, C_D and C_L after
+    `steps` steps, and FP32 against FP64 storage after `cmp_steps`."""
+    case, mk, box = ibm_channel()
+    D, u_in, Nz = 20.0, 0.04, case.Nz
+    q = 0.5 * 1.0 * u_in ** 2 * D * Nz  # C = F / (1/2 rho0 u^2 D Nz)
+
+    def make(st):
+        lat = lbm_amd.Lattice(case.Nx, case.Ny, case.Nz, case.tau, storage=st)
+        lat.upload(**case.fields())
+        lat.init()
+        lat.ibm_setup(mk, box, m_max=20)
+        return lat
+
+    lat = make("f32")
+    lat.ibm_run(warmup)
+    lat.sync()
+    t0 = time.perf_counter()
+    out = lat.ibm_run(steps)
+    code = lat.sync()
+    dt = time.perf_counter() - t0
+    rho, ux, uy, uz = lat.macro()
+    lat.close()
+    plain = lbm_amd.Lattice(case.Nx, case.Ny, case.Nz, case.tau, storage="f32")
+    plain.upload(**case.fields())
+    plain.init()
+    plain.step(warmup)
+    plain_ms = plain.time_steps(200) / 200
+    plain.close()
+    cmp = {}
+    if cmp_steps:
+        f32 = make("f32")
+        o32 = f32.ibm_run(cmp_steps)
+        m32 = f32.macro()
+        f32.close()
+        f64 = make("f64")
+        o64 = f64.ibm_run(cmp_steps)
+        m64 = f64.macro()
+        f64.close()
+        fl = case.type == cases.TYPE_F
+        num = sum(float(((a - b)[fl] ** 2).sum()) for a, b in zip(m32[1:], m64[1:]))
+        den = sum(float((b[fl] ** 2).sum()) for b in m64[1:])
+        cmp = dict(steps=cmp_steps, l2_u_f32_vs_f64=float(np.sqrt(num / den)),
+                   cd_f32=float(o32[-1, 0] / q), cd_f64=float(o64[-1, 0] / q),
+                   cl_f32=float(o32[-1, 1] / q), cl_f64=float(o64[-1, 1] / q))
+    ms = dt / steps * 1e3
+    tail = out[-min(1000, steps):]
+    return dict(
+        workload="IBM cylinder channel 256x128x64 D3Q19 FP32, D=20, 4096 markers, m_max=20, Re=100",
+        status=code, steps=steps, ms_per_step=round(ms, 4),
+        mlups=round(case.nfluid * steps / dt / 1e6, 1),
+        plain_step_ms=round(plain_ms, 4), ibm_share=round(1.0 - plain_ms / ms, 3),
+        cd_last=float(out[-1, 0] / q), cl_last=float(out[-1, 1] / q),
+        cd_mean_last1000=float(tail[:, 0].mean() / q), cl_rms_last1000=float(np.sqrt((tail[:, 1] ** 2).mean()) / q),
+        finite=bool(np.isfinite(out).all() and np.isfinite(ux).all()),
+        f32_vs_f64=cmp)
+
+
 def cpu_baseline(case, budget_s, max_steps):
     """The CPU oracle on a bounded sample of the same workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -146,6 +219,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--storage", default="f32", choices=["f64", "f32", "f16s"])
+    ap.add_argument("--ibm-steps", type=int, default=20000)
+    ap.add_argument("--ibm-cmp-steps", type=int, default=2000)
+    ap.add_argument("--no-ibm", action="store_true")
+    ap.add_argument("--ibm-only", action="store_true", help="run only the IBM channel leg")
     ap.add_argument("--copy-probe-only", action="store_true",
                     help="run only the 1 GiB device copy (counter scaling reference)")
     a = ap.parse_args()
@@ -158,6 +235,10 @@ def main():
         print("bench.py measures one GPU: the multi-GPU path (z-slab halo exchange) is a "
               "design, not a build; run with --gpus 1", file=sys.stderr)
         return 2
+
+    if a.ibm_only:
+        print(json.dumps(ibm_leg(a.ibm_steps, a.ibm_cmp_steps)), flush=True)
+        return 0
 
     case = cases.taylor_green(a.size, u0=0.04, Re=1600.0)
     log(f"bench: TGV {a.size}^3, tau={case.tau:.6f}, warmup={a.warmup}, steps={a.steps}")
@@ -178,6 +259,12 @@ def main():
         log("bench: CPU baseline (oracle, OpenMP)")
         cpu = cpu_baseline(case, a.cpu_seconds, a.cpu_max_steps)
         log(f"bench: cpu {cpu['value']:.2f} MLUPS on {cpu['cores']} threads")
+
+    ibm = None
+    if not a.no_ibm:
+        log(f"bench: IBM channel leg, {a.ibm_steps} steps")
+        ibm = ibm_leg(a.ibm_steps, a.ibm_cmp_steps)
+        log(f"bench: ibm {json.dumps(ibm)}")
 
     copy_gbs = lbm_amd.copy_probe(1 << 30, 20)
     log(f"bench: 1 GiB device copy {copy_gbs:.0f} GB/s")
@@ -222,6 +309,7 @@ def main():
             "speedup_vs_main": round(oth_r["mlups"] / main_r["mlups"], 3),
         },
         "precision": prec,
+        "ibm": ibm,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

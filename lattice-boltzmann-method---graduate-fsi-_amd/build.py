@@ -42,7 +42,12 @@ SOURCES = [
 ]
 HOST_SOURCES = [
     os.path.join(HERE, "host", "LBM.cpp"),
+    os.path.join(HERE, "host", "IBM.cpp"),
 ]
+# reference-style drivers on the drop-in classes (host code only)
+EXAMPLES = ["example_cavity", "example_cylinder_ibm"]
+EXAMPLE_BIN = {"example_cavity": "lbm_example_cavity",
+               "example_cylinder_ibm": "lbm_example_cylinder_ibm"}
 
 
 def _run(cmd: list[str]) -> None:
@@ -74,12 +79,12 @@ def build(jobs: int = 4) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
-    # the reference-style driver on the drop-in class (host code only)
-    ex = os.path.join(HERE, "host", "example_cavity.cpp")
-    exe = os.path.join(HERE, "host", "lbm_example_cavity")
-    _run(["g++", "-O2", "-std=c++17", f"-I{os.path.join(ROOT, 'include')}",
-          f"-I{os.path.join(HERE, 'host')}", ex, "-o", exe, f"-L{HERE}", "-llbm_hip",
-          "-Wl,-rpath,$ORIGIN/.."])
+    for name in EXAMPLES:
+        ex = os.path.join(HERE, "host", name + ".cpp")
+        exe = os.path.join(HERE, "host", EXAMPLE_BIN[name])
+        _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(ROOT, 'include')}",
+              f"-I{os.path.join(HERE, 'host')}", ex, "-o", exe, f"-L{HERE}", "-llbm_hip",
+              "-Wl,-rpath,$ORIGIN/.."])
     return LIB
 
 

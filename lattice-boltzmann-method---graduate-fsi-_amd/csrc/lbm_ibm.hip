@@ -5,26 +5,34 @@
 // Per step:
 //   k_ibm_move        trapezoidal marker motion, u_des = V_CM (lbm.cpp:1154-1161)
 //   k_ibm_box_moments unforced rho, u = sum f c / rho, F = 0 of the box cells
-//                     (lbm.cpp:1262-1285), pulled from f* on the fused path
-//   k_ibm_interp      u_b = sum u phi phi phi over the 4x4x4 stencil clipped to
-//                     the box (lbm.cpp:1288-1309), in the reference's i,j,k order
-//   m = 1..m_max:
-//     k_ibm_force     F_b = 2 rho (u_des - u_b)/dt with rho at the box point
-//                     farthest along the marker normal; drag/lift partials
+//                     (lbm.cpp:1262-1285), pulled from f* on the fused path;
+//                     per-workgroup sums of the interior density
+//   k_ibm_interp      one wavefront per marker, one lane per point of its
+//                     4x4x4 stencil: lane terms u phi phi phi, summed by
+//                     one lane per component in the reference's i, j, k
+//                     order (lbm.cpp:1288-1309, 1363-1387); then the marker
+//                     force of the next pass, F_b = 2 rho (u_des - u_b)/dt
+//                     with rho at the box point farthest along the marker
+//                     normal, and the marker's running drag/lift
 //                     (lbm.cpp:1314-1329)
+//   m = 1..m_max:
 //     k_ibm_spread    Fm = sum F_b ds phi phi phi per box cell, F -= Fm,
-//                     u += Fm/(2 rho) (lbm.cpp:1332-1360).  Atomic-free: each
-//                     workgroup owns a tile of the box, stages the markers
-//                     whose stencil reaches the tile in LDS, sorted by marker
-//                     index so every cell sums in the reference's n order
-//     k_ibm_interp    (lbm.cpp:1363-1387)
-//   k_ibm_finish      FP64 drag, lift (tree sums of the partials), the mean
-//                     density of the interior box cells and the added-mass
-//                     term (lbm.cpp:1105-1132, 1390-1398)
+//                     u += Fm/(2 rho) (lbm.cpp:1332-1360).  Atomic-free: the
+//                     (marker, stencil point) pairs are binned to their cells
+//                     once per marker position, sorted by marker; a workgroup
+//                     takes a chunk of cells, computes their pairs' terms in
+//                     parallel into LDS, and each cell sums its own terms
+//                     serially, in the reference's marker order
+//     k_ibm_interp
+//   k_ibm_finish      FP64 drag, lift (tree sums of the per-marker sums), the
+//                     mean density of the interior box cells and the
+//                     added-mass term (lbm.cpp:1105-1132, 1390-1398)
 // The collision then takes rho, u, F from the box arrays (force mode 3).
+// The stencil weights phi(i - x) of every marker (4 per axis) are computed
+// once per marker position (k_ibm_weights).
 //
-// The marker arithmetic is FP64 with a*b+c contraction off, so each marker
-// and box value is rounded as the reference's C++ rounds it.
+// The marker arithmetic is FP64 with a*b+c contraction off, so each product
+// and sum over the stencil rounds as the reference's C++ does.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
@@ -41,16 +49,14 @@ using namespace lbm;
 
 namespace {
 
-constexpr int TX = 8, TY = 8, TZ = 4;  // spreading tile (256 cells)
-constexpr int TILE_MAX = 2048;         // markers reaching one tile (LDS sort bound)
-constexpr int FIN_THREADS = 256;
+constexpr int CHUNK_E = 1024;  // (marker, point) pairs per spreading workgroup
+constexpr int CHUNK_C = 256;   // cells per spreading workgroup
+constexpr int FIN_THREADS = 1024;
 
 struct Box {
   int x0, x1, y0, y1, z0, z1;  // inclusive, reference indices (i,j,k)
   int nx, ny, nz;
-  int tnx, tny, tnz;
   __host__ __device__ int ncell() const { return nx * ny * nz; }
-  __host__ __device__ int ntile() const { return tnx * tny * tnz; }
 };
 
 // kernel(r), lbm.cpp:1095-1102
@@ -155,79 +161,6 @@ __global__ void k_ibm_inside(Box B, int nm, const double *__restrict__ mx,
   inside[b] = in;
 }
 
-// stencil range of a marker along one axis, clipped to the box:
-// floor(m)-1 .. floor(m)+2 holds every point with phi(r) != 0
-__device__ __forceinline__ void stencil(double m, int lo, int hi, int &a, int &b) {
-  const int f = (int)floor(m);
-  a = max(f - 1, lo);
-  b = min(f + 2, hi);
-}
-
-__global__ void k_ibm_tile_count(Box B, int nm, const double *__restrict__ mx,
-                                 const double *__restrict__ my, const double *__restrict__ mz,
-                                 int32_t *__restrict__ count) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= nm) return;
-  int a0, a1, b0, b1, c0, c1;
-  stencil(mx[n], B.x0, B.x1, a0, a1);
-  stencil(my[n], B.y0, B.y1, b0, b1);
-  stencil(mz[n], B.z0, B.z1, c0, c1);
-  if (a0 > a1 || b0 > b1 || c0 > c1) return;
-  for (int tz = (c0 - B.z0) / TZ; tz <= (c1 - B.z0) / TZ; tz++)
-    for (int ty = (b0 - B.y0) / TY; ty <= (b1 - B.y0) / TY; ty++)
-      for (int tx = (a0 - B.x0) / TX; tx <= (a1 - B.x0) / TX; tx++)
-        atomicAdd(&count[tx + B.tnx * (ty + B.tny * tz)], 1);
-}
-
-__global__ void k_ibm_tile_fill(Box B, int nm, const double *__restrict__ mx,
-                                const double *__restrict__ my, const double *__restrict__ mz,
-                                const int32_t *__restrict__ start, int32_t *__restrict__ cursor,
-                                int32_t *__restrict__ list) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= nm) return;
-  int a0, a1, b0, b1, c0, c1;
-  stencil(mx[n], B.x0, B.x1, a0, a1);
-  stencil(my[n], B.y0, B.y1, b0, b1);
-  stencil(mz[n], B.z0, B.z1, c0, c1);
-  if (a0 > a1 || b0 > b1 || c0 > c1) return;
-  for (int tz = (c0 - B.z0) / TZ; tz <= (c1 - B.z0) / TZ; tz++)
-    for (int ty = (b0 - B.y0) / TY; ty <= (b1 - B.y0) / TY; ty++)
-      for (int tx = (a0 - B.x0) / TX; tx <= (a1 - B.x0) / TX; tx++) {
-        const int t = tx + B.tnx * (ty + B.tny * tz);
-        list[start[t] + atomicAdd(&cursor[t], 1)] = n;
-      }
-}
-
-// Sorts each tile's marker list ascending (bitonic sort in LDS), so that
-// the spreading sums follow the reference's marker order.
-__global__ __launch_bounds__(256) void k_ibm_tile_sort(const int32_t *__restrict__ start,
-                                                       int32_t *__restrict__ list) {
-  __shared__ int32_t s[TILE_MAX];
-  const int t = blockIdx.x;
-  const int b = start[t], len = start[t + 1] - b;
-  if (len <= 1) return;
-  int p2 = 1;
-  while (p2 < len) p2 <<= 1;
-  for (int i = threadIdx.x; i < p2; i += blockDim.x) s[i] = i < len ? list[b + i] : 0x7fffffff;
-  __syncthreads();
-  for (int k = 2; k <= p2; k <<= 1)
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < p2; i += blockDim.x) {
-        const int l = i ^ j;
-        if (l > i) {
-          const bool up = (i & k) == 0;
-          if ((s[i] > s[l]) == up) {
-            const int32_t tmp = s[i];
-            s[i] = s[l];
-            s[l] = tmp;
-          }
-        }
-      }
-      __syncthreads();
-    }
-  for (int i = threadIdx.x; i < len; i += blockDim.x) list[b + i] = s[i];
-}
-
 // ---- per-step kernels ----------------------------------------------------------
 // marker update, lbm.cpp:1154-1161 (u_des is the kernel argument vcm)
 __global__ void k_ibm_move(int nm, double *__restrict__ mx, double *__restrict__ my,
@@ -246,165 +179,260 @@ struct BoxArrays {
   const uint8_t *live;
 };
 
+// Per-marker stencil: base point (floor(x) - 1, ...) and the 4 weights per
+// axis phi(i - x), i = base .. base + 3 (kernel(r), lbm.cpp:1095-1102, with
+// r = (i dx - x)/dx, dx = 1).
+struct Stencils {
+  int32_t *base;  // 3 per marker
+  double *w;      // 12 per marker: wx[4], wy[4], wz[4]
+};
+
+__global__ void k_ibm_weights(int nm, const double *__restrict__ mx, const double *__restrict__ my,
+                              const double *__restrict__ mz, Stencils st) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nm) return;
+  const double p[3] = {mx[n], my[n], mz[n]};
+  for (int a = 0; a < 3; a++) {
+    const int b = (int)floor(p[a]) - 1;
+    st.base[3 * n + a] = b;
+    for (int q = 0; q < 4; q++) st.w[12 * n + 4 * a + q] = peskin(((b + q) * 1.0 - p[a]) / 1.0);
+  }
+}
+
+// ---- binning of (marker, stencil point) pairs to box cells ---------------------
+// pidx[64 n + s] = box index of stencil point s of marker n when it lies in
+// the box on a fluid cell, else -1; cnt[b] = pairs on box cell b.
+__global__ void k_ibm_bin_count(Box B, int nm, Stencils st, const uint8_t *__restrict__ live,
+                                int32_t *__restrict__ cnt, int32_t *__restrict__ pidx) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nm * 64) return;
+  const int n = t >> 6, s = t & 63;
+  const int i = st.base[3 * n + 0] + (s >> 4), j = st.base[3 * n + 1] + ((s >> 2) & 3),
+            k = st.base[3 * n + 2] + (s & 3);
+  int b = -1;
+  if (i >= B.x0 && i <= B.x1 && j >= B.y0 && j <= B.y1 && k >= B.z0 && k <= B.z1) {
+    b = (i - B.x0) + B.nx * ((j - B.y0) + B.ny * (k - B.z0));
+    if (live[b]) atomicAdd(&cnt[b], 1);
+    else b = -1;
+  }
+  pidx[t] = b;
+}
+
+// pairs into their cell's segment (any order; k_ibm_bin_sort orders them)
+__global__ void k_ibm_bin_fill(int nm, const int32_t *__restrict__ pidx, const int32_t *__restrict__ bandpos,
+                               const int32_t *__restrict__ bstart, int32_t *__restrict__ cursor,
+                               int32_t *__restrict__ ent) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nm * 64) return;
+  const int b = pidx[t];
+  if (b < 0) return;
+  ent[bstart[bandpos[b]] + atomicAdd(&cursor[b], 1)] = t;
+}
+
+// One thread per band cell: sort its pairs by marker (a marker has at most
+// one point on a cell), then replace each pair by its marker index and
+// store the pair's three 1D weights.
+__global__ void k_ibm_bin_sort(int nband, const int32_t *__restrict__ bstart, Stencils st,
+                               int32_t *__restrict__ ent, double *__restrict__ ew) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nband) return;
+  const int e0 = bstart[i], e1 = bstart[i + 1];
+  for (int e = e0 + 1; e < e1; e++) {
+    const int32_t v = ent[e];
+    int q = e - 1;
+    while (q >= e0 && ent[q] > v) {
+      ent[q + 1] = ent[q];
+      q--;
+    }
+    ent[q + 1] = v;
+  }
+  for (int e = e0; e < e1; e++) {
+    const int t = ent[e], n = t >> 6, s = t & 63;
+    ew[3 * (size_t)e + 0] = st.w[12 * n + (s >> 4)];
+    ew[3 * (size_t)e + 1] = st.w[12 * n + 4 + ((s >> 2) & 3)];
+    ew[3 * (size_t)e + 2] = st.w[12 * n + 8 + (s & 3)];
+    ent[e] = n;
+  }
+}
+
 // Unforced moments of the box cells, lbm.cpp:1262-1285, summed in l order
 // in FP64.  PULL: the buffer holds f* of the previous step and the cell's
 // f is pulled first (the fused path); otherwise the buffer holds f.  Box
 // cells that are not fluid keep their set-up rho, u (their density is read
-// by the nearest-point force and the interior average).
+// by the nearest-point force and the interior average).  Each workgroup
+// also writes the sum of rho over its interior cells (fixed tree order).
 template <class S, bool PULL>
 __global__ __launch_bounds__(256) void k_ibm_box_moments(const Planes<S> P, const uint8_t *__restrict__ mask,
                                                          KArgs<typename Store<S>::C> a, Box B,
-                                                         BoxArrays o) {
+                                                         BoxArrays o, const uint8_t *__restrict__ inside,
+                                                         double *__restrict__ part) {
   using C = typename Store<S>::C;
+  __shared__ double sh[256];
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= B.ncell()) return;
-  const int x = B.x0 + b % B.nx, y = B.y0 + (b / B.nx) % B.ny, z = B.z0 + b / (B.nx * B.ny);
-  const uint32_t idx = lattice_idx(a.g, x, y, z);
-  const uint8_t mk = mask[idx];
-  o.Fx[b] = 0.0;
-  o.Fy[b] = 0.0;
-  o.Fz[b] = 0.0;
-  if (!(mk & M_FLUID)) {
-    o.rho[b] = (double)a.rho[idx];
-    o.ux[b] = (double)a.ux[idx];
-    o.uy[b] = (double)a.uy[idx];
-    o.uz[b] = (double)a.uz[idx];
-    return;
+  double r_in = 0.0;
+  if (b < B.ncell()) {
+    const int x = B.x0 + b % B.nx, y = B.y0 + (b / B.nx) % B.ny, z = B.z0 + b / (B.nx * B.ny);
+    const uint32_t idx = lattice_idx(a.g, x, y, z);
+    const uint8_t mk = mask[idx];
+    o.Fx[b] = 0.0;
+    o.Fy[b] = 0.0;
+    o.Fz[b] = 0.0;
+    double rho, ux, uy, uz;
+    if (!(mk & M_FLUID)) {
+      rho = (double)a.rho[idx];
+      ux = (double)a.ux[idx];
+      uy = (double)a.uy[idx];
+      uz = (double)a.uz[idx];
+    } else {
+      C f[Q];
+      if constexpr (PULL) {
+        pull_cell<S>(P, mk, idx, a, f);
+      } else {
+        for_dirs([&](auto M) {
+          constexpr int m = decltype(M)::value;
+          f[m] = Store<S>::ld(P.in[m] + idx);
+        });
+      }
+      double r0 = 0, rx = 0, ry = 0, rz = 0;
+      for_dirs([&](auto M) {
+        constexpr int m = decltype(M)::value;
+        double v = (double)f[m];
+        if constexpr (Store<S>::shifted) v = v / (double)Store<S>::K + W[m];
+        r0 += v;
+        rx += v * CX[m];
+        ry += v * CY[m];
+        rz += v * CZ[m];
+      });
+      rho = r0;
+      ux = rx / r0;
+      uy = ry / r0;
+      uz = rz / r0;
+    }
+    o.rho[b] = rho;
+    o.ux[b] = ux;
+    o.uy[b] = uy;
+    o.uz[b] = uz;
+    if (inside[b]) r_in = rho;
   }
-  C f[Q];
-  if constexpr (PULL) {
-    pull_cell<S>(P, mk, idx, a, f);
-  } else {
-    for_dirs([&](auto M) {
-      constexpr int m = decltype(M)::value;
-      f[m] = Store<S>::ld(P.in[m] + idx);
-    });
+  sh[threadIdx.x] = r_in;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
   }
-  double rho = 0, rx = 0, ry = 0, rz = 0;
-  for_dirs([&](auto M) {
-    constexpr int m = decltype(M)::value;
-    double v = (double)f[m];
-    if constexpr (Store<S>::shifted) v = v / (double)Store<S>::K + W[m];
-    rho += v;
-    rx += v * CX[m];
-    ry += v * CY[m];
-    rz += v * CZ[m];
-  });
-  o.rho[b] = rho;
-  o.ux[b] = rx / rho;
-  o.uy[b] = ry / rho;
-  o.uz[b] = rz / rho;
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-// u_b(n) = sum over the clipped stencil of u * phi(rx) phi(ry) phi(rz) in
-// i, j, k order (lbm.cpp:1295-1306); non-IBM cells (solids) are skipped.
-__global__ void k_ibm_interp(Box B, int nm, const double *__restrict__ mx,
-                             const double *__restrict__ my, const double *__restrict__ mz,
-                             BoxArrays o, double *__restrict__ ub) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= nm) return;
-  const double px = mx[n], py = my[n], pz = mz[n];
-  int a0, a1, b0, b1, c0, c1;
-  stencil(px, B.x0, B.x1, a0, a1);
-  stencil(py, B.y0, B.y1, b0, b1);
-  stencil(pz, B.z0, B.z1, c0, c1);
-  double s0 = 0, s1 = 0, s2 = 0;
-  for (int i = a0; i <= a1; i++) {
-    const double kx = peskin((i * 1.0 - px) / 1.0);
-    for (int j = b0; j <= b1; j++) {
-      const double ky = peskin((j * 1.0 - py) / 1.0);
-      for (int k = c0; k <= c1; k++) {
-        const int b = (i - B.x0) + B.nx * ((j - B.y0) + B.ny * (k - B.z0));
-        if (!o.live[b]) continue;
-        const double kz = peskin((k * 1.0 - pz) / 1.0);
-        s0 += o.ux[b] * kx * ky * kz;
-        s1 += o.uy[b] * kx * ky * kz;
-        s2 += o.uz[b] * kx * ky * kz;
+struct ForceArgs {
+  const int32_t *near;   // box index of each marker's density point
+  const double *ds;
+  double *Fb, *Fds;      // F_b and F_b ds (3 per marker)
+  double *dsum;          // running drag, lift per marker (2 per marker)
+  double u0, u1, u2;     // u_des = V_CM
+  int force;             // 0: interpolation only (last pass); 1: also the next force
+  int first;             // 1: start the running drag/lift
+};
+
+// Interpolation of one marker per wavefront (lbm.cpp:1295-1306 and
+// 1370-1384) and the marker force of the next pass (lbm.cpp:1314-1323).
+// Lane s = (a*4 + b)*4 + c is stencil point (base + (a, b, c)); points
+// outside the box or on non-IBM cells add nothing.  Lanes 0..2 sum the
+// 64 terms of one component serially in lane order, which is the
+// reference's i, j, k loop order.
+__global__ __launch_bounds__(256) void k_ibm_interp(int nm, const int32_t *__restrict__ pidx, Stencils st,
+                                                    BoxArrays o, double *__restrict__ ub, ForceArgs fa) {
+  __shared__ double term[4][3][64];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int n = blockIdx.x * 4 + wv;
+  const bool valid = n < nm;
+  double t0 = 0, t1 = 0, t2 = 0;
+  const int b = valid ? pidx[64 * n + lane] : -1;
+  // density at the marker's nearest point, read early (lanes 0..2)
+  double rho_n = 0;
+  if (fa.force && valid && lane < 3) {
+    const int nb = fa.near[n];
+    rho_n = nb >= 0 ? o.rho[nb] : nan("");
+  }
+  if (b >= 0) {
+    const double kx = st.w[12 * n + (lane >> 4)], ky = st.w[12 * n + 4 + ((lane >> 2) & 3)],
+                 kz = st.w[12 * n + 8 + (lane & 3)];
+    t0 = o.ux[b] * kx * ky * kz;
+    t1 = o.uy[b] * kx * ky * kz;
+    t2 = o.uz[b] * kx * ky * kz;
+  }
+  term[wv][0][lane] = t0;
+  term[wv][1][lane] = t1;
+  term[wv][2][lane] = t2;
+  __syncthreads();
+  if (valid && lane < 3) {
+    double acc = 0.0;
+    for (int q = 0; q < 64; q++) acc += term[wv][lane][q];
+    ub[3 * n + lane] = acc;
+    if (fa.force) {
+      const double dt = 1.0, dx = 1.0, dy = 1.0;
+      const double rho = rho_n;
+      const double ud = lane == 0 ? fa.u0 : lane == 1 ? fa.u1 : fa.u2;
+      const double F = 2 * rho * (ud - acc) / dt;
+      fa.Fb[3 * n + lane] = F;
+      fa.Fds[3 * n + lane] = F * fa.ds[n];
+      if (lane < 2) {
+        const double d = -F * (lane == 0 ? dx : dy) * fa.ds[n];
+        fa.dsum[2 * n + lane] = fa.first ? d : fa.dsum[2 * n + lane] + d;
       }
     }
   }
-  ub[3 * n + 0] = s0;
-  ub[3 * n + 1] = s1;
-  ub[3 * n + 2] = s2;
-}
-
-// marker force and drag/lift partials of iteration m, lbm.cpp:1314-1323
-__global__ void k_ibm_force(int nm, const int32_t *__restrict__ near, const double *__restrict__ brho,
-                            const double *__restrict__ ub, const double *__restrict__ ds, double u0,
-                            double u1, double u2, double *__restrict__ Fb, double *__restrict__ dl) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= nm) return;
-  const double dt = 1.0, dx = 1.0, dy = 1.0;
-  const int nb = near[n];
-  const double rho = nb >= 0 ? brho[nb] : nan("");
-  const double f0 = 2 * rho * (u0 - ub[3 * n + 0]) / dt;
-  const double f1 = 2 * rho * (u1 - ub[3 * n + 1]) / dt;
-  const double f2 = 2 * rho * (u2 - ub[3 * n + 2]) / dt;
-  Fb[3 * n + 0] = f0;
-  Fb[3 * n + 1] = f1;
-  Fb[3 * n + 2] = f2;
-  dl[2 * n + 0] = -f0 * dx * ds[n];
-  dl[2 * n + 1] = -f1 * dy * ds[n];
 }
 
 // Spreading and velocity correction, lbm.cpp:1332-1360.  One workgroup per
-// tile of TX x TY x TZ box cells; the markers reaching the tile (sorted by
-// index) are staged in LDS in chunks and every live cell sums their
-// contributions in marker order.
-__global__ __launch_bounds__(256) void k_ibm_spread(Box B, const int32_t *__restrict__ start,
-                                                    const int32_t *__restrict__ list,
-                                                    const double *__restrict__ mx,
-                                                    const double *__restrict__ my,
-                                                    const double *__restrict__ mz,
-                                                    const double *__restrict__ Fb,
-                                                    const double *__restrict__ ds, BoxArrays o) {
-  constexpr int CH = 256;
-  __shared__ double sx[CH], sy[CH], sz[CH], sf0[CH], sf1[CH], sf2[CH], sds[CH];
-  const int t = blockIdx.x;
-  const int tx = t % B.tnx, ty = (t / B.tnx) % B.tny, tz = t / (B.tnx * B.tny);
-  const int lx = threadIdx.x % TX, ly = (threadIdx.x / TX) % TY, lz = threadIdx.x / (TX * TY);
-  const int bx = tx * TX + lx, by = ty * TY + ly, bz = tz * TZ + lz;
-  const bool mine = bx < B.nx && by < B.ny && bz < B.nz;
-  const int b = bx + B.nx * (by + B.ny * bz);
-  const bool live = mine && o.live[b];
-  const double i = B.x0 + bx, j = B.y0 + by, k = B.z0 + bz;
-  double m0 = 0, m1 = 0, m2 = 0;
-  const int l0 = start[t], l1 = start[t + 1];
-  for (int c = l0; c < l1; c += CH) {
-    const int len = min(CH, l1 - c);
-    __syncthreads();
-    if ((int)threadIdx.x < len) {
-      const int n = list[c + threadIdx.x];
-      sx[threadIdx.x] = mx[n];
-      sy[threadIdx.x] = my[n];
-      sz[threadIdx.x] = mz[n];
-      sf0[threadIdx.x] = Fb[3 * n + 0];
-      sf1[threadIdx.x] = Fb[3 * n + 1];
-      sf2[threadIdx.x] = Fb[3 * n + 2];
-      sds[threadIdx.x] = ds[n];
-    }
-    __syncthreads();
-    if (live) {
-      for (int q = 0; q < len; q++) {
-        const double rx = (i * 1.0 - sx[q]) / 1.0, ry = (j * 1.0 - sy[q]) / 1.0,
-                     rz = (k * 1.0 - sz[q]) / 1.0;
-        if (fabs(rx) >= 2 || fabs(ry) >= 2 || fabs(rz) >= 2) continue;  // phi = 0
-        const double kx = peskin(rx), ky = peskin(ry), kz = peskin(rz);
-        m0 += sf0[q] * sds[q] * kx * ky * kz;
-        m1 += sf1[q] * sds[q] * kx * ky * kz;
-        m2 += sf2[q] * sds[q] * kx * ky * kz;
-      }
-    }
+// chunk of band cells (cells that some marker stencil reaches): the terms
+// F_b ds phi phi phi of the chunk's (marker, point) pairs are computed in
+// parallel into LDS, then each cell sums its own terms serially in marker
+// order, as the reference's loop over n does, and updates F and u.
+__global__ __launch_bounds__(256) void k_ibm_spread(const int32_t *__restrict__ chunk,
+                                                    const int32_t *__restrict__ band,
+                                                    const int32_t *__restrict__ bstart,
+                                                    const int32_t *__restrict__ en,
+                                                    const double *__restrict__ ew,
+                                                    const double *__restrict__ Fds, BoxArrays o) {
+  __shared__ double term[3][CHUNK_E];
+  // chunk[2q] = first band cell, chunk[2q+1] = first pair of chunk q
+  const int c0 = chunk[2 * blockIdx.x], c1 = chunk[2 * blockIdx.x + 2];
+  const int e0 = chunk[2 * blockIdx.x + 1], e1 = chunk[2 * blockIdx.x + 3];
+  // the cell's own state, loaded while the pair terms are computed
+  const int i = c0 + (int)threadIdx.x;
+  const bool cell = i < c1;
+  int s0 = 0, s1 = 0, b = 0;
+  double r = 1, fx = 0, fy = 0, fz = 0, ux = 0, uy = 0, uz = 0;
+  if (cell) {
+    s0 = bstart[i] - e0;
+    s1 = bstart[i + 1] - e0;
+    b = band[i];
+    r = o.rho[b];
+    fx = o.Fx[b]; fy = o.Fy[b]; fz = o.Fz[b];
+    ux = o.ux[b]; uy = o.uy[b]; uz = o.uz[b];
   }
-  if (live) {
+  for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x) {
+    const int n = en[e];
+    const double kx = ew[3 * (size_t)e + 0], ky = ew[3 * (size_t)e + 1], kz = ew[3 * (size_t)e + 2];
+    term[0][e - e0] = Fds[3 * n + 0] * kx * ky * kz;
+    term[1][e - e0] = Fds[3 * n + 1] * kx * ky * kz;
+    term[2][e - e0] = Fds[3 * n + 2] * kx * ky * kz;
+  }
+  __syncthreads();
+  if (cell) {
+    double m0 = 0, m1 = 0, m2 = 0;
+    for (int e = s0; e < s1; e++) {
+      m0 += term[0][e];
+      m1 += term[1][e];
+      m2 += term[2][e];
+    }
     const double dt = 1.0;
-    o.Fx[b] += -m0;
-    o.Fy[b] += -m1;
-    o.Fz[b] += -m2;
-    const double r = o.rho[b];
-    o.ux[b] += dt * m0 / (2 * r);
-    o.uy[b] += dt * m1 / (2 * r);
-    o.uz[b] += dt * m2 / (2 * r);
+    o.Fx[b] = fx + -m0;
+    o.Fy[b] = fy + -m1;
+    o.Fz[b] = fz + -m2;
+    o.ux[b] = ux + dt * m0 / (2 * r);
+    o.uy[b] = uy + dt * m1 / (2 * r);
+    o.uz[b] = uz + dt * m2 / (2 * r);
   }
 }
 
@@ -421,28 +449,24 @@ __device__ __forceinline__ double fin_sum(double v, double *sh) {
 }
 
 // Drag, lift and the added-mass term, lbm.cpp:1105-1132, 1390-1398.  One
-// workgroup, fixed summation order.
-__global__ __launch_bounds__(FIN_THREADS) void k_ibm_finish(int nterms, const double *__restrict__ dl,
-                                                            Box B, const uint8_t *__restrict__ inside,
-                                                            const double *__restrict__ brho,
-                                                            double vsolid, double d0, double d1,
-                                                            double *__restrict__ out) {
+// workgroup, fixed summation order: the per-marker running sums and the
+// interior-density partials of k_ibm_box_moments.  ninside = the number of
+// interior box cells (0: the reference divides by zero, NaN here too).
+__global__ __launch_bounds__(FIN_THREADS) void k_ibm_finish(int nm, const double *__restrict__ dsum,
+                                                            int npart, const double *__restrict__ part,
+                                                            double ninside, double vsolid, double d0,
+                                                            double d1, double *__restrict__ out) {
   __shared__ double sh[FIN_THREADS];
-  double a = 0, l = 0, r = 0, c = 0;
-  for (int q = threadIdx.x; q < nterms; q += FIN_THREADS) {
-    a += dl[2 * q + 0];
-    l += dl[2 * q + 1];
+  double a = 0, l = 0, r = 0;
+  for (int q = threadIdx.x; q < nm; q += FIN_THREADS) {
+    a += dsum[2 * q + 0];
+    l += dsum[2 * q + 1];
   }
-  for (int b = threadIdx.x; b < B.ncell(); b += FIN_THREADS)
-    if (inside[b]) {
-      r += brho[b];
-      c += 1.0;
-    }
-  const double drag = fin_sum(a, sh), lift = fin_sum(l, sh), rs = fin_sum(r, sh),
-               cnt = fin_sum(c, sh);
+  for (int q = threadIdx.x; q < npart; q += FIN_THREADS) r += part[q];
+  const double drag = fin_sum(a, sh), lift = fin_sum(l, sh), rs = fin_sum(r, sh);
   if (threadIdx.x == 0) {
     const double dt = 1.0;
-    const double rho = rs / cnt;  // NaN when no cell is interior, as lbm.cpp:1129
+    const double rho = rs / ninside;
     out[0] = drag + rho * vsolid * d0 / dt;
     out[1] = lift + rho * vsolid * d1 / dt;
     out[2] = rho;
@@ -452,33 +476,6 @@ __global__ __launch_bounds__(FIN_THREADS) void k_ibm_finish(int nterms, const do
 __global__ void k_fill_i32(int32_t *p, int n, int32_t v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
-}
-
-__global__ __launch_bounds__(1024) void k_scan_i32(const int32_t *__restrict__ count, int n,
-                                                   int32_t *__restrict__ start) {
-  // single block, n is small (tiles of the IBM box): sequential chunks
-  __shared__ int32_t part[1024];
-  const int per = (n + blockDim.x - 1) / blockDim.x;
-  const int lo = threadIdx.x * per, hi = min(n, lo + per);
-  int32_t s = 0;
-  for (int i = lo; i < hi; i++) s += count[i];
-  part[threadIdx.x] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int32_t acc = 0;
-    for (int i = 0; i < (int)blockDim.x; i++) {
-      const int32_t v = part[i];
-      part[i] = acc;
-      acc += v;
-    }
-    start[n] = acc;
-  }
-  __syncthreads();
-  int32_t acc = part[threadIdx.x];
-  for (int i = lo; i < hi; i++) {
-    start[i] = acc;
-    acc += count[i];
-  }
 }
 
 inline dim3 blocks(int n, int bs = 256) { return dim3((unsigned)((n + bs - 1) / bs)); }
@@ -493,14 +490,25 @@ struct lbm_ibm {
   Box B{};
   bool placed = false;
   double *mx = nullptr, *my = nullptr, *mz = nullptr, *nx = nullptr, *ny = nullptr, *nz = nullptr;
-  double *ds = nullptr, *ub = nullptr, *Fb = nullptr, *dl = nullptr;
+  double *ds = nullptr, *ub = nullptr, *Fb = nullptr, *Fds = nullptr, *dsum = nullptr;
   int32_t *near = nullptr;
-  // box arrays (capacity box_cap cells) and tile arrays (tile_cap tiles)
+  Stencils st{};
+  double ninside = 0;  // interior box cells
+  // box arrays (capacity box_cap cells)
   double *brho = nullptr, *bux = nullptr, *buy = nullptr, *buz = nullptr;
   double *bFx = nullptr, *bFy = nullptr, *bFz = nullptr;
   uint8_t *live = nullptr, *inside = nullptr;
-  int32_t *tcount = nullptr, *tstart = nullptr, *tcursor = nullptr, *tlist = nullptr;
-  size_t box_cap = 0, tile_cap = 0, tlist_cap = 0;
+  double *part = nullptr;  // interior-density partials, one per box-moments workgroup
+  int32_t *cnt = nullptr, *cursor = nullptr, *bandpos = nullptr;  // per box cell
+  size_t box_cap = 0;
+  // (marker, point) binning: pidx (64 per marker), band cells and their
+  // pair segments, spreading chunks
+  int32_t *pidx = nullptr, *band = nullptr, *bstart = nullptr, *ent = nullptr, *chunk = nullptr;
+  double *ew = nullptr;
+  size_t band_cap = 0, bstart_cap = 0, ent_cap = 0, ew_cap = 0, chunk_cap = 0;
+  int nband = 0, nchunk = 0;
+  std::vector<int32_t> h_cnt, h_band, h_bandpos, h_bstart, h_chunk;
+  std::vector<uint8_t> h_inside;
   double *out = nullptr;  // 3 per step of a run
   int out_cap = 0;
   int ready = 0;          // IBLBM done, the next collision uses force mode 3
@@ -523,13 +531,25 @@ static void dfree(T *&p) {
 static void free_box_arrays(lbm_ibm *m) {
   dfree(m->brho); dfree(m->bux); dfree(m->buy); dfree(m->buz);
   dfree(m->bFx); dfree(m->bFy); dfree(m->bFz);
-  dfree(m->live); dfree(m->inside);
+  dfree(m->live); dfree(m->inside); dfree(m->part);
+  dfree(m->cnt); dfree(m->cursor); dfree(m->bandpos);
   m->box_cap = 0;
 }
 
-static void free_tile_arrays(lbm_ibm *m) {
-  dfree(m->tcount); dfree(m->tcursor); dfree(m->tstart);
-  m->tile_cap = 0;
+static void free_bin_arrays(lbm_ibm *m) {
+  dfree(m->band); dfree(m->bstart); dfree(m->ent); dfree(m->chunk); dfree(m->ew);
+  m->band_cap = m->bstart_cap = m->ent_cap = m->ew_cap = m->chunk_cap = 0;
+}
+
+// grows a device array to hold n elements of T (contents not kept)
+template <class T>
+static bool grow(T *&p, size_t &cap, size_t n) {
+  if (n <= cap && p) return true;
+  dfree(p);
+  cap = 0;
+  if (hipMalloc((void **)&p, sizeof(T) * std::max<size_t>(n, 1)) != hipSuccess) return false;
+  cap = std::max<size_t>(n, 1);
+  return true;
 }
 
 void ibm_free(lbm_ctx *c) {
@@ -537,8 +557,7 @@ void ibm_free(lbm_ctx *c) {
   lbm_ibm *m = c->ibm;
   for (void *p : m->allocs) (void)hipFree(p);
   free_box_arrays(m);
-  free_tile_arrays(m);
-  dfree(m->tlist);
+  free_bin_arrays(m);
   dfree(m->out);
   delete m;
   c->ibm = nullptr;
@@ -587,9 +606,11 @@ static hipError_t box_moments_s(lbm_ctx *c, bool pull) {
   const Planes<S> P = planes_of<S>(c, c->cur);
   const auto a = kargs_of<S>(L);
   if (pull)
-    hipLaunchKernelGGL((k_ibm_box_moments<S, true>), blocks(m->B.ncell()), dim3(256), 0, c->stream, P, c->mask, a, m->B, box_arrays(m));
+    hipLaunchKernelGGL((k_ibm_box_moments<S, true>), blocks(m->B.ncell()), dim3(256), 0, c->stream, P, c->mask, a, m->B,
+                       box_arrays(m), m->inside, m->part);
   else
-    hipLaunchKernelGGL((k_ibm_box_moments<S, false>), blocks(m->B.ncell()), dim3(256), 0, c->stream, P, c->mask, a, m->B, box_arrays(m));
+    hipLaunchKernelGGL((k_ibm_box_moments<S, false>), blocks(m->B.ncell()), dim3(256), 0, c->stream, P, c->mask, a, m->B,
+                       box_arrays(m), m->inside, m->part);
   return hipGetLastError();
 }
 
@@ -627,8 +648,8 @@ static int check_box(const lbm_ctx *c, const int b[6]) {
   return LBM_OK;
 }
 
-// Puts the IBM box at b: clears the old box's mask bits, grows the box and
-// tile arrays when needed, marks the new box and points the collision's
+// Puts the IBM box at b: clears the old box's mask bits, grows the box
+// arrays when needed, marks the new box and points the collision's
 // box arguments at it.
 static int place_box(lbm_ctx *c, const int b[6]) {
   lbm_ibm *m = c->ibm;
@@ -642,8 +663,7 @@ static int place_box(lbm_ctx *c, const int b[6]) {
   Box &B = m->B;
   B.x0 = b[0]; B.x1 = b[1]; B.y0 = b[2]; B.y1 = b[3]; B.z0 = b[4]; B.z1 = b[5];
   B.nx = b[1] - b[0] + 1; B.ny = b[3] - b[2] + 1; B.nz = b[5] - b[4] + 1;
-  B.tnx = (B.nx + TX - 1) / TX; B.tny = (B.ny + TY - 1) / TY; B.tnz = (B.nz + TZ - 1) / TZ;
-  const size_t nb = (size_t)B.ncell(), nt = (size_t)B.ntile();
+  const size_t nb = (size_t)B.ncell();
   if (nb > m->box_cap) {
     HIPCHK(hipStreamSynchronize(c->stream));
     free_box_arrays(m);
@@ -652,17 +672,12 @@ static int place_box(lbm_ctx *c, const int b[6]) {
       if (hipMalloc((void **)bp[q], 8 * nb) != hipSuccess) return LBM_ERR_NOMEM;
     if (hipMalloc((void **)&m->live, nb) != hipSuccess) return LBM_ERR_NOMEM;
     if (hipMalloc((void **)&m->inside, nb) != hipSuccess) return LBM_ERR_NOMEM;
+    if (hipMalloc((void **)&m->part, 8 * ((nb + 255) / 256)) != hipSuccess) return LBM_ERR_NOMEM;
+    if (hipMalloc((void **)&m->cnt, 4 * nb) != hipSuccess) return LBM_ERR_NOMEM;
+    if (hipMalloc((void **)&m->cursor, 4 * nb) != hipSuccess) return LBM_ERR_NOMEM;
+    if (hipMalloc((void **)&m->bandpos, 4 * nb) != hipSuccess) return LBM_ERR_NOMEM;
     for (int q = 0; q < 7; q++) HIPCHK(hipMemsetAsync(*bp[q], 0, 8 * nb, c->stream));
     m->box_cap = nb;
-  }
-  if (nt > m->tile_cap) {
-    HIPCHK(hipStreamSynchronize(c->stream));
-    free_tile_arrays(m);
-    if (hipMalloc((void **)&m->tcount, 4 * nt) != hipSuccess ||
-        hipMalloc((void **)&m->tcursor, 4 * nt) != hipSuccess ||
-        hipMalloc((void **)&m->tstart, 4 * (nt + 1)) != hipSuccess)
-      return LBM_ERR_NOMEM;
-    m->tile_cap = nt;
   }
   hipLaunchKernelGGL(k_ibm_mark, blocks((int)nb), dim3(256), 0, c->stream, c->mask, c->g, B, m->live, 1);
   HIPCHK(hipGetLastError());
@@ -672,36 +687,77 @@ static int place_box(lbm_ctx *c, const int b[6]) {
   return LBM_OK;
 }
 
-// nearest points, interior cells, spreading tiles for the current markers
+// Everything that depends on the marker positions: stencil weights, nearest
+// points, interior cells, and the binning of (marker, point) pairs to box
+// cells with the spreading chunks.  Ends with the stream synchronised.
 static int ibm_geometry(lbm_ctx *c) {
   lbm_ibm *m = c->ibm;
   const Box &B = m->B;
+  const int nb = B.ncell(), np = m->nm * 64;
+  hipLaunchKernelGGL(k_ibm_weights, blocks(m->nm), dim3(256), 0, c->stream, m->nm, m->mx, m->my, m->mz, m->st);
   hipLaunchKernelGGL(k_ibm_nearest, dim3(m->nm), dim3(256), 0, c->stream, B, m->mx, m->my, m->mz, m->nx, m->ny, m->nz, m->near);
-  hipLaunchKernelGGL(k_ibm_inside, blocks(B.ncell()), dim3(256), 0, c->stream, B, m->nm, m->mx, m->my, m->mz, m->nx, m->ny, m->nz, m->inside);
-  const int nt = B.ntile();
-  hipLaunchKernelGGL(k_fill_i32, blocks(nt), dim3(256), 0, c->stream, m->tcount, nt, 0);
-  hipLaunchKernelGGL(k_fill_i32, blocks(nt), dim3(256), 0, c->stream, m->tcursor, nt, 0);
-  hipLaunchKernelGGL(k_ibm_tile_count, blocks(m->nm), dim3(256), 0, c->stream, B, m->nm, m->mx, m->my, m->mz, m->tcount);
-  hipLaunchKernelGGL(k_scan_i32, dim3(1), dim3(1024), 0, c->stream, m->tcount, nt, m->tstart);
+  hipLaunchKernelGGL(k_ibm_inside, blocks(nb), dim3(256), 0, c->stream, B, m->nm, m->mx, m->my, m->mz, m->nx, m->ny, m->nz, m->inside);
+  hipLaunchKernelGGL(k_fill_i32, blocks(nb), dim3(256), 0, c->stream, m->cnt, nb, 0);
+  hipLaunchKernelGGL(k_fill_i32, blocks(nb), dim3(256), 0, c->stream, m->cursor, nb, 0);
+  hipLaunchKernelGGL(k_ibm_bin_count, blocks(np), dim3(256), 0, c->stream, B, m->nm, m->st, m->live, m->cnt, m->pidx);
   HIPCHK(hipGetLastError());
-  // list capacity and the per-tile sort bound are checked on the host
-  std::vector<int32_t> cnt(nt);
-  HIPCHK(hipMemcpyAsync(cnt.data(), m->tcount, sizeof(int32_t) * nt, hipMemcpyDeviceToHost, c->stream));
+  m->h_cnt.resize(nb);
+  m->h_inside.resize(nb);
+  HIPCHK(hipMemcpyAsync(m->h_cnt.data(), m->cnt, 4 * (size_t)nb, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(m->h_inside.data(), m->inside, (size_t)nb, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  size_t total = 0;
-  for (int t = 0; t < nt; t++) {
-    if (cnt[t] > TILE_MAX) return LBM_ERR_UNSUPPORTED;
-    total += (size_t)cnt[t];
+  size_t nin = 0;
+  for (uint8_t v : m->h_inside) nin += v;
+  m->ninside = (double)nin;
+  // band cells in box order, their pair segments, and chunks of at most
+  // CHUNK_C cells and CHUNK_E pairs
+  m->h_band.clear();
+  m->h_bandpos.assign(nb, -1);
+  m->h_bstart.assign(1, 0);
+  m->h_chunk.assign(1, 0);
+  int in_chunk_c = 0, in_chunk_e = 0;
+  for (int b = 0; b < nb; b++) {
+    const int k = m->h_cnt[b];
+    if (k == 0) continue;
+    if (k > CHUNK_E) return LBM_ERR_UNSUPPORTED;  // more than CHUNK_E markers reach one cell
+    const int i = (int)m->h_band.size();
+    if (in_chunk_c + 1 > CHUNK_C || in_chunk_e + k > CHUNK_E) {
+      m->h_chunk.push_back(i);
+      in_chunk_c = 0;
+      in_chunk_e = 0;
+    }
+    m->h_bandpos[b] = i;
+    m->h_band.push_back(b);
+    m->h_bstart.push_back(m->h_bstart.back() + k);
+    in_chunk_c++;
+    in_chunk_e += k;
   }
-  if (total > m->tlist_cap) {
-    dfree(m->tlist);
-    const size_t cap = std::max<size_t>(total, 1);
-    if (hipMalloc((void **)&m->tlist, sizeof(int32_t) * cap) != hipSuccess) return LBM_ERR_NOMEM;
-    m->tlist_cap = cap;
+  m->nband = (int)m->h_band.size();
+  m->h_chunk.push_back(m->nband);
+  if (m->nband == 0) m->h_chunk.assign(1, 0);
+  m->nchunk = (int)m->h_chunk.size() - 1;
+  {  // (first band cell, first pair) per chunk boundary
+    std::vector<int32_t> cp(2 * m->h_chunk.size());
+    for (size_t q = 0; q < m->h_chunk.size(); q++) {
+      cp[2 * q] = m->h_chunk[q];
+      cp[2 * q + 1] = m->h_bstart[m->h_chunk[q]];
+    }
+    m->h_chunk.swap(cp);
   }
-  hipLaunchKernelGGL(k_ibm_tile_fill, blocks(m->nm), dim3(256), 0, c->stream, B, m->nm, m->mx, m->my, m->mz, m->tstart, m->tcursor, m->tlist);
-  hipLaunchKernelGGL(k_ibm_tile_sort, dim3(nt), dim3(256), 0, c->stream, m->tstart, m->tlist);
+  const size_t nent = (size_t)m->h_bstart.back();
+  if (!grow(m->band, m->band_cap, (size_t)m->nband + 1) ||
+      !grow(m->bstart, m->bstart_cap, (size_t)m->nband + 1) || !grow(m->ent, m->ent_cap, nent) ||
+      !grow(m->ew, m->ew_cap, 3 * nent))
+    return LBM_ERR_NOMEM;
+  if (!grow(m->chunk, m->chunk_cap, m->h_chunk.size())) return LBM_ERR_NOMEM;
+  if (m->nband) HIPCHK(hipMemcpyAsync(m->band, m->h_band.data(), 4 * (size_t)m->nband, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(m->bstart, m->h_bstart.data(), 4 * m->h_bstart.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(m->bandpos, m->h_bandpos.data(), 4 * (size_t)nb, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(m->chunk, m->h_chunk.data(), 4 * m->h_chunk.size(), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_ibm_bin_fill, blocks(np), dim3(256), 0, c->stream, m->nm, m->pidx, m->bandpos, m->bstart, m->cursor, m->ent);
+  hipLaunchKernelGGL(k_ibm_bin_sort, blocks(std::max(m->nband, 1)), dim3(256), 0, c->stream, m->nband, m->bstart, m->st, m->ent, m->ew);
   HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
   return LBM_OK;
 }
 
@@ -712,14 +768,19 @@ static int ibm_iterate(lbm_ctx *c, const double vcm[3], const double vb[3], doub
   lbm_ibm *m = c->ibm;
   const Box &B = m->B;
   const BoxArrays o = box_arrays(m);
-  hipLaunchKernelGGL(k_ibm_interp, blocks(m->nm, 64), dim3(64), 0, c->stream, B, m->nm, m->mx, m->my, m->mz, o, m->ub);
-  for (int it = 0; it < m->m_max; it++) {
-    hipLaunchKernelGGL(k_ibm_force, blocks(m->nm, 64), dim3(64), 0, c->stream, m->nm, m->near, m->brho, m->ub, m->ds,
-                       vcm[0], vcm[1], vcm[2], m->Fb, m->dl + (size_t)2 * m->nm * it);
-    hipLaunchKernelGGL(k_ibm_spread, dim3(B.ntile()), dim3(256), 0, c->stream, B, m->tstart, m->tlist, m->mx, m->my, m->mz, m->Fb, m->ds, o);
-    hipLaunchKernelGGL(k_ibm_interp, blocks(m->nm, 64), dim3(64), 0, c->stream, B, m->nm, m->mx, m->my, m->mz, o, m->ub);
+  ForceArgs fa{m->near, m->ds, m->Fb, m->Fds, m->dsum, vcm[0], vcm[1], vcm[2], m->m_max > 0, 1};
+  const dim3 gi((unsigned)((m->nm + 3) / 4));
+  hipLaunchKernelGGL(k_ibm_interp, gi, dim3(256), 0, c->stream, m->nm, m->pidx, m->st, o, m->ub, fa);
+  fa.first = 0;
+  for (int it = 1; it <= m->m_max; it++) {
+    if (m->nchunk > 0)
+      hipLaunchKernelGGL(k_ibm_spread, dim3(m->nchunk), dim3(256), 0, c->stream, m->chunk, m->band, m->bstart, m->ent,
+                         m->ew, m->Fds, o);
+    fa.force = it < m->m_max;
+    hipLaunchKernelGGL(k_ibm_interp, gi, dim3(256), 0, c->stream, m->nm, m->pidx, m->st, o, m->ub, fa);
   }
-  hipLaunchKernelGGL(k_ibm_finish, dim3(1), dim3(FIN_THREADS), 0, c->stream, m->nm * m->m_max, m->dl, B, m->inside, m->brho,
+  const int npart = (B.ncell() + 255) / 256;
+  hipLaunchKernelGGL(k_ibm_finish, dim3(1), dim3(FIN_THREADS), 0, c->stream, m->nm, m->dsum, npart, m->part, m->ninside,
                      m->vsolid, vcm[0] - vb[0], vcm[1] - vb[1], out);
   HIPCHK(hipGetLastError());
   return LBM_OK;
@@ -745,7 +806,7 @@ static int ibm_move(lbm_ctx *c, const double vcm[3], const double vb[3]) {
       if (e != LBM_OK) return e;
     }
   }
-  // the nearest points, interior cells and tiles follow a body that moves
+  // the stencils, nearest points, interior cells and bins follow a body that moves
   if (m->moving || moved) return ibm_geometry(c);
   return LBM_OK;
 }
@@ -791,11 +852,18 @@ static int ibm_setup(lbm_ctx *c, const lbm_ibm_desc *d, const double *x, const d
   }
   m->ub = (double *)dalloc(m, 24 * nm);
   m->Fb = (double *)dalloc(m, 24 * nm);
-  m->dl = (double *)dalloc(m, 16 * nm * std::max(1, m->m_max));
+  m->Fds = (double *)dalloc(m, 24 * nm);
+  m->dsum = (double *)dalloc(m, 16 * nm);
   m->near = (int32_t *)dalloc(m, 4 * nm);
-  if (!m->ub || !m->Fb || !m->dl || !m->near) return LBM_ERR_NOMEM;
+  m->st.base = (int32_t *)dalloc(m, 12 * nm);
+  m->st.w = (double *)dalloc(m, 96 * nm);
+  m->pidx = (int32_t *)dalloc(m, 256 * nm);
+  if (!m->pidx) return LBM_ERR_NOMEM;
+  if (!m->ub || !m->Fb || !m->Fds || !m->dsum || !m->near || !m->st.base || !m->st.w) return LBM_ERR_NOMEM;
   HIPCHK(hipMemsetAsync(m->ub, 0, 24 * nm, c->stream));
   HIPCHK(hipMemsetAsync(m->Fb, 0, 24 * nm, c->stream));
+  HIPCHK(hipMemsetAsync(m->Fds, 0, 24 * nm, c->stream));
+  HIPCHK(hipMemsetAsync(m->dsum, 0, 16 * nm, c->stream));
   e = place_box(c, b);
   if (e != LBM_OK) return e;
   e = ibm_geometry(c);

@@ -32,3 +32,68 @@ def test_dropin_class_cavity_matches_oracle(oracle_mod, tmp_path, mode):
     fl = case.type == 0
     assert rel_l2_u(tuple(data[1:]), (r_ux, r_uy, r_uz), fl) < 1e-11
     assert np.abs(data[0] - r_rho)[fl].max() < 1e-12
+
+
+EXE_IBM = os.path.join(ROOT, "lattice-boltzmann-method---graduate-fsi-_amd", "host",
+                       "lbm_example_cylinder_ibm")
+
+
+def _oracle_cylinder(oracle_mod, Nx, Ny, D, steps, motion):
+    """The same driver on the oracle: Cylinder_IBM set-up, markers, and per
+    step the V_CM law, the centre and box update (lbm.cpp:1140-1166), IBLBM,
+    Collision, Streaming.  Returns (Cd, Cl, oracle)."""
+    import math
+    PI = cases.PI_REF
+    u_max, Re = 0.04, 20.0
+    cx, cy = Nx / 5.0, Ny / 2.0
+    case, mk, _ = cases.ibm_cylinder(Nx, Ny, 1, D=D, u_max=u_max, Re=Re, center=(cx, cy))
+    o = oracle_mod.Oracle(case.Nx, case.Ny, case.Nz, case.tau)
+    o.set_cells(**case.fields())
+    o.init()
+    vs = PI * D * D * 0.25
+    L = D + 1.0 if motion == "inline" else D
+    v = np.zeros(3)
+    cd, cl = [], []
+    for step in range(1, steps + 1):
+        vb = v.copy()
+        t = float(step)
+        v = np.zeros(3)
+        if motion == "inline":
+            v[0] = -u_max * math.cos(2 * PI * 0.01 * t)
+        elif motion == "transverse":
+            v[1] = -2 * PI * 0.05 * 0.1 * math.sin(2 * PI * 0.05 * t)
+        cx += (v[0] + vb[0]) * 1.0 / 2
+        cy += (v[1] + vb[1]) * 1.0 / 2
+        box = (math.floor(cx - D), math.ceil(cx + D), math.floor(cy - D), math.ceil(cy + D), 1, 1)
+        e, out = o.ibm_step(mk, v, vb, box, m_max=20, vsolid=vs)
+        assert e == 0 and o.collision() == 0
+        o.streaming()
+        q = 0.5 * 1.0 * u_max ** 2 * L
+        cd.append(out[0] / q)
+        cl.append(out[1] / q)
+    return np.array(cd), np.array(cl), o, case
+
+
+@pytest.mark.parametrize("motion", ["fixed", "inline", "transverse"])
+def test_dropin_iblbm_cylinder_matches_oracle(oracle_mod, tmp_path, motion):
+    """The reference's IBM driver (IBLBM, Collision, Streaming) on the drop-in
+    classes against the oracle: C_D, C_L per step and the final fields."""
+    Nx, Ny, D, steps = 60, 40, 8.0, 30
+    out = tmp_path / "cyl.bin"
+    args = [EXE_IBM, str(Nx), str(Ny), str(D), str(steps), str(out), "f64", motion]
+    print("running", args, flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    cd_ref, cl_ref, o, case = _oracle_cylinder(oracle_mod, Nx, Ny, D, steps, motion)
+    data = np.fromfile(out, dtype=np.float64)
+    cd, cl = data[:steps], data[steps:2 * steps]
+    n = (Nx + 2) * (Ny + 2) * 3
+    fields = data[2 * steps:].reshape(4, Nx + 2, Ny + 2, 3)
+    assert fields.size == 4 * n
+    for got, ref, name in ((cd, cd_ref, "Cd"), (cl, cl_ref, "Cl")):
+        err = np.abs(got - ref).max() / max(1.0, np.abs(ref).max())
+        assert err < 1e-11, f"{name}: {err:.3e}"
+    r_rho, r_ux, r_uy, r_uz = o.macro()
+    fl = case.type == 0
+    assert rel_l2_u(tuple(fields[1:]), (r_ux, r_uy, r_uz), fl) < 1e-11
+    assert np.abs(fields[0] - r_rho)[fl].max() < 1e-12
