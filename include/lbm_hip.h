@@ -121,6 +121,37 @@ int lbm_reduce_fp64(lbm_ctx *ctx, int32_t what, double *out);
 int lbm_export_links(lbm_ctx *ctx, int16_t *type, int32_t *src,
                      uint8_t *kind);
 
+/* ---- thermal branch (#define ENERGY) ---------------------------------------
+ * Same-lattice D3Q19 energy populations g with viscous heating
+ * (ConvLBM, lbm.cpp:121-177), Dirichlet (TYPE_T) and Neumann (TYPE_Q)
+ * walls (lbm.cpp:276-288, 965-1028), T and q moments (lbm.cpp:777-798)
+ * and the optional Boussinesq force of Rayleigh_Benard (lbm.cpp:803-806).
+ * FP64 and FP32 storage. */
+typedef struct lbm_thermal_desc {
+  double tau_g;     /* g relaxation time, 0.5 + alpha/cs2 (LBM.h:155)       */
+  double nu;        /* viscosity in the dissipation Phi = 2 nu rho S:S      *
+                     * (LBM.h nu; cs2 (tau - 1/2) in the reference's cases) */
+  double cp;        /* specific heat, k/(rho0 alpha) (LBM.h:157)             */
+  double j_dot_n;   /* Neumann wall flux (LBM.h:161)                         */
+  double g_beta;    /* Boussinesq: F_y = g_beta (T - T_ref) after MacroProp  */
+  double T_ref;     /* (Tb + Tt)/2 in the reference                          */
+  double T0;        /* FP32 storage keeps g - w T0 (NaN: the mean set-up T   *
+                     * of the TYPE_F cells)                                  */
+  int32_t boussinesq;
+  int32_t reserved[3];
+} lbm_thermal_desc;
+
+/* Writes of lb.fluid[i][j][k].T and .type_Energy (e.g. lbm.cpp:1551-1556)
+ * after lbm_upload_cells and before lbm_init; switches the context to the
+ * thermal step.  type_energy may be NULL (TYPE_F everywhere). */
+int lbm_upload_thermal(lbm_ctx *ctx, const lbm_thermal_desc *desc, const double *T,
+                       const int16_t *type_energy);
+/* Reads of lb.fluid[i][j][k].T and .q (as of the last MacroProp). */
+int lbm_download_thermal(lbm_ctx *ctx, double *T, double *qx, double *qy, double *qz);
+/* Reads of lb.fluid[i][j][k].g, ncells*19 (valid in the same states as
+ * lbm_download_f). */
+int lbm_download_g(lbm_ctx *ctx, double *g);
+
 /* ---- implicit diffuse-interface IBM (IBLBM, lbm.cpp:1135-1417) ---------- */
 typedef struct lbm_ibm_desc {
   int32_t nmarkers; /* Lagrangian markers (number_nodes, LBM.h:340)           */

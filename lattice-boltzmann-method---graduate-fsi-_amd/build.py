@@ -39,6 +39,7 @@ SOURCES = [
     os.path.join(HERE, "csrc", "lbm_capi.hip"),
     os.path.join(HERE, "csrc", "lbm_probe.hip"),
     os.path.join(HERE, "csrc", "lbm_ibm.hip"),
+    os.path.join(HERE, "csrc", "lbm_thermal.hip"),
 ]
 HOST_SOURCES = [
     os.path.join(HERE, "host", "LBM.cpp"),

@@ -18,6 +18,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 TYPE_F, TYPE_S, TYPE_OUT, TYPE_IN = 0, 1, 2, 3
+TYPE_T, TYPE_Q = 6, 7
 # LBM.h:74 and lbm.cpp:1859 use this literal for pi
 PI_REF = 3.14159265
 
@@ -166,6 +167,136 @@ def couette(Nx: int = 2, Ny: int = 30, Nz: int = 2, tau: float = 0.9, u_wall: fl
     t[:, -1, :] = TYPE_S
     ux[:, -1, :] = u_wall
     return Case("couette", Nx, Ny, Nz, tau, t, rho, ux, uy, uz, params=dict(u_wall=u_wall))
+
+
+@dataclass
+class Thermal:
+    """The thermal fields a setup writes (lb.fluid[i][j][k].T, .type_Energy)
+    and the ENERGY case constants (LBM.h:150-161, 206-220)."""
+    T: np.ndarray
+    type_energy: np.ndarray
+    tau_g: float
+    nu: float
+    cp: float
+    j_dot_n: float = 0.0
+    boussinesq: bool = False
+    g_beta: float = 0.0
+    T_ref: float = 0.0
+    params: dict = field(default_factory=dict)
+
+
+def thermal_constants(nu: float, Pr: float, rho0: float = 1.0):
+    """alpha = nu/Pr, tau_g = 0.5 + alpha/cs2, k = cs2 tau_g, cp = k/(rho0
+    alpha) (LBM.h:153-156)."""
+    cs2 = 1.0 / 3.0
+    alpha = nu / Pr
+    tau_g = 0.5 + alpha / (cs2 * 1.0)
+    k = cs2 * tau_g * 1.0
+    cp = k / (rho0 * alpha)
+    return alpha, tau_g, k, cp
+
+
+def couette_thermal(Nx: int = 2, Ny: int = 20, Nz: int = 2, tau: float = 0.9, Re: float = 10.0,
+                    Pr: float = 0.7, Br: float = 10.0, Tb: float = 0.5):
+    """main_setup_Couette with ENERGY (lbm.cpp:1536-1584, LBM.h:165-220):
+    walls at j = 0, Ny+1 (TYPE_S, type_Energy TYPE_T), the top one moving
+    with u_wall = Re nu/(Ny+1); T = Tb on the fluid and the bottom wall,
+    Tt = Tb + u_wall^2/(cp Ec) on the top wall, Ec = Br/Pr.  Returns
+    (case, thermal)."""
+    cs2 = 1.0 / 3.0
+    nu = cs2 * (tau - 0.5)
+    u_wall = Re * nu / (Ny + 1)
+    alpha, tau_g, k, cp = thermal_constants(nu, Pr)
+    Ec = Br / Pr
+    dT = u_wall ** 2 / (cp * Ec)
+    Tt = Tb + dT
+    t, rho, ux, uy, uz = _blank(Nx, Ny, Nz)
+    t[:, 0, :] = TYPE_S
+    t[:, -1, :] = TYPE_S
+    ux[:, -1, :] = u_wall
+    T = np.full(t.shape, Tb)
+    T[:, -1, :] = Tt
+    te = np.zeros(t.shape, np.int16)
+    te[:, 0, :] = TYPE_T
+    te[:, -1, :] = TYPE_T
+    case = Case("couette_thermal", Nx, Ny, Nz, tau, t, rho, ux, uy, uz, params=dict(u_wall=u_wall))
+    th = Thermal(T, te, tau_g, nu, cp, params=dict(Tb=Tb, Tt=Tt, dT=dT, Pr=Pr, Br=Br, alpha=alpha,
+                                                   u_wall=u_wall))
+    return case, th
+
+
+def conduction_channel(Nx: int = 2, Ny: int = 12, Nz: int = 2, tau: float = 0.8, Pr: float = 0.7,
+                       bottom: int = TYPE_T, top: int = TYPE_T, Tb: float = 1.0, Tt: float = 0.5,
+                       T_fluid=None, seed: int = 3):
+    """Fluid at rest between walls at j = 0 and j = Ny+1 (TYPE_S) with
+    type_Energy `bottom` / `top` (TYPE_T Dirichlet or TYPE_Q Neumann), set-up
+    temperatures Tb, Tt; x and z periodic.  The fluid starts at T_fluid
+    (default: a seeded profile between Tt and Tb)."""
+    cs2 = 1.0 / 3.0
+    nu = cs2 * (tau - 0.5)
+    alpha, tau_g, k, cp = thermal_constants(nu, Pr)
+    t, rho, ux, uy, uz = _blank(Nx, Ny, Nz)
+    t[:, 0, :] = TYPE_S
+    t[:, -1, :] = TYPE_S
+    if T_fluid is None:
+        xi = counter_uniform(seed, np.arange(t.size, dtype=np.uint64)).reshape(t.shape)
+        T = 0.5 * (Tb + Tt) + 0.25 * (Tb - Tt) * xi
+    else:
+        T = np.full(t.shape, float(T_fluid))
+    T[:, 0, :] = Tb
+    T[:, -1, :] = Tt
+    te = np.zeros(t.shape, np.int16)
+    te[:, 0, :] = bottom
+    te[:, -1, :] = top
+    case = Case("conduction", Nx, Ny, Nz, tau, t, rho, ux, uy, uz)
+    return case, Thermal(T, te, tau_g, nu, cp, params=dict(Tb=Tb, Tt=Tt, alpha=alpha))
+
+
+def rayleigh_benard(Nx: int = 12, Ny: int = 8, Nz: int = 4, tau: float = 0.8, Pr: float = 0.71,
+                    Tb: float = 2.0, Tt: float = 1.0, g_beta: float = 1e-3):
+    """The pattern of main_setup_RayleighBenard (lbm.cpp:1586-1640): all
+    ghost faces TYPE_S, the i and k faces Neumann (TYPE_Q), the j faces
+    Dirichlet (TYPE_T, set last so they win on the edges) with the bottom at
+    Tb and the top at Tt; fluid at Tt with F_y = g_beta (T - T*), T* =
+    (Tb + Tt)/2, updated by every MacroProp (Boussinesq)."""
+    cs2 = 1.0 / 3.0
+    nu = cs2 * (tau - 0.5)
+    alpha, tau_g, k, cp = thermal_constants(nu, Pr)
+    t, rho, ux, uy, uz = _blank(Nx, Ny, Nz)
+    te = np.zeros(t.shape, np.int16)
+    for sl in ((slice(None), slice(None), 0), (slice(None), slice(None), -1),
+               (0, slice(None), slice(None)), (-1, slice(None), slice(None))):
+        t[sl] = TYPE_S
+        te[sl] = TYPE_Q
+    for sl in ((slice(None), 0, slice(None)), (slice(None), -1, slice(None))):
+        t[sl] = TYPE_S
+        te[sl] = TYPE_T
+    T = np.full(t.shape, Tt)
+    T[:, 0, :] = Tb
+    T[:, -1, :] = Tt
+    T_ref = 0.5 * (Tb + Tt)
+    Fy = np.zeros(t.shape)
+    Fy[t == TYPE_F] = g_beta * (T[t == TYPE_F] - T_ref)
+    case = Case("rayleigh_benard", Nx, Ny, Nz, tau, t, rho, ux, uy, uz, Fx=np.zeros(t.shape), Fy=Fy,
+                Fz=np.zeros(t.shape))
+    th = Thermal(T, te, tau_g, nu, cp, boussinesq=True, g_beta=g_beta, T_ref=T_ref,
+                 params=dict(Tb=Tb, Tt=Tt))
+    return case, th
+
+
+def random_thermal_case(Nx: int, Ny: int, Nz: int, seed: int = 5, tau: float = 0.8,
+                        Pr: float = 0.7):
+    """random_mask_case with temperatures: every cell T in [0.4, 0.9), every
+    solid a Dirichlet (TYPE_T) wall.  Exercises the thermal pull (wrap,
+    frozen links next to inlet cells, moving Dirichlet walls, the averaged
+    wall temperature)."""
+    case = random_mask_case(Nx, Ny, Nz, seed=seed, tau=tau)
+    rng = np.random.default_rng(seed + 1000)
+    T = 0.4 + 0.5 * rng.random(case.shape)
+    te = np.where(case.type == TYPE_S, TYPE_T, TYPE_F).astype(np.int16)
+    nu = (tau - 0.5) / 3.0
+    alpha, tau_g, k, cp = thermal_constants(nu, Pr)
+    return case, Thermal(T, te, tau_g, nu, cp)
 
 
 def poiseuille(Nx: int = 2, Ny: int = 30, Nz: int = 2, tau: float = 0.9, g: float = 1e-6) -> Case:

@@ -105,6 +105,7 @@ static int download_real(lbm_ctx *c, double *dst, const void *src) {
 
 static void free_all(lbm_ctx *c) {
   ibm_free(c);
+  thermal_free(c);
   void *ptrs[] = {c->cur, c->oth, c->type, c->mask, c->links, c->uw[0], c->uw[1], c->uw[2],
                   c->mac[0], c->mac[1], c->mac[2], c->mac[3], c->F[0], c->F[1], c->F[2],
                   c->status, c->first_bad, c->red_partial, c->red_out};
@@ -218,6 +219,7 @@ int lbm_upload_cells(lbm_ctx *c, const int16_t *type, const double *rho, const d
   // the mask it marked)
   ibm_free(c);
   c->box = BoxArgs{};
+  thermal_free(c);  // lbm_upload_thermal follows when the case is thermal
   const size_t n = c->g.ncell;
   const size_t nref = (size_t)c->g.nx * c->g.ny * c->g.nz;
   c->h_type.assign(n, (int16_t)-1);  // row padding
@@ -288,11 +290,19 @@ int lbm_init(lbm_ctx *c) {
   // 1. kernel mask, bounce-back links and wall velocities from the setup
   //    velocities of the solids (what the first Streaming reads)
   HIPCHK(launch_tables(L, c->mask, c->links, c->uw[0], c->uw[1], c->uw[2], c->type));
-  // 2. f = f* = feq (LBM::Init)
-  HIPCHK(launch_init(L, c->cur, c->oth));
+  // 2. f = f* = feq (LBM::Init); with ENERGY also g = g* = geq and the
+  //    thermal wall tables
+  if (c->thermal) {
+    if (int e = thermal_init(c)) return e;
+  } else {
+    HIPCHK(launch_init(L, c->cur, c->oth));
+  }
   if (int e = ibm_remark(c)) return e;
   HIPCHK(hipStreamSynchronize(c->stream));
-  c->walls_from_setup = any_solid ? 1 : 0;
+  // MacroProps after which the wall tables change: the first (solids' rho,
+  // u, T become moments of their populations); with the Boussinesq force
+  // also the second (the solids' F_y set by the first enters their u)
+  c->walls_from_setup = any_solid ? (c->thermal && c->td.boussinesq ? 2 : 1) : 0;
   c->backeq_solid_pending = 0;
   {
     const uint32_t plane = c->g.plane, top = (c->g.nz - 1) * plane;
@@ -307,13 +317,25 @@ int lbm_init(lbm_ctx *c) {
 // After the first MacroProp: the solids' rho,u become the moments of their
 // (never changing) Init populations, and the wall velocities of every later
 // Streaming are averaged from those (lbm.cpp:941-962).
-static int walls_from_moments(lbm_ctx *c) {
+// solids_done: this MacroProp already covered the solids (lbm_macro); the
+// fused step's MacroProp covers fluid cells only.  (The thermal MacroProp
+// with the Boussinesq force is not idempotent: it rewrites the solids' F.)
+static int walls_from_moments(lbm_ctx *c, bool solids_done = false) {
   if (!c->walls_from_setup) return LBM_OK;
   LaunchCtx L = launch_ctx(c);
   L.macro_sel = 16;  // TYPE_S only
-  HIPCHK(launch_macro(L, c->cur));
+  if (!solids_done) {
+    if (c->thermal) {
+      if (int e = thermal_launch(c, 4, 16)) return e;
+    } else {
+      HIPCHK(launch_macro(L, c->cur));
+    }
+  }
   HIPCHK(launch_tables(L, c->mask, c->links, c->uw[0], c->uw[1], c->uw[2], c->type));
-  c->walls_from_setup = 0;
+  if (c->thermal) {
+    if (int e = thermal_tables(c)) return e;
+  }
+  c->walls_from_setup--;
   return LBM_OK;
 }
 
@@ -329,7 +351,11 @@ int lbm_collide(lbm_ctx *c) {
     L.force_mode = 3;
     ibm_consume(c);
   }
-  HIPCHK(launch_collide(L, c->cur));
+  if (c->thermal) {
+    if (int e = thermal_launch(c, 3)) return e;
+  } else {
+    HIPCHK(launch_collide(L, c->cur));
+  }
   c->state = ST_FSTAR;
   return LBM_OK;
 }
@@ -338,7 +364,11 @@ int lbm_stream(lbm_ctx *c) {
   if (!c) return LBM_ERR_ARG;
   if (c->state != ST_FSTAR) return LBM_ERR_STATE;
   HIPCHK(hipSetDevice(c->dev));
-  HIPCHK(launch_stream(launch_ctx(c), c->cur, c->oth, STREAM_ONLY));
+  if (c->thermal) {
+    if (int e = thermal_launch(c, 2)) return e;
+  } else {
+    HIPCHK(launch_stream(launch_ctx(c), c->cur, c->oth, STREAM_ONLY));
+  }
   std::swap(c->cur, c->oth);
   c->state = ST_F;
   return after_stream(c);
@@ -348,8 +378,30 @@ int lbm_macro(lbm_ctx *c) {
   if (!c) return LBM_ERR_ARG;
   if (c->state != ST_F || ibm_active(c)) return LBM_ERR_STATE;
   HIPCHK(hipSetDevice(c->dev));
-  HIPCHK(launch_macro(launch_ctx(c), c->cur));
-  return walls_from_moments(c);
+  if (c->thermal) {
+    if (int e = thermal_launch(c, 4, 1 | 16)) return e;
+  } else {
+    HIPCHK(launch_macro(launch_ctx(c), c->cur));
+  }
+  return walls_from_moments(c, true);
+}
+
+// one launch of the step sequence, plain or thermal
+static int step_launch(lbm_ctx *c, const LaunchCtx &L, int mode) {
+  if (c->thermal) {
+    const int tm = mode == STREAM_COLLIDE ? 0 : mode == STREAM_COLLIDE_MACRO ? 1 : mode == STREAM_ONLY ? 2 : 5;
+    if (int e = thermal_launch(c, tm)) return e;
+  } else {
+    HIPCHK(launch_stream(L, c->cur, c->oth, mode));
+  }
+  std::swap(c->cur, c->oth);
+  return LBM_OK;
+}
+
+static int collide_launch(lbm_ctx *c, const LaunchCtx &L) {
+  if (c->thermal) return thermal_launch(c, 3);
+  HIPCHK(launch_collide(L, c->cur));
+  return LBM_OK;
 }
 
 int lbm_step(lbm_ctx *c, int32_t nsteps, int32_t emit_macro) {
@@ -359,25 +411,22 @@ int lbm_step(lbm_ctx *c, int32_t nsteps, int32_t emit_macro) {
   if (nsteps == 0) return LBM_OK;
   HIPCHK(hipSetDevice(c->dev));
   LaunchCtx L = launch_ctx(c);
-  if (c->state == ST_F) HIPCHK(launch_collide(L, c->cur));
-  if (c->walls_from_setup || c->backeq_solid_pending) {
-    // first step with the setup wall velocities, then switch the tables
-    HIPCHK(launch_stream(L, c->cur, c->oth, STREAM_MACRO));
-    std::swap(c->cur, c->oth);
+  int e;
+  if (c->state == ST_F && (e = collide_launch(c, L)) != LBM_OK) return e;
+  while (c->walls_from_setup || c->backeq_solid_pending) {
+    // steps whose MacroProp changes the wall tables run unfused
+    if ((e = step_launch(c, L, STREAM_MACRO)) != LBM_OK) return e;
     c->state = ST_F;
-    int e = after_stream(c);
+    e = after_stream(c);
     if (e == LBM_OK) e = walls_from_moments(c);
     if (e != LBM_OK) return e;
     if (--nsteps == 0) return LBM_OK;
     L = launch_ctx(c);
-    HIPCHK(launch_collide(L, c->cur));
+    if ((e = collide_launch(c, L)) != LBM_OK) return e;
   }
-  for (int s = 0; s < nsteps - 1; s++) {
-    HIPCHK(launch_stream(L, c->cur, c->oth, STREAM_COLLIDE));
-    std::swap(c->cur, c->oth);
-  }
-  HIPCHK(launch_stream(L, c->cur, c->oth, STREAM_MACRO));
-  std::swap(c->cur, c->oth);
+  for (int s = 0; s < nsteps - 1; s++)
+    if ((e = step_launch(c, L, STREAM_COLLIDE)) != LBM_OK) return e;
+  if ((e = step_launch(c, L, STREAM_MACRO)) != LBM_OK) return e;
   c->state = ST_F;
   return LBM_OK;
 }
@@ -386,23 +435,21 @@ int lbm_time_steps(lbm_ctx *c, int32_t nsteps, double *ms) {
   if (!c || nsteps < 1 || !ms) return LBM_ERR_ARG;
   if (c->state == ST_NONE || ibm_active(c)) return LBM_ERR_STATE;
   HIPCHK(hipSetDevice(c->dev));
-  if (c->walls_from_setup || c->backeq_solid_pending) {
+  while (c->walls_from_setup || c->backeq_solid_pending) {
     int e = lbm_step(c, 1, 1);
     if (e != LBM_OK) return e;
   }
   const LaunchCtx L = launch_ctx(c);
-  if (c->state == ST_F) HIPCHK(launch_collide(L, c->cur));
+  int er;
+  if (c->state == ST_F && (er = collide_launch(c, L)) != LBM_OK) return er;
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   HIPCHK(hipEventRecord(e0, c->stream));
-  for (int s = 0; s < nsteps; s++) {
-    HIPCHK(launch_stream(L, c->cur, c->oth, STREAM_COLLIDE));
-    std::swap(c->cur, c->oth);
-  }
+  for (int s = 0; s < nsteps; s++)
+    if ((er = step_launch(c, L, STREAM_COLLIDE)) != LBM_OK) return er;
   HIPCHK(hipEventRecord(e1, c->stream));
-  HIPCHK(launch_stream(L, c->cur, c->oth, STREAM_MACRO));
-  std::swap(c->cur, c->oth);
+  if ((er = step_launch(c, L, STREAM_MACRO)) != LBM_OK) return er;
   c->state = ST_F;
   HIPCHK(hipEventSynchronize(e1));
   float t = 0.f;
@@ -436,6 +483,7 @@ int lbm_status(lbm_ctx *c, int64_t *first_bad_cell) {
     }
   }
   if (st & 2) return LBM_ERR_DENSITY_RANGE;
+  if (st & 4) return LBM_ERR_TEMPERATURE_RANGE;
   if (st & 1) return LBM_ERR_FEQ_NEGATIVE;
   return LBM_OK;
 }
@@ -558,7 +606,9 @@ int32_t lbm_storage(const lbm_ctx *c) { return c ? c->storage : -1; }
 
 double lbm_bytes_per_cell(const lbm_ctx *c) {
   if (!c) return 0.0;
-  // A/B pull: 19 populations read + 19 written + the 1-byte mask
+  // A/B pull: 19 populations read + 19 written + the 1-byte mask; with
+  // ENERGY also 19 g read + written and the previous Phi, u read + written
+  if (c->thermal) return 4.0 * 19.0 * (double)c->esize + 8.0 * (double)c->csize + 1.0;
   return 2.0 * 19.0 * (double)c->esize + 1.0;
 }
 

@@ -48,6 +48,18 @@ struct lbm_ctx {
   std::vector<int16_t> h_type;  // device order
   lbm_ibm *ibm = nullptr;       // set by lbm_ibm_setup
   lbm::BoxArgs box{};           // the IBM box as force mode 3 reads it
+  // ---- thermal branch (lbm_thermal.hip); the population buffers then hold
+  // 38 planes: f in 0..18, g in 19..37
+  int thermal = 0;
+  lbm_thermal_desc td{};
+  double T0 = 0;                     // storage shift of g (FP32)
+  void *T = nullptr, *qv[3] = {nullptr, nullptr, nullptr};  // macro T, q
+  void *phi = nullptr, *up[3] = {nullptr, nullptr, nullptr};  // previous collision's Phi, u
+  void *rho_w = nullptr;             // rho of the last collision (wall cells)
+  void *Tw = nullptr;                // wall temperature of wall cells
+  uint64_t *tlinks = nullptr;        // bit m: Dirichlet link m; bit 32+m: Neumann
+  int walls_refresh = 0;             // MacroProps left that change the wall tables
+  std::vector<int16_t> h_typeE;      // device order
 };
 
 #define HIPCHK(x)                              \
@@ -55,6 +67,15 @@ struct lbm_ctx {
     hipError_t e_ = (x);                       \
     if (e_ != hipSuccess) return LBM_ERR_HIP;  \
   } while (0)
+
+// thermal branch (lbm_thermal.hip)
+void thermal_free(lbm_ctx *c);
+int thermal_init(lbm_ctx *c);          // tables, f and g = equilibrium
+int thermal_tables(lbm_ctx *c);        // Dirichlet/Neumann links, wall temperature
+// mode: 0 fused stream+collide, 1 as 0 + emit macros, 2 stream only,
+// 3 collide in place, 4 MacroProp of the cells with mask bits `sel`
+int thermal_launch(lbm_ctx *c, int mode, uint8_t sel = 1);
+int thermal_download_g(lbm_ctx *c, double *g_dev);
 
 // launch parameters of the context's current state (lbm_capi.hip)
 lbm::LaunchCtx launch_ctx(const lbm_ctx *c);

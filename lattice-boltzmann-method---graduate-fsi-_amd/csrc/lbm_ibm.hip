@@ -879,6 +879,7 @@ int lbm_ibm_setup(lbm_ctx *c, const lbm_ibm_desc *d, const double *x, const doub
                   const double *ds) {
   if (!c || !d || !x || !y || !z || !nx || !ny || !nz || !ds) return LBM_ERR_ARG;
   if (c->state == ST_NONE) return LBM_ERR_STATE;
+  if (c->thermal) return LBM_ERR_UNSUPPORTED;  // the reference's IBM cases have no ENERGY
   if (d->nmarkers < 1 || d->m_max < 0) return LBM_ERR_ARG;
   if (d->follow && !(d->half >= 0)) return LBM_ERR_ARG;
   const int e = ibm_setup(c, d, x, y, z, nx, ny, nz, ds);

@@ -68,6 +68,20 @@ class _IbmDesc(C.Structure):
     ]
 
 
+class _ThermalDesc(C.Structure):
+    _fields_ = [
+        ("tau_g", C.c_double),
+        ("nu", C.c_double),
+        ("cp", C.c_double),
+        ("j_dot_n", C.c_double),
+        ("g_beta", C.c_double),
+        ("T_ref", C.c_double),
+        ("T0", C.c_double),
+        ("boussinesq", C.c_int32),
+        ("reserved", C.c_int32 * 3),
+    ]
+
+
 # name -> (restype, argtypes) for every entry point of include/lbm_hip.h
 _P = C.c_void_p
 _D = C.POINTER(C.c_double)
@@ -104,6 +118,9 @@ _SIGNATURES = {
     "lbm_ibm_run": (C.c_int, [_P, C.c_int32, _D, _D]),
     "lbm_ibm_markers": (C.c_int, [_P, _D, _D, _D, _D, _D]),
     "lbm_ibm_box": (C.c_int, [_P, C.POINTER(C.c_int32), _D]),
+    "lbm_upload_thermal": (C.c_int, [_P, C.POINTER(_ThermalDesc), _D, C.POINTER(C.c_int16)]),
+    "lbm_download_thermal": (C.c_int, [_P, _D, _D, _D, _D]),
+    "lbm_download_g": (C.c_int, [_P, _D]),
 }
 
 _lib = None
@@ -265,6 +282,32 @@ class Lattice:
         out = np.zeros(3, np.float64)
         _check(self._lib.lbm_reduce_fp64(self._h, int(what), _dp(out)), "lbm_reduce_fp64")
         return out[:3] if what == REDUCE_MOMENTUM else float(out[0])
+
+    # -- thermal branch (ENERGY, lbm.cpp:121-178, 263-289, 636-653, 777-806) ----
+    def upload_thermal(self, T=None, type_energy=None, *, tau_g, nu, cp, j_dot_n=0.0,
+                       boussinesq=False, g_beta=0.0, T_ref=0.0, T0=float("nan")):
+        """After upload(), before init(): lb.fluid[i][j][k].T and .type_Energy
+        and the case's thermal constants."""
+        d = _ThermalDesc(float(tau_g), float(nu), float(cp), float(j_dot_n), float(g_beta),
+                         float(T_ref), float(T0), int(bool(boussinesq)), (C.c_int32 * 3)())
+        Ta = None if T is None else self._f64(T)
+        te = None
+        if type_energy is not None:
+            te = np.ascontiguousarray(np.broadcast_to(np.asarray(type_energy, np.int16), self.shape))
+        _check(self._lib.lbm_upload_thermal(self._h, C.byref(d), _dp(Ta),
+                                            None if te is None else te.ctypes.data_as(C.POINTER(C.c_int16))),
+               "lbm_upload_thermal")
+
+    def thermal(self):
+        """(T, qx, qy, qz) as of the last MacroProp."""
+        out = [np.empty(self.shape) for _ in range(4)]
+        _check(self._lib.lbm_download_thermal(self._h, *[_dp(a) for a in out]), "lbm_download_thermal")
+        return tuple(out)
+
+    def g_populations(self):
+        a = np.empty(self.shape + (19,))
+        _check(self._lib.lbm_download_g(self._h, _dp(a)), "lbm_download_g")
+        return a
 
     # -- immersed boundary (IBLBM, lbm.cpp:1135-1417) ---------------------------
     def ibm_setup(self, markers: dict, box, m_max: int = 20, vsolid: float = 0.0, moving=False,

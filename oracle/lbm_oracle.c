@@ -12,11 +12,19 @@
  *
  * Deliberate differences from the reference, none of which changes a value:
  *  - std::terminate() becomes an error code (ORC_ERR_*); the sweep finishes.
- *  - Streaming builds no per-cell std::vector lists (lbm.cpp:207-249): they
- *    feed only the thermal Neumann wall, which this file does not restate.
+ *  - Streaming builds no per-cell std::vector lists (lbm.cpp:207-249); the
+ *    thermal Neumann wall derives the same index lists from the cell types.
  *    The wall velocity (lbm.cpp:941-962) is computed once per wall cell
  *    instead of once per wall link; it is the same value each time.
  *  - Compile-time case flags become runtime fields.
+ * Where the reference itself is not deterministic or reads uninitialised
+ * memory, this file fixes one behaviour (thermal branch only):
+ *  - F_vis (LBM.h:595) has no initialiser; here it starts at 0.
+ *  - The Neumann wall (lbm.cpp:281-287, 983-1028) reads the cell's g while
+ *    other cells are still pushing into it, in OpenMP order; here every push
+ *    of the pass is done first (a second sweep over the wall cells).  Its
+ *    writes of the solid cells' T are not kept: MacroProp overwrites them
+ *    with the moments of the solids' g before any later read.
  */
 #include "lbm_oracle.h"
 
@@ -49,7 +57,20 @@ struct orc_lbm {
   double cs2;
   orc_cell ***fluid; /* [Nx+2][Ny+2][Nz+2], lbm.cpp:17-28 */
   double *Fm;        /* implicit-IBM Fm[3] per cell (LBM.h:571-573) */
+  struct orc_thermal *th; /* ENERGY fields (LBM.h:590-596), NULL when off */
 };
+
+/* The ENERGY members of LATTICE (LBM.h:590-596) as arrays over LIN(i,j,k),
+ * and the case constants the thermal branch reads (LBM.h:150-161,
+ * 206-220, 233-241). */
+typedef struct orc_thermal {
+  double tau_g, nu, cp, j_dot_n;
+  int boussinesq;       /* Rayleigh_Benard: F[1] = g_beta (T - T_ref) in MacroProp */
+  double g_beta, T_ref; /* lbm.cpp:803-806, T_ref = (Tb + Tt)/2 */
+  double *g, *g_star, *F_vis; /* q per cell */
+  double *T, *q;              /* 1 and 3 per cell */
+  int16_t *typeE;             /* type_Energy */
+} orc_thermal;
 
 /* D3Q19, LBM.h:542-552 */
 static const int d3q19_cx[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0};
@@ -123,6 +144,11 @@ void orc_destroy(orc_lbm *lb) {
   }
   free(lb->fluid);
   free(lb->Fm);
+  if (lb->th) {
+    free(lb->th->g); free(lb->th->g_star); free(lb->th->F_vis);
+    free(lb->th->T); free(lb->th->q); free(lb->th->typeE);
+    free(lb->th);
+  }
   free(lb);
 }
 
@@ -233,6 +259,197 @@ double orc_feq(const orc_lbm *lb, double rho, double ux, double uy, double uz, i
   return calc_feq(lb, &c, l);
 }
 
+/* ---- thermal branch (ENERGY): LBM.h:590-596, lbm.cpp:121-178, 263-289,
+ * 636-653, 777-806, 964-1028 ------------------------------------------- */
+
+int orc_thermal_enable(orc_lbm *lb, double tau_g, double nu, double cp, double j_dot_n,
+                       int boussinesq, double g_beta, double T_ref) {
+  if (lb->th) return 0;
+  const int64_t n = orc_ncells(lb);
+  orc_thermal *th = (orc_thermal *)calloc(1, sizeof(orc_thermal));
+  th->tau_g = tau_g; th->nu = nu; th->cp = cp; th->j_dot_n = j_dot_n;
+  th->boussinesq = boussinesq; th->g_beta = g_beta; th->T_ref = T_ref;
+  th->g = (double *)calloc((size_t)n * lb->q, sizeof(double));
+  th->g_star = (double *)calloc((size_t)n * lb->q, sizeof(double));
+  th->F_vis = (double *)calloc((size_t)n * lb->q, sizeof(double));
+  th->T = (double *)calloc((size_t)n, sizeof(double));
+  th->q = (double *)calloc((size_t)n * 3, sizeof(double));
+  th->typeE = (int16_t *)calloc((size_t)n, sizeof(int16_t)); /* TYPE_F, LBM.h:594 */
+  lb->th = th;
+  return 0;
+}
+
+void orc_set_thermal(orc_lbm *lb, const double *T, const int16_t *typeE) {
+  const int64_t n = orc_ncells(lb);
+  if (!lb->th) return;
+  for (int64_t c = 0; c < n; c++) {
+    if (T) lb->th->T[c] = T[c];
+    if (typeE) lb->th->typeE[c] = typeE[c];
+  }
+}
+
+void orc_get_thermal(const orc_lbm *lb, double *T, double *qx, double *qy, double *qz) {
+  const int64_t n = orc_ncells(lb);
+  if (!lb->th) return;
+  for (int64_t c = 0; c < n; c++) {
+    if (T) T[c] = lb->th->T[c];
+    if (qx) qx[c] = lb->th->q[3 * c + 0];
+    if (qy) qy[c] = lb->th->q[3 * c + 1];
+    if (qz) qz[c] = lb->th->q[3 * c + 2];
+  }
+}
+
+void orc_get_g(const orc_lbm *lb, double *g) {
+  if (!lb->th) return;
+  memcpy(g, lb->th->g, sizeof(double) * (size_t)orc_ncells(lb) * lb->q);
+}
+
+void orc_set_g(orc_lbm *lb, const double *g) {
+  if (!lb->th) return;
+  memcpy(lb->th->g, g, sizeof(double) * (size_t)orc_ncells(lb) * lb->q);
+}
+
+/* Determine_Wall_Temperature, lbm.cpp:965-981: the mean T of the cell's
+ * TYPE_S neighbours (l = 0 included), NaN when there is none. */
+double orc_wall_temperature(const orc_lbm *lb, int i, int j, int k) {
+  int num = 0;
+  double Twall = 0;
+  for (int l = 0; l < lb->q; l++) {
+    int i_nb = (i + lb->cx[l] + (lb->Nx + 2)) % (lb->Nx + 2);
+    int j_nb = (j + lb->cy[l] + (lb->Ny + 2)) % (lb->Ny + 2);
+    int k_nb = (k + lb->cz[l] + (lb->Nz + 2)) % (lb->Nz + 2);
+    if (lb->fluid[i_nb][j_nb][k_nb].type == ORC_TYPE_S) {
+      Twall += lb->th->T[LIN(lb, i_nb, j_nb, k_nb)];
+      num++;
+    }
+  }
+  Twall /= num;
+  return Twall;
+}
+
+/* Calc_geq, lbm.cpp:636-653.  TYPE_S: rho of cell (i,j,k), the wall
+ * temperature of (i,j,k) and the wall velocity uw; otherwise the cell's own
+ * rho, u and T. */
+static double calc_geq(const orc_lbm *lb, int i, int j, int k, int l, short type,
+                       const double uw[3]) {
+  const double cs2 = lb->cs2;
+  const orc_cell *c = &lb->fluid[i][j][k];
+  double g_eq;
+  if (type == ORC_TYPE_S) {
+    double Twall = orc_wall_temperature(lb, i, j, k);
+    double u_dot_u = pow(uw[0], 2) + pow(uw[1], 2) + pow(uw[2], 2);
+    double c_dot_u = uw[0] * lb->cx[l] + uw[1] * lb->cy[l] + uw[2] * lb->cz[l];
+    g_eq = c->density * lb->w[l] * Twall *
+           (1 + c_dot_u / cs2 + pow(c_dot_u, 2) / (2 * cs2 * cs2) - u_dot_u / (2 * cs2) +
+            pow(c_dot_u, 3) / (6 * pow(cs2, 3)) - c_dot_u * u_dot_u / (2 * cs2 * cs2));
+  } else {
+    double u_dot_u = pow(c->ux, 2) + pow(c->uy, 2) + pow(c->uz, 2);
+    double c_dot_u = c->ux * lb->cx[l] + c->uy * lb->cy[l] + c->uz * lb->cz[l];
+    g_eq = c->density * lb->w[l] * lb->th->T[LIN(lb, i, j, k)] *
+           (1 + c_dot_u / cs2 + pow(c_dot_u, 2) / (2 * cs2 * cs2) - u_dot_u / (2 * cs2) +
+            pow(c_dot_u, 3) / (6 * pow(cs2, 3)) - c_dot_u * u_dot_u / (2 * cs2 * cs2));
+  }
+  return g_eq;
+}
+
+double orc_geq(const orc_lbm *lb, int i, int j, int k, int l, int as_solid, double uwx,
+               double uwy, double uwz) {
+  const double uw[3] = {uwx, uwy, uwz};
+  return calc_geq(lb, i, j, k, l, as_solid ? ORC_TYPE_S : ORC_TYPE_F, uw);
+}
+
+/* The ENERGY part of ConvLBM, lbm.cpp:121-177: the strain-rate tensor from
+ * the non-equilibrium f, the dissipation Phi, the source F_vis with its
+ * time derivative, and BGK for g with relaxation time tau_g. */
+static void conv_energy(const orc_lbm *lb, int i, int j, int k) {
+  const orc_thermal *th = lb->th;
+  const double cs2 = lb->cs2, tau = lb->tau, tau_g = th->tau_g;
+  const orc_cell *c = &lb->fluid[i][j][k];
+  const int64_t n = LIN(lb, i, j, k);
+  const int q = lb->q;
+  const int *cx = lb->cx, *cy = lb->cy, *cz = lb->cz;
+  double rho = c->density, ux = c->ux, uy = c->uy, uz = c->uz;
+  double S[3][3];
+  for (int row = 0; row < 3; row++)
+    for (int col = 0; col < 3; col++) {
+      S[row][col] = 0;
+      for (int l = 0; l < q; l++) {
+        double f_eq = calc_feq(lb, c, l);
+        if (row == 0 && col == 0) S[row][col] += cx[l] * cx[l] * (c->f[l] - f_eq);
+        else if (row == 0 && col == 1) S[row][col] += cx[l] * cy[l] * (c->f[l] - f_eq);
+        else if (row == 0 && col == 2) S[row][col] += cx[l] * cz[l] * (c->f[l] - f_eq);
+        else if (row == 1 && col == 1) S[row][col] += cy[l] * cy[l] * (c->f[l] - f_eq);
+        else if (row == 1 && col == 2) S[row][col] += cy[l] * cz[l] * (c->f[l] - f_eq);
+        else if (row == 2 && col == 2) S[row][col] += cz[l] * cz[l] * (c->f[l] - f_eq);
+      }
+    }
+  S[1][0] = S[0][1]; S[2][0] = S[0][2]; S[2][1] = S[1][2];
+  for (int row = 0; row < 3; row++)
+    for (int col = 0; col < 3; col++) S[row][col] = -S[row][col] / (2 * rho * tau * cs2);
+  double Phi = 0;
+  for (int row = 0; row < 3; row++)
+    for (int col = 0; col < 3; col++) Phi += S[row][col] * S[row][col];
+  Phi = 2 * th->nu * rho * Phi;
+  const double zero[3] = {0, 0, 0};
+  for (int l = 0; l < q; l++) {
+    double c_dot_u = ux * cx[l] + uy * cy[l] + uz * cz[l];
+    double F_before = th->F_vis[n * q + l];
+    double F = lb->w[l] * Phi / th->cp * (1 + c_dot_u * (tau_g - 0.5) / (cs2 * tau_g));
+    th->F_vis[n * q + l] = F;
+    double dF_dt = (F - F_before) / ORC_DT;
+    double g_eq = calc_geq(lb, i, j, k, l, c->type, zero);
+    double g = th->g[n * q + l];
+    th->g_star[n * q + l] = g - (g - g_eq) / tau_g + ORC_DT * F + 0.5 * pow(ORC_DT, 2) * dF_dt;
+  }
+}
+
+/* Determine_Twall_Neumann, lbm.cpp:983-1028, for fluid cell (i,j,k) with
+ * wall velocity uw: pos = opposite of every solid direction (in l order),
+ * notpos = the other directions, n = mean of c over pos (not normalised),
+ * T_w = (j.n - sum_notpos g c.n) / sum_pos eq c.n. */
+static double twall_neumann(const orc_lbm *lb, int i, int j, int k, const double uw[3]) {
+  const orc_thermal *th = lb->th;
+  const int q = lb->q;
+  const double cs2 = lb->cs2;
+  const orc_cell *c = &lb->fluid[i][j][k];
+  const int64_t n = LIN(lb, i, j, k);
+  int pos[19], npos = 0, in_pos[19] = {0};
+  for (int l = 0; l < q; l++) {
+    int i_nb = (i + lb->cx[l] + (lb->Nx + 2)) % (lb->Nx + 2);
+    int j_nb = (j + lb->cy[l] + (lb->Ny + 2)) % (lb->Ny + 2);
+    int k_nb = (k + lb->cz[l] + (lb->Nz + 2)) % (lb->Nz + 2);
+    if (lb->fluid[i_nb][j_nb][k_nb].type == ORC_TYPE_S) {
+      pos[npos++] = lb->opp[l];
+      in_pos[lb->opp[l]] = 1;
+    }
+  }
+  double normal[3] = {0, 0, 0};
+  for (int p = 0; p < npos; p++) {
+    normal[0] += lb->cx[pos[p]]; normal[1] += lb->cy[pos[p]]; normal[2] += lb->cz[pos[p]];
+  }
+  normal[0] /= npos; normal[1] /= npos; normal[2] /= npos;
+  double sum_cdotn_notpos = 0;
+  for (int l = 0; l < q; l++) {
+    if (in_pos[l]) continue;
+    sum_cdotn_notpos += th->g[n * q + l] *
+                        (lb->cx[l] * normal[0] + lb->cy[l] * normal[1] + lb->cz[l] * normal[2]);
+  }
+  double uwall_dot_uwall = uw[0] * uw[0] + uw[1] * uw[1] + uw[2] * uw[2];
+  double sum_cdotn_pos = 0;
+  for (int p = 0; p < npos; p++) {
+    int l = pos[p];
+    double uwall_dot_c = uw[0] * lb->cx[l] + uw[1] * lb->cy[l] + uw[2] * lb->cz[l];
+    double eq = lb->w[l] * c->density *
+                (1 + uwall_dot_c / cs2 + pow(uwall_dot_c, 2) / (2 * cs2 * cs2) -
+                 uwall_dot_uwall / (2 * cs2) + pow(uwall_dot_c, 3) / (6 * pow(cs2, 3)) -
+                 uwall_dot_c * uwall_dot_uwall / (2 * cs2 * cs2));
+    sum_cdotn_pos += eq * (lb->cx[l] * normal[0] + lb->cy[l] * normal[1] + lb->cz[l] * normal[2]);
+  }
+  double numerator = th->j_dot_n - sum_cdotn_notpos;
+  double denum = sum_cdotn_pos;
+  return numerator / denum;
+}
+
 /* LBM::Init, lbm.cpp:31-57: f = f_star = feq for every cell, ghosts and
  * solids included (the type filter is commented out, lbm.cpp:37). */
 void orc_init(orc_lbm *lb) {
@@ -245,6 +462,16 @@ void orc_init(orc_lbm *lb) {
           double f_eq = calc_feq(lb, &lb->fluid[i][j][k], l);
           lb->fluid[i][j][k].f[l] = f_eq;
           lb->fluid[i][j][k].f_star[l] = f_eq;
+          if (lb->th) {
+            /* lbm.cpp:45-50: g = g_star = Calc_geq with the cell's own type
+             * and a zero wall velocity */
+            const double zero[3] = {0, 0, 0};
+            double g_eq = calc_geq(lb, i, j, k, l, lb->fluid[i][j][k].type, zero);
+            const int64_t n = LIN(lb, i, j, k);
+            lb->th->g[n * q + l] = g_eq;
+            lb->th->g_star[n * q + l] = g_eq;
+            lb->th->F_vis[n * q + l] = 0.0;
+          }
         }
 }
 
@@ -277,7 +504,10 @@ int orc_collision(orc_lbm *lb) {
   for (int i = 0; i <= Nx + 1; i++)
     for (int j = 0; j <= Ny + 1; j++)
       for (int k = 0; k <= Nz + 1; k++)
-        if (lb->fluid[i][j][k].type == ORC_TYPE_F) bad |= conv_lbm(lb, &lb->fluid[i][j][k]);
+        if (lb->fluid[i][j][k].type == ORC_TYPE_F) {
+          bad |= conv_lbm(lb, &lb->fluid[i][j][k]);
+          if (lb->th) conv_energy(lb, i, j, k);
+        }
   return bad ? ORC_ERR_FEQ_NEGATIVE : ORC_OK;
 }
 
@@ -331,6 +561,7 @@ void orc_streaming(orc_lbm *lb) {
   const int Nx = lb->Nx, Ny = lb->Ny, Nz = lb->Nz, q = lb->q;
   const double cs2 = lb->cs2;
   orc_cell ***fl = lb->fluid;
+  orc_thermal *th = lb->th;
 
   /* fluid part, lbm.cpp:183-295: push from TYPE_F cells */
 #pragma omp parallel for
@@ -347,6 +578,7 @@ void orc_streaming(orc_lbm *lb) {
           orc_cell *nb = &fl[i_nb][j_nb][k_nb];
           if (nb->type == ORC_TYPE_F) {
             nb->f[l] = fl[i][j][k].f_star[l];
+            if (th) th->g[LIN(lb, i_nb, j_nb, k_nb) * q + l] = th->g_star[LIN(lb, i, j, k) * q + l];
           } else if (nb->type == ORC_TYPE_S) {
             if (!have_uw) {
               orc_wall_velocity(lb, i, j, k, uwall);
@@ -354,10 +586,46 @@ void orc_streaming(orc_lbm *lb) {
             }
             double uwall_dot_c = uwall[0] * lb->cx[l] + uwall[1] * lb->cy[l] + uwall[2] * lb->cz[l];
             fl[i][j][k].f[lb->opp[l]] = fl[i][j][k].f_star[l] - 2 * lb->w[l] * uwall_dot_c / cs2;
+            /* thermal Dirichlet wall, lbm.cpp:277-280 (anti-bounce-back);
+             * the Neumann wall (TYPE_Q) runs in the second sweep below */
+            if (th && th->typeE[LIN(lb, i_nb, j_nb, k_nb)] == ORC_TYPE_T) {
+              double g_eq = calc_geq(lb, i, j, k, l, ORC_TYPE_S, uwall);
+              const int64_t n = LIN(lb, i, j, k);
+              th->g[n * q + lb->opp[l]] = -th->g_star[n * q + l] + 2 * g_eq;
+            }
           }
           /* any other type: nothing is written (frozen link) */
         }
       }
+
+  /* thermal Neumann wall, lbm.cpp:281-287: after every push of the pass */
+  if (th) {
+#pragma omp parallel for
+    for (int i = 0; i <= Nx + 1; i++)
+      for (int j = 0; j <= Ny + 1; j++)
+        for (int k = 0; k <= Nz + 1; k++) {
+          if (fl[i][j][k].type != ORC_TYPE_F) continue;
+          int have = 0;
+          double uwall[3] = {0.0, 0.0, 0.0}, Twall = 0.0;
+          for (int l = 0; l < q; l++) {
+            int i_nb = WRAP(i + lb->cx[l], Nx + 2);
+            int j_nb = WRAP(j + lb->cy[l], Ny + 2);
+            int k_nb = WRAP(k + lb->cz[l], Nz + 2);
+            if (fl[i_nb][j_nb][k_nb].type != ORC_TYPE_S) continue;
+            if (th->typeE[LIN(lb, i_nb, j_nb, k_nb)] != ORC_TYPE_Q) continue;
+            if (!have) {
+              orc_wall_velocity(lb, i, j, k, uwall);
+              Twall = twall_neumann(lb, i, j, k, uwall);
+              have = 1;
+            }
+            double uwall_dot_c = uwall[0] * lb->cx[l] + uwall[1] * lb->cy[l] + uwall[2] * lb->cz[l];
+            double uwall_dot_uwall = uwall[0] * uwall[0] + uwall[1] * uwall[1] + uwall[2] * uwall[2];
+            th->g[LIN(lb, i, j, k) * q + lb->opp[l]] =
+                lb->w[l] * fl[i][j][k].density * Twall *
+                (1 + uwall_dot_c / cs2 + pow(uwall_dot_c, 2) / (2 * cs2 * cs2) - uwall_dot_uwall / (2 * cs2));
+          }
+        }
+  }
 
   /* Face loops, lbm.cpp:299-625, serial.  They use D2Q9 direction indices
    * even in D3Q19 (the reference's own code) and the back face writes its
@@ -427,9 +695,28 @@ int orc_macroprop(orc_lbm *lb) {
         c->ux = (rho_ux + 0.5 * ORC_DT * c->F[0]) / rho;
         c->uy = (rho_uy + 0.5 * ORC_DT * c->F[1]) / rho;
         c->uz = (rho_uz + 0.5 * ORC_DT * c->F[2]) / rho;
-        if (c->density < 0 || c->density > 100) bad = 1;
+        if (c->density < 0 || c->density > 100) bad |= 1;
+        if (lb->th) {
+          /* lbm.cpp:777-806 */
+          orc_thermal *th = lb->th;
+          const int64_t n = LIN(lb, i, j, k);
+          double T = 0, sum_gx = 0, sum_gy = 0, sum_gz = 0;
+          for (int l = 0; l < q; l++) {
+            T += th->g[n * q + l];
+            sum_gx += th->g[n * q + l] * lb->cx[l];
+            sum_gy += th->g[n * q + l] * lb->cy[l];
+            sum_gz += th->g[n * q + l] * lb->cz[l];
+          }
+          th->T[n] = T;
+          th->q[3 * n + 0] = sum_gx;
+          th->q[3 * n + 1] = sum_gy;
+          th->q[3 * n + 2] = sum_gz;
+          if (T < 0 || T > 100) bad |= 2;
+          if (th->boussinesq) c->F[1] = th->g_beta * (T - th->T_ref);
+        }
       }
-  return bad ? ORC_ERR_DENSITY_RANGE : ORC_OK;
+  if (bad & 1) return ORC_ERR_DENSITY_RANGE;
+  return bad ? ORC_ERR_TEMPERATURE_RANGE : ORC_OK;
 }
 
 /* main.cpp:212-221 */

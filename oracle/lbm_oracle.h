@@ -131,6 +131,25 @@ void orc_ibm_nearest(int xmin, int xmax, int ymin, int ymax, int zmin,
                      int zmax, double mx, double my, double mz, double nx,
                      double ny, double nz, int *i, int *j, int *k);
 
+/* ---- thermal branch (#define ENERGY): LBM.h:590-596, lbm.cpp:121-178,
+ * 263-289, 636-653, 777-806, 964-1028 ---------------------------------- */
+/* Adds g, g_star, T, q, type_Energy and F_vis to every cell.  tau_g, nu, cp
+ * and j_dot_n are the case constants (LBM.h:150-161); boussinesq = 1 sets
+ * F[1] = g_beta (T - T_ref) in MacroProp (Rayleigh_Benard, lbm.cpp:803-806). */
+int orc_thermal_enable(orc_lbm *lb, double tau_g, double nu, double cp, double j_dot_n,
+                       int boussinesq, double g_beta, double T_ref);
+/* writes of lb.fluid[i][j][k].T and .type_Energy as the setups do */
+void orc_set_thermal(orc_lbm *lb, const double *T, const int16_t *typeE);
+void orc_get_thermal(const orc_lbm *lb, double *T, double *qx, double *qy, double *qz);
+/* g as [cell][l] */
+void orc_get_g(const orc_lbm *lb, double *g);
+void orc_set_g(orc_lbm *lb, const double *g);
+/* Determine_Wall_Temperature, lbm.cpp:965-981 */
+double orc_wall_temperature(const orc_lbm *lb, int i, int j, int k);
+/* Calc_geq, lbm.cpp:636-653 (as_solid: the TYPE_S branch with wall velocity uw) */
+double orc_geq(const orc_lbm *lb, int i, int j, int k, int l, int as_solid, double uwx,
+               double uwy, double uwz);
+
 #ifdef __cplusplus
 }
 #endif

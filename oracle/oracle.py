@@ -49,6 +49,13 @@ _SIG = {
     "orc_ibm_nearest": (None, [C.c_int] * 6 + [C.c_double] * 6 + [C.POINTER(C.c_int)] * 3),
     "orc_ibm_step": (C.c_int, [_P, C.c_int, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D]
                      + [C.c_int] * 7 + [C.c_double, _D]),
+    "orc_thermal_enable": (C.c_int, [_P] + [C.c_double] * 4 + [C.c_int] + [C.c_double] * 2),
+    "orc_set_thermal": (None, [_P, _D, C.POINTER(C.c_int16)]),
+    "orc_get_thermal": (None, [_P, _D, _D, _D, _D]),
+    "orc_get_g": (None, [_P, _D]),
+    "orc_set_g": (None, [_P, _D]),
+    "orc_wall_temperature": (C.c_double, [_P, C.c_int, C.c_int, C.c_int]),
+    "orc_geq": (C.c_double, [_P] + [C.c_int] * 5 + [C.c_double] * 3),
 }
 
 _libs: dict[str, C.CDLL] = {}
@@ -159,6 +166,39 @@ class Oracle:
         f = np.ascontiguousarray(f, dtype=np.float64)
         assert f.shape == self.shape + (self.q,)
         self.lib.orc_set_fstar(self.h, _dp(f))
+
+    # -- thermal branch (ENERGY) -------------------------------------------------
+    def thermal_enable(self, tau_g, nu, cp, j_dot_n=0.0, boussinesq=False, g_beta=0.0, T_ref=0.0):
+        self.lib.orc_thermal_enable(self.h, float(tau_g), float(nu), float(cp), float(j_dot_n),
+                                    int(bool(boussinesq)), float(g_beta), float(T_ref))
+
+    def set_thermal(self, T=None, type_energy=None):
+        Ta = None if T is None else _f64(T, self.shape)
+        te = None
+        if type_energy is not None:
+            te = np.ascontiguousarray(np.broadcast_to(np.asarray(type_energy, np.int16), self.shape))
+        self.lib.orc_set_thermal(self.h, _dp(Ta), None if te is None else te.ctypes.data_as(C.POINTER(C.c_int16)))
+
+    def thermal(self):
+        out = [np.empty(self.shape) for _ in range(4)]
+        self.lib.orc_get_thermal(self.h, *[_dp(a) for a in out])
+        return tuple(out)
+
+    def g(self):
+        a = np.empty(self.shape + (self.q,))
+        self.lib.orc_get_g(self.h, _dp(a))
+        return a
+
+    def set_g(self, g):
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        assert g.shape == self.shape + (self.q,)
+        self.lib.orc_set_g(self.h, _dp(g))
+
+    def wall_temperature(self, i, j, k):
+        return self.lib.orc_wall_temperature(self.h, i, j, k)
+
+    def geq(self, i, j, k, l, as_solid=False, uw=(0.0, 0.0, 0.0)):
+        return self.lib.orc_geq(self.h, i, j, k, l, int(bool(as_solid)), *[float(v) for v in uw])
 
     def feq(self, rho, ux, uy, uz, l):
         return self.lib.orc_feq(self.h, rho, ux, uy, uz, l)

@@ -456,3 +456,135 @@ def test_tgv_divergence_free_and_rho():
     assert (c.type == 0).all() and c.shape == (16, 16, 16)
     assert rel_l2_u((c.ux, c.uy, c.uz), (c.ux, c.uy, c.uz), c.type == 0) == 0.0
     assert k == pytest.approx(2 * np.pi / 16)
+
+
+# ---- a10: thermal branch (ENERGY) ----------------------------------------------------------
+def _thermal_oracle(oracle_mod, case, th):
+    o = oracle_mod.Oracle(case.Nx, case.Ny, case.Nz, case.tau)
+    o.thermal_enable(th.tau_g, th.nu, th.cp, th.j_dot_n, th.boussinesq, th.g_beta, th.T_ref)
+    o.set_cells(**case.fields())
+    o.set_thermal(th.T, th.type_energy)
+    return o
+
+
+def test_thermal_geq_moments(oracle_mod):
+    """Calc_geq (lbm.cpp:646-651) = rho w T (1 + cubic): sum_l geq = rho T and
+    sum_l geq c = rho T u exactly in exact arithmetic (the cubic terms have no
+    zeroth or first moment on D3Q19); the solid branch with zero wall velocity
+    is rho w T_w."""
+    case = cases.random_mask_case(3, 3, 3, seed=2)
+    rng = np.random.default_rng(9)
+    T = 0.3 + rng.random(case.shape)
+    o = oracle_mod.Oracle(case.Nx, case.Ny, case.Nz, case.tau)
+    o.thermal_enable(0.8, 0.1, 2.0)
+    o.set_cells(**case.fields())
+    o.set_thermal(T, None)
+    cx, cy, cz, opp, w = o.tables()
+    for (i, j, k) in [(1, 1, 1), (2, 3, 1), (4, 0, 2)]:
+        g = np.array([o.geq(i, j, k, l) for l in range(19)])
+        r, u = case.rho[i, j, k], np.array([case.ux[i, j, k], case.uy[i, j, k], case.uz[i, j, k]])
+        assert g.sum() == pytest.approx(r * T[i, j, k], rel=1e-14)
+        for c, ua in zip((cx, cy, cz), u):
+            assert (g * c).sum() == pytest.approx(r * T[i, j, k] * ua, rel=1e-12, abs=1e-16)
+    # solid branch, u_w = 0: rho w T_w with T_w = Determine_Wall_Temperature
+    i, j, k = 1, 1, 1
+    tw = o.wall_temperature(i, j, k)
+    for l in range(19):
+        assert o.geq(i, j, k, l, as_solid=True) == pytest.approx(case.rho[i, j, k] * w[l] * tw, rel=1e-15)
+
+
+def test_thermal_wall_temperature_brute_force(oracle_mod):
+    """Determine_Wall_Temperature (lbm.cpp:965-981): mean T over the TYPE_S
+    neighbours x + c_l, wrapped, l = 0 included."""
+    case, th = cases.random_thermal_case(5, 4, 3, seed=4)
+    o = _thermal_oracle(oracle_mod, case, th)
+    cx, cy, cz, opp, w = o.tables()
+    n = case.shape
+    hit = 0
+    for i in range(n[0]):
+        for j in range(n[1]):
+            for k in range(n[2]):
+                vals = [th.T[(i + cx[l]) % n[0], (j + cy[l]) % n[1], (k + cz[l]) % n[2]]
+                        for l in range(19)
+                        if case.type[(i + cx[l]) % n[0], (j + cy[l]) % n[1], (k + cz[l]) % n[2]] == cases.TYPE_S]
+                if not vals:
+                    continue
+                hit += 1
+                assert o.wall_temperature(i, j, k) == pytest.approx(np.mean(vals), rel=1e-14)
+    assert hit > 50
+
+
+def test_thermal_solid_temperature_after_first_macroprop(oracle_mod):
+    """Solids keep their Init g = rho_s w T_w (lbm.cpp:637-641) and MacroProp
+    runs over them (lbm.cpp:777-790), so after the first step a solid's T is
+    rho_s times the mean set-up T of its own solid neighbours: on the walls
+    of a channel that mixes in the opposite wall through the periodic wrap,
+    (9 T_own + 5 T_other)/14."""
+    case, th = cases.conduction_channel(Tb=1.0, Tt=0.5)
+    o = _thermal_oracle(oracle_mod, case, th)
+    o.init()
+    assert o.run(1) == 0
+    T = o.thermal()[0]
+    assert np.allclose(T[:, 0, :], (9 * 1.0 + 5 * 0.5) / 14, rtol=0, atol=1e-15)
+    assert np.allclose(T[:, -1, :], (9 * 0.5 + 5 * 1.0) / 14, rtol=0, atol=1e-15)
+
+
+def test_thermal_dirichlet_conduction_linear_profile(oracle_mod):
+    """Anti-bounce-back Dirichlet walls (lbm.cpp:277-280), fluid at rest:
+    the steady state is linear between the walls' temperatures (after the
+    first MacroProp) at the half-way wall positions j = 1/2, Ny + 1/2."""
+    case, th = cases.conduction_channel(bottom=cases.TYPE_T, top=cases.TYPE_T)
+    o = _thermal_oracle(oracle_mod, case, th)
+    o.init()
+    assert o.run(4000) == 0
+    T = o.thermal()[0]
+    Tb, Tt = th.params["Tb"], th.params["Tt"]
+    Tb2, Tt2 = (9 * Tb + 5 * Tt) / 14, (9 * Tt + 5 * Tb) / 14
+    eta = (np.arange(1, case.Ny + 1) - 0.5) / case.Ny
+    assert np.abs(T[:, 1:-1, :] - (Tb2 + (Tt2 - Tb2) * eta)[None, :, None]).max() < 1e-12
+
+
+def test_thermal_neumann_wall_is_adiabatic(oracle_mod):
+    """Determine_Twall_Neumann (lbm.cpp:983-1028) with j.n = 0 sets the
+    unknown populations so that the wall-normal flux vanishes: with an
+    adiabatic bottom and a Dirichlet top the fluid relaxes to the top wall's
+    temperature, uniform."""
+    case, th = cases.conduction_channel(bottom=cases.TYPE_Q, top=cases.TYPE_T)
+    o = _thermal_oracle(oracle_mod, case, th)
+    o.init()
+    assert o.run(6000) == 0
+    T = o.thermal()[0]
+    Ttop = (9 * th.params["Tt"] + 5 * th.params["Tb"]) / 14
+    assert np.abs(T[:, 1:-1, :] - Ttop).max() < 2e-7
+
+
+def test_thermal_couette_viscous_heating_closed_form(oracle_mod):
+    """Couette flow with viscous heating (main_setup_Couette + ENERGY): the
+    steady temperature is T_b' + (T_t' - T_b') eta + nu U^2/(2 cp alpha)
+    eta (1 - eta), eta = (j - 1/2)/Ny, with the walls' temperatures after the
+    first MacroProp.  Pins the dissipation Phi = 2 nu rho S:S of the
+    non-equilibrium strain rate, the source w Phi/cp and tau_g."""
+    case, th = cases.couette_thermal(Ny=12, Re=10.0)
+    o = _thermal_oracle(oracle_mod, case, th)
+    o.init()
+    assert o.run(5000) == 0
+    T = o.thermal()[0]
+    p = th.params
+    Tb2 = (9 * p["Tb"] + 5 * p["Tt"]) / 14
+    Tt2 = (9 * p["Tt"] + 5 * p["Tb"]) / 14
+    eta = (np.arange(1, case.Ny + 1) - 0.5) / case.Ny
+    U = p["u_wall"]
+    heat = th.nu * U ** 2 / (2 * th.cp * p["alpha"])
+    Ta = Tb2 + (Tt2 - Tb2) * eta + heat * eta * (1 - eta)
+    err = np.abs(T[1, 1:-1, 1] - Ta).max()
+    # the heating term dominates the profile; a dropped or doubled Phi fails
+    assert heat > 2 * abs(Tt2 - Tb2)
+    assert err < 1e-3 * heat
+
+
+def test_thermal_temperature_range_error(oracle_mod):
+    """MacroProp terminates when T < 0 or T > 100 (lbm.cpp:794-797): code 3."""
+    case, th = cases.conduction_channel(Tb=150.0, Tt=150.0, T_fluid=150.0)
+    o = _thermal_oracle(oracle_mod, case, th)
+    o.init()
+    assert o.run(1) == 3
