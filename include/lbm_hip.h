@@ -195,6 +195,32 @@ int lbm_ibm_markers(lbm_ctx *ctx, double *x, double *y, double *z, double *ub,
  * after the last IBLBM). */
 int lbm_ibm_box(lbm_ctx *ctx, int32_t box[6], double center[2]);
 
+/* ---- macroscopic output (OutputVTK, lbm.cpp:2290-2468) ---------------------
+ * The reference's .vtr file: a VTK RectilinearGrid with cells i = 0..Nx+1,
+ * j = 0..Ny+1, k = 1..Nz and appended raw Float32 blocks Density, Velocity
+ * ((float)u / u_scale), [Force density], [Temperature, Heat flux],
+ * CellType (type as Float32), [EnergyCellType], then the coordinates.  The
+ * blocks are packed on the device from the state of the last MacroProp and
+ * copied to pinned host memory on a second stream, so stepping can go on
+ * while the copy runs; lbm_output_vtk_end waits for the copy and writes the
+ * file.  One output can be in flight per context. */
+typedef struct lbm_vtk_desc {
+  double u_scale;     /* velocity = (float)u / u_scale: u_max (the default  *
+                       * build), u_wall/u_max (Velocity_Ratio) or 1.0       */
+  double T_a, T_b;    /* T_mode 1: (float)(T - T_a) / T_b; 2: (float)T/T_a - 1 */
+  int32_t velocity_name; /* 0 "Velocity", 1 "Velocity Ratio",                *
+                          * 2 "Velocity (Dimensional)"                      */
+  int32_t T_mode;        /* 0 (float)T "Temperature"; 1, 2 "Temperature Ratio" */
+  int32_t force;         /* write "Force density" (the IBM build)            */
+  int32_t thermal;       /* write Temperature, Heat flux, EnergyCellType     */
+  int32_t poiseuille_coords; /* y, z coordinates (float)(j dy)/H*2-1, H=Ny+2  */
+  int32_t reserved[3];
+} lbm_vtk_desc;
+int lbm_output_vtk_begin(lbm_ctx *ctx, const lbm_vtk_desc *desc);
+int lbm_output_vtk_end(lbm_ctx *ctx, const char *path);
+/* begin + end */
+int lbm_output_vtk(lbm_ctx *ctx, const lbm_vtk_desc *desc, const char *path);
+
 /* sizes and identification */
 int64_t lbm_ncells(const lbm_ctx *ctx);
 int64_t lbm_nfluid(const lbm_ctx *ctx);

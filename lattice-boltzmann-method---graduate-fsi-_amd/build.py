@@ -40,6 +40,7 @@ SOURCES = [
     os.path.join(HERE, "csrc", "lbm_probe.hip"),
     os.path.join(HERE, "csrc", "lbm_ibm.hip"),
     os.path.join(HERE, "csrc", "lbm_thermal.hip"),
+    os.path.join(HERE, "csrc", "lbm_output.hip"),
 ]
 HOST_SOURCES = [
     os.path.join(HERE, "host", "LBM.cpp"),

@@ -104,6 +104,7 @@ static int download_real(lbm_ctx *c, double *dst, const void *src) {
 }
 
 static void free_all(lbm_ctx *c) {
+  output_free(c);
   ibm_free(c);
   thermal_free(c);
   void *ptrs[] = {c->cur, c->oth, c->type, c->mask, c->links, c->uw[0], c->uw[1], c->uw[2],

@@ -60,6 +60,8 @@ struct lbm_ctx {
   uint64_t *tlinks = nullptr;        // bit m: Dirichlet link m; bit 32+m: Neumann
   int walls_refresh = 0;             // MacroProps left that change the wall tables
   std::vector<int16_t> h_typeE;      // device order
+  int16_t *typeE = nullptr;          // device copy of type_Energy
+  struct lbm_output *out = nullptr;  // VTK output staging (lbm_output.hip)
 };
 
 #define HIPCHK(x)                              \
@@ -76,6 +78,9 @@ int thermal_tables(lbm_ctx *c);        // Dirichlet/Neumann links, wall temperat
 // 3 collide in place, 4 MacroProp of the cells with mask bits `sel`
 int thermal_launch(lbm_ctx *c, int mode, uint8_t sel = 1);
 int thermal_download_g(lbm_ctx *c, double *g_dev);
+
+// VTK output staging (lbm_output.hip)
+void output_free(lbm_ctx *c);
 
 // launch parameters of the context's current state (lbm_capi.hip)
 lbm::LaunchCtx launch_ctx(const lbm_ctx *c);

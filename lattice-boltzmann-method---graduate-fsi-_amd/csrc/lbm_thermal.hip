@@ -558,7 +558,7 @@ int upload_cells_c(lbm_ctx *c, void *dst, const std::vector<double> &h) {
 
 void thermal_free(lbm_ctx *c) {
   void **ptrs[] = {&c->T, &c->qv[0], &c->qv[1], &c->qv[2], &c->phi, &c->up[0], &c->up[1],
-                   &c->up[2], &c->rho_w, &c->Tw, (void **)&c->tlinks};
+                   &c->up[2], &c->rho_w, &c->Tw, (void **)&c->tlinks, (void **)&c->typeE};
   for (void **p : ptrs) {
     if (*p) (void)hipFree(*p);
     *p = nullptr;
@@ -567,10 +567,9 @@ void thermal_free(lbm_ctx *c) {
 }
 
 int thermal_tables(lbm_ctx *c) {
-  int16_t *typeE = nullptr;
-  HIPCHK(hipMalloc((void **)&typeE, 2 * (size_t)c->g.ncell));
-  hipError_t e = hipMemcpyAsync(typeE, c->h_typeE.data(), 2 * (size_t)c->g.ncell, hipMemcpyHostToDevice, c->stream);
-  if (e == hipSuccess) {
+  const int16_t *typeE = c->typeE;
+  hipError_t e = hipSuccess;
+  {
     if (c->storage == LBM_STORE_F64)
       hipLaunchKernelGGL(k_th_tables<double>, grid_of(c), dim3(c->block), 0, c->stream, c->type, typeE, c->mask, c->g,
                          (const double *)c->T, c->tlinks, (double *)c->Tw);
@@ -580,7 +579,6 @@ int thermal_tables(lbm_ctx *c) {
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  (void)hipFree(typeE);
   return e == hipSuccess ? LBM_OK : LBM_ERR_HIP;
 }
 
@@ -683,6 +681,8 @@ int lbm_upload_thermal(lbm_ctx *c, const lbm_thermal_desc *d, const double *T,
   for (void **p : arr)
     if (hipMalloc(p, n * c->csize) != hipSuccess) return LBM_ERR_NOMEM;
   if (hipMalloc((void **)&c->tlinks, n * 8) != hipSuccess) return LBM_ERR_NOMEM;
+  if (hipMalloc((void **)&c->typeE, n * 2) != hipSuccess) return LBM_ERR_NOMEM;
+  HIPCHK(hipMemcpy(c->typeE, c->h_typeE.data(), n * 2, hipMemcpyHostToDevice));
   int e = upload_cells_c(c, c->T, hT);
   if (e != LBM_OK) return e;
   for (int q = 0; q < 3; q++) HIPCHK(hipMemsetAsync(c->qv[q], 0, n * c->csize, c->stream));

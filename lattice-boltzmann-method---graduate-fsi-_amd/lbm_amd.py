@@ -82,6 +82,27 @@ class _ThermalDesc(C.Structure):
     ]
 
 
+class _VtkDesc(C.Structure):
+    _fields_ = [
+        ("u_scale", C.c_double),
+        ("T_a", C.c_double),
+        ("T_b", C.c_double),
+        ("velocity_name", C.c_int32),
+        ("T_mode", C.c_int32),
+        ("force", C.c_int32),
+        ("thermal", C.c_int32),
+        ("poiseuille_coords", C.c_int32),
+        ("reserved", C.c_int32 * 3),
+    ]
+
+
+def vtk_desc(u_scale=1.0, T_a=0.0, T_b=1.0, velocity_name=0, T_mode=0, force=False, thermal=False,
+             poiseuille_coords=False):
+    return _VtkDesc(float(u_scale), float(T_a), float(T_b), int(velocity_name), int(T_mode),
+                    int(bool(force)), int(bool(thermal)), int(bool(poiseuille_coords)),
+                    (C.c_int32 * 3)())
+
+
 # name -> (restype, argtypes) for every entry point of include/lbm_hip.h
 _P = C.c_void_p
 _D = C.POINTER(C.c_double)
@@ -121,6 +142,9 @@ _SIGNATURES = {
     "lbm_upload_thermal": (C.c_int, [_P, C.POINTER(_ThermalDesc), _D, C.POINTER(C.c_int16)]),
     "lbm_download_thermal": (C.c_int, [_P, _D, _D, _D, _D]),
     "lbm_download_g": (C.c_int, [_P, _D]),
+    "lbm_output_vtk_begin": (C.c_int, [_P, C.POINTER(_VtkDesc)]),
+    "lbm_output_vtk_end": (C.c_int, [_P, C.c_char_p]),
+    "lbm_output_vtk": (C.c_int, [_P, C.POINTER(_VtkDesc), C.c_char_p]),
 }
 
 _lib = None
@@ -308,6 +332,18 @@ class Lattice:
         a = np.empty(self.shape + (19,))
         _check(self._lib.lbm_download_g(self._h, _dp(a)), "lbm_download_g")
         return a
+
+    # -- macroscopic output (OutputVTK, lbm.cpp:2290-2468) ----------------------
+    def output_vtk(self, path, **kw):
+        d = vtk_desc(**kw)
+        _check(self._lib.lbm_output_vtk(self._h, C.byref(d), os.fsencode(path)), "lbm_output_vtk")
+
+    def output_vtk_begin(self, **kw):
+        d = vtk_desc(**kw)
+        _check(self._lib.lbm_output_vtk_begin(self._h, C.byref(d)), "lbm_output_vtk_begin")
+
+    def output_vtk_end(self, path):
+        _check(self._lib.lbm_output_vtk_end(self._h, os.fsencode(path)), "lbm_output_vtk_end")
 
     # -- immersed boundary (IBLBM, lbm.cpp:1135-1417) ---------------------------
     def ibm_setup(self, markers: dict, box, m_max: int = 20, vsolid: float = 0.0, moving=False,
