@@ -47,9 +47,10 @@ HOST_SOURCES = [
     os.path.join(HERE, "host", "IBM.cpp"),
 ]
 # reference-style drivers on the drop-in classes (host code only)
-EXAMPLES = ["example_cavity", "example_cylinder_ibm"]
+EXAMPLES = ["example_cavity", "example_cylinder_ibm", "example_couette_thermal"]
 EXAMPLE_BIN = {"example_cavity": "lbm_example_cavity",
-               "example_cylinder_ibm": "lbm_example_cylinder_ibm"}
+               "example_cylinder_ibm": "lbm_example_cylinder_ibm",
+               "example_couette_thermal": "lbm_example_couette_thermal"}
 
 
 def _run(cmd: list[str]) -> None:

@@ -11,6 +11,8 @@ struct LBM::Impl {
   lbm_ctx *ctx = nullptr;
   LATTICE ***fluid = nullptr;
   int nx = 0, ny = 0, nz = 0;  // padded extents
+  bool thermal = false;
+  lbm_thermal_desc td{};
   ~Impl() {
     if (ctx) lbm_destroy(ctx);
     if (fluid) {
@@ -97,10 +99,26 @@ void LBM::Init() {
   check(lbm_upload_cells(m.ctx, type.data(), rho.data(), ux.data(), uy.data(), uz.data(),
                          Fx.data(), Fy.data(), Fz.data()),
         "lbm_upload_cells");
+  if (m.thermal) {
+    std::vector<double> T(n);
+    std::vector<int16_t> te(n);
+    r = 0;
+    for (int i = 0; i < m.nx; i++)
+      for (int j = 0; j < m.ny; j++)
+        for (int k = 0; k < m.nz; k++, r++) {
+          T[r] = fluid[i][j][k].T;
+          te[r] = fluid[i][j][k].type_Energy;
+        }
+    check(lbm_upload_thermal(m.ctx, &m.td, T.data(), te.data()), "lbm_upload_thermal");
+  }
   check(lbm_init(m.ctx), "lbm_init");
-  // the reference leaves f = f_star = feq in every cell
-  std::vector<double> f(n * lbm_host::npop);
+  // the reference leaves f = f_star = feq (and g = geq) in every cell
+  std::vector<double> f(n * lbm_host::npop), g;
   check(lbm_download_f(m.ctx, f.data()), "lbm_download_f");
+  if (m.thermal) {
+    g.resize(n * lbm_host::npop);
+    check(lbm_download_g(m.ctx, g.data()), "lbm_download_g");
+  }
   r = 0;
   for (int i = 0; i < m.nx; i++)
     for (int j = 0; j < m.ny; j++)
@@ -108,8 +126,16 @@ void LBM::Init() {
         for (int l = 0; l < lbm_host::npop; l++) {
           fluid[i][j][k].f[l] = f[r * lbm_host::npop + l];
           fluid[i][j][k].f_star[l] = f[r * lbm_host::npop + l];
+          if (m.thermal) fluid[i][j][k].g[l] = g[r * lbm_host::npop + l];
         }
 }
+
+void LBM::SetThermal(const lbm_thermal_desc &desc) {
+  impl_->thermal = true;
+  impl_->td = desc;
+}
+
+bool LBM::Thermal() const { return impl_->thermal; }
 
 void LBM::Collision() {
   check(lbm_collide(impl_->ctx), "lbm_collide");
@@ -136,7 +162,7 @@ void LBM::SyncMirror(bool populations) {
   std::vector<double> rho(n), ux(n), uy(n), uz(n);
   check(lbm_download_macro(m.ctx, rho.data(), ux.data(), uy.data(), uz.data()),
         "lbm_download_macro");
-  std::vector<double> f, Fx, Fy, Fz;
+  std::vector<double> f, g, Fx, Fy, Fz, T, qx, qy, qz;
   if (populations) {
     f.resize(n * lbm_host::npop);
     Fx.resize(n);
@@ -144,6 +170,17 @@ void LBM::SyncMirror(bool populations) {
     Fz.resize(n);
     check(lbm_download_f(m.ctx, f.data()), "lbm_download_f");
     check(lbm_download_force(m.ctx, Fx.data(), Fy.data(), Fz.data()), "lbm_download_force");
+    if (m.thermal) {
+      g.resize(n * lbm_host::npop);
+      check(lbm_download_g(m.ctx, g.data()), "lbm_download_g");
+    }
+  }
+  if (m.thermal) {
+    T.resize(n);
+    qx.resize(n);
+    qy.resize(n);
+    qz.resize(n);
+    check(lbm_download_thermal(m.ctx, T.data(), qx.data(), qy.data(), qz.data()), "lbm_download_thermal");
   }
   size_t r = 0;
   for (int i = 0; i < m.nx; i++)
@@ -159,6 +196,14 @@ void LBM::SyncMirror(bool populations) {
           c.F[0] = Fx[r];
           c.F[1] = Fy[r];
           c.F[2] = Fz[r];
+          if (m.thermal)
+            for (int l = 0; l < lbm_host::npop; l++) c.g[l] = g[r * lbm_host::npop + l];
+        }
+        if (m.thermal) {
+          c.T = T[r];
+          c.q[0] = qx[r];
+          c.q[1] = qy[r];
+          c.q[2] = qz[r];
         }
       }
 }
@@ -167,3 +212,19 @@ int LBM::getNx() const { return Nx; }
 int LBM::getNy() const { return Ny; }
 int LBM::getNz() const { return Nz; }
 double LBM::getTAU() const { return tau; }
+
+// ---- OutputVTK -------------------------------------------------------------
+namespace {
+lbm_vtk_desc g_vtk = {1.0, 0.0, 1.0, 0, 0, 0, 0, 0, {0, 0, 0}};
+}
+
+void VTK_SetCase(const lbm_vtk_desc &desc) { g_vtk = desc; }
+
+void OutputVTK(int &nout, LBM &lb) {
+  lbm_vtk_desc d = g_vtk;
+  d.force = lbm_ibm_box(lb.ctx(), nullptr, nullptr) == LBM_OK ? 1 : 0;  // the IBM build
+  d.thermal = lb.Thermal() ? 1 : 0;                                      // the ENERGY build
+  char filename[128];
+  std::snprintf(filename, sizeof filename, "./field%06d.vtr", nout);
+  check(lbm_output_vtk(lb.ctx(), &d, filename), "OutputVTK");
+}

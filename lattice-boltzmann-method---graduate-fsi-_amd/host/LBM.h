@@ -37,7 +37,8 @@ namespace lbm_host {
 constexpr int npop = 19;  // D3Q19, LBM.h:544
 }
 
-// LBM.h:558-597 (the D3Q19 build without IBM/ENERGY)
+// LBM.h:558-597, with the ENERGY members (LBM.h:590-596) always present:
+// they are read only when the lattice is thermal (LBM::SetThermal)
 class LATTICE {
  public:
   short type = TYPE_F;
@@ -47,6 +48,11 @@ class LATTICE {
   double uz = 0.0;
   double f[lbm_host::npop], f_star[lbm_host::npop];
   double F[3] = {0, 0, 0};
+  // ENERGY
+  double g[lbm_host::npop];
+  double T = 0.0;
+  double q[3] = {0, 0, 0};
+  short type_Energy = TYPE_F;
 };
 
 class LBM {
@@ -76,6 +82,11 @@ class LBM {
   double getTAU() const;
 
   // -- additions --------------------------------------------------------
+  // The thermal branch (the reference's #define ENERGY and its constants,
+  // LBM.h:150-161): call before Init(); Init() then also uploads
+  // fluid[][][].T and .type_Energy, and MacroProp()/Step() refresh T and q.
+  void SetThermal(const lbm_thermal_desc &desc);
+  bool Thermal() const;
   // n reference steps (Collision, Streaming, MacroProp) as fused launches;
   // refreshes density/ux/uy/uz of the mirror.
   void Step(int n);
@@ -83,3 +94,11 @@ class LBM {
   void SyncMirror(bool populations = false);
   lbm_ctx *ctx() const;
 };
+
+// OutputVTK(int&, LBM&), lbm.cpp:2290-2468: writes ./field%06d.vtr (nout)
+// from the lattice's last MacroProp.  The velocity scale, the block names
+// and the optional blocks come from VTK_SetCase (the reference's compile-time
+// case flags); "Force density" is written when the lattice runs the IBM and
+// the thermal blocks when it is thermal.
+void VTK_SetCase(const lbm_vtk_desc &desc);
+void OutputVTK(int &nout, LBM &lb);
