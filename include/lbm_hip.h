@@ -221,6 +221,38 @@ int lbm_output_vtk_end(lbm_ctx *ctx, const char *path);
 /* begin + end */
 int lbm_output_vtk(lbm_ctx *ctx, const lbm_vtk_desc *desc, const char *path);
 
+/* ---- ensembles and launch-shape tuning (no reference counterpart: the
+ * reference runs one lattice per process) ------------------------------------
+ * A batch steps many independent lattices with one launch per step: the
+ * workgroup tiles of all lattices form one grid (optionally a persistent grid
+ * of a few workgroups per CU, each taking a contiguous run of tiles), and the
+ * launches of a step pair are captured once into a HIP graph and replayed.
+ * Contexts of one batch share device, storage and force mode, have no IBM or
+ * thermal branch, and have been initialised (lbm_init). */
+typedef struct lbm_batch lbm_batch;
+int lbm_batch_create(lbm_ctx *const *ctxs, int32_t n, lbm_batch **out);
+int lbm_batch_destroy(lbm_batch *batch);
+/* nsteps reference steps (Collision, Streaming, MacroProp) of every lattice;
+ * each lattice ends in the state lbm_step(ctx, nsteps, 1) leaves. */
+int lbm_batch_step(lbm_batch *batch, int32_t nsteps);
+/* nsteps batched fused launches timed with HIP events (after bringing every
+ * lattice past its first steps); advances the lattices by nsteps + 1 or more. */
+int lbm_batch_time_steps(lbm_batch *batch, int32_t nsteps, double *ms);
+/* threads per workgroup (= cells per tile: 64, 128, 192 or 256) and
+ * workgroups per CU of a persistent grid (0: one workgroup per tile) */
+int lbm_batch_set_launch(lbm_batch *batch, int32_t block, int32_t per_cu);
+int lbm_batch_get_launch(const lbm_batch *batch, int32_t *block, int32_t *per_cu);
+int64_t lbm_batch_nfluid(const lbm_batch *batch);
+/* Times every launch shape (block 64/128/256 x per_cu 0/1/2/4/8) over nsteps
+ * batched steps and keeps the fastest.  Advances the lattices: initialise
+ * them again before the run that matters. */
+int lbm_batch_autotune(lbm_batch *batch, int32_t nsteps, int32_t *block, int32_t *per_cu,
+                       double *best_ms);
+/* Per-lattice tuner: times lbm_time_steps for block 64/128/256, keeps the
+ * fastest, and remembers it for every later lattice of the same size and
+ * storage in the process.  Advances the lattice. */
+int lbm_autotune(lbm_ctx *ctx, int32_t nsteps, int32_t *block);
+
 /* sizes and identification */
 int64_t lbm_ncells(const lbm_ctx *ctx);
 int64_t lbm_nfluid(const lbm_ctx *ctx);

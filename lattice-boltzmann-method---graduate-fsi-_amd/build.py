@@ -41,6 +41,7 @@ SOURCES = [
     os.path.join(HERE, "csrc", "lbm_ibm.hip"),
     os.path.join(HERE, "csrc", "lbm_thermal.hip"),
     os.path.join(HERE, "csrc", "lbm_output.hip"),
+    os.path.join(HERE, "csrc", "lbm_batch.hip"),
 ]
 HOST_SOURCES = [
     os.path.join(HERE, "host", "LBM.cpp"),

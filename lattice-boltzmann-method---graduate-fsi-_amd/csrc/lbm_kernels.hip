@@ -20,6 +20,8 @@
 //  k_links          the streaming tables as the kernels use them
 //  k_f_out          populations to FP64 for download
 //  k_reduce_*       deterministic FP64 sums (mass, momentum, kinetic energy)
+#include <string.h>
+
 #include <type_traits>
 
 #include "lbm_device.h"
@@ -34,15 +36,36 @@ namespace lbm {
 template <class S, int FORCE, bool COLLIDE, bool EMIT>
 __global__ __launch_bounds__(256) void k_stream(const Planes<S> P, const uint8_t *__restrict__ mask,
                                                 KArgs<typename Store<S>::C> a) {
-  using C = typename Store<S>::C;
   const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= a.g.ncell) return;
-  C f[Q];
-  bulk_loads<S>(P, idx, f);  // in flight together with the mask load
-  const uint8_t mk = (uint8_t)after_loads(mask[idx], f);
-  if (!(mk & M_FLUID)) return;
-  if (mk & (M_WALL | M_WRAP | M_BACKEQ)) patch_cell<S>(P, mk, idx, a, f);
-  finish_cell<S, FORCE, COLLIDE, EMIT>(f, mk, idx, a, P);
+  stream_cell<S, FORCE, COLLIDE, EMIT>(P, mask, a, idx);
+}
+
+// ---- ensembles: one launch steps many independent lattices ----------------
+// Per lattice and buffer parity: its plane pointers, kernel arguments and mask.
+template <class S>
+struct LatArgs {
+  Planes<S> P;
+  KArgs<typename Store<S>::C> a;
+  const uint8_t *mask;
+};
+
+// Workgroup b takes tiles b*per .. b*per + per - 1 of the tile table; tile t
+// is the blockDim.x cells [tile_first[t], tile_first[t] + blockDim.x) of
+// lattice tile_lat[t].  per = 1 with one workgroup per tile, or a contiguous
+// run of tiles per workgroup for a persistent grid of a few workgroups per CU.
+template <class S, int FORCE, bool COLLIDE, bool EMIT>
+__global__ __launch_bounds__(256) void k_stream_batch(const LatArgs<S> *__restrict__ la,
+                                                      const uint32_t *__restrict__ tile_lat,
+                                                      const uint32_t *__restrict__ tile_first,
+                                                      uint32_t ntile, uint32_t per, int parity) {
+  const uint32_t t0 = blockIdx.x * per;
+  const uint32_t t1 = min(ntile, t0 + per);
+  for (uint32_t t = t0; t < t1; t++) {
+    const LatArgs<S> *A = la + 2 * tile_lat[t] + parity;
+    const uint32_t idx = tile_first[t] + threadIdx.x;
+    if (idx < A->a.g.ncell) stream_cell<S, FORCE, COLLIDE, EMIT>(A->P, A->mask, A->a, idx);
+  }
 }
 
 // LBM::Collision (lbm.cpp:60-73) in place, with rho, u and F as stored
@@ -552,6 +575,69 @@ hipError_t launch_f_out(const LaunchCtx &L, const void *buf, double *out) {
       break;
     default:
       return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+size_t batch_args_size(int storage) {
+  switch (storage) {
+    case 0: return sizeof(LatArgs<double>);
+    case 1: return sizeof(LatArgs<float>);
+    default: return sizeof(LatArgs<__half>);
+  }
+}
+
+template <class S>
+static void fill_args_s(const LaunchCtx &L, const void *in, void *out, void *dst) {
+  LatArgs<S> A;
+  A.P = make_planes(L, (const S *)in, (S *)out);
+  A.a = make_args<S>(L);
+  A.mask = L.mask;
+  memcpy(dst, &A, sizeof A);
+}
+
+void batch_fill_args(const LaunchCtx &L, const void *in, void *out, void *dst) {
+  switch (L.storage) {
+    case 0: fill_args_s<double>(L, in, out, dst); break;
+    case 1: fill_args_s<float>(L, in, out, dst); break;
+    default: fill_args_s<__half>(L, in, out, dst); break;
+  }
+}
+
+template <class S, int FORCE>
+static void batch_mode(const BatchLaunch &B) {
+  const auto *la = (const LatArgs<S> *)B.la;
+  const dim3 grid(B.grid), block(B.block);
+  switch (B.mode) {
+    case STREAM_COLLIDE:
+      hipLaunchKernelGGL((k_stream_batch<S, FORCE, true, false>), grid, block, 0, B.stream, la, B.tile_lat, B.tile_first, B.ntile, B.per, B.parity);
+      break;
+    case STREAM_COLLIDE_MACRO:
+      hipLaunchKernelGGL((k_stream_batch<S, FORCE, true, true>), grid, block, 0, B.stream, la, B.tile_lat, B.tile_first, B.ntile, B.per, B.parity);
+      break;
+    case STREAM_MACRO:
+      hipLaunchKernelGGL((k_stream_batch<S, FORCE, false, true>), grid, block, 0, B.stream, la, B.tile_lat, B.tile_first, B.ntile, B.per, B.parity);
+      break;
+    default:
+      hipLaunchKernelGGL((k_stream_batch<S, FORCE, false, false>), grid, block, 0, B.stream, la, B.tile_lat, B.tile_first, B.ntile, B.per, B.parity);
+      break;
+  }
+}
+
+template <class S>
+static void batch_s(const BatchLaunch &B) {
+  switch (B.force_mode) {
+    case 0: batch_mode<S, 0>(B); break;
+    case 1: batch_mode<S, 1>(B); break;
+    default: batch_mode<S, 2>(B); break;
+  }
+}
+
+hipError_t launch_stream_batch(const BatchLaunch &B) {
+  switch (B.storage) {
+    case 0: batch_s<double>(B); break;
+    case 1: batch_s<float>(B); break;
+    default: batch_s<__half>(B); break;
   }
   return hipGetLastError();
 }

@@ -54,6 +54,20 @@ struct LaunchCtx {
   hipStream_t stream;
 };
 
+// one launch over an ensemble of lattices (k_stream_batch)
+struct BatchLaunch {
+  int storage, force_mode, mode;
+  const void *la;  // LatArgs<S>[2 * nlat]: both buffer parities per lattice
+  const uint32_t *tile_lat, *tile_first;
+  uint32_t ntile, per;
+  int parity;
+  unsigned grid, block;
+  hipStream_t stream;
+};
+size_t batch_args_size(int storage);
+void batch_fill_args(const LaunchCtx &L, const void *in, void *out, void *dst);
+hipError_t launch_stream_batch(const BatchLaunch &B);
+
 hipError_t launch_stream(const LaunchCtx &L, const void *fin, void *fout, int mode);
 hipError_t launch_collide(const LaunchCtx &L, void *buf);
 hipError_t launch_macro(const LaunchCtx &L, const void *buf);

@@ -366,4 +366,17 @@ __device__ __forceinline__ void finish_cell(typename Store<S>::C (&f)[Q], uint8_
   });
 }
 
+// The whole hot-kernel body for one cell (see k_stream).
+template <class S, int FORCE, bool COLLIDE, bool EMIT>
+__device__ __forceinline__ void stream_cell(const Planes<S> &P, const uint8_t *__restrict__ mask,
+                                            const KArgs<typename Store<S>::C> &a, uint32_t idx) {
+  using C = typename Store<S>::C;
+  C f[Q];
+  bulk_loads<S>(P, idx, f);  // in flight together with the mask load
+  const uint8_t mk = (uint8_t)after_loads(mask[idx], f);
+  if (!(mk & M_FLUID)) return;
+  if (mk & (M_WALL | M_WRAP | M_BACKEQ)) patch_cell<S>(P, mk, idx, a, f);
+  finish_cell<S, FORCE, COLLIDE, EMIT>(f, mk, idx, a, P);
+}
+
 }  // namespace lbm

@@ -145,6 +145,15 @@ _SIGNATURES = {
     "lbm_output_vtk_begin": (C.c_int, [_P, C.POINTER(_VtkDesc)]),
     "lbm_output_vtk_end": (C.c_int, [_P, C.c_char_p]),
     "lbm_output_vtk": (C.c_int, [_P, C.POINTER(_VtkDesc), C.c_char_p]),
+    "lbm_batch_create": (C.c_int, [C.POINTER(_P), C.c_int32, C.POINTER(_P)]),
+    "lbm_batch_destroy": (C.c_int, [_P]),
+    "lbm_batch_step": (C.c_int, [_P, C.c_int32]),
+    "lbm_batch_time_steps": (C.c_int, [_P, C.c_int32, _D]),
+    "lbm_batch_set_launch": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "lbm_batch_get_launch": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "lbm_batch_nfluid": (C.c_int64, [_P]),
+    "lbm_batch_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _D]),
+    "lbm_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32)]),
 }
 
 _lib = None
@@ -391,6 +400,11 @@ class Lattice:
                "lbm_ibm_markers")
         return dict(x=x, y=y, z=z, ub=ub, Fb=Fb)
 
+    def autotune(self, nsteps: int = 20) -> int:
+        b = C.c_int32(0)
+        _check(self._lib.lbm_autotune(self._h, int(nsteps), C.byref(b)), "lbm_autotune")
+        return b.value
+
     def set_launch(self, block: int, cells_per_thread: int = 1):
         _check(self._lib.lbm_set_launch(self._h, int(block), int(cells_per_thread)), "lbm_set_launch")
 
@@ -410,3 +424,53 @@ class Lattice:
     @property
     def bytes_per_cell(self) -> float:
         return float(self._lib.lbm_bytes_per_cell(self._h))
+
+
+class Batch:
+    """An ensemble of Lattice objects stepped by one launch per step
+    (lbm_batch_*).  The lattices must stay alive while the batch is used."""
+
+    def __init__(self, lattices):
+        self._lib = load_library()
+        self.lattices = list(lattices)
+        arr = (_P * len(self.lattices))(*[lat._h for lat in self.lattices])
+        h = _P()
+        _check(self._lib.lbm_batch_create(arr, len(self.lattices), C.byref(h)), "lbm_batch_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.lbm_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n: int):
+        _check(self._lib.lbm_batch_step(self._h, int(n)), "lbm_batch_step")
+
+    def time_steps(self, n: int) -> float:
+        ms = C.c_double(0)
+        _check(self._lib.lbm_batch_time_steps(self._h, int(n), C.byref(ms)), "lbm_batch_time_steps")
+        return ms.value
+
+    def set_launch(self, block: int, per_cu: int):
+        _check(self._lib.lbm_batch_set_launch(self._h, int(block), int(per_cu)), "lbm_batch_set_launch")
+
+    def get_launch(self):
+        b, p = C.c_int32(0), C.c_int32(0)
+        _check(self._lib.lbm_batch_get_launch(self._h, C.byref(b), C.byref(p)), "lbm_batch_get_launch")
+        return b.value, p.value
+
+    def autotune(self, nsteps: int = 20):
+        b, p, ms = C.c_int32(0), C.c_int32(0), C.c_double(0)
+        _check(self._lib.lbm_batch_autotune(self._h, int(nsteps), C.byref(b), C.byref(p), C.byref(ms)),
+               "lbm_batch_autotune")
+        return b.value, p.value, ms.value
+
+    @property
+    def nfluid(self) -> int:
+        return int(self._lib.lbm_batch_nfluid(self._h))
