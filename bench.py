@@ -164,6 +164,66 @@ def ibm_leg(steps, cmp_steps, warmup=20):
         f32_vs_f64=cmp)
 
 
+def batch_leg(steps, storages=("f32", "f16s"), count=96, tune_steps=10):
+    """The cavity ensemble: `count` lid-driven cavities, edges drawn from
+    {32, 48, 64, 96, 128}, lid velocity and Re from their ranges (seeded).
+    Per storage: sequential per-case launches (each cavity alone, its block
+    size tuned per size class) against one batched launch per step for all
+    cavities (launch shape tuned), both over `steps` fused steps per cavity,
+    HIP-event timed.  The tuners step the cavities, so the timed steps start
+    a few steps after Init."""
+    specs = cases.batch_cavities(count)
+    out = {"cavities": count, "steps": steps,
+           "edges": sorted(set(s["N"] for s in specs)),
+           "fluid_cells": int(sum(s["N"] ** 3 for s in specs))}
+    for st in storages:
+        lats = []
+        for sp in specs:
+            c = cases.cavity(sp["N"], u_lid=sp["u_lid"], Re=sp["Re"])
+            lat = lbm_amd.Lattice(c.Nx, c.Ny, c.Nz, c.tau, storage=st)
+            lat.upload(**c.fields())
+            lat.init()
+            lats.append(lat)
+        # sequential: per size class the tuned block, then each cavity alone
+        tuned = {}
+        for lat, sp in zip(lats, specs):
+            if sp["N"] not in tuned:
+                tuned[sp["N"]] = lat.autotune(tune_steps)
+            lat.set_launch(tuned[sp["N"]])
+        seq_ms = 0.0
+        per_class = {}
+        for lat, sp in zip(lats, specs):
+            t = lat.time_steps(steps)
+            seq_ms += t
+            pc = per_class.setdefault(sp["N"], [0, 0.0])
+            pc[0] += lat.nfluid
+            pc[1] += t
+        codes = [lat.sync() for lat in lats]
+        nfl = sum(lat.nfluid for lat in lats)
+        # batched
+        b = lbm_amd.Batch(lats)
+        blk, per, _ = b.autotune(tune_steps)
+        bat_ms = b.time_steps(steps)
+        b.close()
+        for lat in lats:
+            lat.close()
+        codes_b = [lat.sync() for lat in lats]
+        out[st] = {
+            "sequential_mlups": round(nfl * steps / (seq_ms / 1e3) / 1e6, 1),
+            "batched_mlups": round(nfl * steps / (bat_ms / 1e3) / 1e6, 1),
+            "speedup": round(seq_ms / bat_ms, 3),
+            "sequential_mlups_by_edge": {str(k): round(v[0] * steps / (v[1] / 1e3) / 1e6, 1)
+                                         for k, v in sorted(per_class.items())},
+            "sequential_blocks": {str(k): v for k, v in sorted(tuned.items())},
+            "batched_launch": {"block": blk, "workgroups_per_cu": per},
+            # BGK at tau close to 1/2 (small edge, high Re) diverges as the
+            # reference's would: density out of range, the step goes on
+            "diverged": int(sum(1 for c in codes_b if c != 0)),
+        }
+        log(f"bench: batch {st} {json.dumps(out[st])}")
+    return out
+
+
 def cpu_baseline(case, budget_s, max_steps):
     """The CPU oracle on a bounded sample of the same workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -223,6 +283,9 @@ def main():
     ap.add_argument("--ibm-cmp-steps", type=int, default=2000)
     ap.add_argument("--no-ibm", action="store_true")
     ap.add_argument("--ibm-only", action="store_true", help="run only the IBM channel leg")
+    ap.add_argument("--batch-steps", type=int, default=4000)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-only", action="store_true", help="run only the cavity ensemble leg")
     ap.add_argument("--copy-probe-only", action="store_true",
                     help="run only the 1 GiB device copy (counter scaling reference)")
     a = ap.parse_args()
@@ -238,6 +301,9 @@ def main():
 
     if a.ibm_only:
         print(json.dumps(ibm_leg(a.ibm_steps, a.ibm_cmp_steps)), flush=True)
+        return 0
+    if a.batch_only:
+        print(json.dumps(batch_leg(a.batch_steps)), flush=True)
         return 0
 
     case = cases.taylor_green(a.size, u0=0.04, Re=1600.0)
@@ -265,6 +331,11 @@ def main():
         log(f"bench: IBM channel leg, {a.ibm_steps} steps")
         ibm = ibm_leg(a.ibm_steps, a.ibm_cmp_steps)
         log(f"bench: ibm {json.dumps(ibm)}")
+
+    batch = None
+    if not a.no_batch:
+        log(f"bench: cavity ensemble leg, {a.batch_steps} steps")
+        batch = batch_leg(a.batch_steps)
 
     copy_gbs = lbm_amd.copy_probe(1 << 30, 20)
     log(f"bench: 1 GiB device copy {copy_gbs:.0f} GB/s")
@@ -310,6 +381,7 @@ def main():
         },
         "precision": prec,
         "ibm": ibm,
+        "batch": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
