@@ -53,6 +53,9 @@ LaunchCtx launch_ctx(const lbm_ctx *c) {
   L.first_bad = c->first_bad;
   L.stream = c->stream;
   L.box = c->box;
+  // bounce-back through the solids' slots (the thermal kernels patch their
+  // wall links themselves)
+  L.wall_slots = c->thermal ? 0 : 1;
   return L;
 }
 
@@ -297,6 +300,14 @@ int lbm_init(lbm_ctx *c) {
     if (int e = thermal_init(c)) return e;
   } else {
     HIPCHK(launch_init(L, c->cur, c->oth));
+    if (any_solid && L.wall_slots) {
+      // the solids' slots will carry bounce-back values: their MacroProp
+      // values (moments of their Init populations, which never change) are
+      // taken now, for the wall tables after the first MacroProp
+      LaunchCtx Ls = L;
+      Ls.macro_sel = 16;  // TYPE_S only
+      HIPCHK(launch_macro(Ls, c->cur));
+    }
   }
   if (int e = ibm_remark(c)) return e;
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -328,9 +339,11 @@ static int walls_from_moments(lbm_ctx *c, bool solids_done = false) {
   if (!solids_done) {
     if (c->thermal) {
       if (int e = thermal_launch(c, 4, 16)) return e;
-    } else {
+    } else if (!L.wall_slots) {
       HIPCHK(launch_macro(L, c->cur));
     }
+    // with wall slots the solids' values were taken at lbm_init (and by the
+    // back-face write, after_stream)
   }
   HIPCHK(launch_tables(L, c->mask, c->links, c->uw[0], c->uw[1], c->uw[2], c->type));
   if (c->thermal) {
@@ -382,7 +395,11 @@ int lbm_macro(lbm_ctx *c) {
   if (c->thermal) {
     if (int e = thermal_launch(c, 4, 1 | 16)) return e;
   } else {
-    HIPCHK(launch_macro(launch_ctx(c), c->cur));
+    LaunchCtx L = launch_ctx(c);
+    // with wall slots the solids' populations are no longer in the buffer;
+    // their (constant) MacroProp values are already stored
+    if (L.wall_slots) L.macro_sel = 1;
+    HIPCHK(launch_macro(L, c->cur));
   }
   return walls_from_moments(c, true);
 }

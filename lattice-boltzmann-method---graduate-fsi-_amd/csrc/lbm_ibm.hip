@@ -596,6 +596,7 @@ static KArgs<typename Store<S>::C> kargs_of(const LaunchCtx &L) {
   a.uz = (C *)L.uz;
   a.status = L.status;
   a.first_bad = L.first_bad;
+  a.wall_slots = L.wall_slots;
   return a;
 }
 

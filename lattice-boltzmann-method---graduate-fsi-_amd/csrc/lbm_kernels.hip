@@ -106,6 +106,11 @@ __global__ __launch_bounds__(256) void k_collide(S *__restrict__ buf, const uint
     constexpr int m = decltype(M)::value;
     Store<S>::st(buf + (uint32_t)m * ps + idx, f[m]);
   });
+  if (a.wall_slots && (mk & M_WALL)) {
+    S *planes[Q];
+    for (int m = 0; m < Q; m++) planes[m] = buf + (size_t)m * ps;
+    wall_slot_stores<S>(f, mk, idx, a.links[idx], a, planes);
+  }
   if (neg) flag(a.status, a.first_bad, ST_FEQ_NEG, idx);
 }
 
@@ -236,7 +241,7 @@ __global__ __launch_bounds__(256) void k_tables(const int16_t *__restrict__ type
 
 // TYPE_S cells hit by the back-face inflow write (M_BACKEQ): from the first
 // Streaming on they hold feq2 of their TYPE_IN partner (lbm.cpp:599-611).
-template <class S>
+template <class S, int FORCE>
 __global__ __launch_bounds__(256) void k_backeq_solid(S *__restrict__ A, S *__restrict__ B,
                                                       const uint8_t *__restrict__ mask,
                                                       KArgs<typename Store<S>::C> a) {
@@ -249,13 +254,30 @@ __global__ __launch_bounds__(256) void k_backeq_solid(S *__restrict__ A, S *__re
   const C r = a.rho[s], vx = a.ux[s], vy = a.uy[s], vz = a.uz[s];
   const C uu = vx * vx + vy * vy + vz * vz;
   const uint32_t ps = a.g.ps;
+  C f[Q];
   for_dirs([&](auto M) {
     constexpr int m = decltype(M)::value;
     const C cu = C(CX[m]) * vx + C(CY[m]) * vy + C(CZ[m]) * vz;
-    const S v = Store<S>::enc(feq2_raw<C, Store<S>::shifted>(r, cu, uu, C(W[m]), C(Store<S>::K)));
+    f[m] = feq2_raw<C, Store<S>::shifted>(r, cu, uu, C(W[m]), C(Store<S>::K));
+    const S v = Store<S>::enc(f[m]);
     A[(uint32_t)m * ps + idx] = v;
     B[(uint32_t)m * ps + idx] = v;
   });
+  if (a.wall_slots) {
+    // the solid's slots will carry bounce-back values: its MacroProp values
+    // (moments of the stored populations, lbm.cpp:668-675) are kept now
+    for_dirs([&](auto M) {
+      constexpr int m = decltype(M)::value;
+      f[m] = Store<S>::ld(&A[(uint32_t)m * ps + idx]);
+    });
+    C rho, drho, jx, jy, jz, Fx, Fy, Fz;
+    moments<S>(f, rho, drho, jx, jy, jz);
+    load_force<C, FORCE>(a, idx, Fx, Fy, Fz);
+    a.rho[idx] = rho;
+    a.ux[idx] = (jx + C(0.5) * Fx) / rho;
+    a.uy[idx] = (jy + C(0.5) * Fy) / rho;
+    a.uz[idx] = (jz + C(0.5) * Fz) / rho;
+  }
 }
 
 // The streaming tables exactly as the stream kernels resolve them.
@@ -288,9 +310,10 @@ __global__ __launch_bounds__(256) void k_links(const uint8_t *__restrict__ mask,
   });
 }
 
-// Populations as FP64 f (unshifted), [idx][m].  TYPE_F and TYPE_S cells come
-// from the buffer; other cells report feq of their stored rho,u (their Init
-// value; their slots hold their neighbours' frozen values).
+// Populations as FP64 f (unshifted), [idx][m].  TYPE_F cells (and TYPE_S
+// cells unless their slots carry bounce-back values) come from the buffer;
+// other cells report feq of their stored rho,u (their Init value; their
+// slots hold values for their neighbours).
 template <class S>
 __global__ __launch_bounds__(256) void k_f_out(const S *__restrict__ buf,
                                                const uint8_t *__restrict__ mask,
@@ -306,7 +329,7 @@ __global__ __launch_bounds__(256) void k_f_out(const S *__restrict__ buf,
   for_dirs([&](auto M) {
     constexpr int m = decltype(M)::value;
     double v;
-    if (mk & (M_FLUID | M_SOLID)) {
+    if ((mk & M_FLUID) || ((mk & M_SOLID) && !a.wall_slots)) {
       v = (double)Store<S>::ld(buf + (uint32_t)m * ps + idx);
       if constexpr (Store<S>::shifted) v = v / (double)Store<S>::K + W[m];
     } else {
@@ -408,6 +431,7 @@ static KArgs<typename Store<S>::C> make_args(const LaunchCtx &L) {
   a.status = L.status;
   a.first_bad = L.first_bad;
   a.box = L.box;
+  a.wall_slots = L.wall_slots;
   return a;
 }
 
@@ -538,20 +562,23 @@ hipError_t launch_tables(const LaunchCtx &L, uint8_t *mask, uint32_t *links, voi
   return hipGetLastError();
 }
 
-hipError_t launch_backeq_solid(const LaunchCtx &L, void *A, void *B) {
+template <class S>
+static void backeq_s(const LaunchCtx &L, void *A, void *B) {
   const dim3 grid = grid_for(L.g.ncell, L.block);
+  auto a = make_args<S>(L);
+  switch (L.force_mode) {
+    case 1: hipLaunchKernelGGL((k_backeq_solid<S, 1>), grid, dim3(L.block), 0, L.stream, (S *)A, (S *)B, L.mask, a); break;
+    case 2: hipLaunchKernelGGL((k_backeq_solid<S, 2>), grid, dim3(L.block), 0, L.stream, (S *)A, (S *)B, L.mask, a); break;
+    default: hipLaunchKernelGGL((k_backeq_solid<S, 0>), grid, dim3(L.block), 0, L.stream, (S *)A, (S *)B, L.mask, a); break;
+  }
+}
+
+hipError_t launch_backeq_solid(const LaunchCtx &L, void *A, void *B) {
   switch (L.storage) {
-    case 0:
-      hipLaunchKernelGGL(k_backeq_solid<double>, grid, dim3(L.block), 0, L.stream, (double *)A, (double *)B, L.mask, make_args<double>(L));
-      break;
-    case 1:
-      hipLaunchKernelGGL(k_backeq_solid<float>, grid, dim3(L.block), 0, L.stream, (float *)A, (float *)B, L.mask, make_args<float>(L));
-      break;
-    case 2:
-      hipLaunchKernelGGL(k_backeq_solid<__half>, grid, dim3(L.block), 0, L.stream, (__half *)A, (__half *)B, L.mask, make_args<__half>(L));
-      break;
-    default:
-      return hipErrorInvalidValue;
+    case 0: backeq_s<double>(L, A, B); break;
+    case 1: backeq_s<float>(L, A, B); break;
+    case 2: backeq_s<__half>(L, A, B); break;
+    default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }

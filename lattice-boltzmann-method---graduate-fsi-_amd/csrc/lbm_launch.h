@@ -48,6 +48,7 @@ struct LaunchCtx {
   void *rho, *ux, *uy, *uz;
   const uint8_t *mask;
   BoxArgs box;        // force mode 3
+  int wall_slots;     // bounce-back through the solids' slots
   uint8_t macro_sel;  // k_macro: cells whose mask has any of these bits
   int *status;
   unsigned long long *first_bad;

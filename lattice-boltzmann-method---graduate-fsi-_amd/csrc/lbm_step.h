@@ -23,6 +23,10 @@ struct KArgs {
   C *rho, *ux, *uy, *uz;         // macroscopic arrays
   int *status;
   unsigned long long *first_bad;
+  // Bounce-back through the solid's slots: after its collision a wall cell x
+  // writes f*(x, opp m) + 2 w_m (u_w . c_m)/cs2 into slot m of the solid
+  // x - c_m, which the next pull then reads like any streamed value.
+  int wall_slots;
 };
 
 // Per-direction plane pointers, computed on the host for each launch so
@@ -223,6 +227,41 @@ __device__ __forceinline__ void bulk_loads(const Planes<S> &P, uint32_t idx,
   });
 }
 
+// Bulk loads with the periodic wrap of the padded lattice (lbm.cpp:197-199)
+// folded into the addresses: a lane on the shell adds, per direction, the
+// offset that takes its unwrapped source back into the lattice (+-nx cells
+// in x, +-ny rows in y, +-nz planes in z).  Off the shell every correction is
+// zero, and no lane needs a second, dependent round of loads.
+// The lane's 32-bit byte offset must not go negative: it is taken from a
+// base lowered by the largest negative correction (bias), so that
+// offset = idx*size + bias + correction >= 0 for every lane.
+template <class S>
+__device__ __forceinline__ void bulk_loads_wrapped(const Planes<S> &P, const Geo &g, uint32_t idx,
+                                                   typename Store<S>::C (&f)[Q]) {
+  const uint32_t x = idx % g.px, t = idx / g.px, y = t % g.ny, z = t / g.ny;
+  const uint32_t e = (uint32_t)sizeof(S);
+  const uint32_t ex = g.nx * e, ey = g.ny * g.px * e, ez = g.nz * g.plane * e;
+  const uint32_t bias_elems = g.nx + g.ny * g.px + g.nz * g.plane;
+  const uint32_t bias = bias_elems * e;
+  // byte corrections: a source at x - 1 (direction c_x = +1) on the x = 0
+  // face wraps by +nx cells, a source at x + 1 on the x = nx - 1 face by -nx
+  const uint32_t xp = x == 0 ? ex : 0u, xm = x == g.nx - 1 ? ex : 0u;
+  const uint32_t yp = y == 0 ? ey : 0u, ym = y == g.ny - 1 ? ey : 0u;
+  const uint32_t zp = z == 0 ? ez : 0u, zm = z == g.nz - 1 ? ez : 0u;
+  const uint32_t boff = idx * e + bias;
+  for_dirs([&](auto M) {
+    constexpr int m = decltype(M)::value;
+    uint32_t o = boff;
+    if constexpr (CX[m] > 0) o += xp;
+    if constexpr (CX[m] < 0) o -= xm;
+    if constexpr (CY[m] > 0) o += yp;
+    if constexpr (CY[m] < 0) o -= ym;
+    if constexpr (CZ[m] > 0) o += zp;
+    if constexpr (CZ[m] < 0) o -= zm;
+    f[m] = Store<S>::ld(at(P.src[m] - bias_elems, o));
+  });
+}
+
 // An empty asm statement that takes the bulk-loaded values as inputs and
 // passes the mask byte through: the mask test then depends on every load
 // having completed, so neither the loads nor the test can be moved across
@@ -238,14 +277,19 @@ __device__ __forceinline__ uint32_t after_loads(uint32_t mk, const C (&f)[Q]) {
                  "v"(f[13]), "v"(f[14]), "v"(f[15]), "v"(f[16]), "v"(f[17]), "v"(f[18]));
   return mk;
 }
+// as after_loads, for a second value loaded alongside the mask (the
+// bounce-back link word)
+__device__ __forceinline__ void after_loads2(uint32_t &mk, uint32_t &lk) {
+  asm volatile("" : "+v"(mk), "+v"(lk));
+}
 
-template <class S>
+template <class S, bool WRAPPED = false>
 __device__ __forceinline__ void patch_cell(const Planes<S> &P, uint8_t mk, uint32_t idx,
                                            const KArgs<typename Store<S>::C> &a,
                                            typename Store<S>::C (&f)[Q]) {
   using C = typename Store<S>::C;
   const uint32_t boff = idx * (uint32_t)sizeof(S);
-  if (mk & M_WRAP) {
+  if (!WRAPPED && (mk & M_WRAP)) {
     // periodic wrap of the padded lattice (lbm.cpp:197-199)
     uint32_t x, y, z;
     coords(a.g, idx, x, y, z);
@@ -255,7 +299,7 @@ __device__ __forceinline__ void patch_cell(const Planes<S> &P, uint8_t mk, uint3
       if (src != idx - pull_off<m>(a.g)) f[m] = Store<S>::ld(at(P.in[m], src * (uint32_t)sizeof(S)));
     });
   }
-  if (mk & M_WALL) {
+  if ((mk & M_WALL) && !a.wall_slots) {
     const uint32_t lk = a.links[idx];
     C uwx = C(0), uwy = C(0), uwz = C(0);
     if (mk & M_MOVING) {
@@ -285,12 +329,45 @@ __device__ __forceinline__ void patch_cell(const Planes<S> &P, uint8_t mk, uint3
   }
 }
 
+// mask bits whose cells need patch_cell after the bulk loads
+template <class C>
+__device__ __forceinline__ uint8_t patch_bits(const KArgs<C> &a) {
+  return a.wall_slots ? (uint8_t)(M_WRAP | M_BACKEQ) : (uint8_t)(M_WALL | M_WRAP | M_BACKEQ);
+}
+
+// The bounce-back values a wall cell hands to its solid neighbours' slots
+// (wall_slots): slot m of x - c_m receives f*(x, opp m) + 6 K w_m (u_w . c_m),
+// the value the reference's push writes into f(x, m) (lbm.cpp:268-275).
+template <class S>
+__device__ __forceinline__ void wall_slot_stores(const typename Store<S>::C (&f)[Q], uint8_t mk,
+                                                 uint32_t idx, uint32_t lk,
+                                                 const KArgs<typename Store<S>::C> &a,
+                                                 S *const (&out)[Q]) {
+  using C = typename Store<S>::C;
+  C uwx = C(0), uwy = C(0), uwz = C(0);
+  if (mk & M_MOVING) {
+    uwx = a.uwx[idx];
+    uwy = a.uwy[idx];
+    uwz = a.uwz[idx];
+  }
+  uint32_t x = 0, y = 0, z = 0;
+  if (mk & M_WRAP) coords(a.g, idx, x, y, z);
+  for_dirs([&](auto M) {
+    constexpr int m = decltype(M)::value;
+    if (lk & (1u << m)) {
+      const uint32_t s = (mk & M_WRAP) ? wrap_src(a.g, x, y, z, CX[m], CY[m], CZ[m]) : idx - pull_off<m>(a.g);
+      const C ucw = C(CX[m]) * uwx + C(CY[m]) * uwy + C(CZ[m]) * uwz;
+      Store<S>::st(at(out[m], s * (uint32_t)sizeof(S)), f[OPP[m]] + C(Store<S>::K * 6.0 * W[m]) * ucw);
+    }
+  });
+}
+
 template <class S>
 __device__ __forceinline__ void pull_cell(const Planes<S> &P, uint8_t mk, uint32_t idx,
                                           const KArgs<typename Store<S>::C> &a,
                                           typename Store<S>::C (&f)[Q]) {
   bulk_loads<S>(P, idx, f);
-  if (mk & (M_WALL | M_WRAP | M_BACKEQ)) patch_cell<S>(P, mk, idx, a, f);
+  if (mk & patch_bits(a)) patch_cell<S>(P, mk, idx, a, f);
 }
 
 // Box index of a cell inside the IBM box (mask bit M_BOX).
@@ -341,7 +418,7 @@ __device__ __forceinline__ void cell_state(const typename Store<S>::C (&f)[Q], u
 template <class S, int FORCE, bool COLLIDE, bool EMIT>
 __device__ __forceinline__ void finish_cell(typename Store<S>::C (&f)[Q], uint8_t mk, uint32_t idx,
                                             const KArgs<typename Store<S>::C> &a,
-                                            const Planes<S> &P) {
+                                            const Planes<S> &P, uint32_t lk = 0xffffffffu) {
   using C = typename Store<S>::C;
   if constexpr (COLLIDE || EMIT) {
     C rho, drho, ux, uy, uz, Fx, Fy, Fz;
@@ -364,6 +441,10 @@ __device__ __forceinline__ void finish_cell(typename Store<S>::C (&f)[Q], uint8_
     constexpr int m = decltype(M)::value;
     Store<S>::st(at(P.out[m], boff), f[m]);
   });
+  if constexpr (COLLIDE) {
+    if (a.wall_slots && (mk & M_WALL))
+      wall_slot_stores<S>(f, mk, idx, lk == 0xffffffffu ? a.links[idx] : lk, a, P.out);
+  }
 }
 
 // The whole hot-kernel body for one cell (see k_stream).
@@ -372,11 +453,19 @@ __device__ __forceinline__ void stream_cell(const Planes<S> &P, const uint8_t *_
                                             const KArgs<typename Store<S>::C> &a, uint32_t idx) {
   using C = typename Store<S>::C;
   C f[Q];
-  bulk_loads<S>(P, idx, f);  // in flight together with the mask load
-  const uint8_t mk = (uint8_t)after_loads(mask[idx], f);
+  // the populations (periodic wrap in the addresses), the mask and, where
+  // solids exist, the bounce-back link word are all in flight together
+  bulk_loads_wrapped<S>(P, a.g, idx, f);
+  uint32_t mk = mask[idx];
+  uint32_t lk = 0;
+  if (a.links && a.wall_slots) {
+    lk = a.links[idx];
+    after_loads2(mk, lk);
+  }
+  mk = after_loads(mk, f);
   if (!(mk & M_FLUID)) return;
-  if (mk & (M_WALL | M_WRAP | M_BACKEQ)) patch_cell<S>(P, mk, idx, a, f);
-  finish_cell<S, FORCE, COLLIDE, EMIT>(f, mk, idx, a, P);
+  if (mk & (patch_bits(a) & ~M_WRAP)) patch_cell<S, true>(P, (uint8_t)mk, idx, a, f);
+  finish_cell<S, FORCE, COLLIDE, EMIT>(f, (uint8_t)mk, idx, a, P, lk);
 }
 
 }  // namespace lbm
