@@ -181,13 +181,20 @@ __global__ __launch_bounds__(256) void k_init(S *__restrict__ A, S *__restrict__
 // Per-cell kernel mask, bounce-back link word and wall velocity from the
 // cell types (device order; row-padding cells get mask 0) and the solids'
 // current velocities.
+//
+// With wall slots (line_cells > 0, the cells of one 128-byte line of a
+// population plane), solid and row-padding cells get a fill word instead of
+// link bits: the slots that no wall cell writes, set only where the cell's
+// line also holds a fluid cell.  The stream kernel writes those slots (with
+// values nobody reads), so that the lines a row of fluid cells stores are
+// written whole rather than with holes at the solids and the row padding.
 template <class C>
 __global__ __launch_bounds__(256) void k_tables(const int16_t *__restrict__ type,
                                                 uint8_t *__restrict__ mask,
                                                 uint32_t *__restrict__ links, C *uwx, C *uwy,
                                                 C *uwz, Geo g, const C *__restrict__ ux,
                                                 const C *__restrict__ uy,
-                                                const C *__restrict__ uz) {
+                                                const C *__restrict__ uz, uint32_t line_cells) {
   const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= g.ncell) return;
   uint32_t x, y, z;
@@ -195,6 +202,23 @@ __global__ __launch_bounds__(256) void k_tables(const int16_t *__restrict__ type
   uint8_t mk = 0;
   uint32_t lk = 0;
   C wx = C(0), wy = C(0), wz = C(0);
+  if (line_cells && (x >= g.nx || type[idx] == 1)) {
+    bool line_fluid = false;
+    const uint32_t j0 = idx - idx % line_cells;
+    for (uint32_t j = j0; j < j0 + line_cells && j < g.ncell; j++)
+      if (j % g.px < g.nx && type[j] == 0) line_fluid = true;
+    if (line_fluid) {
+      lk = (1u << Q) - 1;
+      if (x < g.nx) {
+        // slot m of a solid is written by its neighbour x + c_m when that is
+        // a fluid cell (wall_slot_stores)
+        for_dirs([&](auto M) {
+          constexpr int m = decltype(M)::value;
+          if (type[wrap_dst(g, x, y, z, CX[m], CY[m], CZ[m])] == 0) lk &= ~(1u << m);
+        });
+      }
+    }
+  }
   if (x < g.nx) {
     const int16_t t = type[idx];
     if (t == 0) mk |= M_FLUID;
@@ -551,14 +575,15 @@ hipError_t launch_init(const LaunchCtx &L, void *A, void *B) {
 hipError_t launch_tables(const LaunchCtx &L, uint8_t *mask, uint32_t *links, void *uwx,
                          void *uwy, void *uwz, const int16_t *type) {
   const dim3 grid = grid_for(L.g.ncell, L.block);
+  const uint32_t line_cells = L.wall_slots ? 128u / (L.storage == 0 ? 8u : L.storage == 1 ? 4u : 2u) : 0u;
   if (L.storage == 0)
     hipLaunchKernelGGL(k_tables<double>, grid, dim3(L.block), 0, L.stream, type, mask, links,
                        (double *)uwx, (double *)uwy, (double *)uwz, L.g, (const double *)L.ux,
-                       (const double *)L.uy, (const double *)L.uz);
+                       (const double *)L.uy, (const double *)L.uz, line_cells);
   else
     hipLaunchKernelGGL(k_tables<float>, grid, dim3(L.block), 0, L.stream, type, mask, links,
                        (float *)uwx, (float *)uwy, (float *)uwz, L.g, (const float *)L.ux,
-                       (const float *)L.uy, (const float *)L.uz);
+                       (const float *)L.uy, (const float *)L.uz, line_cells);
   return hipGetLastError();
 }
 

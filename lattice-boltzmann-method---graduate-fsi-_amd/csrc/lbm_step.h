@@ -414,11 +414,10 @@ __device__ __forceinline__ void cell_state(const typename Store<S>::C (&f)[Q], u
   }
 }
 
-// Moments, optional macro emission, optional collision, store of one cell.
+// Moments, optional macro emission and optional collision of one fluid cell.
 template <class S, int FORCE, bool COLLIDE, bool EMIT>
-__device__ __forceinline__ void finish_cell(typename Store<S>::C (&f)[Q], uint8_t mk, uint32_t idx,
-                                            const KArgs<typename Store<S>::C> &a,
-                                            const Planes<S> &P, uint32_t lk = 0xffffffffu) {
+__device__ __forceinline__ void update_cell(typename Store<S>::C (&f)[Q], uint8_t mk, uint32_t idx,
+                                            const KArgs<typename Store<S>::C> &a) {
   using C = typename Store<S>::C;
   if constexpr (COLLIDE || EMIT) {
     C rho, drho, ux, uy, uz, Fx, Fy, Fz;
@@ -436,22 +435,22 @@ __device__ __forceinline__ void finish_cell(typename Store<S>::C (&f)[Q], uint8_
     }
     if (bad) flag(a.status, a.first_bad, bad, idx);
   }
-  const uint32_t boff = idx * (uint32_t)sizeof(S);
-  for_dirs([&](auto M) {
-    constexpr int m = decltype(M)::value;
-    Store<S>::st(at(P.out[m], boff), f[m]);
-  });
-  if constexpr (COLLIDE) {
-    if (a.wall_slots && (mk & M_WALL))
-      wall_slot_stores<S>(f, mk, idx, lk == 0xffffffffu ? a.links[idx] : lk, a, P.out);
-  }
 }
 
 // The whole hot-kernel body for one cell (see k_stream).
+//
+// Solid and row-padding cells whose 128-byte lines hold fluid cells join the
+// fluid cells' stores with zeros in the slots of their fill word (all slots
+// of a padding cell when there are no solids): every such line then leaves
+// the wave whole, from one store instruction, instead of with holes at the
+// solids and the padding.  Nothing reads these slots: with wall slots a
+// solid's populations are only bounce-back carriers.  Cells of the other
+// types keep their frozen-link values.
 template <class S, int FORCE, bool COLLIDE, bool EMIT>
 __device__ __forceinline__ void stream_cell(const Planes<S> &P, const uint8_t *__restrict__ mask,
                                             const KArgs<typename Store<S>::C> &a, uint32_t idx) {
   using C = typename Store<S>::C;
+  constexpr uint32_t ALL = (1u << Q) - 1;
   C f[Q];
   // the populations (periodic wrap in the addresses), the mask and, where
   // solids exist, the bounce-back link word are all in flight together
@@ -463,9 +462,31 @@ __device__ __forceinline__ void stream_cell(const Planes<S> &P, const uint8_t *_
     after_loads2(mk, lk);
   }
   mk = after_loads(mk, f);
-  if (!(mk & M_FLUID)) return;
-  if (mk & (patch_bits(a) & ~M_WRAP)) patch_cell<S, true>(P, (uint8_t)mk, idx, a, f);
-  finish_cell<S, FORCE, COLLIDE, EMIT>(f, (uint8_t)mk, idx, a, P, lk);
+  uint32_t st = ALL;  // slots this lane stores
+  if (mk & M_FLUID) {
+    if (mk & (patch_bits(a) & ~M_WRAP)) patch_cell<S, true>(P, (uint8_t)mk, idx, a, f);
+    update_cell<S, FORCE, COLLIDE, EMIT>(f, (uint8_t)mk, idx, a);
+  } else {
+    st = !a.wall_slots ? 0u : a.links ? lk : (mk == 0 ? ALL : 0u);
+    if (!st) return;
+    for (int m = 0; m < Q; m++) f[m] = C(0);
+  }
+  const uint32_t boff = idx * (uint32_t)sizeof(S);
+  if (__builtin_amdgcn_ballot_w64(st != ALL) == 0) {
+    for_dirs([&](auto M) {
+      constexpr int m = decltype(M)::value;
+      Store<S>::st(at(P.out[m], boff), f[m]);
+    });
+  } else {
+    for_dirs([&](auto M) {
+      constexpr int m = decltype(M)::value;
+      if (st & (1u << m)) Store<S>::st(at(P.out[m], boff), f[m]);
+    });
+  }
+  if constexpr (COLLIDE) {
+    if (a.wall_slots && (mk & (M_FLUID | M_WALL)) == (M_FLUID | M_WALL))
+      wall_slot_stores<S>(f, (uint8_t)mk, idx, lk, a, P.out);
+  }
 }
 
 }  // namespace lbm
