@@ -54,6 +54,10 @@ extern "C" {
 #define LBM_STORE_F32 1  /* f - w_i as float                              */
 #define LBM_STORE_F16S 2 /* (f - w_i) * 2^LBM_F16S_SCALE_LOG2 as IEEE half */
 #define LBM_F16S_SCALE_LOG2 12
+/* f - w_i as a 16-bit float of this library's layout: sign, 4 exponent bits
+ * (bias 17), 11 fraction bits, subnormals; |f - w_i| < 0.5, larger values
+ * saturate.  One significant bit more than F16S at the same 2 bytes. */
+#define LBM_STORE_F16C 3
 
 /* lbm_reduce_fp64 quantities */
 #define LBM_REDUCE_MASS 0     /* sum of rho over TYPE_F cells           */
@@ -272,6 +276,12 @@ const char *lbm_error_string(int code);
  * times; *gbs = (read + written bytes) / time.  No reference counterpart: the
  * practical HBM ceiling and the reference point for hardware byte counters. */
 int lbm_copy_probe(int32_t device, int64_t bytes, int32_t reps, double *gbs);
+/* What a population storage keeps of n values: out[i] = in[i] encoded into
+ * storage format `storage` (LBM_STORE_*) by the device and decoded back,
+ * with in[i] the stored quantity in lattice units (f - w_i for the shifted
+ * formats, the F16S scale applied and removed inside).  Host arrays.  No
+ * reference counterpart: it states the storage rounding exactly. */
+int lbm_storage_roundtrip(int32_t device, int32_t storage, const double *in, double *out, int64_t n);
 
 #ifdef __cplusplus
 }

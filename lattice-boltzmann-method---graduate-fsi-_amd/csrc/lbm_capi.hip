@@ -147,7 +147,7 @@ int lbm_create(const lbm_desc *desc, lbm_ctx **out) {
   *out = nullptr;
   if (desc->nx < 1 || desc->ny < 1 || desc->nz < 1) return LBM_ERR_ARG;
   if (desc->lattice != 19) return LBM_ERR_UNSUPPORTED;
-  if (desc->storage < LBM_STORE_F64 || desc->storage > LBM_STORE_F16S) return LBM_ERR_ARG;
+  if (desc->storage < LBM_STORE_F64 || desc->storage > LBM_STORE_F16C) return LBM_ERR_ARG;
   if (!(desc->tau > 0.5) || !std::isfinite(desc->tau)) return LBM_ERR_ARG;
   const uint64_t nx = desc->nx + 2ull, ny = desc->ny + 2ull, nz = desc->nz + 2ull;
   const uint64_t px = (nx + 3) / 4 * 4;  // row pitch: 4-cell chunks start aligned

@@ -103,6 +103,7 @@ struct Store<double> {
   static constexpr double K = 1.0;
   static __device__ __forceinline__ double ld(const double *p) { return *p; }
   static __device__ __forceinline__ void st(double *p, double r) { __builtin_nontemporal_store(r, p); }
+  static __device__ __forceinline__ void st_raw(double *p, double v) { __builtin_nontemporal_store(v, p); }
   static __device__ __forceinline__ double enc(double r) { return r; }
 };
 
@@ -113,6 +114,7 @@ struct Store<float> {
   static constexpr float K = 1.0f;
   static __device__ __forceinline__ float ld(const float *p) { return *p; }
   static __device__ __forceinline__ void st(float *p, float r) { __builtin_nontemporal_store(r, p); }
+  static __device__ __forceinline__ void st_raw(float *p, float v) { __builtin_nontemporal_store(v, p); }
   static __device__ __forceinline__ float enc(float r) { return r; }
 };
 
@@ -129,10 +131,53 @@ struct Store<__half> {
   static __device__ __forceinline__ void st(__half *p, float r) {
     __builtin_nontemporal_store((_Float16)r, reinterpret_cast<_Float16 *>(p));
   }
+  static __device__ __forceinline__ void st_raw(__half *p, __half v) {
+    __builtin_nontemporal_store(*reinterpret_cast<const _Float16 *>(&v), reinterpret_cast<_Float16 *>(p));
+  }
   static __device__ __forceinline__ __half enc(float r) {
     _Float16 h = (_Float16)r;
     return *reinterpret_cast<__half *>(&h);
   }
+};
+
+// LBM_STORE_F16C: f - w_i in a 16-bit float of this library's own layout,
+// sign | 4 exponent bits (bias 17) | 11 fraction bits, with subnormals: one
+// more significant bit than IEEE half over |f - w| < 0.5, the range shifted
+// populations occupy (the largest, f_0 - w_0 = w_0 (rho - 1 - 1.5 rho u^2),
+// stays below 0.5 for |rho - 1| < 1.5).  Values beyond it saturate.
+//
+// Decode: the 15 magnitude bits, shifted left by 12, are an FP32 bit pattern
+// whose exponent field is the format's own; multiplying by 2^110 restores
+// the bias (127 - 17), subnormals included since FP32 denormals are kept.
+// Encode is the inverse, rounded to nearest even on the 12 dropped bits.
+struct f16c_t {
+  uint16_t bits;
+};
+
+__device__ __forceinline__ float f16c_decode(int32_t h) {  // h sign-extended
+  return __uint_as_float(((uint32_t)h << 12) & 0x87ffffffu) * 0x1p110f;
+}
+__device__ __forceinline__ uint32_t f16c_encode(float r) {
+  const uint32_t a = __float_as_uint(__builtin_fabsf(r) * 0x1p-110f);
+  const uint32_t h = min((a + 0x7ffu + ((a >> 12) & 1u)) >> 12, 0x7fffu);
+  return h | ((__float_as_uint(r) >> 16) & 0x8000u);
+}
+
+template <>
+struct Store<f16c_t> {
+  using C = float;
+  static constexpr bool shifted = true;
+  static constexpr float K = 1.0f;
+  static __device__ __forceinline__ float ld(const f16c_t *p) {
+    return f16c_decode(*reinterpret_cast<const int16_t *>(p));
+  }
+  static __device__ __forceinline__ void st(f16c_t *p, float r) {
+    __builtin_nontemporal_store((uint16_t)f16c_encode(r), reinterpret_cast<uint16_t *>(p));
+  }
+  static __device__ __forceinline__ void st_raw(f16c_t *p, f16c_t v) {
+    __builtin_nontemporal_store(v.bits, reinterpret_cast<uint16_t *>(p));
+  }
+  static __device__ __forceinline__ f16c_t enc(float r) { return f16c_t{(uint16_t)f16c_encode(r)}; }
 };
 
 // Cubic equilibrium of LBM::Calc_feq (lbm.cpp:628-633) with cs2 = 1/3:

@@ -619,7 +619,8 @@ static hipError_t box_moments(lbm_ctx *c, bool pull) {
   switch (c->storage) {
     case LBM_STORE_F64: return box_moments_s<double>(c, pull);
     case LBM_STORE_F32: return box_moments_s<float>(c, pull);
-    default: return box_moments_s<__half>(c, pull);
+    case LBM_STORE_F16S: return box_moments_s<__half>(c, pull);
+    default: return box_moments_s<f16c_t>(c, pull);
   }
 }
 

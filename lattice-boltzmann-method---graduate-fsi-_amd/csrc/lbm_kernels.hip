@@ -510,6 +510,7 @@ hipError_t launch_stream(const LaunchCtx &L, const void *fin, void *fout, int mo
     case 0: return stream_s<double>(L, fin, fout, mode);
     case 1: return stream_s<float>(L, fin, fout, mode);
     case 2: return stream_s<__half>(L, fin, fout, mode);
+    case 3: return stream_s<f16c_t>(L, fin, fout, mode);
   }
   return hipErrorInvalidValue;
 }
@@ -541,6 +542,7 @@ hipError_t launch_collide(const LaunchCtx &L, void *buf) {
     case 0: return collide_s<double>(L, buf, false);
     case 1: return collide_s<float>(L, buf, false);
     case 2: return collide_s<__half>(L, buf, false);
+    case 3: return collide_s<f16c_t>(L, buf, false);
   }
   return hipErrorInvalidValue;
 }
@@ -550,6 +552,7 @@ hipError_t launch_macro(const LaunchCtx &L, const void *buf) {
     case 0: return collide_s<double>(L, (void *)buf, true);
     case 1: return collide_s<float>(L, (void *)buf, true);
     case 2: return collide_s<__half>(L, (void *)buf, true);
+    case 3: return collide_s<f16c_t>(L, (void *)buf, true);
   }
   return hipErrorInvalidValue;
 }
@@ -565,6 +568,9 @@ hipError_t launch_init(const LaunchCtx &L, void *A, void *B) {
       break;
     case 2:
       hipLaunchKernelGGL(k_init<__half>, grid, dim3(L.block), 0, L.stream, (__half *)A, (__half *)B, L.mask, make_args<__half>(L));
+      break;
+    case 3:
+      hipLaunchKernelGGL(k_init<f16c_t>, grid, dim3(L.block), 0, L.stream, (f16c_t *)A, (f16c_t *)B, L.mask, make_args<f16c_t>(L));
       break;
     default:
       return hipErrorInvalidValue;
@@ -603,6 +609,7 @@ hipError_t launch_backeq_solid(const LaunchCtx &L, void *A, void *B) {
     case 0: backeq_s<double>(L, A, B); break;
     case 1: backeq_s<float>(L, A, B); break;
     case 2: backeq_s<__half>(L, A, B); break;
+    case 3: backeq_s<f16c_t>(L, A, B); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -625,6 +632,9 @@ hipError_t launch_f_out(const LaunchCtx &L, const void *buf, double *out) {
     case 2:
       hipLaunchKernelGGL(k_f_out<__half>, grid, dim3(L.block), 0, L.stream, (const __half *)buf, L.mask, make_args<__half>(L), out);
       break;
+    case 3:
+      hipLaunchKernelGGL(k_f_out<f16c_t>, grid, dim3(L.block), 0, L.stream, (const f16c_t *)buf, L.mask, make_args<f16c_t>(L), out);
+      break;
     default:
       return hipErrorInvalidValue;
   }
@@ -635,7 +645,8 @@ size_t batch_args_size(int storage) {
   switch (storage) {
     case 0: return sizeof(LatArgs<double>);
     case 1: return sizeof(LatArgs<float>);
-    default: return sizeof(LatArgs<__half>);
+    case 2: return sizeof(LatArgs<__half>);
+    default: return sizeof(LatArgs<f16c_t>);
   }
 }
 
@@ -652,7 +663,8 @@ void batch_fill_args(const LaunchCtx &L, const void *in, void *out, void *dst) {
   switch (L.storage) {
     case 0: fill_args_s<double>(L, in, out, dst); break;
     case 1: fill_args_s<float>(L, in, out, dst); break;
-    default: fill_args_s<__half>(L, in, out, dst); break;
+    case 2: fill_args_s<__half>(L, in, out, dst); break;
+    default: fill_args_s<f16c_t>(L, in, out, dst); break;
   }
 }
 
@@ -689,7 +701,8 @@ hipError_t launch_stream_batch(const BatchLaunch &B) {
   switch (B.storage) {
     case 0: batch_s<double>(B); break;
     case 1: batch_s<float>(B); break;
-    default: batch_s<__half>(B); break;
+    case 2: batch_s<__half>(B); break;
+    default: batch_s<f16c_t>(B); break;
   }
   return hipGetLastError();
 }

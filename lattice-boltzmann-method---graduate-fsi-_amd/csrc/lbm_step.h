@@ -62,6 +62,8 @@ __device__ __forceinline__ void flag(int *status, unsigned long long *first_bad,
   atomicMin(first_bad, (unsigned long long)idx);
 }
 
+__device__ __forceinline__ float fma_c(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_c(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float vmin(float a, float b) { return __builtin_fminf(a, b); }
 __device__ __forceinline__ float vmax(float a, float b) { return __builtin_fmaxf(a, b); }
 __device__ __forceinline__ double vmin(double a, double b) { return __builtin_fmin(a, b); }
@@ -138,13 +140,22 @@ __device__ __forceinline__ bool relax(typename Store<S>::C (&f)[Q], typename Sto
   // raw equilibrium feq = Kw*(drho + rho*A) (shifted) or w*rho*(1 + A)
   // (unshifted) = ra*A + da per weight class; relaxation in the
   // non-equilibrium form f* = feq + (1 - omega)(f - feq)
+  //
+  // K w_q is carried as hi + lo (hi = K w_q rounded to C, lo = the rest), and
+  // rho K w_q = fma(hi, rho, lo rho) rounds once.  With hi alone every
+  // product would carry hi's own rounding, the same sign in every cell and
+  // step: the equilibrium's momentum would be (1 + 36 fl(1/36) - 1) rho u in
+  // FP32, a gain of omega * 7.5e-9 per step that accumulates (1e-4 of the
+  // velocity after 5000 steps of a Taylor-Green vortex).  In FP64 lo is 0.
   C ra[3], da[3];
   {
-    constexpr C wc[3] = {C(1.0 / 3), C(1.0 / 18), C(1.0 / 36)};
+    constexpr double wd[3] = {1.0 / 3, 1.0 / 18, 1.0 / 36};
 #pragma unroll
     for (int q = 0; q < 3; q++) {
-      ra[q] = (K * wc[q]) * rho;
-      da[q] = Store<S>::shifted ? (K * wc[q]) * drho : ra[q];
+      const double kw = Store<S>::K * wd[q];
+      const C hi = C(kw), lo = C(kw - double(hi));
+      ra[q] = fma_c(hi, rho, lo * rho);
+      da[q] = Store<S>::shifted ? fma_c(hi, drho, lo * drho) : ra[q];
     }
   }
   C fpre = C(0), Fu = C(0);
@@ -471,16 +482,22 @@ __device__ __forceinline__ void stream_cell(const Planes<S> &P, const uint8_t *_
     if (!st) return;
     for (int m = 0; m < Q; m++) f[m] = C(0);
   }
+  // encoded once, stored by one of two paths
+  S e[Q];
+  for_dirs([&](auto M) {
+    constexpr int m = decltype(M)::value;
+    e[m] = Store<S>::enc(f[m]);
+  });
   const uint32_t boff = idx * (uint32_t)sizeof(S);
   if (__builtin_amdgcn_ballot_w64(st != ALL) == 0) {
     for_dirs([&](auto M) {
       constexpr int m = decltype(M)::value;
-      Store<S>::st(at(P.out[m], boff), f[m]);
+      Store<S>::st_raw(at(P.out[m], boff), e[m]);
     });
   } else {
     for_dirs([&](auto M) {
       constexpr int m = decltype(M)::value;
-      if (st & (1u << m)) Store<S>::st(at(P.out[m], boff), f[m]);
+      if (st & (1u << m)) Store<S>::st_raw(at(P.out[m], boff), e[m]);
     });
   }
   if constexpr (COLLIDE) {

@@ -620,7 +620,7 @@ int lbm_upload_thermal(lbm_ctx *c, const lbm_thermal_desc *d, const double *T,
                        const int16_t *type_energy) {
   if (!c || !d) return LBM_ERR_ARG;
   if (!c->uploaded || c->state != ST_NONE) return LBM_ERR_STATE;
-  if (c->storage == LBM_STORE_F16S) return LBM_ERR_UNSUPPORTED;
+  if (c->storage == LBM_STORE_F16S || c->storage == LBM_STORE_F16C) return LBM_ERR_UNSUPPORTED;
   if (c->ibm) return LBM_ERR_UNSUPPORTED;
   if (!(d->tau_g > 0.5) || !(d->cp > 0) || !std::isfinite(d->nu)) return LBM_ERR_ARG;
   if ((uint64_t)c->g.ps * 38ull >= (1ull << 32)) return LBM_ERR_UNSUPPORTED;

@@ -30,8 +30,8 @@ LBM_ERR_UNSUPPORTED = -5
 
 TYPE_F, TYPE_S, TYPE_OUT, TYPE_IN, TYPE_P, TYPE_L, TYPE_T, TYPE_Q = range(8)
 
-STORE_F64, STORE_F32, STORE_F16S = 0, 1, 2
-STORAGE_NAMES = {"f64": STORE_F64, "f32": STORE_F32, "f16s": STORE_F16S}
+STORE_F64, STORE_F32, STORE_F16S, STORE_F16C = 0, 1, 2, 3
+STORAGE_NAMES = {"f64": STORE_F64, "f32": STORE_F32, "f16s": STORE_F16S, "f16c": STORE_F16C}
 
 REDUCE_MASS, REDUCE_MOMENTUM, REDUCE_KINETIC = 0, 1, 2
 
@@ -134,6 +134,7 @@ _SIGNATURES = {
     "lbm_get_launch": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "lbm_error_string": (C.c_char_p, [C.c_int]),
     "lbm_copy_probe": (C.c_int, [C.c_int32, C.c_int64, C.c_int32, _D]),
+    "lbm_storage_roundtrip": (C.c_int, [C.c_int32, C.c_int32, _D, _D, C.c_int64]),
     "lbm_ibm_setup": (C.c_int, [_P, C.POINTER(_IbmDesc), _D, _D, _D, _D, _D, _D, _D]),
     "lbm_ibm_apply": (C.c_int, [_P, _D, _D, _D]),
     "lbm_ibm_run": (C.c_int, [_P, C.c_int32, _D, _D]),
@@ -197,6 +198,17 @@ def copy_probe(nbytes: int = 1 << 30, reps: int = 20, device: int = 0) -> float:
     g = C.c_double(0.0)
     _check(load_library().lbm_copy_probe(device, int(nbytes), int(reps), C.byref(g)), "lbm_copy_probe")
     return g.value
+
+
+def storage_roundtrip(values, storage, device: int = 0) -> np.ndarray:
+    """What population storage `storage` keeps of each value: encoded on the
+    device and decoded back (values are f - w_i for the shifted formats)."""
+    x = np.ascontiguousarray(values, dtype=np.float64).ravel()
+    out = np.empty_like(x)
+    st = STORAGE_NAMES[storage] if isinstance(storage, str) else int(storage)
+    _check(load_library().lbm_storage_roundtrip(device, st, x.ctypes.data_as(_D), out.ctypes.data_as(_D),
+                                                x.size), "lbm_storage_roundtrip")
+    return out.reshape(np.shape(values))
 
 
 class Lattice:

@@ -24,7 +24,7 @@ def _lattices(storage):
     return out
 
 
-@pytest.mark.parametrize("storage", ["f64", "f32", "f16s"])
+@pytest.mark.parametrize("storage", ["f64", "f32", "f16s", "f16c"])
 @pytest.mark.parametrize("block,per_cu", [(256, 0), (64, 2), (128, 8)])
 def test_batch_matches_lattices_stepped_alone(storage, block, per_cu):
     alone = _lattices(storage)

@@ -216,7 +216,8 @@ def _apply_step(lat, v, vb):
     return out
 
 
-@pytest.mark.parametrize("storage,tol_u,tol_series", [("f32", 1e-5, 1e-4), ("f16s", 5e-3, 2e-2)])
+@pytest.mark.parametrize("storage,tol_u,tol_series", [("f32", 1e-5, 1e-4), ("f16s", 5e-3, 2e-2),
+                                                      ("f16c", 5e-3, 2e-2)])
 def test_ibm_reduced_storage(oracle_mod, storage, tol_u, tol_series):
     case, mk, box = IBM_CASES["slab"]()
     nsteps = 40

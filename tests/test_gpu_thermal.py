@@ -125,12 +125,13 @@ def test_thermal_temperature_range_error():
 
 def test_thermal_rejects_fp16_and_unset_walls():
     case, th = CASES["couette"]()
-    lat = lbm_amd.Lattice(case.Nx, case.Ny, case.Nz, case.tau, storage="f16s")
-    lat.upload(**case.fields())
-    with pytest.raises(lbm_amd.LBMError) as e:
-        lat.upload_thermal(th.T, th.type_energy, tau_g=th.tau_g, nu=th.nu, cp=th.cp)
-    assert e.value.code == lbm_amd.LBM_ERR_UNSUPPORTED
-    lat.close()
+    for st in ("f16s", "f16c"):
+        lat = lbm_amd.Lattice(case.Nx, case.Ny, case.Nz, case.tau, storage=st)
+        lat.upload(**case.fields())
+        with pytest.raises(lbm_amd.LBMError) as e:
+            lat.upload_thermal(th.T, th.type_energy, tau_g=th.tau_g, nu=th.nu, cp=th.cp)
+        assert e.value.code == lbm_amd.LBM_ERR_UNSUPPORTED
+        lat.close()
     lat = lbm_amd.Lattice(case.Nx, case.Ny, case.Nz, case.tau)
     lat.upload(**case.fields())
     with pytest.raises(lbm_amd.LBMError) as e:  # walls without type_Energy
