@@ -168,6 +168,10 @@ int lbm_create(const lbm_desc *desc, lbm_ctx **out) {
   c->g.plane = (uint32_t)(px * ny);
   c->g.ncell = (uint32_t)ncell;
   c->g.ps = (uint32_t)ps;
+  if (!set_divisors(c->g)) {
+    delete c;
+    return LBM_ERR_ARG;
+  }
   c->storage = desc->storage;
   c->esize = desc->storage == LBM_STORE_F64 ? 8 : (desc->storage == LBM_STORE_F32 ? 4 : 2);
   c->csize = desc->storage == LBM_STORE_F64 ? 8 : 4;

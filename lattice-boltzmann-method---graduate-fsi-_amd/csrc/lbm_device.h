@@ -74,10 +74,10 @@ __device__ __forceinline__ uint32_t wrap_dst(const Geo &g, uint32_t x, uint32_t 
 }
 __device__ __forceinline__ void coords(const Geo &g, uint32_t idx, uint32_t &x, uint32_t &y,
                                        uint32_t &z) {
-  x = idx % g.px;
-  uint32_t t = idx / g.px;
-  y = t % g.ny;
-  z = t / g.ny;
+  const uint32_t t = __umulhi(idx, g.px_m) >> g.px_s;  // idx / px
+  x = idx - t * g.px;
+  z = __umulhi(t, g.ny_m) >> g.ny_s;  // t / ny
+  y = t - z * g.ny;
 }
 // the pull offset of direction m for a cell off the shell: src = idx - off
 template <int m>
@@ -104,6 +104,9 @@ struct Store<double> {
   static __device__ __forceinline__ double ld(const double *p) { return *p; }
   static __device__ __forceinline__ void st(double *p, double r) { __builtin_nontemporal_store(r, p); }
   static __device__ __forceinline__ void st_raw(double *p, double v) { __builtin_nontemporal_store(v, p); }
+  using E = double;  // encoded value in a register (encw / st_e)
+  static __device__ __forceinline__ double encw(double r) { return r; }
+  static __device__ __forceinline__ void st_e(double *p, double v) { __builtin_nontemporal_store(v, p); }
   static __device__ __forceinline__ double enc(double r) { return r; }
 };
 
@@ -115,6 +118,9 @@ struct Store<float> {
   static __device__ __forceinline__ float ld(const float *p) { return *p; }
   static __device__ __forceinline__ void st(float *p, float r) { __builtin_nontemporal_store(r, p); }
   static __device__ __forceinline__ void st_raw(float *p, float v) { __builtin_nontemporal_store(v, p); }
+  using E = float;
+  static __device__ __forceinline__ float encw(float r) { return r; }
+  static __device__ __forceinline__ void st_e(float *p, float v) { __builtin_nontemporal_store(v, p); }
   static __device__ __forceinline__ float enc(float r) { return r; }
 };
 
@@ -134,6 +140,11 @@ struct Store<__half> {
   static __device__ __forceinline__ void st_raw(__half *p, __half v) {
     __builtin_nontemporal_store(*reinterpret_cast<const _Float16 *>(&v), reinterpret_cast<_Float16 *>(p));
   }
+  using E = _Float16;
+  static __device__ __forceinline__ _Float16 encw(float r) { return (_Float16)r; }
+  static __device__ __forceinline__ void st_e(__half *p, _Float16 v) {
+    __builtin_nontemporal_store(v, reinterpret_cast<_Float16 *>(p));
+  }
   static __device__ __forceinline__ __half enc(float r) {
     _Float16 h = (_Float16)r;
     return *reinterpret_cast<__half *>(&h);
@@ -146,28 +157,40 @@ struct Store<__half> {
 // populations occupy (the largest, f_0 - w_0 = w_0 (rho - 1 - 1.5 rho u^2),
 // stays below 0.5 for |rho - 1| < 1.5).  Values beyond it saturate.
 //
-// Decode: the 15 magnitude bits, shifted left by 12, are an FP32 bit pattern
-// whose exponent field is the format's own; multiplying by 2^110 restores
-// the bias (127 - 17), subnormals included since FP32 denormals are kept.
-// Encode is the inverse, rounded to nearest even on the 12 dropped bits.
+// The kernels work on these populations in units of 2^-110 (K): the FP32
+// value whose bit pattern is the 16-bit code shifted left by 12 (sign moved
+// to bit 31) is then the population itself, the format's exponent field
+// being the low 4 bits of FP32's, and its subnormals FP32 denormals (kept,
+// not flushed).  Decoding is a shift and a mask; encoding rounds the FP32
+// value to nearest even on the 12 low bits, after a clamp to the largest
+// code, and moves the sign back.
 struct f16c_t {
   uint16_t bits;
 };
 
+constexpr float F16C_MAX_RAW = 0x1.ffep-112f;  // code 0x7fff in units of 2^-110
+
 __device__ __forceinline__ float f16c_decode(int32_t h) {  // h sign-extended
-  return __uint_as_float(((uint32_t)h << 12) & 0x87ffffffu) * 0x1p110f;
+  return __uint_as_float(((uint32_t)h << 12) & 0x87fff000u);
 }
-__device__ __forceinline__ uint32_t f16c_encode(float r) {
-  const uint32_t a = __float_as_uint(__builtin_fabsf(r) * 0x1p-110f);
-  const uint32_t h = min((a + 0x7ffu + ((a >> 12) & 1u)) >> 12, 0x7fffu);
-  return h | ((__float_as_uint(r) >> 16) & 0x8000u);
+// the code in the high half of the result (the low half is not zero)
+__device__ __forceinline__ uint32_t f16c_encode_hi(float r) {
+  uint32_t b = __float_as_uint(__builtin_amdgcn_fmed3f(r, -F16C_MAX_RAW, F16C_MAX_RAW));
+  b += 0x7ffu + ((b >> 12) & 1u);  // magnitude bits only: no carry reaches the sign
+  // bits 30..16 <- magnitude bits 26..12, bit 31 the sign
+  // v_bfi_b32: (mask & b) | (~mask & (b << 4)), one instruction where the
+  // compiler's own lowering of the select takes three
+  uint32_t d;
+  asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(d) : "s"(0x80000000u), "v"(b), "v"(b << 4));
+  return d;
 }
+__device__ __forceinline__ uint32_t f16c_encode(float r) { return f16c_encode_hi(r) >> 16; }
 
 template <>
 struct Store<f16c_t> {
   using C = float;
   static constexpr bool shifted = true;
-  static constexpr float K = 1.0f;
+  static constexpr float K = 0x1p-110f;
   static __device__ __forceinline__ float ld(const f16c_t *p) {
     return f16c_decode(*reinterpret_cast<const int16_t *>(p));
   }
@@ -178,6 +201,11 @@ struct Store<f16c_t> {
     __builtin_nontemporal_store(v.bits, reinterpret_cast<uint16_t *>(p));
   }
   static __device__ __forceinline__ f16c_t enc(float r) { return f16c_t{(uint16_t)f16c_encode(r)}; }
+  using E = uint32_t;  // the code in the high half: stored by a high-half store
+  static __device__ __forceinline__ uint32_t encw(float r) { return f16c_encode_hi(r); }
+  static __device__ __forceinline__ void st_e(f16c_t *p, uint32_t v) {
+    __builtin_nontemporal_store((uint16_t)(v >> 16), reinterpret_cast<uint16_t *>(p));
+  }
 };
 
 // Cubic equilibrium of LBM::Calc_feq (lbm.cpp:628-633) with cs2 = 1/3:

@@ -17,7 +17,27 @@ struct Geo {
   uint32_t plane;       // px*ny
   uint32_t ncell;       // px*ny*nz (includes row padding)
   uint32_t ps;          // population plane stride (elements), >= ncell
+  // n / px and n / ny for n < 2^31 as (mulhi(n, m) >> s) (set_divisors)
+  uint32_t px_m, px_s, ny_m, ny_s;
 };
+
+// Division by an invariant d >= 2 for 0 <= n < 2^31 (Granlund and
+// Montgomery, "Division by invariant integers using multiplication", 1994):
+// with l = ceil(log2 d) and m = ceil(2^(31+l) / d) < 2^32,
+// floor(n / d) = floor(n m / 2^(31+l)), because n (m d - 2^(31+l)) <
+// 2^31 d.  Returns false when d is out of range.
+inline bool magic_divisor(uint32_t d, uint32_t &m, uint32_t &s) {
+  if (d < 2) return false;
+  uint32_t l = 0;
+  while ((1ull << l) < d) l++;
+  const unsigned __int128 num = (unsigned __int128)1 << (31 + l);
+  m = (uint32_t)((num + d - 1) / d);
+  s = l - 1;
+  return true;
+}
+inline bool set_divisors(Geo &g) {
+  return magic_divisor(g.px, g.px_m, g.px_s) && magic_divisor(g.ny, g.ny_m, g.ny_s);
+}
 
 // stream kernel variants
 enum StreamMode {

@@ -249,7 +249,8 @@ __device__ __forceinline__ void bulk_loads(const Planes<S> &P, uint32_t idx,
 template <class S>
 __device__ __forceinline__ void bulk_loads_wrapped(const Planes<S> &P, const Geo &g, uint32_t idx,
                                                    typename Store<S>::C (&f)[Q]) {
-  const uint32_t x = idx % g.px, t = idx / g.px, y = t % g.ny, z = t / g.ny;
+  uint32_t x, y, z;
+  coords(g, idx, x, y, z);
   const uint32_t e = (uint32_t)sizeof(S);
   const uint32_t ex = g.nx * e, ey = g.ny * g.px * e, ez = g.nz * g.plane * e;
   const uint32_t bias_elems = g.nx + g.ny * g.px + g.nz * g.plane;
@@ -483,21 +484,21 @@ __device__ __forceinline__ void stream_cell(const Planes<S> &P, const uint8_t *_
     for (int m = 0; m < Q; m++) f[m] = C(0);
   }
   // encoded once, stored by one of two paths
-  S e[Q];
+  typename Store<S>::E e[Q];
   for_dirs([&](auto M) {
     constexpr int m = decltype(M)::value;
-    e[m] = Store<S>::enc(f[m]);
+    e[m] = Store<S>::encw(f[m]);
   });
   const uint32_t boff = idx * (uint32_t)sizeof(S);
   if (__builtin_amdgcn_ballot_w64(st != ALL) == 0) {
     for_dirs([&](auto M) {
       constexpr int m = decltype(M)::value;
-      Store<S>::st_raw(at(P.out[m], boff), e[m]);
+      Store<S>::st_e(at(P.out[m], boff), e[m]);
     });
   } else {
     for_dirs([&](auto M) {
       constexpr int m = decltype(M)::value;
-      if (st & (1u << m)) Store<S>::st_raw(at(P.out[m], boff), e[m]);
+      if (st & (1u << m)) Store<S>::st_e(at(P.out[m], boff), e[m]);
     });
   }
   if constexpr (COLLIDE) {

@@ -5,12 +5,12 @@ F16C (sign, 4 exponent bits with bias 17, 11 fraction bits, subnormals) is
 checked against the format's definition: the value of every one of its
 32768 non-negative codes is written down from that definition, and each
 input must come back as the code value nearest to it, ties to the even
-code, saturating at the largest finite value, sign kept.  The input is
-first rounded to FP32 (the kernels encode from FP32), and the encoder rounds
-the FP32 product |x| 2^-110: below 2^-16 (the format's subnormals) that
-product is an FP32 denormal, already rounded to a multiple of 2^-149, so a
-value within 2^-39 of a tie there takes the tie's rounding (numpy's FP32
-product states the same first rounding here).  F16S is
+code, saturating at the largest finite value, sign kept.  Inputs are FP32
+values (the kernels encode FP32 results), which the kernels hold in units
+of 2^-110: below 2^-16 (the format's subnormals) such a value is an FP32
+denormal, already rounded to a multiple of 2^-149, so a value within 2^-39
+of a tie there takes the tie's rounding (numpy's FP32 product |x| 2^-110
+states the same first rounding here).  F16S is
 checked against numpy's IEEE half conversion of x * 2^12, F32 against
 numpy's float32.  All bars are exact equality.
 """
@@ -56,7 +56,7 @@ def _inputs(rng):
     big = np.array([0.4999, 0.5, 0.75, 1.0, 2.0, 1e3, 1e30, -0.5, -1e30])
     small = np.array([0.0, -0.0, 2.0 ** -30, 2.0 ** -28, 3 * 2.0 ** -29, 2.0 ** -27, 2.0 ** -16])
     x = np.concatenate([vals, -vals, mids, -mids, up, dn, rand, big, small])
-    return x
+    return x.astype(np.float32).astype(np.float64)
 
 
 def test_f16c_matches_format_definition():
