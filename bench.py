@@ -2,9 +2,10 @@
 
 Workload: periodic Taylor-Green vortex on 256^3 cells (LBM(254,254,254),
 every cell TYPE_F), u0 = 0.04, Re = 1600, D3Q19 BGK.  `value` is FP32-storage
-(f - w_i) MLUPS with the lattice resident in HBM; the FP16-shifted storage
-rate and both storages' relative L2 errors against an FP64-storage run of
-the same steps are reported beside it.
+(f - w_i) MLUPS with the lattice resident in HBM; the rates of the two
+16-bit shifted storages (F16S: IEEE half of (f - w_i) 2^12; F16C: the
+library's 1-4-11 float of f - w_i) and every storage's relative L2 error
+against an FP64-storage run of the same steps are reported beside it.
 
 One JSON line on stdout:
   value      MLUPS = fluid cells * K / (event-timed seconds of K fused launches)
@@ -67,7 +68,7 @@ def precision_study(case, nsteps):
     the CPU oracle to 1e-12 in the test suite)."""
     out = {}
     fields = {}
-    for st in ("f64", "f32", "f16s"):
+    for st in ("f64", "f32", "f16s", "f16c"):
         lat = lbm_amd.Lattice(case.Nx, case.Ny, case.Nz, case.tau, storage=st)
         lat.upload(**case.fields())
         lat.init()
@@ -82,12 +83,25 @@ def precision_study(case, nsteps):
     r = fields["f64"]
     num_u = lambda a: sum(float(((x - y) ** 2).sum()) for x, y in zip(a[1:], r[1:]))  # noqa: E731
     den_u = sum(float((y ** 2).sum()) for y in r[1:])
-    for st in ("f32", "f16s"):
+    for st in ("f32", "f16s", "f16c"):
         a = fields[st]
         out[st]["l2_u"] = float(np.sqrt(num_u(a) / den_u))
         out[st]["l2_rho"] = float(np.sqrt(((a[0] - r[0]) ** 2).sum() / (r[0] ** 2).sum()))
         out[st]["l2_rho_minus_1"] = float(
             np.sqrt(((a[0] - r[0]) ** 2).sum() / ((r[0] - 1.0) ** 2).sum()))
+    return out
+
+
+def cavity_leg(warmup, steps, size=256, storages=("f32", "f16s", "f16c")):
+    """Lid-driven cavity on size^3 cells (LBM(size-2)^3: five resting walls and
+    the moving lid, u_lid = 0.05, Re = 100), the fused step per storage:
+    MLUPS over TYPE_F cells and algorithmic GB/s, HIP-event timed."""
+    case = cases.cavity(size - 2, u_lid=0.05, Re=100.0)
+    out = {"cells": f"{size}^3", "fluid_cells": int((case.type == cases.TYPE_F).sum())}
+    for st in storages:
+        r = run_rate(case, st, warmup, steps)
+        out[st] = {"mlups": round(r["mlups"], 1), "achieved_gbs": round(r["gbs"], 1),
+                   "frac": round(r["gbs"] / HBM_PEAK_GBS, 4)}
     return out
 
 
@@ -205,9 +219,9 @@ def batch_leg(steps, storages=("f32", "f16s"), count=96, tune_steps=10):
         blk, per, _ = b.autotune(tune_steps)
         bat_ms = b.time_steps(steps)
         b.close()
+        codes_b = [lat.sync() for lat in lats]
         for lat in lats:
             lat.close()
-        codes_b = [lat.sync() for lat in lats]
         out[st] = {
             "sequential_mlups": round(nfl * steps / (seq_ms / 1e3) / 1e6, 1),
             "batched_mlups": round(nfl * steps / (bat_ms / 1e3) / 1e6, 1),
@@ -278,10 +292,11 @@ def main():
     ap.add_argument("--cpu-max-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
-    ap.add_argument("--storage", default="f32", choices=["f64", "f32", "f16s"])
+    ap.add_argument("--storage", default="f32", choices=["f64", "f32", "f16s", "f16c"])
     ap.add_argument("--ibm-steps", type=int, default=20000)
     ap.add_argument("--ibm-cmp-steps", type=int, default=2000)
     ap.add_argument("--no-ibm", action="store_true")
+    ap.add_argument("--no-cavity", action="store_true")
     ap.add_argument("--ibm-only", action="store_true", help="run only the IBM channel leg")
     ap.add_argument("--batch-steps", type=int, default=4000)
     ap.add_argument("--no-batch", action="store_true")
@@ -310,13 +325,23 @@ def main():
     log(f"bench: TGV {a.size}^3, tau={case.tau:.6f}, warmup={a.warmup}, steps={a.steps}")
     main_r = run_rate(case, a.storage, a.warmup, a.steps)
     log(f"bench: {a.storage} {main_r['mlups']:.0f} MLUPS, {main_r['gbs']:.0f} GB/s")
-    other = "f16s" if a.storage != "f16s" else "f32"
-    oth_r = run_rate(case, other, a.warmup, a.steps)
-    log(f"bench: {other} {oth_r['mlups']:.0f} MLUPS, {oth_r['gbs']:.0f} GB/s")
+    others = {}
+    for other in ("f32", "f16s", "f16c"):
+        if other == a.storage:
+            continue
+        oth_r = run_rate(case, other, a.warmup, a.steps)
+        log(f"bench: {other} {oth_r['mlups']:.0f} MLUPS, {oth_r['gbs']:.0f} GB/s")
+        others[other] = {
+            "mlups": round(oth_r["mlups"], 1),
+            "achieved_gbs": round(oth_r["gbs"], 1),
+            "frac": round(oth_r["gbs"] / HBM_PEAK_GBS, 4),
+            "bytes_per_cell": oth_r["bytes_per_cell"],
+            "speedup_vs_main": round(oth_r["mlups"] / main_r["mlups"], 3),
+        }
 
     prec = None
     if not a.no_precision:
-        log(f"bench: precision study, {a.precision_steps} steps x 3 storages")
+        log(f"bench: precision study, {a.precision_steps} steps x 4 storages")
         prec = precision_study(case, a.precision_steps)
         log(f"bench: precision {json.dumps(prec)}")
 
@@ -325,6 +350,12 @@ def main():
         log("bench: CPU baseline (oracle, OpenMP)")
         cpu = cpu_baseline(case, a.cpu_seconds, a.cpu_max_steps)
         log(f"bench: cpu {cpu['value']:.2f} MLUPS on {cpu['cores']} threads")
+
+    cav = None
+    if not a.no_cavity:
+        log("bench: lid-driven cavity 256^3 leg")
+        cav = cavity_leg(a.warmup, a.steps)
+        log(f"bench: cavity {json.dumps(cav)}")
 
     ibm = None
     if not a.no_ibm:
@@ -353,11 +384,12 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"f64": "f64", "f32": "f32", "f16s": "f32"}[a.storage],
+        "dtype": {"f64": "f64", "f32": "f32", "f16s": "f32", "f16c": "f32"}[a.storage],
         "data": "synthetic",
         "config": {
             "workload": f"periodic Taylor-Green vortex {a.size}^3 D3Q19 BGK, u0=0.04, Re=1600",
-            "storage": {"f64": "FP64", "f32": "FP32 (f - w_i)", "f16s": "FP16 (f - w_i)*2^12"}[a.storage],
+            "storage": {"f64": "FP64", "f32": "FP32 (f - w_i)", "f16s": "FP16 (f - w_i)*2^12",
+                        "f16c": "16-bit 1-4-11 float (f - w_i)"}[a.storage],
             "cells": main_r["nfluid"],
             "tau": case.tau,
         },
@@ -372,14 +404,9 @@ def main():
             "copy_ceiling_gbs": round(copy_gbs, 1),
             "frac_of_copy": round(achieved / copy_gbs, 4),
         },
-        other: {
-            "mlups": round(oth_r["mlups"], 1),
-            "achieved_gbs": round(oth_r["gbs"], 1),
-            "frac": round(oth_r["gbs"] / HBM_PEAK_GBS, 4),
-            "bytes_per_cell": oth_r["bytes_per_cell"],
-            "speedup_vs_main": round(oth_r["mlups"] / main_r["mlups"], 3),
-        },
+        **others,
         "precision": prec,
+        "cavity": cav,
         "ibm": ibm,
         "batch": batch,
         "cpu_baseline": cpu,

@@ -68,7 +68,7 @@ class LBM {
 
   LBM(int Nx, int Ny, int Nz, double tau);
   // storage: LBM_STORE_F64 (default, the reference's precision),
-  // LBM_STORE_F32 or LBM_STORE_F16S; device: HIP ordinal
+  // LBM_STORE_F32, LBM_STORE_F16S or LBM_STORE_F16C; device: HIP ordinal
   LBM(int Nx, int Ny, int Nz, double tau, int storage, int device = 0);
 
   void Init();       // lbm.cpp:31-57

@@ -2,7 +2,7 @@
 // 195-221) on the drop-in LBM class: a setup function writes lb.fluid, then
 // Init() and the Collision -> Streaming -> MacroProp loop.
 //
-// usage: lbm_example_cavity N steps out.bin [fused] [f64|f32|f16s]
+// usage: lbm_example_cavity N steps out.bin [fused] [f64|f32|f16s|f16c]
 // Writes density, ux, uy, uz (each (N+2)^3 doubles in fluid[i][j][k] order).
 #include <cstdio>
 #include <cstdlib>
@@ -35,7 +35,7 @@ static LBM main_setup_Cavity(int N, double u_lid, double Re, int storage) {
 
 int main(int argc, char **argv) {
   if (argc < 4) {
-    std::fprintf(stderr, "usage: %s N steps out.bin [fused] [f64|f32|f16s]\n", argv[0]);
+    std::fprintf(stderr, "usage: %s N steps out.bin [fused] [f64|f32|f16s|f16c]\n", argv[0]);
     return 2;
   }
   const int N = std::atoi(argv[1]);
@@ -44,6 +44,7 @@ int main(int argc, char **argv) {
   int storage = LBM_STORE_F64;
   if (argc > 5 && std::strcmp(argv[5], "f32") == 0) storage = LBM_STORE_F32;
   if (argc > 5 && std::strcmp(argv[5], "f16s") == 0) storage = LBM_STORE_F16S;
+  if (argc > 5 && std::strcmp(argv[5], "f16c") == 0) storage = LBM_STORE_F16C;
 
   LBM lb = main_setup_Cavity(N, 0.05, 100.0, storage);
   lb.Init();

@@ -4,7 +4,7 @@
 // from Generate_Cylinder_Marker, and each step is IBLBM, Collision,
 // Streaming.
 //
-// usage: lbm_example_cylinder_ibm Nx Ny D steps out.bin [f64|f32|f16s] [fixed|inline|transverse]
+// usage: lbm_example_cylinder_ibm Nx Ny D steps out.bin [f64|f32|f16s|f16c] [fixed|inline|transverse]
 // Writes Cd[steps], Cl[steps], then density, ux, uy, uz of the final lattice
 // ((Nx+2)(Ny+2)(Nz+2) doubles each, fluid[i][j][k] order), all doubles.
 #include <cmath>
@@ -38,7 +38,7 @@ static LBM main_setup_Cylinder_IBM(int Nx, int Ny, int Nz, double tau, double u_
 
 int main(int argc, char **argv) {
   if (argc < 6) {
-    std::fprintf(stderr, "usage: %s Nx Ny D steps out.bin [f64|f32|f16s] [fixed|inline|transverse]\n",
+    std::fprintf(stderr, "usage: %s Nx Ny D steps out.bin [f64|f32|f16s|f16c] [fixed|inline|transverse]\n",
                  argv[0]);
     return 2;
   }
@@ -48,6 +48,7 @@ int main(int argc, char **argv) {
   int storage = LBM_STORE_F64;
   if (argc > 6 && std::strcmp(argv[6], "f32") == 0) storage = LBM_STORE_F32;
   if (argc > 6 && std::strcmp(argv[6], "f16s") == 0) storage = LBM_STORE_F16S;
+  if (argc > 6 && std::strcmp(argv[6], "f16c") == 0) storage = LBM_STORE_F16C;
   const char *motion = argc > 7 ? argv[7] : "fixed";
 
   // case constants (the role of LBM.h:318-356)
