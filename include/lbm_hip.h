@@ -131,6 +131,18 @@ int lbm_export_links(lbm_ctx *ctx, int16_t *type, int32_t *src,
  * walls (lbm.cpp:276-288, 965-1028), T and q moments (lbm.cpp:777-798)
  * and the optional Boussinesq force of Rayleigh_Benard (lbm.cpp:803-806).
  * FP64 and FP32 storage. */
+/* lbm_thermal_desc.model */
+#define LBM_THERMAL_ENERGY 0 /* the reference's ENERGY branch: D3Q19 g on the *
+                              * same lattice, viscous heating                  */
+#define LBM_THERMAL_D3Q7 1   /* D3Q7 advection-diffusion of T (no reference   *
+                              * counterpart): w0 = 1/4, w = 1/8, cs2 = 1/4,    *
+                              * geq = w T (1 + c.u/cs2), alpha = (tau_g-1/2)/4; *
+                              * walls: type_Energy TYPE_T Dirichlet at the     *
+                              * solid's T (anti-bounce-back), otherwise         *
+                              * adiabatic (bounce-back); nu, cp, j_dot_n unused *
+                              * (j_dot_n must be 0); Boussinesq adds            *
+                              * g_beta (T - T_ref) to F_y at T's time level      */
+
 typedef struct lbm_thermal_desc {
   double tau_g;     /* g relaxation time, 0.5 + alpha/cs2 (LBM.h:155)       */
   double nu;        /* viscosity in the dissipation Phi = 2 nu rho S:S      *
@@ -142,7 +154,8 @@ typedef struct lbm_thermal_desc {
   double T0;        /* FP32 storage keeps g - w T0 (NaN: the mean set-up T   *
                      * of the TYPE_F cells)                                  */
   int32_t boussinesq;
-  int32_t reserved[3];
+  int32_t model;    /* LBM_THERMAL_ENERGY (0) or LBM_THERMAL_D3Q7            */
+  int32_t reserved[2];
 } lbm_thermal_desc;
 
 /* Writes of lb.fluid[i][j][k].T and .type_Energy (e.g. lbm.cpp:1551-1556)
@@ -153,7 +166,7 @@ int lbm_upload_thermal(lbm_ctx *ctx, const lbm_thermal_desc *desc, const double 
 /* Reads of lb.fluid[i][j][k].T and .q (as of the last MacroProp). */
 int lbm_download_thermal(lbm_ctx *ctx, double *T, double *qx, double *qy, double *qz);
 /* Reads of lb.fluid[i][j][k].g, ncells*19 (valid in the same states as
- * lbm_download_f). */
+ * lbm_download_f); ncells*7 for LBM_THERMAL_D3Q7. */
 int lbm_download_g(lbm_ctx *ctx, double *g);
 
 /* ---- implicit diffuse-interface IBM (IBLBM, lbm.cpp:1135-1417) ---------- */

@@ -42,6 +42,7 @@ SOURCES = [
     os.path.join(HERE, "csrc", "lbm_thermal.hip"),
     os.path.join(HERE, "csrc", "lbm_output.hip"),
     os.path.join(HERE, "csrc", "lbm_batch.hip"),
+    os.path.join(HERE, "csrc", "lbm_q7.hip"),
 ]
 HOST_SOURCES = [
     os.path.join(HERE, "host", "LBM.cpp"),

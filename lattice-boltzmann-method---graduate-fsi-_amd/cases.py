@@ -299,6 +299,101 @@ def random_thermal_case(Nx: int, Ny: int, Nz: int, seed: int = 5, tau: float = 0
     return case, Thermal(T, te, tau_g, nu, cp)
 
 
+# ---- D3Q7 temperature (LBM_THERMAL_D3Q7; no reference counterpart) -------
+
+def q7_tau(alpha: float) -> float:
+    """tau_g of D3Q7 for diffusivity alpha: alpha = cs2 (tau_g - 1/2), cs2 = 1/4."""
+    return 0.5 + alpha / 0.25
+
+
+def q7_thermal(T, type_energy, tau_g, boussinesq=False, g_beta=0.0, T_ref=0.0, **params) -> Thermal:
+    return Thermal(T, type_energy, tau_g, nu=0.0, cp=1.0, boussinesq=boussinesq, g_beta=g_beta,
+                   T_ref=T_ref, params=dict(model="d3q7", **params))
+
+
+def q7_conduction(Nx: int = 2, Ny: int = 8, Nz: int = 2, tau: float = 0.8, tau_g: float = 1.0,
+                  Tb: float = 1.0, Tt: float = 0.0, bottom: int = TYPE_T, top: int = TYPE_T,
+                  T_fluid=None, seed: int = 4):
+    """Fluid at rest between walls at j = 0 (T = Tb) and j = Ny+1 (T = Tt),
+    type_Energy `bottom` / `top` (TYPE_T Dirichlet, anything else adiabatic),
+    x and z periodic; the fluid starts at T_fluid (default: seeded values
+    between Tt and Tb).  With two Dirichlet walls the steady profile is
+    T_j = Tb + (Tt - Tb)(j - 1/2)/Ny (walls half-way between cells)."""
+    t, rho, ux, uy, uz = _blank(Nx, Ny, Nz)
+    t[:, 0, :] = TYPE_S
+    t[:, -1, :] = TYPE_S
+    if T_fluid is None:
+        xi = counter_uniform(seed, np.arange(t.size, dtype=np.uint64)).reshape(t.shape)
+        T = 0.5 * (Tb + Tt) + 0.5 * (Tb - Tt) * xi
+    else:
+        T = np.full(t.shape, float(T_fluid))
+    T[:, 0, :] = Tb
+    T[:, -1, :] = Tt
+    te = np.zeros(t.shape, np.int16)
+    te[:, 0, :] = bottom
+    te[:, -1, :] = top
+    case = Case("q7_conduction", Nx, Ny, Nz, tau, t, rho, ux, uy, uz)
+    return case, q7_thermal(T, te, tau_g, Tb=Tb, Tt=Tt)
+
+
+def q7_sine(Np: int = 16, U: float = 0.0, A: float = 0.1, T0: float = 0.5, tau_g: float = 1.0,
+            tau: float = 0.8, Nyz: int = 2):
+    """All-fluid periodic box of Np x (Nyz+2)^2 cells, uniform flow ux = U,
+    T = T0 + A sin(k x), k = 2 pi/Np.  At tau_g = 1 one step maps the mode
+    by exactly lambda = 3/4 + cos(k)/4 - i U sin(k): T_n = T0 + A Im(lambda^n
+    e^{ikx})."""
+    t, rho, ux, uy, uz = _blank(Np - 2, Nyz, Nyz)
+    ux[:] = U
+    k = 2.0 * math.pi / Np
+    x = np.arange(Np, dtype=np.float64)[:, None, None]
+    T = T0 + A * np.sin(k * x) * np.ones(t.shape)
+    case = Case("q7_sine", Np - 2, Nyz, Nyz, tau, t, rho, ux, uy, uz, params=dict(k=k, U=U, A=A, T0=T0))
+    return case, q7_thermal(T, np.zeros(t.shape, np.int16), tau_g)
+
+
+def q7_random_case(Nx: int, Ny: int, Nz: int, seed: int = 9, tau: float = 0.8, tau_g: float = 0.9,
+                   boussinesq: bool = True, g_beta: float = 2e-3):
+    """random_mask_case with temperatures in [0.2, 0.8): half of the solids
+    Dirichlet (TYPE_T), half adiabatic (TYPE_Q), inlet cells freezing links,
+    and the Boussinesq force about T_ref = 0.5."""
+    case = random_mask_case(Nx, Ny, Nz, seed=seed, tau=tau)
+    rng = np.random.default_rng(seed + 2000)
+    T = 0.2 + 0.6 * rng.random(case.shape)
+    te = np.zeros(case.shape, np.int16)
+    solid = case.type == TYPE_S
+    te[solid] = np.where(rng.random(case.shape) < 0.5, TYPE_T, TYPE_Q)[solid]
+    return case, q7_thermal(T, te, tau_g, boussinesq=boussinesq, g_beta=g_beta, T_ref=0.5)
+
+
+def q7_rayleigh_benard(Nx: int = 48, Ny: int = 24, Nz: int = 1, Ra: float = 1e4, Pr: float = 0.71,
+                       tau: float = 0.6, Tb: float = 1.0, Tt: float = 0.0, seed: int = 6,
+                       perturb: float = 1e-3):
+    """Rayleigh-Benard in a slab heated from below: Dirichlet walls at j = 0
+    (Tb) and j = Ny+1 (Tt), x and z periodic.  H = Ny, nu = cs2 (tau - 1/2),
+    alpha = nu/Pr, g_beta = Ra nu alpha / ((Tb - Tt) H^3), F_y = g_beta (T -
+    T_ref), T_ref = (Tb + Tt)/2; the fluid starts on the conduction profile
+    with a seeded perturbation."""
+    nu = (tau - 0.5) / 3.0
+    alpha = nu / Pr
+    H = float(Ny)
+    g_beta = Ra * nu * alpha / ((Tb - Tt) * H ** 3)
+    t, rho, ux, uy, uz = _blank(Nx, Ny, Nz)
+    t[:, 0, :] = TYPE_S
+    t[:, -1, :] = TYPE_S
+    j = np.arange(Ny + 2, dtype=np.float64)[None, :, None]
+    T = (Tb + (Tt - Tb) * (j - 0.5) / Ny) * np.ones(t.shape)
+    xi = counter_uniform(seed, np.arange(t.size, dtype=np.uint64)).reshape(t.shape)
+    T = T + perturb * xi
+    T[:, 0, :] = Tb
+    T[:, -1, :] = Tt
+    te = np.zeros(t.shape, np.int16)
+    te[:, 0, :] = TYPE_T
+    te[:, -1, :] = TYPE_T
+    case = Case("q7_rayleigh_benard", Nx, Ny, Nz, tau, t, rho, ux, uy, uz)
+    return case, q7_thermal(T, te, q7_tau(alpha), boussinesq=True, g_beta=g_beta,
+                            T_ref=0.5 * (Tb + Tt), Ra=Ra, Pr=Pr, nu=nu, alpha=alpha, H=H)
+
+
 def poiseuille(Nx: int = 2, Ny: int = 30, Nz: int = 2, tau: float = 0.9, g: float = 1e-6) -> Case:
     """Channel between TYPE_S walls at j = 0, Ny+1 (main_setup_Poiseuille,
     lbm.cpp:1495-1530) driven by a uniform body force Fx = g in every cell."""

@@ -53,9 +53,9 @@ LaunchCtx launch_ctx(const lbm_ctx *c) {
   L.first_bad = c->first_bad;
   L.stream = c->stream;
   L.box = c->box;
-  // bounce-back through the solids' slots (the thermal kernels patch their
+  // bounce-back through the solids' slots (the ENERGY kernels patch their
   // wall links themselves)
-  L.wall_slots = c->thermal ? 0 : 1;
+  L.wall_slots = c->thermal == 1 ? 0 : 1;
   return L;
 }
 
@@ -300,10 +300,13 @@ int lbm_init(lbm_ctx *c) {
   HIPCHK(launch_tables(L, c->mask, c->links, c->uw[0], c->uw[1], c->uw[2], c->type));
   // 2. f = f* = feq (LBM::Init); with ENERGY also g = g* = geq and the
   //    thermal wall tables
-  if (c->thermal) {
+  if (c->thermal == 1) {
     if (int e = thermal_init(c)) return e;
   } else {
     HIPCHK(launch_init(L, c->cur, c->oth));
+    if (c->thermal == 2) {
+      if (int e = q7_init(c)) return e;
+    }
     if (any_solid && L.wall_slots) {
       // the solids' slots will carry bounce-back values: their MacroProp
       // values (moments of their Init populations, which never change) are
@@ -318,7 +321,7 @@ int lbm_init(lbm_ctx *c) {
   // MacroProps after which the wall tables change: the first (solids' rho,
   // u, T become moments of their populations); with the Boussinesq force
   // also the second (the solids' F_y set by the first enters their u)
-  c->walls_from_setup = any_solid ? (c->thermal && c->td.boussinesq ? 2 : 1) : 0;
+  c->walls_from_setup = any_solid ? (c->thermal == 1 && c->td.boussinesq ? 2 : 1) : 0;
   c->backeq_solid_pending = 0;
   {
     const uint32_t plane = c->g.plane, top = (c->g.nz - 1) * plane;
@@ -341,7 +344,7 @@ static int walls_from_moments(lbm_ctx *c, bool solids_done = false) {
   LaunchCtx L = launch_ctx(c);
   L.macro_sel = 16;  // TYPE_S only
   if (!solids_done) {
-    if (c->thermal) {
+    if (c->thermal == 1) {
       if (int e = thermal_launch(c, 4, 16)) return e;
     } else if (!L.wall_slots) {
       HIPCHK(launch_macro(L, c->cur));
@@ -350,8 +353,10 @@ static int walls_from_moments(lbm_ctx *c, bool solids_done = false) {
     // back-face write, after_stream)
   }
   HIPCHK(launch_tables(L, c->mask, c->links, c->uw[0], c->uw[1], c->uw[2], c->type));
-  if (c->thermal) {
+  if (c->thermal == 1) {
     if (int e = thermal_tables(c)) return e;
+  } else if (c->thermal == 2) {
+    if (int e = q7_tables(c)) return e;
   }
   c->walls_from_setup--;
   return LBM_OK;
@@ -369,8 +374,10 @@ int lbm_collide(lbm_ctx *c) {
     L.force_mode = 3;
     ibm_consume(c);
   }
-  if (c->thermal) {
+  if (c->thermal == 1) {
     if (int e = thermal_launch(c, 3)) return e;
+  } else if (c->thermal == 2) {
+    if (int e = q7_launch(c, Q7_COLLIDE)) return e;
   } else {
     HIPCHK(launch_collide(L, c->cur));
   }
@@ -382,8 +389,10 @@ int lbm_stream(lbm_ctx *c) {
   if (!c) return LBM_ERR_ARG;
   if (c->state != ST_FSTAR) return LBM_ERR_STATE;
   HIPCHK(hipSetDevice(c->dev));
-  if (c->thermal) {
+  if (c->thermal == 1) {
     if (int e = thermal_launch(c, 2)) return e;
+  } else if (c->thermal == 2) {
+    if (int e = q7_launch(c, Q7_STREAM)) return e;
   } else {
     HIPCHK(launch_stream(launch_ctx(c), c->cur, c->oth, STREAM_ONLY));
   }
@@ -396,8 +405,10 @@ int lbm_macro(lbm_ctx *c) {
   if (!c) return LBM_ERR_ARG;
   if (c->state != ST_F || ibm_active(c)) return LBM_ERR_STATE;
   HIPCHK(hipSetDevice(c->dev));
-  if (c->thermal) {
+  if (c->thermal == 1) {
     if (int e = thermal_launch(c, 4, 1 | 16)) return e;
+  } else if (c->thermal == 2) {
+    if (int e = q7_launch(c, Q7_MACRO)) return e;
   } else {
     LaunchCtx L = launch_ctx(c);
     // with wall slots the solids' populations are no longer in the buffer;
@@ -410,9 +421,15 @@ int lbm_macro(lbm_ctx *c) {
 
 // one launch of the step sequence, plain or thermal
 static int step_launch(lbm_ctx *c, const LaunchCtx &L, int mode) {
-  if (c->thermal) {
+  if (c->thermal == 1) {
     const int tm = mode == STREAM_COLLIDE ? 0 : mode == STREAM_COLLIDE_MACRO ? 1 : mode == STREAM_ONLY ? 2 : 5;
     if (int e = thermal_launch(c, tm)) return e;
+  } else if (c->thermal == 2) {
+    const int tm = mode == STREAM_COLLIDE ? Q7_FUSED
+                   : mode == STREAM_COLLIDE_MACRO ? Q7_FUSED_EMIT
+                   : mode == STREAM_ONLY ? Q7_STREAM
+                                         : Q7_STREAM_EMIT;
+    if (int e = q7_launch(c, tm)) return e;
   } else {
     HIPCHK(launch_stream(L, c->cur, c->oth, mode));
   }
@@ -421,7 +438,8 @@ static int step_launch(lbm_ctx *c, const LaunchCtx &L, int mode) {
 }
 
 static int collide_launch(lbm_ctx *c, const LaunchCtx &L) {
-  if (c->thermal) return thermal_launch(c, 3);
+  if (c->thermal == 1) return thermal_launch(c, 3);
+  if (c->thermal == 2) return q7_launch(c, Q7_COLLIDE);
   HIPCHK(launch_collide(L, c->cur));
   return LBM_OK;
 }
@@ -630,7 +648,8 @@ double lbm_bytes_per_cell(const lbm_ctx *c) {
   if (!c) return 0.0;
   // A/B pull: 19 populations read + 19 written + the 1-byte mask; with
   // ENERGY also 19 g read + written and the previous Phi, u read + written
-  if (c->thermal) return 4.0 * 19.0 * (double)c->esize + 8.0 * (double)c->csize + 1.0;
+  if (c->thermal == 1) return 4.0 * 19.0 * (double)c->esize + 8.0 * (double)c->csize + 1.0;
+  if (c->thermal == 2) return 2.0 * 26.0 * (double)c->esize + 1.0;
   return 2.0 * 19.0 * (double)c->esize + 1.0;
 }
 

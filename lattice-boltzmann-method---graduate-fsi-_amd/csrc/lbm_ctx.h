@@ -50,7 +50,7 @@ struct lbm_ctx {
   lbm::BoxArgs box{};           // the IBM box as force mode 3 reads it
   // ---- thermal branch (lbm_thermal.hip); the population buffers then hold
   // 38 planes: f in 0..18, g in 19..37
-  int thermal = 0;
+  int thermal = 0;  // 1: the reference's ENERGY branch, 2: D3Q7 (26 planes)
   lbm_thermal_desc td{};
   double T0 = 0;                     // storage shift of g (FP32)
   void *T = nullptr, *qv[3] = {nullptr, nullptr, nullptr};  // macro T, q
@@ -61,7 +61,22 @@ struct lbm_ctx {
   int walls_refresh = 0;             // MacroProps left that change the wall tables
   std::vector<int16_t> h_typeE;      // device order
   int16_t *typeE = nullptr;          // device copy of type_Energy
-  struct lbm_output *out = nullptr;  // VTK output staging (lbm_output.hip)
+  struct lbm_output *out = nullptr;  // D3Q7 temperature (lbm_q7.hip), thermal == 2
+enum Q7Mode {
+  Q7_FUSED = 0,       // Streaming + MacroProp + next Collision
+  Q7_FUSED_EMIT = 1,  // as Q7_FUSED, and write rho, u, T, q
+  Q7_STREAM = 2,      // Streaming alone
+  Q7_STREAM_EMIT = 3, // Streaming + MacroProp
+  Q7_COLLIDE = 4,     // Collision in place (rho, u, T as stored)
+  Q7_MACRO = 5        // MacroProp of the fluid cells
+};
+int q7_upload(lbm_ctx *c, const lbm_thermal_desc *d, const double *T, const int16_t *type_energy);
+int q7_init(lbm_ctx *c);    // Dirichlet link bits, g = geq
+int q7_tables(lbm_ctx *c);  // Dirichlet link bits (after every k_tables)
+int q7_launch(lbm_ctx *c, int mode);
+int q7_download_g(lbm_ctx *c, double *g_dev);
+
+// VTK output staging (lbm_output.hip)
 };
 
 #define HIPCHK(x)                              \
@@ -78,6 +93,21 @@ int thermal_tables(lbm_ctx *c);        // Dirichlet/Neumann links, wall temperat
 // 3 collide in place, 4 MacroProp of the cells with mask bits `sel`
 int thermal_launch(lbm_ctx *c, int mode, uint8_t sel = 1);
 int thermal_download_g(lbm_ctx *c, double *g_dev);
+
+// D3Q7 temperature (lbm_q7.hip), thermal == 2
+enum Q7Mode {
+  Q7_FUSED = 0,       // Streaming + MacroProp + next Collision
+  Q7_FUSED_EMIT = 1,  // as Q7_FUSED, and write rho, u, T, q
+  Q7_STREAM = 2,      // Streaming alone
+  Q7_STREAM_EMIT = 3, // Streaming + MacroProp
+  Q7_COLLIDE = 4,     // Collision in place (rho, u, T as stored)
+  Q7_MACRO = 5        // MacroProp of the fluid cells
+};
+int q7_upload(lbm_ctx *c, const lbm_thermal_desc *d, const double *T, const int16_t *type_energy);
+int q7_init(lbm_ctx *c);    // Dirichlet link bits, g = geq
+int q7_tables(lbm_ctx *c);  // Dirichlet link bits (after every k_tables)
+int q7_launch(lbm_ctx *c, int mode);
+int q7_download_g(lbm_ctx *c, double *g_dev);
 
 // VTK output staging (lbm_output.hip)
 void output_free(lbm_ctx *c);

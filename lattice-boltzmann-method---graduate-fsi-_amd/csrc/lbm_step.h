@@ -246,9 +246,9 @@ __device__ __forceinline__ void bulk_loads(const Planes<S> &P, uint32_t idx,
 // The lane's 32-bit byte offset must not go negative: it is taken from a
 // base lowered by the largest negative correction (bias), so that
 // offset = idx*size + bias + correction >= 0 for every lane.
-template <class S>
-__device__ __forceinline__ void bulk_loads_wrapped(const Planes<S> &P, const Geo &g, uint32_t idx,
-                                                   typename Store<S>::C (&f)[Q]) {
+template <class S, int N>
+__device__ __forceinline__ void wrapped_loads(const S *const (&src)[N], const Geo &g, uint32_t idx,
+                                              typename Store<S>::C (&f)[N]) {
   uint32_t x, y, z;
   coords(g, idx, x, y, z);
   const uint32_t e = (uint32_t)sizeof(S);
@@ -261,7 +261,7 @@ __device__ __forceinline__ void bulk_loads_wrapped(const Planes<S> &P, const Geo
   const uint32_t yp = y == 0 ? ey : 0u, ym = y == g.ny - 1 ? ey : 0u;
   const uint32_t zp = z == 0 ? ez : 0u, zm = z == g.nz - 1 ? ez : 0u;
   const uint32_t boff = idx * e + bias;
-  for_dirs([&](auto M) {
+  for_dirs_impl(std::make_integer_sequence<int, N>{}, [&](auto M) {
     constexpr int m = decltype(M)::value;
     uint32_t o = boff;
     if constexpr (CX[m] > 0) o += xp;
@@ -270,8 +270,15 @@ __device__ __forceinline__ void bulk_loads_wrapped(const Planes<S> &P, const Geo
     if constexpr (CY[m] < 0) o -= ym;
     if constexpr (CZ[m] > 0) o += zp;
     if constexpr (CZ[m] < 0) o -= zm;
-    f[m] = Store<S>::ld(at(P.src[m] - bias_elems, o));
+    f[m] = Store<S>::ld(at(src[m] - bias_elems, o));
   });
+}
+// the 19 population planes (N < 19: the first N directions, e.g. the seven
+// of D3Q7, which are D3Q19's rest and axis directions)
+template <class S>
+__device__ __forceinline__ void bulk_loads_wrapped(const Planes<S> &P, const Geo &g, uint32_t idx,
+                                                   typename Store<S>::C (&f)[Q]) {
+  wrapped_loads<S, Q>(P.src, g, idx, f);
 }
 
 // An empty asm statement that takes the bulk-loaded values as inputs and

@@ -622,6 +622,12 @@ int lbm_upload_thermal(lbm_ctx *c, const lbm_thermal_desc *d, const double *T,
   if (!c->uploaded || c->state != ST_NONE) return LBM_ERR_STATE;
   if (c->storage == LBM_STORE_F16S || c->storage == LBM_STORE_F16C) return LBM_ERR_UNSUPPORTED;
   if (c->ibm) return LBM_ERR_UNSUPPORTED;
+  if (d->model != LBM_THERMAL_ENERGY && d->model != LBM_THERMAL_D3Q7) return LBM_ERR_ARG;
+  if (d->model == LBM_THERMAL_D3Q7) {
+    HIPCHK(hipSetDevice(c->dev));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return q7_upload(c, d, T, type_energy);
+  }
   if (!(d->tau_g > 0.5) || !(d->cp > 0) || !std::isfinite(d->nu)) return LBM_ERR_ARG;
   if ((uint64_t)c->g.ps * 38ull >= (1ull << 32)) return LBM_ERR_UNSUPPORTED;
   HIPCHK(hipSetDevice(c->dev));
@@ -733,13 +739,14 @@ int lbm_download_g(lbm_ctx *c, double *g) {
   if (!c->thermal || c->state == ST_NONE) return LBM_ERR_STATE;
   HIPCHK(hipSetDevice(c->dev));
   const size_t n = c->g.ncell;
+  const int ng = c->thermal == 2 ? 7 : 19;
   double *d = nullptr;
-  HIPCHK(hipMalloc((void **)&d, n * 19 * sizeof(double)));
-  int e = thermal_download_g(c, d);
+  HIPCHK(hipMalloc((void **)&d, n * ng * sizeof(double)));
+  int e = c->thermal == 2 ? q7_download_g(c, d) : thermal_download_g(c, d);
   std::vector<double> h;
   if (e == LBM_OK) {
-    h.resize(n * 19);
-    if (hipMemcpyAsync(h.data(), d, n * 19 * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+    h.resize(n * ng);
+    if (hipMemcpyAsync(h.data(), d, n * ng * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess)
       e = LBM_ERR_HIP;
   }
@@ -749,8 +756,8 @@ int lbm_download_g(lbm_ctx *c, double *g) {
   for (uint32_t i = 0; i < nx; i++)
     for (uint32_t j = 0; j < ny; j++)
       for (uint32_t k = 0; k < nz; k++)
-        memcpy(g + (((size_t)i * ny + j) * nz + k) * 19, h.data() + (((size_t)k * ny + j) * px + i) * 19,
-               19 * sizeof(double));
+        memcpy(g + (((size_t)i * ny + j) * nz + k) * ng, h.data() + (((size_t)k * ny + j) * px + i) * ng,
+               ng * sizeof(double));
   return LBM_OK;
 }
 

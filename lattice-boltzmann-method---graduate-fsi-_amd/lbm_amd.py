@@ -78,8 +78,12 @@ class _ThermalDesc(C.Structure):
         ("T_ref", C.c_double),
         ("T0", C.c_double),
         ("boussinesq", C.c_int32),
-        ("reserved", C.c_int32 * 3),
+        ("model", C.c_int32),
+        ("reserved", C.c_int32 * 2),
     ]
+
+
+THERMAL_ENERGY, THERMAL_D3Q7 = 0, 1
 
 
 class _VtkDesc(C.Structure):
@@ -329,12 +333,16 @@ class Lattice:
         return out[:3] if what == REDUCE_MOMENTUM else float(out[0])
 
     # -- thermal branch (ENERGY, lbm.cpp:121-178, 263-289, 636-653, 777-806) ----
-    def upload_thermal(self, T=None, type_energy=None, *, tau_g, nu, cp, j_dot_n=0.0,
-                       boussinesq=False, g_beta=0.0, T_ref=0.0, T0=float("nan")):
+    def upload_thermal(self, T=None, type_energy=None, *, tau_g, nu=0.0, cp=1.0, j_dot_n=0.0,
+                       boussinesq=False, g_beta=0.0, T_ref=0.0, T0=float("nan"), model="energy"):
         """After upload(), before init(): lb.fluid[i][j][k].T and .type_Energy
-        and the case's thermal constants."""
+        and the case's thermal constants.  model "energy": the reference's
+        same-lattice D3Q19 g; "d3q7": D3Q7 advection-diffusion (nu, cp
+        unused)."""
+        m = {"energy": THERMAL_ENERGY, "d3q7": THERMAL_D3Q7}[model] if isinstance(model, str) else int(model)
+        self._ng = 7 if m == THERMAL_D3Q7 else 19
         d = _ThermalDesc(float(tau_g), float(nu), float(cp), float(j_dot_n), float(g_beta),
-                         float(T_ref), float(T0), int(bool(boussinesq)), (C.c_int32 * 3)())
+                         float(T_ref), float(T0), int(bool(boussinesq)), m, (C.c_int32 * 2)())
         Ta = None if T is None else self._f64(T)
         te = None
         if type_energy is not None:
@@ -350,7 +358,7 @@ class Lattice:
         return tuple(out)
 
     def g_populations(self):
-        a = np.empty(self.shape + (19,))
+        a = np.empty(self.shape + (getattr(self, "_ng", 19),))
         _check(self._lib.lbm_download_g(self._h, _dp(a)), "lbm_download_g")
         return a
 

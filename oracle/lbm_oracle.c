@@ -58,7 +58,24 @@ struct orc_lbm {
   orc_cell ***fluid; /* [Nx+2][Ny+2][Nz+2], lbm.cpp:17-28 */
   double *Fm;        /* implicit-IBM Fm[3] per cell (LBM.h:571-573) */
   struct orc_thermal *th; /* ENERGY fields (LBM.h:590-596), NULL when off */
+  struct orc_q7 *q7;      /* D3Q7 temperature (no reference counterpart) */
 };
+
+/* D3Q7 temperature (declared in lbm_oracle.h with its specification) */
+typedef struct orc_q7 {
+  double tau_g, g_beta, T_ref;
+  int boussinesq;
+  double *g, *g_star; /* 7 per cell */
+  double *T, *q;      /* 1 and 3 per cell */
+  double *F0y;        /* set-up F_y */
+  int16_t *typeE;
+} orc_q7;
+
+static const int q7_cx[7] = {0, 1, -1, 0, 0, 0, 0};
+static const int q7_cy[7] = {0, 0, 0, 1, -1, 0, 0};
+static const int q7_cz[7] = {0, 0, 0, 0, 0, 1, -1};
+static const int q7_opp[7] = {0, 2, 1, 4, 3, 6, 5};
+static const double q7_w[7] = {0.25, 0.125, 0.125, 0.125, 0.125, 0.125, 0.125};
 
 /* The ENERGY members of LATTICE (LBM.h:590-596) as arrays over LIN(i,j,k),
  * and the case constants the thermal branch reads (LBM.h:150-161,
@@ -148,6 +165,11 @@ void orc_destroy(orc_lbm *lb) {
     free(lb->th->g); free(lb->th->g_star); free(lb->th->F_vis);
     free(lb->th->T); free(lb->th->q); free(lb->th->typeE);
     free(lb->th);
+  }
+  if (lb->q7) {
+    free(lb->q7->g); free(lb->q7->g_star); free(lb->q7->T); free(lb->q7->q);
+    free(lb->q7->F0y); free(lb->q7->typeE);
+    free(lb->q7);
   }
   free(lb);
 }
@@ -309,6 +331,57 @@ void orc_set_g(orc_lbm *lb, const double *g) {
   memcpy(lb->th->g, g, sizeof(double) * (size_t)orc_ncells(lb) * lb->q);
 }
 
+/* ---- D3Q7 temperature (specification in lbm_oracle.h) ---- */
+int orc_q7_enable(orc_lbm *lb, double tau_g, int boussinesq, double g_beta, double T_ref) {
+  if (lb->q7 || lb->th || lb->q != 19) return 0;
+  const size_t n = (size_t)orc_ncells(lb);
+  orc_q7 *t = (orc_q7 *)calloc(1, sizeof(orc_q7));
+  t->tau_g = tau_g;
+  t->boussinesq = boussinesq;
+  t->g_beta = g_beta;
+  t->T_ref = T_ref;
+  t->g = (double *)calloc(n * 7, sizeof(double));
+  t->g_star = (double *)calloc(n * 7, sizeof(double));
+  t->T = (double *)calloc(n, sizeof(double));
+  t->q = (double *)calloc(n * 3, sizeof(double));
+  t->F0y = (double *)calloc(n, sizeof(double));
+  t->typeE = (int16_t *)calloc(n, sizeof(int16_t));
+  lb->q7 = t;
+  return 1;
+}
+
+void orc_q7_set(orc_lbm *lb, const double *T, const int16_t *typeE) {
+  if (!lb->q7) return;
+  for (int i = 0; i <= lb->Nx + 1; i++)
+    for (int j = 0; j <= lb->Ny + 1; j++)
+      for (int k = 0; k <= lb->Nz + 1; k++) {
+        const int64_t c = LIN(lb, i, j, k);
+        if (T) lb->q7->T[c] = T[c];
+        if (typeE) lb->q7->typeE[c] = typeE[c];
+        lb->q7->F0y[c] = lb->fluid[i][j][k].F[1];
+      }
+}
+
+void orc_q7_get(const orc_lbm *lb, double *T, double *qx, double *qy, double *qz) {
+  if (!lb->q7) return;
+  for (int64_t c = 0; c < orc_ncells(lb); c++) {
+    if (T) T[c] = lb->q7->T[c];
+    if (qx) qx[c] = lb->q7->q[3 * c + 0];
+    if (qy) qy[c] = lb->q7->q[3 * c + 1];
+    if (qz) qz[c] = lb->q7->q[3 * c + 2];
+  }
+}
+
+void orc_q7_get_g(const orc_lbm *lb, double *g) {
+  if (!lb->q7) return;
+  memcpy(g, lb->q7->g, sizeof(double) * (size_t)orc_ncells(lb) * 7);
+}
+
+static inline double q7_geq(int i, double T, double ux, double uy, double uz) {
+  double c_dot_u = q7_cx[i] * ux + q7_cy[i] * uy + q7_cz[i] * uz;
+  return q7_w[i] * T * (1 + c_dot_u / 0.25);
+}
+
 /* Determine_Wall_Temperature, lbm.cpp:965-981: the mean T of the cell's
  * TYPE_S neighbours (l = 0 included), NaN when there is none. */
 double orc_wall_temperature(const orc_lbm *lb, int i, int j, int k) {
@@ -462,6 +535,15 @@ void orc_init(orc_lbm *lb) {
           double f_eq = calc_feq(lb, &lb->fluid[i][j][k], l);
           lb->fluid[i][j][k].f[l] = f_eq;
           lb->fluid[i][j][k].f_star[l] = f_eq;
+          if (lb->q7 && l < 7) {
+            const int64_t n = LIN(lb, i, j, k);
+            orc_cell *c = &lb->fluid[i][j][k];
+            lb->q7->g[n * 7 + l] = lb->q7->g_star[n * 7 + l] = q7_geq(l, lb->q7->T[n], c->ux, c->uy, c->uz);
+            /* the first collision already carries the buoyancy of the
+             * set-up T, as every later one carries that of MacroProp's */
+            if (l == 0 && lb->q7->boussinesq && c->type == ORC_TYPE_F)
+              c->F[1] = lb->q7->F0y[n] + lb->q7->g_beta * (lb->q7->T[n] - lb->q7->T_ref);
+          }
           if (lb->th) {
             /* lbm.cpp:45-50: g = g_star = Calc_geq with the cell's own type
              * and a zero wall velocity */
@@ -507,6 +589,14 @@ int orc_collision(orc_lbm *lb) {
         if (lb->fluid[i][j][k].type == ORC_TYPE_F) {
           bad |= conv_lbm(lb, &lb->fluid[i][j][k]);
           if (lb->th) conv_energy(lb, i, j, k);
+          if (lb->q7) {
+            orc_q7 *t = lb->q7;
+            const orc_cell *c = &lb->fluid[i][j][k];
+            const int64_t n = LIN(lb, i, j, k);
+            const double om = ORC_DT / t->tau_g;
+            for (int l = 0; l < 7; l++)
+              t->g_star[n * 7 + l] = t->g[n * 7 + l] * (1 - om) + om * q7_geq(l, t->T[n], c->ux, c->uy, c->uz);
+          }
         }
   return bad ? ORC_ERR_FEQ_NEGATIVE : ORC_OK;
 }
@@ -576,6 +666,19 @@ void orc_streaming(orc_lbm *lb) {
           int j_nb = WRAP(j + lb->cy[l], Ny + 2);
           int k_nb = WRAP(k + lb->cz[l], Nz + 2);
           orc_cell *nb = &fl[i_nb][j_nb][k_nb];
+          if (lb->q7 && l < 7) {
+            /* D3Q7 (lbm_oracle.h); its directions are D3Q19's first seven */
+            orc_q7 *t = lb->q7;
+            const int64_t n = LIN(lb, i, j, k), nn = LIN(lb, i_nb, j_nb, k_nb);
+            if (nb->type == ORC_TYPE_F) {
+              t->g[nn * 7 + l] = t->g_star[n * 7 + l];
+            } else if (nb->type == ORC_TYPE_S) {
+              if (t->typeE[nn] == ORC_TYPE_T)
+                t->g[n * 7 + q7_opp[l]] = -t->g_star[n * 7 + l] + 2 * q7_w[l] * t->T[nn];
+              else
+                t->g[n * 7 + q7_opp[l]] = t->g_star[n * 7 + l];
+            }
+          }
           if (nb->type == ORC_TYPE_F) {
             nb->f[l] = fl[i][j][k].f_star[l];
             if (th) th->g[LIN(lb, i_nb, j_nb, k_nb) * q + l] = th->g_star[LIN(lb, i, j, k) * q + l];
@@ -692,6 +795,24 @@ int orc_macroprop(orc_lbm *lb) {
           rho_uz += c->f[l] * lb->cz[l];
         }
         c->density = rho;
+        if (lb->q7 && c->type == ORC_TYPE_F) {
+          /* D3Q7 (lbm_oracle.h): T and the buoyancy before u */
+          orc_q7 *t = lb->q7;
+          const int64_t n = LIN(lb, i, j, k);
+          double T = 0, qx = 0, qy = 0, qz = 0;
+          for (int l = 0; l < 7; l++) {
+            T += t->g[n * 7 + l];
+            qx += t->g[n * 7 + l] * q7_cx[l];
+            qy += t->g[n * 7 + l] * q7_cy[l];
+            qz += t->g[n * 7 + l] * q7_cz[l];
+          }
+          t->T[n] = T;
+          t->q[3 * n + 0] = qx;
+          t->q[3 * n + 1] = qy;
+          t->q[3 * n + 2] = qz;
+          if (T < 0 || T > 100) bad |= 2;
+          if (t->boussinesq) c->F[1] = t->F0y[n] + t->g_beta * (T - t->T_ref);
+        }
         c->ux = (rho_ux + 0.5 * ORC_DT * c->F[0]) / rho;
         c->uy = (rho_uy + 0.5 * ORC_DT * c->F[1]) / rho;
         c->uz = (rho_uz + 0.5 * ORC_DT * c->F[2]) / rho;

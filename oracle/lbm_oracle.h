@@ -150,6 +150,27 @@ double orc_wall_temperature(const orc_lbm *lb, int i, int j, int k);
 double orc_geq(const orc_lbm *lb, int i, int j, int k, int l, int as_solid, double uwx,
                double uwy, double uwz);
 
+/* ---- D3Q7 temperature (no reference counterpart; the library's
+ * LBM_THERMAL_D3Q7).  The oracle's specification of it, in push form:
+ *   g_i, i = 0..6 over the rest and axis directions, w = 1/4, 1/8 (x6),
+ *   cs2 = 1/4;  geq_i = w_i T (1 + c_i.u/cs2);
+ *   Collision (TYPE_F):  g*_i = g_i (1 - 1/tau_g) + geq_i / tau_g with the
+ *     cell's T and u of the last MacroProp;
+ *   Streaming (TYPE_F x, neighbour x + c_i):  TYPE_F: g_i(nb) = g*_i(x);
+ *     TYPE_S with type_Energy TYPE_T: g_opp(x) = -g*_i(x) + 2 w_i T(nb);
+ *     other TYPE_S: g_opp(x) = g*_i(x);  any other type: nothing (frozen);
+ *   MacroProp (TYPE_F):  T = sum g, q = sum g c, then (boussinesq)
+ *     F[1] = F_y(set-up) + g_beta (T - T_ref), before u = (sum f c + F/2)/rho;
+ *     Init sets F[1] the same way from the set-up T (for the first collision).
+ * A solid's T stays its set-up value. */
+int orc_q7_enable(orc_lbm *lb, double tau_g, int boussinesq, double g_beta, double T_ref);
+/* T and type_Energy of every cell, after orc_set_cells (the set-up F_y the
+ * buoyancy adds to is read here) */
+void orc_q7_set(orc_lbm *lb, const double *T, const int16_t *typeE);
+void orc_q7_get(const orc_lbm *lb, double *T, double *qx, double *qy, double *qz);
+/* g as [cell][i], 7 per cell */
+void orc_q7_get_g(const orc_lbm *lb, double *g);
+
 #ifdef __cplusplus
 }
 #endif

@@ -53,6 +53,10 @@ _SIG = {
     "orc_set_thermal": (None, [_P, _D, C.POINTER(C.c_int16)]),
     "orc_get_thermal": (None, [_P, _D, _D, _D, _D]),
     "orc_get_g": (None, [_P, _D]),
+    "orc_q7_enable": (C.c_int, [_P, C.c_double, C.c_int, C.c_double, C.c_double]),
+    "orc_q7_set": (None, [_P, _D, C.POINTER(C.c_int16)]),
+    "orc_q7_get": (None, [_P, _D, _D, _D, _D]),
+    "orc_q7_get_g": (None, [_P, _D]),
     "orc_set_g": (None, [_P, _D]),
     "orc_wall_temperature": (C.c_double, [_P, C.c_int, C.c_int, C.c_int]),
     "orc_geq": (C.c_double, [_P] + [C.c_int] * 5 + [C.c_double] * 3),
@@ -193,6 +197,28 @@ class Oracle:
         g = np.ascontiguousarray(g, dtype=np.float64)
         assert g.shape == self.shape + (self.q,)
         self.lib.orc_set_g(self.h, _dp(g))
+
+    # -- D3Q7 temperature (no reference counterpart) ----------------------------
+    def q7_enable(self, tau_g, boussinesq=False, g_beta=0.0, T_ref=0.0):
+        assert self.lib.orc_q7_enable(self.h, float(tau_g), int(bool(boussinesq)), float(g_beta), float(T_ref))
+
+    def set_q7(self, T=None, type_energy=None):
+        """After set_cells (the buoyancy adds to the set-up F_y read here)."""
+        Ta = None if T is None else _f64(T, self.shape)
+        te = None
+        if type_energy is not None:
+            te = np.ascontiguousarray(np.broadcast_to(np.asarray(type_energy, np.int16), self.shape))
+        self.lib.orc_q7_set(self.h, _dp(Ta), None if te is None else te.ctypes.data_as(C.POINTER(C.c_int16)))
+
+    def q7_thermal(self):
+        out = [np.empty(self.shape) for _ in range(4)]
+        self.lib.orc_q7_get(self.h, *[_dp(a) for a in out])
+        return tuple(out)
+
+    def q7_g(self):
+        a = np.empty(self.shape + (7,))
+        self.lib.orc_q7_get_g(self.h, _dp(a))
+        return a
 
     def wall_temperature(self, i, j, k):
         return self.lib.orc_wall_temperature(self.h, i, j, k)

@@ -9,6 +9,7 @@ exact conservation laws, and brute-force re-derivations of single passes on
 tiny lattices.  Each test names the plausible mistake it would catch.
 """
 import itertools
+import math
 
 import numpy as np
 import pytest
@@ -588,3 +589,126 @@ def test_thermal_temperature_range_error(oracle_mod):
     o = _thermal_oracle(oracle_mod, case, th)
     o.init()
     assert o.run(1) == 3
+
+
+# ---- D3Q7 temperature (no reference counterpart: pinned by closed forms) ----
+
+def _q7_oracle(oracle_mod, case, th):
+    o = oracle_mod.Oracle(case.Nx, case.Ny, case.Nz, case.tau)
+    o.q7_enable(th.tau_g, th.boussinesq, th.g_beta, th.T_ref)
+    o.set_cells(**case.fields())
+    o.set_q7(th.T, th.type_energy)
+    o.init()
+    return o
+
+
+@pytest.mark.parametrize("U", [0.0, 0.04])
+def test_q7_sine_mode_exact_factor(oracle_mod, U):
+    """At tau_g = 1 one step maps T to sum_i w_i T(x - c_i)(1 + c_i.u/cs2):
+    a sine mode in x with a uniform flow ux = U is multiplied by exactly
+    lambda = 3/4 + cos(k)/4 - i U sin(k) per step (amplitude and phase).
+    Pins w = 1/4, 1/8, cs2 = 1/4, the sign of c.u and the streaming
+    direction of g."""
+    case, th = cases.q7_sine(Np=16, U=U)
+    o = _q7_oracle(oracle_mod, case, th)
+    n = 25
+    assert o.run(n) == 0
+    T = o.q7_thermal()[0]
+    p = case.params
+    lam = 0.75 + 0.25 * math.cos(p["k"]) - 1j * U * math.sin(p["k"])
+    x = np.arange(case.Nx + 2)
+    want = p["T0"] + p["A"] * np.imag(lam ** n * np.exp(1j * p["k"] * x))
+    assert np.abs(T[:, 2, 2] - want).max() < 1e-14
+    # the flow stays uniform (no force): u is U in every cell
+    _, ux, uy, uz = o.macro()
+    assert np.abs(ux - U).max() < 1e-15 and np.abs(uy).max() < 1e-15
+
+
+def test_q7_sine_mode_diffusivity(oracle_mod):
+    """Any tau_g: a long-wave sine mode at rest decays at alpha k^2 per step,
+    alpha = (tau_g - 1/2)/4, to O(k^2)."""
+    for tau_g in (0.7, 1.3):
+        case, th = cases.q7_sine(Np=64, U=0.0, tau_g=tau_g)
+        o = _q7_oracle(oracle_mod, case, th)
+        n = 400
+        assert o.run(n) == 0
+        T = o.q7_thermal()[0][:, 2, 2]
+        p = case.params
+        amp = np.sqrt(2.0 * np.mean((T - p["T0"]) ** 2))
+        rate = -math.log(amp / p["A"]) / n
+        alpha = (tau_g - 0.5) / 4.0
+        assert abs(rate / (alpha * p["k"] ** 2) - 1.0) < 1e-2
+
+
+def test_q7_dirichlet_conduction_linear_profile(oracle_mod):
+    """Two Dirichlet walls, fluid at rest: the steady profile is linear with
+    the wall temperatures at the half-way wall positions,
+    T_j = Tb + (Tt - Tb)(j - 1/2)/Ny (anti-bounce-back is exact for it)."""
+    for tau_g in (0.8, 1.0, 1.6):
+        case, th = cases.q7_conduction(Ny=8, tau_g=tau_g, Tb=1.0, Tt=0.25)
+        o = _q7_oracle(oracle_mod, case, th)
+        assert o.run(6000) == 0
+        T = o.q7_thermal()[0]
+        j = np.arange(1, case.Ny + 1)
+        want = 1.0 + (0.25 - 1.0) * (j - 0.5) / case.Ny
+        assert np.abs(T[:, 1:-1, :] - want[None, :, None]).max() < 1e-10
+
+
+def test_q7_adiabatic_walls_conserve_heat(oracle_mod):
+    """A closed box with adiabatic (bounce-back) walls and solid obstacles,
+    a moving, uneven flow: the total fluid heat sum T is constant (every g
+    population either streams to a fluid cell or bounces back)."""
+    case = cases.random_mask_case(8, 7, 6, seed=21, p_other=0.0)
+    t = case.type
+    for sl in ((0, slice(None), slice(None)), (-1, slice(None), slice(None)),
+               (slice(None), 0, slice(None)), (slice(None), -1, slice(None)),
+               (slice(None), slice(None), 0), (slice(None), slice(None), -1)):
+        t[sl] = cases.TYPE_S
+    case.ux[t == cases.TYPE_S] = 0.0
+    case.uy[t == cases.TYPE_S] = 0.0
+    case.uz[t == cases.TYPE_S] = 0.0
+    rng = np.random.default_rng(3)
+    T = 0.3 + 0.4 * rng.random(case.shape)
+    te = np.full(case.shape, cases.TYPE_Q, np.int16)
+    th = cases.q7_thermal(T, te, 0.9)
+    o = _q7_oracle(oracle_mod, case, th)
+    fl = t == cases.TYPE_F
+    h0 = o.q7_g()[fl].sum()
+    assert o.run(60) == 0
+    h1 = o.q7_thermal()[0][fl].sum()
+    assert abs(h1 - h0) < 1e-12 * h0
+    # and the field did change
+    assert np.abs(o.q7_thermal()[0][fl] - T[fl]).max() > 1e-3
+
+
+def test_q7_isothermal_dirichlet_box_is_a_fixed_point(oracle_mod):
+    """Dirichlet walls and fluid at rest, all at T_w: T stays T_w exactly
+    (-g* + 2 w T_w = w T_w on every wall link).  With a flow the uniform T
+    would not be a fixed point unless div u = 0: the equilibrium
+    w T (1 + c.u/cs2) transports T in conservative form."""
+    case = cases.random_mask_case(6, 6, 6, seed=23, p_other=0.0)
+    case.rho[:] = 1.0
+    for u in (case.ux, case.uy, case.uz):
+        u[:] = 0.0
+    te = np.where(case.type == cases.TYPE_S, cases.TYPE_T, 0).astype(np.int16)
+    th = cases.q7_thermal(np.full(case.shape, 0.625), te, 0.8)
+    o = _q7_oracle(oracle_mod, case, th)
+    assert o.run(30) == 0
+    T = o.q7_thermal()[0]
+    assert np.abs(T[case.type == cases.TYPE_F] - 0.625).max() < 1e-14
+
+
+def test_q7_boussinesq_force_enters_velocity(oracle_mod):
+    """F_y = g_beta (T - T_ref) at T's own time level, the first collision
+    included: in a periodic box at rest with uniform T = T_ref + d, the Guo
+    closed form rho u_y = (n - omega/2 + 1/2) g_beta d after n steps (the
+    first collision starts from the set-up u = 0)."""
+    case, th = cases.q7_sine(Np=8, U=0.0, A=0.0, T0=0.75)
+    th.boussinesq, th.g_beta, th.T_ref = True, 1e-4, 0.5
+    o = _q7_oracle(oracle_mod, case, th)
+    n = 10
+    assert o.run(n) == 0
+    rho, ux, uy, uz = o.macro()
+    d = 0.25
+    omega = 1.0 / case.tau
+    assert np.abs(rho * uy - (n - omega / 2 + 0.5) * th.g_beta * d).max() < 1e-15
