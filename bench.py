@@ -105,6 +105,37 @@ def cavity_leg(warmup, steps, size=256, storages=("f32", "f16s", "f16c")):
     return out
 
 
+def q7_leg(warmup, steps, size=256, storage="f32"):
+    """The fused D3Q19 + D3Q7 step (LBM_THERMAL_D3Q7): the 256^3 Taylor-Green
+    flow carrying T = 0.5 + 0.1 sin(kx) sin(ky) sin(kz), Boussinesq on
+    (g_beta = 1e-5), tau_g = 0.6.  MLUPS and GB/s at 19 + 7 populations
+    read and written per cell (209 B in FP32)."""
+    case = cases.taylor_green(size, u0=0.04, Re=1600.0)
+    k = 2 * np.pi / size
+    x = np.arange(size, dtype=np.float64)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    T = 0.5 + 0.1 * np.sin(k * X) * np.sin(k * Y) * np.sin(k * Z)
+    lat = lbm_amd.Lattice(case.Nx, case.Ny, case.Nz, case.tau, storage=storage)
+    lat.upload(**case.fields())
+    lat.upload_thermal(T, None, tau_g=0.6, boussinesq=True, g_beta=1e-5, T_ref=0.5, model="d3q7")
+    lat.init()
+    lat.step(warmup)
+    code = lat.sync()
+    if code:
+        raise RuntimeError(f"device status {code} during warmup")
+    ms = lat.time_steps(steps)
+    code = lat.sync()
+    n, bpc = lat.nfluid, lat.bytes_per_cell
+    lat.close()
+    if code:
+        raise RuntimeError(f"device status {code} during timed steps")
+    gbs = bpc * n * steps / (ms / 1e3) / 1e9
+    return {"workload": f"Taylor-Green {size}^3 + D3Q7 T, Boussinesq", "storage": storage,
+            "mlups": round(n * steps / (ms / 1e3) / 1e6, 1), "achieved_gbs": round(gbs, 1),
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_cell": bpc,
+            "ms_per_step": round(ms / steps, 5)}
+
+
 def ibm_channel(Nx=256, Ny=128, Nz=64, D=20.0, u_in=0.04, Re=100.0, n_theta=64):
     """Rigid IBM cylinder in a channel: LBM(256,128,64), inlet/outlet faces,
     walls at y and z, u_in = 0.04, Re = 100 on D = 20, +-1% seeded
@@ -297,6 +328,7 @@ def main():
     ap.add_argument("--ibm-cmp-steps", type=int, default=2000)
     ap.add_argument("--no-ibm", action="store_true")
     ap.add_argument("--no-cavity", action="store_true")
+    ap.add_argument("--no-q7", action="store_true")
     ap.add_argument("--ibm-only", action="store_true", help="run only the IBM channel leg")
     ap.add_argument("--batch-steps", type=int, default=4000)
     ap.add_argument("--no-batch", action="store_true")
@@ -357,6 +389,12 @@ def main():
         cav = cavity_leg(a.warmup, a.steps)
         log(f"bench: cavity {json.dumps(cav)}")
 
+    q7 = None
+    if not a.no_q7:
+        log("bench: D3Q7 temperature leg")
+        q7 = q7_leg(a.warmup, a.steps)
+        log(f"bench: d3q7 {json.dumps(q7)}")
+
     ibm = None
     if not a.no_ibm:
         log(f"bench: IBM channel leg, {a.ibm_steps} steps")
@@ -407,6 +445,7 @@ def main():
         **others,
         "precision": prec,
         "cavity": cav,
+        "d3q7": q7,
         "ibm": ibm,
         "batch": batch,
         "cpu_baseline": cpu,

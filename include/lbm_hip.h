@@ -212,6 +212,40 @@ int lbm_ibm_markers(lbm_ctx *ctx, double *x, double *y, double *z, double *ub,
  * after the last IBLBM). */
 int lbm_ibm_box(lbm_ctx *ctx, int32_t box[6], double center[2]);
 
+/* ---- flexible filament (no reference counterpart: the reference has no
+ * flexible-structure solver; the north star's Lagrangian structure update)
+ * ------------------------------------------------------------------------
+ * An inextensible elastic filament in lattice units,
+ *   m_l X_tt = d/ds (T dX/ds) - EI d4X/ds4 + G - gamma X_t + F,  |dX/ds| = 1,
+ * on nnodes nodes (ds = length/(nnodes - 1)), node 0 clamped at `clamp`
+ * along `tangent`, the last node free.  FP64 on the device: explicit
+ * bending, segment tensions from the exact length constraint (Newton with a
+ * tridiagonal Jacobian).  Created straight along `tangent`, at rest. */
+typedef struct lbm_filament lbm_filament;
+typedef struct lbm_filament_desc {
+  int32_t nnodes;       /* 4 .. 256                                          */
+  int32_t substeps;     /* structure steps per time step; 0: chosen below   *
+                         * the explicit bending limit ds^2 sqrt(m_l/EI)/2   */
+  double length;        /* L                                                */
+  double line_density;  /* m_l, mass per unit length                        */
+  double bending;       /* EI                                               */
+  double damping;       /* gamma, force per unit length and velocity        */
+  double body_force[3]; /* G per unit length (weight minus buoyancy)        */
+  double clamp[3];      /* X_0                                              */
+  double tangent[3];    /* unit slope at the clamp                          */
+} lbm_filament_desc;
+int lbm_filament_create(int32_t device, const lbm_filament_desc *desc, lbm_filament **out);
+int lbm_filament_destroy(lbm_filament *f);
+/* the structure steps one time step takes */
+int lbm_filament_substeps(const lbm_filament *f);
+/* node positions now and one time step before (3 x nnodes, node-major;
+ * x_prev NULL: at rest), and the nnodes-1 segment tensions */
+int lbm_filament_set(lbm_filament *f, const double *x_now, const double *x_prev);
+int lbm_filament_get(lbm_filament *f, double *x_now, double *x_prev, double *tension);
+/* nsteps time steps of length dt under the force F per unit length
+ * (3 x nnodes, held over the steps; NULL: none) */
+int lbm_filament_step(lbm_filament *f, const double *force, double dt, int32_t nsteps);
+
 /* ---- macroscopic output (OutputVTK, lbm.cpp:2290-2468) ---------------------
  * The reference's .vtr file: a VTK RectilinearGrid with cells i = 0..Nx+1,
  * j = 0..Ny+1, k = 1..Nz and appended raw Float32 blocks Density, Velocity

@@ -43,6 +43,7 @@ SOURCES = [
     os.path.join(HERE, "csrc", "lbm_output.hip"),
     os.path.join(HERE, "csrc", "lbm_batch.hip"),
     os.path.join(HERE, "csrc", "lbm_q7.hip"),
+    os.path.join(HERE, "csrc", "lbm_filament.hip"),
 ]
 HOST_SOURCES = [
     os.path.join(HERE, "host", "LBM.cpp"),

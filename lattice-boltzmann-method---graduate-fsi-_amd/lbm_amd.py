@@ -86,6 +86,20 @@ class _ThermalDesc(C.Structure):
 THERMAL_ENERGY, THERMAL_D3Q7 = 0, 1
 
 
+class _FilamentDesc(C.Structure):
+    _fields_ = [
+        ("nnodes", C.c_int32),
+        ("substeps", C.c_int32),
+        ("length", C.c_double),
+        ("line_density", C.c_double),
+        ("bending", C.c_double),
+        ("damping", C.c_double),
+        ("body_force", C.c_double * 3),
+        ("clamp", C.c_double * 3),
+        ("tangent", C.c_double * 3),
+    ]
+
+
 class _VtkDesc(C.Structure):
     _fields_ = [
         ("u_scale", C.c_double),
@@ -159,6 +173,12 @@ _SIGNATURES = {
     "lbm_batch_nfluid": (C.c_int64, [_P]),
     "lbm_batch_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _D]),
     "lbm_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32)]),
+    "lbm_filament_create": (C.c_int, [C.c_int32, C.POINTER(_FilamentDesc), C.POINTER(_P)]),
+    "lbm_filament_destroy": (C.c_int, [_P]),
+    "lbm_filament_substeps": (C.c_int, [_P]),
+    "lbm_filament_set": (C.c_int, [_P, _D, _D]),
+    "lbm_filament_get": (C.c_int, [_P, _D, _D, _D]),
+    "lbm_filament_step": (C.c_int, [_P, _D, C.c_double, C.c_int32]),
 }
 
 _lib = None
@@ -213,6 +233,50 @@ def storage_roundtrip(values, storage, device: int = 0) -> np.ndarray:
     _check(load_library().lbm_storage_roundtrip(device, st, x.ctypes.data_as(_D), out.ctypes.data_as(_D),
                                                 x.size), "lbm_storage_roundtrip")
     return out.reshape(np.shape(values))
+
+
+class Filament:
+    """A flexible filament on the device (lbm_filament_*)."""
+
+    def __init__(self, n, length, line_density, bending, damping=0.0, body=(0.0, 0.0, 0.0),
+                 clamp=(0.0, 0.0, 0.0), tangent=(1.0, 0.0, 0.0), substeps=0, device=0):
+        self._lib = load_library()
+        d = _FilamentDesc(int(n), int(substeps), float(length), float(line_density), float(bending),
+                          float(damping), (C.c_double * 3)(*body), (C.c_double * 3)(*clamp),
+                          (C.c_double * 3)(*tangent))
+        h = _P()
+        _check(self._lib.lbm_filament_create(int(device), C.byref(d), C.byref(h)), "lbm_filament_create")
+        self._h = h
+        self.n = int(n)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.lbm_filament_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def substeps(self):
+        return self._lib.lbm_filament_substeps(self._h)
+
+    def set(self, x_now, x_prev=None):
+        a = np.ascontiguousarray(x_now, dtype=np.float64).reshape(self.n, 3)
+        b = None if x_prev is None else np.ascontiguousarray(x_prev, dtype=np.float64).reshape(self.n, 3)
+        _check(self._lib.lbm_filament_set(self._h, _dp(a), _dp(b)), "lbm_filament_set")
+
+    def get(self):
+        x, xp, t = np.empty((self.n, 3)), np.empty((self.n, 3)), np.empty(self.n - 1)
+        _check(self._lib.lbm_filament_get(self._h, _dp(x), _dp(xp), _dp(t)), "lbm_filament_get")
+        return x, xp, t
+
+    def step(self, force=None, dt=1.0, nsteps=1):
+        F = None if force is None else np.ascontiguousarray(force, dtype=np.float64).reshape(self.n, 3)
+        _check(self._lib.lbm_filament_step(self._h, _dp(F), float(dt), int(nsteps)), "lbm_filament_step")
 
 
 class Lattice:

@@ -1060,3 +1060,188 @@ int orc_ibm_step(orc_lbm *lb, int nm, double *mx, double *my, double *mz,
   free(udes);
   return err;
 }
+
+/* ---------------------------------------------------------------------------
+ * Flexible filament (specification in lbm_oracle.h).
+ * ------------------------------------------------------------------------- */
+struct orc_filament {
+  int n, substeps;
+  double ds, ml, ei, gamma, G[3], X0[3], e0[3];
+  double *x, *xp, *T; /* 3n, 3n, n-1 */
+};
+
+int orc_fil_substeps(const orc_filament *f) { return f->substeps; }
+
+orc_filament *orc_fil_create(int n, double length, double line_density, double bending,
+                             double damping, const double body[3], const double clamp[3],
+                             const double tangent[3], int substeps) {
+  if (n < 4) return NULL;
+  orc_filament *f = (orc_filament *)calloc(1, sizeof(orc_filament));
+  f->n = n;
+  f->ds = length / (n - 1);
+  f->ml = line_density;
+  f->ei = bending;
+  f->gamma = damping;
+  for (int d = 0; d < 3; d++) {
+    f->G[d] = body[d];
+    f->X0[d] = clamp[d];
+    f->e0[d] = tangent[d];
+  }
+  if (substeps <= 0) {
+    /* explicit bending: h <= ds^2 sqrt(m_l/EI)/2 (leapfrog with the largest
+     * D4 eigenvalue 16/ds^4), taken with a factor 0.8 */
+    double hc = bending > 0 ? f->ds * f->ds * sqrt(line_density / bending) / 2 : 1.0;
+    substeps = (int)ceil(1.0 / (0.8 * hc));
+    if (substeps < 1) substeps = 1;
+  }
+  f->substeps = substeps;
+  f->x = (double *)calloc(3 * (size_t)n, sizeof(double));
+  f->xp = (double *)calloc(3 * (size_t)n, sizeof(double));
+  f->T = (double *)calloc((size_t)n - 1, sizeof(double));
+  for (int i = 0; i < n; i++)
+    for (int d = 0; d < 3; d++) f->x[3 * i + d] = f->xp[3 * i + d] = clamp[d] + i * f->ds * tangent[d];
+  return f;
+}
+
+void orc_fil_destroy(orc_filament *f) {
+  if (!f) return;
+  free(f->x);
+  free(f->xp);
+  free(f->T);
+  free(f);
+}
+
+void orc_fil_set(orc_filament *f, const double *x_now, const double *x_prev) {
+  memcpy(f->x, x_now, sizeof(double) * 3 * f->n);
+  memcpy(f->xp, x_prev ? x_prev : x_now, sizeof(double) * 3 * f->n);
+}
+
+void orc_fil_get(const orc_filament *f, double *x_now, double *x_prev, double *tension) {
+  if (x_now) memcpy(x_now, f->x, sizeof(double) * 3 * f->n);
+  if (x_prev) memcpy(x_prev, f->xp, sizeof(double) * 3 * f->n);
+  if (tension) memcpy(tension, f->T, sizeof(double) * (f->n - 1));
+}
+
+/* Parallel cyclic reduction of a[i] x[i-1] + b[i] x[i] + c[i] x[i+1] = d[i],
+ * i < m, in place (x returned in d); every row of a stride is formed from
+ * the previous stride's rows. */
+static void fil_pcr(int m, double *a, double *b, double *c, double *d) {
+  double *na = (double *)malloc(sizeof(double) * m), *nb = (double *)malloc(sizeof(double) * m);
+  double *nc = (double *)malloc(sizeof(double) * m), *nd = (double *)malloc(sizeof(double) * m);
+  for (int s = 1; s < m; s <<= 1) {
+    for (int i = 0; i < m; i++) {
+      double al = 0, ga = 0, a2 = 0, c2 = 0, bb = b[i], dd = d[i];
+      if (i - s >= 0) {
+        al = -a[i] / b[i - s];
+        a2 = al * a[i - s];
+        bb = bb + al * c[i - s];
+        dd = dd + al * d[i - s];
+      }
+      if (i + s < m) {
+        ga = -c[i] / b[i + s];
+        c2 = ga * c[i + s];
+        bb = bb + ga * a[i + s];
+        dd = dd + ga * d[i + s];
+      }
+      na[i] = a2;
+      nb[i] = bb;
+      nc[i] = c2;
+      nd[i] = dd;
+    }
+    memcpy(a, na, sizeof(double) * m);
+    memcpy(b, nb, sizeof(double) * m);
+    memcpy(c, nc, sizeof(double) * m);
+    memcpy(d, nd, sizeof(double) * m);
+  }
+  for (int i = 0; i < m; i++) d[i] = d[i] / b[i];
+  free(na); free(nb); free(nc); free(nd);
+}
+
+static void fil_substep(orc_filament *f, const double *F, double h) {
+  const int n = f->n, m = n - 1;
+  const double ds = f->ds, ds4 = ds * ds * ds * ds, ml = f->ml;
+  const double kap = h * h / (ml * ds);
+  double *X = f->x, *Xp = f->xp;
+  double *a = (double *)malloc(sizeof(double) * 3 * m);
+  double *P = (double *)malloc(sizeof(double) * 3 * n);
+  double *xg = (double *)malloc(sizeof(double) * 3 * (n + 3)); /* ghosts: index i+1 */
+  double *ja = (double *)malloc(sizeof(double) * m), *jb = (double *)malloc(sizeof(double) * m);
+  double *jc = (double *)malloc(sizeof(double) * m), *rr = (double *)malloc(sizeof(double) * m);
+  double *dv = (double *)malloc(sizeof(double) * 3 * m);
+  for (int j = 0; j < m; j++)
+    for (int d = 0; d < 3; d++) a[3 * j + d] = (X[3 * (j + 1) + d] - X[3 * j + d]) / ds;
+  /* positions with ghost nodes -1, N, N+1 */
+  for (int i = 0; i < n; i++)
+    for (int d = 0; d < 3; d++) xg[3 * (i + 1) + d] = X[3 * i + d];
+  for (int d = 0; d < 3; d++) {
+    xg[d] = X[3 + d] - 2 * ds * f->e0[d];
+    xg[3 * (n + 1) + d] = 2 * X[3 * (n - 1) + d] - X[3 * (n - 2) + d];
+    xg[3 * (n + 2) + d] = 2 * xg[3 * (n + 1) + d] - 2 * X[3 * (n - 2) + d] + X[3 * (n - 3) + d];
+  }
+  /* predictor */
+  for (int d = 0; d < 3; d++) P[d] = f->X0[d];
+  for (int i = 1; i < n; i++)
+    for (int d = 0; d < 3; d++) {
+      const double d4 = (xg[3 * (i - 1) + d] - 4 * xg[3 * i + d] + 6 * xg[3 * (i + 1) + d] - 4 * xg[3 * (i + 2) + d] +
+                         xg[3 * (i + 3) + d]) / ds4;
+      const double B = -f->ei * d4;
+      const double Fi = F ? F[3 * i + d] : 0.0;
+      const double v = (X[3 * i + d] - Xp[3 * i + d]) / h;
+      P[3 * i + d] = 2 * X[3 * i + d] - Xp[3 * i + d] + h * h / ml * (B + f->G[d] + Fi - f->gamma * v);
+    }
+  /* Newton on |X'_{j+1} - X'_j|^2 = ds^2 */
+  for (int it = 0; it < 3; it++) {
+    for (int j = 0; j < m; j++) {
+      /* X'_{j+1} - X'_j with X'_i = P_i + kap c_i (T_i a_i - T_{i-1} a_{i-1}) */
+      const double cj1 = (j + 1 == n - 1) ? 2.0 : 1.0, cj = 1.0;
+      const double Tj1 = (j + 1 < m) ? f->T[j + 1] : 0.0, Tj = f->T[j], Tjm = (j >= 1) ? f->T[j - 1] : 0.0;
+      for (int d = 0; d < 3; d++) {
+        const double a1 = (j + 1 < m) ? a[3 * (j + 1) + d] : 0.0, a0 = a[3 * j + d],
+                     am = (j >= 1) ? a[3 * (j - 1) + d] : 0.0;
+        const double xn1 = P[3 * (j + 1) + d] + kap * cj1 * (Tj1 * a1 - Tj * a0);
+        const double xn0 = (j == 0) ? P[d] : P[3 * j + d] + kap * cj * (Tj * a0 - Tjm * am);
+        dv[3 * j + d] = xn1 - xn0;
+      }
+      const double dx = dv[3 * j], dy = dv[3 * j + 1], dz = dv[3 * j + 2];
+      rr[j] = -((dx * dx + dy * dy + dz * dz) - ds * ds);
+      /* d(d_j)/dT_{j+1} = kap c_{j+1} a_{j+1}; /dT_j = -kap (c_{j+1} + c_j') a_j
+       * (c_j' = 0 at the clamp); /dT_{j-1} = kap c_j a_{j-1} */
+      const double cjp = (j == 0) ? 0.0 : cj;
+      double s1 = 0, s0 = 0, sm = 0;
+      for (int d = 0; d < 3; d++) {
+        const double a1 = (j + 1 < m) ? a[3 * (j + 1) + d] : 0.0, a0 = a[3 * j + d],
+                     am = (j >= 1) ? a[3 * (j - 1) + d] : 0.0;
+        s1 += dv[3 * j + d] * a1;
+        s0 += dv[3 * j + d] * a0;
+        sm += dv[3 * j + d] * am;
+      }
+      jc[j] = (j + 1 < m) ? 2 * kap * cj1 * s1 : 0.0;
+      jb[j] = -2 * kap * (cj1 + cjp) * s0;
+      ja[j] = (j >= 1) ? 2 * kap * cj * sm : 0.0;
+    }
+    fil_pcr(m, ja, jb, jc, rr);
+    for (int j = 0; j < m; j++) f->T[j] += rr[j];
+  }
+  /* corrector */
+  for (int d = 0; d < 3; d++) {
+    Xp[d] = X[d];
+    X[d] = f->X0[d];
+  }
+  for (int i = 1; i < n; i++) {
+    const double ci = (i == n - 1) ? 2.0 : 1.0;
+    const double Ti = (i < m) ? f->T[i] : 0.0, Tim = f->T[i - 1];
+    for (int d = 0; d < 3; d++) {
+      const double ai = (i < m) ? a[3 * i + d] : 0.0, aim = a[3 * (i - 1) + d];
+      const double xn = P[3 * i + d] + kap * ci * (Ti * ai - Tim * aim);
+      Xp[3 * i + d] = X[3 * i + d];
+      X[3 * i + d] = xn;
+    }
+  }
+  free(a); free(P); free(xg); free(ja); free(jb); free(jc); free(rr); free(dv);
+}
+
+void orc_fil_step(orc_filament *f, const double *force, double dt, int nsteps) {
+  const double h = dt / f->substeps;
+  for (int s = 0; s < nsteps; s++)
+    for (int k = 0; k < f->substeps; k++) fil_substep(f, force, h);
+}

@@ -171,6 +171,35 @@ void orc_q7_get(const orc_lbm *lb, double *T, double *qx, double *qy, double *qz
 /* g as [cell][i], 7 per cell */
 void orc_q7_get_g(const orc_lbm *lb, double *g);
 
+/* ---- flexible filament (no reference counterpart: the reference's
+ * flexible-filament structure solver is absent).  The oracle's
+ * specification, lattice units:
+ *   m_l X_tt = d/ds (T dX/ds) - EI d4X/ds4 + G - gamma X_t + F,  |dX/ds| = 1,
+ * nodes i = 0..N-1 at spacing ds = L/(N-1); node 0 clamped at X0 with slope
+ * e0 (ghost X_{-1} = X_1 - 2 ds e0); free end d2X/ds2 = d3X/ds3 = 0
+ * (ghosts X_N = 2X_{N-1} - X_{N-2}, X_{N+1} = 2X_N - 2X_{N-2} + X_{N-3})
+ * and T = 0; the end node carries half a segment (its tension term is
+ * doubled).  One structure step of size h (K per call, h = dt/K):
+ *   a_j = (X_{j+1} - X_j)/ds                         (tangents at time n)
+ *   P_i = 2X_i - Xp_i + h^2/m_l (B_i + G + F_i - gamma (X_i - Xp_i)/h)
+ *   X'_i = P_i + kappa c_i (T_i a_i - T_{i-1} a_{i-1}),  kappa = h^2/(m_l ds),
+ *          c_i = 1 (c_{N-1} = 2), T_{N-1} = 0, X'_0 = X0;
+ *   the segment tensions T_0..T_{N-2} make |X'_{j+1} - X'_j|^2 = ds^2:
+ *   three Newton steps from the previous T, each a tridiagonal solve by
+ *   parallel cyclic reduction (strides 1, 2, 4, ...; rows beyond the ends
+ *   are zero), in this order of operations. */
+typedef struct orc_filament orc_filament;
+orc_filament *orc_fil_create(int n, double length, double line_density, double bending,
+                             double damping, const double body[3], const double clamp[3],
+                             const double tangent[3], int substeps);
+void orc_fil_destroy(orc_filament *f);
+int orc_fil_substeps(const orc_filament *f);
+/* node-major x,y,z (3n each); x_prev may be NULL (= x_now) */
+void orc_fil_set(orc_filament *f, const double *x_now, const double *x_prev);
+void orc_fil_get(const orc_filament *f, double *x_now, double *x_prev, double *tension);
+/* nsteps calls, each advancing dt with F (3n per unit length, or NULL) */
+void orc_fil_step(orc_filament *f, const double *force, double dt, int nsteps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -57,6 +57,12 @@ _SIG = {
     "orc_q7_set": (None, [_P, _D, C.POINTER(C.c_int16)]),
     "orc_q7_get": (None, [_P, _D, _D, _D, _D]),
     "orc_q7_get_g": (None, [_P, _D]),
+    "orc_fil_create": (_P, [C.c_int] + [C.c_double] * 4 + [_D, _D, _D, C.c_int]),
+    "orc_fil_destroy": (None, [_P]),
+    "orc_fil_substeps": (C.c_int, [_P]),
+    "orc_fil_set": (None, [_P, _D, _D]),
+    "orc_fil_get": (None, [_P, _D, _D, _D]),
+    "orc_fil_step": (None, [_P, _D, C.c_double, C.c_int]),
     "orc_set_g": (None, [_P, _D]),
     "orc_wall_temperature": (C.c_double, [_P, C.c_int, C.c_int, C.c_int]),
     "orc_geq": (C.c_double, [_P] + [C.c_int] * 5 + [C.c_double] * 3),
@@ -265,6 +271,49 @@ class Oracle:
             _dp(markers["ub"]), _dp(markers["Fb"]), _dp(markers["ds"]),
             _dp(vcm), _dp(vcb), *[int(b) for b in box], int(m_max), float(vsolid), _dp(out))
         return err, out
+
+
+class Filament:
+    """The oracle's flexible filament (specification in lbm_oracle.h)."""
+
+    def __init__(self, n, length, line_density, bending, damping=0.0, body=(0.0, 0.0, 0.0),
+                 clamp=(0.0, 0.0, 0.0), tangent=(1.0, 0.0, 0.0), substeps=0):
+        self.lib = load()
+        arr = lambda v: np.ascontiguousarray(v, dtype=np.float64)  # noqa: E731
+        self._keep = [arr(body), arr(clamp), arr(tangent)]
+        self.h = self.lib.orc_fil_create(int(n), float(length), float(line_density), float(bending),
+                                         float(damping), *[_dp(a) for a in self._keep], int(substeps))
+        assert self.h
+        self.n = int(n)
+
+    def close(self):
+        if self.h:
+            self.lib.orc_fil_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def substeps(self):
+        return self.lib.orc_fil_substeps(self.h)
+
+    def set(self, x_now, x_prev=None):
+        a = np.ascontiguousarray(x_now, dtype=np.float64).reshape(self.n, 3)
+        b = None if x_prev is None else np.ascontiguousarray(x_prev, dtype=np.float64).reshape(self.n, 3)
+        self.lib.orc_fil_set(self.h, _dp(a), _dp(b))
+
+    def get(self):
+        x, xp, t = np.empty((self.n, 3)), np.empty((self.n, 3)), np.empty(self.n - 1)
+        self.lib.orc_fil_get(self.h, _dp(x), _dp(xp), _dp(t))
+        return x, xp, t
+
+    def step(self, force=None, dt=1.0, nsteps=1):
+        F = None if force is None else np.ascontiguousarray(force, dtype=np.float64).reshape(self.n, 3)
+        self.lib.orc_fil_step(self.h, _dp(F), float(dt), int(nsteps))
 
 
 def kernel(r: float) -> float:

@@ -712,3 +712,74 @@ def test_q7_boussinesq_force_enters_velocity(oracle_mod):
     d = 0.25
     omega = 1.0 / case.tau
     assert np.abs(rho * uy - (n - omega / 2 + 0.5) * th.g_beta * d).max() < 1e-15
+
+
+# ---- flexible filament (no reference counterpart: closed forms) -----------
+
+def _cantilever(oracle_mod, damping=0.0, body=(0.0, 0.0, 0.0), n=41, L=40.0, ml=1.5, EI=3.0):
+    return oracle_mod.Filament(n, L, ml, EI, damping=damping, body=body, clamp=(10.0, 20.0, 5.0),
+                               tangent=(1.0, 0.0, 0.0))
+
+
+def test_filament_hanging_tension_exact(oracle_mod):
+    """A filament clamped pointing down under its weight m_l g per unit
+    length hangs straight with segment tensions m_l g (L - s_mid) (the free
+    end node carries half a segment), and keeps its segment lengths."""
+    n, L, ml, g = 21, 20.0, 1.5, 1e-3
+    f = oracle_mod.Filament(n, L, ml, 2.0, damping=0.5, body=(0.0, -ml * g, 0.0), clamp=(0.0, 0.0, 0.0),
+                            tangent=(0.0, -1.0, 0.0))
+    f.step(nsteps=20)
+    x, _, T = f.get()
+    ds = L / (n - 1)
+    s_mid = (np.arange(n - 1) + 0.5) * ds
+    assert np.abs(T - ml * g * (L - s_mid)).max() < 1e-12
+    assert np.abs(x[:, 0]).max() < 1e-14 and np.abs(x[:, 2]).max() < 1e-14
+    assert np.abs(np.linalg.norm(np.diff(x, axis=0), axis=1) - ds).max() < 1e-12
+
+
+def test_filament_cantilever_static_deflection(oracle_mod):
+    """Uniform transverse load q on a clamped-free filament, damped to rest:
+    tip deflection qL^4/(8 EI) (small deflection, 1% of L)."""
+    L, EI, ml = 40.0, 3.0, 1.5
+    q = 0.08 * EI / L ** 3
+    w1 = 1.8751040687 ** 2 * math.sqrt(EI / (ml * L ** 4))
+    f = _cantilever(oracle_mod, damping=2 * ml * w1, L=L, EI=EI, ml=ml)
+    F = np.zeros((f.n, 3))
+    F[:, 1] = -q
+    for _ in range(12):
+        f.step(F, nsteps=2000)
+    x, xp, _ = f.get()
+    tip = 20.0 - x[-1, 1]
+    assert abs(tip / (q * L ** 4 / (8 * EI)) - 1) < 2e-3, tip
+    assert np.abs(x - xp).max() < 1e-9  # at rest
+
+
+def test_filament_cantilever_first_mode(oracle_mod):
+    """Released from a small static deflection, the free tip oscillates at
+    the first clamped-free beam mode, omega_1 = 1.8751^2 sqrt(EI/(m_l L^4)),
+    and the segment lengths stay ds."""
+    L, EI, ml = 40.0, 3.0, 1.5
+    w1 = 1.8751040687 ** 2 * math.sqrt(EI / (ml * L ** 4))
+    q = 0.02 * EI / L ** 3
+    f = _cantilever(oracle_mod, damping=2 * ml * w1, L=L, EI=EI, ml=ml)
+    F = np.zeros((f.n, 3))
+    F[:, 1] = -q
+    for _ in range(12):
+        f.step(F, nsteps=2000)
+    x0, _, _ = f.get()
+    g = _cantilever(oracle_mod, L=L, EI=EI, ml=ml)
+    g.set(x0, x0)
+    period = 2 * math.pi / w1
+    nsteps = int(3 * period)
+    tip = np.empty(nsteps)
+    for k in range(nsteps):
+        g.step(None, nsteps=1)
+        tip[k] = g.get()[0][-1, 1] - 20.0
+    # upward zero crossings of the tip, interpolated
+    z = np.flatnonzero((tip[:-1] < 0) & (tip[1:] >= 0))
+    t = z + (-tip[z]) / (tip[z + 1] - tip[z])
+    assert len(t) >= 2
+    measured = 2 * math.pi / np.mean(np.diff(t))
+    assert abs(measured / w1 - 1) < 1e-2, (measured, w1)
+    x = g.get()[0]
+    assert np.abs(np.linalg.norm(np.diff(x, axis=0), axis=1) - L / (f.n - 1)).max() < 1e-12
