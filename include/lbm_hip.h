@@ -246,6 +246,21 @@ int lbm_filament_get(lbm_filament *f, double *x_now, double *x_prev, double *ten
  * (3 x nnodes, held over the steps; NULL: none) */
 int lbm_filament_step(lbm_filament *f, const double *force, double dt, int32_t nsteps);
 
+/* A filament coupled to the lattice through the implicit IBM (no reference
+ * counterpart): its nodes become the markers (weights ds, half at the ends),
+ * each with the desired velocity of its displacement over the last step and
+ * the density of the lattice cell it sits in (a thin body has fluid on both
+ * sides, so the reference's farthest-point-along-the-normal density does not
+ * apply).  desc gives the box (holding the filament's reach) and m_max;
+ * nmarkers, moving, follow and vsolid are set here.  Requires lbm_init; the
+ * filament lives on the lattice's device and must outlive the coupling. */
+int lbm_fsi_setup(lbm_ctx *ctx, lbm_filament *f, const lbm_ibm_desc *desc);
+/* nsteps coupled steps (IBM with the filament's markers, Collision,
+ * Streaming, then the filament step under -sum F_b per unit length, the
+ * force its markers exert on the fluid taken back).  out: nsteps x 6 = the
+ * total force on the filament and its free end's position, per step. */
+int lbm_fsi_run(lbm_ctx *ctx, int32_t nsteps, double *out);
+
 /* ---- macroscopic output (OutputVTK, lbm.cpp:2290-2468) ---------------------
  * The reference's .vtr file: a VTK RectilinearGrid with cells i = 0..Nx+1,
  * j = 0..Ny+1, k = 1..Nz and appended raw Float32 blocks Density, Velocity

@@ -212,6 +212,40 @@ static void fil_free(lbm_filament *f) {
     if ((x) != hipSuccess) return LBM_ERR_HIP;  \
   } while (0)
 
+// the device state the lattice coupling reads (lbm_ibm.hip)
+void filament_state(lbm_filament *f, double **x, double **xp, int *n, double *ds, int *dev) {
+  *x = f->x;
+  *xp = f->xp;
+  *n = f->d.nnodes;
+  *ds = f->ds;
+  *dev = f->dev;
+}
+
+// device-pointer form for the coupled step: force already on the device
+int filament_step_dev(lbm_filament *f, const double *F_dev, double dt, int32_t ncalls, hipStream_t s) {
+  FilArgs a;
+  a.n = f->d.nnodes;
+  a.m = a.n - 1;
+  a.nsub = f->substeps;
+  a.ds = f->ds;
+  a.ml = f->d.line_density;
+  a.ei = f->d.bending;
+  a.gamma = f->d.damping;
+  a.h = dt / f->substeps;
+  for (int q = 0; q < 3; q++) {
+    a.G[q] = f->d.body_force[q];
+    a.X0[q] = f->d.clamp[q];
+    a.e0[q] = f->d.tangent[q];
+  }
+  a.x = f->x;
+  a.xp = f->xp;
+  a.T = f->T;
+  a.F = F_dev;
+  hipLaunchKernelGGL(k_fil_step, dim3(1), dim3(FIL_MAX), 0, s, a, (int)ncalls);
+  return hipGetLastError() == hipSuccess ? LBM_OK : LBM_ERR_HIP;
+}
+
+
 extern "C" {
 
 int lbm_filament_create(int32_t device, const lbm_filament_desc *d, lbm_filament **out) {
@@ -284,30 +318,6 @@ int lbm_filament_get(lbm_filament *f, double *x_now, double *x_prev, double *ten
   if (x_prev) FCHK(hipMemcpy(x_prev, f->xp, b, hipMemcpyDeviceToHost));
   if (tension) FCHK(hipMemcpy(tension, f->T, 8 * (size_t)(f->d.nnodes - 1), hipMemcpyDeviceToHost));
   return LBM_OK;
-}
-
-// device-pointer form for the coupled step: force already on the device
-int filament_step_dev(lbm_filament *f, const double *F_dev, double dt, int32_t ncalls, hipStream_t s) {
-  FilArgs a;
-  a.n = f->d.nnodes;
-  a.m = a.n - 1;
-  a.nsub = f->substeps;
-  a.ds = f->ds;
-  a.ml = f->d.line_density;
-  a.ei = f->d.bending;
-  a.gamma = f->d.damping;
-  a.h = dt / f->substeps;
-  for (int q = 0; q < 3; q++) {
-    a.G[q] = f->d.body_force[q];
-    a.X0[q] = f->d.clamp[q];
-    a.e0[q] = f->d.tangent[q];
-  }
-  a.x = f->x;
-  a.xp = f->xp;
-  a.T = f->T;
-  a.F = F_dev;
-  hipLaunchKernelGGL(k_fil_step, dim3(1), dim3(FIL_MAX), 0, s, a, (int)ncalls);
-  return hipGetLastError() == hipSuccess ? LBM_OK : LBM_ERR_HIP;
 }
 
 int lbm_filament_step(lbm_filament *f, const double *force, double dt, int32_t nsteps) {

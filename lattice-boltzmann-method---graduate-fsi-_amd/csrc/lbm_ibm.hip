@@ -47,6 +47,10 @@
 
 using namespace lbm;
 
+// the filament's device state and step (lbm_filament.hip)
+void filament_state(lbm_filament *f, double **x, double **xp, int *n, double *ds, int *dev);
+int filament_step_dev(lbm_filament *f, const double *F_dev, double dt, int32_t ncalls, hipStream_t s);
+
 namespace {
 
 constexpr int CHUNK_E = 1024;  // (marker, point) pairs per spreading workgroup
@@ -331,6 +335,8 @@ struct ForceArgs {
   double u0, u1, u2;     // u_des = V_CM
   int force;             // 0: interpolation only (last pass); 1: also the next force
   int first;             // 1: start the running drag/lift
+  const double *ud;      // per-marker u_des (3 per marker) in place of V_CM, or null
+  double *fsum;          // running -F_b per marker (3 per marker), or null
 };
 
 // Interpolation of one marker per wavefront (lbm.cpp:1295-1306 and
@@ -371,10 +377,11 @@ __global__ __launch_bounds__(256) void k_ibm_interp(int nm, const int32_t *__res
     if (fa.force) {
       const double dt = 1.0, dx = 1.0, dy = 1.0;
       const double rho = rho_n;
-      const double ud = lane == 0 ? fa.u0 : lane == 1 ? fa.u1 : fa.u2;
+      const double ud = fa.ud ? fa.ud[3 * n + lane] : lane == 0 ? fa.u0 : lane == 1 ? fa.u1 : fa.u2;
       const double F = 2 * rho * (ud - acc) / dt;
       fa.Fb[3 * n + lane] = F;
       fa.Fds[3 * n + lane] = F * fa.ds[n];
+      if (fa.fsum) fa.fsum[3 * n + lane] = fa.first ? -F : fa.fsum[3 * n + lane] - F;
       if (lane < 2) {
         const double d = -F * (lane == 0 ? dx : dy) * fa.ds[n];
         fa.dsum[2 * n + lane] = fa.first ? d : fa.dsum[2 * n + lane] + d;
@@ -473,6 +480,43 @@ __global__ __launch_bounds__(FIN_THREADS) void k_ibm_finish(int nm, const double
   }
 }
 
+// Density point of a thin body's marker: the box cell holding the marker
+// (nearest lattice point), -1 outside the box.
+__global__ void k_ibm_near_cell(Box B, int nm, const double *__restrict__ mx, const double *__restrict__ my,
+                                const double *__restrict__ mz, int32_t *__restrict__ near) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nm) return;
+  const int i = (int)floor(mx[n] + 0.5), j = (int)floor(my[n] + 0.5), k = (int)floor(mz[n] + 0.5);
+  const bool in = i >= B.x0 && i <= B.x1 && j >= B.y0 && j <= B.y1 && k >= B.z0 && k <= B.z1;
+  near[n] = in ? (i - B.x0) + B.nx * ((j - B.y0) + B.ny * (k - B.z0)) : -1;
+}
+
+// Markers of a coupled filament: its node positions and, as desired
+// velocities, the displacement over the last step (dt = 1).
+__global__ void k_fsi_markers(int nm, const double *__restrict__ x, const double *__restrict__ xp,
+                              double *__restrict__ mx, double *__restrict__ my, double *__restrict__ mz,
+                              double *__restrict__ ud) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nm) return;
+  mx[n] = x[3 * n + 0];
+  my[n] = x[3 * n + 1];
+  mz[n] = x[3 * n + 2];
+  for (int d = 0; d < 3; d++) ud[3 * n + d] = x[3 * n + d] - xp[3 * n + d];
+}
+
+// Per coupled step: the total force on the filament, sum over nodes of
+// (-sum F_b) ds, in node order, and the free end's position.
+__global__ __launch_bounds__(64) void k_fsi_out(int nm, const double *__restrict__ fsum, const double *__restrict__ ds,
+                                                const double *__restrict__ x, double *__restrict__ out) {
+  if (threadIdx.x < 3) {
+    const int d = threadIdx.x;
+    double s = 0;
+    for (int n = 0; n < nm; n++) s += fsum[3 * n + d] * ds[n];
+    out[d] = s;
+    out[3 + d] = x[3 * (nm - 1) + d];
+  }
+}
+
 __global__ void k_fill_i32(int32_t *p, int n, int32_t v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -513,6 +557,11 @@ struct lbm_ibm {
   int out_cap = 0;
   int ready = 0;          // IBLBM done, the next collision uses force mode 3
   std::vector<void *> allocs;  // marker arrays, freed with the state
+  // coupled filament (lbm_fsi_setup): its nodes are the markers, with their
+  // own desired velocities and the density of the cell they sit in
+  lbm_filament *fil = nullptr;
+  int near_cell = 0;
+  double *ud = nullptr, *fsum = nullptr;
 };
 
 static void *dalloc(lbm_ibm *m, size_t bytes) {
@@ -697,7 +746,10 @@ static int ibm_geometry(lbm_ctx *c) {
   const Box &B = m->B;
   const int nb = B.ncell(), np = m->nm * 64;
   hipLaunchKernelGGL(k_ibm_weights, blocks(m->nm), dim3(256), 0, c->stream, m->nm, m->mx, m->my, m->mz, m->st);
-  hipLaunchKernelGGL(k_ibm_nearest, dim3(m->nm), dim3(256), 0, c->stream, B, m->mx, m->my, m->mz, m->nx, m->ny, m->nz, m->near);
+  if (m->near_cell)
+    hipLaunchKernelGGL(k_ibm_near_cell, blocks(m->nm), dim3(256), 0, c->stream, B, m->nm, m->mx, m->my, m->mz, m->near);
+  else
+    hipLaunchKernelGGL(k_ibm_nearest, dim3(m->nm), dim3(256), 0, c->stream, B, m->mx, m->my, m->mz, m->nx, m->ny, m->nz, m->near);
   hipLaunchKernelGGL(k_ibm_inside, blocks(nb), dim3(256), 0, c->stream, B, m->nm, m->mx, m->my, m->mz, m->nx, m->ny, m->nz, m->inside);
   hipLaunchKernelGGL(k_fill_i32, blocks(nb), dim3(256), 0, c->stream, m->cnt, nb, 0);
   hipLaunchKernelGGL(k_fill_i32, blocks(nb), dim3(256), 0, c->stream, m->cursor, nb, 0);
@@ -770,7 +822,7 @@ static int ibm_iterate(lbm_ctx *c, const double vcm[3], const double vb[3], doub
   lbm_ibm *m = c->ibm;
   const Box &B = m->B;
   const BoxArrays o = box_arrays(m);
-  ForceArgs fa{m->near, m->ds, m->Fb, m->Fds, m->dsum, vcm[0], vcm[1], vcm[2], m->m_max > 0, 1};
+  ForceArgs fa{m->near, m->ds, m->Fb, m->Fds, m->dsum, vcm[0], vcm[1], vcm[2], m->m_max > 0, 1, m->ud, m->fsum};
   const dim3 gi((unsigned)((m->nm + 3) / 4));
   hipLaunchKernelGGL(k_ibm_interp, gi, dim3(256), 0, c->stream, m->nm, m->pidx, m->st, o, m->ub, fa);
   fa.first = 0;
@@ -782,8 +834,9 @@ static int ibm_iterate(lbm_ctx *c, const double vcm[3], const double vb[3], doub
     hipLaunchKernelGGL(k_ibm_interp, gi, dim3(256), 0, c->stream, m->nm, m->pidx, m->st, o, m->ub, fa);
   }
   const int npart = (B.ncell() + 255) / 256;
-  hipLaunchKernelGGL(k_ibm_finish, dim3(1), dim3(FIN_THREADS), 0, c->stream, m->nm, m->dsum, npart, m->part, m->ninside,
-                     m->vsolid, vcm[0] - vb[0], vcm[1] - vb[1], out);
+  if (out)
+    hipLaunchKernelGGL(k_ibm_finish, dim3(1), dim3(FIN_THREADS), 0, c->stream, m->nm, m->dsum, npart, m->part,
+                       m->ninside, m->vsolid, vcm[0] - vb[0], vcm[1] - vb[1], out);
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }
@@ -978,6 +1031,100 @@ int lbm_ibm_run(lbm_ctx *c, int32_t nsteps, const double *vcm, double *out) {
   if (out) {
     HIPCHK(hipMemcpyAsync(out, m->out, 3 * sizeof(double) * nsteps, hipMemcpyDeviceToHost, c->stream));
   }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return LBM_OK;
+}
+
+// ---- a flexible filament coupled through the IBM ---------------------------
+
+int lbm_fsi_setup(lbm_ctx *c, lbm_filament *f, const lbm_ibm_desc *d) {
+  if (!c || !f || !d) return LBM_ERR_ARG;
+  if (c->state == ST_NONE) return LBM_ERR_STATE;
+  if (c->thermal) return LBM_ERR_UNSUPPORTED;
+  if (d->m_max < 0 || d->follow) return LBM_ERR_ARG;
+  double *fx, *fxp, fds;
+  int n, fdev;
+  filament_state(f, &fx, &fxp, &n, &fds, &fdev);
+  if (fdev != c->dev) return LBM_ERR_ARG;
+  HIPCHK(hipSetDevice(c->dev));
+  std::vector<double> h(3 * (size_t)n), x(n), y(n), z(n), zero(n, 0.0), ds(n, fds);
+  HIPCHK(hipDeviceSynchronize());
+  HIPCHK(hipMemcpy(h.data(), fx, 24 * (size_t)n, hipMemcpyDeviceToHost));
+  for (int i = 0; i < n; i++) {
+    x[i] = h[3 * i];
+    y[i] = h[3 * i + 1];
+    z[i] = h[3 * i + 2];
+  }
+  ds[0] = ds[n - 1] = fds / 2;  // trapezoid weights of the node spacing
+  lbm_ibm_desc dd = *d;
+  dd.nmarkers = n;
+  dd.moving = 1;
+  dd.vsolid = 0;
+  int e = lbm_ibm_setup(c, &dd, x.data(), y.data(), z.data(), zero.data(), zero.data(), zero.data(), ds.data());
+  if (e != LBM_OK) return e;
+  lbm_ibm *m = c->ibm;
+  m->fil = f;
+  m->near_cell = 1;
+  m->ud = (double *)dalloc(m, 24 * (size_t)n);
+  m->fsum = (double *)dalloc(m, 24 * (size_t)n);
+  if (!m->ud || !m->fsum) return LBM_ERR_NOMEM;
+  HIPCHK(hipMemsetAsync(m->fsum, 0, 24 * (size_t)n, c->stream));
+  hipLaunchKernelGGL(k_fsi_markers, blocks(n), dim3(256), 0, c->stream, n, fx, fxp, m->mx, m->my, m->mz, m->ud);
+  HIPCHK(hipGetLastError());
+  return ibm_geometry(c);
+}
+
+// nsteps coupled steps: markers from the filament, the IBM step with their
+// velocities, the lattice step (force mode 3), the filament step under the
+// fluid's force -sum F_b per unit length.  out: nsteps x 6 (total force on
+// the filament, free-end position).
+int lbm_fsi_run(lbm_ctx *c, int32_t nsteps, double *out) {
+  if (!c || nsteps < 0) return LBM_ERR_ARG;
+  if (!c->ibm || !c->ibm->fil) return LBM_ERR_STATE;
+  if (c->state == ST_NONE) return LBM_ERR_STATE;
+  if (nsteps == 0) return LBM_OK;
+  HIPCHK(hipSetDevice(c->dev));
+  lbm_ibm *m = c->ibm;
+  double *fx, *fxp, fds;
+  int n, fdev;
+  filament_state(m->fil, &fx, &fxp, &n, &fds, &fdev);
+  if (m->out_cap < 2 * nsteps) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    dfree(m->out);
+    HIPCHK(hipMalloc((void **)&m->out, 3 * sizeof(double) * 2 * nsteps));
+    m->out_cap = 2 * nsteps;
+  }
+  const double zero[3] = {0, 0, 0};
+  for (int s = 0; s < nsteps; s++) {
+    hipLaunchKernelGGL(k_fsi_markers, blocks(n), dim3(256), 0, c->stream, n, fx, fxp, m->mx, m->my, m->mz, m->ud);
+    HIPCHK(hipGetLastError());
+    int e = ibm_geometry(c);
+    if (e != LBM_OK) return e;
+    LaunchCtx L = launch_ctx(c);
+    L.force_mode = 3;
+    if (c->state == ST_F) {
+      HIPCHK(box_moments(c, false));
+      if ((e = ibm_iterate(c, zero, zero, nullptr)) != LBM_OK) return e;
+      HIPCHK(launch_collide(L, c->cur));
+    } else {
+      HIPCHK(box_moments(c, true));
+      if ((e = ibm_iterate(c, zero, zero, nullptr)) != LBM_OK) return e;
+      HIPCHK(launch_stream(L, c->cur, c->oth, s == nsteps - 1 ? STREAM_COLLIDE_MACRO : STREAM_COLLIDE));
+      std::swap(c->cur, c->oth);
+      if ((e = after_stream(c)) != LBM_OK) return e;
+    }
+    c->state = ST_FSTAR;
+    if ((e = filament_step_dev(m->fil, m->fsum, 1.0, 1, c->stream)) != LBM_OK) return e;
+    hipLaunchKernelGGL(k_fsi_out, dim3(1), dim3(64), 0, c->stream, n, m->fsum, m->ds, fx, m->out + 6 * s);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(launch_stream(launch_ctx(c), c->cur, c->oth, STREAM_ONLY));
+  std::swap(c->cur, c->oth);
+  c->state = ST_F;
+  int e = after_stream(c);
+  if (e != LBM_OK) return e;
+  m->ready = 0;
+  if (out) HIPCHK(hipMemcpyAsync(out, m->out, 6 * sizeof(double) * nsteps, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return LBM_OK;
 }

@@ -173,6 +173,8 @@ _SIGNATURES = {
     "lbm_batch_nfluid": (C.c_int64, [_P]),
     "lbm_batch_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _D]),
     "lbm_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32)]),
+    "lbm_fsi_setup": (C.c_int, [_P, _P, C.POINTER(_IbmDesc)]),
+    "lbm_fsi_run": (C.c_int, [_P, C.c_int32, _D]),
     "lbm_filament_create": (C.c_int, [C.c_int32, C.POINTER(_FilamentDesc), C.POINTER(_P)]),
     "lbm_filament_destroy": (C.c_int, [_P]),
     "lbm_filament_substeps": (C.c_int, [_P]),
@@ -437,6 +439,19 @@ class Lattice:
 
     def output_vtk_end(self, path):
         _check(self._lib.lbm_output_vtk_end(self._h, os.fsencode(path)), "lbm_output_vtk_end")
+
+    # -- a flexible filament coupled through the IBM ----------------------------
+    def fsi_setup(self, filament: "Filament", box, m_max: int = 20):
+        d = _IbmDesc(0, int(m_max), (C.c_int32 * 6)(*[int(b) for b in box]), 1, 0, 0.0,
+                     (C.c_double * 2)(0.0, 0.0), 0.0)
+        _check(self._lib.lbm_fsi_setup(self._h, filament._h, C.byref(d)), "lbm_fsi_setup")
+        self._filament = filament  # the coupling reads it every step
+
+    def fsi_run(self, nsteps: int):
+        """(nsteps, 6): total force on the filament and its free end, per step."""
+        out = np.zeros((int(nsteps), 6))
+        _check(self._lib.lbm_fsi_run(self._h, int(nsteps), _dp(out)), "lbm_fsi_run")
+        return out
 
     # -- immersed boundary (IBLBM, lbm.cpp:1135-1417) ---------------------------
     def ibm_setup(self, markers: dict, box, m_max: int = 20, vsolid: float = 0.0, moving=False,

@@ -1061,6 +1061,76 @@ int orc_ibm_step(orc_lbm *lb, int nm, double *mx, double *my, double *mz,
   return err;
 }
 
+/* The IBM step of a coupled filament (lbm_oracle.h): orc_ibm_step's
+ * unforced moments, interpolation and m_max force/spread/interpolate passes
+ * with the markers where they are, u_des per marker, rho at the box cell
+ * holding the marker, no added mass; fsum = -sum of the passes' F_b. */
+int orc_fsi_ibm_step(orc_lbm *lb, int nm, const double *mx, const double *my, const double *mz,
+                     const double *udes, const double *ds, double *ub, double *Fb, double *fsum,
+                     int xmin, int xmax, int ymin, int ymax, int zmin, int zmax, int m_max) {
+  const int Nx = lb->Nx, Ny = lb->Ny, Nz = lb->Nz, q = lb->q;
+  int err = ORC_OK;
+#pragma omp parallel for
+  for (int i = 0; i <= Nx + 1; i++)
+    for (int j = 0; j <= Ny + 1; j++)
+      for (int k = 0; k <= Nz + 1; k++) {
+        orc_cell *c = &lb->fluid[i][j][k];
+        if (!ibm_cell(c->type)) continue;
+        double rho = 0, rho_u[3] = {0, 0, 0};
+        for (int l = 0; l < q; l++) {
+          rho += c->f[l];
+          rho_u[0] += c->f[l] * lb->cx[l];
+          rho_u[1] += c->f[l] * lb->cy[l];
+          rho_u[2] += c->f[l] * lb->cz[l];
+        }
+        c->density = rho;
+        c->ux = rho_u[0] / rho;
+        c->uy = rho_u[1] / rho;
+        c->uz = rho_u[2] / rho;
+        c->F[0] = 0; c->F[1] = 0; c->F[2] = 0;
+      }
+  ibm_interpolate(lb, nm, mx, my, mz, ub, xmin, xmax, ymin, ymax, zmin, zmax);
+  for (int n = 0; n < 3 * nm; n++) fsum[n] = 0;
+  for (int m = 1; m <= m_max; m++) {
+    for (int n = 0; n < nm; n++) {
+      const int i = (int)floor(mx[n] + 0.5), j = (int)floor(my[n] + 0.5), k = (int)floor(mz[n] + 0.5);
+      double rho = NAN;
+      if (i >= xmin && i <= xmax && j >= ymin && j <= ymax && k >= zmin && k <= zmax)
+        rho = lb->fluid[i][j][k].density;
+      else
+        err = -1;
+      for (int d = 0; d < 3; d++) {
+        Fb[3 * n + d] = 2 * rho * (udes[3 * n + d] - ub[3 * n + d]) / ORC_DT;
+        fsum[3 * n + d] = (m == 1) ? -Fb[3 * n + d] : fsum[3 * n + d] - Fb[3 * n + d];
+      }
+    }
+#pragma omp parallel for
+    for (int i = xmin; i <= xmax; i++)
+      for (int j = ymin; j <= ymax; j++)
+        for (int k = zmin; k <= zmax; k++) {
+          orc_cell *c = &lb->fluid[i][j][k];
+          if (!ibm_cell(c->type)) continue;
+          double *Fm = &lb->Fm[3 * LIN(lb, i, j, k)];
+          Fm[0] = 0; Fm[1] = 0; Fm[2] = 0;
+          for (int n = 0; n < nm; n++) {
+            double rx = (i * ORC_DX - mx[n]) / ORC_DX, ry = (j * ORC_DY - my[n]) / ORC_DY,
+                   rz = (k * ORC_DZ - mz[n]) / ORC_DZ;
+            Fm[0] += Fb[3 * n + 0] * ds[n] * orc_ibm_kernel(rx) * orc_ibm_kernel(ry) * orc_ibm_kernel(rz);
+            Fm[1] += Fb[3 * n + 1] * ds[n] * orc_ibm_kernel(rx) * orc_ibm_kernel(ry) * orc_ibm_kernel(rz);
+            Fm[2] += Fb[3 * n + 2] * ds[n] * orc_ibm_kernel(rx) * orc_ibm_kernel(ry) * orc_ibm_kernel(rz);
+          }
+          c->F[0] += -Fm[0];
+          c->F[1] += -Fm[1];
+          c->F[2] += -Fm[2];
+          c->ux += ORC_DT * Fm[0] / (2 * c->density);
+          c->uy += ORC_DT * Fm[1] / (2 * c->density);
+          c->uz += ORC_DT * Fm[2] / (2 * c->density);
+        }
+    ibm_interpolate(lb, nm, mx, my, mz, ub, xmin, xmax, ymin, ymax, zmin, zmax);
+  }
+  return err;
+}
+
 /* ---------------------------------------------------------------------------
  * Flexible filament (specification in lbm_oracle.h).
  * ------------------------------------------------------------------------- */

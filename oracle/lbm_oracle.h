@@ -124,6 +124,15 @@ int orc_ibm_step(orc_lbm *lb, int nm, double *mx, double *my, double *mz,
                  const double vcm[3], const double vcm_before[3],
                  int xmin, int xmax, int ymin, int ymax, int zmin, int zmax,
                  int m_max, double vsolid, double *out);
+/* The IBM step of a coupled filament (no reference counterpart): as
+ * orc_ibm_step with the markers where they are (no rigid motion), the
+ * desired velocity udes per marker (3 nm), rho at the box cell holding the
+ * marker (floor(x + 1/2) per axis), no added mass; fsum (3 nm) receives
+ * minus the sum over the passes of F_b, the force per unit length on the
+ * filament. */
+int orc_fsi_ibm_step(orc_lbm *lb, int nm, const double *mx, const double *my, const double *mz,
+                     const double *udes, const double *ds, double *ub, double *Fb, double *fsum,
+                     int xmin, int xmax, int ymin, int ymax, int zmin, int zmax, int m_max);
 /* main.cpp:195-211: IBLBM -> Collision -> Streaming (no MacroProp) */
 void orc_get_fm(const orc_lbm *lb, double *Fmx, double *Fmy, double *Fmz);
 /* NearestLatticeCoordinate, lbm.cpp:1074-1092 */

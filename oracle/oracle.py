@@ -57,6 +57,7 @@ _SIG = {
     "orc_q7_set": (None, [_P, _D, C.POINTER(C.c_int16)]),
     "orc_q7_get": (None, [_P, _D, _D, _D, _D]),
     "orc_q7_get_g": (None, [_P, _D]),
+    "orc_fsi_ibm_step": (C.c_int, [_P, C.c_int, _D, _D, _D, _D, _D, _D, _D, _D] + [C.c_int] * 7),
     "orc_fil_create": (_P, [C.c_int] + [C.c_double] * 4 + [_D, _D, _D, C.c_int]),
     "orc_fil_destroy": (None, [_P]),
     "orc_fil_substeps": (C.c_int, [_P]),
@@ -225,6 +226,20 @@ class Oracle:
         a = np.empty(self.shape + (7,))
         self.lib.orc_q7_get_g(self.h, _dp(a))
         return a
+
+    def fsi_ibm_step(self, x, udes, ds, box, m_max):
+        """IBM step of a coupled filament: x (n,3) marker positions, udes
+        (n,3), ds (n,); returns (ub, Fb, fsum), each (n,3)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        n = x.shape[0]
+        mx, my, mz = (np.ascontiguousarray(x[:, d]) for d in range(3))
+        ud = np.ascontiguousarray(udes, dtype=np.float64).reshape(n, 3)
+        dsa = np.ascontiguousarray(ds, dtype=np.float64)
+        ub, Fb, fsum = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+        err = self.lib.orc_fsi_ibm_step(self.h, n, _dp(mx), _dp(my), _dp(mz), _dp(ud), _dp(dsa), _dp(ub), _dp(Fb),
+                                        _dp(fsum), *[int(b) for b in box], int(m_max))
+        assert err == 0, "a marker left the box"
+        return ub, Fb, fsum
 
     def wall_temperature(self, i, j, k):
         return self.lib.orc_wall_temperature(self.h, i, j, k)
