@@ -495,6 +495,63 @@ def ibm_cylinder(Nx: int = 60, Ny: int = 40, Nz: int = 1, D: float = 10.0, u_max
     return case, mk, box
 
 
+def fsi_thermal_channel(Nx: int = 192, Ny: int = 96, Nz: int = 96, u_in: float = 0.03, Re: float = 100.0,
+                        Pr: float = 0.71, Ra: float = 1e4, n_markers: int = 128, length: float | None = None,
+                        density_ratio: float = 1.5, K_b: float = 1e-3, incline_deg: float = 5.0,
+                        seed: int = 1, perturb: float = 0.01):
+    """A flexible filament clamped at its leading end in a heated channel:
+    i = 0 TYPE_IN, i = Nx+1 TYPE_OUT, y and z walls TYPE_S (channel()'s
+    pattern); every wall Dirichlet, j = 0 at T = 1, the others at T = 0,
+    the fluid and the inflow at T = 0; Boussinesq F_y = g_beta T (T_ref =
+    0) with g_beta = Ra nu alpha / H^3 over the height H = Ny.
+
+    The filament (length L, default Ny/2) is clamped at (Nx/4, Ny/2, Nz/2),
+    inclined by incline_deg in x-y, and follows the usual flapping-filament
+    scales: m_l = density_ratio rho0 L, EI = K_b rho0 u_in^2 L^3,
+    Re = u_in L / nu; Pr = nu / alpha (D3Q7: alpha = (tau_g - 1/2)/4).  The
+    IBM box spans the filament's reach (L/2 + 4 across the flow, 4 cells
+    about the clamp plane in z).  Returns (case, thermal, filament keyword
+    arguments, box)."""
+    t, rho, ux, uy, uz = _blank(Nx, Ny, Nz)
+    t[0, :, :] = TYPE_IN
+    t[-1, :, :] = TYPE_OUT
+    t[:, 0, :] = TYPE_S
+    t[:, -1, :] = TYPE_S
+    t[:, :, 0] = TYPE_S
+    t[:, :, -1] = TYPE_S
+    live = t != TYPE_S
+    ux[live] = u_in
+    if perturb:
+        cell = np.arange(t.size, dtype=np.uint64).reshape(t.shape)
+        xi = [counter_uniform(seed, 3 * cell + np.uint64(c)) for c in range(3)]
+        f = t == TYPE_F
+        ux[f] = u_in * (1.0 + perturb * xi[0][f])
+        uy[f] = perturb * u_in * xi[1][f]
+        uz[f] = perturb * u_in * xi[2][f]
+    L = float(length) if length is not None else Ny / 2.0
+    nu = u_in * L / Re
+    tau = 0.5 + 3.0 * nu
+    alpha = nu / Pr
+    g_beta = Ra * nu * alpha / float(Ny) ** 3
+    T = np.zeros(t.shape)
+    T[:, 0, :] = 1.0
+    te = np.zeros(t.shape, np.int16)
+    te[t == TYPE_S] = TYPE_T
+    th = q7_thermal(T, te, q7_tau(alpha), boussinesq=True, g_beta=g_beta, T_ref=0.0, alpha=alpha, H=Ny)
+    a = math.radians(incline_deg)
+    clamp = (Nx / 4.0, Ny / 2.0, Nz / 2.0)
+    fil = dict(n=n_markers, length=L, line_density=density_ratio * L, bending=K_b * u_in ** 2 * L ** 3,
+               clamp=clamp, tangent=(math.cos(a), math.sin(a), 0.0))
+    reach = L / 2.0 + 4.0
+    box = (max(1, int(math.floor(clamp[0] - 4))), min(Nx, int(math.ceil(clamp[0] + L + 4))),
+           max(1, int(math.floor(clamp[1] - reach))), min(Ny, int(math.ceil(clamp[1] + reach))),
+           max(1, int(math.floor(clamp[2] - 4))), min(Nz, int(math.ceil(clamp[2] + 4))))
+    case = Case("fsi_thermal_channel", Nx, Ny, Nz, tau, t, rho, ux, uy, uz,
+                params=dict(u_in=u_in, Re=Re, Pr=Pr, Ra=Ra, nu=nu, alpha=alpha, L=L, K_b=K_b,
+                            density_ratio=density_ratio))
+    return case, th, fil, box
+
+
 def batch_cavities(count: int = 96, seed: int = 1, edges=(32, 48, 64, 96, 128),
                    u_range=(0.02, 0.08), re_range=(50.0, 1000.0)):
     """The seeded cavity ensemble: edge from `edges`, lid velocity and Re

@@ -377,7 +377,7 @@ int lbm_collide(lbm_ctx *c) {
   if (c->thermal == 1) {
     if (int e = thermal_launch(c, 3)) return e;
   } else if (c->thermal == 2) {
-    if (int e = q7_launch(c, Q7_COLLIDE)) return e;
+    if (int e = q7_launch(c, Q7_COLLIDE, L.force_mode)) return e;
   } else {
     HIPCHK(launch_collide(L, c->cur));
   }

@@ -73,7 +73,8 @@ enum Q7Mode {
 int q7_upload(lbm_ctx *c, const lbm_thermal_desc *d, const double *T, const int16_t *type_energy);
 int q7_init(lbm_ctx *c);    // Dirichlet link bits, g = geq
 int q7_tables(lbm_ctx *c);  // Dirichlet link bits (after every k_tables)
-int q7_launch(lbm_ctx *c, int mode);
+// force_mode: the context's unless given (3: the IBM step)
+int q7_launch(lbm_ctx *c, int mode, int force_mode = -1);
 int q7_download_g(lbm_ctx *c, double *g_dev);
 
 // VTK output staging (lbm_output.hip)
@@ -106,7 +107,8 @@ enum Q7Mode {
 int q7_upload(lbm_ctx *c, const lbm_thermal_desc *d, const double *T, const int16_t *type_energy);
 int q7_init(lbm_ctx *c);    // Dirichlet link bits, g = geq
 int q7_tables(lbm_ctx *c);  // Dirichlet link bits (after every k_tables)
-int q7_launch(lbm_ctx *c, int mode);
+// force_mode: the context's unless given (3: the IBM step)
+int q7_launch(lbm_ctx *c, int mode, int force_mode = -1);
 int q7_download_g(lbm_ctx *c, double *g_dev);
 
 // VTK output staging (lbm_output.hip)

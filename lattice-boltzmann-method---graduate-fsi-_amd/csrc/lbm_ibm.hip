@@ -934,7 +934,9 @@ int lbm_ibm_setup(lbm_ctx *c, const lbm_ibm_desc *d, const double *x, const doub
                   const double *ds) {
   if (!c || !d || !x || !y || !z || !nx || !ny || !nz || !ds) return LBM_ERR_ARG;
   if (c->state == ST_NONE) return LBM_ERR_STATE;
-  if (c->thermal) return LBM_ERR_UNSUPPORTED;  // the reference's IBM cases have no ENERGY
+  // the reference's IBM cases have no ENERGY; the D3Q7 branch couples (its
+  // buoyancy added to the IBM force)
+  if (c->thermal == 1) return LBM_ERR_UNSUPPORTED;
   if (d->nmarkers < 1 || d->m_max < 0) return LBM_ERR_ARG;
   if (d->follow && !(d->half >= 0)) return LBM_ERR_ARG;
   const int e = ibm_setup(c, d, x, y, z, nx, ny, nz, ds);
@@ -977,6 +979,22 @@ int lbm_ibm_apply(lbm_ctx *c, const double vcm[3], const double vcm_before[3], d
   return LBM_OK;
 }
 
+// The lattice launches of an IBM step: force mode 3, the D3Q7 kernels when
+// the context carries D3Q7 temperature.
+static int ibm_collide(lbm_ctx *c, LaunchCtx &L) {
+  if (c->thermal == 2) return q7_launch(c, Q7_COLLIDE, 3);
+  HIPCHK(launch_collide(L, c->cur));
+  return LBM_OK;
+}
+static int ibm_stream(lbm_ctx *c, LaunchCtx &L, int mode) {
+  if (c->thermal == 2) {
+    const int tm = mode == STREAM_COLLIDE ? Q7_FUSED : mode == STREAM_COLLIDE_MACRO ? Q7_FUSED_EMIT : Q7_STREAM;
+    return q7_launch(c, tm, 3);
+  }
+  HIPCHK(launch_stream(L, c->cur, c->oth, mode));
+  return LBM_OK;
+}
+
 // nsteps of main.cpp:199-205 (IBLBM, Collision, Streaming).  vcm holds the
 // centre-of-mass velocity before step 1 and after each step
 // ((nsteps+1) x 3 values) or is NULL for a body at rest; out receives
@@ -1007,14 +1025,14 @@ int lbm_ibm_run(lbm_ctx *c, int32_t nsteps, const double *vcm, double *out) {
       HIPCHK(box_moments(c, false));
       e = ibm_iterate(c, vn, vb, m->out + 3 * s);
       if (e != LBM_OK) return e;
-      HIPCHK(launch_collide(L, c->cur));
+      if ((e = ibm_collide(c, L)) != LBM_OK) return e;
     } else {
       // f* is in the buffer: the box pulls f, the fused launch streams,
       // collides with the IBM result and stores the next f*
       HIPCHK(box_moments(c, true));
       e = ibm_iterate(c, vn, vb, m->out + 3 * s);
       if (e != LBM_OK) return e;
-      HIPCHK(launch_stream(L, c->cur, c->oth, s == nsteps - 1 ? STREAM_COLLIDE_MACRO : STREAM_COLLIDE));
+      if ((e = ibm_stream(c, L, s == nsteps - 1 ? STREAM_COLLIDE_MACRO : STREAM_COLLIDE)) != LBM_OK) return e;
       std::swap(c->cur, c->oth);
       e = after_stream(c);
       if (e != LBM_OK) return e;
@@ -1022,7 +1040,10 @@ int lbm_ibm_run(lbm_ctx *c, int32_t nsteps, const double *vcm, double *out) {
     c->state = ST_FSTAR;
   }
   // the last Streaming of the run
-  HIPCHK(launch_stream(launch_ctx(c), c->cur, c->oth, STREAM_ONLY));
+  {
+    LaunchCtx L = launch_ctx(c);
+    if (int e = ibm_stream(c, L, STREAM_ONLY)) return e;
+  }
   std::swap(c->cur, c->oth);
   c->state = ST_F;
   int e = after_stream(c);
@@ -1040,7 +1061,7 @@ int lbm_ibm_run(lbm_ctx *c, int32_t nsteps, const double *vcm, double *out) {
 int lbm_fsi_setup(lbm_ctx *c, lbm_filament *f, const lbm_ibm_desc *d) {
   if (!c || !f || !d) return LBM_ERR_ARG;
   if (c->state == ST_NONE) return LBM_ERR_STATE;
-  if (c->thermal) return LBM_ERR_UNSUPPORTED;
+  if (c->thermal == 1) return LBM_ERR_UNSUPPORTED;
   if (d->m_max < 0 || d->follow) return LBM_ERR_ARG;
   double *fx, *fxp, fds;
   int n, fdev;
@@ -1105,11 +1126,11 @@ int lbm_fsi_run(lbm_ctx *c, int32_t nsteps, double *out) {
     if (c->state == ST_F) {
       HIPCHK(box_moments(c, false));
       if ((e = ibm_iterate(c, zero, zero, nullptr)) != LBM_OK) return e;
-      HIPCHK(launch_collide(L, c->cur));
+      if ((e = ibm_collide(c, L)) != LBM_OK) return e;
     } else {
       HIPCHK(box_moments(c, true));
       if ((e = ibm_iterate(c, zero, zero, nullptr)) != LBM_OK) return e;
-      HIPCHK(launch_stream(L, c->cur, c->oth, s == nsteps - 1 ? STREAM_COLLIDE_MACRO : STREAM_COLLIDE));
+      if ((e = ibm_stream(c, L, s == nsteps - 1 ? STREAM_COLLIDE_MACRO : STREAM_COLLIDE)) != LBM_OK) return e;
       std::swap(c->cur, c->oth);
       if ((e = after_stream(c)) != LBM_OK) return e;
     }
@@ -1118,7 +1139,10 @@ int lbm_fsi_run(lbm_ctx *c, int32_t nsteps, double *out) {
     hipLaunchKernelGGL(k_fsi_out, dim3(1), dim3(64), 0, c->stream, n, m->fsum, m->ds, fx, m->out + 6 * s);
     HIPCHK(hipGetLastError());
   }
-  HIPCHK(launch_stream(launch_ctx(c), c->cur, c->oth, STREAM_ONLY));
+  {
+    LaunchCtx L = launch_ctx(c);
+    if (int e = ibm_stream(c, L, STREAM_ONLY)) return e;
+  }
   std::swap(c->cur, c->oth);
   c->state = ST_F;
   int e = after_stream(c);

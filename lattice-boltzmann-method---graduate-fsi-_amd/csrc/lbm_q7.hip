@@ -145,13 +145,26 @@ __global__ __launch_bounds__(256) void k_q7(const Planes<S> P, const G7<S> G, co
     if (mk & M_BACKEQ) patch_cell<S, true>(P, (uint8_t)mk, idx, a, f);
     const C dT = g_sum(g);
     const C T = t.T0 + dT;
-    C rho, drho, jx, jy, jz, Fx, Fy, Fz, ux, uy, uz;
-    moments<S>(f, rho, drho, jx, jy, jz);
-    q7_force<C, FORCE, BQ>(a, t, idx, T, Fx, Fy, Fz);
-    if constexpr (FORCE != 0 || BQ) velocity(jx + C(0.5) * Fx, jy + C(0.5) * Fy, jz + C(0.5) * Fz, rho, ux, uy, uz);
-    else velocity(jx, jy, jz, rho, ux, uy, uz);
-    int bad = (rho < C(0) || rho > C(100)) ? ST_RHO_RANGE : 0;
-    if (T < C(0) || T > C(100)) bad |= ST_T_RANGE;
+    C rho, drho, Fx, Fy, Fz, ux, uy, uz;
+    int bad = 0;
+    if constexpr (FORCE == 3) {
+      // the IBM step (no MacroProp): the box's rho, u, F, unforced moments
+      // elsewhere; the buoyancy of this cell's T added on top
+      cell_state<S, 3>(f, (uint8_t)mk, idx, a, rho, drho, ux, uy, uz, Fx, Fy, Fz);
+      if constexpr (BQ) {
+        const C b = t.g_beta * (T - t.T_ref);
+        Fy = Fy + b;
+        uy = uy + C(0.5) * b / rho;
+      }
+    } else {
+      C jx, jy, jz;
+      moments<S>(f, rho, drho, jx, jy, jz);
+      q7_force<C, FORCE, BQ>(a, t, idx, T, Fx, Fy, Fz);
+      if constexpr (FORCE != 0 || BQ) velocity(jx + C(0.5) * Fx, jy + C(0.5) * Fy, jz + C(0.5) * Fz, rho, ux, uy, uz);
+      else velocity(jx, jy, jz, rho, ux, uy, uz);
+      bad = (rho < C(0) || rho > C(100)) ? ST_RHO_RANGE : 0;
+      if (T < C(0) || T > C(100)) bad |= ST_T_RANGE;
+    }
     if constexpr (EMIT) {
       a.rho[idx] = rho;
       a.ux[idx] = ux;
@@ -218,10 +231,34 @@ __global__ __launch_bounds__(256) void k_q7_collide(S *__restrict__ buf, const u
   C f[Q], g[Q7];
   for (int m = 0; m < Q; m++) f[m] = Store<S>::ld(buf + (uint32_t)m * ps + idx);
   for (int m = 0; m < Q7; m++) g[m] = Store<S>::ld(buf + (uint32_t)(Q + m) * ps + idx);
-  const C rho = a.rho[idx], drho = rho - C(1), ux = a.ux[idx], uy = a.uy[idx], uz = a.uz[idx];
-  const C T = t.T[idx];
-  C Fx, Fy, Fz;
-  q7_force<C, FORCE, BQ>(a, t, idx, T, Fx, Fy, Fz);
+  C rho, drho, ux, uy, uz, T, Fx, Fy, Fz;
+  if constexpr (FORCE == 3) {
+    // the IBM step: as k_collide's mode 3, with T from the cell's own g
+    // and its buoyancy added; rho, u, T, q are emitted
+    cell_state<S, 3>(f, mk, idx, a, rho, drho, ux, uy, uz, Fx, Fy, Fz);
+    T = t.T0 + g_sum(g);
+    if constexpr (BQ) {
+      const C b = t.g_beta * (T - t.T_ref);
+      Fy = Fy + b;
+      uy = uy + C(0.5) * b / rho;
+    }
+    a.rho[idx] = rho;
+    a.ux[idx] = ux;
+    a.uy[idx] = uy;
+    a.uz[idx] = uz;
+    t.T[idx] = T;
+    t.qx[idx] = g[1] - g[2];
+    t.qy[idx] = g[3] - g[4];
+    t.qz[idx] = g[5] - g[6];
+  } else {
+    rho = a.rho[idx];
+    drho = rho - C(1);
+    ux = a.ux[idx];
+    uy = a.uy[idx];
+    uz = a.uz[idx];
+    T = t.T[idx];
+    q7_force<C, FORCE, BQ>(a, t, idx, T, Fx, Fy, Fz);
+  }
   const bool neg = relax<S, (FORCE != 0 || BQ) ? 2 : 0>(f, rho, drho, ux, uy, uz, Fx, Fy, Fz, a.omega);
   g_relax(g, T, T - t.T0, ux, uy, uz, t.omega_g);
   for (int m = 0; m < Q; m++) Store<S>::st(buf + (uint32_t)m * ps + idx, f[m]);
@@ -375,6 +412,7 @@ KArgs<C> kargs7(const LaunchCtx &L) {
   a.uz = (C *)L.uz;
   a.status = L.status;
   a.first_bad = L.first_bad;
+  a.box = L.box;
   a.wall_slots = 1;
   return a;
 }
@@ -429,18 +467,19 @@ void launch_fb(lbm_ctx *c, int mode) {
 }
 
 template <class S, bool BQ>
-void launch_b(lbm_ctx *c, int mode) {
-  switch (c->force_mode) {
+void launch_b(lbm_ctx *c, int mode, int fm) {
+  switch (fm) {
     case 1: launch_fb<S, 1, BQ>(c, mode); break;
     case 2: launch_fb<S, 2, BQ>(c, mode); break;
+    case 3: launch_fb<S, 3, BQ>(c, mode); break;
     default: launch_fb<S, 0, BQ>(c, mode); break;
   }
 }
 
 template <class S>
-void launch_s7(lbm_ctx *c, int mode) {
-  if (c->td.boussinesq) launch_b<S, true>(c, mode);
-  else launch_b<S, false>(c, mode);
+void launch_s7(lbm_ctx *c, int mode, int fm) {
+  if (c->td.boussinesq) launch_b<S, true>(c, mode, fm);
+  else launch_b<S, false>(c, mode, fm);
 }
 
 int upload_field(lbm_ctx *c, void *dst, const std::vector<double> &h) {
@@ -456,9 +495,10 @@ int upload_field(lbm_ctx *c, void *dst, const std::vector<double> &h) {
 
 }  // namespace
 
-int q7_launch(lbm_ctx *c, int mode) {
-  if (c->storage == LBM_STORE_F64) launch_s7<double>(c, mode);
-  else launch_s7<float>(c, mode);
+int q7_launch(lbm_ctx *c, int mode, int force_mode) {
+  const int fm = force_mode < 0 ? c->force_mode : force_mode;
+  if (c->storage == LBM_STORE_F64) launch_s7<double>(c, mode, fm);
+  else launch_s7<float>(c, mode, fm);
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }

@@ -935,6 +935,39 @@ static void ibm_interpolate(orc_lbm *lb, int nm, const double *mx, const double 
   }
 }
 
+/* D3Q7 (lbm_oracle.h) under an IBM step, which replaces MacroProp: T and q
+ * of every fluid cell from its g, and the buoyancy of T added to the IBM's
+ * force and velocity (half of it, as MacroProp's u). */
+static void q7_ibm_buoyancy(orc_lbm *lb) {
+  orc_q7 *t = lb->q7;
+  if (!t) return;
+  const int Nx = lb->Nx, Ny = lb->Ny, Nz = lb->Nz;
+#pragma omp parallel for
+  for (int i = 0; i <= Nx + 1; i++)
+    for (int j = 0; j <= Ny + 1; j++)
+      for (int k = 0; k <= Nz + 1; k++) {
+        orc_cell *c = &lb->fluid[i][j][k];
+        if (c->type != ORC_TYPE_F) continue;
+        const int64_t n = LIN(lb, i, j, k);
+        double T = 0, qx = 0, qy = 0, qz = 0;
+        for (int l = 0; l < 7; l++) {
+          T += t->g[n * 7 + l];
+          qx += t->g[n * 7 + l] * q7_cx[l];
+          qy += t->g[n * 7 + l] * q7_cy[l];
+          qz += t->g[n * 7 + l] * q7_cz[l];
+        }
+        t->T[n] = T;
+        t->q[3 * n + 0] = qx;
+        t->q[3 * n + 1] = qy;
+        t->q[3 * n + 2] = qz;
+        if (t->boussinesq) {
+          const double b = t->g_beta * (T - t->T_ref);
+          c->F[1] += b;
+          c->uy += 0.5 * b / c->density;
+        }
+      }
+}
+
 /* IBLBM, implicit branch, lbm.cpp:1135-1167 and 1260-1417. */
 int orc_ibm_step(orc_lbm *lb, int nm, double *mx, double *my, double *mz,
                  const double *nx, const double *ny, const double *nz,
@@ -1058,6 +1091,7 @@ int orc_ibm_step(orc_lbm *lb, int nm, double *mx, double *my, double *mz,
   out[1] = Lift;
   out[2] = rho_avg;
   free(udes);
+  q7_ibm_buoyancy(lb);
   return err;
 }
 
@@ -1128,6 +1162,7 @@ int orc_fsi_ibm_step(orc_lbm *lb, int nm, const double *mx, const double *my, co
         }
     ibm_interpolate(lb, nm, mx, my, mz, ub, xmin, xmax, ymin, ymax, zmin, zmax);
   }
+  q7_ibm_buoyancy(lb);
   return err;
 }
 

@@ -117,6 +117,9 @@ double orc_ibm_kernel(double r);
  * the interior cells is multiplied by it (lbm.cpp:1392-1398).
  * out[0] = Drag, out[1] = Lift (raw sums, before the C_D/C_L normalisation),
  * out[2] = the average interior density used for added mass.
+ * With D3Q7 on (no reference counterpart) the step ends as MacroProp's D3Q7
+ * part would: T and q of every fluid cell from g, and the Boussinesq force
+ * g_beta (T - T_ref) added to F_y and half of it over rho to u_y.
  */
 int orc_ibm_step(orc_lbm *lb, int nm, double *mx, double *my, double *mz,
                  const double *nx, const double *ny, const double *nz,
@@ -129,7 +132,7 @@ int orc_ibm_step(orc_lbm *lb, int nm, double *mx, double *my, double *mz,
  * desired velocity udes per marker (3 nm), rho at the box cell holding the
  * marker (floor(x + 1/2) per axis), no added mass; fsum (3 nm) receives
  * minus the sum over the passes of F_b, the force per unit length on the
- * filament. */
+ * filament.  D3Q7 temperature enters as in orc_ibm_step. */
 int orc_fsi_ibm_step(orc_lbm *lb, int nm, const double *mx, const double *my, const double *mz,
                      const double *udes, const double *ds, double *ub, double *Fb, double *fsum,
                      int xmin, int xmax, int ymin, int ymax, int zmin, int zmax, int m_max);
