@@ -209,6 +209,78 @@ def ibm_leg(steps, cmp_steps, warmup=20):
         f32_vs_f64=cmp)
 
 
+def fsi_thermal_leg(steps, cmp_steps=500, warmup=20):
+    """The flexible filament in the heated channel (cases.fsi_thermal_channel
+    at its defaults): LBM(192,96,96) D3Q19 + D3Q7 (Pr 0.71), inflow 0.03,
+    the j = 0 wall at T = 1 and the rest at T = 0, Boussinesq at Ra = 1e4,
+    a 128-marker filament with K_b = 1e-3 and density ratio 1.5 clamped at
+    its leading end, FP32 populations.  The whole coupled step per time step
+    (filament geometry, IBM passes, fused D3Q19+D3Q7 step, FP64 structure
+    step), beside the fused D3Q19+D3Q7 step alone on the same lattice; then
+    FP32 against FP64 populations over `cmp_steps` steps."""
+    case, th, fil, box = cases.fsi_thermal_channel()
+
+    def make(st):
+        lat = lbm_amd.Lattice(case.Nx, case.Ny, case.Nz, case.tau, storage=st)
+        lat.upload(**case.fields())
+        lat.upload_thermal(th.T, th.type_energy, tau_g=th.tau_g, boussinesq=True, g_beta=th.g_beta,
+                           T_ref=th.T_ref, model="d3q7")
+        lat.init()
+        f = lbm_amd.Filament(**fil)
+        lat.fsi_setup(f, box, m_max=5)
+        return lat, f
+
+    lat, f = make("f32")
+    lat.fsi_run(warmup)
+    lat.sync()
+    t0 = time.perf_counter()
+    out = lat.fsi_run(steps)
+    code = lat.sync()
+    dt = time.perf_counter() - t0
+    x = f.get()[0]
+    T = lat.thermal()[0]
+    fl = case.type == cases.TYPE_F
+    # heated-wall Nusselt number: -dT/dy at the half-way wall times H / dT
+    nu_wall = float(((1.0 - T[1:-1, 1, 1:-1]) / 0.5).mean() * case.Ny)
+    lat.close()
+    f.close()
+    plain = lbm_amd.Lattice(case.Nx, case.Ny, case.Nz, case.tau, storage="f32")
+    plain.upload(**case.fields())
+    plain.upload_thermal(th.T, th.type_energy, tau_g=th.tau_g, boussinesq=True, g_beta=th.g_beta,
+                         T_ref=th.T_ref, model="d3q7")
+    plain.init()
+    plain.step(warmup)
+    plain_ms = plain.time_steps(200) / 200
+    plain_code = plain.sync()
+    plain.close()
+    cmp = {}
+    if cmp_steps:
+        res = {}
+        for st in ("f32", "f64"):
+            l2, f2 = make(st)
+            l2.fsi_run(cmp_steps)
+            res[st] = (f2.get()[0], l2.macro(), l2.thermal()[0])
+            l2.close()
+            f2.close()
+        (x32, m32, T32), (x64, m64, T64) = res["f32"], res["f64"]
+        num = sum(float(((a - b)[fl] ** 2).sum()) for a, b in zip(m32[1:], m64[1:]))
+        den = sum(float((b[fl] ** 2).sum()) for b in m64[1:])
+        cmp = dict(steps=cmp_steps, l2_u_f32_vs_f64=float(np.sqrt(num / den)),
+                   max_dT_f32_vs_f64=float(np.abs(T32 - T64)[fl].max()),
+                   max_dx_filament=float(np.abs(x32 - x64).max()))
+    ms = dt / steps * 1e3
+    cells = case.Nx * case.Ny * case.Nz
+    return dict(
+        workload="heated channel 192x96x96 D3Q19 + D3Q7 (Pr 0.71, Ra 1e4), inflow 0.03, 128-marker filament "
+                 "K_b 1e-3, density ratio 1.5, FP32",
+        status=code, steps=steps, ms_per_step=round(ms, 4), mlups=round(cells * steps / dt / 1e6, 1),
+        plain_step_ms=round(plain_ms, 4), coupling_share=round(1.0 - plain_ms / ms, 3),
+        tip=[round(float(v), 4) for v in x[-1]], force_last=[float(v) for v in out[-1, :3]],
+        nu_heated_wall=round(nu_wall, 3), T_mean=round(float(T[fl].mean()), 5),
+        T_min=float(T[fl].min()), plain_status=plain_code,
+        finite=bool(np.isfinite(out).all()), f32_vs_f64=cmp)
+
+
 def batch_leg(steps, storages=("f32", "f16s"), count=96, tune_steps=10):
     """The cavity ensemble: `count` lid-driven cavities, edges drawn from
     {32, 48, 64, 96, 128}, lid velocity and Re from their ranges (seeded).
@@ -330,6 +402,9 @@ def main():
     ap.add_argument("--no-cavity", action="store_true")
     ap.add_argument("--no-q7", action="store_true")
     ap.add_argument("--ibm-only", action="store_true", help="run only the IBM channel leg")
+    ap.add_argument("--fsi-steps", type=int, default=10000)
+    ap.add_argument("--no-fsi", action="store_true")
+    ap.add_argument("--fsi-only", action="store_true", help="run only the heated-channel filament leg")
     ap.add_argument("--batch-steps", type=int, default=4000)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-only", action="store_true", help="run only the cavity ensemble leg")
@@ -351,6 +426,9 @@ def main():
         return 0
     if a.batch_only:
         print(json.dumps(batch_leg(a.batch_steps)), flush=True)
+        return 0
+    if a.fsi_only:
+        print(json.dumps(fsi_thermal_leg(a.fsi_steps)), flush=True)
         return 0
 
     case = cases.taylor_green(a.size, u0=0.04, Re=1600.0)
@@ -401,6 +479,12 @@ def main():
         ibm = ibm_leg(a.ibm_steps, a.ibm_cmp_steps)
         log(f"bench: ibm {json.dumps(ibm)}")
 
+    fsi = None
+    if not a.no_fsi:
+        log(f"bench: heated-channel filament leg, {a.fsi_steps} steps")
+        fsi = fsi_thermal_leg(a.fsi_steps)
+        log(f"bench: fsi {json.dumps(fsi)}")
+
     batch = None
     if not a.no_batch:
         log(f"bench: cavity ensemble leg, {a.batch_steps} steps")
@@ -447,6 +531,7 @@ def main():
         "cavity": cav,
         "d3q7": q7,
         "ibm": ibm,
+        "fsi_thermal": fsi,
         "batch": batch,
         "cpu_baseline": cpu,
     }

@@ -504,17 +504,266 @@ __global__ void k_fsi_markers(int nm, const double *__restrict__ x, const double
   for (int d = 0; d < 3; d++) ud[3 * n + d] = x[3 * n + d] - xp[3 * n + d];
 }
 
+// ---- the IBM passes of a coupled filament -------------------------------------
+// A filament's markers move every step, so its kernels take the marker
+// positions straight from the filament's nodes and derive the stencil base,
+// the weights and the density cell themselves (the same expressions as
+// k_ibm_weights and k_ibm_near_cell, hence the same values); no binning and
+// no host round trip.
+struct FsiArgs {
+  Box B;
+  int nm;
+  const double *x, *xp;  // the nodes now and one step before (3 per node)
+  const double *ds;
+  const uint8_t *live;
+  double *ub, *Fb, *Fds, *fsum;  // 3 per marker
+  // written by the first pass, for the marker getters
+  double *mx, *my, *mz, *ud;
+  Stencils st;
+  int32_t *near;
+};
+
+__device__ __forceinline__ int box_cell(const Box &B, int i, int j, int k) {
+  const bool in = i >= B.x0 && i <= B.x1 && j >= B.y0 && j <= B.y1 && k >= B.z0 && k <= B.z1;
+  return in ? (i - B.x0) + B.nx * ((j - B.y0) + B.ny * (k - B.z0)) : -1;
+}
+
+// Interpolation and the next pass's marker force for a filament: as
+// k_ibm_interp (one wavefront per marker, lane = stencil point, lanes 0..2
+// sum in lane order), with u_des = the node's displacement over the last
+// step, rho at the box cell holding the marker, and the running
+// fsum = -sum F_b.  `first` starts fsum and writes the markers' positions,
+// u_des, stencils and density cells.
+__global__ __launch_bounds__(256) void k_fsi_interp(FsiArgs a, BoxArrays o, int force, int first) {
+  __shared__ double term[4][3][64];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int n = blockIdx.x * 4 + wv;
+  const bool valid = n < a.nm;
+  double p0 = 0, p1 = 0, p2 = 0, xpl = 0, dsn = 0, fs = 0;
+  if (valid) {
+    p0 = a.x[3 * n + 0];
+    p1 = a.x[3 * n + 1];
+    p2 = a.x[3 * n + 2];
+    if (lane < 3) {
+      xpl = a.xp[3 * n + lane];
+      dsn = a.ds[n];
+      if (!first) fs = a.fsum[3 * n + lane];
+    }
+  }
+  const int b0 = (int)floor(p0) - 1, b1 = (int)floor(p1) - 1, b2 = (int)floor(p2) - 1;
+  const int pa = lane >> 4, pb = (lane >> 2) & 3, pc = lane & 3;
+  int b = valid ? box_cell(a.B, b0 + pa, b1 + pb, b2 + pc) : -1;
+  const int nb = box_cell(a.B, (int)floor(p0 + 0.5), (int)floor(p1 + 0.5), (int)floor(p2 + 0.5));
+  double rho_n = 0;
+  if (force && valid && lane < 3) rho_n = nb >= 0 ? o.rho[nb] : nan("");
+  // the cell's velocity is loaded with its live flag (box arrays cover
+  // every box cell), the term kept for live cells only
+  double t0 = 0, t1 = 0, t2 = 0;
+  if (b >= 0) {
+    const bool lv = o.live[b] != 0;
+    const double vx = o.ux[b], vy = o.uy[b], vz = o.uz[b];
+    const double kx = peskin(((b0 + pa) * 1.0 - p0) / 1.0), ky = peskin(((b1 + pb) * 1.0 - p1) / 1.0),
+                 kz = peskin(((b2 + pc) * 1.0 - p2) / 1.0);
+    if (lv) {
+      t0 = vx * kx * ky * kz;
+      t1 = vy * kx * ky * kz;
+      t2 = vz * kx * ky * kz;
+    }
+  }
+  term[wv][0][lane] = t0;
+  term[wv][1][lane] = t1;
+  term[wv][2][lane] = t2;
+  __syncthreads();
+  if (!valid) return;
+  const double pl = lane == 0 ? p0 : lane == 1 ? p1 : p2;
+  if (lane < 3) {
+    double acc = 0.0;
+    for (int q = 0; q < 64; q++) acc += term[wv][lane][q];
+    a.ub[3 * n + lane] = acc;
+    if (force) {
+      const double dt = 1.0;
+      const double ud = pl - xpl;
+      const double F = 2 * rho_n * (ud - acc) / dt;
+      a.Fb[3 * n + lane] = F;
+      a.Fds[3 * n + lane] = F * dsn;
+      a.fsum[3 * n + lane] = first ? -F : fs - F;
+    }
+  }
+  if (first) {
+    if (lane < 3) {
+      (lane == 0 ? a.mx : lane == 1 ? a.my : a.mz)[n] = pl;
+      a.ud[3 * n + lane] = pl - xpl;
+      a.st.base[3 * n + lane] = lane == 0 ? b0 : lane == 1 ? b1 : b2;
+    }
+    if (lane < 12) {
+      const int ax = lane >> 2, q = lane & 3;
+      const int bb = ax == 0 ? b0 : ax == 1 ? b1 : b2;
+      const double p = ax == 0 ? p0 : ax == 1 ? p1 : p2;
+      a.st.w[12 * n + lane] = peskin(((bb + q) * 1.0 - p) / 1.0);
+    }
+    if (lane == 0) a.near[n] = nb;
+  }
+}
+
+// Spreading and velocity correction for a filament (at most FSI_MAX
+// markers): one thread per box cell gathers F_b ds phi phi phi over the
+// markers whose stencil holds the cell, in marker order (the reference's
+// loop over n, lbm.cpp:1332-1360; the terms and their order equal
+// k_ibm_spread's).  A wavefront takes a 4x4x4 tile of box cells.  The
+// markers' stencils (from the step's first k_fsi_interp) and forces are
+// staged in LDS, their loads issued together with the cell's own state, and
+// markers go in batches of FSI_BATCH with the batch's stencil hull: a
+// wavefront whose tile the hull misses skips the batch on one test, the
+// others add their hit terms in marker order.
+constexpr int FSI_MAX = 256;
+constexpr int FSI_BATCH = 8;
+__global__ __launch_bounds__(256) void k_fsi_spread(FsiArgs a, BoxArrays o) {
+  __shared__ int4 sb[FSI_MAX];
+  __shared__ int4 hull[FSI_MAX / FSI_BATCH][2];  // lowest base, highest base + 3
+  __shared__ double sw[FSI_MAX * 12];
+  __shared__ double sf[FSI_MAX * 3];
+  __shared__ int32_t sbf[FSI_MAX * 3];
+  const int nm = a.nm;
+  const int npad = (nm + FSI_BATCH - 1) / FSI_BATCH * FSI_BATCH;
+  // the thread's cell: tile t of the box (x fastest), lane (x, y, z) in it
+  const Box &B = a.B;
+  const int tx = (B.nx + 3) >> 2, ty = (B.ny + 3) >> 2;
+  const int t = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)), lane = threadIdx.x & 63;
+  const int ox = 4 * (t % tx), oy = 4 * ((t / tx) % ty), oz = 4 * (t / (tx * ty));
+  const int cx = ox + (lane & 3), cy = oy + ((lane >> 2) & 3), cz = oz + (lane >> 4);
+  const bool inb = cx < B.nx && cy < B.ny && cz < B.nz;
+  const int b = inb ? cx + B.nx * (cy + B.ny * cz) : 0;
+  // every global load first: the cell's state, the markers' positions and forces
+  bool active = false;
+  double r = 1, fx = 0, fy = 0, fz = 0, ux = 0, uy = 0, uz = 0;
+  if (inb) {
+    active = o.live[b] != 0;
+    r = o.rho[b];
+    fx = o.Fx[b];
+    fy = o.Fy[b];
+    fz = o.Fz[b];
+    ux = o.ux[b];
+    uy = o.uy[b];
+    uz = o.uz[b];
+  }
+  // the stencils of this step's markers (written by its first k_fsi_interp)
+#pragma unroll
+  for (int u = 0; u < 3 * FSI_MAX / 256; u++) {
+    const int q = (int)threadIdx.x + 256 * u;
+    const bool real = q < 3 * nm;
+    sbf[q] = real ? a.st.base[q] : -(1 << 28);  // no cell is within 4 of it
+    sf[q] = real ? a.Fds[q] : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < 12 * FSI_MAX / 256; u++) {
+    const int q = (int)threadIdx.x + 256 * u;
+    sw[q] = q < 12 * nm ? a.st.w[q] : 0.0;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < npad; q += blockDim.x) sb[q] = make_int4(sbf[3 * q], sbf[3 * q + 1], sbf[3 * q + 2], 0);
+  __syncthreads();
+  for (int q = threadIdx.x; q < npad / FSI_BATCH; q += blockDim.x) {
+    int4 lo = make_int4(1 << 29, 1 << 29, 1 << 29, 0), hi = make_int4(-(1 << 29), -(1 << 29), -(1 << 29), 0);
+    for (int u = 0; u < FSI_BATCH; u++) {
+      const int n = q * FSI_BATCH + u;
+      if (n >= nm) break;
+      const int4 v = sb[n];
+      lo.x = min(lo.x, v.x);
+      lo.y = min(lo.y, v.y);
+      lo.z = min(lo.z, v.z);
+      hi.x = max(hi.x, v.x + 3);
+      hi.y = max(hi.y, v.y + 3);
+      hi.z = max(hi.z, v.z + 3);
+    }
+    hull[q][0] = lo;
+    hull[q][1] = hi;
+  }
+  __syncthreads();
+  // the tile in reference indices
+  const int i0 = B.x0 + ox, j0 = B.y0 + oy, k0 = B.z0 + oz;
+  const int i = B.x0 + cx, j = B.y0 + cy, k = B.z0 + cz;
+  // the batches whose hull meets the tile (uniform over the wavefront)
+  uint32_t reach = 0;
+#pragma unroll
+  for (int q = 0; q < FSI_MAX / FSI_BATCH; q++) {
+    const int4 lo = hull[q][0], hi = hull[q][1];
+    const bool meets = q * FSI_BATCH < nm && lo.x <= i0 + 3 && hi.x >= i0 && lo.y <= j0 + 3 && hi.y >= j0 &&
+                       lo.z <= k0 + 3 && hi.z >= k0;
+    reach |= (uint32_t)meets << q;
+  }
+  reach = __builtin_amdgcn_readfirstlane(reach);
+  double m0 = 0, m1 = 0, m2 = 0;
+  bool any = false;
+  while (reach) {
+    const int n0 = FSI_BATCH * __builtin_ctz(reach);
+    reach &= reach - 1;
+    uint32_t di[FSI_BATCH], dj[FSI_BATCH], dk[FSI_BATCH];
+    uint32_t hit = 0;
+#pragma unroll
+    for (int u = 0; u < FSI_BATCH; u++) {
+      const int4 q = sb[n0 + u];
+      di[u] = (uint32_t)(i - q.x);
+      dj[u] = (uint32_t)(j - q.y);
+      dk[u] = (uint32_t)(k - q.z);
+      hit |= (uint32_t)((di[u] | dj[u] | dk[u]) < 4u) << u;
+    }
+    hit = active ? hit : 0u;
+    if (!__ballot(hit != 0)) continue;
+    any = any || hit != 0;
+#pragma unroll
+    for (int u = 0; u < FSI_BATCH; u++) {
+      const int n = n0 + u;
+      const double kx = sw[12 * n + (di[u] & 3)], ky = sw[12 * n + 4 + (dj[u] & 3)],
+                   kz = sw[12 * n + 8 + (dk[u] & 3)];
+      const double t0 = sf[3 * n + 0] * kx * ky * kz;
+      const double t1 = sf[3 * n + 1] * kx * ky * kz;
+      const double t2 = sf[3 * n + 2] * kx * ky * kz;
+      const bool h = (hit >> u) & 1u;
+      m0 = h ? m0 + t0 : m0;
+      m1 = h ? m1 + t1 : m1;
+      m2 = h ? m2 + t2 : m2;
+    }
+  }
+  if (!any) return;
+  const double dt = 1.0;
+  o.Fx[b] = fx + -m0;
+  o.Fy[b] = fy + -m1;
+  o.Fz[b] = fz + -m2;
+  o.ux[b] = ux + dt * m0 / (2 * r);
+  o.uy[b] = uy + dt * m1 / (2 * r);
+  o.uz[b] = uz + dt * m2 / (2 * r);
+}
+
+// 4x4x4 tiles of a box, 4 per workgroup
+inline dim3 fsi_spread_grid(const Box &B) {
+  const int tiles = ((B.nx + 3) >> 2) * ((B.ny + 3) >> 2) * ((B.nz + 3) >> 2);
+  return dim3((unsigned)((tiles + 3) / 4));
+}
+
 // Per coupled step: the total force on the filament, sum over nodes of
-// (-sum F_b) ds, in node order, and the free end's position.
+// (-sum F_b) ds (lane-strided partial sums, then a wavefront tree), and the
+// free end's position.
 __global__ __launch_bounds__(64) void k_fsi_out(int nm, const double *__restrict__ fsum, const double *__restrict__ ds,
                                                 const double *__restrict__ x, double *__restrict__ out) {
-  if (threadIdx.x < 3) {
-    const int d = threadIdx.x;
-    double s = 0;
-    for (int n = 0; n < nm; n++) s += fsum[3 * n + d] * ds[n];
-    out[d] = s;
-    out[3 + d] = x[3 * (nm - 1) + d];
+  const int l = threadIdx.x;
+  double s0 = 0, s1 = 0, s2 = 0;
+  for (int n = l; n < nm; n += 64) {
+    const double w = ds[n];
+    s0 += fsum[3 * n + 0] * w;
+    s1 += fsum[3 * n + 1] * w;
+    s2 += fsum[3 * n + 2] * w;
   }
+  for (int o = 32; o > 0; o >>= 1) {
+    s0 += __shfl_down(s0, o, 64);
+    s1 += __shfl_down(s1, o, 64);
+    s2 += __shfl_down(s2, o, 64);
+  }
+  if (l == 0) {
+    out[0] = s0;
+    out[1] = s1;
+    out[2] = s2;
+  }
+  if (l < 3) out[3 + l] = x[3 * (nm - 1) + l];
 }
 
 __global__ void k_fill_i32(int32_t *p, int n, int32_t v) {
@@ -841,6 +1090,27 @@ static int ibm_iterate(lbm_ctx *c, const double vcm[3], const double vb[3], doub
   return LBM_OK;
 }
 
+// The IBM passes of a coupled filament's step after the box moments: the
+// first interpolation (with the markers' geometry), then m_max spread /
+// interpolate passes; the last interpolation computes no force.
+static int fsi_iterate(lbm_ctx *c) {
+  lbm_ibm *m = c->ibm;
+  double *fx, *fxp, fds;
+  int n, fdev;
+  filament_state(m->fil, &fx, &fxp, &n, &fds, &fdev);
+  const FsiArgs a{m->B, n, fx, fxp, m->ds, m->live, m->ub, m->Fb, m->Fds, m->fsum,
+                  m->mx, m->my, m->mz, m->ud, m->st, m->near};
+  const BoxArrays o = box_arrays(m);
+  const dim3 gi((unsigned)((n + 3) / 4));
+  hipLaunchKernelGGL(k_fsi_interp, gi, dim3(256), 0, c->stream, a, o, m->m_max > 0 ? 1 : 0, 1);
+  for (int it = 1; it <= m->m_max; it++) {
+    hipLaunchKernelGGL(k_fsi_spread, fsi_spread_grid(m->B), dim3(256), 0, c->stream, a, o);
+    hipLaunchKernelGGL(k_fsi_interp, gi, dim3(256), 0, c->stream, a, o, it < m->m_max ? 1 : 0, 0);
+  }
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
 // Marker motion of one IBLBM call (lbm.cpp:1154-1166): the trapezoid rule
 // for the markers and, when the box follows the body, for its centre; then
 // the box and the geometry that depends on the marker positions.
@@ -1067,6 +1337,7 @@ int lbm_fsi_setup(lbm_ctx *c, lbm_filament *f, const lbm_ibm_desc *d) {
   int n, fdev;
   filament_state(f, &fx, &fxp, &n, &fds, &fdev);
   if (fdev != c->dev) return LBM_ERR_ARG;
+  if (n > FSI_MAX) return LBM_ERR_UNSUPPORTED;  // k_fsi_spread's LDS tables
   HIPCHK(hipSetDevice(c->dev));
   std::vector<double> h(3 * (size_t)n), x(n), y(n), z(n), zero(n, 0.0), ds(n, fds);
   HIPCHK(hipDeviceSynchronize());
@@ -1115,21 +1386,17 @@ int lbm_fsi_run(lbm_ctx *c, int32_t nsteps, double *out) {
     HIPCHK(hipMalloc((void **)&m->out, 3 * sizeof(double) * 2 * nsteps));
     m->out_cap = 2 * nsteps;
   }
-  const double zero[3] = {0, 0, 0};
   for (int s = 0; s < nsteps; s++) {
-    hipLaunchKernelGGL(k_fsi_markers, blocks(n), dim3(256), 0, c->stream, n, fx, fxp, m->mx, m->my, m->mz, m->ud);
-    HIPCHK(hipGetLastError());
-    int e = ibm_geometry(c);
-    if (e != LBM_OK) return e;
+    int e;
     LaunchCtx L = launch_ctx(c);
     L.force_mode = 3;
     if (c->state == ST_F) {
       HIPCHK(box_moments(c, false));
-      if ((e = ibm_iterate(c, zero, zero, nullptr)) != LBM_OK) return e;
+      if ((e = fsi_iterate(c)) != LBM_OK) return e;
       if ((e = ibm_collide(c, L)) != LBM_OK) return e;
     } else {
       HIPCHK(box_moments(c, true));
-      if ((e = ibm_iterate(c, zero, zero, nullptr)) != LBM_OK) return e;
+      if ((e = fsi_iterate(c)) != LBM_OK) return e;
       if ((e = ibm_stream(c, L, s == nsteps - 1 ? STREAM_COLLIDE_MACRO : STREAM_COLLIDE)) != LBM_OK) return e;
       std::swap(c->cur, c->oth);
       if ((e = after_stream(c)) != LBM_OK) return e;

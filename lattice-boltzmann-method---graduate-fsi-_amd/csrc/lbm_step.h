@@ -56,10 +56,31 @@ __device__ __forceinline__ void load_force(const KArgs<C> &a, uint32_t idx, C &F
   }
 }
 
+// Posts a cell's status bits and index from the lanes that call it (a
+// divergent branch): at most one atomic pair per wavefront -- the OR of the
+// calling lanes' bits (status bits are below 256) and the smallest index,
+// which is the first calling lane's unless a later lane holds a smaller one
+// (then every calling lane posts its own).  Both words only grow (bits) or
+// shrink (index) during a launch, so a post that the current value already
+// covers is skipped: a launch with many bad cells does not serialise on
+// the two words.
 __device__ __forceinline__ void flag(int *status, unsigned long long *first_bad, int bits,
                                      uint32_t idx) {
-  atomicOr(status, bits);
-  atomicMin(first_bad, (unsigned long long)idx);
+  int all = 0;
+  for (int b = 0; b < 8; b++)
+    if (__ballot(bits & (1 << b))) all |= 1 << b;
+  const uint32_t first = __builtin_amdgcn_readfirstlane(idx);
+  const bool lead = (int)__lane_id() == __ffsll((unsigned long long)__ballot(1)) - 1;
+  const unsigned long long fb = __hip_atomic_load(first_bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (__ballot(idx < first)) {
+    if (idx < fb) atomicMin(first_bad, (unsigned long long)idx);
+  } else if (lead && first < fb) {
+    atomicMin(first_bad, (unsigned long long)first);
+  }
+  if (lead) {
+    const int st = __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((st & all) != all) atomicOr(status, all);
+  }
 }
 
 __device__ __forceinline__ float fma_c(float a, float b, float c) { return __builtin_fmaf(a, b, c); }

@@ -178,6 +178,22 @@ def test_density_range_error_reports_cell(oracle_mod):
     assert cell == np.ravel_multi_index((3, 4, 5), case.shape)
 
 
+def test_density_range_error_many_cells_reports_first():
+    """Several bad cells, some in one wavefront: the reported cell is the
+    first in the device's storage order (k, then j, then i)."""
+    case = cases.cavity(8)
+    bad = [(6, 4, 5), (3, 4, 5), (5, 4, 5), (2, 7, 6)]
+    for c in bad:
+        case.rho[c] = 1000.0
+    lat = _gpu(case)
+    lat.step(1)
+    code, cell = lat.status()
+    assert code == lbm_amd.LBM_ERR_DENSITY_RANGE
+    first = min(bad, key=lambda c: (c[2], c[1], c[0]))
+    assert cell == np.ravel_multi_index(first, case.shape)
+    lat.close()
+
+
 def test_negative_equilibrium_error():
     case = cases.cavity(8)
     case.ux[4, 4, 4] = 1.5  # feq < 0 terminates the collision (lbm.cpp:114-117)
