@@ -63,9 +63,11 @@ def run_rate(case, storage, warmup, steps):
 
 
 def precision_study(case, nsteps):
-    """Relative L2 of u and of rho-1 over TYPE_F cells after nsteps, FP32 and
-    FP16S storage against FP64 storage (GPU FP64 is itself checked against
-    the CPU oracle to 1e-12 in the test suite)."""
+    """Relative L2 of u and of rho-1 over TYPE_F cells after nsteps, FP32,
+    FP16S and F16C storage against FP64 storage (GPU FP64 is itself checked
+    against the CPU oracle to 1e-12 in the test suite); per storage the
+    kinetic-energy ratio E(n)/E(0) and the run's MLUPS and algorithmic GB/s
+    (HIP events around the nsteps)."""
     out = {}
     fields = {}
     for st in ("f64", "f32", "f16s", "f16c"):
@@ -73,13 +75,15 @@ def precision_study(case, nsteps):
         lat.upload(**case.fields())
         lat.init()
         ke0 = lat.reduce(lbm_amd.REDUCE_KINETIC)
-        lat.step(nsteps)
+        ms = lat.time_steps(nsteps)
         code = lat.sync()
         rho, ux, uy, uz = lat.macro()
         ke1 = lat.reduce(lbm_amd.REDUCE_KINETIC)
+        n, bpc = lat.nfluid, lat.bytes_per_cell
         lat.close()
         fields[st] = (rho, ux, uy, uz)
-        out[st] = dict(status=code, ke_ratio=ke1 / ke0)
+        out[st] = dict(status=code, ke_ratio=ke1 / ke0, mlups=round(n * nsteps / (ms / 1e3) / 1e6, 1),
+                       achieved_gbs=round(bpc * n * nsteps / (ms / 1e3) / 1e9, 1))
     r = fields["f64"]
     num_u = lambda a: sum(float(((x - y) ** 2).sum()) for x, y in zip(a[1:], r[1:]))  # noqa: E731
     den_u = sum(float((y ** 2).sum()) for y in r[1:])
@@ -391,6 +395,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--precision-steps", type=int, default=5000)
+    ap.add_argument("--precision-sizes", default="256,384",
+                    help="edges of the Taylor-Green precision study (comma separated)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-max-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
@@ -450,10 +456,18 @@ def main():
         }
 
     prec = None
+    prec_more = {}
     if not a.no_precision:
-        log(f"bench: precision study, {a.precision_steps} steps x 4 storages")
-        prec = precision_study(case, a.precision_steps)
-        log(f"bench: precision {json.dumps(prec)}")
+        for edge in [int(v) for v in a.precision_sizes.split(",") if v]:
+            log(f"bench: precision study {edge}^3, {a.precision_steps} steps x 4 storages")
+            pc = case if edge == a.size else cases.taylor_green(edge, u0=0.04, Re=1600.0)
+            r = precision_study(pc, a.precision_steps)
+            log(f"bench: precision {edge}^3 {json.dumps(r)}")
+            if edge == a.size:
+                prec = r
+            else:
+                prec_more[f"precision_{edge}"] = dict(workload=f"Taylor-Green {edge}^3, u0=0.04, Re=1600",
+                                                      steps=a.precision_steps, **r)
 
     cpu = None
     if not a.no_cpu:
@@ -528,6 +542,7 @@ def main():
         },
         **others,
         "precision": prec,
+        **prec_more,
         "cavity": cav,
         "d3q7": q7,
         "ibm": ibm,
