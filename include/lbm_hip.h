@@ -212,6 +212,20 @@ int lbm_ibm_markers(lbm_ctx *ctx, double *x, double *y, double *z, double *ub,
  * after the last IBLBM). */
 int lbm_ibm_box(lbm_ctx *ctx, int32_t box[6], double center[2]);
 
+/* ---- momentum exchange on bounce-back bodies ------------------------------
+ * MEA_CD and MEA_CL (lbm.cpp:3085-3163, called at main.cpp:246-249 and
+ * 369-372 after Collision, Streaming and MacroProp) as one FP64 reduction:
+ * over the TYPE_F cells of the inclusive box {x0,x1,y0,y1,z0,z1} (reference
+ * indices, within 0..N+1) and each direction l whose neighbour x - c_l is
+ * TYPE_S, out[0] = Px = -sum (f*_l + f_opp(l)) c_lx, out[1] = Py =
+ * +sum (f*_l + f_opp(l)) c_ly (the reference's two signs), out[2] = Pz =
+ * +sum (f*_l + f_opp(l)) c_lz (no reference counterpart).  f* is the last
+ * collision's output, f the streamed populations: valid after lbm_step,
+ * lbm_stream or an IBM/FSI run (LBM_ERR_STATE after lbm_init alone or after
+ * lbm_collide).  The reference divides by 1/2 rho0 u_max^2 D (cylinder,
+ * lbm.cpp:3114) or D^2 PI/4 (sphere, lbm.cpp:3118) for C_L and C_D. */
+int lbm_momentum_exchange(lbm_ctx *ctx, const int32_t box[6], double out[3]);
+
 /* ---- flexible filament (no reference counterpart: the reference has no
  * flexible-structure solver; the north star's Lagrangian structure update)
  * ------------------------------------------------------------------------

@@ -44,6 +44,7 @@ SOURCES = [
     os.path.join(HERE, "csrc", "lbm_batch.hip"),
     os.path.join(HERE, "csrc", "lbm_q7.hip"),
     os.path.join(HERE, "csrc", "lbm_filament.hip"),
+    os.path.join(HERE, "csrc", "lbm_mea.hip"),
 ]
 HOST_SOURCES = [
     os.path.join(HERE, "host", "LBM.cpp"),

@@ -427,6 +427,76 @@ def random_mask_case(Nx: int, Ny: int, Nz: int, seed: int = 7, p_solid: float = 
     return Case("random_mask", Nx, Ny, Nz, tau, t, rho, ux, uy, uz, params=dict(seed=seed))
 
 
+def cylinder_bb(Nx: int = 100, Ny: int = 100, Nz: int = 1, D: float = 20.0, u_max: float = 0.1,
+                Re: float = 200.0, center=None) -> Case:
+    """main_setup_Cylinder with Generate_Cylinder (lbm.cpp:1429-1487): cells
+    i = 1..Nx, j = 1..Ny, k = 0..Nz whose centre (i - 1/2, j - 1/2) lies
+    within D/2 of the axis (Nx/5, Ny/2) become TYPE_S; then i = 0 TYPE_IN,
+    i = Nx+1 and j = 0, Ny+1 TYPE_OUT, the k ghost layers TYPE_F when
+    Nz != 1; F/IN/OUT cells start at rho = 1, u = (u_max, 0, 0), solids at
+    the LATTICE defaults rho = 1, u = 0 (LBM.h:559-563).  tau = 1/2 + nu/cs2
+    with nu = u_max D / Re (LBM.h:102-103).  params['mea_box'] is the
+    reference's MEA box (the int conversion of centre -+ D, k = 1..Nz as in
+    main.cpp:246 for Nz = 1)."""
+    t, rho, ux, uy, uz = _blank(Nx, Ny, Nz)
+    cx, cy = center if center is not None else (Nx / 5.0, Ny / 2.0)
+    i = np.arange(Nx + 2)[:, None, None]
+    j = np.arange(Ny + 2)[None, :, None]
+    k = np.arange(Nz + 2)[None, None, :]
+    rx, ry = i * 1.0 - 0.5, j * 1.0 - 0.5
+    inside = np.sqrt((cx - rx) ** 2 + (cy - ry) ** 2) <= D / 2.0
+    region = (i >= 1) & (i <= Nx) & (j >= 1) & (j <= Ny) & (k >= 0) & (k <= Nz)
+    t[np.broadcast_to(inside & region, t.shape)] = TYPE_S
+    t[0, :, :] = TYPE_IN
+    t[-1, :, :] = TYPE_OUT
+    t[:, 0, :] = TYPE_OUT
+    t[:, -1, :] = TYPE_OUT
+    if Nz != 1:
+        t[:, :, 0] = TYPE_F
+        t[:, :, -1] = TYPE_F
+    live = t != TYPE_S
+    ux[live] = u_max
+    nu = u_max * D / Re
+    tau = 0.5 + nu / (1.0 / 3.0)
+    box = (max(0, int(cx - D)), int(cx + D), max(0, int(cy - D)), int(cy + D), 1, Nz)
+    return Case("cylinder_bb", Nx, Ny, Nz, tau, t, rho, ux, uy, uz,
+                params=dict(u_max=u_max, Re=Re, D=D, nu=nu, center=(cx, cy), mea_box=box,
+                            mea_norm=0.5 * 1.0 * u_max ** 2 * D))
+
+
+def sphere_bb(Nx: int = 800, Ny: int = 400, Nz: int = 400, D: float = 20.0, u_max: float = 0.1,
+              Re: float = 200.0, center=None) -> Case:
+    """main_setup_Sphere with Generate_Sphere (lbm.cpp:1759-1820): cells
+    i = 1..Nx, j = 1..Ny, k = 0..Nz whose centre (i - 1/2, j - 1/2, k - 1/2)
+    lies within D/2 of (Nx/5, Ny/2, Nz/2) become TYPE_S (rho = 1, u = 0);
+    then i = 0 TYPE_IN and the other five faces TYPE_OUT; F/IN/OUT cells
+    start at rho = 1, u = (u_max, 0, 0).  params['mea_box'] is the
+    reference's MEA box (main.cpp:369) and params['mea_norm'] the
+    C_D/C_L normalisation 1/2 rho0 u^2 D^2 PI/4 (lbm.cpp:3122)."""
+    t, rho, ux, uy, uz = _blank(Nx, Ny, Nz)
+    cx, cy, cz = center if center is not None else (Nx / 5.0, Ny / 2.0, Nz / 2.0)
+    i = np.arange(Nx + 2)[:, None, None]
+    j = np.arange(Ny + 2)[None, :, None]
+    k = np.arange(Nz + 2)[None, None, :]
+    d = np.sqrt((cx - (i - 0.5)) ** 2 + (cy - (j - 0.5)) ** 2 + (cz - (k - 0.5)) ** 2)
+    region = (i >= 1) & (i <= Nx) & (j >= 1) & (j <= Ny) & (k >= 0) & (k <= Nz)
+    t[(d <= D / 2.0) & region] = TYPE_S
+    t[0, :, :] = TYPE_IN
+    t[-1, :, :] = TYPE_OUT
+    t[:, 0, :] = TYPE_OUT
+    t[:, -1, :] = TYPE_OUT
+    t[:, :, 0] = TYPE_OUT
+    t[:, :, -1] = TYPE_OUT
+    live = t != TYPE_S
+    ux[live] = u_max
+    nu = u_max * D / Re
+    tau = 0.5 + nu / (1.0 / 3.0)
+    box = (int(cx - D), int(cx + D), int(cy - D), int(cy + D), int(cz - D), int(cz + D))
+    return Case("sphere_bb", Nx, Ny, Nz, tau, t, rho, ux, uy, uz,
+                params=dict(u_max=u_max, Re=Re, D=D, nu=nu, center=(cx, cy, cz), mea_box=box,
+                            mea_norm=0.5 * 1.0 * u_max ** 2 * D * D * PI_REF / 4))
+
+
 def cylinder_markers(center_x: float, center_y: float, D: float, n_theta: int,
                      z_levels) -> dict:
     """Markers on a cylinder of diameter D along z, in the pattern of

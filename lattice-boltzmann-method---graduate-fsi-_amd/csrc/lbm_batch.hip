@@ -172,6 +172,7 @@ int batched(lbm_batch *b, int32_t nsteps, double *ms) {
   for (lbm_ctx *c : b->ctx) {
     if ((K + 1) % 2) std::swap(c->cur, c->oth);
     c->state = ST_F;
+    c->fstar_ready = 1;  // the last launch streamed from oth
   }
   HIPCHK(hipEventRecord(b->done, b->stream));
   for (lbm_ctx *c : b->ctx) HIPCHK(hipStreamWaitEvent(c->stream, b->done, 0));

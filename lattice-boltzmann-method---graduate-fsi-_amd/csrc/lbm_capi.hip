@@ -329,6 +329,7 @@ int lbm_init(lbm_ctx *c) {
       if (c->h_type[d] == LBM_TYPE_IN && c->h_type[top + d] == LBM_TYPE_S) c->backeq_solid_pending = 1;
   }
   c->state = ST_F;
+  c->fstar_ready = 0;
   return LBM_OK;
 }
 
@@ -397,6 +398,7 @@ int lbm_stream(lbm_ctx *c) {
     HIPCHK(launch_stream(launch_ctx(c), c->cur, c->oth, STREAM_ONLY));
   }
   std::swap(c->cur, c->oth);
+  c->fstar_ready = 1;
   c->state = ST_F;
   return after_stream(c);
 }
@@ -434,6 +436,7 @@ static int step_launch(lbm_ctx *c, const LaunchCtx &L, int mode) {
     HIPCHK(launch_stream(L, c->cur, c->oth, mode));
   }
   std::swap(c->cur, c->oth);
+  c->fstar_ready = 1;
   return LBM_OK;
 }
 

@@ -35,6 +35,9 @@ struct lbm_ctx {
   unsigned long long *first_bad = nullptr;
   double *red_partial = nullptr, *red_out = nullptr;
   int state = ST_NONE;
+  // the last buffer operation was a streaming that pulled from oth: oth holds
+  // the f* that produced cur's f (what the momentum exchange reads)
+  int fstar_ready = 0;
   int uploaded = 0;
   // The reference's first Streaming reads the solids' setup velocities; from
   // the first MacroProp on (lbm.cpp:672-675 runs over solids too) it reads

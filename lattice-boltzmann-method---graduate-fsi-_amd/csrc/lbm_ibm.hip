@@ -1304,6 +1304,7 @@ int lbm_ibm_run(lbm_ctx *c, int32_t nsteps, const double *vcm, double *out) {
       if (e != LBM_OK) return e;
       if ((e = ibm_stream(c, L, s == nsteps - 1 ? STREAM_COLLIDE_MACRO : STREAM_COLLIDE)) != LBM_OK) return e;
       std::swap(c->cur, c->oth);
+      c->fstar_ready = 1;
       e = after_stream(c);
       if (e != LBM_OK) return e;
     }
@@ -1315,6 +1316,7 @@ int lbm_ibm_run(lbm_ctx *c, int32_t nsteps, const double *vcm, double *out) {
     if (int e = ibm_stream(c, L, STREAM_ONLY)) return e;
   }
   std::swap(c->cur, c->oth);
+  c->fstar_ready = 1;
   c->state = ST_F;
   int e = after_stream(c);
   if (e != LBM_OK) return e;
@@ -1399,6 +1401,7 @@ int lbm_fsi_run(lbm_ctx *c, int32_t nsteps, double *out) {
       if ((e = fsi_iterate(c)) != LBM_OK) return e;
       if ((e = ibm_stream(c, L, s == nsteps - 1 ? STREAM_COLLIDE_MACRO : STREAM_COLLIDE)) != LBM_OK) return e;
       std::swap(c->cur, c->oth);
+      c->fstar_ready = 1;
       if ((e = after_stream(c)) != LBM_OK) return e;
     }
     c->state = ST_FSTAR;
@@ -1411,6 +1414,7 @@ int lbm_fsi_run(lbm_ctx *c, int32_t nsteps, double *out) {
     if (int e = ibm_stream(c, L, STREAM_ONLY)) return e;
   }
   std::swap(c->cur, c->oth);
+  c->fstar_ready = 1;
   c->state = ST_F;
   int e = after_stream(c);
   if (e != LBM_OK) return e;

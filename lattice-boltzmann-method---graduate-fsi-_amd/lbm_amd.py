@@ -175,6 +175,7 @@ _SIGNATURES = {
     "lbm_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32)]),
     "lbm_fsi_setup": (C.c_int, [_P, _P, C.POINTER(_IbmDesc)]),
     "lbm_fsi_run": (C.c_int, [_P, C.c_int32, _D]),
+    "lbm_momentum_exchange": (C.c_int, [_P, C.POINTER(C.c_int32), _D]),
     "lbm_filament_create": (C.c_int, [C.c_int32, C.POINTER(_FilamentDesc), C.POINTER(_P)]),
     "lbm_filament_destroy": (C.c_int, [_P]),
     "lbm_filament_substeps": (C.c_int, [_P]),
@@ -469,6 +470,14 @@ class Lattice:
                 ("x", "y", "z", "nx", "ny", "nz", "ds")]
         self._ibm_n = n
         _check(self._lib.lbm_ibm_setup(self._h, C.byref(d), *[_dp(a) for a in arrs]), "lbm_ibm_setup")
+
+    def momentum_exchange(self, box):
+        """MEA_CD / MEA_CL's raw momentum sums (Px, Py, Pz) over the
+        inclusive box (x0, x1, y0, y1, z0, z1), lbm.cpp:3085-3163."""
+        b = (C.c_int32 * 6)(*[int(v) for v in box])
+        out = np.zeros(3)
+        _check(self._lib.lbm_momentum_exchange(self._h, b, _dp(out)), "lbm_momentum_exchange")
+        return out
 
     def ibm_box(self):
         b = (C.c_int32 * 6)()

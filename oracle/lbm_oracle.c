@@ -935,6 +935,33 @@ static void ibm_interpolate(orc_lbm *lb, int nm, const double *mx, const double 
   }
 }
 
+/* MEA_CD / MEA_CL, lbm.cpp:3085-3163: over the box's TYPE_F cells and the
+ * directions l whose neighbour x - c_l is TYPE_S, sum (f*_l + f_opp(l)) with
+ * -c_lx (drag) and +c_ly (lift), i, j, k, l serially; out[2] takes +c_lz.
+ * The neighbour index wraps as the streaming's does (the reference indexes
+ * x - c_l unwrapped, which stays in the lattice for boxes inside 1..N). */
+void orc_mea(const orc_lbm *lb, const int box[6], double out[3]) {
+  double Px = 0, Py = 0, Pz = 0;
+  for (int i = box[0]; i <= box[1]; ++i)
+    for (int j = box[2]; j <= box[3]; ++j)
+      for (int k = box[4]; k <= box[5]; ++k) {
+        const orc_cell *c = &lb->fluid[i][j][k];
+        if (c->type != ORC_TYPE_F) continue;
+        for (int l = 0; l < lb->q; ++l) {
+          const int i_nb = WRAP(i - lb->cx[l], lb->Nx + 2), j_nb = WRAP(j - lb->cy[l], lb->Ny + 2),
+                    k_nb = WRAP(k - lb->cz[l], lb->Nz + 2);
+          if (lb->fluid[i_nb][j_nb][k_nb].type != ORC_TYPE_S) continue;
+          Px += -(c->f_star[l] + c->f[lb->opp[l]]) * lb->cx[l];
+          Py += (c->f_star[l] + c->f[lb->opp[l]]) * lb->cy[l];
+          Pz += (c->f_star[l] + c->f[lb->opp[l]]) * lb->cz[l];
+        }
+      }
+  /* pow(dx,3)/dt = 1 */
+  out[0] = Px;
+  out[1] = Py;
+  out[2] = Pz;
+}
+
 /* D3Q7 (lbm_oracle.h) under an IBM step, which replaces MacroProp: T and q
  * of every fluid cell from its g, and the buoyancy of T added to the IBM's
  * force and velocity (half of it, as MacroProp's u). */

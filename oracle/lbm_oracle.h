@@ -127,6 +127,14 @@ int orc_ibm_step(orc_lbm *lb, int nm, double *mx, double *my, double *mz,
                  const double vcm[3], const double vcm_before[3],
                  int xmin, int xmax, int ymin, int ymax, int zmin, int zmax,
                  int m_max, double vsolid, double *out);
+/* MEA_CD / MEA_CL's momentum sums (lbm.cpp:3085-3163) over the inclusive
+ * box {x0,x1,y0,y1,z0,z1}, evaluated where the reference does (after
+ * Collision, Streaming, MacroProp: f* of the last collision, f streamed):
+ * out[0] = Px = -sum (f*_l + f_opp) c_lx, out[1] = Py = +sum (...) c_ly
+ * over TYPE_F cells and links whose x - c_l is TYPE_S; out[2] = Pz, the y
+ * form in z (the reference has none). */
+void orc_mea(const orc_lbm *lb, const int box[6], double out[3]);
+
 /* The IBM step of a coupled filament (no reference counterpart): as
  * orc_ibm_step with the markers where they are (no rigid motion), the
  * desired velocity udes per marker (3 nm), rho at the box cell holding the

@@ -58,6 +58,7 @@ _SIG = {
     "orc_q7_get": (None, [_P, _D, _D, _D, _D]),
     "orc_q7_get_g": (None, [_P, _D]),
     "orc_fsi_ibm_step": (C.c_int, [_P, C.c_int, _D, _D, _D, _D, _D, _D, _D, _D] + [C.c_int] * 7),
+    "orc_mea": (None, [_P, C.POINTER(C.c_int), _D]),
     "orc_fil_create": (_P, [C.c_int] + [C.c_double] * 4 + [_D, _D, _D, C.c_int]),
     "orc_fil_destroy": (None, [_P]),
     "orc_fil_substeps": (C.c_int, [_P]),
@@ -240,6 +241,13 @@ class Oracle:
                                         _dp(fsum), *[int(b) for b in box], int(m_max))
         assert err == 0, "a marker left the box"
         return ub, Fb, fsum
+
+    def mea(self, box):
+        """MEA_CD / MEA_CL's raw momentum sums (Px, Py, Pz) over the box."""
+        b = (C.c_int * 6)(*[int(v) for v in box])
+        out = np.zeros(3)
+        self.lib.orc_mea(self.h, b, _dp(out))
+        return out
 
     def wall_temperature(self, i, j, k):
         return self.lib.orc_wall_temperature(self.h, i, j, k)

@@ -783,3 +783,82 @@ def test_filament_cantilever_first_mode(oracle_mod):
     assert abs(measured / w1 - 1) < 1e-2, (measured, w1)
     x = g.get()[0]
     assert np.abs(np.linalg.norm(np.diff(x, axis=0), axis=1) - L / (f.n - 1)).max() < 1e-12
+
+
+# ---- momentum exchange (MEA_CD / MEA_CL, lbm.cpp:3085-3163) ---------------------
+def _walled(axis, N=(4, 5, 3), rho=1.2, tau=0.8):
+    t, r, ux, uy, uz = cases._blank(*N)
+    lo = [slice(None)] * 3
+    hi = [slice(None)] * 3
+    lo[axis] = 0
+    hi[axis] = -1
+    t[tuple(lo)] = cases.TYPE_S
+    t[tuple(hi)] = cases.TYPE_S
+    r[:] = rho
+    return cases.Case("walled", *N, tau, t, r, ux, uy, uz)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_mea_rest_fluid_wall_pressure(oracle_mod, axis):
+    """Fluid at rest at density rho between two walls normal to `axis`: every
+    wall link carries f* = f = w rho, so a box holding the low wall's fluid
+    layer sums 2 rho (1/18 + 4/36) = rho/3 per cell, with the sign the
+    reference gives that component (-c_x for Px, +c_y for Py, +c_z for Pz);
+    the whole lattice sums to zero.  Catches a dropped f or f* term (rho/6),
+    the link test x + c_l in place of x - c_l (sign flip) and a wrong
+    component sign."""
+    rho = 1.2
+    case = _walled(axis, rho=rho)
+    o = oracle_mod.run_case(case, 3)
+    shape = case.shape
+    layer = int(np.prod([shape[d] for d in range(3) if d != axis]))
+    box = []
+    for d in range(3):
+        box += [0, (shape[d] - 1) // 2] if d == axis else [0, shape[d] - 1]
+    P = o.mea(box)
+    want = (-1.0 if axis == 0 else 1.0) * rho / 3 * layer
+    assert abs(P[axis] - want) < 1e-13 * layer
+    assert np.abs(np.delete(P, axis)).max() < 1e-13 * layer
+    full = [0, shape[0] - 1, 0, shape[1] - 1, 0, shape[2] - 1]
+    assert np.abs(o.mea(full)).max() < 1e-13 * layer
+
+
+def test_mea_moving_lid_second_step_closed_form(oracle_mod):
+    """Fluid at rest (rho = 1.2) under a lid at j = Ny+1 moving with U along
+    x.  Step 1's bounce-back sets the top cells' links from the lid to
+    f_l = w rho - 6 w U c_opp(l),x (no rho factor, lbm.cpp:274): x-momentum
+    U/3, density unchanged.  Step 2 relaxes them towards feq(rho, U/(3 rho)),
+    whose x-momentum on those links is 6 w rho u = U/18 (the cubic terms
+    cancel), and the streamed f_opp(l) come from cells still at rest.  So
+    after two steps Px = -(Nx+2)(Nz+2) ((1 - omega) U/3 + omega U/18) and
+    Py = 0.  Catches a missing wall-velocity term, a rho factor in the
+    bounce-back, a wrong relaxation weight and a wrong sign of Px."""
+    U, tau = 0.05, 0.8
+    case = cases.couette(Nx=4, Ny=6, Nz=3, tau=tau, u_wall=U)
+    case.rho[:] = 1.2
+    o = oracle_mod.run_case(case, 2)
+    om = 1.0 / tau
+    cells = (case.Nx + 2) * (case.Nz + 2)
+    want = -cells * ((1 - om) * U / 3 + om * U / 18)
+    P = o.mea([0, case.Nx + 1, 0, case.Ny + 1, 0, case.Nz + 1])
+    assert abs(P[0] - want) < 1e-13 * cells
+    assert abs(P[1]) < 1e-12 * cells and abs(P[2]) < 1e-13 * cells
+
+
+def test_voxel_bodies_follow_the_generators():
+    """Generate_Cylinder / Generate_Sphere (lbm.cpp:1429-1450, 1759-1785):
+    cell centres (i - 1/2, ...) within D/2 of the centre; the solid count
+    approaches the disc area / ball volume; the MEA boxes are the int
+    conversions of centre -+ D (main.cpp:246, 369)."""
+    c = cases.cylinder_bb()
+    solid = c.type[1:-1, 1:-1, 1] == cases.TYPE_S
+    assert abs(solid.sum() / (math.pi * 10 ** 2) - 1) < 0.02
+    assert c.params["mea_box"] == (0, 40, 30, 70, 1, 1)
+    # later faces win (lbm.cpp:1468-1471): the j faces override the inlet's edges
+    assert (c.type[0, 1:-1] == cases.TYPE_IN).all() and (c.type[:, 0, :] == cases.TYPE_OUT).all()
+    s = cases.sphere_bb(80, 40, 40, D=16.0)
+    vol = (s.type == cases.TYPE_S).sum()
+    assert abs(vol / (4 / 3 * math.pi * 8 ** 3) - 1) < 0.03
+    cx, cy, cz = s.params["center"]
+    assert s.type[int(cx + 0.5), int(cy + 0.5), int(cz + 0.5)] == cases.TYPE_S
+    assert s.params["mea_box"] == (0, 32, 4, 36, 4, 36)
