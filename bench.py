@@ -285,6 +285,45 @@ def fsi_thermal_leg(steps, cmp_steps=500, warmup=20):
         finite=bool(np.isfinite(out).all()), f32_vs_f64=cmp)
 
 
+def sphere_leg(steps, every=100, storage="f32", warmup=20):
+    """The reference's Sphere_BB case at its own size (cases.sphere_bb
+    defaults: LBM(800,400,400), D = 20 at (160, 200, 200), u_max = 0.1,
+    Re = 200, inlet at i = 0, the other faces TYPE_OUT), bounce-back sphere:
+    fused steps with MEA_CD / MEA_CL over the reference's box every `every`
+    steps (main.cpp:368-372), wall-clock MLUPS including those evaluations,
+    and the coefficients' history from the impulsive start."""
+    case = cases.sphere_bb()
+    box, q = case.params["mea_box"], case.params["mea_norm"]
+    lat = lbm_amd.Lattice(case.Nx, case.Ny, case.Nz, case.tau, storage=storage)
+    lat.upload(**case.fields())
+    n = lat.nfluid
+    del case
+    lat.init()
+    lat.step(warmup)
+    code = lat.sync()
+    if code:
+        raise RuntimeError(f"device status {code} during warmup")
+    cd, cl = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps // every):
+        lat.step(every)
+        P = lat.momentum_exchange(box)
+        cd.append(float(P[0] / q))
+        cl.append(float(P[1] / q))
+    code = lat.sync()
+    dt = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    for _ in range(10):
+        lat.momentum_exchange(box)
+    mea_ms = (time.perf_counter() - t1) / 10 * 1e3
+    step_ms = lat.time_steps(50) / 50
+    lat.close()
+    return dict(workload="Sphere_BB 800x400x400 D3Q19, D=20, u_max=0.1, Re=200, bounce-back, FP32",
+                storage=storage, status=code, steps=steps, mlups_with_mea=round(n * steps / dt / 1e6, 1),
+                step_ms=round(step_ms, 4), mlups=round(n / (step_ms / 1e3) / 1e6, 1), mea_ms=round(mea_ms, 3),
+                cd=[round(v, 5) for v in cd], cl=[round(v, 6) for v in cl])
+
+
 def batch_leg(steps, storages=("f32", "f16s"), count=96, tune_steps=10):
     """The cavity ensemble: `count` lid-driven cavities, edges drawn from
     {32, 48, 64, 96, 128}, lid velocity and Re from their ranges (seeded).
@@ -409,6 +448,9 @@ def main():
     ap.add_argument("--no-q7", action="store_true")
     ap.add_argument("--ibm-only", action="store_true", help="run only the IBM channel leg")
     ap.add_argument("--fsi-steps", type=int, default=10000)
+    ap.add_argument("--sphere-steps", type=int, default=2000)
+    ap.add_argument("--no-sphere", action="store_true")
+    ap.add_argument("--sphere-only", action="store_true", help="run only the Sphere_BB leg")
     ap.add_argument("--no-fsi", action="store_true")
     ap.add_argument("--fsi-only", action="store_true", help="run only the heated-channel filament leg")
     ap.add_argument("--batch-steps", type=int, default=4000)
@@ -435,6 +477,9 @@ def main():
         return 0
     if a.fsi_only:
         print(json.dumps(fsi_thermal_leg(a.fsi_steps)), flush=True)
+        return 0
+    if a.sphere_only:
+        print(json.dumps(sphere_leg(a.sphere_steps)), flush=True)
         return 0
 
     case = cases.taylor_green(a.size, u0=0.04, Re=1600.0)
@@ -499,6 +544,12 @@ def main():
         fsi = fsi_thermal_leg(a.fsi_steps)
         log(f"bench: fsi {json.dumps(fsi)}")
 
+    sphere = None
+    if not a.no_sphere:
+        log(f"bench: Sphere_BB 800x400x400 leg, {a.sphere_steps} steps")
+        sphere = sphere_leg(a.sphere_steps)
+        log(f"bench: sphere {json.dumps(sphere)}")
+
     batch = None
     if not a.no_batch:
         log(f"bench: cavity ensemble leg, {a.batch_steps} steps")
@@ -547,6 +598,7 @@ def main():
         "d3q7": q7,
         "ibm": ibm,
         "fsi_thermal": fsi,
+        "sphere_bb": sphere,
         "batch": batch,
         "cpu_baseline": cpu,
     }

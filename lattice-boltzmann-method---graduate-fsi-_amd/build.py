@@ -51,10 +51,11 @@ HOST_SOURCES = [
     os.path.join(HERE, "host", "IBM.cpp"),
 ]
 # reference-style drivers on the drop-in classes (host code only)
-EXAMPLES = ["example_cavity", "example_cylinder_ibm", "example_couette_thermal"]
+EXAMPLES = ["example_cavity", "example_cylinder_ibm", "example_couette_thermal", "example_cylinder_bb"]
 EXAMPLE_BIN = {"example_cavity": "lbm_example_cavity",
                "example_cylinder_ibm": "lbm_example_cylinder_ibm",
-               "example_couette_thermal": "lbm_example_couette_thermal"}
+               "example_couette_thermal": "lbm_example_couette_thermal",
+               "example_cylinder_bb": "lbm_example_cylinder_bb"}
 
 
 def _run(cmd: list[str]) -> None:

@@ -220,6 +220,27 @@ lbm_vtk_desc g_vtk = {1.0, 0.0, 1.0, 0, 0, 0, 0, 0, {0, 0, 0}};
 
 void VTK_SetCase(const lbm_vtk_desc &desc) { g_vtk = desc; }
 
+namespace {
+double g_mea_norm = 0.5 * 1.0 * 0.1 * 0.1 * 20.0;  // Cylinder_Basic: rho0 1, u_max 0.1, D 20
+
+double mea_sum(LBM &lb, int xmin, int xmax, int ymin, int ymax, int zmin, int zmax, int comp) {
+  const int32_t box[6] = {xmin, xmax, ymin, ymax, zmin, zmax};
+  double P[3];
+  check(lbm_momentum_exchange(lb.ctx(), box, P), "lbm_momentum_exchange");
+  return P[comp];
+}
+}  // namespace
+
+void MEA_SetCase(double rho0, double u_max, double ref) { g_mea_norm = 0.5 * rho0 * u_max * u_max * ref; }
+
+double MEA_CL(LBM &lb, int xmin, int xmax, int ymin, int ymax, int zmin, int zmax) {
+  return mea_sum(lb, xmin, xmax, ymin, ymax, zmin, zmax, 1) / g_mea_norm;
+}
+
+double MEA_CD(LBM &lb, int xmin, int xmax, int ymin, int ymax, int zmin, int zmax) {
+  return mea_sum(lb, xmin, xmax, ymin, ymax, zmin, zmax, 0) / g_mea_norm;
+}
+
 void OutputVTK(int &nout, LBM &lb) {
   lbm_vtk_desc d = g_vtk;
   d.force = lbm_ibm_box(lb.ctx(), nullptr, nullptr) == LBM_OK ? 1 : 0;  // the IBM build

@@ -95,6 +95,17 @@ class LBM {
   lbm_ctx *ctx() const;
 };
 
+// MEA_CL / MEA_CD (lbm.cpp:3085-3163, LBM.h:790-792): lift and drag
+// coefficients from the momentum exchange on the bounce-back links in the
+// box, evaluated on the device after Streaming (as the reference calls them
+// after the step's MacroProp).  The normalisation 1/2 rho0 u_max^2 ref --
+// ref = D for a cylinder, 2D/sqrt(3) for the tandem triangle, D^2 PI/4 for a
+// sphere -- comes from MEA_SetCase (the reference's compile-time case
+// constants, lbm.cpp:3113-3120).
+void MEA_SetCase(double rho0, double u_max, double ref);
+double MEA_CL(LBM &lb, int xmin, int xmax, int ymin, int ymax, int zmin, int zmax);
+double MEA_CD(LBM &lb, int xmin, int xmax, int ymin, int ymax, int zmin, int zmax);
+
 // OutputVTK(int&, LBM&), lbm.cpp:2290-2468: writes ./field%06d.vtr (nout)
 // from the lattice's last MacroProp.  The velocity scale, the block names
 // and the optional blocks come from VTK_SetCase (the reference's compile-time
