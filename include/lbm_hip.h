@@ -212,6 +212,27 @@ int lbm_ibm_markers(lbm_ctx *ctx, double *x, double *y, double *z, double *ub,
  * after the last IBLBM). */
 int lbm_ibm_box(lbm_ctx *ctx, int32_t box[6], double center[2]);
 
+/* ---- MORE_DETAILS fields (LBM.h:55) ---------------------------------------
+ * MacroProp's pressure = rho/3 and viscous stress
+ * sigma_ab = -(tau - 1/2)/tau sum_l f_l (c_la - u_a)(c_lb - u_b)
+ * (lbm.cpp:681-705) of every cell, from the state the last MacroProp saw
+ * (after lbm_step or lbm_macro).  sigma: ncells*9 ([cell][a][b]), pressure:
+ * ncells; either may be NULL. */
+int lbm_download_stress(lbm_ctx *ctx, double *sigma, double *pressure);
+/* LBM::Sup_MacroProp(n) (lbm.cpp:865-939; called every Nt steps after the
+ * step's MacroProp, main.cpp:389-391): running averages and rms of u, of
+ * the pressure and of the fluctuation products, told = (n - nt) dt, on the
+ * device in FP64 from the last MacroProp's rho, u.  The first call
+ * allocates the 26 FP64 fields per cell (zeros, the LATTICE defaults). */
+int lbm_stats_update(lbm_ctx *ctx, int32_t n, int32_t nt);
+/* The statistics in fluid[i][j][k] order: avg_u, rms_u ncells*3; avg_p,
+ * rms_p ncells; avg_rs, rms_rs ncells*9 ([cell][a][b]).  NULL skips a field.
+ * The reference's vorticity statistics stay zero (its vorticity is never
+ * computed, lbm.cpp:707-762) and are not kept.  LBM_ERR_STATE before the
+ * first lbm_stats_update. */
+int lbm_download_stats(lbm_ctx *ctx, double *avg_u, double *rms_u, double *avg_p, double *rms_p,
+                       double *avg_rs, double *rms_rs);
+
 /* ---- momentum exchange on bounce-back bodies ------------------------------
  * MEA_CD and MEA_CL (lbm.cpp:3085-3163, called at main.cpp:246-249 and
  * 369-372 after Collision, Streaming and MacroProp) as one FP64 reduction:

@@ -112,7 +112,7 @@ static void free_all(lbm_ctx *c) {
   thermal_free(c);
   void *ptrs[] = {c->cur, c->oth, c->type, c->mask, c->links, c->uw[0], c->uw[1], c->uw[2],
                   c->mac[0], c->mac[1], c->mac[2], c->mac[3], c->F[0], c->F[1], c->F[2],
-                  c->status, c->first_bad, c->red_partial, c->red_out};
+                  c->status, c->first_bad, c->red_partial, c->red_out, c->stats};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -38,6 +38,9 @@ struct lbm_ctx {
   // the last buffer operation was a streaming that pulled from oth: oth holds
   // the f* that produced cur's f (what the momentum exchange reads)
   int fstar_ready = 0;
+  // running statistics of Sup_MacroProp (lbm_stats_update): STAT_PLANES
+  // FP64 planes of ncell values, allocated on the first update
+  double *stats = nullptr;
   int uploaded = 0;
   // The reference's first Streaming reads the solids' setup velocities; from
   // the first MacroProp on (lbm.cpp:672-675 runs over solids too) it reads

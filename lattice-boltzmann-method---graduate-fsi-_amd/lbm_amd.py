@@ -176,6 +176,9 @@ _SIGNATURES = {
     "lbm_fsi_setup": (C.c_int, [_P, _P, C.POINTER(_IbmDesc)]),
     "lbm_fsi_run": (C.c_int, [_P, C.c_int32, _D]),
     "lbm_momentum_exchange": (C.c_int, [_P, C.POINTER(C.c_int32), _D]),
+    "lbm_download_stress": (C.c_int, [_P, _D, _D]),
+    "lbm_stats_update": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "lbm_download_stats": (C.c_int, [_P, _D, _D, _D, _D, _D, _D]),
     "lbm_filament_create": (C.c_int, [C.c_int32, C.POINTER(_FilamentDesc), C.POINTER(_P)]),
     "lbm_filament_destroy": (C.c_int, [_P]),
     "lbm_filament_substeps": (C.c_int, [_P]),
@@ -470,6 +473,26 @@ class Lattice:
                 ("x", "y", "z", "nx", "ny", "nz", "ds")]
         self._ibm_n = n
         _check(self._lib.lbm_ibm_setup(self._h, C.byref(d), *[_dp(a) for a in arrs]), "lbm_ibm_setup")
+
+    def stress(self):
+        """MORE_DETAILS: (sigma (..., 3, 3), pressure) of every cell from the
+        last MacroProp's state (lbm.cpp:681-705)."""
+        sig = np.empty(self.shape + (3, 3))
+        p = np.empty(self.shape)
+        _check(self._lib.lbm_download_stress(self._h, _dp(sig), _dp(p)), "lbm_download_stress")
+        return sig, p
+
+    def stats_update(self, n, nt):
+        """Sup_MacroProp(n) with output interval nt (lbm.cpp:865-939)."""
+        _check(self._lib.lbm_stats_update(self._h, int(n), int(nt)), "lbm_stats_update")
+
+    def stats(self):
+        """(avg_u, rms_u, avg_p, rms_p, avg_rs, rms_rs), shapes (...,3),
+        (...,3), (...), (...), (...,3,3), (...,3,3)."""
+        out = [np.empty(self.shape + (3,)), np.empty(self.shape + (3,)), np.empty(self.shape),
+               np.empty(self.shape), np.empty(self.shape + (3, 3)), np.empty(self.shape + (3, 3))]
+        _check(self._lib.lbm_download_stats(self._h, *[_dp(a) for a in out]), "lbm_download_stats")
+        return tuple(out)
 
     def momentum_exchange(self, box):
         """MEA_CD / MEA_CL's raw momentum sums (Px, Py, Pz) over the

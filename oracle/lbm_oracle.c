@@ -59,6 +59,7 @@ struct orc_lbm {
   double *Fm;        /* implicit-IBM Fm[3] per cell (LBM.h:571-573) */
   struct orc_thermal *th; /* ENERGY fields (LBM.h:590-596), NULL when off */
   struct orc_q7 *q7;      /* D3Q7 temperature (no reference counterpart) */
+  double *stats;          /* MORE_DETAILS averages and rms: 26 per cell, [cell][26] */
 };
 
 /* D3Q7 temperature (declared in lbm_oracle.h with its specification) */
@@ -161,6 +162,7 @@ void orc_destroy(orc_lbm *lb) {
   }
   free(lb->fluid);
   free(lb->Fm);
+  free(lb->stats);
   if (lb->th) {
     free(lb->th->g); free(lb->th->g_star); free(lb->th->F_vis);
     free(lb->th->T); free(lb->th->q); free(lb->th->typeE);
@@ -932,6 +934,95 @@ static void ibm_interpolate(orc_lbm *lb, int nm, const double *mx, const double 
           ub[3 * n + 1] += c->uy * orc_ibm_kernel(rx) * orc_ibm_kernel(ry) * orc_ibm_kernel(rz);
           ub[3 * n + 2] += c->uz * orc_ibm_kernel(rx) * orc_ibm_kernel(ry) * orc_ibm_kernel(rz);
         }
+  }
+}
+
+/* MacroProp's MORE_DETAILS block, lbm.cpp:681-705: pressure = rho/3 and the
+ * viscous stress of every cell from its f and u, the reference's loop and
+ * product order. */
+void orc_stress(const orc_lbm *lb, double *sigma, double *pressure) {
+  const int Nx = lb->Nx, Ny = lb->Ny, Nz = lb->Nz;
+  const double tau = lb->tau;
+  for (int i = 0; i <= Nx + 1; i++)
+    for (int j = 0; j <= Ny + 1; j++)
+      for (int k = 0; k <= Nz + 1; k++) {
+        const orc_cell *c = &lb->fluid[i][j][k];
+        const int64_t n = LIN(lb, i, j, k);
+        double sig[3][3];
+        for (int alphaa = 0; alphaa < 3; alphaa++)
+          for (int beta = alphaa; beta < 3; beta++) {
+            const int *ca = alphaa == 0 ? lb->cx : alphaa == 1 ? lb->cy : lb->cz;
+            const int *cb = beta == 0 ? lb->cx : beta == 1 ? lb->cy : lb->cz;
+            const double ua = alphaa == 0 ? c->ux : alphaa == 1 ? c->uy : c->uz;
+            const double ub = beta == 0 ? c->ux : beta == 1 ? c->uy : c->uz;
+            double sum_sigma = 0;
+            for (int l = 0; l < lb->q; l++) sum_sigma += c->f[l] * (ca[l] - ua) * (cb[l] - ub);
+            sig[alphaa][beta] = -(tau - 0.5) * sum_sigma / tau;
+            sig[beta][alphaa] = sig[alphaa][beta];
+          }
+        for (int a = 0; a < 3; a++)
+          for (int b = 0; b < 3; b++) sigma[n * 9 + 3 * a + b] = sig[a][b];
+        pressure[n] = c->density / 3.0;
+      }
+}
+
+/* LBM::Sup_MacroProp(n), lbm.cpp:865-939 (dt = 1, told = (n - Nt) dt), per
+ * cell in the reference's statement order: averages first (the new average
+ * of u enters the fluctuation products), then the rms values, with the
+ * reference's forms kept (rms_pressure weights pressure^2 by told; the rms of
+ * the Reynolds stress compares the raw products u_a u_b with the averaged
+ * fluctuation products).  Layout per cell: avg_u[3], rms_u[3], avg_p,
+ * rms_p, avg_rs[9], rms_rs[9].  Vorticity is never computed by the
+ * reference (lbm.cpp:707-762 is commented out), so its statistics stay 0
+ * and are not kept. */
+void orc_stats_update(orc_lbm *lb, int n_step, int Nt) {
+  const int64_t ncell = orc_ncells(lb);
+  if (!lb->stats) lb->stats = (double *)calloc((size_t)ncell * 26, sizeof(double));
+  const double dt = 1.0, told = (n_step - Nt) * dt;
+  for (int64_t n = 0; n < ncell; n++) {
+    const int64_t i = n / ((int64_t)(lb->Ny + 2) * (lb->Nz + 2));
+    const int64_t j = (n / (lb->Nz + 2)) % (lb->Ny + 2), k = n % (lb->Nz + 2);
+    const orc_cell *c = &lb->fluid[i][j][k];
+    double *s = lb->stats + n * 26;
+    double *avg_velocity = s, *rms_velocity = s + 3, *avg_pressure = s + 6, *rms_pressure = s + 7;
+    double(*avg_RS)[3] = (double(*)[3])(s + 8), (*rms_RS)[3] = (double(*)[3])(s + 17);
+    const double ux = c->ux, uy = c->uy, uz = c->uz, p = c->density / 3.0;
+    avg_velocity[0] = (avg_velocity[0] * told + ux * dt) / (told + dt);
+    avg_velocity[1] = (avg_velocity[1] * told + uy * dt) / (told + dt);
+    avg_velocity[2] = (avg_velocity[2] * told + uz * dt) / (told + dt);
+    *avg_pressure = (*avg_pressure * told + p * 1) / (told + 1);
+    const double ux_avg = avg_velocity[0], uy_avg = avg_velocity[1], uz_avg = avg_velocity[2];
+    const double d[3] = {ux - ux_avg, uy - uy_avg, uz - uz_avg};
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) avg_RS[a][b] = (avg_RS[a][b] * told + d[a] * d[b] * 1) / (told + 1);
+    for (int a = 0; a < 3; a++)
+      rms_velocity[a] = sqrt((pow(rms_velocity[a], 2) * told + pow(d[a], 2) * 1) / (told + 1));
+    *rms_pressure = sqrt((pow(p, 2) * told + pow(p - *avg_pressure, 2) * 1) / (told + 1));
+    const double uu = ux * ux, uv = ux * uy, uw = ux * uz, vv = uy * uy, vw = uy * uz, ww = uz * uz;
+    const double prod[3][3] = {{uu, uv, uw}, {uv, vv, vw}, {uw, vw, ww}};
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++)
+        rms_RS[a][b] = sqrt((pow(rms_RS[a][b], 2) * told + pow(prod[a][b] - avg_RS[a][b], 2) * 1) / (told + 1));
+  }
+}
+
+/* the statistics as [cell] arrays (3, 3, 1, 1, 9, 9 values per cell); zeros
+ * before the first update */
+void orc_get_stats(const orc_lbm *lb, double *avg_u, double *rms_u, double *avg_p, double *rms_p,
+                   double *avg_rs, double *rms_rs) {
+  const int64_t ncell = orc_ncells(lb);
+  for (int64_t n = 0; n < ncell; n++) {
+    const double *s = lb->stats ? lb->stats + n * 26 : NULL;
+    for (int q = 0; q < 3; q++) {
+      if (avg_u) avg_u[n * 3 + q] = s ? s[q] : 0.0;
+      if (rms_u) rms_u[n * 3 + q] = s ? s[3 + q] : 0.0;
+    }
+    if (avg_p) avg_p[n] = s ? s[6] : 0.0;
+    if (rms_p) rms_p[n] = s ? s[7] : 0.0;
+    for (int q = 0; q < 9; q++) {
+      if (avg_rs) avg_rs[n * 9 + q] = s ? s[8 + q] : 0.0;
+      if (rms_rs) rms_rs[n * 9 + q] = s ? s[17 + q] : 0.0;
+    }
   }
 }
 

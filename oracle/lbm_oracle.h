@@ -127,6 +127,17 @@ int orc_ibm_step(orc_lbm *lb, int nm, double *mx, double *my, double *mz,
                  const double vcm[3], const double vcm_before[3],
                  int xmin, int xmax, int ymin, int ymax, int zmin, int zmax,
                  int m_max, double vsolid, double *out);
+/* MORE_DETAILS (LBM.h:55).  orc_stress: MacroProp's pressure = rho/3 and
+ * viscous stress sigma_ab = -(tau - 1/2)/tau sum_l f_l (c_la - u_a)(c_lb -
+ * u_b) of every cell (lbm.cpp:681-705), sigma [cell][3][3], pressure [cell].
+ * orc_stats_update: Sup_MacroProp(n) (lbm.cpp:865-939) with Nt the output
+ * interval; orc_get_stats: avg_u, rms_u [cell][3], avg_p, rms_p [cell],
+ * avg_rs, rms_rs [cell][3][3]. */
+void orc_stress(const orc_lbm *lb, double *sigma, double *pressure);
+void orc_stats_update(orc_lbm *lb, int n, int Nt);
+void orc_get_stats(const orc_lbm *lb, double *avg_u, double *rms_u, double *avg_p, double *rms_p,
+                   double *avg_rs, double *rms_rs);
+
 /* MEA_CD / MEA_CL's momentum sums (lbm.cpp:3085-3163) over the inclusive
  * box {x0,x1,y0,y1,z0,z1}, evaluated where the reference does (after
  * Collision, Streaming, MacroProp: f* of the last collision, f streamed):

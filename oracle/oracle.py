@@ -59,6 +59,9 @@ _SIG = {
     "orc_q7_get_g": (None, [_P, _D]),
     "orc_fsi_ibm_step": (C.c_int, [_P, C.c_int, _D, _D, _D, _D, _D, _D, _D, _D] + [C.c_int] * 7),
     "orc_mea": (None, [_P, C.POINTER(C.c_int), _D]),
+    "orc_stress": (None, [_P, _D, _D]),
+    "orc_stats_update": (None, [_P, C.c_int, C.c_int]),
+    "orc_get_stats": (None, [_P, _D, _D, _D, _D, _D, _D]),
     "orc_fil_create": (_P, [C.c_int] + [C.c_double] * 4 + [_D, _D, _D, C.c_int]),
     "orc_fil_destroy": (None, [_P]),
     "orc_fil_substeps": (C.c_int, [_P]),
@@ -241,6 +244,25 @@ class Oracle:
                                         _dp(fsum), *[int(b) for b in box], int(m_max))
         assert err == 0, "a marker left the box"
         return ub, Fb, fsum
+
+    def stress(self):
+        """MORE_DETAILS: (sigma (..., 3, 3), pressure) of every cell."""
+        sig = np.empty(self.shape + (3, 3))
+        p = np.empty(self.shape)
+        self.lib.orc_stress(self.h, _dp(sig), _dp(p))
+        return sig, p
+
+    def stats_update(self, n, nt):
+        """Sup_MacroProp(n) with output interval nt."""
+        self.lib.orc_stats_update(self.h, int(n), int(nt))
+
+    def stats(self):
+        """(avg_u, rms_u, avg_p, rms_p, avg_rs, rms_rs), shapes (...,3),
+        (...,3), (...), (...), (...,3,3), (...,3,3)."""
+        out = [np.empty(self.shape + (3,)), np.empty(self.shape + (3,)), np.empty(self.shape),
+               np.empty(self.shape), np.empty(self.shape + (3, 3)), np.empty(self.shape + (3, 3))]
+        self.lib.orc_get_stats(self.h, *[_dp(a) for a in out])
+        return tuple(out)
 
     def mea(self, box):
         """MEA_CD / MEA_CL's raw momentum sums (Px, Py, Pz) over the box."""

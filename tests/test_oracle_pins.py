@@ -862,3 +862,84 @@ def test_voxel_bodies_follow_the_generators():
     cx, cy, cz = s.params["center"]
     assert s.type[int(cx + 0.5), int(cy + 0.5), int(cz + 0.5)] == cases.TYPE_S
     assert s.params["mea_box"] == (0, 32, 4, 36, 4, 36)
+
+
+# ---- MORE_DETAILS: stress and running statistics (lbm.cpp:681-705, 865-939) -------
+def test_stress_of_equilibrium_is_isotropic(oracle_mod):
+    """f = feq(rho, u) (Init, then MacroProp recovers u exactly): the second
+    central moment of the D3Q19 cubic equilibrium is rho cs2 delta (the
+    cubic terms are odd), so sigma = -(tau - 1/2)/tau rho/3 delta and the
+    pressure is rho/3.  Catches a missing u subtraction (rho u u terms), a
+    wrong (tau - 1/2)/tau factor and a wrong sign."""
+    case = cases.random_mask_case(5, 4, 3, seed=8, p_solid=0.0)
+    rng = np.random.default_rng(8)
+    case.rho[:] = 1.0 + 0.1 * rng.standard_normal(case.shape)
+    for comp in (case.ux, case.uy, case.uz):
+        comp[:] = 0.04 * rng.standard_normal(case.shape)
+    o = oracle_mod.run_case(case, 0)
+    assert o.macroprop() == 0
+    sig, p = o.stress()
+    rho = o.macro()[0]
+    want = -(case.tau - 0.5) / case.tau * rho / 3
+    assert np.abs(sig - want[..., None, None] * np.eye(3)).max() < 1e-14
+    assert np.abs(p - rho / 3).max() < 1e-15
+
+
+def test_stress_in_steady_couette_is_rho_nu_shear(oracle_mod):
+    """Steady Couette flow (the reference's validation case, lbm.cpp:3238-
+    3251): sigma_xy = rho nu U / H with H = Ny between half-way walls, to
+    1e-8 (the reference's closed form uses h = Ny + 1).  Catches a wrong
+    viscosity factor and the xy/yx assignment."""
+    U, tau, Ny = 0.001, 0.8, 10
+    case = cases.couette(Nx=2, Ny=Ny, Nz=2, tau=tau, u_wall=U)
+    o = oracle_mod.run_case(case, 8000)
+    sig, _ = o.stress()
+    want = (tau - 0.5) / 3 * U / Ny
+    s = sig[1, 1:-1, 1]
+    assert np.abs(s[:, 0, 1] / want - 1).max() < 1e-8
+    assert np.array_equal(s[:, 0, 1], s[:, 1, 0])
+
+
+def test_stats_of_a_steady_field_closed_forms(oracle_mod):
+    """Sup_MacroProp on a field that does not change (every Nt = 10 steps,
+    n = 10..60): the averages equal the field (the weights sum to one),
+    the velocity rms and the averaged fluctuation products vanish, and the
+    reference's forms give rms_p = p sqrt(told/(told + 1)) (pressure^2
+    weighted by told) and rms_rs = |u_a u_b| (raw products against the
+    averaged fluctuations)."""
+    case = cases.random_mask_case(4, 3, 3, seed=5)
+    rng = np.random.default_rng(5)
+    o = oracle_mod.Oracle(case.Nx, case.Ny, case.Nz, case.tau)
+    rho = 1.0 + 0.05 * rng.random(case.shape)
+    u = [0.03 * rng.standard_normal(case.shape) for _ in range(3)]
+    o.set_cells(type=case.type, rho=rho, ux=u[0], uy=u[1], uz=u[2])
+    nt = 10
+    for n in range(nt, 7 * nt, nt):
+        o.stats_update(n, nt)
+    avg_u, rms_u, avg_p, rms_p, avg_rs, rms_rs = o.stats()
+    U = np.stack(u, -1)
+    told = 6 * nt - nt
+    assert np.abs(avg_u - U).max() < 1e-16
+    assert np.abs(rms_u).max() < 1e-16 and np.abs(avg_rs).max() < 1e-18
+    assert np.abs(avg_p - rho / 3).max() < 1e-15
+    assert np.abs(rms_p - rho / 3 * np.sqrt(told / (told + 1))).max() < 1e-15
+    assert np.abs(rms_rs - np.abs(U[..., :, None] * U[..., None, :])).max() < 1e-17
+
+
+def test_stats_running_mean_of_a_linear_signal(oracle_mod):
+    """Nt = 1 (told = n - 1): u_x = n d at update n gives the arithmetic mean
+    (n + 1) d / 2, and the reference's running rms (each sample against the
+    average at its own time) rms_k^2 = (k - 1)(2k - 1) d^2 / 24, the same
+    value as the averaged product avg_rs_xx."""
+    case = cases.random_mask_case(3, 3, 3, seed=2, p_solid=0.0)
+    o = oracle_mod.Oracle(case.Nx, case.Ny, case.Nz, case.tau)
+    d = 0.01
+    K = 9
+    for n in range(1, K + 1):
+        o.set_cells(type=case.type, ux=np.full(case.shape, n * d))
+        o.stats_update(n, 1)
+    avg_u, rms_u, _, _, avg_rs, _ = o.stats()
+    assert np.abs(avg_u[..., 0] - (K + 1) * d / 2).max() < 1e-16
+    want = (K - 1) * (2 * K - 1) * d * d / 24
+    assert np.abs(rms_u[..., 0] - np.sqrt(want)).max() < 1e-16
+    assert np.abs(avg_rs[..., 0, 0] - want).max() < 1e-18
