@@ -2,6 +2,8 @@
 #include "IBM.h"
 
 #include <cmath>
+#include <fstream>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 
@@ -131,4 +133,29 @@ void IBLBM(LBM &lb, std::vector<MARKER> &marker, std::vector<double> &ds, std::v
   const double q = 0.5 * c.rho0 * c.u_max * c.u_max * L;
   Cl.push_back(out[1] / q);
   Cd.push_back(out[0] / q);
+}
+
+void OutputVTK_Marker(int &time, std::vector<MARKER> &marker) {
+  const IBM_CASE &c = g_case;
+  const int number_nodes = c.number_nodes;
+  std::stringstream output_filename;
+  output_filename << "wall_t " << int(time / c.Nt) << ".vtk";
+  std::ofstream out(output_filename.str().c_str());
+  out << "# vtk DataFile Version 3.0\n";
+  out << "wall_state\n";
+  out << "ASCII\n";
+  out << "DATASET POLYDATA\n";
+  out << "POINTS " << number_nodes << " float\n";
+  for (int n = 0; n < number_nodes; n++) out << marker[n].x << " " << marker[n].y << " " << marker[n].z << " \n";
+  out << "LINES " << number_nodes << " " << 3 * number_nodes << "\n";
+  for (int n = 0; n < number_nodes; ++n) out << "2 " << n % number_nodes << " " << (n + 1) % number_nodes << "\n";
+  out << "VERTICES 1 " << number_nodes + 1 << "\n";
+  out << number_nodes << " ";
+  for (int n = 0; n < number_nodes; n++) out << n << " ";
+}
+
+void OutputCSV(std::vector<double> &values, const std::string &name, double time_per_sample) {
+  std::ofstream ofs(name);
+  ofs << "time,error\n";
+  for (size_t i = 0; i < values.size(); i++) ofs << (i + 1) * time_per_sample << "," << values[i] << "\n";
 }

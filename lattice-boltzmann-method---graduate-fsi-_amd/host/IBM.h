@@ -17,6 +17,7 @@
 // ds and the marker normals are read at the first IBLBM call on a lattice.
 #pragma once
 
+#include <string>
 #include <vector>
 
 #include "LBM.h"
@@ -58,6 +59,7 @@ struct IBM_CASE {
   double amplitude = 0;       // TRANSVERSE_OSCILLATION
   double coef_length = 0;     // C_L, C_D length: D (0 selects D), D + dx for inline oscillation
   int zmin = 1, zmax = 1;     // box z range (the reference's 2D slab: 1..1)
+  int Nt = 10;                // output interval (steps), names the marker files
 };
 
 void IBM_SetCase(const IBM_CASE &c);
@@ -77,3 +79,14 @@ void Generate_Cylinder_Marker(std::vector<MARKER> &marker);
 // device and appends C_L and C_D.  The next lb.Collision() uses its result.
 void IBLBM(LBM &lb, std::vector<MARKER> &marker, std::vector<double> &ds, std::vector<double> &Cl,
            std::vector<double> &Cd, double V_CM[], double t);
+
+// OutputVTK_Marker, lbm.cpp:3033-3077: "wall_t <time/Nt>.vtk" in the working
+// directory, ASCII POLYDATA with the first number_nodes markers as POINTS, a
+// closed LINES loop and one VERTICES cell (the positions IBLBM last moved).
+void OutputVTK_Marker(int &time, std::vector<MARKER> &marker);
+
+// OutputCSV, lbm.cpp:3169-3186: "time,error" then one row per value, the
+// time column (i + 1) * time_per_sample.  The reference chooses that factor by
+// case (Nt dt for the analytic cases, dt u_max/D for the IBM cylinder,
+// dt frequency for the inline oscillation, Nt dt u_max/D otherwise).
+void OutputCSV(std::vector<double> &values, const std::string &name, double time_per_sample);

@@ -96,6 +96,11 @@ int main(int argc, char **argv) {
     lb.Streaming();
   }
   lb.SyncMirror(false);
+  // main.cpp:236 and 254-257: the marker file and the coefficient series
+  int time = numstep;
+  OutputVTK_Marker(time, marker);
+  OutputCSV(CD, "Cd.csv", dt * u_max / D);
+  OutputCSV(CL, "Cl.csv", dt * u_max / D);
 
   FILE *fp = std::fopen(argv[5], "wb");
   if (!fp) return 1;

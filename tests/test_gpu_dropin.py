@@ -74,6 +74,32 @@ def _oracle_cylinder(oracle_mod, Nx, Ny, D, steps, motion):
     return np.array(cd), np.array(cl), o, case
 
 
+def _check_marker_files(tmp_path, Nx, Ny, D, steps, motion, out):
+    """OutputVTK_Marker (lbm.cpp:3033-3077) and OutputCSV (lbm.cpp:3169-3186)
+    as the driver wrote them: 'wall_t <steps/Nt>.vtk' with the markers as
+    POINTS, a closed LINES loop and one VERTICES cell; 'Cd.csv' with
+    time = (i + 1) dt u_max / D and the coefficients (6 significant
+    digits, the stream default)."""
+    n = int(round(cases.PI_REF * D))
+    lines = (tmp_path / f"wall_t {steps // 10}.vtk").read_text().split("\n")
+    assert lines[:5] == ["# vtk DataFile Version 3.0", "wall_state", "ASCII", "DATASET POLYDATA",
+                         f"POINTS {n} float"]
+    pts = np.array([[float(v) for v in ln.split()] for ln in lines[5:5 + n]])
+    if motion == "fixed":
+        mk = cases.cylinder_markers(Nx / 5.0, Ny / 2.0, D, n, np.array([1.0]))
+        assert np.abs(pts - np.stack([mk["x"], mk["y"], mk["z"]], 1)).max() < 1e-4
+    assert lines[5 + n] == f"LINES {n} {3 * n}"
+    assert lines[6 + n:6 + 2 * n] == [f"2 {i} {(i + 1) % n}" for i in range(n)]
+    assert lines[6 + 2 * n] == f"VERTICES 1 {n + 1}"
+    assert lines[7 + 2 * n].split() == [str(n)] + [str(i) for i in range(n)]
+    rows = (tmp_path / "Cd.csv").read_text().split()
+    assert rows[0] == "time,error" and len(rows) == steps + 1
+    cd = np.fromfile(out, dtype=np.float64)[:steps]
+    for i, row in enumerate(rows[1:]):
+        t, v = (float(x) for x in row.split(","))
+        assert abs(t - (i + 1) * 0.04 / D) < 1e-6 * (i + 1) and abs(v - cd[i]) <= 1e-5 * max(1.0, abs(cd[i]))
+
+
 @pytest.mark.parametrize("motion", ["fixed", "inline", "transverse"])
 def test_dropin_iblbm_cylinder_matches_oracle(oracle_mod, tmp_path, motion):
     """The reference's IBM driver (IBLBM, Collision, Streaming) on the drop-in
@@ -82,8 +108,9 @@ def test_dropin_iblbm_cylinder_matches_oracle(oracle_mod, tmp_path, motion):
     out = tmp_path / "cyl.bin"
     args = [EXE_IBM, str(Nx), str(Ny), str(D), str(steps), str(out), "f64", motion]
     print("running", args, flush=True)
-    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120, cwd=tmp_path)
     assert r.returncode == 0, r.stderr
+    _check_marker_files(tmp_path, Nx, Ny, D, steps, motion, out)
     cd_ref, cl_ref, o, case = _oracle_cylinder(oracle_mod, Nx, Ny, D, steps, motion)
     data = np.fromfile(out, dtype=np.float64)
     cd, cl = data[:steps], data[steps:2 * steps]
