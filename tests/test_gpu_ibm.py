@@ -284,3 +284,43 @@ def test_ibm_call_order_and_box_checks():
     lat.step(2)
     assert lat.sync() == 0
     lat.close()
+
+
+@pytest.mark.parametrize("path", ["fused", "apply"])
+def test_ibm_with_d3q7_buoyancy_fp64(oracle_mod, path):
+    """The rigid IBM on a D3Q19 + D3Q7 context: each IBM step ends with the
+    D3Q7 temperature of every fluid cell and its Boussinesq force added to
+    the IBM's force and velocity; the oracle's IBM step does the same."""
+    case, mk, box = cases.ibm_cylinder(48, 36, 1, D=8.0, Re=20.0, perturb=0.0, center=(14.0, 18.0))
+    rng = np.random.default_rng(31)
+    T = 0.2 + 0.6 * rng.random(case.shape)
+    th = cases.q7_thermal(T, np.zeros(case.shape, np.int16), 0.9, boussinesq=True, g_beta=1e-3, T_ref=0.5)
+    nsteps = 12
+    o = oracle_mod.Oracle(case.Nx, case.Ny, case.Nz, case.tau)
+    o.q7_enable(th.tau_g, True, th.g_beta, th.T_ref)
+    o.set_cells(**case.fields())
+    o.set_q7(th.T, th.type_energy)
+    o.init()
+    omk = {k: v.copy() for k, v in mk.items()}
+    ref = _oracle_run(o, omk, box, np.zeros((nsteps + 1, 3)))
+    lat = lbm_amd.Lattice(case.Nx, case.Ny, case.Nz, case.tau, storage="f64")
+    lat.upload(**case.fields())
+    lat.upload_thermal(th.T, th.type_energy, tau_g=th.tau_g, boussinesq=True, g_beta=th.g_beta,
+                       T_ref=th.T_ref, model="d3q7")
+    lat.init()
+    lat.ibm_setup(mk, box, m_max=20)
+    if path == "fused":
+        got = lat.ibm_run(nsteps)
+    else:
+        got = []
+        for _ in range(nsteps):
+            got.append(lat.ibm_apply())
+            lat.collision()
+            lat.streaming()
+        got = np.array(got)
+    assert lat.sync() == 0
+    _cmp_series(got, ref, 1e-11, "d3q7")
+    _cmp_fields(lat, o, case, 1e-11, 1e-12)
+    fl = case.type == cases.TYPE_F
+    assert np.abs(lat.thermal()[0] - o.q7_thermal()[0])[fl].max() < 1e-12
+    lat.close()
