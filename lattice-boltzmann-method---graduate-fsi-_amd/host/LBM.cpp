@@ -1,6 +1,7 @@
 // LBM.cpp -- the drop-in LBM class over the C-ABI (see LBM.h).
 #include "LBM.h"
 
+#include <cmath>
 #include <cstdio>
 #include <exception>
 #include <iostream>
@@ -208,6 +209,8 @@ void LBM::SyncMirror(bool populations) {
       }
 }
 
+void LBM::Sup_MacroProp(int n, int Nt) { check(lbm_stats_update(impl_->ctx, n, Nt), "Sup_MacroProp"); }
+
 int LBM::getNx() const { return Nx; }
 int LBM::getNy() const { return Ny; }
 int LBM::getNz() const { return Nz; }
@@ -248,4 +251,111 @@ void OutputVTK(int &nout, LBM &lb) {
   char filename[128];
   std::snprintf(filename, sizeof filename, "./field%06d.vtr", nout);
   check(lbm_output_vtk(lb.ctx(), &d, filename), "OutputVTK");
+}
+
+void OutputAVGVTK(int &nout, LBM &lb) {
+  const int Nx = lb.getNx(), Ny = lb.getNy(), Nz = lb.getNz();
+  const size_t nx = Nx + 2, ny = Ny + 2, nz = Nz + 2, n = nx * ny * nz;
+  std::vector<double> rho(n), ux(n), uy(n), uz(n), sig(9 * n), p(n);
+  std::vector<double> au(3 * n, 0.0), ru(3 * n, 0.0), ap(n, 0.0), rp(n, 0.0), ars(9 * n, 0.0), rrs(9 * n, 0.0);
+  lbm_ctx *c = lb.ctx();
+  check(lbm_download_macro(c, rho.data(), ux.data(), uy.data(), uz.data()), "OutputAVGVTK");
+  check(lbm_download_stress(c, sig.data(), p.data()), "OutputAVGVTK");
+  const int e = lbm_download_stats(c, au.data(), ru.data(), ap.data(), rp.data(), ars.data(), rrs.data());
+  if (e != LBM_ERR_STATE) check(e, "OutputAVGVTK");  // before any Sup_MacroProp: the LATTICE zeros
+  const double u_max = g_vtk.u_scale, D = 1.0, dx = 1.0, dy = 1.0, dz = 1.0;
+  const int zcomp = Nz;
+  char filename[128];
+  std::snprintf(filename, sizeof filename, "./avgfield%06d.vtr", nout);
+  FILE *fp = std::fopen(filename, "wb");
+  if (!fp) throw std::runtime_error(std::string("OutputAVGVTK: cannot open ") + filename);
+  const long cells = (long)(Nx + 2) * (Ny + 2) * zcomp;
+  unsigned long offset = 0;
+  std::fprintf(fp, "<?xml version=\"1.0\"?>\n");
+  std::fprintf(fp, "<VTKFile type=\"RectilinearGrid\" version=\"1.0\" byte_order=\"LittleEndian\" header_type=\"UInt32\">\n");
+  std::fprintf(fp, "  <RectilinearGrid WholeExtent=\"0 %d 0 %d 0 %d\">\n", Nx + 2, Ny + 2, zcomp);
+  std::fprintf(fp, "  <Piece Extent=\"0 %d 0 %d 0 %d\">\n", Nx + 2, Ny + 2, zcomp);
+  std::fprintf(fp, "    <PointData>\n");
+  std::fprintf(fp, "    </PointData>\n");
+  std::fprintf(fp, "    <CellData>\n");
+  struct Block {
+    const char *name;
+    int comps;
+  };
+  const Block blocks[] = {{"Density", 1},          {"Velocity", 3},         {"Avg velocity", 3},
+                          {"Rms velocity", 3},     {"Pressure", 1},         {"Avg pressure", 1},
+                          {"Rms pressure", 1},     {"Viscous Stress", 9},   {"Vorticity", 3},
+                          {"Avg vorticity", 3},    {"Rms vorticity", 3},    {"Avg reynolds Stress", 9},
+                          {"Rms reynolds Stress", 9}};
+  for (const Block &b : blocks) {
+    if (b.comps == 1)
+      std::fprintf(fp, "      <DataArray type=\"Float32\" Name=\"%s\" format=\"appended\" offset=\"%ld\" />\n", b.name,
+                   (long)offset);
+    else
+      std::fprintf(fp,
+                   "      <DataArray type=\"Float32\" Name=\"%s\" NumberOfComponents=\"%d\" format=\"appended\" "
+                   "offset=\"%ld\" />\n",
+                   b.name, b.comps, (long)offset);
+    offset += 4 + b.comps * 4 * cells;
+  }
+  std::fprintf(fp, "      <DataArray type=\"Float32\" Name=\"CellType\" format=\"appended\" offset=\"%ld\" />\n", (long)offset);
+  offset += 4 + 1 * 4 * cells;
+  std::fprintf(fp, "    </CellData>\n");
+  std::fprintf(fp, "    <Coordinates>\n");
+  std::fprintf(fp, "      <DataArray type=\"Float32\" Name=\"CoordinateX\" format=\"appended\" offset=\"%ld\" />\n", (long)offset);
+  offset += 4 + 1 * 4 * (Nx + 3);
+  std::fprintf(fp, "      <DataArray type=\"Float32\" Name=\"CoordinateY\" format=\"appended\" offset=\"%ld\" />\n", (long)offset);
+  offset += 4 + 1 * 4 * (Ny + 3);
+  std::fprintf(fp, "      <DataArray type=\"Float32\" Name=\"CoordinateZ\" format=\"appended\" offset=\"%ld\" />\n", (long)offset);
+  std::fprintf(fp, "    </Coordinates>\n");
+  std::fprintf(fp, "  </Piece>\n");
+  std::fprintf(fp, "  </RectilinearGrid>\n");
+  std::fprintf(fp, "  <AppendedData encoding=\"raw\">");
+  std::fprintf(fp, "_");
+  // one block: its byte count, then comps values per cell for k = 1..zcomp,
+  // j, i (x fastest), each value as the reference rounds it
+  auto block = [&](int comps, auto value) {
+    const unsigned int array_size = (unsigned int)(comps * 4 * cells);
+    std::fwrite(&array_size, sizeof(int), 1, fp);
+    for (int k = 1; k <= zcomp; k++)
+      for (int j = 0; j < Ny + 2; j++)
+        for (int i = 0; i < Nx + 2; i++) {
+          const size_t r = ((size_t)i * ny + j) * nz + k;
+          for (int q = 0; q < comps; q++) {
+            const float val32 = value(r, q);
+            std::fwrite(&val32, sizeof(float), 1, fp);
+          }
+        }
+  };
+  const double *uvec[3] = {ux.data(), uy.data(), uz.data()};
+  block(1, [&](size_t r, int) { return (float)rho[r]; });
+  block(3, [&](size_t r, int q) { return (float)((float)uvec[q][r] / u_max); });
+  block(3, [&](size_t r, int q) { return (float)((float)au[3 * r + q] / u_max); });
+  block(3, [&](size_t r, int q) { return (float)((float)ru[3 * r + q] / u_max); });
+  block(1, [&](size_t r, int) { return (float)p[r]; });
+  block(1, [&](size_t r, int) { return (float)ap[r]; });
+  block(1, [&](size_t r, int) { return (float)rp[r]; });
+  block(9, [&](size_t r, int q) { return (float)sig[9 * r + q]; });
+  for (int v = 0; v < 3; v++)  // vorticity, its average and rms: never computed by the reference
+    block(3, [&](size_t, int) { return (float)((float)0.0 / (u_max / D)); });
+  block(9, [&](size_t r, int q) { return (float)((float)ars[9 * r + q] / (std::pow(u_max, 2))); });
+  block(9, [&](size_t r, int q) { return (float)((float)rrs[9 * r + q] / (std::pow(u_max, 2))); });
+  block(1, [&](size_t r, int) {
+    const int i = (int)(r / (ny * nz)), j = (int)((r / nz) % ny), k = (int)(r % nz);
+    return (float)(int)lb.fluid[i][j][k].type;
+  });
+  auto coords = [&](int count, double d) {
+    const unsigned int array_size = (unsigned int)(1 * 4 * count);
+    std::fwrite(&array_size, sizeof(int), 1, fp);
+    for (int i = 0; i < count; i++) {
+      const float val32 = (float)(i * d);
+      std::fwrite(&val32, sizeof(float), 1, fp);
+    }
+  };
+  coords(Nx + 3, dx);
+  coords(Ny + 3, dy);
+  coords(zcomp + 1, dz);
+  std::fprintf(fp, "  </AppendedData>\n");
+  std::fprintf(fp, "</VTKFile>\n");
+  std::fclose(fp);
 }

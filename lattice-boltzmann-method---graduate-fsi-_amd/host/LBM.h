@@ -92,6 +92,10 @@ class LBM {
   void Step(int n);
   // Copies density/ux/uy/uz (and f, F when asked) from the device.
   void SyncMirror(bool populations = false);
+  // LBM::Sup_MacroProp(n) of the MORE_DETAILS build (lbm.cpp:865-939):
+  // running averages and rms on the device after the step's MacroProp; Nt is
+  // the output interval (the reference's #define Nt 10, LBM.h:493).
+  void Sup_MacroProp(int n, int Nt = 10);
   lbm_ctx *ctx() const;
 };
 
@@ -113,3 +117,11 @@ double MEA_CD(LBM &lb, int xmin, int xmax, int ymin, int ymax, int zmin, int zma
 // the thermal blocks when it is thermal.
 void VTK_SetCase(const lbm_vtk_desc &desc);
 void OutputVTK(int &nout, LBM &lb);
+// OutputAVGVTK(int&, LBM&), lbm.cpp:2645-3030 (the MORE_DETAILS build):
+// writes ./avgfield%06d.vtr (nout) with density, velocity, the averages and
+// rms of Sup_MacroProp, pressure, viscous stress, the (zero) vorticity
+// fields, the Reynolds-stress statistics and the cell types, scaled as the
+// reference does (velocities by u_max = the VTK_SetCase velocity scale,
+// Reynolds stresses by u_max^2).  The reference leaves this file's name
+// unset outside its tandem-triangle case.
+void OutputAVGVTK(int &nout, LBM &lb);

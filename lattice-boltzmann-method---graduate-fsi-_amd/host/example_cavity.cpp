@@ -2,8 +2,10 @@
 // 195-221) on the drop-in LBM class: a setup function writes lb.fluid, then
 // Init() and the Collision -> Streaming -> MacroProp loop.
 //
-// usage: lbm_example_cavity N steps out.bin [fused] [f64|f32|f16s|f16c]
+// usage: lbm_example_cavity N steps out.bin [fused|details] [f64|f32|f16s|f16c]
 // Writes density, ux, uy, uz (each (N+2)^3 doubles in fluid[i][j][k] order).
+// "details": the MORE_DETAILS loop (Sup_MacroProp every Nt = 10 steps after
+// MacroProp, main.cpp:389-391) and OutputAVGVTK(1) into ./avgfield000001.vtr.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -41,6 +43,7 @@ int main(int argc, char **argv) {
   const int N = std::atoi(argv[1]);
   const int numstep = std::atoi(argv[2]);
   const bool fused = argc > 4 && std::strcmp(argv[4], "fused") == 0;
+  const bool details = argc > 4 && std::strcmp(argv[4], "details") == 0;
   int storage = LBM_STORE_F64;
   if (argc > 5 && std::strcmp(argv[5], "f32") == 0) storage = LBM_STORE_F32;
   if (argc > 5 && std::strcmp(argv[5], "f16s") == 0) storage = LBM_STORE_F16S;
@@ -55,7 +58,15 @@ int main(int argc, char **argv) {
       lb.Collision();
       lb.Streaming();
       lb.MacroProp();
+      if (details && step % 10 == 0) lb.Sup_MacroProp(step);
     }
+  }
+  if (details) {
+    lbm_vtk_desc vd{};
+    vd.u_scale = 0.05;  // u_max: the lid velocity
+    VTK_SetCase(vd);
+    int nout = 1;
+    OutputAVGVTK(nout, lb);
   }
   FILE *fp = std::fopen(argv[3], "wb");
   if (!fp) return 1;

@@ -206,3 +206,71 @@ def test_dropin_bounce_back_cylinder_mea_matches_oracle(oracle_mod, tmp_path):
     fl = case.type == 0
     assert rel_l2_u(tuple(fields[1:]), (r_ux, r_uy, r_uz), fl) < 1e-11
     assert np.abs(fields[0] - r_rho)[fl].max() < 1e-12
+
+
+def _read_appended_vtr(path):
+    """(header text, [raw float32 blocks]) of a VTK XML file with raw
+    appended data (UInt32 byte counts)."""
+    data = path.read_bytes()
+    mark = data.index(b'<AppendedData encoding="raw">_') + len(b'<AppendedData encoding="raw">_')
+    header, pos, blocks = data[:mark].decode(), mark, []
+    while not data[pos:].startswith(b"  </AppendedData>"):
+        size = int(np.frombuffer(data[pos:pos + 4], dtype=np.uint32)[0])
+        blocks.append(np.frombuffer(data[pos + 4:pos + 4 + size], dtype=np.float32))
+        pos += 4 + size
+    assert data[pos:] == b"  </AppendedData>\n</VTKFile>\n"
+    return header, blocks
+
+
+def test_dropin_more_details_avg_vtk(oracle_mod, tmp_path):
+    """The MORE_DETAILS driver (Sup_MacroProp every Nt = 10 steps) and
+    OutputAVGVTK (lbm.cpp:2645-3030) on the drop-in class: the file's 13
+    cell blocks, cell types and coordinates against the oracle's fields and
+    statistics, scaled and rounded as the reference writes them."""
+    N, steps = 10, 60
+    out = tmp_path / "cav.bin"
+    r = subprocess.run([EXE, str(N), str(steps), str(out), "details"], capture_output=True, text=True,
+                       timeout=120, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    header, blocks = _read_appended_vtr(tmp_path / "avgfield000001.vtr")
+    names = ["Density", "Velocity", "Avg velocity", "Rms velocity", "Pressure", "Avg pressure", "Rms pressure",
+             "Viscous Stress", "Vorticity", "Avg vorticity", "Rms vorticity", "Avg reynolds Stress",
+             "Rms reynolds Stress", "CellType"]
+    for nm in names:
+        assert f'Name="{nm}"' in header
+    assert len(blocks) == len(names) + 3
+    case = cases.cavity(N, u_lid=0.05, Re=100.0)
+    o = oracle_mod.run_case(case, 0)
+    for step in range(1, steps + 1):
+        assert o.collision() == 0
+        o.streaming()
+        assert o.macroprop() == 0
+        if step % 10 == 0:
+            o.stats_update(step, 10)
+    rho, ux, uy, uz = o.macro()
+    sig, p = o.stress()
+    au, ru, ap, rp, ars, rrs = o.stats()
+    u_max = 0.05
+    n = N + 2
+
+    def vtk_order(a, comps):  # [i][j][k](comp) -> k = 1..N, j, i, comp
+        a = a.reshape(n, n, n, comps)[:, :, 1:N + 1]
+        return a.transpose(2, 1, 0, 3)
+
+    fl = vtk_order(case.type == cases.TYPE_F, 1)[..., 0]
+    want = [np.float32(rho), np.float32(np.float32(np.stack([ux, uy, uz], -1)) / u_max),
+            np.float32(np.float32(au) / u_max), np.float32(np.float32(ru) / u_max), np.float32(p),
+            np.float32(ap), np.float32(rp), np.float32(sig.reshape(sig.shape[:3] + (9,))),
+            np.zeros(case.shape + (3,)), np.zeros(case.shape + (3,)), np.zeros(case.shape + (3,)),
+            np.float32(np.float32(ars.reshape(ars.shape[:3] + (9,))) / u_max ** 2),
+            np.float32(np.float32(rrs.reshape(rrs.shape[:3] + (9,))) / u_max ** 2),
+            np.float32(case.type)]
+    for nm, got, w in zip(names, blocks, want):
+        comps = 1 if w.ndim == 3 else w.shape[-1]
+        g = got.reshape(N, n, n, comps)
+        ww = vtk_order(np.asarray(w, np.float64), comps)
+        scale = max(float(np.abs(ww[fl]).max()), 1e-30)
+        assert np.abs(g[fl] - ww[fl]).max() <= 1e-6 * scale, nm
+    assert np.array_equal(blocks[len(names) - 1].reshape(N, n, n), vtk_order(case.type.astype(np.float32), 1)[..., 0])
+    for b, count in zip(blocks[len(names):], (n + 1, n + 1, N + 1)):
+        assert np.array_equal(b, np.arange(count, dtype=np.float32))
