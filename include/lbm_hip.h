@@ -184,6 +184,12 @@ typedef struct lbm_ibm_desc {
   double vsolid;    /* added-mass volume (Vsolid, LBM.h:355)                 */
   double center[2]; /* body centre (center_x, center_y of LBM.h) at set-up  */
   double half;      /* box half-width (D in lbm.cpp:1165)                   */
+  int32_t scheme;   /* 0: Implicit_Diffuse_Interface_Scheme (the reference's *
+                     * default, LBM.h:36; m_max passes); 1: the explicit   *
+                     * branch (lbm.cpp:1171-1258: one pass, F = +spread,    *
+                     * density at the marker's local nearest point, drag   *
+                     * and lift summed over the box cells); explicit needs *
+                     * an unforced set-up and a box that does not follow   */
 } lbm_ibm_desc;
 
 /* Markers of the body (Generate_Cylinder_Marker, lbm.cpp:1857-1877), their

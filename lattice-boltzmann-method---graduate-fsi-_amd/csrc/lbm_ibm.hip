@@ -145,6 +145,35 @@ __global__ __launch_bounds__(256) void k_ibm_nearest(Box B, const double *__rest
   }
 }
 
+// The explicit branch's density point (lbm.cpp:1216-1222): NearestLatticeCoordinate
+// over the marker's floor/ceil neighbourhood (z: floor(z)..ceil(z), the 2D
+// call's 1..1 for markers at z = 1), starting from cell (0, 0, 1), which
+// stays when no point lies ahead along the normal.  Box index, or -1 when
+// the point is outside the box (the force is then NaN).
+__global__ void k_ibm_near_local(Box B, int nm, const double *__restrict__ mx, const double *__restrict__ my,
+                                 const double *__restrict__ mz, const double *__restrict__ nx,
+                                 const double *__restrict__ ny, const double *__restrict__ nz,
+                                 int32_t *__restrict__ near) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nm) return;
+  const double px = mx[n], py = my[n], pz = mz[n], qx = nx[n], qy = ny[n], qz = nz[n];
+  int i = 0, j = 0, k = 1;
+  double dot = 0;
+  for (int x = (int)floor(px); x <= (int)ceil(px); x++)
+    for (int y = (int)floor(py); y <= (int)ceil(py); y++)
+      for (int z = (int)floor(pz); z <= (int)ceil(pz); z++) {
+        const double d = (x - px) * qx + (y - py) * qy + (z - pz) * qz;
+        if (d >= dot && d > 0) {
+          i = x;
+          j = y;
+          k = z;
+          dot = d;
+        }
+      }
+  const bool in = i >= B.x0 && i <= B.x1 && j >= B.y0 && j <= B.y1 && k >= B.z0 && k <= B.z1;
+  near[n] = in ? (i - B.x0) + B.nx * ((j - B.y0) + B.ny * (k - B.z0)) : -1;
+}
+
 // Calc_AverageDensity's selection, lbm.cpp:1110-1124: box cells (any type)
 // with (p - x_n).n_n <= 0 for every marker.
 __global__ void k_ibm_inside(Box B, int nm, const double *__restrict__ mx,
@@ -400,7 +429,8 @@ __global__ __launch_bounds__(256) void k_ibm_spread(const int32_t *__restrict__ 
                                                     const int32_t *__restrict__ bstart,
                                                     const int32_t *__restrict__ en,
                                                     const double *__restrict__ ew,
-                                                    const double *__restrict__ Fds, BoxArrays o) {
+                                                    const double *__restrict__ Fds, BoxArrays o,
+                                                    double fsign) {
   __shared__ double term[3][CHUNK_E];
   // chunk[2q] = first band cell, chunk[2q+1] = first pair of chunk q
   const int c0 = chunk[2 * blockIdx.x], c1 = chunk[2 * blockIdx.x + 2];
@@ -433,10 +463,12 @@ __global__ __launch_bounds__(256) void k_ibm_spread(const int32_t *__restrict__ 
       m1 += term[1][e];
       m2 += term[2][e];
     }
+    // the implicit branch subtracts the spread force, the explicit one adds
+    // it (lbm.cpp:1347-1349 against 1244-1246); u gains +Fm/(2 rho) in both
     const double dt = 1.0;
-    o.Fx[b] = fx + -m0;
-    o.Fy[b] = fy + -m1;
-    o.Fz[b] = fz + -m2;
+    o.Fx[b] = fx + fsign * m0;
+    o.Fy[b] = fy + fsign * m1;
+    o.Fz[b] = fz + fsign * m2;
     o.ux[b] = ux + dt * m0 / (2 * r);
     o.uy[b] = uy + dt * m1 / (2 * r);
     o.uz[b] = uz + dt * m2 / (2 * r);
@@ -453,6 +485,31 @@ __device__ __forceinline__ double fin_sum(double v, double *sh) {
   const double r = sh[0];
   __syncthreads();
   return r;
+}
+
+// The explicit branch's drag and lift, summed over the box's F/IN/OUT cells
+// (lbm.cpp:1252-1253: -F_x dx^2, -F_y dx^2): per block a fixed-order tree
+// sum, stored as (drag, lift) pairs that k_ibm_finish reduces like the
+// per-marker sums of the implicit branch.
+__global__ __launch_bounds__(256) void k_ibm_cell_force(Box B, BoxArrays o, double *__restrict__ pairs) {
+  __shared__ double sh[256];
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  double fx = 0, fy = 0;
+  if (b < B.ncell() && o.live[b]) {
+    const double dx = 1.0;
+    fx = -o.Fx[b] * (dx * dx);
+    fy = -o.Fy[b] * (dx * dx);
+  }
+  for (int c = 0; c < 2; c++) {
+    sh[threadIdx.x] = c == 0 ? fx : fy;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) pairs[2 * blockIdx.x + c] = sh[0];
+    __syncthreads();
+  }
 }
 
 // Drag, lift and the added-mass term, lbm.cpp:1105-1132, 1390-1398.  One
@@ -811,6 +868,9 @@ struct lbm_ibm {
   lbm_filament *fil = nullptr;
   int near_cell = 0;
   double *ud = nullptr, *fsum = nullptr;
+  int scheme = 0;              // 0 implicit, 1 explicit (one pass, lbm.cpp:1171-1258)
+  double *cell_pairs = nullptr;  // explicit: per-block (drag, lift) of the box cells
+  size_t pairs_cap = 0;
 };
 
 static void *dalloc(lbm_ibm *m, size_t bytes) {
@@ -857,6 +917,7 @@ void ibm_free(lbm_ctx *c) {
   free_box_arrays(m);
   free_bin_arrays(m);
   dfree(m->out);
+  dfree(m->cell_pairs);
   delete m;
   c->ibm = nullptr;
 }
@@ -997,6 +1058,9 @@ static int ibm_geometry(lbm_ctx *c) {
   hipLaunchKernelGGL(k_ibm_weights, blocks(m->nm), dim3(256), 0, c->stream, m->nm, m->mx, m->my, m->mz, m->st);
   if (m->near_cell)
     hipLaunchKernelGGL(k_ibm_near_cell, blocks(m->nm), dim3(256), 0, c->stream, B, m->nm, m->mx, m->my, m->mz, m->near);
+  else if (m->scheme == 1)
+    hipLaunchKernelGGL(k_ibm_near_local, blocks(m->nm), dim3(256), 0, c->stream, B, m->nm, m->mx, m->my, m->mz, m->nx,
+                       m->ny, m->nz, m->near);
   else
     hipLaunchKernelGGL(k_ibm_nearest, dim3(m->nm), dim3(256), 0, c->stream, B, m->mx, m->my, m->mz, m->nx, m->ny, m->nz, m->near);
   hipLaunchKernelGGL(k_ibm_inside, blocks(nb), dim3(256), 0, c->stream, B, m->nm, m->mx, m->my, m->mz, m->nx, m->ny, m->nz, m->inside);
@@ -1071,18 +1135,39 @@ static int ibm_iterate(lbm_ctx *c, const double vcm[3], const double vb[3], doub
   lbm_ibm *m = c->ibm;
   const Box &B = m->B;
   const BoxArrays o = box_arrays(m);
-  ForceArgs fa{m->near, m->ds, m->Fb, m->Fds, m->dsum, vcm[0], vcm[1], vcm[2], m->m_max > 0, 1, m->ud, m->fsum};
+  const bool expl = m->scheme == 1;
+  ForceArgs fa{m->near, m->ds, m->Fb, m->Fds, m->dsum, vcm[0], vcm[1], vcm[2], expl || m->m_max > 0, 1, m->ud, m->fsum};
   const dim3 gi((unsigned)((m->nm + 3) / 4));
   hipLaunchKernelGGL(k_ibm_interp, gi, dim3(256), 0, c->stream, m->nm, m->pidx, m->st, o, m->ub, fa);
   fa.first = 0;
+  const int npart = (B.ncell() + 255) / 256;
+  if (expl) {
+    // one pass: the unforced interpolation's force, spread with + (the
+    // markers keep the unforced u_b), drag and lift summed over the cells
+    if (m->nchunk > 0)
+      hipLaunchKernelGGL(k_ibm_spread, dim3(m->nchunk), dim3(256), 0, c->stream, m->chunk, m->band, m->bstart, m->ent,
+                         m->ew, m->Fds, o, 1.0);
+    if (out) {
+      if ((size_t)npart > m->pairs_cap) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        dfree(m->cell_pairs);
+        HIPCHK(hipMalloc((void **)&m->cell_pairs, 16 * (size_t)npart));
+        m->pairs_cap = npart;
+      }
+      hipLaunchKernelGGL(k_ibm_cell_force, dim3((unsigned)npart), dim3(256), 0, c->stream, B, o, m->cell_pairs);
+      hipLaunchKernelGGL(k_ibm_finish, dim3(1), dim3(FIN_THREADS), 0, c->stream, npart, m->cell_pairs, npart, m->part,
+                         m->ninside, m->vsolid, vcm[0] - vb[0], vcm[1] - vb[1], out);
+    }
+    HIPCHK(hipGetLastError());
+    return LBM_OK;
+  }
   for (int it = 1; it <= m->m_max; it++) {
     if (m->nchunk > 0)
       hipLaunchKernelGGL(k_ibm_spread, dim3(m->nchunk), dim3(256), 0, c->stream, m->chunk, m->band, m->bstart, m->ent,
-                         m->ew, m->Fds, o);
+                         m->ew, m->Fds, o, -1.0);
     fa.force = it < m->m_max;
     hipLaunchKernelGGL(k_ibm_interp, gi, dim3(256), 0, c->stream, m->nm, m->pidx, m->st, o, m->ub, fa);
   }
-  const int npart = (B.ncell() + 255) / 256;
   if (out)
     hipLaunchKernelGGL(k_ibm_finish, dim3(1), dim3(FIN_THREADS), 0, c->stream, m->nm, m->dsum, npart, m->part,
                        m->ninside, m->vsolid, vcm[0] - vb[0], vcm[1] - vb[1], out);
@@ -1159,6 +1244,7 @@ static int ibm_setup(lbm_ctx *c, const lbm_ibm_desc *d, const double *x, const d
   m->cx = d->center[0];
   m->cy = d->center[1];
   m->half = d->half;
+  m->scheme = d->scheme;
   int b[6] = {d->box[0], d->box[1], d->box[2], d->box[3], d->box[4], d->box[5]};
   if (m->follow) {
     m->B.z0 = b[4];
@@ -1209,6 +1295,12 @@ int lbm_ibm_setup(lbm_ctx *c, const lbm_ibm_desc *d, const double *x, const doub
   if (c->thermal == 1) return LBM_ERR_UNSUPPORTED;
   if (d->nmarkers < 1 || d->m_max < 0) return LBM_ERR_ARG;
   if (d->follow && !(d->half >= 0)) return LBM_ERR_ARG;
+  if (d->scheme != 0 && d->scheme != 1) return LBM_ERR_ARG;
+  // the explicit branch leaves F of cells outside the box as it was (the
+  // set-up force, or the last IBM force of a cell the box has left); the
+  // IBM collision takes F = 0 there, so only unforced set-ups with a box
+  // that stays in place are accepted
+  if (d->scheme == 1 && (d->follow || c->force_mode != 0)) return LBM_ERR_UNSUPPORTED;
   const int e = ibm_setup(c, d, x, y, z, nx, ny, nz, ds);
   if (e != LBM_OK && c->ibm) {
     // leave no half-built body behind: clear its marks, drop the state
@@ -1335,6 +1427,7 @@ int lbm_fsi_setup(lbm_ctx *c, lbm_filament *f, const lbm_ibm_desc *d) {
   if (c->state == ST_NONE) return LBM_ERR_STATE;
   if (c->thermal == 1) return LBM_ERR_UNSUPPORTED;
   if (d->m_max < 0 || d->follow) return LBM_ERR_ARG;
+  if (d->scheme != 0) return LBM_ERR_UNSUPPORTED;  // the filament couples through the implicit branch
   double *fx, *fxp, fds;
   int n, fdev;
   filament_state(f, &fx, &fxp, &n, &fds, &fdev);

@@ -106,6 +106,16 @@ void IBLBM(LBM &lb, std::vector<MARKER> &marker, std::vector<double> &ds, std::v
     d.center[0] = c.center_x;
     d.center[1] = c.center_y;
     d.half = c.D;
+    d.scheme = c.scheme;
+    if (c.scheme == 1 && c.motion == IBM_CASE::FIXED) {
+      // a body at rest: the box the reference recomputes every step never
+      // moves (lbm.cpp:1165-1167), so it is placed once
+      d.follow = 0;
+      d.box[0] = (int)std::floor(c.center_x - c.D);
+      d.box[1] = (int)std::ceil(c.center_x + c.D);
+      d.box[2] = (int)std::floor(c.center_y - c.D);
+      d.box[3] = (int)std::ceil(c.center_y + c.D);
+    }
     check(lbm_ibm_setup(ctx, &d, x.data(), y.data(), z.data(), nx.data(), ny.data(), nz.data(),
                         ds.data()),
           "lbm_ibm_setup");

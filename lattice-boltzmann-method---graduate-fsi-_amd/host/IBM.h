@@ -60,6 +60,7 @@ struct IBM_CASE {
   double coef_length = 0;     // C_L, C_D length: D (0 selects D), D + dx for inline oscillation
   int zmin = 1, zmax = 1;     // box z range (the reference's 2D slab: 1..1)
   int Nt = 10;                // output interval (steps), names the marker files
+  int scheme = 0;             // 0 Implicit_Diffuse_Interface_Scheme, 1 the explicit branch
 };
 
 void IBM_SetCase(const IBM_CASE &c);

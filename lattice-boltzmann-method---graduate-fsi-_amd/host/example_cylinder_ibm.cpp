@@ -5,6 +5,7 @@
 // Streaming.
 //
 // usage: lbm_example_cylinder_ibm Nx Ny D steps out.bin [f64|f32|f16s|f16c] [fixed|inline|transverse]
+//        [implicit|explicit]
 // Writes Cd[steps], Cl[steps], then density, ux, uy, uz of the final lattice
 // ((Nx+2)(Ny+2)(Nz+2) doubles each, fluid[i][j][k] order), all doubles.
 #include <cmath>
@@ -50,6 +51,7 @@ int main(int argc, char **argv) {
   if (argc > 6 && std::strcmp(argv[6], "f16s") == 0) storage = LBM_STORE_F16S;
   if (argc > 6 && std::strcmp(argv[6], "f16c") == 0) storage = LBM_STORE_F16C;
   const char *motion = argc > 7 ? argv[7] : "fixed";
+  const bool expl = argc > 8 && std::strcmp(argv[8], "explicit") == 0;
 
   // case constants (the role of LBM.h:318-356)
   const double PI = 3.14159265, Re = 20.0, u_max = 0.04;
@@ -62,6 +64,7 @@ int main(int argc, char **argv) {
   cs.number_nodes = (int)std::lround(PI * D);
   cs.m_max = 20;
   cs.Vsolid = PI * D * D * 0.25;
+  cs.scheme = expl ? 1 : 0;  // Explicit_ / Implicit_Diffuse_Interface_Scheme (LBM.h:35-36)
   if (std::strcmp(motion, "inline") == 0) {
     cs.motion = IBM_CASE::INLINE_OSCILLATING;
     cs.frequency = 0.01;

@@ -65,6 +65,7 @@ class _IbmDesc(C.Structure):
         ("vsolid", C.c_double),
         ("center", C.c_double * 2),
         ("half", C.c_double),
+        ("scheme", C.c_int32),
     ]
 
 
@@ -459,16 +460,17 @@ class Lattice:
 
     # -- immersed boundary (IBLBM, lbm.cpp:1135-1417) ---------------------------
     def ibm_setup(self, markers: dict, box, m_max: int = 20, vsolid: float = 0.0, moving=False,
-                  follow=None):
+                  follow=None, scheme: str = "implicit"):
         """markers: dict of float64 arrays x, y, z, nx, ny, nz, ds; box:
         (xmin, xmax, ymin, ymax, zmin, zmax) inclusive lattice indices.
         follow: None, or (center_x, center_y, half) for a box that follows
         the body as the reference's does (then only zmin, zmax of box are
-        used)."""
+        used).  scheme: "implicit" (the reference's default) or "explicit"."""
         n = int(markers["x"].size)
         fc = (0.0, 0.0, 0.0) if follow is None else tuple(float(v) for v in follow)
         d = _IbmDesc(n, int(m_max), (C.c_int32 * 6)(*[int(b) for b in box]), int(bool(moving)),
-                     int(follow is not None), float(vsolid), (C.c_double * 2)(fc[0], fc[1]), fc[2])
+                     int(follow is not None), float(vsolid), (C.c_double * 2)(fc[0], fc[1]), fc[2],
+                     {"implicit": 0, "explicit": 1}[scheme])
         arrs = [np.ascontiguousarray(markers[k], dtype=np.float64) for k in
                 ("x", "y", "z", "nx", "ny", "nz", "ds")]
         self._ibm_n = n

@@ -1213,6 +1213,125 @@ int orc_ibm_step(orc_lbm *lb, int nm, double *mx, double *my, double *mz,
   return err;
 }
 
+/* NearestLatticeCoordinate over [xmin,xmax]x[ymin,ymax]x[zmin,zmax] as the
+ * explicit scheme calls it, lbm.cpp:1074-1092: the last point (in x, y, z
+ * loop order) with the largest dot product > 0 along the normal; the
+ * caller's start values stay when none qualifies. */
+static void nearest_local(int xmin, int xmax, int ymin, int ymax, int zmin, int zmax, double mx, double my,
+                          double mz, double nx, double ny, double nz, int *i, int *j, int *k) {
+  double dot = 0;
+  for (int x = xmin; x <= xmax; x++)
+    for (int y = ymin; y <= ymax; y++)
+      for (int z = zmin; z <= zmax; z++) {
+        const double dot_trial = (x - mx) * nx + (y - my) * ny + (z - mz) * nz;
+        if (dot_trial >= dot && dot_trial > 0) {
+          *i = x; *j = y; *k = z;
+          dot = dot_trial;
+        }
+      }
+}
+
+/* IBLBM, explicit branch (Explicit_Diffuse_Interface_Scheme), lbm.cpp:
+ * 1135-1167, 1171-1258 and 1390-1398. */
+int orc_ibm_step_explicit(orc_lbm *lb, int nm, double *mx, double *my, double *mz,
+                          const double *nx, const double *ny, const double *nz,
+                          double *ub, double *Fb, const double *ds,
+                          const double vcm[3], const double vcm_before[3],
+                          int xmin, int xmax, int ymin, int ymax, int zmin, int zmax,
+                          double vsolid, double *out) {
+  const int Nx = lb->Nx, Ny = lb->Ny, Nz = lb->Nz, q = lb->q;
+  double Drag = 0, Lift = 0;
+  int err = ORC_OK;
+  /* marker update, lbm.cpp:1154-1161 (u_des = V_CM) */
+  for (int n = 0; n < nm; n++) {
+    mx[n] += (vcm[0] + vcm_before[0]) * ORC_DT / 2;
+    my[n] += (vcm[1] + vcm_before[1]) * ORC_DT / 2;
+    mz[n] += (vcm[2] + vcm_before[2]) * ORC_DT / 2;
+  }
+  /* unforced moments of F/IN/OUT cells, F kept (lbm.cpp:1173-1193) */
+#pragma omp parallel for
+  for (int i = 0; i <= Nx + 1; i++)
+    for (int j = 0; j <= Ny + 1; j++)
+      for (int k = 0; k <= Nz + 1; k++) {
+        orc_cell *c = &lb->fluid[i][j][k];
+        if (!ibm_cell(c->type)) continue;
+        double rho = 0, rho_u[3] = {0, 0, 0};
+        for (int l = 0; l < q; l++) {
+          rho += c->f[l];
+          rho_u[0] += c->f[l] * lb->cx[l];
+          rho_u[1] += c->f[l] * lb->cy[l];
+          rho_u[2] += c->f[l] * lb->cz[l];
+        }
+        c->density = rho;
+        c->ux = rho_u[0] / rho;
+        c->uy = rho_u[1] / rho;
+        c->uz = rho_u[2] / rho;
+      }
+  /* unforced interpolation, lbm.cpp:1196-1212 */
+  ibm_interpolate(lb, nm, mx, my, mz, ub, xmin, xmax, ymin, ymax, zmin, zmax);
+  /* boundary force with rho at the local nearest point, lbm.cpp:1215-1230
+   * (z range floor(z)..ceil(z): the reference's 2D call passes 1, 1 for
+   * markers at z = 1) */
+  for (int n = 0; n < nm; n++) {
+    int i = 0, j = 0, k = 1;
+    nearest_local((int)floor(mx[n]), (int)ceil(mx[n]), (int)floor(my[n]), (int)ceil(my[n]), (int)floor(mz[n]),
+                  (int)ceil(mz[n]), mx[n], my[n], mz[n], nx[n], ny[n], nz[n], &i, &j, &k);
+    const double rho = lb->fluid[i][j][k].density;
+    Fb[3 * n + 0] = 2 * rho * (vcm[0] - ub[3 * n + 0]) / ORC_DT;
+    Fb[3 * n + 1] = 2 * rho * (vcm[1] - ub[3 * n + 1]) / ORC_DT;
+    Fb[3 * n + 2] = 2 * rho * (vcm[2] - ub[3 * n + 2]) / ORC_DT;
+  }
+  /* distribution, forced velocity and the cell-summed drag and lift,
+   * lbm.cpp:1233-1256 (serial: the reference's reduction order is the
+   * loop order) */
+  for (int i = xmin; i <= xmax; i++)
+    for (int j = ymin; j <= ymax; j++)
+      for (int k = zmin; k <= zmax; k++) {
+        orc_cell *c = &lb->fluid[i][j][k];
+        if (!ibm_cell(c->type)) continue;
+        c->F[0] = 0; c->F[1] = 0; c->F[2] = 0;
+        for (int n = 0; n < nm; n++) {
+          double rx = (i * ORC_DX - mx[n]) / ORC_DX, ry = (j * ORC_DY - my[n]) / ORC_DY,
+                 rz = (k * ORC_DZ - mz[n]) / ORC_DZ;
+          c->F[0] += Fb[3 * n + 0] * ds[n] * orc_ibm_kernel(rx) * orc_ibm_kernel(ry) * orc_ibm_kernel(rz);
+          c->F[1] += Fb[3 * n + 1] * ds[n] * orc_ibm_kernel(rx) * orc_ibm_kernel(ry) * orc_ibm_kernel(rz);
+          c->F[2] += Fb[3 * n + 2] * ds[n] * orc_ibm_kernel(rx) * orc_ibm_kernel(ry) * orc_ibm_kernel(rz);
+        }
+        c->ux += ORC_DT * c->F[0] / (2 * c->density);
+        c->uy += ORC_DT * c->F[1] / (2 * c->density);
+        c->uz += ORC_DT * c->F[2] / (2 * c->density);
+        Drag += -c->F[0] * pow(ORC_DX, 2);
+        Lift += -c->F[1] * pow(ORC_DX, 2);
+      }
+  /* Calc_AverageDensity and added mass, lbm.cpp:1105-1132, 1392-1398 */
+  double rho_avg = 0;
+  int number = 0;
+  for (int i = xmin; i <= xmax; i++)
+    for (int j = ymin; j <= ymax; j++)
+      for (int k = zmin; k <= zmax; k++) {
+        int is_dot_negative = 1;
+        for (int n = 0; n < nm; n++) {
+          double v0 = i * ORC_DX - mx[n], v1 = j * ORC_DY - my[n], v2 = k * ORC_DZ - mz[n];
+          if (v0 * nx[n] + v1 * ny[n] + v2 * nz[n] > 0) {
+            is_dot_negative = 0;
+            break;
+          }
+        }
+        if (is_dot_negative) {
+          rho_avg += lb->fluid[i][j][k].density;
+          number++;
+        }
+      }
+  rho_avg /= number;
+  Drag += rho_avg * vsolid * (vcm[0] - vcm_before[0]) / ORC_DT;
+  Lift += rho_avg * vsolid * (vcm[1] - vcm_before[1]) / ORC_DT;
+  out[0] = Drag;
+  out[1] = Lift;
+  out[2] = rho_avg;
+  q7_ibm_buoyancy(lb);
+  return err;
+}
+
 /* The IBM step of a coupled filament (lbm_oracle.h): orc_ibm_step's
  * unforced moments, interpolation and m_max force/spread/interpolate passes
  * with the markers where they are, u_des per marker, rho at the box cell

@@ -146,6 +146,21 @@ void orc_get_stats(const orc_lbm *lb, double *avg_u, double *rms_u, double *avg_
  * form in z (the reference has none). */
 void orc_mea(const orc_lbm *lb, const int box[6], double out[3]);
 
+/* IBLBM's explicit branch (Explicit_Diffuse_Interface_Scheme, lbm.cpp:
+ * 1171-1258; the reference's default build uses the implicit one): unforced
+ * moments of F/IN/OUT cells (their F kept), unforced interpolation, F_b =
+ * 2 rho (u_des - u_b)/dt with rho at NearestLatticeCoordinate over the
+ * marker's floor/ceil neighbourhood (starting from cell (0,0,1)), then for
+ * the box's F/IN/OUT cells F = +sum F_b ds phi phi phi, u += F/(2 rho),
+ * Drag and Lift summed over those cells (-F_x dx^2, -F_y dx^2), plus the
+ * added mass; out as orc_ibm_step's. */
+int orc_ibm_step_explicit(orc_lbm *lb, int nm, double *mx, double *my, double *mz,
+                          const double *nx, const double *ny, const double *nz,
+                          double *ub, double *Fb, const double *ds,
+                          const double vcm[3], const double vcm_before[3],
+                          int xmin, int xmax, int ymin, int ymax, int zmin, int zmax,
+                          double vsolid, double *out);
+
 /* The IBM step of a coupled filament (no reference counterpart): as
  * orc_ibm_step with the markers where they are (no rigid motion), the
  * desired velocity udes per marker (3 nm), rho at the box cell holding the

@@ -59,6 +59,8 @@ _SIG = {
     "orc_q7_get_g": (None, [_P, _D]),
     "orc_fsi_ibm_step": (C.c_int, [_P, C.c_int, _D, _D, _D, _D, _D, _D, _D, _D] + [C.c_int] * 7),
     "orc_mea": (None, [_P, C.POINTER(C.c_int), _D]),
+    "orc_ibm_step_explicit": (C.c_int, [_P, C.c_int, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D]
+                              + [C.c_int] * 6 + [C.c_double, _D]),
     "orc_stress": (None, [_P, _D, _D]),
     "orc_stats_update": (None, [_P, C.c_int, C.c_int]),
     "orc_get_stats": (None, [_P, _D, _D, _D, _D, _D, _D]),
@@ -300,9 +302,10 @@ class Oracle:
         return (np.array(cx[:]), np.array(cy[:]), np.array(cz[:]), np.array(opp[:]),
                 np.array(w[:]))
 
-    def ibm_step(self, markers: dict, vcm, vcm_before, box, m_max=20, vsolid=0.0):
-        """One implicit IBLBM call; markers is a dict of float64 arrays
-        x, y, z, nx, ny, nz, ds (updated in place: x, y, z, ub, Fb)."""
+    def ibm_step(self, markers: dict, vcm, vcm_before, box, m_max=20, vsolid=0.0, scheme="implicit"):
+        """One IBLBM call (implicit, or the explicit branch); markers is a
+        dict of float64 arrays x, y, z, nx, ny, nz, ds (updated in place:
+        x, y, z, ub, Fb)."""
         n = markers["x"].size
         for k in ("ub", "Fb"):
             if k not in markers:
@@ -310,6 +313,13 @@ class Oracle:
         out = np.zeros(3)
         vcm = np.asarray(vcm, np.float64)
         vcb = np.asarray(vcm_before, np.float64)
+        if scheme == "explicit":
+            err = self.lib.orc_ibm_step_explicit(
+                self.h, n, _dp(markers["x"]), _dp(markers["y"]), _dp(markers["z"]),
+                _dp(markers["nx"]), _dp(markers["ny"]), _dp(markers["nz"]),
+                _dp(markers["ub"]), _dp(markers["Fb"]), _dp(markers["ds"]),
+                _dp(vcm), _dp(vcb), *[int(b) for b in box], float(vsolid), _dp(out))
+            return err, out
         err = self.lib.orc_ibm_step(
             self.h, n, _dp(markers["x"]), _dp(markers["y"]), _dp(markers["z"]),
             _dp(markers["nx"]), _dp(markers["ny"]), _dp(markers["nz"]),

@@ -41,11 +41,11 @@ def _oracle(oracle_mod, case):
     return o
 
 
-def _oracle_run(o, mk, box, vcm, m_max=20, vsolid=0.0):
+def _oracle_run(o, mk, box, vcm, m_max=20, vsolid=0.0, scheme="implicit"):
     """vcm: (nsteps+1, 3); returns (nsteps, 3) Drag, Lift, interior rho."""
     outs = []
     for s in range(vcm.shape[0] - 1):
-        e, out = o.ibm_step(mk, vcm[s + 1], vcm[s], box, m_max=m_max, vsolid=vsolid)
+        e, out = o.ibm_step(mk, vcm[s + 1], vcm[s], box, m_max=m_max, vsolid=vsolid, scheme=scheme)
         assert e == 0
         assert o.collision() == 0
         o.streaming()
@@ -323,4 +323,72 @@ def test_ibm_with_d3q7_buoyancy_fp64(oracle_mod, path):
     _cmp_fields(lat, o, case, 1e-11, 1e-12)
     fl = case.type == cases.TYPE_F
     assert np.abs(lat.thermal()[0] - o.q7_thermal()[0])[fl].max() < 1e-12
+    lat.close()
+
+
+@pytest.mark.parametrize("name", sorted(IBM_CASES))
+@pytest.mark.parametrize("path", ["fused", "apply"])
+def test_ibm_explicit_scheme_fp64(oracle_mod, name, path):
+    """The explicit branch (one pass, F = +spread, density at the marker's
+    local nearest point, cell-summed drag and lift, lbm.cpp:1171-1258)
+    against the oracle's restatement, fused and step by step."""
+    case, mk, box = IBM_CASES[name]()
+    nsteps = 12
+    o = _oracle(oracle_mod, case)
+    omk = {k: v.copy() for k, v in mk.items()}
+    ref = _oracle_run(o, omk, box, np.zeros((nsteps + 1, 3)), scheme="explicit")
+    lat = lbm_amd.Lattice(case.Nx, case.Ny, case.Nz, case.tau, storage="f64")
+    lat.upload(**case.fields())
+    lat.init()
+    lat.ibm_setup(mk, box, scheme="explicit")
+    if path == "fused":
+        got = lat.ibm_run(nsteps)
+    else:
+        got = np.array([_apply_step(lat, (0, 0, 0), (0, 0, 0)) for _ in range(nsteps)])
+    assert lat.sync() == 0
+    _cmp_series(got, ref, 1e-11, "explicit " + name)
+    _cmp_fields(lat, o, case, 1e-11, 1e-12)
+    _cmp_markers(lat, omk, 1e-10)
+    lat.close()
+
+
+def test_ibm_explicit_moving_markers_fixed_box_fp64(oracle_mod):
+    """Transverse oscillation of the markers inside a box that stays in
+    place, with added mass: the explicit branch against the oracle."""
+    case, mk, box = cases.ibm_cylinder(48, 36, 1, D=8.0, Re=20.0, perturb=0.0, center=(14.0, 18.0))
+    vs = case.params["vsolid"]
+    nsteps = 14
+    vcm = _transverse_vcm(nsteps)
+    o = _oracle(oracle_mod, case)
+    omk = {k: v.copy() for k, v in mk.items()}
+    ref = _oracle_run(o, omk, box, vcm, vsolid=vs, scheme="explicit")
+    lat = lbm_amd.Lattice(case.Nx, case.Ny, case.Nz, case.tau, storage="f64")
+    lat.upload(**case.fields())
+    lat.init()
+    lat.ibm_setup(mk, box, vsolid=vs, moving=True, scheme="explicit")
+    got = lat.ibm_run(nsteps, vcm)
+    assert lat.sync() == 0
+    _cmp_series(got, ref, 1e-11, "explicit moving")
+    _cmp_fields(lat, o, case, 1e-11, 1e-12)
+    _cmp_markers(lat, omk, 1e-10)
+    lat.close()
+
+
+def test_ibm_explicit_rejects_follow_and_forced_setups():
+    case, mk, box = IBM_CASES["slab"]()
+    lat = lbm_amd.Lattice(case.Nx, case.Ny, case.Nz, case.tau, storage="f64")
+    lat.upload(**case.fields())
+    lat.init()
+    with pytest.raises(lbm_amd.LBMError) as e:
+        lat.ibm_setup(mk, box, scheme="explicit", follow=(case.Nx / 5.0, case.Ny / 2.0, 8.0))
+    assert e.value.code == lbm_amd.LBM_ERR_UNSUPPORTED
+    lat.close()
+    f = dict(case.fields())
+    f["Fx"] = np.full(case.shape, 1e-6)
+    lat = lbm_amd.Lattice(case.Nx, case.Ny, case.Nz, case.tau, storage="f64")
+    lat.upload(**f)
+    lat.init()
+    with pytest.raises(lbm_amd.LBMError) as e:
+        lat.ibm_setup(mk, box, scheme="explicit")
+    assert e.value.code == lbm_amd.LBM_ERR_UNSUPPORTED
     lat.close()

@@ -943,3 +943,58 @@ def test_stats_running_mean_of_a_linear_signal(oracle_mod):
     want = (K - 1) * (2 * K - 1) * d * d / 24
     assert np.abs(rms_u[..., 0] - np.sqrt(want)).max() < 1e-16
     assert np.abs(avg_rs[..., 0, 0] - want).max() < 1e-18
+
+
+# ---- IBM, explicit branch (lbm.cpp:1171-1258) ------------------------------------
+def test_ibm_explicit_uniform_flow_cell_summed_drag(oracle_mod):
+    """Uniform flow u0 past markers at rest in the 2D slab: the unforced
+    interpolation gives u0 phi(0) = u0/2, F_b = -rho u0, and the spread
+    force sums over the box cells to sum_n F_b ds * 1/2 (the z-weight
+    again), so the cell-summed drag is u0 sum(ds)/2 -- half the implicit
+    branch's marker-summed m_max = 1 value.  The spread force enters with
+    + (lbm.cpp:1244-1246): each box cell's u gains F/2 rho."""
+    u0 = (0.03, -0.01, 0.0)
+    o = _uniform_ibm_lattice(oracle_mod, u0)
+    mk = cases.cylinder_markers(7.2, 7.4, 6.0, 24, [1.0])
+    err, out = o.ibm_step(mk, (0, 0, 0), (0, 0, 0), (2, 13, 2, 13, 1, 1), scheme="explicit")
+    assert err == 0
+    Fb = mk["Fb"].reshape(-1, 3)
+    assert np.abs(Fb[:, 0] + u0[0]).max() < 1e-15 and np.abs(Fb[:, 1] + u0[1]).max() < 1e-15
+    ds = mk["ds"].sum()
+    assert out[0] == pytest.approx(u0[0] * ds / 2, rel=1e-12)
+    assert out[1] == pytest.approx(u0[1] * ds / 2, rel=1e-12)
+    Fx, _, _ = o.force()
+    _, ux, _, _ = o.macro()
+    box = (slice(2, 14), slice(2, 14), 1)
+    assert np.abs(ux[box] - (u0[0] + Fx[box] / 2)).max() < 1e-15
+    assert Fx[box].min() < -1e-4  # the force opposes the flow at the markers
+
+
+def test_ibm_explicit_nearest_point_brute_force(oracle_mod):
+    """The explicit branch's density point (NearestLatticeCoordinate over the
+    marker's floor/ceil neighbourhood, lbm.cpp:1218-1222): with the fluid at
+    rest and the body moving at V, F_b = 2 rho_near V, so F_b/2V names the
+    chosen cell.  Expected: the last (x, y) in loop order with the largest
+    positive dot product along the normal, from a density field whose value
+    differs at every cell."""
+    t, rho, ux, uy, uz = cases._blank(16, 16, 1)
+    i = np.arange(18)[:, None, None]
+    j = np.arange(18)[None, :, None]
+    rho[:] = 1.0 + 0.001 * (i + 20 * j) + 0 * rho
+    o = oracle_mod.Oracle(16, 16, 1, 0.8)
+    o.set_cells(type=t, rho=rho, ux=ux, uy=uy, uz=uz)
+    o.init()
+    mk = cases.cylinder_markers(7.2, 7.4, 6.0, 24, [1.0])
+    V = (0.02, 0.0, 0.0)
+    err, out = o.ibm_step(mk, V, V, (2, 13, 2, 13, 1, 1), scheme="explicit")
+    assert err == 0
+    got = mk["Fb"].reshape(-1, 3)[:, 0] / (2 * V[0])
+    for n in range(24):
+        x, y = mk["x"][n], mk["y"][n]
+        best, pick = 0.0, (0, 0)
+        for xi in range(math.floor(x), math.ceil(x) + 1):
+            for yi in range(math.floor(y), math.ceil(y) + 1):
+                d = (xi - x) * mk["nx"][n] + (yi - y) * mk["ny"][n]
+                if d >= best and d > 0:
+                    best, pick = d, (xi, yi)
+        assert got[n] == pytest.approx(rho[pick[0], pick[1], 1], rel=1e-12), n
