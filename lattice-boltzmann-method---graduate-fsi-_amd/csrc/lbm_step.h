@@ -184,7 +184,6 @@ __device__ __forceinline__ bool relax(typename Store<S>::C (&f)[Q], typename Sto
     fpre = C(1) - omega * C(0.5);
     Fu = Fx * ux + Fy * uy + Fz * uz;
   }
-  C amin = mh, amax = mh;  // A of the rest direction is -1.5uu
   // rest direction
   {
     const C fe = ra[0] * mh + da[0];
@@ -200,8 +199,6 @@ __device__ __forceinline__ bool relax(typename Store<S>::C (&f)[Q], typename Sto
       const C s_ = C(4.5) * t2 + mh;
       const C p_ = cu * (C(4.5) * t2 + c3);
       const C Ap = s_ + p_, Am = s_ - p_;
-      amin = vmin(amin, vmin(Ap, Am));
-      amax = vmax(amax, vmax(Ap, Am));
       const C fep = ra[q] * Ap + da[q], fem = ra[q] * Am + da[q];
       C fp = fep + om1 * (f[m] - fep);
       C fm = fem + om1 * (f[m + 1] - fem);
@@ -217,7 +214,26 @@ __device__ __forceinline__ bool relax(typename Store<S>::C (&f)[Q], typename Sto
       f[m + 1] = fm;
     }
   });
-  // feq = rho w (1 + A) < 0 for some direction (lbm.cpp:114-117)
+  // feq = rho w (1 + A) < 0 for some direction (lbm.cpp:114-117).  For
+  // |u|^2 <= 0.09 every 1 + A >= 0.23: over D3Q19 the smallest A is at
+  // c.u = -sqrt(2)|u| (A grows with c.u while 13.5 a^2 + 9a + 3 - 4.5|u|^2
+  // has no root, i.e. |u|^2 < 1/3), 1 + A = 1 - 3 sqrt(2) U + 7.5 U^2 -
+  // 4.5 sqrt(2) U^3 >= 0.23 at U = 0.3, far above rounding; feq then has the
+  // sign of rho.  Faster cells (and NaN) take the direction-by-direction test.
+  if (uu <= C(0.09)) return rho < C(0);
+  C amin = mh, amax = mh;  // A of the rest direction is -1.5uu
+  for_dirs([&](auto M) {
+    constexpr int m = decltype(M)::value;
+    if constexpr (m % 2 == 1) {
+      const C cu = C(CX[m]) * ux + C(CY[m]) * uy + C(CZ[m]) * uz;
+      const C t2 = cu * cu;
+      const C s_ = C(4.5) * t2 + mh;
+      const C p_ = cu * (C(4.5) * t2 + c3);
+      const C Ap = s_ + p_, Am = s_ - p_;
+      amin = vmin(amin, vmin(Ap, Am));
+      amax = vmax(amax, vmax(Ap, Am));
+    }
+  });
   return (rho > C(0) && C(1) + amin < C(0)) || (rho < C(0) && C(1) + amax > C(0));
 }
 
