@@ -204,6 +204,33 @@ def test_negative_equilibrium_error():
     assert cell == np.ravel_multi_index((4, 4, 4), case.shape)
 
 
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_negative_equilibrium_sign_of_rho_for_slow_cells(oracle_mod, storage):
+    """A slow cell (|u|^2 <= 0.09, where every 1 + A of the cubic
+    equilibrium is >= 0.23) with rho < 0 has feq < 0 in every direction: the
+    collision reports it as the reference terminates (lbm.cpp:114-117); a
+    fast cell with rho > 0 takes the direction-by-direction test."""
+    case = cases.cavity(8)
+    case.rho[3, 4, 5] = -0.5
+    case.ux[3, 4, 5] = 0.2
+    ref = oracle_mod.run_case(case, 0)
+    assert ref.collision() == 1
+    lat = _gpu(case, storage)
+    lat.collision()
+    code, cell = lat.status()
+    assert code == lbm_amd.LBM_ERR_FEQ_NEGATIVE
+    assert cell == np.ravel_multi_index((3, 4, 5), case.shape)
+    lat.close()
+    case = cases.cavity(8)
+    case.ux[3, 4, 5], case.uy[3, 4, 5] = 0.5, -0.2  # |u|^2 = 0.29: feq > 0 still
+    ref = oracle_mod.run_case(case, 0)
+    assert ref.collision() == 0
+    lat = _gpu(case, storage)
+    lat.collision()
+    assert lat.status()[0] == 0
+    lat.close()
+
+
 def test_fp64_reductions_match_host_sums():
     case = cases.taylor_green(24, u0=0.04, Re=200.0)
     lat = _gpu(case, "f64")
