@@ -45,6 +45,7 @@ SOURCES = [
     os.path.join(HERE, "csrc", "lbm_q7.hip"),
     os.path.join(HERE, "csrc", "lbm_filament.hip"),
     os.path.join(HERE, "csrc", "lbm_mea.hip"),
+    os.path.join(HERE, "csrc", "lbm_faces.hip"),
     os.path.join(HERE, "csrc", "lbm_stats.hip"),
 ]
 HOST_SOURCES = [

@@ -427,6 +427,34 @@ def random_mask_case(Nx: int, Ny: int, Nz: int, seed: int = 7, p_solid: float = 
     return Case("random_mask", Nx, Ny, Nz, tau, t, rho, ux, uy, uz, params=dict(seed=seed))
 
 
+def face_case(Nx: int, Ny: int, Nz: int, seed: int = 3, p_solid: float = 0.1, u_amp: float = 0.03,
+              tau: float = 0.8, shell=(TYPE_OUT, TYPE_IN, TYPE_S, TYPE_F, 4)) -> Case:
+    """Every shell cell drawn from `shell` (outlets, inlets, solids, fluid,
+    TYPE_P), the interior fluid with random solids and a few interior
+    outlet / TYPE_P cells: the face loops of Streaming (lbm.cpp:299-625) on
+    every face, edge and corner, in both orders of neighbouring types, with
+    fluid, solid and other-type inner neighbours."""
+    rng = np.random.default_rng(seed)
+    shape = (Nx + 2, Ny + 2, Nz + 2)
+    t = np.zeros(shape, np.int16)
+    inner = (slice(1, -1),) * 3
+    r = rng.random(shape)
+    t[r < p_solid] = TYPE_S
+    t[(r >= p_solid) & (r < p_solid + 0.02)] = np.int16(TYPE_OUT)
+    t[(r >= p_solid + 0.02) & (r < p_solid + 0.04)] = np.int16(4)
+    shell_t = np.asarray(shell, np.int16)[rng.integers(0, len(shell), shape)]
+    keep = t[inner].copy()
+    t[...] = shell_t
+    t[inner] = keep
+    rho = 1.0 + 0.01 * rng.standard_normal(shape)
+    ux = u_amp * rng.standard_normal(shape)
+    uy = u_amp * rng.standard_normal(shape)
+    uz = u_amp * rng.standard_normal(shape)
+    still = (t == TYPE_S) & (rng.random(shape) < 0.5)
+    ux[still] = uy[still] = uz[still] = 0.0
+    return Case("faces", Nx, Ny, Nz, tau, t, rho, ux, uy, uz, params=dict(seed=seed))
+
+
 def cylinder_bb(Nx: int = 100, Ny: int = 100, Nz: int = 1, D: float = 20.0, u_max: float = 0.1,
                 Re: float = 200.0, center=None) -> Case:
     """main_setup_Cylinder with Generate_Cylinder (lbm.cpp:1429-1487): cells

@@ -206,6 +206,9 @@ int lbm_batch_create(lbm_ctx *const *ctxs, int32_t n, lbm_batch **out) {
     if (c->dev != c0->dev || c->storage != c0->storage || c->force_mode != c0->force_mode)
       return LBM_ERR_UNSUPPORTED;
     if (c->thermal || c->ibm || c->force_mode == 3) return LBM_ERR_UNSUPPORTED;
+    // inlet/outlet (and other non-fluid, non-solid) cells: their face loops
+    // run per lattice between the steps, which one batched launch has not
+    if (c->fnslot) return LBM_ERR_UNSUPPORTED;
     for (int j = 0; j < i; j++)
       if (ctxs[j] == c) return LBM_ERR_ARG;
   }

@@ -108,6 +108,7 @@ static int download_real(lbm_ctx *c, double *dst, const void *src) {
 
 static void free_all(lbm_ctx *c) {
   output_free(c);
+  faces_free(c);
   ibm_free(c);
   thermal_free(c);
   void *ptrs[] = {c->cur, c->oth, c->type, c->mask, c->links, c->uw[0], c->uw[1], c->uw[2],
@@ -122,6 +123,10 @@ int after_stream(lbm_ctx *c) {
   if (!c->backeq_solid_pending) return LBM_OK;
   HIPCHK(launch_backeq_solid(launch_ctx(c), c->cur, c->oth));
   c->backeq_solid_pending = 0;
+  // the written solids' new moments enter the wall velocities from the next
+  // MacroProp on, also when an earlier MacroProp already replaced the setup
+  // velocities
+  if (c->walls_from_setup < 1) c->walls_from_setup = 1;
   return LBM_OK;
 }
 
@@ -301,12 +306,16 @@ int lbm_init(lbm_ctx *c) {
   // 2. f = f* = feq (LBM::Init); with ENERGY also g = g* = geq and the
   //    thermal wall tables
   if (c->thermal == 1) {
+    faces_free(c);  // the ENERGY branch keeps these cells at their set-up values
     if (int e = thermal_init(c)) return e;
   } else {
     HIPCHK(launch_init(L, c->cur, c->oth));
     if (c->thermal == 2) {
       if (int e = q7_init(c)) return e;
     }
+    // the inlet/outlet cells' own populations (before the solids' macro
+    // entries below change)
+    if (int e = faces_setup(c)) return e;
     if (any_solid && L.wall_slots) {
       // the solids' slots will carry bounce-back values: their MacroProp
       // values (moments of their Init populations, which never change) are
@@ -400,6 +409,8 @@ int lbm_stream(lbm_ctx *c) {
   std::swap(c->cur, c->oth);
   c->fstar_ready = 1;
   c->state = ST_F;
+  // the face loops; the other-type cells' MacroProp waits for lbm_macro
+  if (int e = faces_step(c, false, false)) return e;
   return after_stream(c);
 }
 
@@ -418,6 +429,7 @@ int lbm_macro(lbm_ctx *c) {
     if (L.wall_slots) L.macro_sel = 1;
     HIPCHK(launch_macro(L, c->cur));
   }
+  if (int e = faces_macro(c, false)) return e;
   return walls_from_moments(c, true);
 }
 
@@ -437,7 +449,7 @@ static int step_launch(lbm_ctx *c, const LaunchCtx &L, int mode) {
   }
   std::swap(c->cur, c->oth);
   c->fstar_ready = 1;
-  return LBM_OK;
+  return faces_step(c, false);
 }
 
 static int collide_launch(lbm_ctx *c, const LaunchCtx &L) {
@@ -570,6 +582,7 @@ int lbm_download_f(lbm_ctx *c, double *f) {
   double *d = nullptr;
   HIPCHK(hipMalloc((void **)&d, n * 19 * sizeof(double)));
   hipError_t e = launch_f_out(launch_ctx(c), c->cur, d);
+  if (e == hipSuccess) e = faces_f_out(c, d);
   std::vector<double> h;
   if (e == hipSuccess) {
     h.resize(n * 19);

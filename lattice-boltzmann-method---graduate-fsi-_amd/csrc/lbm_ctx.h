@@ -68,22 +68,15 @@ struct lbm_ctx {
   std::vector<int16_t> h_typeE;      // device order
   int16_t *typeE = nullptr;          // device copy of type_Energy
   struct lbm_output *out = nullptr;  // D3Q7 temperature (lbm_q7.hip), thermal == 2
-enum Q7Mode {
-  Q7_FUSED = 0,       // Streaming + MacroProp + next Collision
-  Q7_FUSED_EMIT = 1,  // as Q7_FUSED, and write rho, u, T, q
-  Q7_STREAM = 2,      // Streaming alone
-  Q7_STREAM_EMIT = 3, // Streaming + MacroProp
-  Q7_COLLIDE = 4,     // Collision in place (rho, u, T as stored)
-  Q7_MACRO = 5        // MacroProp of the fluid cells
-};
-int q7_upload(lbm_ctx *c, const lbm_thermal_desc *d, const double *T, const int16_t *type_energy);
-int q7_init(lbm_ctx *c);    // Dirichlet link bits, g = geq
-int q7_tables(lbm_ctx *c);  // Dirichlet link bits (after every k_tables)
-// force_mode: the context's unless given (3: the IBM step)
-int q7_launch(lbm_ctx *c, int mode, int force_mode = -1);
-int q7_download_g(lbm_ctx *c, double *g_dev);
-
-// VTK output staging (lbm_output.hip)
+  // ---- cells of the other types (IN, OUT, P, L, T, Q) and the solids that an
+  // outlet copies from (lbm_faces.hip): FP64 populations and rho, u per slot
+  int32_t *fslot = nullptr;   // per cell: its slot or -1
+  double *fof = nullptr;      // 19 per slot
+  double *fom = nullptr;      // rho, ux, uy, uz per slot
+  int32_t *fcells = nullptr;  // cell of each slot
+  int32_t *fmid = nullptr;    // slotted cells of the other types with 0 < k < Nz+1
+  int fnslot = 0, fnmid = 0;
+  int face_x = 0, face_y = 0;  // some IN/OUT cell on the i (j) faces
 };
 
 #define HIPCHK(x)                              \
@@ -119,6 +112,18 @@ int q7_download_g(lbm_ctx *c, double *g_dev);
 
 // VTK output staging (lbm_output.hip)
 void output_free(lbm_ctx *c);
+
+// face loops of Streaming and MacroProp of the cells of the other types
+// (lbm_faces.hip): slots and Init populations (at lbm_init); the loops and,
+// with macro, MacroProp after a streaming launch (`oth` its source);
+// MacroProp alone; the slotted cells' populations into a download of f.
+// ibm: the moments of an IBM run (F = 0, TYPE_IN/TYPE_OUT only, no range
+// check).
+void faces_free(lbm_ctx *c);
+int faces_setup(lbm_ctx *c);
+int faces_step(lbm_ctx *c, bool ibm, bool macro = true);
+int faces_macro(lbm_ctx *c, bool ibm);
+hipError_t faces_f_out(lbm_ctx *c, double *out);
 
 // launch parameters of the context's current state (lbm_capi.hip)
 lbm::LaunchCtx launch_ctx(const lbm_ctx *c);

@@ -1330,6 +1330,8 @@ int lbm_ibm_apply(lbm_ctx *c, const double vcm[3], const double vcm_before[3], d
   }
   int e = ibm_move(c, vcm, vcm_before);
   if (e != LBM_OK) return e;
+  // the IBM step's moments of the inlet/outlet cells (lbm.cpp:1262-1285)
+  if ((e = faces_macro(c, true)) != LBM_OK) return e;
   HIPCHK(box_moments(c, false));
   e = ibm_iterate(c, vcm, vcm_before, m->out);
   if (e != LBM_OK) return e;
@@ -1374,6 +1376,9 @@ int lbm_ibm_run(lbm_ctx *c, int32_t nsteps, const double *vcm, double *out) {
     HIPCHK(hipMalloc((void **)&m->out, 3 * sizeof(double) * nsteps));
     m->out_cap = nsteps;
   }
+  // the first IBM step's moments of the inlet/outlet cells; later steps take
+  // them after each streaming (lbm.cpp:1262-1285)
+  if (int e = faces_macro(c, true)) return e;
   const double zero[3] = {0, 0, 0};
   for (int s = 0; s < nsteps; s++) {
     const double *vb = vcm ? vcm + 3 * s : zero;
@@ -1397,6 +1402,7 @@ int lbm_ibm_run(lbm_ctx *c, int32_t nsteps, const double *vcm, double *out) {
       if ((e = ibm_stream(c, L, s == nsteps - 1 ? STREAM_COLLIDE_MACRO : STREAM_COLLIDE)) != LBM_OK) return e;
       std::swap(c->cur, c->oth);
       c->fstar_ready = 1;
+      if ((e = faces_step(c, true)) != LBM_OK) return e;
       e = after_stream(c);
       if (e != LBM_OK) return e;
     }
@@ -1410,7 +1416,8 @@ int lbm_ibm_run(lbm_ctx *c, int32_t nsteps, const double *vcm, double *out) {
   std::swap(c->cur, c->oth);
   c->fstar_ready = 1;
   c->state = ST_F;
-  int e = after_stream(c);
+  int e = faces_step(c, true);
+  if (e == LBM_OK) e = after_stream(c);
   if (e != LBM_OK) return e;
   m->ready = 0;
   if (out) {
@@ -1481,6 +1488,7 @@ int lbm_fsi_run(lbm_ctx *c, int32_t nsteps, double *out) {
     HIPCHK(hipMalloc((void **)&m->out, 3 * sizeof(double) * 2 * nsteps));
     m->out_cap = 2 * nsteps;
   }
+  if (int e = faces_macro(c, true)) return e;
   for (int s = 0; s < nsteps; s++) {
     int e;
     LaunchCtx L = launch_ctx(c);
@@ -1495,6 +1503,7 @@ int lbm_fsi_run(lbm_ctx *c, int32_t nsteps, double *out) {
       if ((e = ibm_stream(c, L, s == nsteps - 1 ? STREAM_COLLIDE_MACRO : STREAM_COLLIDE)) != LBM_OK) return e;
       std::swap(c->cur, c->oth);
       c->fstar_ready = 1;
+      if ((e = faces_step(c, true)) != LBM_OK) return e;
       if ((e = after_stream(c)) != LBM_OK) return e;
     }
     c->state = ST_FSTAR;
@@ -1509,7 +1518,8 @@ int lbm_fsi_run(lbm_ctx *c, int32_t nsteps, double *out) {
   std::swap(c->cur, c->oth);
   c->fstar_ready = 1;
   c->state = ST_F;
-  int e = after_stream(c);
+  int e = faces_step(c, true);
+  if (e == LBM_OK) e = after_stream(c);
   if (e != LBM_OK) return e;
   m->ready = 0;
   if (out) HIPCHK(hipMemcpyAsync(out, m->out, 6 * sizeof(double) * nsteps, hipMemcpyDeviceToHost, c->stream));
