@@ -91,10 +91,11 @@ int lbm_upload_cells(lbm_ctx *ctx, const int16_t *type, const double *rho,
 int lbm_init(lbm_ctx *ctx);
 /* LBM::Collision + ConvLBM, lbm.cpp:60-179 (TYPE_F cells, BGK + Guo). */
 int lbm_collide(lbm_ctx *ctx);
-/* LBM::Streaming, lbm.cpp:182-295 (pull form, half-way bounce-back with
- * the per-cell wall velocity of lbm.cpp:941-962, frozen links). */
+/* LBM::Streaming, lbm.cpp:182-625 (pull form, half-way bounce-back with
+ * the per-cell wall velocity of lbm.cpp:941-962, frozen links, then the six
+ * face loops of the inlet/outlet cells, lbm.cpp:299-625). */
 int lbm_stream(lbm_ctx *ctx);
-/* LBM::MacroProp, lbm.cpp:657-812. */
+/* LBM::MacroProp, lbm.cpp:657-812 (every cell, inlet/outlet cells too). */
 int lbm_macro(lbm_ctx *ctx);
 /* main.cpp:212-221 repeated nsteps times (Collision, Streaming, MacroProp)
  * as one fused stream-collide launch per step.  emit_macro != 0 writes the
@@ -335,7 +336,9 @@ int lbm_output_vtk(lbm_ctx *ctx, const lbm_vtk_desc *desc, const char *path);
  * of a few workgroups per CU, each taking a contiguous run of tiles), and the
  * launches of a step pair are captured once into a HIP graph and replayed.
  * Contexts of one batch share device, storage and force mode, have no IBM or
- * thermal branch, and have been initialised (lbm_init). */
+ * thermal branch and no cell that is neither fluid nor solid (inlet/outlet
+ * cells need the face loops between steps: LBM_ERR_UNSUPPORTED), and have
+ * been initialised (lbm_init). */
 typedef struct lbm_batch lbm_batch;
 int lbm_batch_create(lbm_ctx *const *ctxs, int32_t n, lbm_batch **out);
 int lbm_batch_destroy(lbm_batch *batch);
