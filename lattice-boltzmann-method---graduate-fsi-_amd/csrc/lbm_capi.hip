@@ -410,7 +410,7 @@ int lbm_stream(lbm_ctx *c) {
   c->fstar_ready = 1;
   c->state = ST_F;
   // the face loops; the other-type cells' MacroProp waits for lbm_macro
-  if (int e = faces_step(c, false, false)) return e;
+  if (int e = faces_step(c, false, 0)) return e;
   return after_stream(c);
 }
 
@@ -449,7 +449,9 @@ static int step_launch(lbm_ctx *c, const LaunchCtx &L, int mode) {
   }
   std::swap(c->cur, c->oth);
   c->fstar_ready = 1;
-  return faces_step(c, false);
+  // MacroProp of the other-type cells: the inlets every step, all of them
+  // on steps that emit rho, u
+  return faces_step(c, false, mode == STREAM_ONLY ? 0 : mode == STREAM_COLLIDE ? 1 : 2);
 }
 
 static int collide_launch(lbm_ctx *c, const LaunchCtx &L) {

@@ -71,12 +71,13 @@ struct lbm_ctx {
   // ---- cells of the other types (IN, OUT, P, L, T, Q) and the solids that an
   // outlet copies from (lbm_faces.hip): FP64 populations and rho, u per slot
   int32_t *fslot = nullptr;   // per cell: its slot or -1
-  double *fof = nullptr;      // 19 per slot
-  double *fom = nullptr;      // rho, ux, uy, uz per slot
+  double *fof = nullptr;      // 19 per slot, direction-major
+  double *fom = nullptr;      // rho, ux, uy, uz per slot, component-major
   int32_t *fcells = nullptr;  // cell of each slot
-  int32_t *fmid = nullptr;    // slotted cells of the other types with 0 < k < Nz+1
-  int fnslot = 0, fnmid = 0;
-  int face_x = 0, face_y = 0;  // some IN/OUT cell on the i (j) faces
+  void *fxe = nullptr, *fye = nullptr, *fze = nullptr, *fme = nullptr;  // work lists
+  int fnslot = 0;
+  int fnx = 0, fnx0 = 0, fny = 0, fny0 = 0, fnz = 0;  // i-face, j-face and column entries
+  int fnm = 0, fnm_in = 0;                             // MacroProp entries (inlets first)
 };
 
 #define HIPCHK(x)                              \
@@ -114,14 +115,14 @@ int q7_download_g(lbm_ctx *c, double *g_dev);
 void output_free(lbm_ctx *c);
 
 // face loops of Streaming and MacroProp of the cells of the other types
-// (lbm_faces.hip): slots and Init populations (at lbm_init); the loops and,
-// with macro, MacroProp after a streaming launch (`oth` its source);
-// MacroProp alone; the slotted cells' populations into a download of f.
-// ibm: the moments of an IBM run (F = 0, TYPE_IN/TYPE_OUT only, no range
-// check).
+// (lbm_faces.hip): slots and Init populations (at lbm_init); the loops after
+// a streaming launch (`oth` its source) with MacroProp of no cell (macro 0),
+// the inlets (1) or every cell of the other types (2); MacroProp alone; the
+// slotted cells' populations into a download of f.  ibm: the moments of an
+// IBM run (F = 0, TYPE_IN/TYPE_OUT only, no range check).
 void faces_free(lbm_ctx *c);
 int faces_setup(lbm_ctx *c);
-int faces_step(lbm_ctx *c, bool ibm, bool macro = true);
+int faces_step(lbm_ctx *c, bool ibm, int macro);
 int faces_macro(lbm_ctx *c, bool ibm);
 hipError_t faces_f_out(lbm_ctx *c, double *out);
 

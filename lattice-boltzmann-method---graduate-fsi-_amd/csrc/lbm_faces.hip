@@ -16,12 +16,13 @@
 // (their links stay frozen).  It sets the populations and the rho, u that
 // the reference holds for its inlet and outlet cells.  Here each cell of the
 // other types (IN, OUT, P, L, T, Q, wherever it lies), and each solid that an
-// outlet copies from, has a slot: its 19 populations in `of` and its rho, u
-// in `om`, both FP64.  Their plane entries keep carrying the frozen values of
-// their fluid neighbours.  So does a fluid cell at k = Nz+1 under an inlet
-// at k = 0 that an outlet copies from: the k = 0 loop overwrites all its 19
-// populations, so its frozen links (from cells of the other types) hold that
-// write, which its slot keeps, not their Init values.
+// outlet copies from, has a slot: its 19 populations and its rho, u in FP64
+// arrays, direction-major (`of[l * nslot + k]`, `om[q * nslot + k]`).  Their
+// plane entries keep carrying the frozen values of their fluid neighbours.
+// So does a fluid cell at k = Nz+1 under an inlet at k = 0 that an outlet
+// copies from: the k = 0 loop overwrites all its 19 populations, so its
+// frozen links (from cells of the other types) hold that write, which its
+// slot keeps, not their Init values.
 //
 // Per step, after the streaming launch that pulled from buffer `src`:
 //   launch 1  faces i = 0 and i = Nx+1 (they write disjoint cells and read
@@ -35,6 +36,13 @@
 // reference's order gives.  A fluid neighbour's population is its pulled
 // value, read again from `src` where the streaming read it (plane l at
 // x - c_l: a fluid cell's f*, a solid's bounce-back slot or a frozen value).
+//
+// The launches walk host-built lists of the inlet/outlet cells (16 or 32
+// bytes per entry, read coalesced) in slot order, so the slotted arrays are
+// read and written in runs.  MacroProp: every step for the inlets (their
+// rho, u feed the next step's equilibrium and the k = 0 write), for the
+// other cells on steps that emit rho, u (their values reach nothing else;
+// their density check runs then).
 //
 // In IBM runs (main.cpp:197-211) MacroProp is not called: the IBM step takes
 // rho = sum f, u = sum f c / rho of the TYPE_IN/TYPE_OUT cells with F = 0 and
@@ -53,15 +61,41 @@ namespace {
 
 constexpr int FACE_THREADS = 256;
 
+// source codes of a face-loop entry
+constexpr int32_t SRC_IN = -1;     // TYPE_IN: its own equilibrium
+constexpr int32_t SRC_INERT = -2;  // another type: no loop action, MacroProp only
+constexpr int32_t SRC_NONE = -3;   // not a cell of the other types
+// MacroProp kinds of a listed cell
+constexpr int32_t K_IN = 1, K_OUT = 2, K_INERT = 3;
+
+// one x or y face cell: d (slot ds) takes from s (code ss, below), or an
+// inlet's equilibrium (s = SRC_IN)
+struct XYEntry {
+  int32_t d, ds, s, ss;
+};
+// one (i,j) column: its k = Nz+1 and k = 0 end cells
+struct ZEntry {
+  int32_t top, ts, tsrc, tss;
+  int32_t bot, bs, bsrc, bss;
+};
+// a cell's MacroProp (the other slotted cells)
+struct MEntry {
+  int32_t idx, slot, kind, pad;
+};
+// source slot code ss: >= 0 a slotted cell of another type or solid (its
+// own value); -1 a fluid cell (its pulled value); <= -2 a fluid cell under
+// the k = 0 inflow write with slot -ss-2 (pulled, or that write on a frozen
+// link)
+
 template <class S>
 struct FaceArgs {
   using C = typename Store<S>::C;
   Geo g;
   const S *src;          // the source buffer of the last streaming launch
   const int16_t *type;   // cell types, padded order
-  const int32_t *slot;   // slot of each cell, -1 for cells without one
-  double *of;            // 19 populations per slot
-  double *om;            // rho, ux, uy, uz per slot
+  double *of;            // populations, direction-major
+  double *om;            // rho, ux, uy, uz, component-major
+  int32_t ns;            // slots
   C *rho, *ux, *uy, *uz;       // the context's macro arrays
   const C *Fx, *Fy, *Fz;       // force mode 1
   double Fu[3];                // force mode 2
@@ -70,90 +104,63 @@ struct FaceArgs {
   unsigned long long *first_bad;
 };
 
-__device__ __forceinline__ uint32_t cell(const Geo &g, uint32_t x, uint32_t y, uint32_t z) {
-  return (z * g.ny + y) * g.px + x;
-}
-
-// Population L of cell s as the face loops see it: a slotted cell's own
-// value; for a fluid cell the value the streaming pulled into it, or, on a
-// frozen link of a slotted fluid cell, the last k = 0 inflow write.
+// Population L of source cell s (slot code ss) as the face loops see it.
 template <class S, int L>
-__device__ __forceinline__ double pop(const FaceArgs<S> &a, uint32_t s) {
-  const int32_t k = a.slot[s];
-  if (k >= 0 && a.type[s] != LBM_TYPE_F) return a.of[(size_t)k * Q + L];
+__device__ __forceinline__ double pop(const FaceArgs<S> &a, uint32_t s, int32_t ss) {
+  if (ss >= 0) return a.of[(size_t)L * a.ns + ss];
   uint32_t x, y, z;
   coords(a.g, s, x, y, z);
   const uint32_t q = wrap_src(a.g, x, y, z, CX[L], CY[L], CZ[L]);
-  if (k >= 0) {
+  if (ss <= -2) {
     const int16_t tq = a.type[q];
-    if (tq != LBM_TYPE_F && tq != LBM_TYPE_S) return a.of[(size_t)k * Q + L];
+    if (tq != LBM_TYPE_F && tq != LBM_TYPE_S) return a.of[(size_t)L * a.ns + (-ss - 2)];
   }
   double v = (double)Store<S>::ld(a.src + (size_t)L * a.g.ps + q);
   if constexpr (Store<S>::shifted) v = v / (double)Store<S>::K + W[L];
   return v;
 }
 
-// TYPE_OUT: f[D] of cell d = f[D] of its inner neighbour s
+// TYPE_OUT: f[D] of slot ds = f[D] of its inner neighbour s
 template <class S, int D0, int D1, int D2>
-__device__ __forceinline__ void out_copy(const FaceArgs<S> &a, uint32_t d, uint32_t s) {
-  double *o = a.of + (size_t)a.slot[d] * Q;
-  const double v0 = pop<S, D0>(a, s), v1 = pop<S, D1>(a, s), v2 = pop<S, D2>(a, s);
-  o[D0] = v0;
-  o[D1] = v1;
-  o[D2] = v2;
+__device__ __forceinline__ void out_copy(const FaceArgs<S> &a, int32_t ds, uint32_t s, int32_t ss) {
+  const double v0 = pop<S, D0>(a, s, ss), v1 = pop<S, D1>(a, s, ss), v2 = pop<S, D2>(a, s, ss);
+  a.of[(size_t)D0 * a.ns + ds] = v0;
+  a.of[(size_t)D1 * a.ns + ds] = v1;
+  a.of[(size_t)D2 * a.ns + ds] = v2;
 }
 
-// TYPE_IN: the second-order equilibrium of cell s's rho, u into slot k
+// TYPE_IN: the second-order equilibrium of slot k's rho, u into slot kd
 // (lbm.cpp:325-337), the reference's expression with cs2 = 1/3
 template <class S>
-__device__ __forceinline__ void in_feq2(const FaceArgs<S> &a, uint32_t s, int32_t k) {
+__device__ __forceinline__ void in_feq2(const FaceArgs<S> &a, int32_t k, int32_t kd) {
 #pragma clang fp contract(off)
-  const double *m = a.om + (size_t)a.slot[s] * 4;
-  const double rho = m[0], ux = m[1], uy = m[2], uz = m[3];
+  const double rho = a.om[k], ux = a.om[a.ns + k], uy = a.om[2 * (size_t)a.ns + k],
+               uz = a.om[3 * (size_t)a.ns + k];
   const double cs2 = 1.0 / 3.0;
   const double u_dot_u = ux * ux + uy * uy + uz * uz;
-  double *o = a.of + (size_t)k * Q;
   for_dirs([&](auto M) {
     constexpr int l = decltype(M)::value;
     const double c_dot_u = ux * CX[l] + uy * CY[l] + uz * CZ[l];
-    o[l] = rho * W[l] * (1 + c_dot_u / cs2 + (c_dot_u * c_dot_u) / (2 * cs2 * cs2) - u_dot_u / (2 * cs2));
+    a.of[(size_t)l * a.ns + kd] =
+        rho * W[l] * (1 + c_dot_u / cs2 + (c_dot_u * c_dot_u) / (2 * cs2 * cs2) - u_dot_u / (2 * cs2));
   });
 }
 
+// MacroProp of one cell of the other types (lbm.cpp:668-675, 772-775): the
+// sums in the reference's l order.  IBM moments: inlets and outlets only.
 template <class S>
-__device__ __forceinline__ void face_cell(const FaceArgs<S> &a, uint32_t d, uint32_t s, int face) {
-  const int16_t t = a.type[d];
-  if (t == LBM_TYPE_IN) {
-    in_feq2(a, d, a.slot[d]);
-  } else if (t == LBM_TYPE_OUT) {
-    switch (face) {
-      case 0: out_copy<S, 1, 5, 8>(a, d, s); break;  // i = 0, lbm.cpp:303-306
-      case 1: out_copy<S, 3, 6, 7>(a, d, s); break;  // i = Nx+1, lbm.cpp:357-360
-      case 2: out_copy<S, 4, 7, 8>(a, d, s); break;  // j = Ny+1, lbm.cpp:412-415
-      case 3: out_copy<S, 2, 5, 6>(a, d, s); break;  // j = 0, lbm.cpp:467-470
-      default: out_copy<S, 4, 7, 8>(a, d, s); break; // k = Nz+1, k = 0
-    }
-  }
-}
-
-// MacroProp of one slotted cell of the other types (lbm.cpp:668-675, 772-775):
-// the sums in the reference's l order.
-template <class S>
-__device__ __forceinline__ void other_macro(const FaceArgs<S> &a, uint32_t idx) {
+__device__ __forceinline__ void other_macro(const FaceArgs<S> &a, uint32_t idx, int32_t k, int32_t kind) {
 #pragma clang fp contract(off)
   using C = typename Store<S>::C;
-  const int16_t t = a.type[idx];
-  if (t == LBM_TYPE_F || t == LBM_TYPE_S) return;
-  if (a.mode == 3 && t != LBM_TYPE_IN && t != LBM_TYPE_OUT) return;
-  const int32_t k = a.slot[idx];
-  const double *f = a.of + (size_t)k * Q;
+  if (a.mode == 3 && kind == K_INERT) return;
   double rho = 0, jx = 0, jy = 0, jz = 0;
   for_dirs([&](auto M) {
     constexpr int l = decltype(M)::value;
-    rho += f[l];
-    jx += f[l] * CX[l];
-    jy += f[l] * CY[l];
-    jz += f[l] * CZ[l];
+    const double f = a.of[(size_t)l * a.ns + k];
+    rho += f;
+    jx += f * CX[l];
+    jy += f * CY[l];
+    jz += f * CZ[l];
   });
   double Fx = 0, Fy = 0, Fz = 0;
   if (a.mode == 1) {
@@ -166,11 +173,10 @@ __device__ __forceinline__ void other_macro(const FaceArgs<S> &a, uint32_t idx) 
     Fz = a.Fu[2];
   }
   const double ux = (jx + 0.5 * Fx) / rho, uy = (jy + 0.5 * Fy) / rho, uz = (jz + 0.5 * Fz) / rho;
-  double *m = a.om + (size_t)k * 4;
-  m[0] = rho;
-  m[1] = ux;
-  m[2] = uy;
-  m[3] = uz;
+  a.om[k] = rho;
+  a.om[a.ns + k] = ux;
+  a.om[2 * (size_t)a.ns + k] = uy;
+  a.om[3 * (size_t)a.ns + k] = uz;
   a.rho[idx] = (C)rho;
   a.ux[idx] = (C)ux;
   a.uy[idx] = (C)uy;
@@ -178,87 +184,76 @@ __device__ __forceinline__ void other_macro(const FaceArgs<S> &a, uint32_t idx) 
   if (a.mode != 3 && (rho < 0 || rho > 100)) flag(a.status, a.first_bad, ST_RHO_RANGE, idx);
 }
 
-// launch 1 (x faces, AXIS 0) and launch 2 (y faces, AXIS 1): thread t
-// handles one cell of face 2*AXIS + (t >= n), n = cells per face
+// launch 1 (AXIS 0: entries [0, n0) on i = 0, then i = Nx+1) and launch 2
+// (AXIS 1: j = Ny+1, then j = 0)
 template <class S, int AXIS>
-__global__ __launch_bounds__(FACE_THREADS) void k_faces_xy(FaceArgs<S> a) {
-  const Geo &g = a.g;
-  const uint32_t n = AXIS == 0 ? g.ny * g.nz : g.nx * g.nz;
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 2 * n) return;
-  const uint32_t side = t >= n, r = t - side * n;
-  uint32_t d, s;
-  if constexpr (AXIS == 0) {
-    const uint32_t y = r % g.ny, z = r / g.ny;
-    d = cell(g, side ? g.nx - 1 : 0, y, z);
-    s = cell(g, side ? g.nx - 2 : 1, y, z);
+__global__ __launch_bounds__(FACE_THREADS) void k_faces_xy(FaceArgs<S> a, const XYEntry *__restrict__ e, int n0,
+                                                           int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const XYEntry v = e[t];
+  if (v.s == SRC_IN) {
+    in_feq2(a, v.ds, v.ds);
+  } else if (AXIS == 0) {
+    if (t < n0) out_copy<S, 1, 5, 8>(a, v.ds, (uint32_t)v.s, v.ss);  // i = 0, lbm.cpp:303-306
+    else out_copy<S, 3, 6, 7>(a, v.ds, (uint32_t)v.s, v.ss);         // i = Nx+1, lbm.cpp:357-360
   } else {
-    const uint32_t x = r % g.nx, z = r / g.nx;
-    // face 2 is j = Ny+1, face 3 is j = 0
-    d = cell(g, x, side ? 0 : g.ny - 1, z);
-    s = cell(g, x, side ? 1 : g.ny - 2, z);
+    if (t < n0) out_copy<S, 4, 7, 8>(a, v.ds, (uint32_t)v.s, v.ss);  // j = Ny+1, lbm.cpp:412-415
+    else out_copy<S, 2, 5, 6>(a, v.ds, (uint32_t)v.s, v.ss);         // j = 0, lbm.cpp:467-470
   }
-  if (a.slot[d] < 0) return;
-  face_cell(a, d, s, 2 * AXIS + (int)side);
 }
 
-// launch 3: threads [0, nx*ny) take the columns, the rest the listed
-// slotted cells with 0 < k < Nz+1 (MacroProp only); macro = 0: the loops
-// alone (a Streaming call without the MacroProp of a fused step)
+__device__ __forceinline__ bool macro_due(int macro, int32_t src) {
+  if (src == SRC_NONE || macro == 0) return false;
+  return macro == 2 || src == SRC_IN;
+}
+__device__ __forceinline__ int32_t kind_of(int32_t src) {
+  return src == SRC_IN ? K_IN : src >= 0 ? K_OUT : K_INERT;
+}
+
+// launch 3: threads [0, nz) the columns, then [nz, nz + nm) the listed
+// cells' MacroProp.  loops = 0: MacroProp alone.  macro: 0 none, 1 the
+// inlets, 2 every cell (the listed cells come inlets first).
 template <class S>
-__global__ __launch_bounds__(FACE_THREADS) void k_faces_z(FaceArgs<S> a, const int32_t *__restrict__ mid,
-                                                          int nmid, int macro) {
-  const Geo &g = a.g;
-  const uint32_t ncol = g.nx * g.ny;
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ncol) {
-    if (macro && t - ncol < (uint32_t)nmid) other_macro(a, (uint32_t)mid[t - ncol]);
+__global__ __launch_bounds__(FACE_THREADS) void k_faces_z(FaceArgs<S> a, const ZEntry *__restrict__ ze, int nz,
+                                                          const MEntry *__restrict__ me, int nm, int loops,
+                                                          int macro) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nz) {
+    if (t - nz < nm) {
+      const MEntry v = me[t - nz];
+      other_macro(a, (uint32_t)v.idx, v.slot, v.kind);
+    }
     return;
   }
-  const uint32_t x = t % g.nx, y = t / g.nx;
-  const uint32_t top = cell(g, x, y, g.nz - 1), bot = cell(g, x, y, 0);
-  const bool st = a.slot[top] >= 0, sb = a.slot[bot] >= 0;
-  if (!st && !sb) return;
-  if (st) face_cell(a, top, cell(g, x, y, g.nz - 2), 4);  // k = Nz+1, lbm.cpp:518-570
-  if (sb) {                                                 // k = 0, lbm.cpp:572-625
-    const int16_t tb = a.type[bot];
-    if (tb == LBM_TYPE_OUT) out_copy<S, 4, 7, 8>(a, bot, cell(g, x, y, 1));
-    else if (tb == LBM_TYPE_IN && st) in_feq2(a, bot, a.slot[top]);
+  const ZEntry v = ze[t];
+  if (loops) {
+    if (v.tsrc >= 0) out_copy<S, 4, 7, 8>(a, v.ts, (uint32_t)v.tsrc, v.tss);  // k = Nz+1, lbm.cpp:518-570
+    else if (v.tsrc == SRC_IN) in_feq2(a, v.ts, v.ts);
+    if (v.bsrc >= 0) out_copy<S, 4, 7, 8>(a, v.bs, (uint32_t)v.bsrc, v.bss);  // k = 0, lbm.cpp:572-625
+    else if (v.bsrc == SRC_IN && v.ts >= 0) in_feq2(a, v.bs, v.ts);
   }
-  if (!macro) return;
-  if (st) other_macro(a, top);
-  if (sb) other_macro(a, bot);
-}
-
-// MacroProp of every slotted cell of the other types (lbm_macro; the moments
-// at the start of an IBM run)
-template <class S>
-__global__ __launch_bounds__(FACE_THREADS) void k_faces_macro(FaceArgs<S> a, const int32_t *__restrict__ cells,
-                                                              int n) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) other_macro(a, (uint32_t)cells[t]);
+  if (macro_due(macro, v.tsrc)) other_macro(a, (uint32_t)v.top, v.ts, kind_of(v.tsrc));
+  if (macro_due(macro, v.bsrc)) other_macro(a, (uint32_t)v.bot, v.bs, kind_of(v.bsrc));
 }
 
 // Init (lbm.cpp:31-57): f = the cubic equilibrium of the set-up rho, u
 template <class S>
-__global__ __launch_bounds__(FACE_THREADS) void k_faces_init(FaceArgs<S> a, const int32_t *__restrict__ cells,
-                                                             int n) {
+__global__ __launch_bounds__(FACE_THREADS) void k_faces_init(FaceArgs<S> a, const int32_t *__restrict__ cells) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
+  if (k >= a.ns) return;
   const uint32_t idx = (uint32_t)cells[k];
   const double rho = (double)a.rho[idx], ux = (double)a.ux[idx], uy = (double)a.uy[idx], uz = (double)a.uz[idx];
   const double uu = ux * ux + uy * uy + uz * uz;
-  double *o = a.of + (size_t)k * Q;
   for_dirs([&](auto M) {
     constexpr int l = decltype(M)::value;
     const double cu = ux * CX[l] + uy * CY[l] + uz * CZ[l];
-    o[l] = rho * W[l] * (1.0 + eq_A(cu, uu));
+    a.of[(size_t)l * a.ns + k] = rho * W[l] * (1.0 + eq_A(cu, uu));
   });
-  double *m = a.om + (size_t)k * 4;
-  m[0] = rho;
-  m[1] = ux;
-  m[2] = uy;
-  m[3] = uz;
+  a.om[k] = rho;
+  a.om[a.ns + k] = ux;
+  a.om[2 * (size_t)a.ns + k] = uy;
+  a.om[3 * (size_t)a.ns + k] = uz;
 }
 
 // the populations of the slotted cells but the fluid ones into a download
@@ -271,7 +266,7 @@ __global__ __launch_bounds__(FACE_THREADS) void k_faces_f_out(const double *__re
   if (t >= n * Q) return;
   const int k = t / Q, l = t - k * Q;
   if (type[cells[k]] == LBM_TYPE_F) return;
-  out[(size_t)cells[k] * Q + l] = of[t];
+  out[(size_t)cells[k] * Q + l] = of[(size_t)l * n + k];
 }
 
 template <class S>
@@ -281,9 +276,9 @@ FaceArgs<S> face_args(const lbm_ctx *c, int mode) {
   a.g = c->g;
   a.src = (const S *)c->oth;
   a.type = c->type;
-  a.slot = c->fslot;
   a.of = c->fof;
   a.om = c->fom;
+  a.ns = c->fnslot;
   a.rho = (C *)c->mac[0];
   a.ux = (C *)c->mac[1];
   a.uy = (C *)c->mac[2];
@@ -301,47 +296,36 @@ FaceArgs<S> face_args(const lbm_ctx *c, int mode) {
 unsigned blocks(uint64_t n) { return (unsigned)((n + FACE_THREADS - 1) / FACE_THREADS); }
 
 template <class S>
-hipError_t faces_step_s(lbm_ctx *c, int mode, int macro) {
+hipError_t faces_pass_s(lbm_ctx *c, int mode, int loops, int macro) {
   const FaceArgs<S> a = face_args<S>(c, mode);
-  const Geo &g = c->g;
-  if (c->face_x) hipLaunchKernelGGL((k_faces_xy<S, 0>), dim3(blocks(2ull * g.ny * g.nz)), dim3(FACE_THREADS), 0, c->stream, a);
-  if (c->face_y) hipLaunchKernelGGL((k_faces_xy<S, 1>), dim3(blocks(2ull * g.nx * g.nz)), dim3(FACE_THREADS), 0, c->stream, a);
-  hipLaunchKernelGGL(k_faces_z<S>, dim3(blocks((uint64_t)g.nx * g.ny + c->fnmid)), dim3(FACE_THREADS), 0, c->stream,
-                     a, c->fmid, c->fnmid, macro);
+  if (loops && c->fnx)
+    hipLaunchKernelGGL((k_faces_xy<S, 0>), dim3(blocks(c->fnx)), dim3(FACE_THREADS), 0, c->stream, a,
+                       (const XYEntry *)c->fxe, c->fnx0, c->fnx);
+  if (loops && c->fny)
+    hipLaunchKernelGGL((k_faces_xy<S, 1>), dim3(blocks(c->fny)), dim3(FACE_THREADS), 0, c->stream, a,
+                       (const XYEntry *)c->fye, c->fny0, c->fny);
+  const int nm = macro == 2 ? c->fnm : macro == 1 ? c->fnm_in : 0;
+  if (c->fnz + nm)
+    hipLaunchKernelGGL(k_faces_z<S>, dim3(blocks((uint64_t)c->fnz + nm)), dim3(FACE_THREADS), 0, c->stream, a,
+                       (const ZEntry *)c->fze, c->fnz, (const MEntry *)c->fme, nm, loops, macro);
   return hipGetLastError();
 }
 
-template <class S>
-hipError_t faces_macro_s(lbm_ctx *c, int mode) {
-  const FaceArgs<S> a = face_args<S>(c, mode);
-  hipLaunchKernelGGL(k_faces_macro<S>, dim3(blocks((uint64_t)c->fnslot)), dim3(FACE_THREADS), 0, c->stream, a,
-                     c->fcells, c->fnslot);
-  return hipGetLastError();
+hipError_t dispatch_pass(lbm_ctx *c, int mode, int loops, int macro) {
+  switch (c->storage) {
+    case LBM_STORE_F64: return faces_pass_s<double>(c, mode, loops, macro);
+    case LBM_STORE_F32: return faces_pass_s<float>(c, mode, loops, macro);
+    case LBM_STORE_F16S: return faces_pass_s<__half>(c, mode, loops, macro);
+    default: return faces_pass_s<f16c_t>(c, mode, loops, macro);
+  }
 }
 
 template <class S>
 hipError_t faces_init_s(lbm_ctx *c) {
   const FaceArgs<S> a = face_args<S>(c, 0);
   hipLaunchKernelGGL(k_faces_init<S>, dim3(blocks((uint64_t)c->fnslot)), dim3(FACE_THREADS), 0, c->stream, a,
-                     c->fcells, c->fnslot);
+                     c->fcells);
   return hipGetLastError();
-}
-
-hipError_t dispatch_step(lbm_ctx *c, int mode, int macro) {
-  switch (c->storage) {
-    case LBM_STORE_F64: return faces_step_s<double>(c, mode, macro);
-    case LBM_STORE_F32: return faces_step_s<float>(c, mode, macro);
-    case LBM_STORE_F16S: return faces_step_s<__half>(c, mode, macro);
-    default: return faces_step_s<f16c_t>(c, mode, macro);
-  }
-}
-hipError_t dispatch_macro(lbm_ctx *c, int mode) {
-  switch (c->storage) {
-    case LBM_STORE_F64: return faces_macro_s<double>(c, mode);
-    case LBM_STORE_F32: return faces_macro_s<float>(c, mode);
-    case LBM_STORE_F16S: return faces_macro_s<__half>(c, mode);
-    default: return faces_macro_s<f16c_t>(c, mode);
-  }
 }
 hipError_t dispatch_init(lbm_ctx *c) {
   switch (c->storage) {
@@ -352,114 +336,177 @@ hipError_t dispatch_init(lbm_ctx *c) {
   }
 }
 
+template <class T>
+bool upload_list(lbm_ctx *c, void **dst, const std::vector<T> &v) {
+  if (v.empty()) return true;
+  if (hipMalloc(dst, v.size() * sizeof(T)) != hipSuccess) return false;
+  return hipMemcpyAsync(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->stream) == hipSuccess;
+}
+
 }  // namespace
 
 void faces_free(lbm_ctx *c) {
-  void *ptrs[] = {c->fslot, c->fof, c->fom, c->fcells, c->fmid};
+  void *ptrs[] = {c->fslot, c->fof, c->fom, c->fcells, c->fxe, c->fye, c->fze, c->fme};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   c->fslot = nullptr;
   c->fof = nullptr;
   c->fom = nullptr;
   c->fcells = nullptr;
-  c->fmid = nullptr;
-  c->fnslot = c->fnmid = 0;
-  c->face_x = c->face_y = 0;
+  c->fxe = c->fye = c->fze = c->fme = nullptr;
+  c->fnslot = c->fnx = c->fnx0 = c->fny = c->fny0 = c->fnz = c->fnm = c->fnm_in = 0;
 }
 
-// Slots for the cells of the other types and for the solids that an outlet
-// copies from, their Init populations, and the work lists.  Call after the
-// set-up rho, u are in the macro arrays and before anything overwrites the
-// solids' entries there.  The ENERGY branch keeps these cells at their set-up
-// values (no slots).
+// Slots for the cells of the other types, for the solids that an outlet
+// copies from and for the fluid cells under the k = 0 inflow write that an
+// outlet copies from; their Init populations; the work lists.  Call after
+// the set-up rho, u are in the macro arrays and before anything overwrites
+// the solids' entries there.  The ENERGY branch keeps these cells at their
+// set-up values (no slots).
 int faces_setup(lbm_ctx *c) {
   faces_free(c);
   if (c->thermal == 1) return LBM_OK;
   const lbm::Geo &g = c->g;
   const std::vector<int16_t> &t = c->h_type;
-  auto at = [&](uint32_t x, uint32_t y, uint32_t z) { return (size_t)((z * g.ny + y) * g.px + x); };
-  std::vector<int32_t> slot(g.ncell, -1);
-  std::vector<int32_t> cells, mid;
-  auto give = [&](size_t d) {
+  auto at = [&](uint32_t x, uint32_t y, uint32_t z) { return (int32_t)((z * g.ny + y) * g.px + x); };
+  auto other = [&](int32_t d) { return t[d] != LBM_TYPE_F && t[d] != LBM_TYPE_S; };
+  auto io = [&](int32_t d) { return t[d] == LBM_TYPE_IN || t[d] == LBM_TYPE_OUT; };
+  bool any = false;
+  for (uint32_t z = 0; z < g.nz && !any; z++)
+    for (uint32_t y = 0; y < g.ny && !any; y++)
+      for (uint32_t x = 0; x < g.nx; x++)
+        if (other(at(x, y, z))) {
+          any = true;
+          break;
+        }
+  if (!any) return LBM_OK;
+  std::vector<int32_t> slot(g.ncell, -1), cells;
+  auto give = [&](int32_t d) {
     if (slot[d] < 0) {
       slot[d] = (int32_t)cells.size();
-      cells.push_back((int32_t)d);
+      cells.push_back(d);
     }
   };
-  for (uint32_t z = 0; z < g.nz; z++)
-    for (uint32_t y = 0; y < g.ny; y++)
-      for (uint32_t x = 0; x < g.nx; x++) {
-        const size_t d = at(x, y, z);
-        if (t[d] != LBM_TYPE_F && t[d] != LBM_TYPE_S) {
-          give(d);
-          if (z > 0 && z < g.nz - 1) mid.push_back((int32_t)d);
-        }
+  // slots in the order the launches visit the cells: the face lists, then
+  // the other cells of the other types
+  std::vector<std::pair<int32_t, int32_t>> xl, yl;  // (d, s) per face cell
+  int nx0 = 0, ny0 = 0;
+  for (int side = 0; side < 2; side++) {
+    for (uint32_t z = 0; z < g.nz; z++)
+      for (uint32_t y = 0; y < g.ny; y++) {
+        const int32_t d = at(side ? g.nx - 1 : 0, y, z), s = at(side ? g.nx - 2 : 1, y, z);
+        if (io(d)) xl.push_back({d, s});
       }
-  if (cells.empty()) return LBM_OK;
-  // solids that some outlet copies from (each face's inner neighbour), and
-  // fluid cells at k = Nz+1 under an inlet at k = 0 that some outlet copies
-  // from
-  const size_t top = (size_t)(g.nz - 1) * g.plane;
-  auto src_solid = [&](size_t d, size_t s) {
-    if (t[d] != LBM_TYPE_OUT) return;
-    if (t[s] == LBM_TYPE_S || (t[s] == LBM_TYPE_F && s >= top && t[s - top] == LBM_TYPE_IN)) give(s);
-  };
-  int fx = 0, fy = 0;
-  for (uint32_t z = 0; z < g.nz; z++)
-    for (uint32_t y = 0; y < g.ny; y++) {
-      src_solid(at(0, y, z), at(1, y, z));
-      src_solid(at(g.nx - 1, y, z), at(g.nx - 2, y, z));
-      for (size_t d : {at(0, y, z), at(g.nx - 1, y, z)})
-        if (t[d] == LBM_TYPE_IN || t[d] == LBM_TYPE_OUT) fx = 1;
-    }
-  for (uint32_t z = 0; z < g.nz; z++)
-    for (uint32_t x = 0; x < g.nx; x++) {
-      src_solid(at(x, g.ny - 1, z), at(x, g.ny - 2, z));
-      src_solid(at(x, 0, z), at(x, 1, z));
-      for (size_t d : {at(x, 0, z), at(x, g.ny - 1, z)})
-        if (t[d] == LBM_TYPE_IN || t[d] == LBM_TYPE_OUT) fy = 1;
-    }
+    if (side == 0) nx0 = (int)xl.size();
+  }
+  for (int side = 0; side < 2; side++) {  // j = Ny+1 first
+    for (uint32_t z = 0; z < g.nz; z++)
+      for (uint32_t x = 0; x < g.nx; x++) {
+        const int32_t d = at(x, side ? 0 : g.ny - 1, z), s = at(x, side ? 1 : g.ny - 2, z);
+        if (io(d)) yl.push_back({d, s});
+      }
+    if (side == 0) ny0 = (int)yl.size();
+  }
+  for (auto &e : xl) give(e.first);
+  for (auto &e : yl) give(e.first);
   for (uint32_t y = 0; y < g.ny; y++)
     for (uint32_t x = 0; x < g.nx; x++) {
-      src_solid(at(x, y, g.nz - 1), at(x, y, g.nz - 2));
-      src_solid(at(x, y, 0), at(x, y, 1));
+      if (other(at(x, y, g.nz - 1))) give(at(x, y, g.nz - 1));
+      if (other(at(x, y, 0))) give(at(x, y, 0));
     }
+  for (uint32_t z = 1; z + 1 < g.nz; z++)
+    for (uint32_t y = 0; y < g.ny; y++)
+      for (uint32_t x = 0; x < g.nx; x++)
+        if (other(at(x, y, z))) give(at(x, y, z));
+  // outlet sources that need a slot: solids, and fluid cells at k = Nz+1
+  // under an inlet at k = 0
+  const int32_t top = (int32_t)((g.nz - 1) * g.plane);
+  auto backeq_fluid = [&](int32_t s) { return t[s] == LBM_TYPE_F && s >= top && t[s - top] == LBM_TYPE_IN; };
+  auto need_src = [&](int32_t d, int32_t s) {
+    if (t[d] == LBM_TYPE_OUT && (t[s] == LBM_TYPE_S || backeq_fluid(s))) give(s);
+  };
+  for (auto &e : xl) need_src(e.first, e.second);
+  for (auto &e : yl) need_src(e.first, e.second);
+  for (uint32_t y = 0; y < g.ny; y++)
+    for (uint32_t x = 0; x < g.nx; x++) {
+      need_src(at(x, y, g.nz - 1), at(x, y, g.nz - 2));
+      need_src(at(x, y, 0), at(x, y, 1));
+    }
+  auto ss_of = [&](int32_t s) {
+    if (slot[s] < 0) return (int32_t)-1;
+    return t[s] == LBM_TYPE_F ? -slot[s] - 2 : slot[s];
+  };
+  auto src_code = [&](int32_t d, int32_t s) {
+    if (!other(d)) return SRC_NONE;
+    if (t[d] == LBM_TYPE_IN) return SRC_IN;
+    if (t[d] == LBM_TYPE_OUT) return s;
+    return SRC_INERT;
+  };
+  std::vector<XYEntry> xe, ye;
+  for (auto &e : xl) xe.push_back({e.first, slot[e.first], src_code(e.first, e.second), ss_of(e.second)});
+  for (auto &e : yl) ye.push_back({e.first, slot[e.first], src_code(e.first, e.second), ss_of(e.second)});
+  std::vector<ZEntry> ze;
+  for (uint32_t y = 0; y < g.ny; y++)
+    for (uint32_t x = 0; x < g.nx; x++) {
+      const int32_t tp = at(x, y, g.nz - 1), bt = at(x, y, 0);
+      const int32_t tpi = at(x, y, g.nz - 2), bti = at(x, y, 1);
+      if (slot[tp] < 0 && slot[bt] < 0) continue;
+      ze.push_back({tp, slot[tp], src_code(tp, tpi), ss_of(tpi), bt, slot[bt], src_code(bt, bti), ss_of(bti)});
+    }
+  // MacroProp list: the other-type cells off the k faces, inlets first
+  std::vector<MEntry> me;
+  for (int pass = 0; pass < 2; pass++)
+    for (size_t k = 0; k < cells.size(); k++) {
+      const int32_t d = cells[k];
+      const uint32_t z = (uint32_t)d / g.plane;
+      if (!other(d) || z == 0 || z == g.nz - 1) continue;
+      const bool inl = t[d] == LBM_TYPE_IN;
+      if (inl != (pass == 0)) continue;
+      me.push_back({d, (int32_t)k, inl ? K_IN : t[d] == LBM_TYPE_OUT ? K_OUT : K_INERT, 0});
+    }
+  int nm_in = 0;
+  for (auto &v : me) nm_in += v.kind == K_IN;
   const size_t ns = cells.size();
   if (ns > (size_t)INT32_MAX / Q) return LBM_ERR_UNSUPPORTED;
-  if (hipMalloc((void **)&c->fslot, g.ncell * sizeof(int32_t)) != hipSuccess ||
-      hipMalloc((void **)&c->fof, ns * Q * sizeof(double)) != hipSuccess ||
-      hipMalloc((void **)&c->fom, ns * 4 * sizeof(double)) != hipSuccess ||
-      hipMalloc((void **)&c->fcells, ns * sizeof(int32_t)) != hipSuccess ||
-      hipMalloc((void **)&c->fmid, std::max<size_t>(mid.size(), 1) * sizeof(int32_t)) != hipSuccess) {
+  bool ok = hipMalloc((void **)&c->fslot, g.ncell * sizeof(int32_t)) == hipSuccess &&
+            hipMalloc((void **)&c->fof, ns * Q * sizeof(double)) == hipSuccess &&
+            hipMalloc((void **)&c->fom, ns * 4 * sizeof(double)) == hipSuccess &&
+            upload_list(c, (void **)&c->fcells, cells) && upload_list(c, &c->fxe, xe) &&
+            upload_list(c, &c->fye, ye) && upload_list(c, &c->fze, ze) && upload_list(c, &c->fme, me) &&
+            hipMemcpyAsync(c->fslot, slot.data(), g.ncell * sizeof(int32_t), hipMemcpyHostToDevice, c->stream) ==
+                hipSuccess;
+  if (!ok) {
+    (void)hipStreamSynchronize(c->stream);
     faces_free(c);
     return LBM_ERR_NOMEM;
   }
   c->fnslot = (int)ns;
-  c->fnmid = (int)mid.size();
-  c->face_x = fx;
-  c->face_y = fy;
-  HIPCHK(hipMemcpyAsync(c->fslot, slot.data(), g.ncell * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(c->fcells, cells.data(), ns * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
-  if (!mid.empty())
-    HIPCHK(hipMemcpyAsync(c->fmid, mid.data(), mid.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  c->fnx = (int)xe.size();
+  c->fnx0 = nx0;
+  c->fny = (int)ye.size();
+  c->fny0 = ny0;
+  c->fnz = (int)ze.size();
+  c->fnm = (int)me.size();
+  c->fnm_in = nm_in;
   HIPCHK(dispatch_init(c));
   // the host vectors go out of scope: the copies must have landed
   HIPCHK(hipStreamSynchronize(c->stream));
   return LBM_OK;
 }
 
-// The face loops and (macro) MacroProp of the slotted cells after a
-// streaming launch (the buffers already swapped: `oth` is the launch's
-// source).  ibm: the moments of an IBM run instead of MacroProp.
-int faces_step(lbm_ctx *c, bool ibm, bool macro) {
+// The face loops after a streaming launch (the buffers already swapped:
+// `oth` is the launch's source) and MacroProp of the slotted cells: macro 0
+// none, 1 the inlets, 2 every cell.  ibm: the moments of an IBM run instead
+// of MacroProp.
+int faces_step(lbm_ctx *c, bool ibm, int macro) {
   if (!c->fnslot) return LBM_OK;
-  HIPCHK(dispatch_step(c, ibm ? 3 : c->force_mode, macro ? 1 : 0));
+  HIPCHK(dispatch_pass(c, ibm ? 3 : c->force_mode, 1, macro));
   return LBM_OK;
 }
 
 int faces_macro(lbm_ctx *c, bool ibm) {
   if (!c->fnslot) return LBM_OK;
-  HIPCHK(dispatch_macro(c, ibm ? 3 : c->force_mode));
+  HIPCHK(dispatch_pass(c, ibm ? 3 : c->force_mode, 0, 2));
   return LBM_OK;
 }
 

@@ -47,6 +47,7 @@ __global__ __launch_bounds__(256) void k_stress(const S *__restrict__ buf, const
                                                 const typename Store<S>::C *__restrict__ vy,
                                                 const typename Store<S>::C *__restrict__ vz, int wall_slots,
                                                 const int32_t *__restrict__ fslot, const double *__restrict__ fof,
+                                                int nslot,
                                                 double tau, double *__restrict__ sig, double *__restrict__ pres) {
   using C = typename Store<S>::C;
   const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -63,7 +64,7 @@ __global__ __launch_bounds__(256) void k_stress(const S *__restrict__ buf, const
     constexpr int m = decltype(M)::value;
     double f;
     if (k >= 0) {
-      f = fof[(size_t)k * Q + m];
+      f = fof[(size_t)m * nslot + k];
     } else if (from_buf) {
       f = (double)Store<S>::ld(buf + (size_t)m * g.ps + idx);
       if constexpr (Store<S>::shifted) f = f / (double)Store<S>::K + W[m];
@@ -148,7 +149,7 @@ void launch_stress(const lbm_ctx *c, double *sig, double *pres) {
   const LaunchCtx L = launch_ctx(c);
   hipLaunchKernelGGL(k_stress<S>, grid_of(c->g.ncell), dim3(256), 0, c->stream, (const S *)c->cur, c->mask, c->g,
                      (const C *)c->mac[0], (const C *)c->mac[1], (const C *)c->mac[2], (const C *)c->mac[3],
-                     L.wall_slots, c->fslot, c->fof, c->d.tau, sig, pres);
+                     L.wall_slots, c->fslot, c->fof, c->fnslot, c->d.tau, sig, pres);
 }
 
 // reference order fluid[i][j][k] <- device order, `comp` values per cell
