@@ -1402,7 +1402,10 @@ int lbm_ibm_run(lbm_ctx *c, int32_t nsteps, const double *vcm, double *out) {
       if ((e = ibm_stream(c, L, s == nsteps - 1 ? STREAM_COLLIDE_MACRO : STREAM_COLLIDE)) != LBM_OK) return e;
       std::swap(c->cur, c->oth);
       c->fstar_ready = 1;
-      if ((e = faces_step(c, true, 1)) != LBM_OK) return e;
+      // the face loops of this launch's streaming and the next IBM step's
+      // moments of the inlets (of every inlet/outlet cell on the last step:
+      // the values the reference holds after the run)
+      if ((e = faces_step(c, true, s == nsteps - 1 ? 2 : 1)) != LBM_OK) return e;
       e = after_stream(c);
       if (e != LBM_OK) return e;
     }
@@ -1416,7 +1419,7 @@ int lbm_ibm_run(lbm_ctx *c, int32_t nsteps, const double *vcm, double *out) {
   std::swap(c->cur, c->oth);
   c->fstar_ready = 1;
   c->state = ST_F;
-  int e = faces_step(c, true, 2);
+  int e = faces_step(c, true, 0);  // no IBM step follows inside the run
   if (e == LBM_OK) e = after_stream(c);
   if (e != LBM_OK) return e;
   m->ready = 0;
@@ -1503,7 +1506,10 @@ int lbm_fsi_run(lbm_ctx *c, int32_t nsteps, double *out) {
       if ((e = ibm_stream(c, L, s == nsteps - 1 ? STREAM_COLLIDE_MACRO : STREAM_COLLIDE)) != LBM_OK) return e;
       std::swap(c->cur, c->oth);
       c->fstar_ready = 1;
-      if ((e = faces_step(c, true, 1)) != LBM_OK) return e;
+      // the face loops of this launch's streaming and the next IBM step's
+      // moments of the inlets (of every inlet/outlet cell on the last step:
+      // the values the reference holds after the run)
+      if ((e = faces_step(c, true, s == nsteps - 1 ? 2 : 1)) != LBM_OK) return e;
       if ((e = after_stream(c)) != LBM_OK) return e;
     }
     c->state = ST_FSTAR;
@@ -1518,7 +1524,7 @@ int lbm_fsi_run(lbm_ctx *c, int32_t nsteps, double *out) {
   std::swap(c->cur, c->oth);
   c->fstar_ready = 1;
   c->state = ST_F;
-  int e = faces_step(c, true, 2);
+  int e = faces_step(c, true, 0);  // no IBM step follows inside the run
   if (e == LBM_OK) e = after_stream(c);
   if (e != LBM_OK) return e;
   m->ready = 0;
