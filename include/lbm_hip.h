@@ -87,7 +87,10 @@ int lbm_upload_cells(lbm_ctx *ctx, const int16_t *type, const double *rho,
                      const double *ux, const double *uy, const double *uz,
                      const double *Fx, const double *Fy, const double *Fz);
 
-/* LBM::Init, lbm.cpp:31-57: f = f_star = feq(rho,u) in every cell. */
+/* LBM::Init, lbm.cpp:31-57: f = f_star = feq(rho,u) in every cell.
+ * LBM_ERR_UNSUPPORTED for an ENERGY set-up with a TYPE_IN cell at k = 0
+ * over a fluid or solid cell at k = Nz+1: the back-face write of g into that
+ * cell (lbm.cpp:611-617) is not applied by the ENERGY kernels. */
 int lbm_init(lbm_ctx *ctx);
 /* LBM::Collision + ConvLBM, lbm.cpp:60-179 (TYPE_F cells, BGK + Guo). */
 int lbm_collide(lbm_ctx *ctx);

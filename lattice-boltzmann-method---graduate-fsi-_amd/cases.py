@@ -289,8 +289,11 @@ def random_thermal_case(Nx: int, Ny: int, Nz: int, seed: int = 5, tau: float = 0
     """random_mask_case with temperatures: every cell T in [0.4, 0.9), every
     solid a Dirichlet (TYPE_T) wall.  Exercises the thermal pull (wrap,
     frozen links next to inlet cells, moving Dirichlet walls, the averaged
-    wall temperature)."""
+    wall temperature).  No inlet on the k = 0 layer: its back-face write
+    would carry g into the fluid or solid cells at k = Nz+1, which the
+    device's ENERGY branch rejects."""
     case = random_mask_case(Nx, Ny, Nz, seed=seed, tau=tau)
+    case.type[:, :, 0][case.type[:, :, 0] == TYPE_IN] = TYPE_F
     rng = np.random.default_rng(seed + 1000)
     T = 0.4 + 0.5 * rng.random(case.shape)
     te = np.where(case.type == TYPE_S, TYPE_T, TYPE_F).astype(np.int16)

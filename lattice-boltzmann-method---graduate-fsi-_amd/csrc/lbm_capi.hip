@@ -306,6 +306,13 @@ int lbm_init(lbm_ctx *c) {
   // 2. f = f* = feq (LBM::Init); with ENERGY also g = g* = geq and the
   //    thermal wall tables
   if (c->thermal == 1) {
+    // the back-face inflow write also sets g = rho w T (1 + ...) of cell
+    // (i,j,Nz+1) (lbm.cpp:611-617), which the ENERGY kernels do not apply:
+    // rejected where that cell is fluid or solid (its g reaches the fluid)
+    const uint32_t plane = c->g.plane, top = (c->g.nz - 1) * plane;
+    for (uint32_t d = 0; d < plane; d++)
+      if (c->h_type[d] == LBM_TYPE_IN && (c->h_type[top + d] == LBM_TYPE_F || c->h_type[top + d] == LBM_TYPE_S))
+        return LBM_ERR_UNSUPPORTED;
     faces_free(c);  // the ENERGY branch keeps these cells at their set-up values
     if (int e = thermal_init(c)) return e;
   } else {

@@ -629,23 +629,46 @@ void orc_wall_velocity(const orc_lbm *lb, int i, int j, int k, double uw[3]) {
   }
 }
 
-/* second-order inflow equilibrium of the face loops (lbm.cpp:337 etc.) */
-static inline void inflow_feq(const orc_lbm *lb, orc_cell *dst, const orc_cell *src) {
+/* second-order inflow equilibrium of the face loops (lbm.cpp:325-345 etc.)
+ * from the rho, u (and, with ENERGY, T) of cell s into cell d:
+ * f = rho w (1 + ...), g = rho w T (1 + ...) */
+static inline void inflow_feq(const orc_lbm *lb, int di, int dj, int dk, int si, int sj, int sk) {
   const double cs2 = lb->cs2;
+  orc_cell *dst = &lb->fluid[di][dj][dk];
+  const orc_cell *src = &lb->fluid[si][sj][sk];
   double rho = src->density, ux = src->ux, uy = src->uy, uz = src->uz;
   double u_dot_u = pow(ux, 2) + pow(uy, 2) + pow(uz, 2);
+  const orc_thermal *th = lb->th;
+  const double T = th ? th->T[LIN(lb, si, sj, sk)] : 0.0;
   for (int l = 0; l < lb->q; l++) {
     double c_dot_u = ux * lb->cx[l] + uy * lb->cy[l] + uz * lb->cz[l];
     double f_eq = rho * lb->w[l] *
                   (1 + c_dot_u / cs2 + pow(c_dot_u, 2) / (2 * cs2 * cs2) - u_dot_u / (2 * cs2));
     dst->f[l] = f_eq;
+    if (th) {
+      double g_eq = rho * lb->w[l] * T *
+                    (1 + c_dot_u / cs2 + pow(c_dot_u, 2) / (2 * cs2 * cs2) - u_dot_u / (2 * cs2));
+      th->g[LIN(lb, di, dj, dk) * lb->q + l] = g_eq;
+    }
   }
 }
 
-static inline void copy3(orc_cell *dst, const orc_cell *src, int a, int b, int c) {
+/* outlet: populations a, b, c (and with ENERGY g a, b, c) of cell s into
+ * cell d (lbm.cpp:303-311 etc.) */
+static inline void copy3(const orc_lbm *lb, int di, int dj, int dk, int si, int sj, int sk, int a, int b,
+                         int c) {
+  orc_cell *dst = &lb->fluid[di][dj][dk];
+  const orc_cell *src = &lb->fluid[si][sj][sk];
   dst->f[a] = src->f[a];
   dst->f[b] = src->f[b];
   dst->f[c] = src->f[c];
+  if (lb->th) {
+    double *g = lb->th->g;
+    const int64_t d = LIN(lb, di, dj, dk) * lb->q, s = LIN(lb, si, sj, sk) * lb->q;
+    g[d + a] = g[s + a];
+    g[d + b] = g[s + b];
+    g[d + c] = g[s + c];
+  }
 }
 
 /* LBM::Streaming, lbm.cpp:182-626. */
@@ -734,48 +757,50 @@ void orc_streaming(orc_lbm *lb) {
 
   /* Face loops, lbm.cpp:299-625, serial.  They use D2Q9 direction indices
    * even in D3Q19 (the reference's own code) and the back face writes its
-   * inflow into k = Nz+1 (lbm.cpp:610).  None of them writes a TYPE_F cell. */
+   * inflow into k = Nz+1 (lbm.cpp:610).  With ENERGY they move g alike
+   * (outlet copies, inlet g_eq = rho w T (1 + ...)).  Only that back-face
+   * write can reach a TYPE_F cell. */
   /* Left (i = 0), lbm.cpp:300-352 */
   for (int j = 0; j <= Ny + 1; j++)
     for (int k = 0; k <= Nz + 1; k++) {
-      orc_cell *c = &fl[0][j][k];
-      if (c->type == ORC_TYPE_OUT) copy3(c, &fl[1][j][k], 1, 5, 8);
-      else if (c->type == ORC_TYPE_IN) inflow_feq(lb, c, c);
+      short t = fl[0][j][k].type;
+      if (t == ORC_TYPE_OUT) copy3(lb, 0, j, k, 1, j, k, 1, 5, 8);
+      else if (t == ORC_TYPE_IN) inflow_feq(lb, 0, j, k, 0, j, k);
     }
   /* Right (i = Nx+1), lbm.cpp:354-407 */
   for (int j = 0; j <= Ny + 1; j++)
     for (int k = 0; k <= Nz + 1; k++) {
-      orc_cell *c = &fl[Nx + 1][j][k];
-      if (c->type == ORC_TYPE_OUT) copy3(c, &fl[Nx][j][k], 3, 6, 7);
-      else if (c->type == ORC_TYPE_IN) inflow_feq(lb, c, c);
+      short t = fl[Nx + 1][j][k].type;
+      if (t == ORC_TYPE_OUT) copy3(lb, Nx + 1, j, k, Nx, j, k, 3, 6, 7);
+      else if (t == ORC_TYPE_IN) inflow_feq(lb, Nx + 1, j, k, Nx + 1, j, k);
     }
   /* Top (j = Ny+1), lbm.cpp:409-462 */
   for (int i = 0; i <= Nx + 1; i++)
     for (int k = 0; k <= Nz + 1; k++) {
-      orc_cell *c = &fl[i][Ny + 1][k];
-      if (c->type == ORC_TYPE_OUT) copy3(c, &fl[i][Ny][k], 4, 7, 8);
-      else if (c->type == ORC_TYPE_IN) inflow_feq(lb, c, c);
+      short t = fl[i][Ny + 1][k].type;
+      if (t == ORC_TYPE_OUT) copy3(lb, i, Ny + 1, k, i, Ny, k, 4, 7, 8);
+      else if (t == ORC_TYPE_IN) inflow_feq(lb, i, Ny + 1, k, i, Ny + 1, k);
     }
   /* Bottom (j = 0), lbm.cpp:464-516 */
   for (int i = 0; i <= Nx + 1; i++)
     for (int k = 0; k <= Nz + 1; k++) {
-      orc_cell *c = &fl[i][0][k];
-      if (c->type == ORC_TYPE_OUT) copy3(c, &fl[i][1][k], 2, 5, 6);
-      else if (c->type == ORC_TYPE_IN) inflow_feq(lb, c, c);
+      short t = fl[i][0][k].type;
+      if (t == ORC_TYPE_OUT) copy3(lb, i, 0, k, i, 1, k, 2, 5, 6);
+      else if (t == ORC_TYPE_IN) inflow_feq(lb, i, 0, k, i, 0, k);
     }
   /* Front (k = Nz+1), lbm.cpp:518-570 */
   for (int i = 0; i <= Nx + 1; i++)
     for (int j = 0; j <= Ny + 1; j++) {
-      orc_cell *c = &fl[i][j][Nz + 1];
-      if (c->type == ORC_TYPE_OUT) copy3(c, &fl[i][j][Nz], 4, 7, 8);
-      else if (c->type == ORC_TYPE_IN) inflow_feq(lb, c, c);
+      short t = fl[i][j][Nz + 1].type;
+      if (t == ORC_TYPE_OUT) copy3(lb, i, j, Nz + 1, i, j, Nz, 4, 7, 8);
+      else if (t == ORC_TYPE_IN) inflow_feq(lb, i, j, Nz + 1, i, j, Nz + 1);
     }
   /* Back (k = 0), lbm.cpp:572-625: reads cell k = 0, writes k = Nz+1 */
   for (int i = 0; i <= Nx + 1; i++)
     for (int j = 0; j <= Ny + 1; j++) {
-      orc_cell *c = &fl[i][j][0];
-      if (c->type == ORC_TYPE_OUT) copy3(c, &fl[i][j][1], 4, 7, 8);
-      else if (c->type == ORC_TYPE_IN) inflow_feq(lb, &fl[i][j][Nz + 1], c);
+      short t = fl[i][j][0].type;
+      if (t == ORC_TYPE_OUT) copy3(lb, i, j, 0, i, j, 1, 4, 7, 8);
+      else if (t == ORC_TYPE_IN) inflow_feq(lb, i, j, Nz + 1, i, j, 0);
     }
 }
 

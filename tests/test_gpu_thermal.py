@@ -138,3 +138,18 @@ def test_thermal_rejects_fp16_and_unset_walls():
         lat.upload_thermal(th.T, None, tau_g=th.tau_g, nu=th.nu, cp=th.cp)
     assert e.value.code == lbm_amd.LBM_ERR_UNSUPPORTED
     lat.close()
+
+
+def test_thermal_rejects_inflow_write_into_fluid():
+    """The k = 0 inlet's back-face write sets g of cell (i,j,Nz+1) too
+    (lbm.cpp:611-617); the ENERGY kernels do not apply it, so a set-up where
+    that cell is fluid or solid is rejected at lbm_init."""
+    case, th = cases.couette_thermal()
+    case.type[1, 3, 0] = cases.TYPE_IN  # fluid above it at k = Nz+1 (periodic z)
+    lat = lbm_amd.Lattice(case.Nx, case.Ny, case.Nz, case.tau)
+    lat.upload(**case.fields())
+    lat.upload_thermal(th.T, th.type_energy, tau_g=th.tau_g, nu=th.nu, cp=th.cp)
+    with pytest.raises(lbm_amd.LBMError) as e:
+        lat.init()
+    assert e.value.code == lbm_amd.LBM_ERR_UNSUPPORTED
+    lat.close()

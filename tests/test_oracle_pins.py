@@ -582,6 +582,59 @@ def test_thermal_wall_temperature_brute_force(oracle_mod):
     assert hit > 50
 
 
+def test_thermal_face_loops_closed_forms(oracle_mod):
+    """ENERGY part of the face loops (lbm.cpp:307-312, 339-345, 523-527,
+    611-617): an inlet's g becomes rho w T (1 + ...) of its own rho, u, T,
+    which sums to rho T, so MacroProp's T = sum g grows by the factor rho
+    per step, T(n) = rho^n T(0), while rho and u stay; an outlet on the
+    i = Nx+1 face holds its inner neighbour's streamed g3, g6, g7 and its
+    own previous g elsewhere; the k = 0 inlet writes g of cell (i,j,Nz+1)
+    with the T of (i,j,0)."""
+    Nx, Ny, Nz = 4, 3, 3
+    shape = (Nx + 2, Ny + 2, Nz + 2)
+    t = np.zeros(shape, np.int16)
+    t[0, 1:-1, 1:-1] = 3      # inlets, i = 0 face interior
+    t[-1, 1:-1, 1:-1] = 2     # outlets, i = Nx+1 face interior
+    t[2, 2, 0] = 3            # an inlet on the k = 0 face ...
+    t[2, 2, -1] = 2           # ... under an outlet at k = Nz+1
+    rng = np.random.default_rng(8)
+    rho = 1 + 0.03 * rng.standard_normal(shape)
+    u = 0.02 * rng.standard_normal((3,) + shape)
+    T = 0.3 + 0.4 * rng.random(shape)
+    te = np.zeros(shape, np.int16)
+    nu = (0.8 - 0.5) / 3
+    alpha, tau_g, k, cp = cases.thermal_constants(nu, 0.7)
+    o = oracle_mod.Oracle(Nx, Ny, Nz, 0.8)
+    o.thermal_enable(tau_g, nu, cp)
+    o.set_cells(type=t, rho=rho, ux=u[0], uy=u[1], uz=u[2])
+    o.set_thermal(T, te)
+    o.init()
+    steps = 5
+    assert o.run(steps) == 0
+    Tn = o.thermal()[0]
+    r = o.macro()[0]
+    inl = np.zeros(shape, bool)
+    inl[0, 1:-1, 1:-1] = True
+    assert np.allclose(r[inl], rho[inl], rtol=1e-14, atol=0)
+    assert np.allclose(Tn[inl], rho[inl] ** steps * T[inl], rtol=1e-13, atol=0)
+    # one more Streaming: the outlets' copies and the k = 0 write
+    g_old = o.g()
+    assert o.collision() == 0
+    o.streaming()
+    g = o.g()
+    copied, kept = [3, 6, 7], [m for m in range(19) if m not in (3, 6, 7)]
+    face = (slice(1, Ny + 1), slice(1, Nz + 1))
+    assert np.array_equal(g[-1][face][..., copied], g[-2][face][..., copied])
+    assert np.array_equal(g[-1][face][..., kept], g_old[-1][face][..., kept])
+    cx, cy, cz, _, w = o.tables()
+    rr, ux, uy, uz = o.macro()
+    Tk = o.thermal()[0]
+    uvec = np.array([ux[2, 2, 0], uy[2, 2, 0], uz[2, 2, 0]])
+    cu = cx * uvec[0] + cy * uvec[1] + cz * uvec[2]
+    want = rr[2, 2, 0] * w * Tk[2, 2, 0] * (1 + 3 * cu + 4.5 * cu ** 2 - 1.5 * (uvec @ uvec))
+    assert np.allclose(g[2, 2, -1], want, rtol=1e-13, atol=0)
+
+
 def test_thermal_solid_temperature_after_first_macroprop(oracle_mod):
     """Solids keep their Init g = rho_s w T_w (lbm.cpp:637-641) and MacroProp
     runs over them (lbm.cpp:777-790), so after the first step a solid's T is
