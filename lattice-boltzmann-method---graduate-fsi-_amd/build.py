@@ -53,11 +53,13 @@ HOST_SOURCES = [
     os.path.join(HERE, "host", "IBM.cpp"),
 ]
 # reference-style drivers on the drop-in classes (host code only)
-EXAMPLES = ["example_cavity", "example_cylinder_ibm", "example_couette_thermal", "example_cylinder_bb"]
+EXAMPLES = ["example_cavity", "example_cylinder_ibm", "example_couette_thermal", "example_cylinder_bb",
+            "example_helpers"]
 EXAMPLE_BIN = {"example_cavity": "lbm_example_cavity",
                "example_cylinder_ibm": "lbm_example_cylinder_ibm",
                "example_couette_thermal": "lbm_example_couette_thermal",
-               "example_cylinder_bb": "lbm_example_cylinder_bb"}
+               "example_cylinder_bb": "lbm_example_cylinder_bb",
+               "example_helpers": "lbm_example_helpers"}
 
 
 def _run(cmd: list[str]) -> None:

@@ -211,6 +211,78 @@ void LBM::SyncMirror(bool populations) {
 
 void LBM::Sup_MacroProp(int n, int Nt) { check(lbm_stats_update(impl_->ctx, n, Nt), "Sup_MacroProp"); }
 
+// ---- per-cell helpers on the mirror (lbm.cpp:628-653, 941-981) ---------------
+namespace {
+// D3Q19, LBM.h:546-551
+const int kCx[lbm_host::npop] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0};
+const int kCy[lbm_host::npop] = {0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1};
+const int kCz[lbm_host::npop] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1};
+const double kW[lbm_host::npop] = {1.0 / 3,  1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 18,
+                                   1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36,
+                                   1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36};
+const double kCs2 = 1.0 / 3.0;  // dx^2/(3 dt^2), LBM.h:86
+
+// the bracket of the cubic equilibrium, lbm.cpp:631
+double cubic(double c_dot_u, double u_dot_u) {
+  return 1 + c_dot_u / kCs2 + std::pow(c_dot_u, 2) / (2 * kCs2 * kCs2) - u_dot_u / (2 * kCs2) +
+         std::pow(c_dot_u, 3) / (6 * std::pow(kCs2, 3)) - c_dot_u * u_dot_u / (2 * kCs2 * kCs2);
+}
+}  // namespace
+
+double LBM::Calc_feq(int i, int j, int k, int l) {
+  const LATTICE &c = fluid[i][j][k];
+  const double u_dot_u = std::pow(c.ux, 2) + std::pow(c.uy, 2) + std::pow(c.uz, 2);
+  const double c_dot_u = c.ux * kCx[l] + c.uy * kCy[l] + c.uz * kCz[l];
+  return c.density * kW[l] * cubic(c_dot_u, u_dot_u);
+}
+
+double LBM::Calc_geq(int i, int j, int k, int l, short type, std::vector<double> uwall) {
+  const LATTICE &c = fluid[i][j][k];
+  if (type == TYPE_S) {
+    const double Twall = Determine_Wall_Temperature(i, j, k);
+    const double u_dot_u = std::pow(uwall[0], 2) + std::pow(uwall[1], 2) + std::pow(uwall[2], 2);
+    const double c_dot_u = uwall[0] * kCx[l] + uwall[1] * kCy[l] + uwall[2] * kCz[l];
+    return c.density * kW[l] * Twall * cubic(c_dot_u, u_dot_u);
+  }
+  const double u_dot_u = std::pow(c.ux, 2) + std::pow(c.uy, 2) + std::pow(c.uz, 2);
+  const double c_dot_u = c.ux * kCx[l] + c.uy * kCy[l] + c.uz * kCz[l];
+  return c.density * kW[l] * c.T * cubic(c_dot_u, u_dot_u);
+}
+
+void LBM::Determine_Wall_Velocity(int i, int j, int k, std::vector<double> &uwall) {
+  int num = 0;
+  for (int l = 0; l < lbm_host::npop; l++) {
+    const LATTICE &n = fluid[(i + kCx[l] + Nx + 2) % (Nx + 2)][(j + kCy[l] + Ny + 2) % (Ny + 2)]
+                            [(k + kCz[l] + Nz + 2) % (Nz + 2)];
+    if (n.type == TYPE_S && (n.ux != 0 || n.uy != 0 || n.uz != 0)) {
+      uwall[0] += n.ux;
+      uwall[1] += n.uy;
+      uwall[2] += n.uz;
+      num++;
+    }
+  }
+  if (num != 0) {
+    uwall[0] /= num;
+    uwall[1] /= num;
+    uwall[2] /= num;
+  }
+}
+
+double LBM::Determine_Wall_Temperature(int i, int j, int k) {
+  int num = 0;
+  double Twall = 0;
+  for (int l = 0; l < lbm_host::npop; l++) {
+    const LATTICE &n = fluid[(i + kCx[l] + Nx + 2) % (Nx + 2)][(j + kCy[l] + Ny + 2) % (Ny + 2)]
+                            [(k + kCz[l] + Nz + 2) % (Nz + 2)];
+    if (n.type == TYPE_S) {
+      Twall += n.T;
+      num++;
+    }
+  }
+  Twall /= num;  // no solid neighbour: 0/0, as the reference
+  return Twall;
+}
+
 int LBM::getNx() const { return Nx; }
 int LBM::getNy() const { return Ny; }
 int LBM::getNz() const { return Nz; }

@@ -92,6 +92,16 @@ class LBM {
   void Step(int n);
   // Copies density/ux/uy/uz (and f, F when asked) from the device.
   void SyncMirror(bool populations = false);
+  // The reference's public per-cell helpers, evaluated on the mirror (the
+  // types, density, u and T of fluid[][][] as the last MacroProp, Step or
+  // SyncMirror left them): the cubic equilibria (lbm.cpp:628-653) and the
+  // wall velocity and temperature a fluid cell's wall links use
+  // (lbm.cpp:941-981).  ConvLBM and Determine_Twall_Neumann, the per-cell
+  // bodies of Collision and Streaming, run inside the device kernels.
+  double Calc_feq(int i, int j, int k, int l);
+  double Calc_geq(int i, int j, int k, int l, short type, std::vector<double> uwall);
+  void Determine_Wall_Velocity(int i, int j, int k, std::vector<double> &uwall);
+  double Determine_Wall_Temperature(int i, int j, int k);
   // LBM::Sup_MacroProp(n) of the MORE_DETAILS build (lbm.cpp:865-939):
   // running averages and rms on the device after the step's MacroProp; Nt is
   // the output interval (the reference's #define Nt 10, LBM.h:493).

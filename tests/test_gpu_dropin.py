@@ -297,3 +297,75 @@ def test_dropin_iblbm_explicit_scheme_matches_oracle(oracle_mod, tmp_path):
     fl = case.type != cases.TYPE_S  # fluid, inlet and outlet cells
     assert rel_l2_u(tuple(fields[1:]), (r_ux, r_uy, r_uz), fl) < 1e-11
     assert np.abs(fields[0] - r_rho)[fl].max() < 1e-12
+
+
+EXE_HELP = os.path.join(ROOT, "lattice-boltzmann-method---graduate-fsi-_amd", "host", "lbm_example_helpers")
+
+
+def test_dropin_cell_helpers_match_oracle(oracle_mod, tmp_path):
+    """The reference's public per-cell helpers of class LBM on the drop-in's
+    mirror (Calc_feq, Determine_Wall_Velocity: lbm.cpp:628-633, 941-962)
+    after 30 steps of a lid-driven cavity, against the oracle's functions on
+    its own state after the same steps; FP64 storage, to 1e-12."""
+    N, steps = 6, 30
+    out = tmp_path / "h.bin"
+    args = [EXE_HELP, "cavity", str(N), str(steps), str(out)]
+    print("running", args, flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    n = N + 2
+    data = np.fromfile(out, dtype=np.float64)
+    feq = data[: n ** 3 * 19].reshape(n, n, n, 19)
+    uw = data[n ** 3 * 19:].reshape(n, n, n, 3)
+    case = cases.cavity(N, u_lid=0.05, Re=100.0)
+    o = oracle_mod.run_case(case, steps)
+    rho, ux, uy, uz = o.macro()
+    want = np.array([[[[o.feq(rho[i, j, k], ux[i, j, k], uy[i, j, k], uz[i, j, k], l) for l in range(19)]
+                       for k in range(n)] for j in range(n)] for i in range(n)])
+    assert np.abs(feq - want).max() < 1e-12
+    moving = 0
+    for i, j, k in zip(*np.nonzero(case.type == cases.TYPE_F)):
+        w = o.wall_velocity(i, j, k)
+        assert np.abs(uw[i, j, k] - w).max() < 1e-14
+        moving += bool(np.any(w != 0))
+    assert moving > 0
+
+
+def test_dropin_thermal_helpers_match_oracle(oracle_mod, tmp_path):
+    """Calc_geq (fluid and wall forms) and Determine_Wall_Temperature
+    (lbm.cpp:636-653, 965-981) on the drop-in's mirror after 40 steps of the
+    thermal Couette case, against the oracle's; FP64 storage, to 1e-12."""
+    Ny, steps = 8, 40
+    out = tmp_path / "t.bin"
+    args = [EXE_HELP, "couette", str(Ny), str(steps), str(out)]
+    print("running", args, flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    shape = (4, Ny + 2, 4)
+    ncell = 4 * (Ny + 2) * 4
+    data = np.fromfile(out, dtype=np.float64)
+    geq = data[: ncell * 19].reshape(shape + (19,))
+    wall = data[ncell * 19:].reshape(shape + (20,))
+    case, th = cases.couette_thermal(Ny=Ny)
+    o = oracle_mod.Oracle(case.Nx, case.Ny, case.Nz, case.tau)
+    o.thermal_enable(th.tau_g, th.nu, th.cp, th.j_dot_n, th.boussinesq, th.g_beta, th.T_ref)
+    o.set_cells(**case.fields())
+    o.set_thermal(th.T, th.type_energy)
+    o.init()
+    assert o.run(steps) == 0
+    for i in range(shape[0]):
+        for j in range(shape[1]):
+            for k in range(shape[2]):
+                want = [o.geq(i, j, k, l) for l in range(19)]
+                assert np.abs(geq[i, j, k] - want).max() < 1e-12, (i, j, k)
+    hit = 0
+    for i, j, k in zip(*np.nonzero(case.type == cases.TYPE_F)):
+        if j not in (1, Ny):
+            continue
+        hit += 1
+        uw = o.wall_velocity(i, j, k)
+        want = [o.geq(i, j, k, l, as_solid=True, uw=uw) for l in range(19)]
+        assert np.abs(wall[i, j, k, :19] - want).max() < 1e-12, (i, j, k)
+        assert abs(wall[i, j, k, 19] - o.wall_temperature(i, j, k)) < 1e-13
+    assert hit > 0
+
