@@ -175,21 +175,23 @@ def test_faces_ibm_run_fp64(oracle_mod):
     lat.close()
 
 
-def test_faces_fp32_against_fp64():
-    """FP32 storage keeps the inlet/outlet cells in FP64: their rho, u
-    follow FP64 storage's to the FP32 fluid's own deviation."""
+@pytest.mark.parametrize("storage,tol", [("f32", 1e-6), ("f16s", 1e-4), ("f16c", 5e-5)])
+def test_faces_reduced_storage_against_fp64(storage, tol):
+    """FP32 and 16-bit storages keep the inlet/outlet cells in FP64: their
+    rho, u follow FP64 storage's to the fluid's own storage error (absolute
+    bound stated per storage, over every non-solid cell after 40 steps)."""
     case = FACE_CASES["inlets_outlets"]()
     out = {}
-    for st in ("f64", "f32"):
+    for st in ("f64", storage):
         lat = _gpu(case, st)
         lat.step(40)
         assert lat.sync() == 0
         out[st] = lat.macro()
         lat.close()
     m = case.type != cases.TYPE_S
-    assert np.abs(out["f32"][0] - out["f64"][0])[m].max() < 1e-5
-    for q in (1, 2, 3):
-        assert np.abs(out["f32"][q] - out["f64"][q])[m].max() < 1e-5
+    errs = [float(np.abs(out[storage][q] - out["f64"][q])[m].max()) for q in range(4)]
+    print(storage, "max |d rho|, |d u|:", errs, flush=True)
+    assert max(errs) < tol
 
 
 def test_faces_density_range_flag():
