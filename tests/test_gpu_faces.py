@@ -64,6 +64,20 @@ def test_faces_fused_steps_fp64(oracle_mod, name):
     lat.close()
 
 
+@pytest.mark.parametrize("seed", [21, 22, 23, 24])
+def test_faces_random_shells_all_types_fp64(oracle_mod, seed):
+    """Shells drawn from every cell type the face loops meet (outlet, inlet,
+    solid, fluid, TYPE_P, TYPE_L), interior outlet/P cells, 30 fused steps:
+    rho, u and f of every non-solid cell."""
+    case = cases.face_case(5, 4, 3, seed=seed, shell=(cases.TYPE_OUT, cases.TYPE_IN, cases.TYPE_S, cases.TYPE_F, 4, 5))
+    ref = oracle_mod.run_case(case, 30)
+    lat = _gpu(case)
+    lat.step(30)
+    assert lat.sync() == 0
+    _cmp_all(lat, ref, case)
+    lat.close()
+
+
 @pytest.mark.parametrize("name", ["random_shell", "inlets_outlets"])
 def test_faces_reference_calls_fp64(oracle_mod, name):
     """Collision, Streaming, MacroProp as separate calls, compared after
