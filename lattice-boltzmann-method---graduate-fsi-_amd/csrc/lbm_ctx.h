@@ -76,8 +76,9 @@ struct lbm_ctx {
   int32_t *fcells = nullptr;  // cell of each slot
   void *fxe = nullptr, *fye = nullptr, *fze = nullptr, *fme = nullptr;  // work lists
   int fnslot = 0;
-  int fnx = 0, fnx0 = 0, fny = 0, fny0 = 0, fnz = 0;  // i-face, j-face and column entries
-  int fnm = 0, fnm_in = 0;                             // MacroProp entries (inlets first)
+  int fnx = 0, fnx0 = 0, fny = 0, fny0 = 0;  // i-face and j-face entries
+  int fnz = 0, fnz_macro = 0;                 // columns with loop work; with MacroProp only too
+  int fnm = 0, fnm_in = 0, fnm_xy = 0;        // MacroProp entries: all, inlets, inlets the i/j launches take
 };
 
 #define HIPCHK(x)                              \

@@ -30,7 +30,9 @@
 //   launch 2  faces j = Ny+1 and j = 0 (likewise);
 //   launch 3  per (i,j) column the k = Nz+1 loop, then the k = 0 loop, then
 //             MacroProp of the column's two end cells; MacroProp of the
-//             other slotted cells.
+//             other slotted cells (an inlet that only its own i or j loop
+//             writes takes its MacroProp in launch 1 or 2, right after its
+//             equilibrium, so launch 3 is skipped when nothing else is due).
 // A loop thus reads the values the earlier loops of this step wrote and the
 // previous step's values of cells that later loops write, as the
 // reference's order gives.  A fluid neighbour's population is its pulled
@@ -69,7 +71,8 @@ constexpr int32_t SRC_NONE = -3;   // not a cell of the other types
 constexpr int32_t K_IN = 1, K_OUT = 2, K_INERT = 3;
 
 // one x or y face cell: d (slot ds) takes from s (code ss, below), or an
-// inlet's equilibrium (s = SRC_IN)
+// inlet's equilibrium (s = SRC_IN; then ss = 1 when no later loop writes the
+// cell, so its MacroProp can follow in the same thread)
 struct XYEntry {
   int32_t d, ds, s, ss;
 };
@@ -185,15 +188,17 @@ __device__ __forceinline__ void other_macro(const FaceArgs<S> &a, uint32_t idx, 
 }
 
 // launch 1 (AXIS 0: entries [0, n0) on i = 0, then i = Nx+1) and launch 2
-// (AXIS 1: j = Ny+1, then j = 0)
+// (AXIS 1: j = Ny+1, then j = 0); macro != 0: MacroProp of the inlets that
+// no later loop writes
 template <class S, int AXIS>
 __global__ __launch_bounds__(FACE_THREADS) void k_faces_xy(FaceArgs<S> a, const XYEntry *__restrict__ e, int n0,
-                                                           int n) {
+                                                           int n, int macro) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const XYEntry v = e[t];
   if (v.s == SRC_IN) {
     in_feq2(a, v.ds, v.ds);
+    if (macro && v.ss == 1) other_macro(a, (uint32_t)v.d, v.ds, K_IN);
   } else if (AXIS == 0) {
     if (t < n0) out_copy<S, 1, 5, 8>(a, v.ds, (uint32_t)v.s, v.ss);  // i = 0, lbm.cpp:303-306
     else out_copy<S, 3, 6, 7>(a, v.ds, (uint32_t)v.s, v.ss);         // i = Nx+1, lbm.cpp:357-360
@@ -295,19 +300,26 @@ FaceArgs<S> face_args(const lbm_ctx *c, int mode) {
 
 unsigned blocks(uint64_t n) { return (unsigned)((n + FACE_THREADS - 1) / FACE_THREADS); }
 
+// The MacroProp list holds the inlets whose MacroProp the x/y launches do
+// (fnm_xy of them) first, then the other inlets, then the other cells: a
+// loop pass starts after the first group, MacroProp alone takes them all.
 template <class S>
 hipError_t faces_pass_s(lbm_ctx *c, int mode, int loops, int macro) {
   const FaceArgs<S> a = face_args<S>(c, mode);
   if (loops && c->fnx)
     hipLaunchKernelGGL((k_faces_xy<S, 0>), dim3(blocks(c->fnx)), dim3(FACE_THREADS), 0, c->stream, a,
-                       (const XYEntry *)c->fxe, c->fnx0, c->fnx);
+                       (const XYEntry *)c->fxe, c->fnx0, c->fnx, macro);
   if (loops && c->fny)
     hipLaunchKernelGGL((k_faces_xy<S, 1>), dim3(blocks(c->fny)), dim3(FACE_THREADS), 0, c->stream, a,
-                       (const XYEntry *)c->fye, c->fny0, c->fny);
-  const int nm = macro == 2 ? c->fnm : macro == 1 ? c->fnm_in : 0;
-  if (c->fnz + nm)
-    hipLaunchKernelGGL(k_faces_z<S>, dim3(blocks((uint64_t)c->fnz + nm)), dim3(FACE_THREADS), 0, c->stream, a,
-                       (const ZEntry *)c->fze, c->fnz, (const MEntry *)c->fme, nm, loops, macro);
+                       (const XYEntry *)c->fye, c->fny0, c->fny, macro);
+  const int first = loops ? c->fnm_xy : 0;
+  const int nm = macro == 2 ? c->fnm - first : macro == 1 ? c->fnm_in - first : 0;
+  // columns with loop work; on passes with every cell's MacroProp also the
+  // columns whose ends only take MacroProp
+  const int nz = (loops && macro != 2) ? c->fnz : c->fnz_macro;
+  if (nz + nm > 0)
+    hipLaunchKernelGGL(k_faces_z<S>, dim3(blocks((uint64_t)nz + nm)), dim3(FACE_THREADS), 0, c->stream, a,
+                       (const ZEntry *)c->fze, nz, (const MEntry *)c->fme + first, nm, loops, macro);
   return hipGetLastError();
 }
 
@@ -354,7 +366,7 @@ void faces_free(lbm_ctx *c) {
   c->fom = nullptr;
   c->fcells = nullptr;
   c->fxe = c->fye = c->fze = c->fme = nullptr;
-  c->fnslot = c->fnx = c->fnx0 = c->fny = c->fny0 = c->fnz = c->fnm = c->fnm_in = 0;
+  c->fnslot = c->fnx = c->fnx0 = c->fny = c->fny0 = c->fnz = c->fnz_macro = c->fnm = c->fnm_in = c->fnm_xy = 0;
 }
 
 // Slots for the cells of the other types, for the solids that an outlet
@@ -442,30 +454,62 @@ int faces_setup(lbm_ctx *c) {
     if (t[d] == LBM_TYPE_OUT) return s;
     return SRC_INERT;
   };
+  // an inlet on an i face off the j and k faces (or on a j face off the i
+  // and k faces) is written by its own loop only: its MacroProp can follow
+  // its equilibrium in the same thread
+  std::vector<char> xy_final(g.ncell, 0);
+  auto final_in = [&](int32_t d, bool on_x) {
+    const uint32_t x = (uint32_t)d % g.px, y = ((uint32_t)d / g.px) % g.ny, z = (uint32_t)d / g.plane;
+    if (t[d] != LBM_TYPE_IN || z == 0 || z == g.nz - 1) return false;
+    return on_x ? (y != 0 && y != g.ny - 1) : (x != 0 && x != g.nx - 1);
+  };
   std::vector<XYEntry> xe, ye;
-  for (auto &e : xl) xe.push_back({e.first, slot[e.first], src_code(e.first, e.second), ss_of(e.second)});
-  for (auto &e : yl) ye.push_back({e.first, slot[e.first], src_code(e.first, e.second), ss_of(e.second)});
-  std::vector<ZEntry> ze;
+  for (auto &e : xl) {
+    const bool fin = final_in(e.first, true);
+    if (fin) xy_final[e.first] = 1;
+    const int32_t s = src_code(e.first, e.second);
+    xe.push_back({e.first, slot[e.first], s, s == SRC_IN ? (int32_t)fin : ss_of(e.second)});
+  }
+  for (auto &e : yl) {
+    const bool fin = final_in(e.first, false);
+    if (fin) xy_final[e.first] = 1;
+    const int32_t s = src_code(e.first, e.second);
+    ye.push_back({e.first, slot[e.first], s, s == SRC_IN ? (int32_t)fin : ss_of(e.second)});
+  }
+  // columns: those with a loop action or a MacroProp first (a loop pass
+  // takes fnz of them), then those with an other-type end and no loop
+  // action, which only MacroProp alone visits
+  std::vector<ZEntry> ze, ze_macro;
   for (uint32_t y = 0; y < g.ny; y++)
     for (uint32_t x = 0; x < g.nx; x++) {
       const int32_t tp = at(x, y, g.nz - 1), bt = at(x, y, 0);
       const int32_t tpi = at(x, y, g.nz - 2), bti = at(x, y, 1);
       if (slot[tp] < 0 && slot[bt] < 0) continue;
-      ze.push_back({tp, slot[tp], src_code(tp, tpi), ss_of(tpi), bt, slot[bt], src_code(bt, bti), ss_of(bti)});
+      const ZEntry z{tp, slot[tp], src_code(tp, tpi), ss_of(tpi), bt, slot[bt], src_code(bt, bti), ss_of(bti)};
+      if (z.tsrc == SRC_NONE && z.bsrc == SRC_NONE) continue;  // slotted sources only: nothing to do
+      const bool act = z.tsrc >= 0 || z.tsrc == SRC_IN || z.bsrc >= 0 || z.bsrc == SRC_IN;
+      (act ? ze : ze_macro).push_back(z);
     }
-  // MacroProp list: the other-type cells off the k faces, inlets first
+  const int nz_loop = (int)ze.size();
+  ze.insert(ze.end(), ze_macro.begin(), ze_macro.end());
+  // MacroProp list: the other-type cells off the k faces; the inlets done by
+  // the x/y launches, then the other inlets, then the rest
   std::vector<MEntry> me;
-  for (int pass = 0; pass < 2; pass++)
+  for (int pass = 0; pass < 3; pass++)
     for (size_t k = 0; k < cells.size(); k++) {
       const int32_t d = cells[k];
       const uint32_t z = (uint32_t)d / g.plane;
       if (!other(d) || z == 0 || z == g.nz - 1) continue;
       const bool inl = t[d] == LBM_TYPE_IN;
-      if (inl != (pass == 0)) continue;
+      const int group = inl ? (xy_final[d] ? 0 : 1) : 2;
+      if (group != pass) continue;
       me.push_back({d, (int32_t)k, inl ? K_IN : t[d] == LBM_TYPE_OUT ? K_OUT : K_INERT, 0});
     }
-  int nm_in = 0;
-  for (auto &v : me) nm_in += v.kind == K_IN;
+  int nm_in = 0, nm_xy = 0;
+  for (auto &v : me) {
+    nm_in += v.kind == K_IN;
+    nm_xy += xy_final[v.idx];
+  }
   const size_t ns = cells.size();
   if (ns > (size_t)INT32_MAX / Q) return LBM_ERR_UNSUPPORTED;
   bool ok = hipMalloc((void **)&c->fslot, g.ncell * sizeof(int32_t)) == hipSuccess &&
@@ -485,9 +529,11 @@ int faces_setup(lbm_ctx *c) {
   c->fnx0 = nx0;
   c->fny = (int)ye.size();
   c->fny0 = ny0;
-  c->fnz = (int)ze.size();
+  c->fnz = nz_loop;
+  c->fnz_macro = (int)ze.size();
   c->fnm = (int)me.size();
   c->fnm_in = nm_in;
+  c->fnm_xy = nm_xy;
   HIPCHK(dispatch_init(c));
   // the host vectors go out of scope: the copies must have landed
   HIPCHK(hipStreamSynchronize(c->stream));
