@@ -58,6 +58,24 @@ def test_vtk_plain_cavity(oracle_mod, tmp_path):
     lat.close()
 
 
+def test_vtk_cylinder_with_inlet_and_outlets(oracle_mod, tmp_path):
+    """The bounce-back cylinder (inlet at i = 0, outlets on the other x and
+    y faces): every block, the inlet and outlet cells' density and velocity
+    included, from the face loops on the device."""
+    case = cases.cylinder_bb(40, 24, 1, D=8.0, u_max=0.05, Re=30.0)
+    ref = oracle_mod.run_case(case, 40)
+    lat = lbm_amd.Lattice(case.Nx, case.Ny, case.Nz, case.tau)
+    lat.upload(**case.fields())
+    lat.init()
+    lat.step(40)
+    pg, pr = tmp_path / "g.vtr", tmp_path / "r.vtr"
+    lat.output_vtk(str(pg), u_scale=0.05)
+    rho, ux, uy, uz = ref.macro()
+    vtk_oracle.write_vtr(pr, case.Nx, case.Ny, case.Nz, rho, ux, uy, uz, case.type, u_scale=0.05)
+    _compare(pg, pr)
+    lat.close()
+
+
 def test_vtk_ibm_force_density(oracle_mod, tmp_path):
     case, mk, box = cases.ibm_cylinder(40, 30, 1, D=8.0, Re=20.0, perturb=0.01)
     o = oracle_mod.Oracle(case.Nx, case.Ny, case.Nz, case.tau)
