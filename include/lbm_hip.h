@@ -258,7 +258,8 @@ int lbm_download_stats(lbm_ctx *ctx, double *avg_u, double *rms_u, double *avg_p
 int lbm_momentum_exchange(lbm_ctx *ctx, const int32_t box[6], double out[3]);
 
 /* ---- flexible filament (no reference counterpart: the reference has no
- * flexible-structure solver; the north star's Lagrangian structure update)
+ * flexible-structure solver; this is the Lagrangian structure update of a
+ * coupled fluid-structure run)
  * ------------------------------------------------------------------------
  * An inextensible elastic filament in lattice units,
  *   m_l X_tt = d/ds (T dX/ds) - EI d4X/ds4 + G - gamma X_t + F,  |dX/ds| = 1,
