@@ -47,6 +47,7 @@ FACE_CASES = {
     "slab_nz1": lambda: cases.face_case(7, 5, 1, seed=4),
     "random_mask": lambda: cases.random_mask_case(6, 5, 4, seed=21),
     "cylinder_2d": lambda: cases.cylinder_bb(40, 24, 1, D=8.0, u_max=0.05, Re=30.0),
+    "sphere_3d": lambda: cases.sphere_bb(30, 18, 18, D=6.0, u_max=0.05, Re=30.0),
 }
 
 
