@@ -202,11 +202,20 @@ def ibm_leg(steps, cmp_steps, warmup=20):
                    cl_f32=float(o32[-1, 1] / q), cl_f64=float(o64[-1, 1] / q))
     ms = dt / steps * 1e3
     tail = out[-min(1000, steps):]
+    # the IBM part's algorithmic traffic: per marker and pass (1 + m_max
+    # interpolations, m_max spreadings) its 64 stencil cells' rho, u read
+    # (16 B) and F written (12 B)
+    nm, m_max = mk["x"].size, 20
+    ibm_bytes = nm * (2 * m_max + 1) * 64 * (16 + 12)
+    ibm_ms = ms - plain_ms
+    ibm_gbs = ibm_bytes / (ibm_ms / 1e3) / 1e9
     return dict(
         workload="IBM cylinder channel 256x128x64 D3Q19 FP32, D=20, 4096 markers, m_max=20, Re=100",
         status=code, steps=steps, ms_per_step=round(ms, 4),
         mlups=round(case.nfluid * steps / dt / 1e6, 1),
         plain_step_ms=round(plain_ms, 4), ibm_share=round(1.0 - plain_ms / ms, 3),
+        ibm_ms_per_step=round(ibm_ms, 4), ibm_traffic_bytes=ibm_bytes,
+        ibm_achieved_gbs=round(ibm_gbs, 1), ibm_frac=round(ibm_gbs / HBM_PEAK_GBS, 4),
         cd_last=float(out[-1, 0] / q), cl_last=float(out[-1, 1] / q),
         cd_mean_last1000=float(tail[:, 0].mean() / q), cl_rms_last1000=float(np.sqrt((tail[:, 1] ** 2).mean()) / q),
         finite=bool(np.isfinite(out).all() and np.isfinite(ux).all()),
@@ -324,78 +333,95 @@ def sphere_leg(steps, every=100, storage="f32", warmup=20):
                 cd=[round(v, 5) for v in cd], cl=[round(v, 6) for v in cl])
 
 
-def batch_leg(steps, storages=("f32", "f16s"), count=96, tune_steps=10):
+def batch_leg(steps, storages=("f32", "f16s"), count=96, tune_steps=10, class_steps=1000):
     """The cavity ensemble: `count` lid-driven cavities, edges drawn from
     {32, 48, 64, 96, 128}, lid velocity and Re from their ranges (seeded).
-    Per storage: sequential per-case launches (each cavity alone, its block
-    size tuned per size class) against one batched launch per step for all
-    cavities (launch shape tuned), both over `steps` fused steps per cavity,
-    HIP-event timed.  The tuners step the cavities, so the timed steps start
-    a few steps after Init."""
+    Per storage, each run from Init: sequential per-case launches (each
+    cavity alone, launch shape tuned per size class) against one batched
+    launch per step for all cavities (launch shape tuned), both over `steps`
+    fused steps per cavity, HIP-event timed.  The tuners time dry runs, so
+    they leave the cavities as they are.  BGK close to tau = 1/2 diverges
+    (the reference stops there too), and a diverged cavity still costs its
+    bandwidth: the rates are also given over the cavities that stay in range
+    (batched again, on their own), and per edge class (`class_steps` steps
+    each way)."""
     specs = cases.batch_cavities(count)
     out = {"cavities": count, "steps": steps,
            "edges": sorted(set(s["N"] for s in specs)),
            "fluid_cells": int(sum(s["N"] ** 3 for s in specs))}
+    cavs = [cases.cavity(sp["N"], u_lid=sp["u_lid"], Re=sp["Re"]) for sp in specs]
+
+    def fresh(lats, which):
+        for i in which:
+            lats[i].upload(**cavs[i].fields())
+            lats[i].init()
+
+    def batched(lats, which, nsteps):
+        fresh(lats, which)
+        b = lbm_amd.Batch([lats[i] for i in which])
+        blk, per, _ = b.autotune(tune_steps)
+        ms = b.time_steps(nsteps)
+        b.close()
+        return ms, blk, per, [lats[i].sync() for i in which]
+
     for st in storages:
-        lats = []
-        for sp in specs:
-            c = cases.cavity(sp["N"], u_lid=sp["u_lid"], Re=sp["Re"])
-            lat = lbm_amd.Lattice(c.Nx, c.Ny, c.Nz, c.tau, storage=st)
-            lat.upload(**c.fields())
-            lat.init()
-            lats.append(lat)
-        # sequential: per size class the tuned block, then each cavity alone
+        lats = [lbm_amd.Lattice(c.Nx, c.Ny, c.Nz, c.tau, storage=st) for c in cavs]
+        fresh(lats, range(count))
+        # sequential: per size class the tuned shape, then each cavity alone
         tuned = {}
         for lat, sp in zip(lats, specs):
             if sp["N"] not in tuned:
                 tuned[sp["N"]] = lat.autotune(tune_steps)
-            lat.set_launch(tuned[sp["N"]])
-        seq_ms = 0.0
-        per_class = {}
-        for lat, sp in zip(lats, specs):
-            t = lat.time_steps(steps)
-            seq_ms += t
-            pc = per_class.setdefault(sp["N"], [0, 0.0])
-            pc[0] += lat.nfluid
-            pc[1] += t
-        codes = [lat.sync() for lat in lats]
-        nfl = sum(lat.nfluid for lat in lats)
-        # batched
-        b = lbm_amd.Batch(lats)
-        blk, per, _ = b.autotune(tune_steps)
-        bat_ms = b.time_steps(steps)
-        b.close()
-        codes_b = [lat.sync() for lat in lats]
+            lat.set_launch(*tuned[sp["N"]])
+        seq = [lat.time_steps(steps) for lat in lats]
+        codes_seq = [lat.sync() for lat in lats]
+        nfl = [lat.nfluid for lat in lats]
+        bat_ms, blk, per, codes = batched(lats, range(count), steps)
+        stable = [i for i, c in enumerate(codes) if c == 0]
+        res = {
+            "sequential_mlups": round(sum(nfl) * steps / (sum(seq) / 1e3) / 1e6, 1),
+            "batched_mlups": round(sum(nfl) * steps / (bat_ms / 1e3) / 1e6, 1),
+            "speedup": round(sum(seq) / bat_ms, 3),
+            "batched_frac": round(sum(n * lats[i].bytes_per_cell for i, n in enumerate(nfl)) * steps
+                                  / (bat_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+            "sequential_blocks": {str(k): list(v) for k, v in sorted(tuned.items())},
+            "batched_launch": {"block": blk, "workgroups_per_cu": per},
+            "diverged": count - len(stable),
+            "diverged_sequential": int(sum(1 for c in codes_seq if c != 0)),
+        }
+        if stable:
+            ms_s, _, _, codes_s = batched(lats, stable, steps)
+            nf = sum(nfl[i] for i in stable)
+            res["stable"] = {
+                "cavities": len(stable), "fluid_cells": nf,
+                "sequential_mlups": round(nf * steps / (sum(seq[i] for i in stable) / 1e3) / 1e6, 1),
+                "batched_mlups": round(nf * steps / (ms_s / 1e3) / 1e6, 1),
+                "batched_frac": round(sum(nfl[i] * lats[i].bytes_per_cell for i in stable) * steps
+                                      / (ms_s / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                "still_in_range": int(sum(1 for c in codes_s if c == 0))}
+        by_edge = {}
+        for N in sorted(set(sp["N"] for sp in specs)):
+            which = [i for i, sp in enumerate(specs) if sp["N"] == N]
+            fresh(lats, which)
+            t_seq = sum(lats[i].time_steps(class_steps) for i in which)
+            ms_b, blk_b, per_b, _ = batched(lats, which, class_steps)
+            nf = sum(nfl[i] for i in which)
+            by_edge[str(N)] = {"cavities": len(which),
+                               "sequential_mlups": round(nf * class_steps / (t_seq / 1e3) / 1e6, 1),
+                               "batched_mlups": round(nf * class_steps / (ms_b / 1e3) / 1e6, 1),
+                               "batched_launch": [blk_b, per_b]}
+        res["by_edge"] = by_edge
         for lat in lats:
             lat.close()
-        out[st] = {
-            "sequential_mlups": round(nfl * steps / (seq_ms / 1e3) / 1e6, 1),
-            "batched_mlups": round(nfl * steps / (bat_ms / 1e3) / 1e6, 1),
-            "speedup": round(seq_ms / bat_ms, 3),
-            "sequential_mlups_by_edge": {str(k): round(v[0] * steps / (v[1] / 1e3) / 1e6, 1)
-                                         for k, v in sorted(per_class.items())},
-            "sequential_blocks": {str(k): v for k, v in sorted(tuned.items())},
-            "batched_launch": {"block": blk, "workgroups_per_cu": per},
-            # BGK at tau close to 1/2 (small edge, high Re) diverges as the
-            # reference's would: density out of range, the step goes on
-            "diverged": int(sum(1 for c in codes_b if c != 0)),
-        }
-        log(f"bench: batch {st} {json.dumps(out[st])}")
+        out[st] = res
+        log(f"bench: batch {st} {json.dumps(res)}")
     return out
 
 
-def cpu_baseline(case, budget_s, max_steps):
-    """The CPU oracle on a bounded sample of the same workload."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle  # the CPU checker/baseline
-
-    threads = os.cpu_count() or 1
-    try:
-        threads = len(os.sched_getaffinity(0))
-    except AttributeError:
-        pass
-    oracle.set_threads(threads)
-    o = oracle.Oracle(case.Nx, case.Ny, case.Nz, case.tau)
+def _oracle_rate(oracle, case, budget_s, max_steps, opt="O3"):
+    """Steps of the oracle on `case` after one warm-up step, until the budget
+    or max_steps: (steps, seconds, MLUPS over TYPE_F cells)."""
+    o = oracle.Oracle(case.Nx, case.Ny, case.Nz, case.tau, opt=opt)
     o.set_cells(**case.fields())
     o.init()
     o.run(1)  # first-touch / warm-up
@@ -408,10 +434,45 @@ def cpu_baseline(case, budget_s, max_steps):
         n += 1
     dt = time.perf_counter() - t0
     o.close()
-    nfl = case.nfluid
-    return dict(value=nfl * n / dt / 1e6, unit="MLUPS", cores=threads, kind="port",
-                sample=f"TGV 256^3 FP64, {n} steps after 1 warm-up step ({dt:.1f} s); "
-                       "oracle/lbm_oracle.c -O3 -fopenmp, AoS 368-B cells, 3 sweeps/step")
+    return n, dt, case.nfluid * n / dt / 1e6
+
+
+def cpu_baseline(case, budget_s, max_steps, o0_budget_s, c1_steps):
+    """The CPU oracle (the reference's OpenMP step restated: AoS 368-B cells,
+    three sweeps per step) on this host: the headline is a bounded sample of
+    the same 256^3 workload built -O3 -fopenmp; beside it the -O0 build (the
+    reference makefile's flags) on a smaller sample, and the lid-driven
+    cavity LBM(64,64,64) (Re 100, u_lid 0.05) for its full 2000 steps."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # the CPU checker/baseline
+
+    threads = os.cpu_count() or 1
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    oracle.set_threads(threads)
+    n, dt, mlups = _oracle_rate(oracle, case, budget_s, max_steps)
+    out = dict(value=mlups, unit="MLUPS", cores=threads, kind="port",
+               sample=f"TGV 256^3 FP64, {n} steps after 1 warm-up step ({dt:.1f} s of a {budget_s:.0f} s "
+                      f"budget); oracle/lbm_oracle.c -O3 -fopenmp, AoS 368-B cells, 3 sweeps/step, "
+                      f"{threads} threads, OMP wall clock")
+    if o0_budget_s > 0:
+        n0, dt0, m0 = _oracle_rate(oracle, case, o0_budget_s, max_steps, opt="O0")
+        out["o0"] = dict(mlups=round(m0, 3), steps=n0, seconds=round(dt0, 2),
+                         flags="-O0 -fopenmp (makefile:6-7)")
+    if c1_steps > 0:
+        c1 = cases.cavity(64, u_lid=0.05, Re=100.0)
+        o = oracle.Oracle(c1.Nx, c1.Ny, c1.Nz, c1.tau)
+        o.set_cells(**c1.fields())
+        o.init()
+        t0 = time.perf_counter()
+        e = o.run(c1_steps)
+        dt1 = time.perf_counter() - t0
+        o.close()
+        out["c1_cavity_64"] = dict(steps=c1_steps, seconds=round(dt1, 2), status=e,
+                                   mlups=round(c1.nfluid * c1_steps / dt1 / 1e6, 2))
+    return out
 
 
 def load_traffic(storage):
@@ -436,8 +497,10 @@ def main():
     ap.add_argument("--precision-steps", type=int, default=5000)
     ap.add_argument("--precision-sizes", default="256,384",
                     help="edges of the Taylor-Green precision study (comma separated)")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--cpu-max-steps", type=int, default=10)
+    ap.add_argument("--cpu-seconds", type=float, default=30.0)
+    ap.add_argument("--cpu-max-steps", type=int, default=100)
+    ap.add_argument("--cpu-o0-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-c1-steps", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--storage", default="f32", choices=["f64", "f32", "f16s", "f16c"])
@@ -517,7 +580,7 @@ def main():
     cpu = None
     if not a.no_cpu:
         log("bench: CPU baseline (oracle, OpenMP)")
-        cpu = cpu_baseline(case, a.cpu_seconds, a.cpu_max_steps)
+        cpu = cpu_baseline(case, a.cpu_seconds, a.cpu_max_steps, a.cpu_o0_seconds, a.cpu_c1_steps)
         log(f"bench: cpu {cpu['value']:.2f} MLUPS on {cpu['cores']} threads")
 
     cav = None

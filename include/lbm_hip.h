@@ -358,14 +358,19 @@ int lbm_batch_set_launch(lbm_batch *batch, int32_t block, int32_t per_cu);
 int lbm_batch_get_launch(const lbm_batch *batch, int32_t *block, int32_t *per_cu);
 int64_t lbm_batch_nfluid(const lbm_batch *batch);
 /* Times every launch shape (block 64/128/256 x per_cu 0/1/2/4/8) over nsteps
- * batched steps and keeps the fastest.  Advances the lattices: initialise
- * them again before the run that matters. */
+ * batched launches and keeps the fastest.  The timed launches are dry runs:
+ * each lattice's fused kernel reads its current populations and writes a
+ * scratch buffer (allocated for the call and freed), so the lattices'
+ * populations, macroscopic fields, status and step count are unchanged. */
 int lbm_batch_autotune(lbm_batch *batch, int32_t nsteps, int32_t *block, int32_t *per_cu,
                        double *best_ms);
-/* Per-lattice tuner: times lbm_time_steps for block 64/128/256, keeps the
- * fastest, and remembers it for every later lattice of the same size and
- * storage in the process.  Advances the lattice. */
-int lbm_autotune(lbm_ctx *ctx, int32_t nsteps, int32_t *block);
+/* Per-lattice tuner: times nsteps dry-run launches of the fused kernel (as
+ * lbm_batch_autotune: the lattice's state is unchanged) for every launch
+ * shape block 64/128/256 x cells per thread 1/2/4, keeps the fastest (also
+ * for every later lattice of the same size and storage in the process) and
+ * sets it on the context.  Plain D3Q19 contexts (LBM_ERR_UNSUPPORTED with a
+ * thermal branch, LBM_ERR_STATE under IBM).  Either pointer may be NULL. */
+int lbm_autotune(lbm_ctx *ctx, int32_t nsteps, int32_t *block, int32_t *cells_per_thread);
 
 /* sizes and identification */
 int64_t lbm_ncells(const lbm_ctx *ctx);
@@ -374,9 +379,11 @@ int32_t lbm_storage(const lbm_ctx *ctx);
 /* Timed run of nsteps fused launches bracketed by HIP events on the
  * context's stream; *ms receives the elapsed milliseconds. */
 int lbm_time_steps(lbm_ctx *ctx, int32_t nsteps, double *ms);
-/* Launch shape of the stream kernels: threads per block (64..256, a
- * multiple of 64); cells per thread must be 1.  No reference counterpart:
- * the launch-shape knob. */
+/* Launch shape of the fused stream kernel: threads per block (64..256, a
+ * multiple of 64) and cells per thread (1, 2 or 4: each thread updates that
+ * many cells, one per grid-stride pass over the lattice).  The other
+ * kernels use the block size with one cell per thread.  No reference
+ * counterpart: the launch-shape knob. */
 int lbm_set_launch(lbm_ctx *ctx, int32_t block, int32_t cells_per_thread);
 int lbm_get_launch(const lbm_ctx *ctx, int32_t *block, int32_t *cells_per_thread);
 /* Bytes one fused launch moves per fluid cell (algorithmic). */

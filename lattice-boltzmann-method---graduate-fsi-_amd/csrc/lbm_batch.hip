@@ -158,15 +158,14 @@ int batched(lbm_batch *b, int32_t nsteps, double *ms) {
   if ((e = upload_args(b)) != LBM_OK) return e;
   if ((e = ensure_graph(b)) != LBM_OK) return e;
   const int K = nsteps - 1;
-  hipEvent_t t0 = nullptr, t1 = nullptr;
+  EventPair ev;
   if (ms) {
-    HIPCHK(hipEventCreate(&t0));
-    HIPCHK(hipEventCreate(&t1));
-    HIPCHK(hipEventRecord(t0, b->stream));
+    HIPCHK(ev.create());
+    HIPCHK(hipEventRecord(ev.t0, b->stream));
   }
   for (int p = 0; p < K / 2; p++) HIPCHK(hipGraphLaunch(b->exec, b->stream));
   if (K % 2) HIPCHK(launch_stream_batch(launch_of(b, STREAM_COLLIDE, 0)));
-  if (ms) HIPCHK(hipEventRecord(t1, b->stream));
+  if (ms) HIPCHK(hipEventRecord(ev.t1, b->stream));
   HIPCHK(launch_stream_batch(launch_of(b, STREAM_MACRO, K % 2)));
   // K + 1 launches: an odd count leaves every lattice in its other buffer
   for (lbm_ctx *c : b->ctx) {
@@ -176,19 +175,110 @@ int batched(lbm_batch *b, int32_t nsteps, double *ms) {
   }
   HIPCHK(hipEventRecord(b->done, b->stream));
   for (lbm_ctx *c : b->ctx) HIPCHK(hipStreamWaitEvent(c->stream, b->done, 0));
-  if (ms) {
-    HIPCHK(hipEventSynchronize(t1));
-    float t = 0.f;
-    HIPCHK(hipEventElapsedTime(&t, t0, t1));
-    *ms = t;
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
-  }
+  if (ms) return ev.elapsed(ms);
   return LBM_OK;
 }
 
-// sizes already tuned in this process: lattice cells -> threads per block
-std::map<uint64_t, int> g_tuned;
+// sizes already tuned in this process: (lattice cells, storage) -> (threads
+// per block, cells per thread)
+std::map<uint64_t, std::pair<int, int>> g_tuned;
+
+// Dry runs of the launch-shape tuners.  The timed launches are the fused
+// stream+collide kernel of each lattice reading its current buffer, as a
+// real step does, but writing a scratch buffer of the same layout, with the
+// status words in scratch too: the caller's populations, macroscopic
+// arrays, status, buffer order and step state stay as they were.  (The
+// current buffer may hold f rather than f*; the kernel does the same work
+// either way.)
+struct DryRun {
+  std::vector<lbm_ctx *> ctx;
+  std::vector<void *> out;
+  int *status = nullptr;
+  unsigned long long *first_bad = nullptr;
+  hipStream_t stream = nullptr;
+  ~DryRun() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (void *p : out) (void)hipFree(p);
+    if (status) (void)hipFree(status);
+    if (first_bad) (void)hipFree(first_bad);
+  }
+  int add(lbm_ctx *c) {
+    if (!status) {
+      HIPCHK(hipMalloc((void **)&status, sizeof(int)));
+      HIPCHK(hipMalloc((void **)&first_bad, sizeof(unsigned long long)));
+    }
+    void *p = nullptr;
+    if (hipMalloc(&p, c->esize * (size_t)c->g.ps * 19) != hipSuccess) return LBM_ERR_NOMEM;
+    ctx.push_back(c);
+    out.push_back(p);
+    stream = c->stream;
+    return LBM_OK;
+  }
+  LaunchCtx launch(size_t l, int block, int cpt) const {
+    LaunchCtx L = launch_ctx(ctx[l]);
+    L.block = block;
+    L.cpt = cpt;
+    L.status = status;
+    L.first_bad = first_bad;
+    return L;
+  }
+  // nsteps dry launches of one context's stream kernel, event-timed
+  int time_ctx(lbm_ctx *c, int block, int cpt, int nsteps, double *ms) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const LaunchCtx L = launch(0, block, cpt);
+    HIPCHK(launch_stream(L, c->cur, out[0], STREAM_COLLIDE));  // warm-up
+    EventPair ev;
+    HIPCHK(ev.create());
+    HIPCHK(hipEventRecord(ev.t0, c->stream));
+    for (int s = 0; s < nsteps; s++) HIPCHK(launch_stream(L, c->cur, out[0], STREAM_COLLIDE));
+    HIPCHK(hipEventRecord(ev.t1, c->stream));
+    return ev.elapsed(ms);
+  }
+  // nsteps dry batched launches of shape (block, per_cu): a graph of a
+  // launch pair replayed, as lbm_batch_step replays its own
+  int time_batch(lbm_batch *b, int block, int per_cu, int nsteps, double *ms) {
+    for (lbm_ctx *c : ctx) HIPCHK(hipStreamSynchronize(c->stream));
+    int e = lbm_batch_set_launch(b, block, per_cu);
+    if (e == LBM_OK) e = build_tiles(b);
+    if (e != LBM_OK) return e;
+    const size_t n = ctx.size();
+    std::vector<unsigned char> h(2 * n * b->args_size, 0);
+    for (size_t l = 0; l < n; l++) {
+      const LaunchCtx L = launch(l, block, 1);
+      batch_fill_args(L, ctx[l]->cur, out[l], &h[(2 * l + 0) * b->args_size]);
+      batch_fill_args(L, ctx[l]->cur, out[l], &h[(2 * l + 1) * b->args_size]);
+    }
+    void *d = nullptr;
+    HIPCHK(hipMalloc(&d, h.size()));
+    struct Free {
+      void *p;
+      hipGraph_t g = nullptr;
+      hipGraphExec_t x = nullptr;
+      ~Free() {
+        if (x) (void)hipGraphExecDestroy(x);
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipFree(p);
+      }
+    } fr{d};
+    HIPCHK(hipMemcpyAsync(d, h.data(), h.size(), hipMemcpyHostToDevice, b->stream));
+    BatchLaunch B = launch_of(b, STREAM_COLLIDE, 0);
+    B.la = d;
+    HIPCHK(launch_stream_batch(B));  // warm-up
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e0 = launch_stream_batch(B);
+    hipError_t e1 = launch_stream_batch(B);
+    hipError_t ec = hipStreamEndCapture(b->stream, &fr.g);
+    if (e0 != hipSuccess || e1 != hipSuccess || ec != hipSuccess) return LBM_ERR_HIP;
+    HIPCHK(hipGraphInstantiate(&fr.x, fr.g, nullptr, nullptr, 0));
+    EventPair ev;
+    HIPCHK(ev.create());
+    HIPCHK(hipEventRecord(ev.t0, b->stream));
+    for (int s = 0; s < nsteps / 2; s++) HIPCHK(hipGraphLaunch(fr.x, b->stream));
+    if (nsteps % 2) HIPCHK(launch_stream_batch(B));
+    HIPCHK(hipEventRecord(ev.t1, b->stream));
+    return ev.elapsed(ms);
+  }
+};
 
 }  // namespace
 
@@ -289,23 +379,35 @@ int64_t lbm_batch_nfluid(const lbm_batch *b) { return b ? b->nfluid : -1; }
 
 int lbm_batch_autotune(lbm_batch *b, int32_t nsteps, int32_t *block, int32_t *per_cu, double *best_ms) {
   if (!b || nsteps < 2) return LBM_ERR_ARG;
+  for (lbm_ctx *c : b->ctx)
+    if (c->state == ST_NONE) return LBM_ERR_STATE;
+  HIPCHK(hipSetDevice(b->dev));
   const int blocks[] = {64, 128, 256};
   const int pers[] = {0, 1, 2, 4, 8};
+  const int b0 = b->block, p0 = b->per_cu;
+  DryRun dry;
+  for (lbm_ctx *c : b->ctx)
+    if (int e = dry.add(c)) return e;
   double best = 1e30;
-  int bb = b->block, bp = b->per_cu;
+  int bb = b0, bp = p0;
   for (int blk : blocks)
     for (int p : pers) {
-      int e = lbm_batch_set_launch(b, blk, p);
-      if (e != LBM_OK) return e;
       double ms = 0;
-      e = lbm_batch_time_steps(b, nsteps, &ms);
-      if (e != LBM_OK) return e;
+      int e = dry.time_batch(b, blk, p, nsteps, &ms);
+      if (e != LBM_OK) {
+        b->block = b0;
+        b->per_cu = p0;
+        return e;
+      }
       if (ms < best) {
         best = ms;
         bb = blk;
         bp = p;
       }
     }
+  // the tile table and graph of the last timed shape are rebuilt for the pick
+  b->block = b0;
+  b->per_cu = p0;
   int e = lbm_batch_set_launch(b, bb, bp);
   if (e != LBM_OK) return e;
   if (block) *block = bb;
@@ -314,31 +416,33 @@ int lbm_batch_autotune(lbm_batch *b, int32_t nsteps, int32_t *block, int32_t *pe
   return LBM_OK;
 }
 
-int lbm_autotune(lbm_ctx *c, int32_t nsteps, int32_t *block) {
+int lbm_autotune(lbm_ctx *c, int32_t nsteps, int32_t *block, int32_t *cells_per_thread) {
   if (!c || nsteps < 1) return LBM_ERR_ARG;
   if (c->state == ST_NONE || c->ibm) return LBM_ERR_STATE;
+  if (c->thermal) return LBM_ERR_UNSUPPORTED;
+  HIPCHK(hipSetDevice(c->dev));
   const uint64_t key = (uint64_t)c->g.ncell * 4 + (uint64_t)c->storage;
   auto it = g_tuned.find(key);
-  if (it != g_tuned.end()) {
-    c->block = it->second;
-    if (block) *block = it->second;
-    return LBM_OK;
+  if (it == g_tuned.end()) {
+    DryRun dry;
+    if (int e = dry.add(c)) return e;
+    double best = 1e30;
+    std::pair<int, int> pick{c->block, c->cpt};
+    for (int blk : {64, 128, 256})
+      for (int cpt : {1, 2, 4}) {
+        double ms = 0;
+        if (int e = dry.time_ctx(c, blk, cpt, nsteps, &ms)) return e;
+        if (ms < best) {
+          best = ms;
+          pick = {blk, cpt};
+        }
+      }
+    it = g_tuned.emplace(key, pick).first;
   }
-  double best = 1e30;
-  int bb = c->block;
-  for (int blk : {64, 128, 256}) {
-    c->block = blk;
-    double ms = 0;
-    int e = lbm_time_steps(c, nsteps, &ms);
-    if (e != LBM_OK) return e;
-    if (ms < best) {
-      best = ms;
-      bb = blk;
-    }
-  }
-  c->block = bb;
-  g_tuned[key] = bb;
-  if (block) *block = bb;
+  c->block = it->second.first;
+  c->cpt = it->second.second;
+  if (block) *block = c->block;
+  if (cells_per_thread) *cells_per_thread = c->cpt;
   return LBM_OK;
 }
 

@@ -32,6 +32,7 @@ LaunchCtx launch_ctx(const lbm_ctx *c) {
   L.storage = c->storage;
   L.force_mode = c->force_mode;
   L.block = c->block;
+  L.cpt = c->cpt;
   L.omega = 1.0 / c->d.tau;  // omega = dt/tau, lbm.cpp:76
   L.Fu[0] = c->Fu[0];
   L.Fu[1] = c->Fu[1];
@@ -119,8 +120,10 @@ static void free_all(lbm_ctx *c) {
   if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
-int after_stream(lbm_ctx *c) {
-  if (!c->backeq_solid_pending) return LBM_OK;
+int after_stream(lbm_ctx *c, bool forced_inlets) {
+  if (!c->backeq_solid_pending && !(forced_inlets && c->backeq_dynamic)) return LBM_OK;
+  // the inlets' rho, u are still those of the last MacroProp, which the
+  // reference's k = 0 loop reads (lbm.cpp:599-611)
   HIPCHK(launch_backeq_solid(launch_ctx(c), c->cur, c->oth));
   c->backeq_solid_pending = 0;
   // the written solids' new moments enter the wall velocities from the next
@@ -279,6 +282,19 @@ int lbm_upload_cells(lbm_ctx *c, const int16_t *type, const double *rho, const d
       if ((e = upload_real(c, c->F[q], Fs[q], 0.0)) != LBM_OK) return e;
     }
   }
+  // a TYPE_IN cell at k = 0 under a TYPE_S cell at k = Nz+1 that carries a
+  // force: the back-face write into that solid changes every step
+  c->inlet_forced_over_solid = 0;
+  if (c->force_mode != 0) {
+    const uint32_t plane = c->g.plane, top = (c->g.nz - 1) * plane;
+    for (uint32_t d = 0; d < plane && !c->inlet_forced_over_solid; d++) {
+      if (c->h_type[d] != LBM_TYPE_IN || c->h_type[top + d] != LBM_TYPE_S) continue;
+      const uint32_t i = d % c->g.px, j = d / c->g.px;
+      const size_t r = ((size_t)i * c->g.ny + j) * c->g.nz;  // (i, j, 0)
+      for (int q = 0; q < 3; q++)
+        if (c->force_mode == 2 ? c->Fu[q] != 0.0 : Fs[q] && Fs[q][r] != 0.0) c->inlet_forced_over_solid = 1;
+    }
+  }
   c->uploaded = 1;
   c->state = ST_NONE;
   return LBM_OK;
@@ -344,6 +360,7 @@ int lbm_init(lbm_ctx *c) {
     for (uint32_t d = 0; d < plane; d++)
       if (c->h_type[d] == LBM_TYPE_IN && c->h_type[top + d] == LBM_TYPE_S) c->backeq_solid_pending = 1;
   }
+  c->backeq_dynamic = c->backeq_solid_pending && c->inlet_forced_over_solid;
   c->state = ST_F;
   c->fstar_ready = 0;
   return LBM_OK;
@@ -416,9 +433,9 @@ int lbm_stream(lbm_ctx *c) {
   std::swap(c->cur, c->oth);
   c->fstar_ready = 1;
   c->state = ST_F;
+  if (int e = after_stream(c, true)) return e;
   // the face loops; the other-type cells' MacroProp waits for lbm_macro
-  if (int e = faces_step(c, false, 0)) return e;
-  return after_stream(c);
+  return faces_step(c, false, 0);
 }
 
 int lbm_macro(lbm_ctx *c) {
@@ -456,6 +473,9 @@ static int step_launch(lbm_ctx *c, const LaunchCtx &L, int mode) {
   }
   std::swap(c->cur, c->oth);
   c->fstar_ready = 1;
+  // the back-face write into solids, before the face loops' MacroProp
+  // replaces the inlets' rho, u
+  if (int e = after_stream(c, true)) return e;
   // MacroProp of the other-type cells: the inlets every step, all of them
   // on steps that emit rho, u
   return faces_step(c, false, mode == STREAM_ONLY ? 0 : mode == STREAM_COLLIDE ? 1 : 2);
@@ -477,12 +497,11 @@ int lbm_step(lbm_ctx *c, int32_t nsteps, int32_t emit_macro) {
   LaunchCtx L = launch_ctx(c);
   int e;
   if (c->state == ST_F && (e = collide_launch(c, L)) != LBM_OK) return e;
-  while (c->walls_from_setup || c->backeq_solid_pending) {
+  while (c->walls_from_setup || c->backeq_solid_pending || c->backeq_dynamic) {
     // steps whose MacroProp changes the wall tables run unfused
     if ((e = step_launch(c, L, STREAM_MACRO)) != LBM_OK) return e;
     c->state = ST_F;
-    e = after_stream(c);
-    if (e == LBM_OK) e = walls_from_moments(c);
+    e = walls_from_moments(c);
     if (e != LBM_OK) return e;
     if (--nsteps == 0) return LBM_OK;
     L = launch_ctx(c);
@@ -499,6 +518,15 @@ int lbm_time_steps(lbm_ctx *c, int32_t nsteps, double *ms) {
   if (!c || nsteps < 1 || !ms) return LBM_ERR_ARG;
   if (c->state == ST_NONE || ibm_active(c)) return LBM_ERR_STATE;
   HIPCHK(hipSetDevice(c->dev));
+  if (c->backeq_dynamic) {
+    // every step runs unfused (its wall tables change): time lbm_step
+    EventPair ev;
+    HIPCHK(ev.create());
+    HIPCHK(hipEventRecord(ev.t0, c->stream));
+    if (int e = lbm_step(c, nsteps, 1)) return e;
+    HIPCHK(hipEventRecord(ev.t1, c->stream));
+    return ev.elapsed(ms);
+  }
   while (c->walls_from_setup || c->backeq_solid_pending) {
     int e = lbm_step(c, 1, 1);
     if (e != LBM_OK) return e;
@@ -506,22 +534,15 @@ int lbm_time_steps(lbm_ctx *c, int32_t nsteps, double *ms) {
   const LaunchCtx L = launch_ctx(c);
   int er;
   if (c->state == ST_F && (er = collide_launch(c, L)) != LBM_OK) return er;
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
-  HIPCHK(hipEventRecord(e0, c->stream));
+  EventPair ev;
+  HIPCHK(ev.create());
+  HIPCHK(hipEventRecord(ev.t0, c->stream));
   for (int s = 0; s < nsteps; s++)
     if ((er = step_launch(c, L, STREAM_COLLIDE)) != LBM_OK) return er;
-  HIPCHK(hipEventRecord(e1, c->stream));
+  HIPCHK(hipEventRecord(ev.t1, c->stream));
   if ((er = step_launch(c, L, STREAM_MACRO)) != LBM_OK) return er;
   c->state = ST_F;
-  HIPCHK(hipEventSynchronize(e1));
-  float t = 0.f;
-  HIPCHK(hipEventElapsedTime(&t, e0, e1));
-  *ms = (double)t;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  return LBM_OK;
+  return ev.elapsed(ms);
 }
 
 int lbm_sync(lbm_ctx *c) {
@@ -651,15 +672,16 @@ int lbm_reduce_fp64(lbm_ctx *c, int32_t what, double *out) {
 
 int lbm_set_launch(lbm_ctx *c, int32_t block, int32_t cells_per_thread) {
   if (!c || block < 64 || block > 256 || block % 64 != 0) return LBM_ERR_ARG;
-  if (cells_per_thread != 1) return LBM_ERR_UNSUPPORTED;
+  if (cells_per_thread != 1 && cells_per_thread != 2 && cells_per_thread != 4) return LBM_ERR_ARG;
   c->block = block;
+  c->cpt = cells_per_thread;
   return LBM_OK;
 }
 
 int lbm_get_launch(const lbm_ctx *c, int32_t *block, int32_t *cells_per_thread) {
   if (!c) return LBM_ERR_ARG;
   if (block) *block = c->block;
-  if (cells_per_thread) *cells_per_thread = 1;
+  if (cells_per_thread) *cells_per_thread = c->cpt;
   return LBM_OK;
 }
 

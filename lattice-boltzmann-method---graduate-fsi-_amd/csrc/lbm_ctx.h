@@ -49,8 +49,15 @@ struct lbm_ctx {
   // TYPE_S cells at k = Nz+1 under a TYPE_IN cell at k = 0 take feq2 of that
   // cell at the first Streaming (the back-face inflow loop, lbm.cpp:599-611)
   int backeq_solid_pending = 0;
+  // ... and, when such an inlet carries a force, at every Streaming: its u
+  // gains F/(2 rho) at every MacroProp (lbm.cpp:672-675 over its feq2
+  // populations), so the solid's moments and the wall velocities next to it
+  // change every step
+  int backeq_dynamic = 0;
+  int inlet_forced_over_solid = 0;  // set by lbm_upload_cells
   int64_t nfluid = 0;
   int block = 256;
+  int cpt = 1;  // cells per thread of the stream kernel
   std::vector<int16_t> h_type;  // device order
   lbm_ibm *ibm = nullptr;       // set by lbm_ibm_setup
   lbm::BoxArgs box{};           // the IBM box as force mode 3 reads it
@@ -86,6 +93,27 @@ struct lbm_ctx {
     hipError_t e_ = (x);                       \
     if (e_ != hipSuccess) return LBM_ERR_HIP;  \
   } while (0)
+
+// Two timing events, destroyed on every return path.
+struct EventPair {
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  hipError_t create() {
+    hipError_t e = hipEventCreate(&t0);
+    return e == hipSuccess ? hipEventCreate(&t1) : e;
+  }
+  // waits for t1; *ms = t1 - t0
+  int elapsed(double *ms) const {
+    float t = 0.f;
+    if (hipEventSynchronize(t1) != hipSuccess || hipEventElapsedTime(&t, t0, t1) != hipSuccess)
+      return LBM_ERR_HIP;
+    *ms = (double)t;
+    return LBM_OK;
+  }
+  ~EventPair() {
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+  }
+};
 
 // thermal branch (lbm_thermal.hip)
 void thermal_free(lbm_ctx *c);
@@ -129,8 +157,11 @@ hipError_t faces_f_out(lbm_ctx *c, double *out);
 
 // launch parameters of the context's current state (lbm_capi.hip)
 lbm::LaunchCtx launch_ctx(const lbm_ctx *c);
-// work the reference does right after a Streaming call (lbm_capi.hip)
-int after_stream(lbm_ctx *c);
+// the back-face inflow write into solids that the reference's Streaming
+// makes (lbm_capi.hip): the first time, and with forced_inlets at every
+// Streaming of a context whose forced inlets change it (IBM runs take the
+// inlets' moments without force, so theirs stays as first written)
+int after_stream(lbm_ctx *c, bool forced_inlets = false);
 // the IBM state (lbm_ibm.hip): release; set up at all; IBLBM done and
 // waiting for its collision; mark that collision done
 void ibm_free(lbm_ctx *c);

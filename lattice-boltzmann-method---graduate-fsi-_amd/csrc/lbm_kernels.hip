@@ -30,15 +30,30 @@
 
 namespace lbm {
 
-// One cell per thread.
+// CPT cells per thread, one per grid-stride pass: pass r of the grid covers
+// the contiguous cells [r G, (r + 1) G), G = gridDim.x * blockDim.x, so every
+// pass keeps the one-contiguous-run-per-plane access of one cell per thread.
+// (A run-time pass count makes the compiler duplicate the whole cell body,
+// which costs the one-cell kernel instruction-cache room: the count is a
+// template argument, and CPT > 1 exists for the fused launch only.)
 // COLLIDE: relax and store f*; otherwise store f (post-streaming).
 // EMIT   : write rho, u (u = (sum f c + F/2)/rho, lbm.cpp:672-675).
-template <class S, int FORCE, bool COLLIDE, bool EMIT>
+template <class S, int FORCE, bool COLLIDE, bool EMIT, int CPT>
 __global__ __launch_bounds__(256) void k_stream(const Planes<S> P, const uint8_t *__restrict__ mask,
                                                 KArgs<typename Store<S>::C> a) {
-  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= a.g.ncell) return;
-  stream_cell<S, FORCE, COLLIDE, EMIT>(P, mask, a, idx);
+  if constexpr (CPT == 1) {
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= a.g.ncell) return;
+    stream_cell<S, FORCE, COLLIDE, EMIT>(P, mask, a, idx);
+  } else {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t idx0 = blockIdx.x * blockDim.x + threadIdx.x;
+#pragma unroll
+    for (int r = 0; r < CPT; r++) {
+      const uint32_t idx = idx0 + (uint32_t)r * stride;
+      if (idx < a.g.ncell) stream_cell<S, FORCE, COLLIDE, EMIT>(P, mask, a, idx);
+    }
+  }
 }
 
 // ---- ensembles: one launch steps many independent lattices ----------------
@@ -137,9 +152,11 @@ __global__ __launch_bounds__(256) void k_macro(const S *__restrict__ buf,
   C Fx, Fy, Fz;
   load_force<C, (FORCE == 3 ? 0 : FORCE)>(a, idx, Fx, Fy, Fz);
   a.rho[idx] = rho;
-  a.ux[idx] = (jx + C(0.5) * Fx) / rho;
-  a.uy[idx] = (jy + C(0.5) * Fy) / rho;
-  a.uz[idx] = (jz + C(0.5) * Fz) / rho;
+  C ux, uy, uz;
+  velocity(jx + C(0.5) * Fx, jy + C(0.5) * Fy, jz + C(0.5) * Fz, rho, ux, uy, uz);
+  a.ux[idx] = ux;
+  a.uy[idx] = uy;
+  a.uz[idx] = uz;
   if (rho < C(0) || rho > C(100)) flag(a.status, a.first_bad, ST_RHO_RANGE, idx);
 }
 
@@ -263,8 +280,14 @@ __global__ __launch_bounds__(256) void k_tables(const int16_t *__restrict__ type
   }
 }
 
-// TYPE_S cells hit by the back-face inflow write (M_BACKEQ): from the first
-// Streaming on they hold feq2 of their TYPE_IN partner (lbm.cpp:599-611).
+// TYPE_S cells hit by the back-face inflow write (M_BACKEQ): at every
+// Streaming they take feq2 of their TYPE_IN partner's rho, u as of the last
+// MacroProp (lbm.cpp:599-611).  With wall slots a solid's buffer entries are
+// bounce-back slots that the face loops may still read (an outlet copying a
+// fluid neighbour's pulled value), so only the solid's MacroProp values
+// (moments of the stored populations, lbm.cpp:668-675) are written; they
+// are what the wall tables and the output read.  Without wall slots the
+// populations themselves are written to both buffers.
 template <class S, int FORCE>
 __global__ __launch_bounds__(256) void k_backeq_solid(S *__restrict__ A, S *__restrict__ B,
                                                       const uint8_t *__restrict__ mask,
@@ -282,25 +305,23 @@ __global__ __launch_bounds__(256) void k_backeq_solid(S *__restrict__ A, S *__re
   for_dirs([&](auto M) {
     constexpr int m = decltype(M)::value;
     const C cu = C(CX[m]) * vx + C(CY[m]) * vy + C(CZ[m]) * vz;
-    f[m] = feq2_raw<C, Store<S>::shifted>(r, cu, uu, C(W[m]), C(Store<S>::K));
-    const S v = Store<S>::enc(f[m]);
-    A[(uint32_t)m * ps + idx] = v;
-    B[(uint32_t)m * ps + idx] = v;
+    const S v = Store<S>::enc(feq2_raw<C, Store<S>::shifted>(r, cu, uu, C(W[m]), C(Store<S>::K)));
+    f[m] = Store<S>::ld(&v);  // the stored value
+    if (!a.wall_slots) {
+      A[(uint32_t)m * ps + idx] = v;
+      B[(uint32_t)m * ps + idx] = v;
+    }
   });
   if (a.wall_slots) {
-    // the solid's slots will carry bounce-back values: its MacroProp values
-    // (moments of the stored populations, lbm.cpp:668-675) are kept now
-    for_dirs([&](auto M) {
-      constexpr int m = decltype(M)::value;
-      f[m] = Store<S>::ld(&A[(uint32_t)m * ps + idx]);
-    });
     C rho, drho, jx, jy, jz, Fx, Fy, Fz;
     moments<S>(f, rho, drho, jx, jy, jz);
     load_force<C, FORCE>(a, idx, Fx, Fy, Fz);
     a.rho[idx] = rho;
-    a.ux[idx] = (jx + C(0.5) * Fx) / rho;
-    a.uy[idx] = (jy + C(0.5) * Fy) / rho;
-    a.uz[idx] = (jz + C(0.5) * Fz) / rho;
+    C ux, uy, uz;
+    velocity(jx + C(0.5) * Fx, jy + C(0.5) * Fy, jz + C(0.5) * Fz, rho, ux, uy, uz);
+    a.ux[idx] = ux;
+    a.uy[idx] = uy;
+    a.uz[idx] = uz;
   }
 }
 
@@ -476,8 +497,19 @@ template <class S, int FORCE, bool COLLIDE, bool EMIT>
 static void stream_launch(const LaunchCtx &L, const S *in, S *out) {
   auto a = make_args<S>(L);
   const Planes<S> P = make_planes(L, in, out);
-  hipLaunchKernelGGL((k_stream<S, FORCE, COLLIDE, EMIT>), grid_for(L.g.ncell, L.block),
-                     dim3(L.block), 0, L.stream, P, L.mask, a);
+  const int cpt = (COLLIDE && !EMIT && (L.cpt == 2 || L.cpt == 4)) ? L.cpt : 1;
+  const dim3 grid = grid_for(L.g.ncell, L.block * cpt);
+  if constexpr (COLLIDE && !EMIT) {
+    if (cpt == 2) {
+      hipLaunchKernelGGL((k_stream<S, FORCE, COLLIDE, EMIT, 2>), grid, dim3(L.block), 0, L.stream, P, L.mask, a);
+      return;
+    }
+    if (cpt == 4) {
+      hipLaunchKernelGGL((k_stream<S, FORCE, COLLIDE, EMIT, 4>), grid, dim3(L.block), 0, L.stream, P, L.mask, a);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_stream<S, FORCE, COLLIDE, EMIT, 1>), grid, dim3(L.block), 0, L.stream, P, L.mask, a);
 }
 
 template <class S, int FORCE>

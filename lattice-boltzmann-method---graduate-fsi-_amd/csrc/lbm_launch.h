@@ -60,6 +60,7 @@ struct LaunchCtx {
   int storage;     // LBM_STORE_*
   int force_mode;  // 0 none, 1 per-cell field, 2 uniform
   int block;       // threads per block
+  int cpt;         // cells per thread of the fused k_stream launch (grid-stride passes)
   double omega;
   double Fu[3];
   const void *Fx, *Fy, *Fz;

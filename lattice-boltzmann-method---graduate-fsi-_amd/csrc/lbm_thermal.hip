@@ -210,9 +210,7 @@ __global__ __launch_bounds__(256) void k_thermal(const Planes<S> Pf, const Plane
       load_force<C, FORCE>(a, idx, Fx, Fy, Fz);
     }
     // u = (sum f c + F/2)/rho with the force before this MacroProp's update
-    ux = (jx + C(0.5) * Fx) / rho;
-    uy = (jy + C(0.5) * Fy) / rho;
-    uz = (jz + C(0.5) * Fz) / rho;
+    velocity(jx + C(0.5) * Fx, jy + C(0.5) * Fy, jz + C(0.5) * Fz, rho, ux, uy, uz);
     C Ts = C(0), qx = C(0), qy = C(0), qz = C(0);
     for_dirs([&](auto M) {
       constexpr int m = decltype(M)::value;

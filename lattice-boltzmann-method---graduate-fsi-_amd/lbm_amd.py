@@ -173,7 +173,7 @@ _SIGNATURES = {
     "lbm_batch_get_launch": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "lbm_batch_nfluid": (C.c_int64, [_P]),
     "lbm_batch_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _D]),
-    "lbm_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32)]),
+    "lbm_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "lbm_fsi_setup": (C.c_int, [_P, _P, C.POINTER(_IbmDesc)]),
     "lbm_fsi_run": (C.c_int, [_P, C.c_int32, _D]),
     "lbm_momentum_exchange": (C.c_int, [_P, C.POINTER(C.c_int32), _D]),
@@ -533,10 +533,12 @@ class Lattice:
                "lbm_ibm_markers")
         return dict(x=x, y=y, z=z, ub=ub, Fb=Fb)
 
-    def autotune(self, nsteps: int = 20) -> int:
-        b = C.c_int32(0)
-        _check(self._lib.lbm_autotune(self._h, int(nsteps), C.byref(b)), "lbm_autotune")
-        return b.value
+    def autotune(self, nsteps: int = 20):
+        """Dry-run tuning of the launch shape (the lattice's state is not
+        changed); returns (threads per block, cells per thread), now set."""
+        b, v = C.c_int32(0), C.c_int32(0)
+        _check(self._lib.lbm_autotune(self._h, int(nsteps), C.byref(b), C.byref(v)), "lbm_autotune")
+        return b.value, v.value
 
     def set_launch(self, block: int, cells_per_thread: int = 1):
         _check(self._lib.lbm_set_launch(self._h, int(block), int(cells_per_thread)), "lbm_set_launch")

@@ -65,14 +65,25 @@ def test_batch_then_single_lattice_calls():
 
 def test_batch_autotune_and_rejections():
     ens = _lattices("f32")
+    twins = _lattices("f32")
+    for lat in ens + twins:
+        lat.step(3)
+    f0 = [lat.populations() for lat in ens]
     batch = lbm_amd.Batch(ens)
     b, p, ms = batch.autotune(4)
     assert b in (64, 128, 256) and p in (0, 1, 2, 4, 8) and ms > 0
     assert batch.get_launch() == (b, p)
+    # dry runs: the lattices are as before, and step on as their twins do
+    for lat, f in zip(ens, f0):
+        assert np.array_equal(lat.populations(), f)
+    batch.step(5)
+    for lat, twin in zip(ens, twins):
+        twin.step(5)
+        assert np.array_equal(lat.populations(), twin.populations())
     batch.close()
     mixed = _lattices("f32")[:2] + _lattices("f64")[:1]
     with pytest.raises(lbm_amd.LBMError) as e:
         lbm_amd.Batch(mixed)
     assert e.value.code == lbm_amd.LBM_ERR_UNSUPPORTED
     lat = ens[0]
-    assert lat.autotune(4) in (64, 128, 256)
+    assert lat.autotune(4)[0] in (64, 128, 256)

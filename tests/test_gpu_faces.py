@@ -127,6 +127,52 @@ def test_faces_with_force_fp64(oracle_mod, mode):
     lat.close()
 
 
+def _inlet_over_solid_case(mode):
+    """k = 0 inlets under k = Nz+1 solids over most of the (i, j) plane (the
+    back-face inflow loop writes feq2 of each inlet into its solid at every
+    Streaming, lbm.cpp:599-611), a forced lattice: the inlets' u grows by
+    F/(2 rho) per MacroProp, so the solids' moments and the wall velocities
+    of the fluid cells below them change every step."""
+    case = cases.face_case(6, 5, 4, seed=9, shell=(cases.TYPE_OUT, cases.TYPE_IN, cases.TYPE_S, cases.TYPE_F))
+    case.type[1:-1, 1:-1, 0] = cases.TYPE_IN
+    case.type[1:-1, 1:-1, -1] = cases.TYPE_S
+    case.type[2, 3, -1] = cases.TYPE_F  # a fluid cell in the solid layer: the fluid write too
+    rng = np.random.default_rng(4)
+    if mode == "field":
+        F = 3e-5 * rng.standard_normal((3,) + case.shape)
+    else:
+        F = np.broadcast_to(np.array([3e-5, -2e-5, 1e-5])[:, None, None, None], (3,) + case.shape).copy()
+    case.Fx, case.Fy, case.Fz = F[0], F[1], F[2]
+    return case
+
+
+@pytest.mark.parametrize("mode", ["field", "uniform"])
+def test_faces_forced_inlets_over_solids_fp64(oracle_mod, mode):
+    """Fused steps and the reference's separate calls against the oracle with
+    forced k = 0 inlets over k = Nz+1 solids: every Streaming rewrites the
+    solids from the inlets' previous MacroProp (a regression case: the solids
+    were written once, from the inlets' values one MacroProp late)."""
+    case = _inlet_over_solid_case(mode)
+    steps = 20
+    ref = oracle_mod.run_case(case, steps)
+    lat = _gpu(case)
+    lat.step(steps)
+    assert lat.sync() == 0
+    _cmp_all(lat, ref, case)
+    lat.close()
+    ref = oracle_mod.run_case(case, 0)
+    lat = _gpu(case)
+    for _ in range(5):
+        assert ref.collision() == 0
+        ref.streaming()
+        lat.collision()
+        lat.streaming()
+        assert ref.macroprop() == 0
+        lat.macroprop()
+        _cmp_all(lat, ref, case)
+    lat.close()
+
+
 def test_faces_on_d3q7_context(oracle_mod):
     """The flow populations of a D3Q19 + D3Q7 context (26-plane buffers)."""
     case, th = cases.q7_random_case(7, 6, 5, seed=9)

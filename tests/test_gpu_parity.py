@@ -262,20 +262,112 @@ def test_tgv_256_conservation(storage):
     assert np.abs(p).max() / m0 < tol
 
 
-@pytest.mark.parametrize("storage", ["f32", "f16s", "f16c"])
-@pytest.mark.parametrize("block,vec", [(64, 1), (128, 1)])
-def test_launch_shapes_agree(storage, block, vec):
-    """Every launch shape the autotuner may pick gives the same fields as the
-    default (one cell per thread, 256 threads)."""
+@pytest.mark.parametrize("storage", ["f64", "f32", "f16s", "f16c"])
+@pytest.mark.parametrize("block,cpt", [(64, 1), (128, 1), (256, 2), (128, 4), (64, 4)])
+def test_launch_shapes_agree(storage, block, cpt):
+    """Every launch shape the autotuner may pick (threads per block, cells
+    per thread) gives bit for bit the fields of the default (256 threads, one
+    cell per thread): each cell's update does not depend on the shape."""
     case = cases.taylor_green(32, u0=0.04, Re=200.0)
-    case.type[5:9, 10:12, 3:20] = 1  # a solid block: wall chunks take the per-cell path
+    case.type[5:9, 10:12, 3:20] = 1  # a solid block: wall cells and their slots
     out = []
-    for shape in ((256, 1), (block, vec)):
+    for shape in ((256, 1), (block, cpt)):
         lat = _gpu(case, storage)
         lat.set_launch(*shape)
+        assert lat.get_launch() == shape
         lat.step(50)
         assert lat.status()[0] == 0
-        out.append(lat.macro())
+        out.append((lat.macro(), lat.populations()))
+        lat.close()
+    for a, b in zip(out[0][0], out[1][0]):
+        assert np.array_equal(a, b), f"{storage} {block}x{cpt}"
+    assert np.array_equal(out[0][1], out[1][1])
+
+
+def test_launch_shape_rejections():
+    lat = _gpu(cases.taylor_green(8), "f32")
+    for shape in ((32, 1), (320, 1), (96, 1), (256, 3), (256, 0), (256, 8)):
+        with pytest.raises(lbm_amd.LBMError) as e:
+            lat.set_launch(*shape)
+        assert e.value.code == lbm_amd.LBM_ERR_ARG
+    assert lat.get_launch() == (256, 1)
+    lat.close()
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16c"])
+def test_autotune_leaves_the_lattice_unchanged(storage):
+    """The tuner times dry runs into scratch buffers: populations, fields,
+    status and the step sequence of the lattice are as before, and the next
+    steps give what an untuned twin gives."""
+    case = cases.cavity(20, u_lid=0.05, Re=100.0)
+    tuned, twin = _gpu(case, storage), _gpu(case, storage)
+    for lat in (tuned, twin):
+        lat.step(7)
+    f0, m0 = tuned.populations(), tuned.macro()
+    block, cpt = tuned.autotune(6)
+    assert block in (64, 128, 256) and cpt in (1, 2, 4)
+    assert tuned.get_launch() == (block, cpt)
+    assert np.array_equal(tuned.populations(), f0)
+    for a, b in zip(tuned.macro(), m0):
+        assert np.array_equal(a, b)
+    assert tuned.status() == (0, -1)
+    for lat in (tuned, twin):
+        lat.step(9)
+    assert np.array_equal(tuned.populations(), twin.populations())
+    # a lattice in the middle of a step (after Collision) is left there too
+    tuned.collision()
+    twin.collision()
+    tuned.autotune(3)
+    tuned.streaming(); tuned.macroprop()
+    twin.streaming(); twin.macroprop()
+    for a, b in zip(tuned.macro(), twin.macro()):
+        assert np.array_equal(a, b)
+    tuned.close()
+    twin.close()
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("storage,tol", [("f32", 1e-5)])
+def test_taylor_green_low_viscosity_fp32_against_oracle(oracle_mod, storage, tol):
+    """FP32 storage and arithmetic against the FP64 oracle at the Taylor-Green
+    vortex's tau = 0.50306 (48^3, u0 = 0.04, Re = 300: the 256^3 Re = 1600
+    case's tau), 2000 steps: relative L2 of u within the 1e-5 FP32 bar.
+    At this tau the collision (omega = 1.988) feeds any bias of u into the
+    momentum each step; the velocity's reciprocal-based rounding drifted to
+    2e-5 here (8e-5 at 64^3 after 5000 steps)."""
+    case = cases.taylor_green(48, u0=0.04, Re=300.0)
+    assert case.tau == pytest.approx(0.5030557749, abs=1e-9)
+    lat = _gpu(case, storage)
+    lat.step(2000)
+    assert lat.status()[0] == 0
+    ref = oracle_mod.run_case(case, 2000)
+    rho, ux, uy, uz = lat.macro()
+    r_rho, r_ux, r_uy, r_uz = ref.macro()
     fl = case.type == 0
-    eu = rel_l2_u(out[1][1:], out[0][1:], fl)
-    assert eu <= 1e-6, f"{storage} {block}x{vec}: {eu:.3e}"
+    eu = rel_l2_u((ux, uy, uz), (r_ux, r_uy, r_uz), fl)
+    er = rel_l2(rho - 1, r_rho - 1, fl)
+    print(f"TGV 48^3 tau 0.50306 {storage} 2000 steps: rel L2 u {eu:.3e}, rho-1 {er:.3e}")
+    assert eu <= tol
+    assert er <= 1e-3
+
+
+@pytest.mark.timeout(300)
+def test_taylor_green_256_fp32_against_fp64_5000_steps():
+    """At the benchmark's own size: 256^3 Taylor-Green (u0 = 0.04, Re = 1600),
+    5000 steps, FP32 storage against FP64 storage (itself within 1e-11 of the
+    oracle in the tests above): relative L2 of u within the 1e-5 FP32 bar,
+    and no drift of the kinetic energy beyond 1e-6 relative."""
+    case = cases.taylor_green(256, u0=0.04, Re=1600.0)
+    res = {}
+    for st in ("f64", "f32"):
+        lat = _gpu(case, st)
+        lat.step(5000)
+        assert lat.status()[0] == 0
+        res[st] = (lat.macro(), lat.reduce(lbm_amd.REDUCE_KINETIC))
+        lat.close()
+    (m64, k64), (m32, k32) = res["f64"], res["f32"]
+    fl = case.type == 0
+    eu = rel_l2_u(m32[1:], m64[1:], fl)
+    print(f"TGV 256^3 5000 steps FP32 vs FP64: rel L2 u {eu:.3e}, KE ratio {k32 / k64:.9f}")
+    assert eu <= 1e-5
+    assert abs(k32 / k64 - 1) < 1e-6
