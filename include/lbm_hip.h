@@ -121,6 +121,16 @@ int lbm_download_force(lbm_ctx *ctx, double *Fx, double *Fy, double *Fz);
  * counterpart; the reference never sums mass). */
 int lbm_reduce_fp64(lbm_ctx *ctx, int32_t what, double *out);
 
+/* lbm_reduce_fp64 without allocation, for callers that cannot allocate
+ * device memory (code under graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  Same sums, same
+ * result bit for bit; the partial sums go to one fixed device array of the
+ * library (1.5 MB per device, held until the process ends) instead of the
+ * context's buffers.  Lattices of up to 268,435,456 padded cells
+ * (LBM_ERR_UNSUPPORTED beyond).  Calls that overlap on one device race on
+ * that array. */
+int lbm_reduce_fp64_noalloc(lbm_ctx *ctx, int32_t what, double *out);
+
 /* The streaming tables the device kernels use, for bit-exact comparison with
  * lbm.cpp:186-295: per cell and direction m, src = linear index of the
  * cell x - c_m (wrapped modulo N+2 per axis, lbm.cpp:197-199) and

@@ -47,6 +47,7 @@ SOURCES = [
     os.path.join(HERE, "csrc", "lbm_mea.hip"),
     os.path.join(HERE, "csrc", "lbm_faces.hip"),
     os.path.join(HERE, "csrc", "lbm_stats.hip"),
+    os.path.join(HERE, "csrc", "lbm_noalloc.hip"),
 ]
 HOST_SOURCES = [
     os.path.join(HERE, "host", "LBM.cpp"),

@@ -174,6 +174,7 @@ _SIGNATURES = {
     "lbm_batch_nfluid": (C.c_int64, [_P]),
     "lbm_batch_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _D]),
     "lbm_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "lbm_reduce_fp64_noalloc": (C.c_int, [_P, C.c_int32, _D]),
     "lbm_fsi_setup": (C.c_int, [_P, _P, C.POINTER(_IbmDesc)]),
     "lbm_fsi_run": (C.c_int, [_P, C.c_int32, _D]),
     "lbm_momentum_exchange": (C.c_int, [_P, C.POINTER(C.c_int32), _D]),
@@ -401,6 +402,12 @@ class Lattice:
     def reduce(self, what: int):
         out = np.zeros(3, np.float64)
         _check(self._lib.lbm_reduce_fp64(self._h, int(what), _dp(out)), "lbm_reduce_fp64")
+        return out[:3] if what == REDUCE_MOMENTUM else float(out[0])
+
+    def reduce_noalloc(self, what: int):
+        """reduce() through lbm_reduce_fp64_noalloc (no device allocation)."""
+        out = np.zeros(3, np.float64)
+        _check(self._lib.lbm_reduce_fp64_noalloc(self._h, int(what), _dp(out)), "lbm_reduce_fp64_noalloc")
         return out[:3] if what == REDUCE_MOMENTUM else float(out[0])
 
     # -- thermal branch (ENERGY, lbm.cpp:121-178, 263-289, 636-653, 777-806) ----
