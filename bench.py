@@ -508,6 +508,7 @@ def main():
     ap.add_argument("--ibm-cmp-steps", type=int, default=2000)
     ap.add_argument("--no-ibm", action="store_true")
     ap.add_argument("--no-cavity", action="store_true")
+    ap.add_argument("--cavity-only", action="store_true", help="run only the 256^3 lid-driven cavity leg")
     ap.add_argument("--no-q7", action="store_true")
     ap.add_argument("--ibm-only", action="store_true", help="run only the IBM channel leg")
     ap.add_argument("--fsi-steps", type=int, default=10000)
@@ -534,6 +535,9 @@ def main():
 
     if a.ibm_only:
         print(json.dumps(ibm_leg(a.ibm_steps, a.ibm_cmp_steps)), flush=True)
+        return 0
+    if a.cavity_only:
+        print(json.dumps(cavity_leg(a.warmup, a.steps)), flush=True)
         return 0
     if a.batch_only:
         print(json.dumps(batch_leg(a.batch_steps)), flush=True)

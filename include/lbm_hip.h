@@ -203,7 +203,11 @@ typedef struct lbm_ibm_desc {
                      * branch (lbm.cpp:1171-1258: one pass, F = +spread,    *
                      * density at the marker's local nearest point, drag   *
                      * and lift summed over the box cells); explicit needs *
-                     * an unforced set-up and a box that does not follow   */
+                     * an unforced set-up, a box that does not follow, and *
+                     * every marker's density point inside the box         *
+                     * (LBM_ERR_UNSUPPORTED otherwise: a marker on a       *
+                     * lattice point with no neighbour ahead along its     *
+                     * normal reads cell (0,0,1), lbm.cpp:1216-1225)       */
 } lbm_ibm_desc;
 
 /* Markers of the body (Generate_Cylinder_Marker, lbm.cpp:1857-1877), their

@@ -419,6 +419,156 @@ __global__ __launch_bounds__(256) void k_ibm_interp(int nm, const int32_t *__res
   }
 }
 
+// ---- the implicit passes in marker space (rigid bodies) --------------------
+// The m_max passes of lbm.cpp:1311-1387 are linear in the markers' forces:
+// the cells' rho does not change during an IBM step, so one pass moves the
+// interpolated velocity of marker n by
+//   sum_x phi_n(x) Fm(x)/(2 rho(x)) = sum_k H_nk (u_des,k - u_b,k),
+//   H_nk = rho_k ds_k sum_x phi_n(x) phi_k(x)/rho(x),
+// with x over the live box cells that both stencils reach, and rho_k the
+// density point of marker k (F_b,k = 2 rho_k (u_des,k - u_b,k)/dt).  The
+// passes therefore run on the markers alone (H has a few dozen entries per
+// row: only markers closer than 4 cells on every axis share stencil cells),
+// one launch per pass, and the cells receive the sum of all passes' forces
+// A_k = sum_m F_b,k^m ds_k in one spreading at the end: F = -sum_k phi_k A_k,
+// u += sum_k phi_k A_k/(2 rho), which is the reference's field after its
+// m_max spreadings.  Drag and lift still sum -F_b ds over every pass (Q12).
+
+// Rows of H are stored ELL-style: R entries per marker (R a multiple of 64),
+// the row's markers first (sorted), then padding entries (the marker
+// itself with H = 0).
+
+// The first interpolation (u_b of the unforced field, summed as k_ibm_interp
+// sums), each marker's rho_n ds_n (rho_n its density point's value), and
+// row n of G (H_nk = G_nk rho_k ds_k): one wavefront per marker.  The row's
+// markers and their stencils are loaded with the cell values; phi_n/rho of
+// the 64 stencil points goes to LDS; then each lane takes row entries and
+// sums over all 64 points with k's weights shifted to n's stencil (zero off
+// the overlap), so every lane reads the same LDS word at a time.
+__global__ __launch_bounds__(256) void k_ibm_interp_h(int nm, int R, const int32_t *__restrict__ pidx, Stencils st,
+                                                      BoxArrays o, const int32_t *__restrict__ near,
+                                                      const double *__restrict__ ds,
+                                                      const int32_t *__restrict__ nlen,
+                                                      const int32_t *__restrict__ nbr, double *__restrict__ G,
+                                                      double *__restrict__ rn, double *__restrict__ cds,
+                                                      double *__restrict__ ub) {
+  __shared__ double term[4][3][64];
+  __shared__ double qn[4][64];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int n = blockIdx.x * 4 + wv;
+  const bool valid = n < nm;
+  const int b = valid ? pidx[64 * n + lane] : -1;
+  const int len = valid ? nlen[n] : 0;
+  // up to two row entries per lane (rows of at most 128 entries take this
+  // path; longer ones loop below)
+  const int k0 = lane < len ? nbr[(size_t)n * R + lane] : -1;
+  const int k1 = lane + 64 < len ? nbr[(size_t)n * R + lane + 64] : -1;
+  double t0 = 0, t1 = 0, t2 = 0, q = 0;
+  if (b >= 0) {
+    const double kx = st.w[12 * n + (lane >> 4)], ky = st.w[12 * n + 4 + ((lane >> 2) & 3)],
+                 kz = st.w[12 * n + 8 + (lane & 3)];
+    t0 = o.ux[b] * kx * ky * kz;
+    t1 = o.uy[b] * kx * ky * kz;
+    t2 = o.uz[b] * kx * ky * kz;
+    q = kx * ky * kz / o.rho[b];
+  }
+  term[wv][0][lane] = t0;
+  term[wv][1][lane] = t1;
+  term[wv][2][lane] = t2;
+  qn[wv][lane] = q;
+  __syncthreads();
+  if (!valid) return;
+  if (lane < 3) {
+    double acc = 0.0;
+    for (int s = 0; s < 64; s++) acc += term[wv][lane][s];
+    ub[3 * n + lane] = acc;
+  }
+  if (lane == 3) {
+    const int nn = near[n];
+    const double r = nn >= 0 ? o.rho[nn] : nan("");
+    rn[n] = r;
+    cds[n] = r * ds[n];
+  }
+  const int bx = st.base[3 * n + 0], by = st.base[3 * n + 1], bz = st.base[3 * n + 2];
+  for (int r = lane; r < len; r += 64) {
+    const int k = r == lane ? k0 : r == lane + 64 ? k1 : nbr[(size_t)n * R + r];
+    const int dx = st.base[3 * k + 0] - bx, dy = st.base[3 * k + 1] - by, dz = st.base[3 * k + 2] - bz;
+    // k's weights on n's stencil points (0 where k's stencil does not reach)
+    double px[4], py[4], pz[4];
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      const int ax = a - dx, ay = a - dy, az = a - dz;
+      px[a] = (ax >= 0 && ax < 4) ? st.w[12 * k + ax] : 0.0;
+      py[a] = (ay >= 0 && ay < 4) ? st.w[12 * k + 4 + ay] : 0.0;
+      pz[a] = (az >= 0 && az < 4) ? st.w[12 * k + 8 + az] : 0.0;
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const double pac = px[a] * py[c];
+#pragma unroll
+        for (int z = 0; z < 4; z++) acc += qn[wv][(a * 4 + c) * 4 + z] * (pac * pz[z]);
+      }
+    G[(size_t)n * R + r] = acc;
+  }
+}
+
+// Pass m of the implicit scheme on the markers: F_b = 2 rho_n (u_des - u_b)
+// from the previous pass's u_b (lbm.cpp:1316-1320), its drag and lift
+// (lbm.cpp:1322-1323) and its share of A; then u_b after the pass's
+// spreading and re-interpolation, u_b += sum_k H_nk (u_des - u_b,k).  One
+// wavefront per marker: the lanes take the row's entries (two rounds of
+// loads: the row, then the entries' u_b and rho_k ds_k) and a fixed
+// butterfly sums them.
+__global__ __launch_bounds__(256) void k_ibm_pass(int nm, int R, const double *__restrict__ rn,
+                                                  const double *__restrict__ cds,
+                                                  const double *__restrict__ ds, const int32_t *__restrict__ nbr,
+                                                  const double *__restrict__ G, const double *__restrict__ ub_in,
+                                                  double *__restrict__ ub_out, double u0, double u1, double u2,
+                                                  double *__restrict__ A, double *__restrict__ dsum,
+                                                  double *__restrict__ Fb, int first, int last) {
+  const int lane = threadIdx.x & 63;
+  const int n = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (n >= nm) return;  // wave-uniform
+  // the marker's own values, loaded with the row
+  const int l3 = lane < 3 ? lane : 0;
+  const double ubn = ub_in[3 * n + l3], dsn = ds[n], rho = rn[n];
+  double Ao = 0, Do = 0;
+  if (!first && lane < 3) {
+    Ao = A[3 * n + lane];
+    if (lane < 2) Do = dsum[2 * n + lane];
+  }
+  double s0 = 0, s1 = 0, s2 = 0;
+  for (int r = lane; r < R; r += 64) {
+    const size_t e = (size_t)n * R + r;
+    const int k = nbr[e];
+    const double h = G[e] * cds[k];
+    s0 += h * (u0 - ub_in[3 * k + 0]);
+    s1 += h * (u1 - ub_in[3 * k + 1]);
+    s2 += h * (u2 - ub_in[3 * k + 2]);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    s0 += __shfl_xor(s0, off);
+    s1 += __shfl_xor(s1, off);
+    s2 += __shfl_xor(s2, off);
+  }
+  if (lane >= 3) return;
+  const double ud = lane == 0 ? u0 : lane == 1 ? u1 : u2;
+  const double sum = lane == 0 ? s0 : lane == 1 ? s1 : s2;
+  const double dt = 1.0, dx = 1.0, dy = 1.0;
+  const double F = 2 * rho * (ud - ubn) / dt;
+  const double a = F * dsn;
+  A[3 * n + lane] = first ? a : Ao + a;
+  if (lane < 2) {
+    const double dd = -F * (lane == 0 ? dx : dy) * dsn;
+    dsum[2 * n + lane] = first ? dd : Do + dd;
+  }
+  if (last) Fb[3 * n + lane] = F;
+  ub_out[3 * n + lane] = ubn + sum;
+}
+
 // Spreading and velocity correction, lbm.cpp:1332-1360.  One workgroup per
 // chunk of band cells (cells that some marker stencil reaches): the terms
 // F_b ds phi phi phi of the chunk's (marker, point) pairs are computed in
@@ -869,6 +1019,13 @@ struct lbm_ibm {
   int near_cell = 0;
   double *ud = nullptr, *fsum = nullptr;
   int scheme = 0;              // 0 implicit, 1 explicit (one pass, lbm.cpp:1171-1258)
+  // marker-space passes (rigid implicit): markers closer than 4 cells on
+  // every axis (CSR rows, the marker itself included), H per entry, the
+  // second u_b buffer of the passes
+  int32_t *nlen = nullptr, *nbr = nullptr;  // row lengths; R entries per row
+  double *H = nullptr, *ub2 = nullptr;  // H: G_nk (times rho_k ds_k in the passes)
+  double *rn = nullptr, *cds = nullptr;  // density point value rho_n per marker, rho_n ds_n
+  int R = 0;
   double *cell_pairs = nullptr;  // explicit: per-block (drag, lift) of the box cells
   size_t pairs_cap = 0;
 };
@@ -960,26 +1117,27 @@ static KArgs<typename Store<S>::C> kargs_of(const LaunchCtx &L) {
 }
 
 template <class S>
-static hipError_t box_moments_s(lbm_ctx *c, bool pull) {
+static hipError_t box_moments_s(lbm_ctx *c, bool pull, hipStream_t st) {
   lbm_ibm *m = c->ibm;
   const LaunchCtx L = launch_ctx(c);
   const Planes<S> P = planes_of<S>(c, c->cur);
   const auto a = kargs_of<S>(L);
   if (pull)
-    hipLaunchKernelGGL((k_ibm_box_moments<S, true>), blocks(m->B.ncell()), dim3(256), 0, c->stream, P, c->mask, a, m->B,
+    hipLaunchKernelGGL((k_ibm_box_moments<S, true>), blocks(m->B.ncell()), dim3(256), 0, st, P, c->mask, a, m->B,
                        box_arrays(m), m->inside, m->part);
   else
-    hipLaunchKernelGGL((k_ibm_box_moments<S, false>), blocks(m->B.ncell()), dim3(256), 0, c->stream, P, c->mask, a, m->B,
+    hipLaunchKernelGGL((k_ibm_box_moments<S, false>), blocks(m->B.ncell()), dim3(256), 0, st, P, c->mask, a, m->B,
                        box_arrays(m), m->inside, m->part);
   return hipGetLastError();
 }
 
-static hipError_t box_moments(lbm_ctx *c, bool pull) {
+static hipError_t box_moments(lbm_ctx *c, bool pull, hipStream_t st = nullptr) {
+  if (!st) st = c->stream;
   switch (c->storage) {
-    case LBM_STORE_F64: return box_moments_s<double>(c, pull);
-    case LBM_STORE_F32: return box_moments_s<float>(c, pull);
-    case LBM_STORE_F16S: return box_moments_s<__half>(c, pull);
-    default: return box_moments_s<f16c_t>(c, pull);
+    case LBM_STORE_F64: return box_moments_s<double>(c, pull, st);
+    case LBM_STORE_F32: return box_moments_s<float>(c, pull, st);
+    case LBM_STORE_F16S: return box_moments_s<__half>(c, pull, st);
+    default: return box_moments_s<f16c_t>(c, pull, st);
   }
 }
 
@@ -1072,7 +1230,18 @@ static int ibm_geometry(lbm_ctx *c) {
   m->h_inside.resize(nb);
   HIPCHK(hipMemcpyAsync(m->h_cnt.data(), m->cnt, 4 * (size_t)nb, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipMemcpyAsync(m->h_inside.data(), m->inside, (size_t)nb, hipMemcpyDeviceToHost, c->stream));
+  std::vector<int32_t> h_near;
+  if (m->scheme == 1 && !m->near_cell) {
+    h_near.resize(m->nm);
+    HIPCHK(hipMemcpyAsync(h_near.data(), m->near, 4 * (size_t)m->nm, hipMemcpyDeviceToHost, c->stream));
+  }
   HIPCHK(hipStreamSynchronize(c->stream));
+  // the explicit branch reads rho at the marker's local nearest point, or at
+  // (0, 0, 1) when no neighbour lies ahead along its normal (lbm.cpp:1216-
+  // 1225, e.g. a marker on a lattice point): a cell outside the box, whose
+  // IBM-step density the device does not compute
+  for (int32_t v : h_near)
+    if (v < 0) return LBM_ERR_UNSUPPORTED;
   size_t nin = 0;
   for (uint8_t v : m->h_inside) nin += v;
   m->ninside = (double)nin;
@@ -1131,45 +1300,64 @@ static int ibm_geometry(lbm_ctx *c) {
 // One IBLBM call's device work after the box moments: interpolation, m_max
 // force/spread/interpolate passes, and the drag/lift/added-mass sums into
 // out[0..2].
-static int ibm_iterate(lbm_ctx *c, const double vcm[3], const double vb[3], double *out) {
+static int ibm_iterate(lbm_ctx *c, const double vcm[3], const double vb[3], double *out, hipStream_t st = nullptr) {
   lbm_ibm *m = c->ibm;
+  if (!st) st = c->stream;
   const Box &B = m->B;
   const BoxArrays o = box_arrays(m);
   const bool expl = m->scheme == 1;
   ForceArgs fa{m->near, m->ds, m->Fb, m->Fds, m->dsum, vcm[0], vcm[1], vcm[2], expl || m->m_max > 0, 1, m->ud, m->fsum};
   const dim3 gi((unsigned)((m->nm + 3) / 4));
-  hipLaunchKernelGGL(k_ibm_interp, gi, dim3(256), 0, c->stream, m->nm, m->pidx, m->st, o, m->ub, fa);
-  fa.first = 0;
   const int npart = (B.ncell() + 255) / 256;
   if (expl) {
+    hipLaunchKernelGGL(k_ibm_interp, gi, dim3(256), 0, st, m->nm, m->pidx, m->st, o, m->ub, fa);
     // one pass: the unforced interpolation's force, spread with + (the
     // markers keep the unforced u_b), drag and lift summed over the cells
     if (m->nchunk > 0)
-      hipLaunchKernelGGL(k_ibm_spread, dim3(m->nchunk), dim3(256), 0, c->stream, m->chunk, m->band, m->bstart, m->ent,
+      hipLaunchKernelGGL(k_ibm_spread, dim3(m->nchunk), dim3(256), 0, st, m->chunk, m->band, m->bstart, m->ent,
                          m->ew, m->Fds, o, 1.0);
     if (out) {
       if ((size_t)npart > m->pairs_cap) {
-        HIPCHK(hipStreamSynchronize(c->stream));
+        HIPCHK(hipStreamSynchronize(st));
         dfree(m->cell_pairs);
         HIPCHK(hipMalloc((void **)&m->cell_pairs, 16 * (size_t)npart));
         m->pairs_cap = npart;
       }
-      hipLaunchKernelGGL(k_ibm_cell_force, dim3((unsigned)npart), dim3(256), 0, c->stream, B, o, m->cell_pairs);
-      hipLaunchKernelGGL(k_ibm_finish, dim3(1), dim3(FIN_THREADS), 0, c->stream, npart, m->cell_pairs, npart, m->part,
+      hipLaunchKernelGGL(k_ibm_cell_force, dim3((unsigned)npart), dim3(256), 0, st, B, o, m->cell_pairs);
+      hipLaunchKernelGGL(k_ibm_finish, dim3(1), dim3(FIN_THREADS), 0, st, npart, m->cell_pairs, npart, m->part,
                          m->ninside, m->vsolid, vcm[0] - vb[0], vcm[1] - vb[1], out);
     }
     HIPCHK(hipGetLastError());
     return LBM_OK;
   }
-  for (int it = 1; it <= m->m_max; it++) {
-    if (m->nchunk > 0)
-      hipLaunchKernelGGL(k_ibm_spread, dim3(m->nchunk), dim3(256), 0, c->stream, m->chunk, m->band, m->bstart, m->ent,
+  if (m->nbr && !m->ud) {
+    // the passes in marker space, then one spreading of their summed force
+    // (u_b buffers alternate so that the last pass writes m->ub)
+    double *buf[2] = {m->ub, m->ub2};
+    const int s0 = m->m_max % 2;
+    hipLaunchKernelGGL(k_ibm_interp_h, gi, dim3(256), 0, st, m->nm, m->R, m->pidx, m->st, o, m->near, m->ds,
+                       m->nlen, m->nbr, m->H, m->rn, m->cds, buf[s0]);
+    for (int it = 1; it <= m->m_max; it++)
+      hipLaunchKernelGGL(k_ibm_pass, gi, dim3(256), 0, st, m->nm, m->R, m->rn, m->cds, m->ds, m->nbr, m->H,
+                         buf[(it - 1 + s0) % 2], buf[(it + s0) % 2], vcm[0], vcm[1], vcm[2], m->Fds, m->dsum, m->Fb,
+                         it == 1 ? 1 : 0, it == m->m_max ? 1 : 0);
+    if (m->m_max > 0 && m->nchunk > 0)
+      hipLaunchKernelGGL(k_ibm_spread, dim3(m->nchunk), dim3(256), 0, st, m->chunk, m->band, m->bstart, m->ent,
                          m->ew, m->Fds, o, -1.0);
-    fa.force = it < m->m_max;
-    hipLaunchKernelGGL(k_ibm_interp, gi, dim3(256), 0, c->stream, m->nm, m->pidx, m->st, o, m->ub, fa);
+  } else {
+    hipLaunchKernelGGL(k_ibm_interp, gi, dim3(256), 0, st, m->nm, m->pidx, m->st, o, m->ub, fa);
+    fa.first = 0;
+    for (int it = 1; it <= m->m_max; it++) {
+      if (m->nchunk > 0)
+        hipLaunchKernelGGL(k_ibm_spread, dim3(m->nchunk), dim3(256), 0, st, m->chunk, m->band, m->bstart,
+                           m->ent, m->ew, m->Fds, o, -1.0);
+      fa.force = it < m->m_max;
+      hipLaunchKernelGGL(k_ibm_interp, gi, dim3(256), 0, st, m->nm, m->pidx, m->st, o, m->ub, fa);
+    }
   }
+
   if (out)
-    hipLaunchKernelGGL(k_ibm_finish, dim3(1), dim3(FIN_THREADS), 0, c->stream, m->nm, m->dsum, npart, m->part,
+    hipLaunchKernelGGL(k_ibm_finish, dim3(1), dim3(FIN_THREADS), 0, st, m->nm, m->dsum, npart, m->part,
                        m->ninside, m->vsolid, vcm[0] - vb[0], vcm[1] - vb[1], out);
   HIPCHK(hipGetLastError());
   return LBM_OK;
@@ -1201,9 +1389,13 @@ static int fsi_iterate(lbm_ctx *c) {
 // the box and the geometry that depends on the marker positions.
 static int ibm_move(lbm_ctx *c, const double vcm[3], const double vb[3]) {
   lbm_ibm *m = c->ibm;
-  hipLaunchKernelGGL(k_ibm_move, blocks(m->nm), dim3(256), 0, c->stream, m->nm, m->mx, m->my, m->mz, vcm[0], vcm[1], vcm[2], vb[0], vb[1], vb[2]);
-  HIPCHK(hipGetLastError());
   const bool moved = vcm[0] + vb[0] != 0 || vcm[1] + vb[1] != 0 || vcm[2] + vb[2] != 0;
+  // (V_CM + V_CM_before) dt/2 = 0 leaves every marker where it is
+  if (moved) {
+    hipLaunchKernelGGL(k_ibm_move, blocks(m->nm), dim3(256), 0, c->stream, m->nm, m->mx, m->my, m->mz, vcm[0], vcm[1],
+                       vcm[2], vb[0], vb[1], vb[2]);
+    HIPCHK(hipGetLastError());
+  }
   if (m->follow && moved) {
     const double dt = 1.0;
     m->cx += (vcm[0] + vb[0]) * dt / 2;
@@ -1218,6 +1410,62 @@ static int ibm_move(lbm_ctx *c, const double vcm[3], const double vb[3]) {
   }
   // the stencils, nearest points, interior cells and bins follow a body that moves
   if (m->moving || moved) return ibm_geometry(c);
+  return LBM_OK;
+}
+
+// The rows of H for the marker-space passes: for each marker the markers
+// closer than 4 cells on every axis (itself included), sorted.  Stencils
+// (floor(x) - 1 .. floor(x) + 2 per axis) of markers 4 or more cells apart
+// on some axis never share a cell, and a rigid motion keeps the list.
+// Buckets of 4 cells; rows longer than 256 entries on average (markers far
+// denser than the lattice) keep the pass-by-pass path.
+static int build_neighbours(lbm_ibm *m, const double *x, const double *y, const double *z) {
+  const int nm = m->nm;
+  std::vector<std::pair<uint64_t, int>> key(nm);
+  auto bucket = [](double v) { return (int64_t)std::floor(v / 4.0); };
+  auto pack = [](int64_t a, int64_t b, int64_t c) {
+    return ((uint64_t)(a & 0x1fffff) << 42) | ((uint64_t)(b & 0x1fffff) << 21) | (uint64_t)(c & 0x1fffff);
+  };
+  for (int n = 0; n < nm; n++) key[n] = {pack(bucket(x[n]), bucket(y[n]), bucket(z[n])), n};
+  std::sort(key.begin(), key.end());
+  std::vector<std::vector<int32_t>> rows(nm);
+  size_t total = 0, maxlen = 0;
+  for (int n = 0; n < nm; n++) {
+    std::vector<int32_t> &row = rows[n];
+    const int64_t bx = bucket(x[n]), by = bucket(y[n]), bz = bucket(z[n]);
+    for (int64_t i = bx - 1; i <= bx + 1; i++)
+      for (int64_t j = by - 1; j <= by + 1; j++)
+        for (int64_t k = bz - 1; k <= bz + 1; k++) {
+          const uint64_t kk = pack(i, j, k);
+          auto it = std::lower_bound(key.begin(), key.end(), std::make_pair(kk, -1));
+          for (; it != key.end() && it->first == kk; ++it) {
+            const int q = it->second;
+            if (std::fabs(x[q] - x[n]) < 4.0 && std::fabs(y[q] - y[n]) < 4.0 && std::fabs(z[q] - z[n]) < 4.0)
+              row.push_back(q);
+          }
+        }
+    std::sort(row.begin(), row.end());
+    total += row.size();
+    maxlen = std::max(maxlen, row.size());
+    if (total > (size_t)256 * nm) return LBM_OK;  // too dense: keep the pass-by-pass path
+  }
+  const int R = (int)((maxlen + 63) / 64 * 64);
+  std::vector<int32_t> len(nm), idx((size_t)nm * R);
+  for (int n = 0; n < nm; n++) {
+    len[n] = (int32_t)rows[n].size();
+    for (int r = 0; r < R; r++) idx[(size_t)n * R + r] = r < len[n] ? rows[n][r] : n;
+  }
+  m->R = R;
+  m->nlen = (int32_t *)dalloc(m, 4 * (size_t)nm);
+  m->nbr = (int32_t *)dalloc(m, 4 * idx.size());
+  m->H = (double *)dalloc(m, 8 * idx.size());
+  m->ub2 = (double *)dalloc(m, 24 * (size_t)nm);
+  m->rn = (double *)dalloc(m, 8 * (size_t)nm);
+  m->cds = (double *)dalloc(m, 8 * (size_t)nm);
+  if (!m->nlen || !m->nbr || !m->H || !m->ub2 || !m->rn || !m->cds) return LBM_ERR_NOMEM;
+  HIPCHK(hipMemcpy(m->nlen, len.data(), 4 * (size_t)nm, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(m->nbr, idx.data(), 4 * idx.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(m->H, 0, 8 * idx.size()));  // the padding entries stay 0
   return LBM_OK;
 }
 
@@ -1275,6 +1523,11 @@ static int ibm_setup(lbm_ctx *c, const lbm_ibm_desc *d, const double *x, const d
   HIPCHK(hipMemsetAsync(m->Fb, 0, 24 * nm, c->stream));
   HIPCHK(hipMemsetAsync(m->Fds, 0, 24 * nm, c->stream));
   HIPCHK(hipMemsetAsync(m->dsum, 0, 16 * nm, c->stream));
+  // rigid bodies of the implicit scheme run the passes in marker space
+  if (m->scheme == 0) {
+    e = build_neighbours(m, x, y, z);
+    if (e != LBM_OK) return e;
+  }
   e = place_box(c, b);
   if (e != LBM_OK) return e;
   e = ibm_geometry(c);

@@ -5,7 +5,10 @@
  * baseline.  Nothing in the product library links or calls this file.
  *
  * Every function cites the reference lines (lbm.cpp / LBM.h / main.cpp of
- * DestinyFlow01/Lattice-Boltzmann-Method---Graduate-FSI-) it restates.  The
+ * DestinyFlow01/Lattice-Boltzmann-Method---Graduate-FSI-) it restates, and
+ * is checked by tests/test_oracle_pins.py or tests/test_oracle_pins_coupling.py
+ * against what the mathematics fixes (closed forms, invariants, brute-force
+ * re-derivations on tiny lattices) rather than against its own formulas.  The
  * arithmetic is written in the reference's own expression order (including
  * its pow() calls) so that a gcc build without -ffast-math reproduces the
  * reference's FP64 rounding wherever the reference itself is deterministic.

@@ -392,3 +392,28 @@ def test_ibm_explicit_rejects_follow_and_forced_setups():
         lat.ibm_setup(mk, box, scheme="explicit")
     assert e.value.code == lbm_amd.LBM_ERR_UNSUPPORTED
     lat.close()
+
+
+def test_ibm_explicit_rejects_density_points_outside_the_box(oracle_mod):
+    """A marker on a lattice point has no neighbour ahead along its normal
+    (floor = ceil on every axis), so the explicit branch reads the density
+    of cell (0, 0, 1), the start value of its search (lbm.cpp:1216-1225):
+    the oracle does; the device, which computes the IBM step's density only
+    inside the box, refuses the set-up instead of returning NaN forces."""
+    case, mk, box = IBM_CASES["slab"]()
+    mk = {k: v.copy() for k, v in mk.items()}
+    mk["x"][0] = float(round(mk["x"][0]))
+    mk["y"][0] = float(round(mk["y"][0]))
+    o = oracle_mod.Oracle(case.Nx, case.Ny, case.Nz, case.tau)
+    o.set_cells(**case.fields())
+    o.init()
+    omk = {k: v.copy() for k, v in mk.items()}
+    err, _ = o.ibm_step(omk, (0, 0, 0), (0, 0, 0), box, scheme="explicit")
+    assert err == 0 and np.isfinite(omk["Fb"]).all()
+    lat = lbm_amd.Lattice(case.Nx, case.Ny, case.Nz, case.tau, storage="f64")
+    lat.upload(**case.fields())
+    lat.init()
+    with pytest.raises(lbm_amd.LBMError) as e:
+        lat.ibm_setup(mk, box, scheme="explicit")
+    assert e.value.code == lbm_amd.LBM_ERR_UNSUPPORTED
+    lat.close()

@@ -1105,6 +1105,9 @@ def test_ibm_explicit_nearest_point_brute_force(oracle_mod):
     o.set_cells(type=t, rho=rho, ux=ux, uy=uy, uz=uz)
     o.init()
     mk = cases.cylinder_markers(7.2, 7.4, 6.0, 24, [1.0])
+    # marker 0 on a lattice point: no neighbour lies ahead along its normal
+    # and the search keeps its start (0, 0, 1)
+    mk["x"][0], mk["y"][0] = float(round(mk["x"][0])), float(round(mk["y"][0]))
     V = (0.02, 0.0, 0.0)
     err, out = o.ibm_step(mk, V, V, (2, 13, 2, 13, 1, 1), scheme="explicit")
     assert err == 0
