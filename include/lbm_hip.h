@@ -87,10 +87,8 @@ int lbm_upload_cells(lbm_ctx *ctx, const int16_t *type, const double *rho,
                      const double *ux, const double *uy, const double *uz,
                      const double *Fx, const double *Fy, const double *Fz);
 
-/* LBM::Init, lbm.cpp:31-57: f = f_star = feq(rho,u) in every cell.
- * LBM_ERR_UNSUPPORTED for an ENERGY set-up with a TYPE_IN cell at k = 0
- * over a fluid or solid cell at k = Nz+1: the back-face write of g into that
- * cell (lbm.cpp:611-617) is not applied by the ENERGY kernels. */
+/* LBM::Init, lbm.cpp:31-57: f = f_star = feq(rho,u) in every cell (with
+ * ENERGY also g = g_star = Calc_geq, lbm.cpp:636-653). */
 int lbm_init(lbm_ctx *ctx);
 /* LBM::Collision + ConvLBM, lbm.cpp:60-179 (TYPE_F cells, BGK + Guo). */
 int lbm_collide(lbm_ctx *ctx);
@@ -144,7 +142,9 @@ int lbm_export_links(lbm_ctx *ctx, int16_t *type, int32_t *src,
  * (ConvLBM, lbm.cpp:121-177), Dirichlet (TYPE_T) and Neumann (TYPE_Q)
  * walls (lbm.cpp:276-288, 965-1028), T and q moments (lbm.cpp:777-798)
  * and the optional Boussinesq force of Rayleigh_Benard (lbm.cpp:803-806).
- * FP64 and FP32 storage. */
+ * The face loops of Streaming move g as they move f (outlet copies, inlet
+ * g = rho w T (1 + ...), the k = 0 inflow write into cell (i,j,Nz+1),
+ * lbm.cpp:309-313, 340-346, 613-619).  FP64 and FP32 storage. */
 /* lbm_thermal_desc.model */
 #define LBM_THERMAL_ENERGY 0 /* the reference's ENERGY branch: D3Q19 g on the *
                               * same lattice, viscous heating                  */

@@ -289,17 +289,44 @@ def random_thermal_case(Nx: int, Ny: int, Nz: int, seed: int = 5, tau: float = 0
     """random_mask_case with temperatures: every cell T in [0.4, 0.9), every
     solid a Dirichlet (TYPE_T) wall.  Exercises the thermal pull (wrap,
     frozen links next to inlet cells, moving Dirichlet walls, the averaged
-    wall temperature).  No inlet on the k = 0 layer: its back-face write
-    would carry g into the fluid or solid cells at k = Nz+1, which the
-    device's ENERGY branch rejects."""
+    wall temperature).  No inlet on the k = 0 layer (thermal_face_case has
+    them)."""
     case = random_mask_case(Nx, Ny, Nz, seed=seed, tau=tau)
     case.type[:, :, 0][case.type[:, :, 0] == TYPE_IN] = TYPE_F
-    rng = np.random.default_rng(seed + 1000)
+    return case, thermal_fields(case, seed=seed + 1000, Pr=Pr)
+
+
+def thermal_fields(case: Case, seed: int, Pr: float = 0.7, boussinesq: bool = False,
+                   g_beta: float = 0.0, T_ref: float = 0.65) -> Thermal:
+    """Temperatures for any case: every cell T in [0.4, 0.9), every solid a
+    Dirichlet (TYPE_T) wall, the ENERGY constants of case.tau and Pr.  With
+    the Boussinesq force the case gets F = (0, g_beta (T - T_ref), 0) in
+    every cell (force mode 1), which MacroProp then rewrites."""
+    rng = np.random.default_rng(seed)
     T = 0.4 + 0.5 * rng.random(case.shape)
     te = np.where(case.type == TYPE_S, TYPE_T, TYPE_F).astype(np.int16)
-    nu = (tau - 0.5) / 3.0
+    nu = (case.tau - 0.5) / 3.0
     alpha, tau_g, k, cp = thermal_constants(nu, Pr)
-    return case, Thermal(T, te, tau_g, nu, cp)
+    if boussinesq:
+        case.Fx = np.zeros(case.shape)
+        case.Fy = g_beta * (T - T_ref)
+        case.Fz = np.zeros(case.shape)
+    return Thermal(T, te, tau_g, nu, cp, boussinesq=boussinesq, g_beta=g_beta, T_ref=T_ref)
+
+
+def thermal_face_case(Nx: int, Ny: int, Nz: int, seed: int = 3,
+                      shell=(TYPE_OUT, TYPE_IN, TYPE_S, TYPE_F, 4), back_inlets=(), boussinesq=False,
+                      g_beta: float = 1e-3):
+    """face_case with temperatures (thermal_fields): the face loops of
+    Streaming on g as well as f (lbm.cpp:299-625, ENERGY), MacroProp's T of
+    the inlet/outlet cells.  back_inlets: (i, j, type) triples that put a
+    TYPE_IN cell at (i, j, 0) and a cell of `type` at (i, j, Nz+1), whose f
+    and g the back-face inflow write then replaces every step."""
+    case = face_case(Nx, Ny, Nz, seed=seed, shell=shell)
+    for i, j, top in back_inlets:
+        case.type[i, j, 0] = TYPE_IN
+        case.type[i, j, -1] = top
+    return case, thermal_fields(case, seed=seed + 2000, boussinesq=boussinesq, g_beta=g_beta)
 
 
 # ---- D3Q7 temperature (LBM_THERMAL_D3Q7; no reference counterpart) -------

@@ -125,6 +125,7 @@ int after_stream(lbm_ctx *c, bool forced_inlets) {
   // the inlets' rho, u are still those of the last MacroProp, which the
   // reference's k = 0 loop reads (lbm.cpp:599-611)
   HIPCHK(launch_backeq_solid(launch_ctx(c), c->cur, c->oth));
+  if (int e = thermal_backeq_solid(c)) return e;  // ENERGY: their g too
   c->backeq_solid_pending = 0;
   // the written solids' new moments enter the wall velocities from the next
   // MacroProp on, also when an earlier MacroProp already replaced the setup
@@ -322,15 +323,9 @@ int lbm_init(lbm_ctx *c) {
   // 2. f = f* = feq (LBM::Init); with ENERGY also g = g* = geq and the
   //    thermal wall tables
   if (c->thermal == 1) {
-    // the back-face inflow write also sets g = rho w T (1 + ...) of cell
-    // (i,j,Nz+1) (lbm.cpp:611-617), which the ENERGY kernels do not apply:
-    // rejected where that cell is fluid or solid (its g reaches the fluid)
-    const uint32_t plane = c->g.plane, top = (c->g.nz - 1) * plane;
-    for (uint32_t d = 0; d < plane; d++)
-      if (c->h_type[d] == LBM_TYPE_IN && (c->h_type[top + d] == LBM_TYPE_F || c->h_type[top + d] == LBM_TYPE_S))
-        return LBM_ERR_UNSUPPORTED;
-    faces_free(c);  // the ENERGY branch keeps these cells at their set-up values
     if (int e = thermal_init(c)) return e;
+    // the inlet/outlet cells' own f, g (the face loops move both)
+    if (int e = faces_setup(c)) return e;
   } else {
     HIPCHK(launch_init(L, c->cur, c->oth));
     if (c->thermal == 2) {
@@ -360,7 +355,10 @@ int lbm_init(lbm_ctx *c) {
     for (uint32_t d = 0; d < plane; d++)
       if (c->h_type[d] == LBM_TYPE_IN && c->h_type[top + d] == LBM_TYPE_S) c->backeq_solid_pending = 1;
   }
-  c->backeq_dynamic = c->backeq_solid_pending && c->inlet_forced_over_solid;
+  // the write changes every step with a force on the inlet (its u), and
+  // under ENERGY (its T = sum g of the last write, i.e. rho T of the one
+  // before)
+  c->backeq_dynamic = c->backeq_solid_pending && (c->inlet_forced_over_solid || c->thermal == 1);
   c->state = ST_F;
   c->fstar_ready = 0;
   return LBM_OK;
@@ -424,6 +422,7 @@ int lbm_stream(lbm_ctx *c) {
   if (c->state != ST_FSTAR) return LBM_ERR_STATE;
   HIPCHK(hipSetDevice(c->dev));
   if (c->thermal == 1) {
+    if (int e = thermal_face_sources(c)) return e;
     if (int e = thermal_launch(c, 2)) return e;
   } else if (c->thermal == 2) {
     if (int e = q7_launch(c, Q7_STREAM)) return e;
@@ -461,6 +460,7 @@ int lbm_macro(lbm_ctx *c) {
 static int step_launch(lbm_ctx *c, const LaunchCtx &L, int mode) {
   if (c->thermal == 1) {
     const int tm = mode == STREAM_COLLIDE ? 0 : mode == STREAM_COLLIDE_MACRO ? 1 : mode == STREAM_ONLY ? 2 : 5;
+    if (int e = thermal_face_sources(c)) return e;
     if (int e = thermal_launch(c, tm)) return e;
   } else if (c->thermal == 2) {
     const int tm = mode == STREAM_COLLIDE ? Q7_FUSED

@@ -76,11 +76,15 @@ struct lbm_ctx {
   int16_t *typeE = nullptr;          // device copy of type_Energy
   struct lbm_output *out = nullptr;  // D3Q7 temperature (lbm_q7.hip), thermal == 2
   // ---- cells of the other types (IN, OUT, P, L, T, Q) and the solids that an
-  // outlet copies from (lbm_faces.hip): FP64 populations and rho, u per slot
+  // outlet copies from (lbm_faces.hip): FP64 populations and rho, u per slot.
+  // ENERGY (thermal == 1): g too, and every cell an outlet copies from has a
+  // slot; the fluid ones hold their pulled f, g (thermal_face_sources)
   int32_t *fslot = nullptr;   // per cell: its slot or -1
-  double *fof = nullptr;      // 19 per slot, direction-major
-  double *fom = nullptr;      // rho, ux, uy, uz per slot, component-major
+  double *fof = nullptr;      // 19 f per slot (ENERGY: then 19 g), direction-major
+  double *fom = nullptr;      // rho, ux, uy, uz (ENERGY: then T, qx, qy, qz) per slot, component-major
   int32_t *fcells = nullptr;  // cell of each slot
+  int32_t *fsrc = nullptr;    // ENERGY: (cell, slot) of the fluid cells an outlet copies from
+  int fnsrc = 0;
   void *fxe = nullptr, *fye = nullptr, *fze = nullptr, *fme = nullptr;  // work lists
   int fnslot = 0;
   int fnx = 0, fnx0 = 0, fny = 0, fny0 = 0;  // i-face and j-face entries
@@ -154,6 +158,9 @@ int faces_setup(lbm_ctx *c);
 int faces_step(lbm_ctx *c, bool ibm, int macro);
 int faces_macro(lbm_ctx *c, bool ibm);
 hipError_t faces_f_out(lbm_ctx *c, double *out);
+hipError_t faces_g_out(lbm_ctx *c, double *out);  // ENERGY: g of the slotted cells
+int thermal_face_sources(lbm_ctx *c);              // before a pulling ENERGY launch
+int thermal_backeq_solid(lbm_ctx *c);              // g of the back-face write into solids
 
 // launch parameters of the context's current state (lbm_capi.hip)
 lbm::LaunchCtx launch_ctx(const lbm_ctx *c);

@@ -45,6 +45,7 @@ constexpr uint8_t M_BOX = 128;
 // sticky status bits
 constexpr int ST_FEQ_NEG = 1;
 constexpr int ST_RHO_RANGE = 2;
+constexpr int ST_T_RANGE = 4;  // temperature outside [0,100] (thermal branches)
 
 template <int... Ms, class F>
 __device__ __forceinline__ void for_dirs_impl(std::integer_sequence<int, Ms...>, F &&f) {

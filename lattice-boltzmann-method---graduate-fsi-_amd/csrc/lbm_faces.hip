@@ -49,6 +49,17 @@
 // In IBM runs (main.cpp:197-211) MacroProp is not called: the IBM step takes
 // rho = sum f, u = sum f c / rho of the TYPE_IN/TYPE_OUT cells with F = 0 and
 // no range check (lbm.cpp:1262-1285, 1173-1193), which mode 3 does.
+//
+// ENERGY: the loops move g alike (outlet copies of g[a, b, c], inlet
+// g = rho w T (1 + 3cu + 4.5cu^2 - 1.5uu) with the inlet's own T,
+// lbm.cpp:309-313, 340-346, 613-619), and MacroProp adds T = sum g,
+// q = sum g c, the temperature check and the Boussinesq F_y (lbm.cpp:
+// 777-806).  The slots hold g after f and T, q after rho, u.  Every cell an
+// outlet copies from has a slot: without wall slots a fluid cell's pulled
+// values exist nowhere after the fused launch, so thermal_face_sources
+// (lbm_thermal.hip) writes them into its slot, from the same buffer and
+// tables, right before each pulling launch.  MacroProp of every slotted
+// cell runs on every step that has one (F_y of an outlet enters its u).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -96,8 +107,8 @@ struct FaceArgs {
   Geo g;
   const S *src;          // the source buffer of the last streaming launch
   const int16_t *type;   // cell types, padded order
-  double *of;            // populations, direction-major
-  double *om;            // rho, ux, uy, uz, component-major
+  double *of;            // populations, direction-major (ENERGY: f rows, then g rows)
+  double *om;            // rho, ux, uy, uz (ENERGY: then T, qx, qy, qz), component-major
   int32_t ns;            // slots
   C *rho, *ux, *uy, *uz;       // the context's macro arrays
   const C *Fx, *Fy, *Fz;       // force mode 1
@@ -105,6 +116,10 @@ struct FaceArgs {
   int mode;                    // force mode 0/1/2; 3: the IBM step's moments
   int *status;
   unsigned long long *first_bad;
+  int th;                      // ENERGY
+  C *T, *qx, *qy, *qz;         // ENERGY: the context's T, q
+  C *Fyb;                      // ENERGY with the Boussinesq force: F_y, rewritten by MacroProp
+  double g_beta, T_ref;
 };
 
 // Population L of source cell s (slot code ss) as the face loops see it.
@@ -123,29 +138,41 @@ __device__ __forceinline__ double pop(const FaceArgs<S> &a, uint32_t s, int32_t 
   return v;
 }
 
-// TYPE_OUT: f[D] of slot ds = f[D] of its inner neighbour s
+// TYPE_OUT: f[D] of slot ds = f[D] of its inner neighbour s (ENERGY: g[D]
+// too, from the source's slot, which every source has there)
 template <class S, int D0, int D1, int D2>
 __device__ __forceinline__ void out_copy(const FaceArgs<S> &a, int32_t ds, uint32_t s, int32_t ss) {
   const double v0 = pop<S, D0>(a, s, ss), v1 = pop<S, D1>(a, s, ss), v2 = pop<S, D2>(a, s, ss);
   a.of[(size_t)D0 * a.ns + ds] = v0;
   a.of[(size_t)D1 * a.ns + ds] = v1;
   a.of[(size_t)D2 * a.ns + ds] = v2;
+  if (a.th && ss >= 0) {  // under ENERGY every outlet source has a slot
+    const size_t g0 = (size_t)Q * a.ns;
+    const double w0 = a.of[g0 + (size_t)D0 * a.ns + ss], w1 = a.of[g0 + (size_t)D1 * a.ns + ss],
+                 w2 = a.of[g0 + (size_t)D2 * a.ns + ss];
+    a.of[g0 + (size_t)D0 * a.ns + ds] = w0;
+    a.of[g0 + (size_t)D1 * a.ns + ds] = w1;
+    a.of[g0 + (size_t)D2 * a.ns + ds] = w2;
+  }
 }
 
 // TYPE_IN: the second-order equilibrium of slot k's rho, u into slot kd
-// (lbm.cpp:325-337), the reference's expression with cs2 = 1/3
+// (lbm.cpp:325-337), the reference's expression with cs2 = 1/3; ENERGY: g
+// with slot k's T too (lbm.cpp:340-346)
 template <class S>
 __device__ __forceinline__ void in_feq2(const FaceArgs<S> &a, int32_t k, int32_t kd) {
 #pragma clang fp contract(off)
   const double rho = a.om[k], ux = a.om[a.ns + k], uy = a.om[2 * (size_t)a.ns + k],
                uz = a.om[3 * (size_t)a.ns + k];
+  const double T = a.th ? a.om[4 * (size_t)a.ns + k] : 0.0;
   const double cs2 = 1.0 / 3.0;
   const double u_dot_u = ux * ux + uy * uy + uz * uz;
   for_dirs([&](auto M) {
     constexpr int l = decltype(M)::value;
     const double c_dot_u = ux * CX[l] + uy * CY[l] + uz * CZ[l];
-    a.of[(size_t)l * a.ns + kd] =
-        rho * W[l] * (1 + c_dot_u / cs2 + (c_dot_u * c_dot_u) / (2 * cs2 * cs2) - u_dot_u / (2 * cs2));
+    const double br = 1 + c_dot_u / cs2 + (c_dot_u * c_dot_u) / (2 * cs2 * cs2) - u_dot_u / (2 * cs2);
+    a.of[(size_t)l * a.ns + kd] = rho * W[l] * br;
+    if (a.th) a.of[(size_t)(Q + l) * a.ns + kd] = rho * W[l] * T * br;
   });
 }
 
@@ -184,7 +211,30 @@ __device__ __forceinline__ void other_macro(const FaceArgs<S> &a, uint32_t idx, 
   a.ux[idx] = (C)ux;
   a.uy[idx] = (C)uy;
   a.uz[idx] = (C)uz;
-  if (a.mode != 3 && (rho < 0 || rho > 100)) flag(a.status, a.first_bad, ST_RHO_RANGE, idx);
+  int bad = (a.mode != 3 && (rho < 0 || rho > 100)) ? ST_RHO_RANGE : 0;
+  if (a.th) {
+    // lbm.cpp:777-806
+    double T = 0, qx = 0, qy = 0, qz = 0;
+    for_dirs([&](auto M) {
+      constexpr int l = decltype(M)::value;
+      const double g = a.of[(size_t)(Q + l) * a.ns + k];
+      T += g;
+      qx += g * CX[l];
+      qy += g * CY[l];
+      qz += g * CZ[l];
+    });
+    a.om[4 * (size_t)a.ns + k] = T;
+    a.om[5 * (size_t)a.ns + k] = qx;
+    a.om[6 * (size_t)a.ns + k] = qy;
+    a.om[7 * (size_t)a.ns + k] = qz;
+    a.T[idx] = (C)T;
+    a.qx[idx] = (C)qx;
+    a.qy[idx] = (C)qy;
+    a.qz[idx] = (C)qz;
+    if (T < 0 || T > 100) bad |= ST_T_RANGE;
+    if (a.Fyb) a.Fyb[idx] = (C)(a.g_beta * (T - a.T_ref));
+  }
+  if (bad) flag(a.status, a.first_bad, bad, idx);
 }
 
 // launch 1 (AXIS 0: entries [0, n0) on i = 0, then i = Nx+1) and launch 2
@@ -242,23 +292,55 @@ __global__ __launch_bounds__(FACE_THREADS) void k_faces_z(FaceArgs<S> a, const Z
   if (macro_due(macro, v.bsrc)) other_macro(a, (uint32_t)v.bot, v.bs, kind_of(v.bsrc));
 }
 
-// Init (lbm.cpp:31-57): f = the cubic equilibrium of the set-up rho, u
+// Init (lbm.cpp:31-57): f = the cubic equilibrium of the set-up rho, u;
+// ENERGY: g = Calc_geq (lbm.cpp:636-653), rho w T (1 + ...) with the cell's
+// set-up T, for a solid rho w T_w with T_w the mean set-up T of its solid
+// neighbours (Determine_Wall_Temperature, lbm.cpp:965-981, l = 0 included)
 template <class S>
 __global__ __launch_bounds__(FACE_THREADS) void k_faces_init(FaceArgs<S> a, const int32_t *__restrict__ cells) {
+  using C = typename Store<S>::C;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= a.ns) return;
   const uint32_t idx = (uint32_t)cells[k];
   const double rho = (double)a.rho[idx], ux = (double)a.ux[idx], uy = (double)a.uy[idx], uz = (double)a.uz[idx];
   const double uu = ux * ux + uy * uy + uz * uz;
+  double T = 0.0;
+  const bool solid = a.type[idx] == LBM_TYPE_S;
+  if (a.th) {
+    if (solid) {
+      uint32_t x, y, z;
+      coords(a.g, idx, x, y, z);
+      C s = C(0);
+      int num = 0;
+      for_dirs([&](auto M) {
+        constexpr int l = decltype(M)::value;
+        const uint32_t n = wrap_dst(a.g, x, y, z, CX[l], CY[l], CZ[l]);
+        if (a.type[n] == LBM_TYPE_S) {
+          s += a.T[n];
+          num++;
+        }
+      });
+      T = (double)(s / C(num));
+    } else {
+      T = (double)a.T[idx];
+    }
+  }
   for_dirs([&](auto M) {
     constexpr int l = decltype(M)::value;
     const double cu = ux * CX[l] + uy * CY[l] + uz * CZ[l];
     a.of[(size_t)l * a.ns + k] = rho * W[l] * (1.0 + eq_A(cu, uu));
+    if (a.th) a.of[(size_t)(Q + l) * a.ns + k] = solid ? rho * W[l] * T : rho * W[l] * T * (1.0 + eq_A(cu, uu));
   });
   a.om[k] = rho;
   a.om[a.ns + k] = ux;
   a.om[2 * (size_t)a.ns + k] = uy;
   a.om[3 * (size_t)a.ns + k] = uz;
+  if (a.th) {
+    a.om[4 * (size_t)a.ns + k] = (double)a.T[idx];
+    a.om[5 * (size_t)a.ns + k] = 0.0;
+    a.om[6 * (size_t)a.ns + k] = 0.0;
+    a.om[7 * (size_t)a.ns + k] = 0.0;
+  }
 }
 
 // the populations of the slotted cells but the fluid ones into a download
@@ -272,6 +354,18 @@ __global__ __launch_bounds__(FACE_THREADS) void k_faces_f_out(const double *__re
   const int k = t / Q, l = t - k * Q;
   if (type[cells[k]] == LBM_TYPE_F) return;
   out[(size_t)cells[k] * Q + l] = of[(size_t)l * n + k];
+}
+
+// ENERGY: the g of the slotted cells but the fluid ones into a download of g
+__global__ __launch_bounds__(FACE_THREADS) void k_faces_g_out(const double *__restrict__ of,
+                                                              const int32_t *__restrict__ cells,
+                                                              const int16_t *__restrict__ type, int n,
+                                                              double *__restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * Q) return;
+  const int k = t / Q, l = t - k * Q;
+  if (type[cells[k]] == LBM_TYPE_F) return;
+  out[(size_t)cells[k] * Q + l] = of[(size_t)(Q + l) * n + k];
 }
 
 template <class S>
@@ -295,6 +389,14 @@ FaceArgs<S> face_args(const lbm_ctx *c, int mode) {
   a.mode = mode;
   a.status = c->status;
   a.first_bad = c->first_bad;
+  a.th = c->thermal == 1;
+  a.T = a.th ? (C *)c->T : nullptr;
+  a.qx = a.th ? (C *)c->qv[0] : nullptr;
+  a.qy = a.th ? (C *)c->qv[1] : nullptr;
+  a.qz = a.th ? (C *)c->qv[2] : nullptr;
+  a.Fyb = a.th && c->td.boussinesq ? (C *)c->F[1] : nullptr;
+  a.g_beta = c->td.g_beta;
+  a.T_ref = c->td.T_ref;
   return a;
 }
 
@@ -358,26 +460,28 @@ bool upload_list(lbm_ctx *c, void **dst, const std::vector<T> &v) {
 }  // namespace
 
 void faces_free(lbm_ctx *c) {
-  void *ptrs[] = {c->fslot, c->fof, c->fom, c->fcells, c->fxe, c->fye, c->fze, c->fme};
+  void *ptrs[] = {c->fslot, c->fof, c->fom, c->fcells, c->fxe, c->fye, c->fze, c->fme, c->fsrc};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   c->fslot = nullptr;
   c->fof = nullptr;
   c->fom = nullptr;
   c->fcells = nullptr;
+  c->fsrc = nullptr;
   c->fxe = c->fye = c->fze = c->fme = nullptr;
   c->fnslot = c->fnx = c->fnx0 = c->fny = c->fny0 = c->fnz = c->fnz_macro = c->fnm = c->fnm_in = c->fnm_xy = 0;
+  c->fnsrc = 0;
 }
 
 // Slots for the cells of the other types, for the solids that an outlet
 // copies from and for the fluid cells under the k = 0 inflow write that an
-// outlet copies from; their Init populations; the work lists.  Call after
-// the set-up rho, u are in the macro arrays and before anything overwrites
-// the solids' entries there.  The ENERGY branch keeps these cells at their
-// set-up values (no slots).
+// outlet copies from (ENERGY: for every cell an outlet copies from); their
+// Init populations; the work lists.  Call after the set-up rho, u (and T)
+// are in the macro arrays and before anything overwrites the solids'
+// entries there.
 int faces_setup(lbm_ctx *c) {
   faces_free(c);
-  if (c->thermal == 1) return LBM_OK;
+  const bool th = c->thermal == 1;
   const lbm::Geo &g = c->g;
   const std::vector<int16_t> &t = c->h_type;
   auto at = [&](uint32_t x, uint32_t y, uint32_t z) { return (int32_t)((z * g.ny + y) * g.px + x); };
@@ -435,7 +539,7 @@ int faces_setup(lbm_ctx *c) {
   const int32_t top = (int32_t)((g.nz - 1) * g.plane);
   auto backeq_fluid = [&](int32_t s) { return t[s] == LBM_TYPE_F && s >= top && t[s - top] == LBM_TYPE_IN; };
   auto need_src = [&](int32_t d, int32_t s) {
-    if (t[d] == LBM_TYPE_OUT && (t[s] == LBM_TYPE_S || backeq_fluid(s))) give(s);
+    if (t[d] == LBM_TYPE_OUT && (th || t[s] == LBM_TYPE_S || backeq_fluid(s))) give(s);
   };
   for (auto &e : xl) need_src(e.first, e.second);
   for (auto &e : yl) need_src(e.first, e.second);
@@ -446,7 +550,7 @@ int faces_setup(lbm_ctx *c) {
     }
   auto ss_of = [&](int32_t s) {
     if (slot[s] < 0) return (int32_t)-1;
-    return t[s] == LBM_TYPE_F ? -slot[s] - 2 : slot[s];
+    return t[s] == LBM_TYPE_F && !th ? -slot[s] - 2 : slot[s];
   };
   auto src_code = [&](int32_t d, int32_t s) {
     if (!other(d)) return SRC_NONE;
@@ -510,11 +614,21 @@ int faces_setup(lbm_ctx *c) {
     nm_in += v.kind == K_IN;
     nm_xy += xy_final[v.idx];
   }
-  const size_t ns = cells.size();
-  if (ns > (size_t)INT32_MAX / Q) return LBM_ERR_UNSUPPORTED;
+  // ENERGY: the fluid cells with a slot (outlet sources), whose pulled
+  // values thermal_face_sources writes
+  std::vector<int32_t> fsrc;
+  if (th)
+    for (size_t k = 0; k < cells.size(); k++)
+      if (t[cells[k]] == LBM_TYPE_F) {
+        fsrc.push_back(cells[k]);
+        fsrc.push_back((int32_t)k);
+      }
+  const size_t ns = cells.size(), nrow = th ? 2 * Q : Q, nmac = th ? 8 : 4;
+  if (ns > (size_t)INT32_MAX / nrow) return LBM_ERR_UNSUPPORTED;
   bool ok = hipMalloc((void **)&c->fslot, g.ncell * sizeof(int32_t)) == hipSuccess &&
-            hipMalloc((void **)&c->fof, ns * Q * sizeof(double)) == hipSuccess &&
-            hipMalloc((void **)&c->fom, ns * 4 * sizeof(double)) == hipSuccess &&
+            hipMalloc((void **)&c->fof, ns * nrow * sizeof(double)) == hipSuccess &&
+            hipMalloc((void **)&c->fom, ns * nmac * sizeof(double)) == hipSuccess &&
+            upload_list(c, (void **)&c->fsrc, fsrc) &&
             upload_list(c, (void **)&c->fcells, cells) && upload_list(c, &c->fxe, xe) &&
             upload_list(c, &c->fye, ye) && upload_list(c, &c->fze, ze) && upload_list(c, &c->fme, me) &&
             hipMemcpyAsync(c->fslot, slot.data(), g.ncell * sizeof(int32_t), hipMemcpyHostToDevice, c->stream) ==
@@ -534,6 +648,7 @@ int faces_setup(lbm_ctx *c) {
   c->fnm = (int)me.size();
   c->fnm_in = nm_in;
   c->fnm_xy = nm_xy;
+  c->fnsrc = (int)(fsrc.size() / 2);
   HIPCHK(dispatch_init(c));
   // the host vectors go out of scope: the copies must have landed
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -546,6 +661,7 @@ int faces_setup(lbm_ctx *c) {
 // of MacroProp.
 int faces_step(lbm_ctx *c, bool ibm, int macro) {
   if (!c->fnslot) return LBM_OK;
+  if (c->thermal == 1 && macro == 1) macro = 2;  // ENERGY: every slotted cell's MacroProp, every step
   HIPCHK(dispatch_pass(c, ibm ? 3 : c->force_mode, 1, macro));
   return LBM_OK;
 }
@@ -559,6 +675,13 @@ int faces_macro(lbm_ctx *c, bool ibm) {
 hipError_t faces_f_out(lbm_ctx *c, double *out) {
   if (!c->fnslot) return hipSuccess;
   hipLaunchKernelGGL(k_faces_f_out, dim3(blocks((uint64_t)c->fnslot * Q)), dim3(FACE_THREADS), 0, c->stream,
+                     c->fof, c->fcells, c->type, c->fnslot, out);
+  return hipGetLastError();
+}
+
+hipError_t faces_g_out(lbm_ctx *c, double *out) {
+  if (!c->fnslot || c->thermal != 1) return hipSuccess;
+  hipLaunchKernelGGL(k_faces_g_out, dim3(blocks((uint64_t)c->fnslot * Q)), dim3(FACE_THREADS), 0, c->stream,
                      c->fof, c->fcells, c->type, c->fnslot, out);
   return hipGetLastError();
 }

@@ -36,7 +36,6 @@ using namespace lbm;
 namespace {
 
 constexpr int Q7 = 7;
-constexpr int ST_T_RANGE = 4;
 constexpr uint32_t ALL19 = (1u << Q) - 1;
 // links bit 20 + m (fluid cells): the D3Q7 link m ends on a TYPE_T solid
 constexpr int DIRICHLET_BIT = 20;

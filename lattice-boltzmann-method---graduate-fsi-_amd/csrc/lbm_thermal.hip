@@ -22,6 +22,13 @@
 //
 // Storage: FP64 holds g; FP32 holds g - w T0 (T0 near the mean temperature)
 // so that small temperature differences keep their digits.
+//
+// The face loops' g (lbm_faces.hip) reach the fluid through the back-face
+// inflow write only: a cell at k = Nz+1 above a TYPE_IN cell at k = 0 takes
+// g = rho w T (1 + 3cu + 4.5cu^2 - 1.5uu) of that inlet after every Streaming
+// (lbm.cpp:613-619) -- in the pull of a fluid cell (backeq_g), in both
+// buffers of a solid (k_th_backeq_solid, whose T then feeds the wall
+// temperature tables).
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
@@ -37,7 +44,6 @@ using namespace lbm;
 
 namespace {
 
-constexpr int ST_T_RANGE = 4;
 
 enum ThMode {
   TH_FUSED = 0,        // Streaming + MacroProp + next Collision
@@ -79,6 +85,24 @@ __device__ __forceinline__ typename Store<S>::C g_raw(typename Store<S>::C val, 
 template <class C>
 __device__ __forceinline__ C one_plus_A(C cu, C uu) {
   return C(1) + eq_A(cu, uu);
+}
+
+// The back-face inflow write's g (lbm.cpp:613-619): rho w T (1 + 3cu +
+// 4.5cu^2 - 1.5uu) of the TYPE_IN cell s (its rho, u, T of the last
+// MacroProp), as raw storage values.
+template <class S>
+__device__ __forceinline__ void backeq_g(uint32_t s, const KArgs<typename Store<S>::C> &a,
+                                         const ThArgs<typename Store<S>::C> &t,
+                                         typename Store<S>::C (&g)[Q]) {
+  using C = typename Store<S>::C;
+  const C r = a.rho[s], vx = a.ux[s], vy = a.uy[s], vz = a.uz[s], T = t.T[s];
+  const C uu = vx * vx + vy * vy + vz * vz;
+  for_dirs([&](auto M) {
+    constexpr int m = decltype(M)::value;
+    const C cu = C(CX[m]) * vx + C(CY[m]) * vy + C(CZ[m]) * vz;
+    const C B = C(3) * cu + C(4.5) * cu * cu - C(1.5) * uu;
+    g[m] = g_raw<S>(r * C(W[m]) * T * (C(1) + B), W[m], t.T0);
+  });
 }
 
 // Pull of g for one fluid cell: periodic wrap and the thermal walls.  f must
@@ -190,6 +214,7 @@ __global__ __launch_bounds__(256) void k_thermal(const Planes<S> Pf, const Plane
     bulk_loads<S>(Pg, idx, g);
     if (mk & (M_WALL | M_WRAP | M_BACKEQ)) patch_cell<S>(Pf, mk, idx, a, f);
     if (mk & (M_WALL | M_WRAP)) pull_g<S>(Pg, mk, idx, a, t, g);
+    if (mk & M_BACKEQ) backeq_g<S>(idx - (a.g.nz - 1) * a.g.plane, a, t, g);
   } else {
     for_dirs([&](auto M) {
       constexpr int m = decltype(M)::value;
@@ -425,6 +450,74 @@ __global__ __launch_bounds__(256) void k_th_init(S *__restrict__ A, S *__restric
   t.rho_w[idx] = a.rho[idx];
 }
 
+// The pulled f and g of the fluid cells an outlet copies from, into their
+// face slots (FP64, direction-major: f rows, then g rows): the values the
+// next pulling launch computes for them, from the same buffer and tables,
+// before its back-face inflow write.  Under that write (M_BACKEQ) the links
+// from cells of the other types keep the slot's value: the reference's
+// frozen link holds the previous step's write there.
+template <class S>
+__global__ __launch_bounds__(256) void k_th_face_src(const Planes<S> Pf, const Planes<S> Pg,
+                                                     const uint8_t *__restrict__ mask,
+                                                     const int16_t *__restrict__ type,
+                                                     KArgs<typename Store<S>::C> a,
+                                                     ThArgs<typename Store<S>::C> t,
+                                                     const int32_t *__restrict__ list, int n,
+                                                     double *__restrict__ of, int ns) {
+  using C = typename Store<S>::C;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t idx = (uint32_t)list[2 * k];
+  const size_t slot = (size_t)list[2 * k + 1];
+  C f[Q], g[Q];
+  bulk_loads<S>(Pf, idx, f);
+  bulk_loads<S>(Pg, idx, g);
+  const uint8_t mk0 = mask[idx];
+  const uint8_t mk = mk0 & (uint8_t)~M_BACKEQ;
+  if (mk & (M_WALL | M_WRAP)) {
+    patch_cell<S>(Pf, mk, idx, a, f);
+    pull_g<S>(Pg, mk, idx, a, t, g);
+  }
+  uint32_t x = 0, y = 0, z = 0;
+  if (mk0 & M_BACKEQ) coords(a.g, idx, x, y, z);
+  for_dirs([&](auto M) {
+    constexpr int m = decltype(M)::value;
+    if (mk0 & M_BACKEQ) {
+      const int16_t tq = type[wrap_src(a.g, x, y, z, CX[m], CY[m], CZ[m])];
+      if (tq != LBM_TYPE_F && tq != LBM_TYPE_S) return;
+    }
+    double fv = (double)f[m], gv = (double)g[m];
+    if constexpr (Store<S>::shifted) {
+      fv = fv / (double)Store<S>::K + W[m];
+      gv = gv + W[m] * (double)t.T0;
+    }
+    of[(size_t)m * ns + slot] = fv;
+    of[(size_t)(Q + m) * ns + slot] = gv;
+  });
+}
+
+// The back-face inflow write's g into the solids at k = Nz+1 above a
+// TYPE_IN cell (lbm.cpp:613-619), both buffers (f: k_backeq_solid)
+template <class S>
+__global__ __launch_bounds__(256) void k_th_backeq_solid(S *__restrict__ A, S *__restrict__ B,
+                                                         const uint8_t *__restrict__ mask,
+                                                         KArgs<typename Store<S>::C> a,
+                                                         ThArgs<typename Store<S>::C> t) {
+  using C = typename Store<S>::C;
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.g.ncell) return;
+  if ((mask[idx] & (M_SOLID | M_BACKEQ)) != (M_SOLID | M_BACKEQ)) return;
+  C g[Q];
+  backeq_g<S>(idx - (a.g.nz - 1) * a.g.plane, a, t, g);
+  const uint32_t ps = a.g.ps;
+  for_dirs([&](auto M) {
+    constexpr int m = decltype(M)::value;
+    const S v = Store<S>::enc(g[m]);
+    A[(uint32_t)(Q + m) * ps + idx] = v;
+    B[(uint32_t)(Q + m) * ps + idx] = v;
+  });
+}
+
 // g as FP64 values, [idx][m]
 template <class S>
 __global__ __launch_bounds__(256) void k_g_out(const S *__restrict__ buf, Geo g,
@@ -612,6 +705,35 @@ int thermal_download_g(lbm_ctx *c, double *g_dev) {
   return LBM_OK;
 }
 
+template <class S>
+void face_src_s(lbm_ctx *c) {
+  const LaunchCtx L = launch_ctx(c);
+  const Planes<S> Pf = planes<S>(c, c->cur, nullptr, 0), Pg = planes<S>(c, c->cur, nullptr, Q);
+  hipLaunchKernelGGL(k_th_face_src<S>, dim3((c->fnsrc + 255) / 256), dim3(256), 0, c->stream, Pf, Pg, c->mask,
+                     c->type, kargs<S>(L), thargs<S>(c), c->fsrc, c->fnsrc, c->fof, c->fnslot);
+}
+
+int thermal_face_sources(lbm_ctx *c) {
+  if (c->thermal != 1 || !c->fnsrc) return LBM_OK;
+  if (c->storage == LBM_STORE_F64) face_src_s<double>(c);
+  else face_src_s<float>(c);
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
+int thermal_backeq_solid(lbm_ctx *c) {
+  if (c->thermal != 1) return LBM_OK;
+  const LaunchCtx L = launch_ctx(c);
+  if (c->storage == LBM_STORE_F64)
+    hipLaunchKernelGGL(k_th_backeq_solid<double>, grid_of(c), dim3(c->block), 0, c->stream, (double *)c->cur,
+                       (double *)c->oth, c->mask, kargs<double>(L), thargs<double>(c));
+  else
+    hipLaunchKernelGGL(k_th_backeq_solid<float>, grid_of(c), dim3(c->block), 0, c->stream, (float *)c->cur,
+                       (float *)c->oth, c->mask, kargs<float>(L), thargs<float>(c));
+  HIPCHK(hipGetLastError());
+  return LBM_OK;
+}
+
 extern "C" {
 
 int lbm_upload_thermal(lbm_ctx *c, const lbm_thermal_desc *d, const double *T,
@@ -741,6 +863,7 @@ int lbm_download_g(lbm_ctx *c, double *g) {
   double *d = nullptr;
   HIPCHK(hipMalloc((void **)&d, n * ng * sizeof(double)));
   int e = c->thermal == 2 ? q7_download_g(c, d) : thermal_download_g(c, d);
+  if (e == LBM_OK && c->thermal == 1 && faces_g_out(c, d) != hipSuccess) e = LBM_ERR_HIP;
   std::vector<double> h;
   if (e == LBM_OK) {
     h.resize(n * ng);

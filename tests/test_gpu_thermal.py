@@ -140,16 +140,112 @@ def test_thermal_rejects_fp16_and_unset_walls():
     lat.close()
 
 
-def test_thermal_rejects_inflow_write_into_fluid():
-    """The k = 0 inlet's back-face write sets g of cell (i,j,Nz+1) too
-    (lbm.cpp:611-617); the ENERGY kernels do not apply it, so a set-up where
-    that cell is fluid or solid is rejected at lbm_init."""
+def _cmp_faces(lat, o, case, tol_rho=1e-13, tol_u=1e-12, tol_T=1e-12, tol_pop=1e-12):
+    """rho, u, T, q of every cell but the solids; f and g of the fluid and
+    the other-type cells (the face loops' own values there)."""
+    m = case.type != cases.TYPE_S
+    got, ref = lat.macro(), o.macro()
+    er = float(np.abs(got[0] - ref[0])[m].max())
+    scale = max(float(np.abs(np.stack(ref[1:]))[..., m].max()), 1e-30)
+    eu = max(float(np.abs(got[q] - ref[q])[m].max()) for q in (1, 2, 3)) / scale
+    th, rth = lat.thermal(), o.thermal()
+    eT = float(np.abs(th[0] - rth[0])[m].max())
+    eq = max(float(np.abs(th[q] - rth[q])[m].max()) for q in (1, 2, 3))
+    ef = float(np.abs(lat.populations() - o.f())[m].max())
+    eg = float(np.abs(lat.g_populations() - o.g())[m].max())
+    print(f"rho {er:.2e}  u {eu:.2e}  T {eT:.2e}  q {eq:.2e}  f {ef:.2e}  g {eg:.2e}")
+    assert er <= tol_rho, f"rho {er:.3e}"
+    assert eu <= tol_u, f"u {eu:.3e}"
+    assert eT <= tol_T, f"T {eT:.3e}"
+    assert eq <= tol_T, f"q {eq:.3e}"
+    assert ef <= tol_pop, f"f {ef:.3e}"
+    assert eg <= tol_pop, f"g {eg:.3e}"
+
+
+FACE_TH = {
+    # outlets, inlets, solids, fluid and TYPE_P on the shell, inlets at k = 0
+    # under fluid and under solid cells at k = Nz+1
+    "shell": dict(Nx=5, Ny=4, Nz=3, seed=3, back_inlets=((2, 2, cases.TYPE_F), (3, 1, cases.TYPE_S))),
+    "outlets_inlets": dict(Nx=5, Ny=6, Nz=3, seed=11, shell=(cases.TYPE_OUT, cases.TYPE_IN),
+                           back_inlets=((1, 1, cases.TYPE_F), (4, 5, cases.TYPE_S), (2, 3, cases.TYPE_OUT))),
+    "all_types": dict(Nx=5, Ny=4, Nz=3, seed=22,
+                      shell=(cases.TYPE_OUT, cases.TYPE_IN, cases.TYPE_S, cases.TYPE_F, 4, 5)),
+    "boussinesq": dict(Nx=5, Ny=4, Nz=3, seed=7, boussinesq=True,
+                       back_inlets=((2, 2, cases.TYPE_F), (3, 1, cases.TYPE_S))),
+}
+
+
+@pytest.mark.parametrize("name", sorted(FACE_TH))
+def test_thermal_face_loops_fp64(oracle_mod, name):
+    """The ENERGY face loops on the device (lbm.cpp:299-625 with g: outlet
+    copies, inlet g = rho w T (1 + ...), the k = 0 write into (i,j,Nz+1))
+    and MacroProp's T, q of the inlet/outlet cells, fused steps, against the
+    oracle's Collision, Streaming, MacroProp."""
+    case, th = cases.thermal_face_case(**FACE_TH[name])
+    steps = 30
+    o = _oracle(oracle_mod, case, th)
+    assert o.run(steps) == 0
+    lat = _gpu(case, th)
+    lat.step(steps)
+    assert lat.sync() == 0
+    _cmp_faces(lat, o, case)
+    lat.close()
+
+
+def test_thermal_face_loops_reference_calls_fp64(oracle_mod):
+    """The same through lbm_collide, lbm_stream, lbm_macro (the drop-in
+    call sequence), checked after every step."""
+    case, th = cases.thermal_face_case(**FACE_TH["shell"])
+    o = _oracle(oracle_mod, case, th)
+    lat = _gpu(case, th)
+    for _ in range(10):
+        assert o.collision() == 0
+        o.streaming()
+        assert o.macroprop() == 0
+        lat.collision()
+        lat.streaming()
+        lat.macroprop()
+        assert lat.sync() == 0
+        _cmp_faces(lat, o, case)
+    lat.close()
+
+
+def test_thermal_face_loops_fp32(oracle_mod):
+    """FP32 storage against the FP64 oracle, 100 steps: T of every non-solid
+    cell (inlets included, whose T = sum g of the last inflow write grows
+    as rho T) within 1e-5 of the largest |T| of the oracle's, u within 1e-5
+    (relative L2)."""
+    case, th = cases.thermal_face_case(**FACE_TH["shell"])
+    steps = 100
+    o = _oracle(oracle_mod, case, th)
+    assert o.run(steps) == 0
+    lat = _gpu(case, th, storage="f32")
+    lat.step(steps)
+    assert lat.sync() == 0
+    m = case.type != cases.TYPE_S
+    T, r_T = lat.thermal()[0], o.thermal()[0]
+    eT = float(np.abs(T - r_T)[m].max()) / float(np.abs(r_T[m]).max())
+    _, ux, uy, uz = lat.macro()
+    _, r_ux, r_uy, r_uz = o.macro()
+    eu = rel_l2_u((ux, uy, uz), (r_ux, r_uy, r_uz), m)
+    print(f"f32 faces: max|dT|/max|T| {eT:.3e}, rel L2 u {eu:.3e}")
+    assert eT <= 1e-5
+    assert eu <= 1e-5
+    lat.close()
+
+
+def test_thermal_inflow_write_into_fluid_and_solid_couette(oracle_mod):
+    """The heated Couette flow with a k = 0 inlet under a fluid cell and one
+    under the moving wall: the back-face write carries the inlet's g into
+    the fluid, and into the wall, whose T = sum g then enters the Dirichlet
+    wall temperature of its fluid neighbours (lbm.cpp:613-619, 965-981)."""
     case, th = cases.couette_thermal()
     case.type[1, 3, 0] = cases.TYPE_IN  # fluid above it at k = Nz+1 (periodic z)
-    lat = lbm_amd.Lattice(case.Nx, case.Ny, case.Nz, case.tau)
-    lat.upload(**case.fields())
-    lat.upload_thermal(th.T, th.type_energy, tau_g=th.tau_g, nu=th.nu, cp=th.cp)
-    with pytest.raises(lbm_amd.LBMError) as e:
-        lat.init()
-    assert e.value.code == lbm_amd.LBM_ERR_UNSUPPORTED
+    case.type[1, 0, 0] = cases.TYPE_IN  # the bottom wall above it
+    o = _oracle(oracle_mod, case, th)
+    assert o.run(40) == 0
+    lat = _gpu(case, th)
+    lat.step(40)
+    assert lat.sync() == 0
+    _cmp_faces(lat, o, case)
     lat.close()
