@@ -59,6 +59,16 @@ extern "C" {
  * saturate.  One significant bit more than F16S at the same 2 bytes. */
 #define LBM_STORE_F16C 3
 
+/* lbm_desc.bounce_back: how the half-way bounce-back of Streaming
+ * (lbm.cpp:268-275) reaches the pull.  AUTO: a face of the padded lattice
+ * whose cells are all TYPE_S has its wall links resolved in the pull address
+ * (the fluid cell reads its own f*(x, opp m)); every other wall link goes
+ * through the solid's slot, which the wall cell writes after its collision.
+ * SLOTS: every wall link through the solid's slot.  Both give the same
+ * results (bit for bit at resting walls). */
+#define LBM_BB_AUTO 0
+#define LBM_BB_SLOTS 1
+
 /* lbm_reduce_fp64 quantities */
 #define LBM_REDUCE_MASS 0     /* sum of rho over TYPE_F cells           */
 #define LBM_REDUCE_MOMENTUM 1 /* out[0..2] = sum of rho*u over TYPE_F   */
@@ -72,7 +82,8 @@ typedef struct lbm_desc {
   int32_t storage;    /* LBM_STORE_*                                        */
   int32_t device;     /* HIP device ordinal                                 */
   double tau;         /* relaxation time; omega = dt/tau (lbm.cpp:76)       */
-  int32_t reserved[8];
+  int32_t bounce_back; /* LBM_BB_*                                          */
+  int32_t reserved[7];
 } lbm_desc;
 
 /* LBM::LBM(Nx,Ny,Nz,tau), lbm.cpp:17-28: allocates the device lattice. */
@@ -400,6 +411,12 @@ int lbm_time_steps(lbm_ctx *ctx, int32_t nsteps, double *ms);
  * counterpart: the launch-shape knob. */
 int lbm_set_launch(lbm_ctx *ctx, int32_t block, int32_t cells_per_thread);
 int lbm_get_launch(const lbm_ctx *ctx, int32_t *block, int32_t *cells_per_thread);
+/* After lbm_init: the faces whose wall links the pull resolves in its
+ * address (bit 0: i = 0, 1: i = Nx+1, 2: j = 0, 3: j = Ny+1, 4: k = 0,
+ * 5: k = Nz+1; see LBM_BB_AUTO) and whether some wall link still goes
+ * through a solid's slot (1) or none does (0).  No reference counterpart:
+ * it reports how Streaming's bounce-back (lbm.cpp:268-275) is laid out. */
+int lbm_bounce_back_faces(const lbm_ctx *ctx, int32_t *faces, int32_t *slot_links);
 /* Bytes one fused launch moves per fluid cell (algorithmic). */
 double lbm_bytes_per_cell(const lbm_ctx *ctx);
 const char *lbm_error_string(int code);

@@ -21,6 +21,7 @@
 
 #include "../../include/lbm_hip.h"
 #include "lbm_ctx.h"
+#include "lbm_device.h"
 #include "lbm_launch.h"
 
 using namespace lbm;
@@ -57,7 +58,63 @@ LaunchCtx launch_ctx(const lbm_ctx *c) {
   // bounce-back through the solids' slots (the ENERGY kernels patch their
   // wall links themselves)
   L.wall_slots = c->thermal == 1 ? 0 : 1;
+  L.slot_links = c->links != nullptr && (c->g.fold == 0 || c->slot_links);
   return L;
+}
+
+// Faces of the padded lattice whose cells are all TYPE_S (Geo::fold bits),
+// and whether some fluid cell still borders a solid off those faces.
+static uint32_t solid_faces(const lbm_ctx *c) {
+  const Geo &g = c->g;
+  const auto solid = [&](uint32_t x, uint32_t y, uint32_t z) {
+    return c->h_type[((size_t)z * g.ny + y) * g.px + x] == LBM_TYPE_S;
+  };
+  uint32_t f = 63;
+  for (uint32_t z = 0; z < g.nz; z++)
+    for (uint32_t y = 0; y < g.ny; y++) {
+      if (!solid(0, y, z)) f &= ~1u;
+      if (!solid(g.nx - 1, y, z)) f &= ~2u;
+    }
+  for (uint32_t z = 0; z < g.nz; z++)
+    for (uint32_t x = 0; x < g.nx; x++) {
+      if (!solid(x, 0, z)) f &= ~4u;
+      if (!solid(x, g.ny - 1, z)) f &= ~8u;
+    }
+  for (uint32_t y = 0; y < g.ny; y++)
+    for (uint32_t x = 0; x < g.nx; x++) {
+      if (!solid(x, y, 0)) f &= ~16u;
+      if (!solid(x, y, g.nz - 1)) f &= ~32u;
+    }
+  return f;
+}
+
+static bool unfolded_wall_links(const lbm_ctx *c) {
+  const Geo &g = c->g;
+  const uint32_t f = g.fold;
+  for (uint32_t z = 0; z < g.nz; z++)
+    for (uint32_t y = 0; y < g.ny; y++)
+      for (uint32_t x = 0; x < g.nx; x++) {
+        if (c->h_type[((size_t)z * g.ny + y) * g.px + x] != LBM_TYPE_S) continue;
+        if (((f & 1u) && x == 0) || ((f & 2u) && x == g.nx - 1) || ((f & 4u) && y == 0) ||
+            ((f & 8u) && y == g.ny - 1) || ((f & 16u) && z == 0) || ((f & 32u) && z == g.nz - 1))
+          continue;
+        for (int m = 1; m < Q; m++) {  // the pushing target x + c_m, wrapped
+          const uint32_t xs = (x + g.nx + CX[m]) % g.nx, ys = (y + g.ny + CY[m]) % g.ny,
+                         zs = (z + g.nz + CZ[m]) % g.nz;
+          if (c->h_type[((size_t)zs * g.ny + ys) * g.px + xs] == LBM_TYPE_F) return true;
+        }
+      }
+  return false;
+}
+
+// The shell fold is used when the population offsets it forms fit 32 bits:
+// lane offset (idx + ps + plane + px + 1) elements plus a fold or wrap
+// correction of at most ps + plane + px + 1 or nx + plane + nz plane.
+static bool fold_fits(const lbm_ctx *c) {
+  const Geo &g = c->g;
+  const uint64_t bias = (uint64_t)g.ps + g.plane + g.px + 1;
+  const uint64_t wrap = (uint64_t)g.nx + g.plane + (uint64_t)g.nz * g.plane;
+  return ((uint64_t)g.ncell + bias + std::max(bias, wrap)) * c->esize < (1ull << 32);
 }
 
 // reference order r = (i*ny + j)*nz + k  <->  device order d = (k*ny + j)*px + i
@@ -158,6 +215,7 @@ int lbm_create(const lbm_desc *desc, lbm_ctx **out) {
   if (desc->lattice != 19) return LBM_ERR_UNSUPPORTED;
   if (desc->storage < LBM_STORE_F64 || desc->storage > LBM_STORE_F16C) return LBM_ERR_ARG;
   if (!(desc->tau > 0.5) || !std::isfinite(desc->tau)) return LBM_ERR_ARG;
+  if (desc->bounce_back != LBM_BB_AUTO && desc->bounce_back != LBM_BB_SLOTS) return LBM_ERR_ARG;
   const uint64_t nx = desc->nx + 2ull, ny = desc->ny + 2ull, nz = desc->nz + 2ull;
   const uint64_t px = (nx + 3) / 4 * 4;  // row pitch: 4-cell chunks start aligned
   const uint64_t ncell = px * ny * nz;
@@ -315,6 +373,14 @@ int lbm_init(lbm_ctx *c) {
   if (any_solid && !c->links) {
     HIPCHK(hipMalloc((void **)&c->links, n * 4));
     for (int q = 0; q < 3; q++) HIPCHK(hipMalloc(&c->uw[q], n * c->csize));
+  }
+  // wall links into all-solid shell faces resolved in the pull address (the
+  // ENERGY and D3Q7 kernels keep their own wall handling)
+  c->g.fold = 0;
+  c->slot_links = any_solid;
+  if (any_solid && c->thermal == 0 && c->d.bounce_back == LBM_BB_AUTO && fold_fits(c)) {
+    c->g.fold = solid_faces(c);
+    c->slot_links = c->g.fold == 0 || unfolded_wall_links(c);
   }
   LaunchCtx L = launch_ctx(c);
   // 1. kernel mask, bounce-back links and wall velocities from the setup
@@ -682,6 +748,14 @@ int lbm_get_launch(const lbm_ctx *c, int32_t *block, int32_t *cells_per_thread) 
   if (!c) return LBM_ERR_ARG;
   if (block) *block = c->block;
   if (cells_per_thread) *cells_per_thread = c->cpt;
+  return LBM_OK;
+}
+
+int lbm_bounce_back_faces(const lbm_ctx *c, int32_t *faces, int32_t *slot_links) {
+  if (!c) return LBM_ERR_ARG;
+  if (c->state == ST_NONE) return LBM_ERR_STATE;
+  if (faces) *faces = (int32_t)c->g.fold;
+  if (slot_links) *slot_links = c->slot_links;
   return LBM_OK;
 }
 

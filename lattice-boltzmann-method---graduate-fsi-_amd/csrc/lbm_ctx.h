@@ -55,6 +55,10 @@ struct lbm_ctx {
   // change every step
   int backeq_dynamic = 0;
   int inlet_forced_over_solid = 0;  // set by lbm_upload_cells
+  // bounce-back of the solid shell faces (g.fold, set by lbm_init): some
+  // fluid cell has a wall link that is not folded, so the stream kernel
+  // reads link words and writes solids' slots
+  int slot_links = 0;
   int64_t nfluid = 0;
   int block = 256;
   int cpt = 1;  // cells per thread of the stream kernel

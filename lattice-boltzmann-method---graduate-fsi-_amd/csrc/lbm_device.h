@@ -86,6 +86,62 @@ __device__ __forceinline__ uint32_t pull_off(const Geo &g) {
   return (uint32_t)(CX[m] + CY[m] * (int)g.px + CZ[m] * (int)g.plane);
 }
 
+// ---- shell fold ------------------------------------------------------------
+// A fluid cell x whose pull source x - c_m (wrapped) lies on a face of the
+// padded lattice listed in g.fold (every cell of that face TYPE_S) takes the
+// bounce-back value f*(x, opp m) + 6 w_m (u_w . c_m) from its own cell: plane
+// opp m, which nothing else reads.  Source on face x = 0 for c_x > 0 means
+// x == 1 (or x == 0 wrapping onto x = nx-1 when that face is the solid one);
+// likewise for the other five faces.  Bit m of the result: direction m
+// folds.  Cells off the fluid interior get meaningless bits.
+template <int A, int SGN>
+constexpr uint32_t axis_dirs() {
+  uint32_t b = 0;
+  for (int m = 0; m < Q; m++) {
+    const int c = A == 0 ? CX[m] : A == 1 ? CY[m] : CZ[m];
+    if (SGN > 0 ? c > 0 : c < 0) b |= 1u << m;
+  }
+  return b;
+}
+struct FoldLane {
+  bool xp, xm, yp, ym, zp, zm;  // directions with c > 0 / c < 0 on each axis fold
+};
+__device__ __forceinline__ FoldLane fold_lane(const Geo &g, uint32_t x, uint32_t y, uint32_t z) {
+  const uint32_t f = g.fold;
+  FoldLane l;
+  l.xp = ((f & 1u) && x == 1) || ((f & 2u) && x == 0);
+  l.xm = ((f & 2u) && x == g.nx - 2) || ((f & 1u) && x == g.nx - 1);
+  l.yp = ((f & 4u) && y == 1) || ((f & 8u) && y == 0);
+  l.ym = ((f & 8u) && y == g.ny - 2) || ((f & 4u) && y == g.ny - 1);
+  l.zp = ((f & 16u) && z == 1) || ((f & 32u) && z == 0);
+  l.zm = ((f & 32u) && z == g.nz - 2) || ((f & 16u) && z == g.nz - 1);
+  return l;
+}
+template <int m>
+__device__ __forceinline__ bool folds(const FoldLane &l) {
+  bool r = false;
+  if constexpr (CX[m] > 0) r = r || l.xp;
+  if constexpr (CX[m] < 0) r = r || l.xm;
+  if constexpr (CY[m] > 0) r = r || l.yp;
+  if constexpr (CY[m] < 0) r = r || l.ym;
+  if constexpr (CZ[m] > 0) r = r || l.zp;
+  if constexpr (CZ[m] < 0) r = r || l.zm;
+  return r;
+}
+__device__ __forceinline__ uint32_t fold_mask(const Geo &g, uint32_t x, uint32_t y, uint32_t z) {
+  if (!g.fold) return 0u;
+  const FoldLane l = fold_lane(g, x, y, z);
+  return (l.xp ? axis_dirs<0, 1>() : 0u) | (l.xm ? axis_dirs<0, -1>() : 0u) |
+         (l.yp ? axis_dirs<1, 1>() : 0u) | (l.ym ? axis_dirs<1, -1>() : 0u) |
+         (l.zp ? axis_dirs<2, 1>() : 0u) | (l.zm ? axis_dirs<2, -1>() : 0u);
+}
+// cell on a face listed in g.fold
+__device__ __forceinline__ bool on_fold_face(const Geo &g, uint32_t x, uint32_t y, uint32_t z) {
+  const uint32_t f = g.fold;
+  return ((f & 1u) && x == 0) || ((f & 2u) && x == g.nx - 1) || ((f & 4u) && y == 0) ||
+         ((f & 8u) && y == g.ny - 1) || ((f & 16u) && z == 0) || ((f & 32u) && z == g.nz - 1);
+}
+
 // ---- population storage -------------------------------------------------
 // Stored value s, compute ("raw") value r and population f:
 //   double: r = s = f                          K = 1  (unshifted, FP64)

@@ -120,6 +120,7 @@ struct FaceArgs {
   C *T, *qx, *qy, *qz;         // ENERGY: the context's T, q
   C *Fyb;                      // ENERGY with the Boussinesq force: F_y, rewritten by MacroProp
   double g_beta, T_ref;
+  const C *uwx, *uwy, *uwz;    // wall velocities (the folded links' moving-wall term)
 };
 
 // Population L of source cell s (slot code ss) as the face loops see it.
@@ -128,6 +129,14 @@ __device__ __forceinline__ double pop(const FaceArgs<S> &a, uint32_t s, int32_t 
   if (ss >= 0) return a.of[(size_t)L * a.ns + ss];
   uint32_t x, y, z;
   coords(a.g, s, x, y, z);
+  if ((fold_mask(a.g, x, y, z) >> L) & 1u) {
+    // a wall link folded into the pull: f*(s, opp L) from s's own plane
+    // opp L plus the moving-wall term, the value the pull took
+    double v = (double)Store<S>::ld(a.src + (size_t)OPP[L] * a.g.ps + s);
+    if constexpr (Store<S>::shifted) v = v / (double)Store<S>::K + W[L];
+    const double ucw = CX[L] * (double)a.uwx[s] + CY[L] * (double)a.uwy[s] + CZ[L] * (double)a.uwz[s];
+    return v + 6.0 * W[L] * ucw;
+  }
   const uint32_t q = wrap_src(a.g, x, y, z, CX[L], CY[L], CZ[L]);
   if (ss <= -2) {
     const int16_t tq = a.type[q];
@@ -397,6 +406,9 @@ FaceArgs<S> face_args(const lbm_ctx *c, int mode) {
   a.Fyb = a.th && c->td.boussinesq ? (C *)c->F[1] : nullptr;
   a.g_beta = c->td.g_beta;
   a.T_ref = c->td.T_ref;
+  a.uwx = (const C *)c->uw[0];
+  a.uwy = (const C *)c->uw[1];
+  a.uwz = (const C *)c->uw[2];
   return a;
 }
 

@@ -1113,6 +1113,7 @@ static KArgs<typename Store<S>::C> kargs_of(const LaunchCtx &L) {
   a.status = L.status;
   a.first_bad = L.first_bad;
   a.wall_slots = L.wall_slots;
+  a.slot_links = L.slot_links;
   return a;
 }
 

@@ -226,9 +226,10 @@ __global__ __launch_bounds__(256) void k_tables(const int16_t *__restrict__ type
       if (j % g.px < g.nx && type[j] == 0) line_fluid = true;
     if (line_fluid) {
       lk = (1u << Q) - 1;
-      if (x < g.nx) {
-        // slot m of a solid is written by its neighbour x + c_m when that is
-        // a fluid cell (wall_slot_stores)
+      // slot m of a solid is written by its neighbour x + c_m when that is
+      // a fluid cell (wall_slot_stores), unless the solid's face is folded
+      // (its wall links are resolved in the pull address)
+      if (x < g.nx && !on_fold_face(g, x, y, z)) {
         for_dirs([&](auto M) {
           constexpr int m = decltype(M)::value;
           if (type[wrap_dst(g, x, y, z, CX[m], CY[m], CZ[m])] == 0) lk &= ~(1u << m);
@@ -245,11 +246,14 @@ __global__ __launch_bounds__(256) void k_tables(const int16_t *__restrict__ type
       mk |= M_WRAP;
     if (z == g.nz - 1 && type[idx - (g.nz - 1) * g.plane] == 3) mk |= M_BACKEQ;
     if (t == 0) {
-      // bounce-back links: source x - c_m is solid
+      // bounce-back links: source x - c_m is solid; the link word keeps
+      // those not folded into the pull address (fold_mask)
+      uint32_t all = 0;
       for_dirs([&](auto M) {
         constexpr int m = decltype(M)::value;
-        if (type[wrap_src(g, x, y, z, CX[m], CY[m], CZ[m])] == 1) lk |= 1u << m;
+        if (type[wrap_src(g, x, y, z, CX[m], CY[m], CZ[m])] == 1) all |= 1u << m;
       });
+      lk = all & ~fold_mask(g, x, y, z);
       // Determine_Wall_Velocity, lbm.cpp:941-962: neighbours x + c_l in l order
       int num = 0;
       for_dirs([&](auto M) {
@@ -268,7 +272,7 @@ __global__ __launch_bounds__(256) void k_tables(const int16_t *__restrict__ type
         wz /= C(num);
       }
       if (lk) mk |= M_WALL;
-      if (lk && (wx != C(0) || wy != C(0) || wz != C(0))) mk |= M_MOVING;
+      if (all && (wx != C(0) || wy != C(0) || wz != C(0))) mk |= M_MOVING;
     }
   }
   mask[idx] = mk;
@@ -342,7 +346,7 @@ __global__ __launch_bounds__(256) void k_links(const uint8_t *__restrict__ mask,
   }
   uint32_t x, y, z;
   coords(g, idx, x, y, z);
-  const uint32_t lk = (mk & M_WALL) ? links[idx] : 0u;
+  const uint32_t lk = ((mk & M_WALL) ? links[idx] : 0u) | fold_mask(g, x, y, z);
   for_dirs([&](auto M) {
     constexpr int m = decltype(M)::value;
     const uint32_t src = (mk & M_WRAP) ? wrap_src(g, x, y, z, CX[m], CY[m], CZ[m])
@@ -477,6 +481,7 @@ static KArgs<typename Store<S>::C> make_args(const LaunchCtx &L) {
   a.first_bad = L.first_bad;
   a.box = L.box;
   a.wall_slots = L.wall_slots;
+  a.slot_links = L.slot_links;
   return a;
 }
 

@@ -19,6 +19,11 @@ struct Geo {
   uint32_t ps;          // population plane stride (elements), >= ncell
   // n / px and n / ny for n < 2^31 as (mulhi(n, m) >> s) (set_divisors)
   uint32_t px_m, px_s, ny_m, ny_s;
+  // faces of the padded lattice whose cells are all TYPE_S and whose wall
+  // links the pull resolves in the address (bit 0: x = 0, 1: x = nx-1,
+  // 2: y = 0, 3: y = ny-1, 4: z = 0, 5: z = nz-1); 0 without solids, in the
+  // thermal branches and when the lattice is too large for 32-bit offsets
+  uint32_t fold;
 };
 
 // Division by an invariant d >= 2 for 0 <= n < 2^31 (Granlund and
@@ -70,6 +75,7 @@ struct LaunchCtx {
   const uint8_t *mask;
   BoxArgs box;        // force mode 3
   int wall_slots;     // bounce-back through the solids' slots
+  int slot_links;     // ... of which some are not folded (Geo::fold): links are read
   uint8_t macro_sel;  // k_macro: cells whose mask has any of these bits
   int *status;
   unsigned long long *first_bad;

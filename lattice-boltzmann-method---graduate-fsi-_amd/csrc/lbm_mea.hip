@@ -57,8 +57,10 @@ __global__ __launch_bounds__(MEA_THREADS) void k_mea(const S *__restrict__ fstar
     const int i = B.x0 + (int)(b % B.nx), j = B.y0 + (int)((b / B.nx) % B.ny), k = B.z0 + (int)(b / (B.nx * B.ny));
     const uint32_t idx = ((uint32_t)k * g.ny + (uint32_t)j) * g.px + (uint32_t)i;
     const uint8_t mk = mask[idx];
-    if (!(mk & M_FLUID) || !(mk & M_WALL)) continue;
-    const uint32_t lk = links[idx];
+    if (!(mk & M_FLUID)) continue;
+    // the wall links: the link word's and those folded into the pull
+    const uint32_t lk = ((mk & M_WALL) ? links[idx] : 0u) | fold_mask(g, (uint32_t)i, (uint32_t)j, (uint32_t)k);
+    if (!lk) continue;
     for_dirs([&](auto M) {
       constexpr int m = decltype(M)::value;
       if constexpr (m > 0) {

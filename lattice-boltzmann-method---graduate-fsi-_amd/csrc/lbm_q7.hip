@@ -413,6 +413,7 @@ KArgs<C> kargs7(const LaunchCtx &L) {
   a.first_bad = L.first_bad;
   a.box = L.box;
   a.wall_slots = 1;
+  a.slot_links = L.links != nullptr;
   return a;
 }
 

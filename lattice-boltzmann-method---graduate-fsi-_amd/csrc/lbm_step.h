@@ -27,6 +27,9 @@ struct KArgs {
   // writes f*(x, opp m) + 2 w_m (u_w . c_m)/cs2 into slot m of the solid
   // x - c_m, which the next pull then reads like any streamed value.
   int wall_slots;
+  // the stream kernel reads the link words: some wall link is not folded
+  // into the pull address (g.fold), so some wall cell writes a solid's slot
+  int slot_links;
 };
 
 // Per-direction plane pointers, computed on the host for each launch so
@@ -332,6 +335,98 @@ __device__ __forceinline__ void bulk_loads_wrapped(const Planes<S> &P, const Geo
   wrapped_loads<S, Q>(P.src, g, idx, f);
 }
 
+// Bulk loads with the periodic wrap and the shell fold (g.fold) both in the
+// addresses.  A folded direction m reads the cell's own plane opp m,
+//   in[opp m] + idx = src[m] + idx + (opp m - m) ps + off_m,
+// instead of its source's slot, so the solids of a folded face need no
+// bounce-back stores and the pull no link word.  The lane offset is taken
+// from a base lowered by ps + plane + px + 1 elements, the most negative
+// fold correction (it exceeds every wrap correction); the host enables the
+// fold only where every resulting offset fits 32 bits.  Every address stays
+// inside the population allocation: a folded one is the cell's own entry,
+// the others are those of wrapped_loads.
+template <class S>
+__device__ __forceinline__ void folded_loads(const Planes<S> &P, const Geo &g, uint32_t idx,
+                                             typename Store<S>::C (&f)[Q]) {
+  uint32_t x, y, z;
+  coords(g, idx, x, y, z);
+  const uint32_t e = (uint32_t)sizeof(S);
+  const uint32_t ex = g.nx * e, ey = g.ny * g.px * e, ez = g.nz * g.plane * e;
+  const uint32_t bias_elems = g.ps + g.plane + g.px + 1;
+  const uint32_t xp = x == 0 ? ex : 0u, xm = x == g.nx - 1 ? ex : 0u;
+  const uint32_t yp = y == 0 ? ey : 0u, ym = y == g.ny - 1 ? ey : 0u;
+  const uint32_t zp = z == 0 ? ez : 0u, zm = z == g.nz - 1 ? ez : 0u;
+  const FoldLane fl = fold_lane(g, x, y, z);
+  const uint32_t boff = (idx + bias_elems) * e;
+  const uint32_t pse = g.ps * e, pxe = g.px * e, ple = g.plane * e;
+  for_dirs([&](auto M) {
+    constexpr int m = decltype(M)::value;
+    uint32_t o = boff;
+    if constexpr (CX[m] > 0) o += xp;
+    if constexpr (CX[m] < 0) o -= xm;
+    if constexpr (CY[m] > 0) o += yp;
+    if constexpr (CY[m] < 0) o -= ym;
+    if constexpr (CZ[m] > 0) o += zp;
+    if constexpr (CZ[m] < 0) o -= zm;
+    if constexpr (m > 0) {
+      // (opp m - m) ps + off_m in bytes, modulo 2^32
+      const uint32_t d = (OPP[m] > m ? pse : 0u - pse) + (uint32_t)CX[m] * e + (uint32_t)CY[m] * pxe +
+                         (uint32_t)CZ[m] * ple;
+      o = folds<m>(fl) ? boff + d : o;
+    }
+    f[m] = Store<S>::ld(at(P.src[m] - bias_elems, o));
+  });
+}
+
+// The moving-wall term of the folded directions, 6 K w_m (u_w . c_m)
+// (lbm.cpp:268-275), for a fluid cell with a wall velocity.
+template <class S>
+__device__ __forceinline__ void fold_moving(const KArgs<typename Store<S>::C> &a, uint32_t idx,
+                                            typename Store<S>::C (&f)[Q]) {
+  using C = typename Store<S>::C;
+  uint32_t x, y, z;
+  coords(a.g, idx, x, y, z);
+  const FoldLane fl = fold_lane(a.g, x, y, z);
+  const C uwx = a.uwx[idx], uwy = a.uwy[idx], uwz = a.uwz[idx];
+  for_dirs([&](auto M) {
+    constexpr int m = decltype(M)::value;
+    if constexpr (m > 0) {
+      if (folds<m>(fl)) {
+        const C ucw = C(CX[m]) * uwx + C(CY[m]) * uwy + C(CZ[m]) * uwz;
+        f[m] = f[m] + C(Store<S>::K * 6.0 * W[m]) * ucw;
+      }
+    }
+  });
+}
+
+// The folded directions of a cell pulled by bulk_loads and patch_cell (the
+// kernels off the hot path): reloaded from the cell's own planes opp m.
+template <class S>
+__device__ __forceinline__ void fold_patch(const Planes<S> &P, uint8_t mk, uint32_t idx,
+                                           const KArgs<typename Store<S>::C> &a,
+                                           typename Store<S>::C (&f)[Q]) {
+  using C = typename Store<S>::C;
+  uint32_t x, y, z;
+  coords(a.g, idx, x, y, z);
+  const FoldLane fl = fold_lane(a.g, x, y, z);
+  const uint32_t boff = idx * (uint32_t)sizeof(S);
+  C uwx = C(0), uwy = C(0), uwz = C(0);
+  if (mk & M_MOVING) {
+    uwx = a.uwx[idx];
+    uwy = a.uwy[idx];
+    uwz = a.uwz[idx];
+  }
+  for_dirs([&](auto M) {
+    constexpr int m = decltype(M)::value;
+    if constexpr (m > 0) {
+      if (folds<m>(fl)) {
+        const C ucw = C(CX[m]) * uwx + C(CY[m]) * uwy + C(CZ[m]) * uwz;
+        f[m] = Store<S>::ld(at(P.in[OPP[m]], boff)) + C(Store<S>::K * 6.0 * W[m]) * ucw;
+      }
+    }
+  });
+}
+
 // An empty asm statement that takes the bulk-loaded values as inputs and
 // passes the mask byte through: the mask test then depends on every load
 // having completed, so neither the loads nor the test can be moved across
@@ -438,6 +533,10 @@ __device__ __forceinline__ void pull_cell(const Planes<S> &P, uint8_t mk, uint32
                                           typename Store<S>::C (&f)[Q]) {
   bulk_loads<S>(P, idx, f);
   if (mk & patch_bits(a)) patch_cell<S>(P, mk, idx, a, f);
+  if (a.g.fold) {
+    // the k = 0 inflow write (M_BACKEQ) replaced every population
+    if (!(mk & M_BACKEQ)) fold_patch<S>(P, mk, idx, a, f);
+  }
 }
 
 // Box index of a cell inside the IBM box (mask bit M_BOX).
@@ -522,22 +621,30 @@ __device__ __forceinline__ void stream_cell(const Planes<S> &P, const uint8_t *_
   using C = typename Store<S>::C;
   constexpr uint32_t ALL = (1u << Q) - 1;
   C f[Q];
-  // the populations (periodic wrap in the addresses), the mask and, where
-  // solids exist, the bounce-back link word are all in flight together
-  bulk_loads_wrapped<S>(P, a.g, idx, f);
+  // the populations (periodic wrap and shell fold in the addresses), the
+  // mask and, where some wall link is not folded, the bounce-back link word
+  // are all in flight together
+  if (a.g.fold) folded_loads<S>(P, a.g, idx, f);
+  else bulk_loads_wrapped<S>(P, a.g, idx, f);
   uint32_t mk = mask[idx];
   uint32_t lk = 0;
-  if (a.links && a.wall_slots) {
+  if (a.slot_links && a.wall_slots) {
     lk = a.links[idx];
     after_loads2(mk, lk);
   }
   mk = after_loads(mk, f);
+  // with no link word, solid and row-padding lanes write every slot (nothing
+  // reads them) in waves that hold a fluid cell, so those lines leave whole
+  const bool wave_fluid = __builtin_amdgcn_ballot_w64(mk & M_FLUID) != 0;
   uint32_t st = ALL;  // slots this lane stores
   if (mk & M_FLUID) {
+    if (a.g.fold && (mk & M_MOVING)) fold_moving<S>(a, idx, f);
     if (mk & (patch_bits(a) & ~M_WRAP)) patch_cell<S, true>(P, (uint8_t)mk, idx, a, f);
     update_cell<S, FORCE, COLLIDE, EMIT>(f, (uint8_t)mk, idx, a);
   } else {
-    st = !a.wall_slots ? 0u : a.links ? lk : (mk == 0 ? ALL : 0u);
+    st = !a.wall_slots ? 0u
+         : a.slot_links ? lk
+                        : ((mk == 0 || (mk & M_SOLID)) && wave_fluid ? ALL : 0u);
     if (!st) return;
     for (int m = 0; m < Q; m++) f[m] = C(0);
   }

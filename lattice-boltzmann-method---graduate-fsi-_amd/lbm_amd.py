@@ -34,6 +34,9 @@ STORE_F64, STORE_F32, STORE_F16S, STORE_F16C = 0, 1, 2, 3
 STORAGE_NAMES = {"f64": STORE_F64, "f32": STORE_F32, "f16s": STORE_F16S, "f16c": STORE_F16C}
 
 REDUCE_MASS, REDUCE_MOMENTUM, REDUCE_KINETIC = 0, 1, 2
+# lbm_desc.bounce_back: solid shell faces folded into the pull address, or
+# every wall link through the solid's slot
+BB_AUTO, BB_SLOTS = 0, 1
 
 
 class LBMError(RuntimeError):
@@ -51,7 +54,8 @@ class _Desc(C.Structure):
         ("storage", C.c_int32),
         ("device", C.c_int32),
         ("tau", C.c_double),
-        ("reserved", C.c_int32 * 8),
+        ("bounce_back", C.c_int32),
+        ("reserved", C.c_int32 * 7),
     ]
 
 
@@ -151,6 +155,7 @@ _SIGNATURES = {
     "lbm_bytes_per_cell": (C.c_double, [_P]),
     "lbm_set_launch": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "lbm_get_launch": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "lbm_bounce_back_faces": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "lbm_error_string": (C.c_char_p, [C.c_int]),
     "lbm_copy_probe": (C.c_int, [C.c_int32, C.c_int64, C.c_int32, _D]),
     "lbm_storage_roundtrip": (C.c_int, [C.c_int32, C.c_int32, _D, _D, C.c_int64]),
@@ -294,11 +299,12 @@ class Lattice:
     macroprop, plus step(n) (the fused fast path) and downloads.
     """
 
-    def __init__(self, Nx: int, Ny: int, Nz: int, tau: float, storage="f64", device=0):
+    def __init__(self, Nx: int, Ny: int, Nz: int, tau: float, storage="f64", device=0,
+                 bounce_back=BB_AUTO):
         lib = load_library()
         if isinstance(storage, str):
             storage = STORAGE_NAMES[storage]
-        d = _Desc(Nx, Ny, Nz, 19, storage, device, float(tau))
+        d = _Desc(Nx, Ny, Nz, 19, storage, device, float(tau), int(bounce_back))
         h = _P()
         _check(lib.lbm_create(C.byref(d), C.byref(h)), "lbm_create")
         self._lib = lib
@@ -554,6 +560,13 @@ class Lattice:
         b, v = C.c_int32(0), C.c_int32(0)
         _check(self._lib.lbm_get_launch(self._h, C.byref(b), C.byref(v)), "lbm_get_launch")
         return b.value, v.value
+
+    def bounce_back_faces(self):
+        """(faces, slot_links) after init: the folded-face bits (i = 0, Nx+1,
+        j = 0, Ny+1, k = 0, Nz+1) and whether some wall link uses a slot."""
+        f, s = C.c_int32(), C.c_int32()
+        _check(self._lib.lbm_bounce_back_faces(self._h, C.byref(f), C.byref(s)), "lbm_bounce_back_faces")
+        return f.value, s.value
 
     @property
     def ncells(self) -> int:
