@@ -413,9 +413,12 @@ int lbm_set_launch(lbm_ctx *ctx, int32_t block, int32_t cells_per_thread);
 int lbm_get_launch(const lbm_ctx *ctx, int32_t *block, int32_t *cells_per_thread);
 /* After lbm_init: the faces whose wall links the pull resolves in its
  * address (bit 0: i = 0, 1: i = Nx+1, 2: j = 0, 3: j = Ny+1, 4: k = 0,
- * 5: k = Nz+1; see LBM_BB_AUTO) and whether some wall link still goes
- * through a solid's slot (1) or none does (0).  No reference counterpart:
- * it reports how Streaming's bounce-back (lbm.cpp:268-275) is laid out. */
+ * 5: k = Nz+1; see LBM_BB_AUTO), in bits 8-13 the faces of inlet/outlet
+ * cells (all cells neither fluid nor solid, bar the solids of such a face)
+ * whose frozen-link values the fused launch rewrites in place, and whether
+ * some wall link still goes through a solid's slot (1) or none does (0).
+ * No reference counterpart: it reports how Streaming's bounce-back and
+ * frozen links (lbm.cpp:252-292) are laid out. */
 int lbm_bounce_back_faces(const lbm_ctx *ctx, int32_t *faces, int32_t *slot_links);
 /* Bytes one fused launch moves per fluid cell (algorithmic). */
 double lbm_bytes_per_cell(const lbm_ctx *ctx);

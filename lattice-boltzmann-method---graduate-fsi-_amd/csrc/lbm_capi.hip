@@ -88,6 +88,37 @@ static uint32_t solid_faces(const lbm_ctx *c) {
   return f;
 }
 
+// Keep faces (Geo::fold bits 8-13): faces, not folded, whose cells are all
+// neither fluid nor solid, except solids of a folded face.
+static uint32_t keep_faces(const lbm_ctx *c) {
+  const Geo &g = c->g;
+  const uint32_t f = g.fold;
+  const auto ok = [&](uint32_t x, uint32_t y, uint32_t z) {
+    const int16_t t = c->h_type[((size_t)z * g.ny + y) * g.px + x];
+    if (t != LBM_TYPE_F && t != LBM_TYPE_S) return true;
+    if (t == LBM_TYPE_F) return false;
+    return ((f & 1u) && x == 0) || ((f & 2u) && x == g.nx - 1) || ((f & 4u) && y == 0) ||
+           ((f & 8u) && y == g.ny - 1) || ((f & 16u) && z == 0) || ((f & 32u) && z == g.nz - 1);
+  };
+  uint32_t k = 63u & ~f;
+  for (uint32_t z = 0; z < g.nz; z++)
+    for (uint32_t y = 0; y < g.ny; y++) {
+      if (!ok(0, y, z)) k &= ~1u;
+      if (!ok(g.nx - 1, y, z)) k &= ~2u;
+    }
+  for (uint32_t z = 0; z < g.nz; z++)
+    for (uint32_t x = 0; x < g.nx; x++) {
+      if (!ok(x, 0, z)) k &= ~4u;
+      if (!ok(x, g.ny - 1, z)) k &= ~8u;
+    }
+  for (uint32_t y = 0; y < g.ny; y++)
+    for (uint32_t x = 0; x < g.nx; x++) {
+      if (!ok(x, y, 0)) k &= ~16u;
+      if (!ok(x, y, g.nz - 1)) k &= ~32u;
+    }
+  return k << 8;
+}
+
 static bool unfolded_wall_links(const lbm_ctx *c) {
   const Geo &g = c->g;
   const uint32_t f = g.fold;
@@ -378,9 +409,12 @@ int lbm_init(lbm_ctx *c) {
   // ENERGY and D3Q7 kernels keep their own wall handling)
   c->g.fold = 0;
   c->slot_links = any_solid;
-  if (any_solid && c->thermal == 0 && c->d.bounce_back == LBM_BB_AUTO && fold_fits(c)) {
-    c->g.fold = solid_faces(c);
-    c->slot_links = c->g.fold == 0 || unfolded_wall_links(c);
+  if (c->thermal == 0 && c->d.bounce_back == LBM_BB_AUTO && fold_fits(c)) {
+    if (any_solid) {
+      c->g.fold = solid_faces(c);
+      c->slot_links = c->g.fold == 0 || unfolded_wall_links(c);
+    }
+    c->g.fold |= keep_faces(c);
   }
   LaunchCtx L = launch_ctx(c);
   // 1. kernel mask, bounce-back links and wall velocities from the setup

@@ -129,15 +129,17 @@ __device__ __forceinline__ bool folds(const FoldLane &l) {
   return r;
 }
 __device__ __forceinline__ uint32_t fold_mask(const Geo &g, uint32_t x, uint32_t y, uint32_t z) {
-  if (!g.fold) return 0u;
+  if (!(g.fold & 63u)) return 0u;
   const FoldLane l = fold_lane(g, x, y, z);
   return (l.xp ? axis_dirs<0, 1>() : 0u) | (l.xm ? axis_dirs<0, -1>() : 0u) |
          (l.yp ? axis_dirs<1, 1>() : 0u) | (l.ym ? axis_dirs<1, -1>() : 0u) |
          (l.zp ? axis_dirs<2, 1>() : 0u) | (l.zm ? axis_dirs<2, -1>() : 0u);
 }
-// cell on a face listed in g.fold
+// cell on a face listed in bits 0-5 (fold) or, with shift 8, bits 8-13
+// (keep) of g.fold
+template <int SHIFT = 0>
 __device__ __forceinline__ bool on_fold_face(const Geo &g, uint32_t x, uint32_t y, uint32_t z) {
-  const uint32_t f = g.fold;
+  const uint32_t f = g.fold >> SHIFT;
   return ((f & 1u) && x == 0) || ((f & 2u) && x == g.nx - 1) || ((f & 4u) && y == 0) ||
          ((f & 8u) && y == g.ny - 1) || ((f & 16u) && z == 0) || ((f & 32u) && z == g.nz - 1);
 }

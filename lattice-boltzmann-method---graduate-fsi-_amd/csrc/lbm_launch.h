@@ -21,8 +21,12 @@ struct Geo {
   uint32_t px_m, px_s, ny_m, ny_s;
   // faces of the padded lattice whose cells are all TYPE_S and whose wall
   // links the pull resolves in the address (bit 0: x = 0, 1: x = nx-1,
-  // 2: y = 0, 3: y = ny-1, 4: z = 0, 5: z = nz-1); 0 without solids, in the
-  // thermal branches and when the lattice is too large for 32-bit offsets
+  // 2: y = 0, 3: y = ny-1, 4: z = 0, 5: z = nz-1); bits 8-13, the same
+  // faces: every cell neither fluid nor solid (inlets, outlets) except
+  // solids of a folded face, so the stream kernel rewrites those cells'
+  // frozen-link values unchanged instead of leaving holes in its stores.
+  // 0 in the thermal branches and when the lattice is too large for 32-bit
+  // offsets
   uint32_t fold;
 };
 

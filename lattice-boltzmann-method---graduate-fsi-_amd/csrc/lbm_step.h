@@ -339,14 +339,17 @@ __device__ __forceinline__ void bulk_loads_wrapped(const Planes<S> &P, const Geo
 // addresses.  A folded direction m reads the cell's own plane opp m,
 //   in[opp m] + idx = src[m] + idx + (opp m - m) ps + off_m,
 // instead of its source's slot, so the solids of a folded face need no
-// bounce-back stores and the pull no link word.  The lane offset is taken
-// from a base lowered by ps + plane + px + 1 elements, the most negative
-// fold correction (it exceeds every wrap correction); the host enables the
-// fold only where every resulting offset fits 32 bits.  Every address stays
-// inside the population allocation: a folded one is the cell's own entry,
-// the others are those of wrapped_loads.
+// bounce-back stores and the pull no link word.  A lane on a keep face
+// (bits 8-13) reads its own cell in every plane, in[m] + idx = src[m] + idx
+// + off_m: an inlet or outlet cell there stores its frozen-link values back
+// unchanged.  The lane offset is taken from a base lowered by ps + plane +
+// px + 1 elements, the most negative correction (it exceeds every wrap
+// correction); the host enables the fold only where every resulting offset
+// fits 32 bits.  Every address stays inside the population allocation: a
+// folded or kept one is the cell's own entry, the others are those of
+// wrapped_loads.  Returns whether the lane is on a keep face.
 template <class S>
-__device__ __forceinline__ void folded_loads(const Planes<S> &P, const Geo &g, uint32_t idx,
+__device__ __forceinline__ bool folded_loads(const Planes<S> &P, const Geo &g, uint32_t idx,
                                              typename Store<S>::C (&f)[Q]) {
   uint32_t x, y, z;
   coords(g, idx, x, y, z);
@@ -357,6 +360,7 @@ __device__ __forceinline__ void folded_loads(const Planes<S> &P, const Geo &g, u
   const uint32_t yp = y == 0 ? ey : 0u, ym = y == g.ny - 1 ? ey : 0u;
   const uint32_t zp = z == 0 ? ez : 0u, zm = z == g.nz - 1 ? ez : 0u;
   const FoldLane fl = fold_lane(g, x, y, z);
+  const bool kp = on_fold_face<8>(g, x, y, z);
   const uint32_t boff = (idx + bias_elems) * e;
   const uint32_t pse = g.ps * e, pxe = g.px * e, ple = g.plane * e;
   for_dirs([&](auto M) {
@@ -369,13 +373,15 @@ __device__ __forceinline__ void folded_loads(const Planes<S> &P, const Geo &g, u
     if constexpr (CZ[m] > 0) o += zp;
     if constexpr (CZ[m] < 0) o -= zm;
     if constexpr (m > 0) {
-      // (opp m - m) ps + off_m in bytes, modulo 2^32
-      const uint32_t d = (OPP[m] > m ? pse : 0u - pse) + (uint32_t)CX[m] * e + (uint32_t)CY[m] * pxe +
-                         (uint32_t)CZ[m] * ple;
+      // off_m and (opp m - m) ps + off_m in bytes, modulo 2^32
+      const uint32_t off = (uint32_t)CX[m] * e + (uint32_t)CY[m] * pxe + (uint32_t)CZ[m] * ple;
+      const uint32_t d = (OPP[m] > m ? pse : 0u - pse) + off;
       o = folds<m>(fl) ? boff + d : o;
+      o = kp ? boff + off : o;
     }
     f[m] = Store<S>::ld(at(P.src[m] - bias_elems, o));
   });
+  return kp;
 }
 
 // The moving-wall term of the folded directions, 6 K w_m (u_w . c_m)
@@ -394,34 +400,6 @@ __device__ __forceinline__ void fold_moving(const KArgs<typename Store<S>::C> &a
       if (folds<m>(fl)) {
         const C ucw = C(CX[m]) * uwx + C(CY[m]) * uwy + C(CZ[m]) * uwz;
         f[m] = f[m] + C(Store<S>::K * 6.0 * W[m]) * ucw;
-      }
-    }
-  });
-}
-
-// The folded directions of a cell pulled by bulk_loads and patch_cell (the
-// kernels off the hot path): reloaded from the cell's own planes opp m.
-template <class S>
-__device__ __forceinline__ void fold_patch(const Planes<S> &P, uint8_t mk, uint32_t idx,
-                                           const KArgs<typename Store<S>::C> &a,
-                                           typename Store<S>::C (&f)[Q]) {
-  using C = typename Store<S>::C;
-  uint32_t x, y, z;
-  coords(a.g, idx, x, y, z);
-  const FoldLane fl = fold_lane(a.g, x, y, z);
-  const uint32_t boff = idx * (uint32_t)sizeof(S);
-  C uwx = C(0), uwy = C(0), uwz = C(0);
-  if (mk & M_MOVING) {
-    uwx = a.uwx[idx];
-    uwy = a.uwy[idx];
-    uwz = a.uwz[idx];
-  }
-  for_dirs([&](auto M) {
-    constexpr int m = decltype(M)::value;
-    if constexpr (m > 0) {
-      if (folds<m>(fl)) {
-        const C ucw = C(CX[m]) * uwx + C(CY[m]) * uwy + C(CZ[m]) * uwz;
-        f[m] = Store<S>::ld(at(P.in[OPP[m]], boff)) + C(Store<S>::K * 6.0 * W[m]) * ucw;
       }
     }
   });
@@ -531,12 +509,11 @@ template <class S>
 __device__ __forceinline__ void pull_cell(const Planes<S> &P, uint8_t mk, uint32_t idx,
                                           const KArgs<typename Store<S>::C> &a,
                                           typename Store<S>::C (&f)[Q]) {
-  bulk_loads<S>(P, idx, f);
-  if (mk & patch_bits(a)) patch_cell<S>(P, mk, idx, a, f);
-  if (a.g.fold) {
-    // the k = 0 inflow write (M_BACKEQ) replaced every population
-    if (!(mk & M_BACKEQ)) fold_patch<S>(P, mk, idx, a, f);
-  }
+  // the periodic wrap and the shell fold in the addresses, as stream_cell
+  if (a.g.fold) folded_loads<S>(P, a.g, idx, f);
+  else bulk_loads_wrapped<S>(P, a.g, idx, f);
+  if (a.g.fold && (mk & M_MOVING)) fold_moving<S>(a, idx, f);
+  if (mk & (patch_bits(a) & ~M_WRAP)) patch_cell<S, true>(P, mk, idx, a, f);
 }
 
 // Box index of a cell inside the IBM box (mask bit M_BOX).
@@ -624,7 +601,8 @@ __device__ __forceinline__ void stream_cell(const Planes<S> &P, const uint8_t *_
   // the populations (periodic wrap and shell fold in the addresses), the
   // mask and, where some wall link is not folded, the bounce-back link word
   // are all in flight together
-  if (a.g.fold) folded_loads<S>(P, a.g, idx, f);
+  bool keep = false;  // on a keep face: rewrite the loaded own values
+  if (a.g.fold) keep = folded_loads<S>(P, a.g, idx, f);
   else bulk_loads_wrapped<S>(P, a.g, idx, f);
   uint32_t mk = mask[idx];
   uint32_t lk = 0;
@@ -641,6 +619,8 @@ __device__ __forceinline__ void stream_cell(const Planes<S> &P, const uint8_t *_
     if (a.g.fold && (mk & M_MOVING)) fold_moving<S>(a, idx, f);
     if (mk & (patch_bits(a) & ~M_WRAP)) patch_cell<S, true>(P, (uint8_t)mk, idx, a, f);
     update_cell<S, FORCE, COLLIDE, EMIT>(f, (uint8_t)mk, idx, a);
+  } else if (keep && (mk & M_OTHER) && wave_fluid) {
+    // an inlet/outlet cell of a keep face: its own (frozen-link) values back
   } else {
     st = !a.wall_slots ? 0u
          : a.slot_links ? lk
