@@ -518,6 +518,8 @@ def main():
     ap.add_argument("--no-fsi", action="store_true")
     ap.add_argument("--fsi-only", action="store_true", help="run only the heated-channel filament leg")
     ap.add_argument("--batch-steps", type=int, default=4000)
+    ap.add_argument("--batch-class-steps", type=int, default=1000,
+                    help="steps of each edge class's sequential and batched runs")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-only", action="store_true", help="run only the cavity ensemble leg")
     ap.add_argument("--copy-probe-only", action="store_true",
@@ -540,7 +542,7 @@ def main():
         print(json.dumps(cavity_leg(a.warmup, a.steps)), flush=True)
         return 0
     if a.batch_only:
-        print(json.dumps(batch_leg(a.batch_steps)), flush=True)
+        print(json.dumps(batch_leg(a.batch_steps, class_steps=a.batch_class_steps)), flush=True)
         return 0
     if a.fsi_only:
         print(json.dumps(fsi_thermal_leg(a.fsi_steps)), flush=True)
@@ -620,7 +622,7 @@ def main():
     batch = None
     if not a.no_batch:
         log(f"bench: cavity ensemble leg, {a.batch_steps} steps")
-        batch = batch_leg(a.batch_steps)
+        batch = batch_leg(a.batch_steps, class_steps=a.batch_class_steps)
 
     copy_gbs = lbm_amd.copy_probe(1 << 30, 20)
     log(f"bench: 1 GiB device copy {copy_gbs:.0f} GB/s")

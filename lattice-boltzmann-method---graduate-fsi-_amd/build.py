@@ -36,6 +36,7 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-p
 
 SOURCES = [
     os.path.join(HERE, "csrc", "lbm_kernels.hip"),
+    os.path.join(HERE, "csrc", "lbm_stream_fold.hip"),
     os.path.join(HERE, "csrc", "lbm_capi.hip"),
     os.path.join(HERE, "csrc", "lbm_probe.hip"),
     os.path.join(HERE, "csrc", "lbm_ibm.hip"),

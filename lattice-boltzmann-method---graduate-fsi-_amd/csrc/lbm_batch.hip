@@ -34,6 +34,7 @@ struct lbm_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int64_t nfluid = 0;
+  int fold = 0;  // some lattice has folded or kept faces: the FOLD kernel
 };
 
 namespace {
@@ -68,6 +69,22 @@ int build_tiles(lbm_batch *b) {
   return LBM_OK;
 }
 
+// The kernel body of the batch: with folded or kept faces in some lattice,
+// the FOLD body, which also serves lattices without them provided their
+// offsets fit its 32-bit lane offsets.
+int batch_layout(lbm_batch *b) {
+  int fold = 0;
+  for (lbm_ctx *c : b->ctx) fold |= c->g.fold != 0;
+  if (fold)
+    for (lbm_ctx *c : b->ctx)
+      if (c->g.fold == 0 && !fold_fits(c)) return LBM_ERR_UNSUPPORTED;
+  if (fold != b->fold) {
+    b->fold = fold;
+    drop_graph(b);
+  }
+  return LBM_OK;
+}
+
 BatchLaunch launch_of(const lbm_batch *b, int mode, int parity) {
   BatchLaunch B{};
   B.storage = b->storage;
@@ -86,6 +103,7 @@ BatchLaunch launch_of(const lbm_batch *b, int mode, int parity) {
   B.grid = (b->ntile + B.per - 1) / B.per;
   B.block = (unsigned)b->block;
   B.parity = parity;
+  B.fold = b->fold;
   B.stream = b->stream;
   return B;
 }
@@ -93,6 +111,7 @@ BatchLaunch launch_of(const lbm_batch *b, int mode, int parity) {
 // kernel arguments of every lattice: parity 0 reads the context's current
 // buffer, parity 1 the other one
 int upload_args(lbm_batch *b) {
+  if (int e = batch_layout(b)) return e;
   const size_t n = b->ctx.size();
   b->h_args.assign(2 * n * b->args_size, 0);
   for (size_t l = 0; l < n; l++) {
@@ -315,6 +334,10 @@ int lbm_batch_create(lbm_ctx *const *ctxs, int32_t n, lbm_batch **out) {
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->dev) == hipSuccess && ncu > 0)
     b->ncu = ncu;
   for (lbm_ctx *c : b->ctx) b->nfluid += c->nfluid;
+  if (int e = batch_layout(b)) {
+    lbm_batch_destroy(b);
+    return e;
+  }
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess ||
       hipMalloc(&b->d_args, 2 * (size_t)n * b->args_size) != hipSuccess) {

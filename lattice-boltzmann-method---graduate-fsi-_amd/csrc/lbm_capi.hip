@@ -141,7 +141,7 @@ static bool unfolded_wall_links(const lbm_ctx *c) {
 // The shell fold is used when the population offsets it forms fit 32 bits:
 // lane offset (idx + ps + plane + px + 1) elements plus a fold or wrap
 // correction of at most ps + plane + px + 1 or nx + plane + nz plane.
-static bool fold_fits(const lbm_ctx *c) {
+bool fold_fits(const lbm_ctx *c) {
   const Geo &g = c->g;
   const uint64_t bias = (uint64_t)g.ps + g.plane + g.px + 1;
   const uint64_t wrap = (uint64_t)g.nx + g.plane + (uint64_t)g.nz * g.plane;

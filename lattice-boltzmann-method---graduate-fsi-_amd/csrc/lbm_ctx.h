@@ -159,6 +159,8 @@ void output_free(lbm_ctx *c);
 // IBM run (F = 0, TYPE_IN/TYPE_OUT only, no range check).
 void faces_free(lbm_ctx *c);
 int faces_setup(lbm_ctx *c);
+// the folded pull's 32-bit lane offsets fit this lattice (lbm_capi.hip)
+bool fold_fits(const lbm_ctx *c);
 int faces_step(lbm_ctx *c, bool ibm, int macro);
 int faces_macro(lbm_ctx *c, bool ibm);
 hipError_t faces_f_out(lbm_ctx *c, double *out);

@@ -93,6 +93,7 @@ struct BatchLaunch {
   const uint32_t *tile_lat, *tile_first;
   uint32_t ntile, per;
   int parity;
+  int fold;  // some lattice has folded or kept faces (Geo::fold)
   unsigned grid, block;
   hipStream_t stream;
 };
@@ -101,6 +102,9 @@ void batch_fill_args(const LaunchCtx &L, const void *in, void *out, void *dst);
 hipError_t launch_stream_batch(const BatchLaunch &B);
 
 hipError_t launch_stream(const LaunchCtx &L, const void *fin, void *fout, int mode);
+// the same launches for lattices with folded or kept faces (lbm_stream_fold.hip)
+hipError_t launch_stream_fold(const LaunchCtx &L, const void *fin, void *fout, int mode);
+hipError_t launch_stream_batch_fold(const BatchLaunch &B);
 hipError_t launch_collide(const LaunchCtx &L, void *buf);
 hipError_t launch_macro(const LaunchCtx &L, const void *buf);
 hipError_t launch_init(const LaunchCtx &L, void *A, void *B);

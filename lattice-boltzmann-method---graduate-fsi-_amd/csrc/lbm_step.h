@@ -583,6 +583,23 @@ __device__ __forceinline__ void update_cell(typename Store<S>::C (&f)[Q], uint8_
   }
 }
 
+// The loads of stream_cell: the 19 populations (FOLD: folded_loads, else
+// the wrapped bulk loads), the mask and the link word, completed together.
+template <class S, bool FOLD>
+__device__ __forceinline__ uint32_t issue_loads(const Planes<S> &P, const uint8_t *__restrict__ mask,
+                                                const KArgs<typename Store<S>::C> &a, uint32_t idx,
+                                                typename Store<S>::C (&f)[Q], uint32_t &lk, bool &keep) {
+  if constexpr (FOLD) keep = folded_loads<S>(P, a.g, idx, f);
+  else bulk_loads_wrapped<S>(P, a.g, idx, f);
+  uint32_t mk = mask[idx];
+  lk = 0;
+  if (a.slot_links && a.wall_slots) {
+    lk = a.links[idx];
+    after_loads2(mk, lk);
+  }
+  return after_loads(mk, f);
+}
+
 // The whole hot-kernel body for one cell (see k_stream).
 //
 // Solid and row-padding cells whose 128-byte lines hold fluid cells join the
@@ -592,34 +609,27 @@ __device__ __forceinline__ void update_cell(typename Store<S>::C (&f)[Q], uint8_
 // solids and the padding.  Nothing reads these slots: with wall slots a
 // solid's populations are only bounce-back carriers.  Cells of the other
 // types keep their frozen-link values.
-template <class S, int FORCE, bool COLLIDE, bool EMIT>
+template <class S, int FORCE, bool COLLIDE, bool EMIT, bool FOLD>
 __device__ __forceinline__ void stream_cell(const Planes<S> &P, const uint8_t *__restrict__ mask,
                                             const KArgs<typename Store<S>::C> &a, uint32_t idx) {
   using C = typename Store<S>::C;
   constexpr uint32_t ALL = (1u << Q) - 1;
   C f[Q];
-  // the populations (periodic wrap and shell fold in the addresses), the
-  // mask and, where some wall link is not folded, the bounce-back link word
-  // are all in flight together
+  // the populations (periodic wrap and, with FOLD, the shell fold in the
+  // addresses), the mask and, where some wall link is not folded, the
+  // bounce-back link word are all in flight together
   bool keep = false;  // on a keep face: rewrite the loaded own values
-  if (a.g.fold) keep = folded_loads<S>(P, a.g, idx, f);
-  else bulk_loads_wrapped<S>(P, a.g, idx, f);
-  uint32_t mk = mask[idx];
-  uint32_t lk = 0;
-  if (a.slot_links && a.wall_slots) {
-    lk = a.links[idx];
-    after_loads2(mk, lk);
-  }
-  mk = after_loads(mk, f);
+  uint32_t lk;
+  uint32_t mk = issue_loads<S, FOLD>(P, mask, a, idx, f, lk, keep);
   // with no link word, solid and row-padding lanes write every slot (nothing
   // reads them) in waves that hold a fluid cell, so those lines leave whole
   const bool wave_fluid = __builtin_amdgcn_ballot_w64(mk & M_FLUID) != 0;
   uint32_t st = ALL;  // slots this lane stores
   if (mk & M_FLUID) {
-    if (a.g.fold && (mk & M_MOVING)) fold_moving<S>(a, idx, f);
+    if (FOLD && (mk & M_MOVING)) fold_moving<S>(a, idx, f);
     if (mk & (patch_bits(a) & ~M_WRAP)) patch_cell<S, true>(P, (uint8_t)mk, idx, a, f);
     update_cell<S, FORCE, COLLIDE, EMIT>(f, (uint8_t)mk, idx, a);
-  } else if (keep && (mk & M_OTHER) && wave_fluid) {
+  } else if (FOLD && keep && (mk & M_OTHER) && wave_fluid) {
     // an inlet/outlet cell of a keep face: its own (frozen-link) values back
   } else {
     st = !a.wall_slots ? 0u
