@@ -526,32 +526,39 @@ __global__ __launch_bounds__(64 * MK_WG) void k_ibm_interp_h(int nm, int R, cons
 // wavefront per marker: the lanes take the row's entries (two rounds of
 // loads: the row, then the entries' u_b and rho_k ds_k) and a fixed
 // butterfly sums them.
+// PM: per-marker desired velocities udv (a coupled filament's nodes): the
+// running fsum = -sum F_b takes the place of the drag/lift pairs.
+template <bool PM>
 __global__ __launch_bounds__(64 * MK_WG) void k_ibm_pass(int nm, int R, const double *__restrict__ rn,
                                                   const double *__restrict__ cds,
                                                   const double *__restrict__ ds, const int32_t *__restrict__ nbr,
                                                   const double *__restrict__ G, const double *__restrict__ ub_in,
                                                   double *__restrict__ ub_out, double u0, double u1, double u2,
-                                                  double *__restrict__ A, double *__restrict__ dsum,
-                                                  double *__restrict__ Fb, int first, int last) {
+                                                  const double *__restrict__ udv, double *__restrict__ A,
+                                                  double *__restrict__ dsum, double *__restrict__ Fb, int first,
+                                                  int last) {
   const int lane = threadIdx.x & 63;
   const int n = blockIdx.x * MK_WG + (threadIdx.x >> 6);
   if (n >= nm) return;  // wave-uniform
   // the marker's own values, loaded with the row
   const int l3 = lane < 3 ? lane : 0;
   const double ubn = ub_in[3 * n + l3], dsn = ds[n], rho = rn[n];
+  const double udn = PM ? udv[3 * n + l3] : (lane == 0 ? u0 : lane == 1 ? u1 : u2);
   double Ao = 0, Do = 0;
   if (!first && lane < 3) {
     Ao = A[3 * n + lane];
-    if (lane < 2) Do = dsum[2 * n + lane];
+    if (PM) Do = dsum[3 * n + lane];
+    else if (lane < 2) Do = dsum[2 * n + lane];
   }
   double s0 = 0, s1 = 0, s2 = 0;
   for (int r = lane; r < R; r += 64) {
     const size_t e = (size_t)n * R + r;
     const int k = nbr[e];
     const double h = G[e] * cds[k];
-    s0 += h * (u0 - ub_in[3 * k + 0]);
-    s1 += h * (u1 - ub_in[3 * k + 1]);
-    s2 += h * (u2 - ub_in[3 * k + 2]);
+    const double v0 = PM ? udv[3 * k + 0] : u0, v1 = PM ? udv[3 * k + 1] : u1, v2 = PM ? udv[3 * k + 2] : u2;
+    s0 += h * (v0 - ub_in[3 * k + 0]);
+    s1 += h * (v1 - ub_in[3 * k + 1]);
+    s2 += h * (v2 - ub_in[3 * k + 2]);
   }
   for (int off = 32; off > 0; off >>= 1) {
     s0 += __shfl_xor(s0, off);
@@ -559,13 +566,14 @@ __global__ __launch_bounds__(64 * MK_WG) void k_ibm_pass(int nm, int R, const do
     s2 += __shfl_xor(s2, off);
   }
   if (lane >= 3) return;
-  const double ud = lane == 0 ? u0 : lane == 1 ? u1 : u2;
   const double sum = lane == 0 ? s0 : lane == 1 ? s1 : s2;
   const double dt = 1.0, dx = 1.0, dy = 1.0;
-  const double F = 2 * rho * (ud - ubn) / dt;
+  const double F = 2 * rho * (udn - ubn) / dt;
   const double a = F * dsn;
   A[3 * n + lane] = first ? a : Ao + a;
-  if (lane < 2) {
+  if (PM) {
+    dsum[3 * n + lane] = first ? -F : Do - F;  // fsum
+  } else if (lane < 2) {
     const double dd = -F * (lane == 0 ? dx : dy) * dsn;
     dsum[2 * n + lane] = first ? dd : Do + dd;
   }
@@ -838,6 +846,101 @@ __global__ __launch_bounds__(256) void k_fsi_interp(FsiArgs a, BoxArrays o, int 
   }
 }
 
+// The filament's IBM passes in marker space (as the rigid body's, with the
+// nodes' own desired velocities): the first interpolation with the markers'
+// geometry (k_fsi_interp with first = 1, force = 0: the same stencils,
+// weights, density cells and u_b), rho_n and rho_n ds_n, and row n of G over
+// every marker k (entry r = k, R >= nm):
+//   G_nk = sum_x phi_n(x) phi_k(x)/rho(x)
+// over the live box cells both stencils reach, 0 where they share none.  The
+// nodes move every step, so each wave takes marker k's stencil base and
+// weights from its position, with k_fsi_interp's expressions.
+__global__ __launch_bounds__(64 * MK_WG) void k_fsi_interp_h(FsiArgs a, BoxArrays o, int R, double *__restrict__ G,
+                                                          double *__restrict__ rn, double *__restrict__ cds) {
+  __shared__ double term[MK_WG][3][64];
+  __shared__ double qn[MK_WG][64];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int n = blockIdx.x * MK_WG + wv;
+  const bool valid = n < a.nm;
+  double p0 = 0, p1 = 0, p2 = 0, xpl = 0;
+  if (valid) {
+    p0 = a.x[3 * n + 0];
+    p1 = a.x[3 * n + 1];
+    p2 = a.x[3 * n + 2];
+    if (lane < 3) xpl = a.xp[3 * n + lane];
+  }
+  const int b0 = (int)floor(p0) - 1, b1 = (int)floor(p1) - 1, b2 = (int)floor(p2) - 1;
+  const int pa = lane >> 4, pb = (lane >> 2) & 3, pc = lane & 3;
+  const int b = valid ? box_cell(a.B, b0 + pa, b1 + pb, b2 + pc) : -1;
+  const int nb = box_cell(a.B, (int)floor(p0 + 0.5), (int)floor(p1 + 0.5), (int)floor(p2 + 0.5));
+  double t0 = 0, t1 = 0, t2 = 0, q = 0;
+  if (b >= 0) {
+    const bool lv = o.live[b] != 0;
+    const double vx = o.ux[b], vy = o.uy[b], vz = o.uz[b], rb = o.rho[b];
+    const double kx = peskin(((b0 + pa) * 1.0 - p0) / 1.0), ky = peskin(((b1 + pb) * 1.0 - p1) / 1.0),
+                 kz = peskin(((b2 + pc) * 1.0 - p2) / 1.0);
+    if (lv) {
+      t0 = vx * kx * ky * kz;
+      t1 = vy * kx * ky * kz;
+      t2 = vz * kx * ky * kz;
+      q = kx * ky * kz / rb;
+    }
+  }
+  term[wv][0][lane] = t0;
+  term[wv][1][lane] = t1;
+  term[wv][2][lane] = t2;
+  qn[wv][lane] = q;
+  __syncthreads();
+  if (!valid) return;
+  const double pl = lane == 0 ? p0 : lane == 1 ? p1 : p2;
+  if (lane < 3) {
+    double acc = 0.0;
+    for (int s = 0; s < 64; s++) acc += term[wv][lane][s];
+    a.ub[3 * n + lane] = acc;
+    (lane == 0 ? a.mx : lane == 1 ? a.my : a.mz)[n] = pl;
+    a.ud[3 * n + lane] = pl - xpl;
+    a.st.base[3 * n + lane] = lane == 0 ? b0 : lane == 1 ? b1 : b2;
+  }
+  if (lane < 12) {
+    const int ax = lane >> 2, qq = lane & 3;
+    const int bb = ax == 0 ? b0 : ax == 1 ? b1 : b2;
+    const double p = ax == 0 ? p0 : ax == 1 ? p1 : p2;
+    a.st.w[12 * n + lane] = peskin(((bb + qq) * 1.0 - p) / 1.0);
+  }
+  if (lane == 0) a.near[n] = nb;
+  if (lane == 3) {
+    const double r = nb >= 0 ? o.rho[nb] : nan("");
+    rn[n] = r;
+    cds[n] = r * a.ds[n];
+  }
+  for (int r = lane; r < a.nm; r += 64) {
+    const double x0 = a.x[3 * r + 0], x1 = a.x[3 * r + 1], x2 = a.x[3 * r + 2];
+    const int c0 = (int)floor(x0) - 1, c1 = (int)floor(x1) - 1, c2 = (int)floor(x2) - 1;
+    const int dx = c0 - b0, dy = c1 - b1, dz = c2 - b2;
+    double acc = 0.0;
+    if (dx > -4 && dx < 4 && dy > -4 && dy < 4 && dz > -4 && dz < 4) {
+      // k's weights on n's stencil points (0 where k's stencil does not reach)
+      double px[4], py[4], pz[4];
+#pragma unroll
+      for (int s = 0; s < 4; s++) {
+        const int ax = s - dx, ay = s - dy, az = s - dz;
+        px[s] = (ax >= 0 && ax < 4) ? peskin(((c0 + ax) * 1.0 - x0) / 1.0) : 0.0;
+        py[s] = (ay >= 0 && ay < 4) ? peskin(((c1 + ay) * 1.0 - x1) / 1.0) : 0.0;
+        pz[s] = (az >= 0 && az < 4) ? peskin(((c2 + az) * 1.0 - x2) / 1.0) : 0.0;
+      }
+#pragma unroll
+      for (int s = 0; s < 4; s++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+          const double pac = px[s] * py[c];
+#pragma unroll
+          for (int z = 0; z < 4; z++) acc += qn[wv][(s * 4 + c) * 4 + z] * (pac * pz[z]);
+        }
+    }
+    G[(size_t)n * R + r] = acc;
+  }
+}
+
 // Spreading and velocity correction for a filament (at most FSI_MAX
 // markers): one thread per box cell gathers F_b ds phi phi phi over the
 // markers whose stencil holds the cell, in marker order (the reference's
@@ -1052,6 +1155,9 @@ struct lbm_ibm {
   double *H = nullptr, *ub2 = nullptr;  // H: G_nk (times rho_k ds_k in the passes)
   double *rn = nullptr, *cds = nullptr;  // density point value rho_n per marker, rho_n ds_n
   int R = 0;
+  // a coupled filament's passes in marker space: R >= nm entries per row,
+  // entry r = marker r (the nodes move, so G is recomputed every step)
+  int fsi_rows = 0;
   double *cell_pairs = nullptr;  // explicit: per-block (drag, lift) of the box cells
   size_t pairs_cap = 0;
 };
@@ -1371,9 +1477,9 @@ static int ibm_iterate(lbm_ctx *c, const double vcm[3], const double vb[3], doub
     hipLaunchKernelGGL(k_ibm_interp_h, gm, dim3(64 * MK_WG), 0, st, m->nm, m->R, m->pidx, m->st, o, m->near, m->ds,
                        m->nlen, m->nbr, m->H, m->rn, m->cds, buf[s0]);
     for (int it = 1; it <= m->m_max; it++)
-      hipLaunchKernelGGL(k_ibm_pass, gm, dim3(64 * MK_WG), 0, st, m->nm, m->R, m->rn, m->cds, m->ds, m->nbr, m->H,
-                         buf[(it - 1 + s0) % 2], buf[(it + s0) % 2], vcm[0], vcm[1], vcm[2], m->Fds, m->dsum, m->Fb,
-                         it == 1 ? 1 : 0, it == m->m_max ? 1 : 0);
+      hipLaunchKernelGGL(k_ibm_pass<false>, gm, dim3(64 * MK_WG), 0, st, m->nm, m->R, m->rn, m->cds, m->ds, m->nbr,
+                         m->H, buf[(it - 1 + s0) % 2], buf[(it + s0) % 2], vcm[0], vcm[1], vcm[2], nullptr, m->Fds,
+                         m->dsum, m->Fb, it == 1 ? 1 : 0, it == m->m_max ? 1 : 0);
     if (m->m_max > 0 && m->nchunk > 0) {
       // the drag/lift sums as one more workgroup of the spreading launch
       FinArgs f = fin;
@@ -1412,6 +1518,23 @@ static int fsi_iterate(lbm_ctx *c) {
   const FsiArgs a{m->B, n, fx, fxp, m->ds, m->live, m->ub, m->Fb, m->Fds, m->fsum,
                   m->mx, m->my, m->mz, m->ud, m->st, m->near};
   const BoxArrays o = box_arrays(m);
+  if (m->fsi_rows && m->m_max > 0) {
+    // the passes in marker space, then one spreading of their summed force
+    // (u_b buffers alternate so that the last pass writes m->ub)
+    double *buf[2] = {m->ub, m->ub2};
+    const int s0 = m->m_max % 2;
+    FsiArgs a0 = a;
+    a0.ub = buf[s0];
+    const dim3 gm((unsigned)((n + MK_WG - 1) / MK_WG));
+    hipLaunchKernelGGL(k_fsi_interp_h, gm, dim3(64 * MK_WG), 0, c->stream, a0, o, m->R, m->H, m->rn, m->cds);
+    for (int it = 1; it <= m->m_max; it++)
+      hipLaunchKernelGGL(k_ibm_pass<true>, gm, dim3(64 * MK_WG), 0, c->stream, n, m->R, m->rn, m->cds, m->ds, m->nbr,
+                         m->H, buf[(it - 1 + s0) % 2], buf[(it + s0) % 2], 0.0, 0.0, 0.0, m->ud, m->Fds, m->fsum,
+                         m->Fb, it == 1 ? 1 : 0, it == m->m_max ? 1 : 0);
+    hipLaunchKernelGGL(k_fsi_spread, fsi_spread_grid(m->B), dim3(256), 0, c->stream, a, o);
+    HIPCHK(hipGetLastError());
+    return LBM_OK;
+  }
   const dim3 gi((unsigned)((n + 3) / 4));
   hipLaunchKernelGGL(k_fsi_interp, gi, dim3(256), 0, c->stream, a, o, m->m_max > 0 ? 1 : 0, 1);
   for (int it = 1; it <= m->m_max; it++) {
@@ -1757,6 +1880,24 @@ int lbm_fsi_setup(lbm_ctx *c, lbm_filament *f, const lbm_ibm_desc *d) {
   m->fsum = (double *)dalloc(m, 24 * (size_t)n);
   if (!m->ud || !m->fsum) return LBM_ERR_NOMEM;
   HIPCHK(hipMemsetAsync(m->fsum, 0, 24 * (size_t)n, c->stream));
+  {
+    // marker-space rows over all nodes: row n, entry r = node r; the padding
+    // entries name the node itself with G = 0
+    const int R = (n + 63) / 64 * 64;
+    std::vector<int32_t> idx((size_t)n * R);
+    for (int i = 0; i < n; i++)
+      for (int r = 0; r < R; r++) idx[(size_t)i * R + r] = r < n ? r : i;
+    m->R = R;
+    m->nbr = (int32_t *)dalloc(m, 4 * idx.size());
+    m->H = (double *)dalloc(m, 8 * idx.size());
+    if (!m->ub2) m->ub2 = (double *)dalloc(m, 24 * (size_t)n);
+    if (!m->rn) m->rn = (double *)dalloc(m, 8 * (size_t)n);
+    if (!m->cds) m->cds = (double *)dalloc(m, 8 * (size_t)n);
+    if (!m->nbr || !m->H || !m->ub2 || !m->rn || !m->cds) return LBM_ERR_NOMEM;
+    HIPCHK(hipMemcpy(m->nbr, idx.data(), 4 * idx.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(m->H, 0, 8 * idx.size()));
+    m->fsi_rows = 1;
+  }
   hipLaunchKernelGGL(k_fsi_markers, blocks(n), dim3(256), 0, c->stream, n, fx, fxp, m->mx, m->my, m->mz, m->ud);
   HIPCHK(hipGetLastError());
   return ibm_geometry(c);
