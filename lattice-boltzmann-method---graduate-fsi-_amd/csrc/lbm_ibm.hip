@@ -624,6 +624,240 @@ static void launch_passes(hipStream_t st, int nm, int R, int m_max, const double
   }
 }
 
+// ---- paired passes: the implicit iteration with the squared operator -------
+// With e = u_des - u_b and C = I - H, pass m maps e to C e, so after an
+// optional single pass (m_max odd) the remaining 2q passes are q steps of
+// Q = C^2 = I - 2H + H^2 (exact in exact arithmetic; the sums round
+// differently, within a few ulps):
+//   t_0 = e, t_{i+1} = Q t_i, W = t_0 + ... + t_{q-1},
+//   sum of the 2q passes' e = W + C W,  e before the last pass = C t_{q-1},
+//   e after it = Q t_{q-1}.
+// The forces follow as in the single passes (F_b = 2 rho_n e per pass,
+// A = sum F_b ds, drag/lift -F_b ds, the last F_b kept, u_b = u_des - e),
+// with the q - 1 products by Q and one closing launch instead of 2q
+// dependent launches.  Rows of Q cover the markers that share a stencil
+// cell with a marker that shares one with n (sorted, R2 entries).
+
+// Q = I - 2H + H^2 per step (H_nk = G_nk rho_k ds_k changes with the
+// cells' rho): one workgroup per row, SQ_WAVES wavefronts taking every
+// SQ_WAVES-th entry j of row n and adding (G_nj cds_j) G_jk over row j into
+// their own window of marker indices [base_n, base_n + SQ_WIN) in LDS, in
+// entry order (LDS atomic adds: a wavefront's adds to one word land in
+// program order, and no two lanes of one add share a word); wavefront 0
+// first adds -2 G_nk.  Then one thread per entry of Q's row: Q_nk = [k == n]
+// + cds_k (sum of the windows at k).  Entries with G = 0 (the rows'
+// padding among them) add nothing.  Measured on 4096 markers with 72-entry
+// rows: 32.5 us, bound by the 21 M scattered adds (staging each workgroup's
+// union of rows in LDS and giving a wavefront a whole row took 57 us).
+constexpr int SQ_WIN = 1024;
+constexpr int SQ_WAVES = 4;
+constexpr int SQ_UNROLL = 9;  // rows j loaded together per wavefront
+__global__ __launch_bounds__(64 * SQ_WAVES) void k_ibm_square(int R, const int32_t *__restrict__ nbr,
+                                                              const double *__restrict__ G,
+                                                              const double *__restrict__ cds, int R2,
+                                                              const int32_t *__restrict__ nbr2,
+                                                              const int32_t *__restrict__ len2,
+                                                              const int32_t *__restrict__ base2,
+                                                              double *__restrict__ Q) {
+  __shared__ double win[SQ_WAVES][SQ_WIN];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int n = blockIdx.x;
+  const int base = base2[n];
+  // row n: lane l holds entries l and l + 64 (R <= 128)
+  const size_t rn = (size_t)n * R;
+  const int ja = lane < R ? nbr[rn + lane] : n, jb = lane + 64 < R ? nbr[rn + lane + 64] : n;
+  const double ga = lane < R ? G[rn + lane] : 0.0, gb = lane + 64 < R ? G[rn + lane + 64] : 0.0;
+  const double ca = ga * cds[ja], cb = gb * cds[jb];
+  for (int i = lane; i < SQ_WIN; i += 64) win[wv][i] = 0.0;
+  __syncthreads();
+  if (wv == 0) {
+    if (ga != 0.0) atomicAdd(&win[0][ja - base], -2.0 * ga);
+    if (gb != 0.0) atomicAdd(&win[0][jb - base], -2.0 * gb);
+  }
+  for (int t0 = wv; t0 < R; t0 += SQ_WAVES * SQ_UNROLL) {
+    int ka[SQ_UNROLL], kb[SQ_UNROLL];
+    double gka[SQ_UNROLL], gkb[SQ_UNROLL], c[SQ_UNROLL];
+#pragma unroll
+    for (int u = 0; u < SQ_UNROLL; u++) {
+      const int t = t0 + SQ_WAVES * u;
+      const int tl = t & 63;
+      const int j = t < 64 ? __shfl(ja, tl) : __shfl(jb, tl);
+      const double cj = t < 64 ? __shfl(ca, tl) : __shfl(cb, tl);
+      c[u] = t < R ? cj : 0.0;
+      const size_t rj = (size_t)j * R;
+      ka[u] = lane < R ? nbr[rj + lane] : j;
+      kb[u] = lane + 64 < R ? nbr[rj + lane + 64] : j;
+      gka[u] = lane < R ? G[rj + lane] : 0.0;
+      gkb[u] = lane + 64 < R ? G[rj + lane + 64] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < SQ_UNROLL; u++) {
+      if (c[u] == 0.0) continue;  // uniform over the wavefront
+      if (gka[u] != 0.0) atomicAdd(&win[wv][ka[u] - base], c[u] * gka[u]);
+      if (gkb[u] != 0.0) atomicAdd(&win[wv][kb[u] - base], c[u] * gkb[u]);
+    }
+  }
+  __syncthreads();
+  const int p = threadIdx.x;
+  if (p < R2) {
+    const size_t e = (size_t)n * R2 + p;
+    const int k = nbr2[e];
+    double q = 0.0;
+    if (p < len2[n]) {
+      const int w = k - base;
+      const double v = (win[0][w] + win[1][w]) + (win[2][w] + win[3][w]);
+      q = (k == n ? 1.0 : 0.0) + cds[k] * v;
+    }
+    Q[e] = q;
+  }
+}
+
+// t_out = Q t_in and W += t_in, one wavefront per marker.  UB: t_in is
+// u_des - u_b of the u_b array `tin` (the first product).
+template <bool UB>
+__global__ __launch_bounds__(64 * MK_WG) void k_ibm_qpass(int nm, int R2, const int32_t *__restrict__ nbr2,
+                                                   const double *__restrict__ Q, const double *__restrict__ tin,
+                                                   double *__restrict__ tout, double *__restrict__ W, double u0,
+                                                   double u1, double u2, int first) {
+  const int lane = threadIdx.x & 63;
+  const int n = blockIdx.x * MK_WG + (threadIdx.x >> 6);
+  if (n >= nm) return;  // wave-uniform
+  const int l3 = lane < 3 ? lane : 0;
+  const double ul = lane == 0 ? u0 : lane == 1 ? u1 : u2;
+  const double tn = UB ? ul - tin[3 * n + l3] : tin[3 * n + l3];
+  const double wo = first ? 0.0 : W[3 * n + l3];
+  double s0 = 0, s1 = 0, s2 = 0;
+  const size_t row = (size_t)n * R2;
+  for (int r0 = 0; r0 < R2; r0 += 4 * 64) {
+    int k[4];
+    double q[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int r = r0 + 64 * u + lane;
+      k[u] = r < R2 ? nbr2[row + r] : n;
+      q[u] = r < R2 ? Q[row + r] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int kk = k[u];
+      const double a0 = tin[3 * kk + 0], a1 = tin[3 * kk + 1], a2 = tin[3 * kk + 2];
+      s0 += q[u] * (UB ? u0 - a0 : a0);
+      s1 += q[u] * (UB ? u1 - a1 : a1);
+      s2 += q[u] * (UB ? u2 - a2 : a2);
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    s0 += __shfl_xor(s0, off);
+    s1 += __shfl_xor(s1, off);
+    s2 += __shfl_xor(s2, off);
+  }
+  if (lane >= 3) return;
+  tout[3 * n + lane] = lane == 0 ? s0 : lane == 1 ? s1 : s2;
+  W[3 * n + lane] = first ? tn : wo + tn;
+}
+
+// The closing launch of the paired passes, from t = t_{q-1} (UB: from
+// u_des - u_b of the u_b array `t`) and W = t_0 + ... + t_{q-2} (HW = 0:
+// none): s = 2 (W + t) - H (W + t), e_last = t - H t, e_after = Q t; then
+// A += 2 rho_n s ds_n, the drag/lift pairs -2 rho_n s ds_n, F_b = 2 rho_n
+// e_last, u_b = u_des - e_after.  `first`: A and the pairs start here.
+template <bool UB>
+__global__ __launch_bounds__(64 * MK_WG) void k_ibm_qfinal(int nm, int R, const int32_t *__restrict__ nbr,
+                                                    const double *__restrict__ G, const double *__restrict__ cds,
+                                                    int R2, const int32_t *__restrict__ nbr2,
+                                                    const double *__restrict__ Q, const double *__restrict__ t,
+                                                    const double *__restrict__ W, int HW, const double *__restrict__ rn,
+                                                    const double *__restrict__ ds, double u0, double u1, double u2,
+                                                    double *__restrict__ A, double *__restrict__ dsum,
+                                                    double *__restrict__ Fb, double *__restrict__ ub, int first) {
+  const int lane = threadIdx.x & 63;
+  const int n = blockIdx.x * MK_WG + (threadIdx.x >> 6);
+  if (n >= nm) return;  // wave-uniform
+  const int l3 = lane < 3 ? lane : 0;
+  const double ul = lane == 0 ? u0 : lane == 1 ? u1 : u2;
+  const double tn = UB ? ul - t[3 * n + l3] : t[3 * n + l3];
+  const double Wn = (HW ? W[3 * n + l3] : 0.0) + tn;
+  const double dsn = ds[n], rho = rn[n];
+  double Ao = 0, Do = 0;
+  if (!first && lane < 3) {
+    Ao = A[3 * n + lane];
+    if (lane < 2) Do = dsum[2 * n + lane];
+  }
+  // H (W + t) and H t over row n of H
+  double h0 = 0, h1 = 0, h2 = 0, g0 = 0, g1 = 0, g2 = 0;
+  for (int r0 = 0; r0 < R; r0 += 2 * 64) {
+    int k[2];
+    double gv[2];
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const int r = r0 + 64 * u + lane;
+      k[u] = r < R ? nbr[(size_t)n * R + r] : n;
+      gv[u] = r < R ? G[(size_t)n * R + r] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const int kk = k[u];
+      const double h = gv[u] * cds[kk];
+      const double a0 = t[3 * kk + 0], a1 = t[3 * kk + 1], a2 = t[3 * kk + 2];
+      const double t0 = UB ? u0 - a0 : a0, t1 = UB ? u1 - a1 : a1, t2 = UB ? u2 - a2 : a2;
+      const double w0 = (HW ? W[3 * kk + 0] : 0.0) + t0, w1 = (HW ? W[3 * kk + 1] : 0.0) + t1,
+                   w2 = (HW ? W[3 * kk + 2] : 0.0) + t2;
+      h0 += h * w0;
+      h1 += h * w1;
+      h2 += h * w2;
+      g0 += h * t0;
+      g1 += h * t1;
+      g2 += h * t2;
+    }
+  }
+  // Q t over row n of Q
+  double q0 = 0, q1 = 0, q2 = 0;
+  for (int r0 = 0; r0 < R2; r0 += 4 * 64) {
+    int k[4];
+    double qv[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int r = r0 + 64 * u + lane;
+      k[u] = r < R2 ? nbr2[(size_t)n * R2 + r] : n;
+      qv[u] = r < R2 ? Q[(size_t)n * R2 + r] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int kk = k[u];
+      const double a0 = t[3 * kk + 0], a1 = t[3 * kk + 1], a2 = t[3 * kk + 2];
+      q0 += qv[u] * (UB ? u0 - a0 : a0);
+      q1 += qv[u] * (UB ? u1 - a1 : a1);
+      q2 += qv[u] * (UB ? u2 - a2 : a2);
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    h0 += __shfl_xor(h0, off);
+    h1 += __shfl_xor(h1, off);
+    h2 += __shfl_xor(h2, off);
+    g0 += __shfl_xor(g0, off);
+    g1 += __shfl_xor(g1, off);
+    g2 += __shfl_xor(g2, off);
+    q0 += __shfl_xor(q0, off);
+    q1 += __shfl_xor(q1, off);
+    q2 += __shfl_xor(q2, off);
+  }
+  if (lane >= 3) return;
+  const double hw = lane == 0 ? h0 : lane == 1 ? h1 : h2;
+  const double ht = lane == 0 ? g0 : lane == 1 ? g1 : g2;
+  const double qt = lane == 0 ? q0 : lane == 1 ? q1 : q2;
+  const double dt = 1.0, dx = 1.0, dy = 1.0;
+  const double sm = 2.0 * Wn - hw;  // the 2q passes' summed e
+  const double F = 2 * rho * sm / dt;
+  const double a = F * dsn;
+  A[3 * n + lane] = first ? a : Ao + a;
+  if (lane < 2) {
+    const double dd = -F * (lane == 0 ? dx : dy) * dsn;
+    dsum[2 * n + lane] = first ? dd : Do + dd;
+  }
+  Fb[3 * n + lane] = 2 * rho * (tn - ht) / dt;
+  ub[3 * n + lane] = ul - qt;
+}
+
 template <int T = FIN_THREADS>
 __device__ __forceinline__ double fin_sum(double v, double *sh) {
   sh[threadIdx.x] = v;
@@ -1198,6 +1432,12 @@ struct lbm_ibm {
   double *H = nullptr, *ub2 = nullptr;  // H: G_nk (times rho_k ds_k in the passes)
   double *rn = nullptr, *cds = nullptr;  // density point value rho_n per marker, rho_n ds_n
   int R = 0;
+  // paired passes (m_max >= 2): rows of Q = (I - H)^2, R2 entries each
+  // (len2 real, the rest padding), the LDS window base of each row, Q per
+  // step, the products' t buffers, W and the u_b of a leading single pass
+  int R2 = 0;
+  int32_t *nbr2 = nullptr, *len2 = nullptr, *base2 = nullptr;
+  double *Q2 = nullptr, *sq_t[2] = {nullptr, nullptr}, *sq_w = nullptr, *sq_ub = nullptr;
   // a coupled filament's passes in marker space: R >= nm entries per row,
   // entry r = marker r (the nodes move, so G is recomputed every step)
   int fsi_rows = 0;
@@ -1517,10 +1757,43 @@ static int ibm_iterate(lbm_ctx *c, const double vcm[3], const double vb[3], doub
     double *buf[2] = {m->ub, m->ub2};
     const int s0 = m->m_max % 2;
     const dim3 gm((unsigned)((m->nm + MK_WG - 1) / MK_WG));
-    hipLaunchKernelGGL(k_ibm_interp_h, gm, dim3(64 * MK_WG), 0, st, m->nm, m->R, m->pidx, m->st, o, m->near, m->ds,
-                       m->nlen, m->nbr, m->H, m->rn, m->cds, buf[s0]);
-    launch_passes<false>(st, m->nm, m->R, m->m_max, m->rn, m->cds, m->ds, m->nbr, m->H, buf, vcm, nullptr, m->Fds,
-                         m->dsum, m->Fb);
+    if (m->nbr2 && m->m_max >= 2) {
+      // paired passes: the first interpolation into ub2, Q's rows, a single
+      // pass when m_max is odd, q - 1 products by Q, the closing launch
+      hipLaunchKernelGGL(k_ibm_interp_h, gm, dim3(64 * MK_WG), 0, st, m->nm, m->R, m->pidx, m->st, o, m->near,
+                         m->ds, m->nlen, m->nbr, m->H, m->rn, m->cds, m->ub2);
+      hipLaunchKernelGGL(k_ibm_square, dim3((unsigned)m->nm), dim3(64 * SQ_WAVES), 0, st, m->R, m->nbr, m->H, m->cds,
+                         m->R2, m->nbr2, m->len2, m->base2, m->Q2);
+      const int lead = m->m_max % 2, q = (m->m_max - lead) / 2;
+      double *src = m->ub2;
+      if (lead) {
+        hipLaunchKernelGGL((k_ibm_pass<false, 64>), gm, dim3(64 * MK_WG), 0, st, m->nm, m->R, m->rn, m->cds, m->ds,
+                           m->nbr, m->H, m->ub2, m->sq_ub, vcm[0], vcm[1], vcm[2], nullptr, m->Fds, m->dsum, m->Fb,
+                           1, 0);
+        src = m->sq_ub;
+      }
+      for (int i = 0; i + 1 < q; i++) {
+        if (i == 0)
+          hipLaunchKernelGGL(k_ibm_qpass<true>, gm, dim3(64 * MK_WG), 0, st, m->nm, m->R2, m->nbr2, m->Q2, src,
+                             m->sq_t[0], m->sq_w, vcm[0], vcm[1], vcm[2], 1);
+        else
+          hipLaunchKernelGGL(k_ibm_qpass<false>, gm, dim3(64 * MK_WG), 0, st, m->nm, m->R2, m->nbr2, m->Q2,
+                             m->sq_t[(i - 1) % 2], m->sq_t[i % 2], m->sq_w, vcm[0], vcm[1], vcm[2], 0);
+      }
+      if (q == 1)
+        hipLaunchKernelGGL(k_ibm_qfinal<true>, gm, dim3(64 * MK_WG), 0, st, m->nm, m->R, m->nbr, m->H, m->cds, m->R2,
+                           m->nbr2, m->Q2, src, m->sq_w, 0, m->rn, m->ds, vcm[0], vcm[1], vcm[2], m->Fds, m->dsum,
+                           m->Fb, m->ub, lead ? 0 : 1);
+      else
+        hipLaunchKernelGGL(k_ibm_qfinal<false>, gm, dim3(64 * MK_WG), 0, st, m->nm, m->R, m->nbr, m->H, m->cds,
+                           m->R2, m->nbr2, m->Q2, m->sq_t[(q - 2) % 2], m->sq_w, 1, m->rn, m->ds, vcm[0], vcm[1],
+                           vcm[2], m->Fds, m->dsum, m->Fb, m->ub, lead ? 0 : 1);
+    } else {
+      hipLaunchKernelGGL(k_ibm_interp_h, gm, dim3(64 * MK_WG), 0, st, m->nm, m->R, m->pidx, m->st, o, m->near,
+                         m->ds, m->nlen, m->nbr, m->H, m->rn, m->cds, buf[s0]);
+      launch_passes<false>(st, m->nm, m->R, m->m_max, m->rn, m->cds, m->ds, m->nbr, m->H, buf, vcm, nullptr,
+                           m->Fds, m->dsum, m->Fb);
+    }
     if (m->m_max > 0 && m->nchunk > 0) {
       // the drag/lift sums as one more workgroup of the spreading launch
       FinArgs f = fin;
@@ -1614,6 +1887,54 @@ static int ibm_move(lbm_ctx *c, const double vcm[3], const double vb[3]) {
   return LBM_OK;
 }
 
+// The rows of Q = (I - H)^2 for the paired passes: for each marker the
+// union of its neighbours' rows, sorted.  Kept only when every row spans
+// fewer than SQ_WIN marker indices (the square kernel's LDS window) and has
+// at most 64 SQ_WAVES entries (one thread per entry); otherwise the single
+// passes run alone.
+static int build_square(lbm_ibm *m, const std::vector<std::vector<int32_t>> &rows) {
+  const int nm = m->nm;
+  std::vector<int32_t> stamp(nm, -1), base(nm), len(nm);
+  std::vector<std::vector<int32_t>> r2(nm);
+  size_t maxlen = 0;
+  for (int n = 0; n < nm; n++) {
+    std::vector<int32_t> &row = r2[n];
+    for (int32_t j : rows[n])
+      for (int32_t k : rows[j])
+        if (stamp[k] != n) {
+          stamp[k] = n;
+          row.push_back(k);
+        }
+    std::sort(row.begin(), row.end());
+    if (row.back() - row.front() >= SQ_WIN) return LBM_OK;
+    base[n] = row.front();
+    len[n] = (int32_t)row.size();
+    maxlen = std::max(maxlen, row.size());
+  }
+  const int R2 = (int)((maxlen + 7) / 8 * 8);
+  if (R2 > 64 * SQ_WAVES) return LBM_OK;
+  std::vector<int32_t> idx((size_t)nm * R2);
+  for (int n = 0; n < nm; n++)
+    for (int r = 0; r < R2; r++) idx[(size_t)n * R2 + r] = r < len[n] ? r2[n][r] : n;
+  m->nbr2 = (int32_t *)dalloc(m, 4 * idx.size());
+  m->len2 = (int32_t *)dalloc(m, 4 * (size_t)nm);
+  m->base2 = (int32_t *)dalloc(m, 4 * (size_t)nm);
+  m->Q2 = (double *)dalloc(m, 8 * idx.size());
+  m->sq_t[0] = (double *)dalloc(m, 24 * (size_t)nm);
+  m->sq_t[1] = (double *)dalloc(m, 24 * (size_t)nm);
+  m->sq_w = (double *)dalloc(m, 24 * (size_t)nm);
+  m->sq_ub = (double *)dalloc(m, 24 * (size_t)nm);
+  if (!m->nbr2 || !m->len2 || !m->base2 || !m->Q2 || !m->sq_t[0] || !m->sq_t[1] || !m->sq_w || !m->sq_ub) {
+    m->nbr2 = nullptr;
+    return LBM_ERR_NOMEM;
+  }
+  HIPCHK(hipMemcpy(m->nbr2, idx.data(), 4 * idx.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(m->len2, len.data(), 4 * (size_t)nm, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(m->base2, base.data(), 4 * (size_t)nm, hipMemcpyHostToDevice));
+  m->R2 = R2;
+  return LBM_OK;
+}
+
 // The rows of H for the marker-space passes: for each marker the markers
 // closer than 4 cells on every axis (itself included), sorted.  Stencils
 // (floor(x) - 1 .. floor(x) + 2 per axis) of markers 4 or more cells apart
@@ -1651,6 +1972,10 @@ static int build_neighbours(lbm_ibm *m, const double *x, const double *y, const 
     if (total > (size_t)256 * nm) return LBM_OK;  // too dense: keep the pass-by-pass path
   }
   const int R = (int)((maxlen + 7) / 8 * 8);
+  if (m->m_max >= 2 && R <= 128) {
+    int e = build_square(m, rows);
+    if (e != LBM_OK) return e;
+  }
   std::vector<int32_t> len(nm), idx((size_t)nm * R);
   for (int n = 0; n < nm; n++) {
     len[n] = (int32_t)rows[n].size();

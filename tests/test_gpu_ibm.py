@@ -137,6 +137,36 @@ def test_ibm_fused_run_fp64(oracle_mod, name):
     lat.close()
 
 
+@pytest.mark.parametrize("m_max", [1, 2, 3, 4, 7])
+def test_ibm_iteration_counts_fp64(oracle_mod, m_max):
+    """The m_max passes run as a single pass (m_max odd) plus products by
+    (I - H)^2 and a closing launch: every branch of that schedule (no
+    product, one, several; with and without the single pass) against the
+    oracle's m_max spread/interpolate passes, fused and step-by-step."""
+    case, mk, box = IBM_CASES["span"]()
+    nsteps = 6
+    o = _oracle(oracle_mod, case)
+    omk = {k: v.copy() for k, v in mk.items()}
+    ref = _oracle_run(o, omk, box, np.zeros((nsteps + 1, 3)), m_max=m_max)
+    lat = _setup(case, mk, box, m_max=m_max)
+    got = lat.ibm_run(nsteps)
+    assert lat.sync() == 0
+    _cmp_series(got, ref, 1e-11, f"m_max={m_max}")
+    _cmp_fields(lat, o, case, 1e-11, 1e-12)
+    _cmp_markers(lat, omk, 1e-10)
+    lat.close()
+    lat = _setup(case, mk, box, m_max=m_max)
+    got = []
+    for s in range(nsteps):
+        got.append(lat.ibm_apply(np.zeros(3), np.zeros(3)))
+        lat.collision()
+        lat.streaming()
+    assert lat.sync() == 0
+    _cmp_series(np.array(got), ref, 1e-11, f"m_max={m_max} apply")
+    _cmp_fields(lat, o, case, 1e-11, 1e-12)
+    lat.close()
+
+
 def test_ibm_moving_body_added_mass_fp64(oracle_mod):
     """Transverse oscillation: the markers move every step (nearest points,
     interior cells and tiles recomputed) and the added-mass term is live."""
