@@ -663,13 +663,18 @@ __global__ __launch_bounds__(64 * SQ_WAVES) void k_ibm_square(int R, const int32
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int n = blockIdx.x;
   const int base = base2[n];
-  // row n: lane l holds entries l and l + 64 (R <= 128)
+  // row n: lane l holds entries l and l + 64 (R <= 128); the thread's entry
+  // of Q's row and its cds, loaded with them
   const size_t rn = (size_t)n * R;
   const int ja = lane < R ? nbr[rn + lane] : n, jb = lane + 64 < R ? nbr[rn + lane + 64] : n;
   const double ga = lane < R ? G[rn + lane] : 0.0, gb = lane + 64 < R ? G[rn + lane + 64] : 0.0;
-  const double ca = ga * cds[ja], cb = gb * cds[jb];
+  const int p = threadIdx.x;
+  const size_t e2 = (size_t)n * R2 + p;
+  const bool real = p < R2 && p < len2[n];
+  const int k2 = real ? nbr2[e2] : n;
+  const double ca = ga * cds[ja], cb = gb * cds[jb], c2 = cds[k2];
+  // each wavefront writes only its own window until the barrier below
   for (int i = lane; i < SQ_WIN; i += 64) win[wv][i] = 0.0;
-  __syncthreads();
   if (wv == 0) {
     if (ga != 0.0) atomicAdd(&win[0][ja - base], -2.0 * ga);
     if (gb != 0.0) atomicAdd(&win[0][jb - base], -2.0 * gb);
@@ -698,17 +703,14 @@ __global__ __launch_bounds__(64 * SQ_WAVES) void k_ibm_square(int R, const int32
     }
   }
   __syncthreads();
-  const int p = threadIdx.x;
   if (p < R2) {
-    const size_t e = (size_t)n * R2 + p;
-    const int k = nbr2[e];
     double q = 0.0;
-    if (p < len2[n]) {
-      const int w = k - base;
+    if (real) {
+      const int w = k2 - base;
       const double v = (win[0][w] + win[1][w]) + (win[2][w] + win[3][w]);
-      q = (k == n ? 1.0 : 0.0) + cds[k] * v;
+      q = (k2 == n ? 1.0 : 0.0) + c2 * v;
     }
-    Q[e] = q;
+    Q[e2] = q;
   }
 }
 
