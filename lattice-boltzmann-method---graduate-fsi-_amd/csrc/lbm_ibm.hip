@@ -541,7 +541,7 @@ __global__ __launch_bounds__(64 * MK_WG) void k_ibm_pass(int nm, int R, const do
                                                   int last) {
   static_assert(L >= 4 && L <= 64 && (L & (L - 1)) == 0, "lanes per marker: a power of two, 4..64");
   const int lane = threadIdx.x & 63, j = lane & (L - 1);
-  const int n0 = (blockIdx.x * MK_WG + (threadIdx.x >> 6)) * (64 / L);
+  const int n0 = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 / L);
   if (n0 >= nm) return;  // wave-uniform
   const int nr = n0 + lane / L;
   const bool valid = nr < nm;  // the group's lanes keep shuffling when not
@@ -602,16 +602,17 @@ __global__ __launch_bounds__(64 * MK_WG) void k_ibm_pass(int nm, int R, const do
 
 // The passes of one IBM step, one wavefront per marker (groups of fewer
 // lanes, several markers per wavefront, measured slower: 6.9 against 5.1 us
-// per pass on 4096 markers with 72-entry rows), the u_b buffers alternating
-// so that the last pass writes ub[0].
+// per pass on 4096 markers with 72-entry rows), `waves` wavefronts per
+// workgroup, the u_b buffers alternating so that the last pass writes ub[0].
 template <bool PM>
 static void launch_passes(hipStream_t st, int nm, int R, int m_max, const double *rn, const double *cds,
                           const double *ds, const int32_t *nbr, const double *G, double *const ub[2],
-                          const double u[3], const double *udv, double *A, double *dsum, double *Fb) {
+                          const double u[3], const double *udv, double *A, double *dsum, double *Fb,
+                          int waves = MK_WG) {
   const int s0 = m_max % 2;
   const int L = 64;
-  const int per_wg = MK_WG * (64 / L);
-  const dim3 grid((unsigned)((nm + per_wg - 1) / per_wg)), blk(64 * MK_WG);
+  const int per_wg = waves * (64 / L);
+  const dim3 grid((unsigned)((nm + per_wg - 1) / per_wg)), blk(64 * waves);
   for (int it = 1; it <= m_max; it++) {
     const double *in = ub[(it - 1 + s0) % 2];
     double *out = ub[(it + s0) % 2];
@@ -785,52 +786,46 @@ __global__ __launch_bounds__(64 * MK_WG) void k_ibm_qfinal(int nm, int R, const 
     Ao = A[3 * n + lane];
     if (lane < 2) Do = dsum[2 * n + lane];
   }
-  // H (W + t) and H t over row n of H
-  double h0 = 0, h1 = 0, h2 = 0, g0 = 0, g1 = 0, g2 = 0;
-  for (int r0 = 0; r0 < R; r0 += 2 * 64) {
-    int k[2];
-    double gv[2];
+  // H (W + t) and H t over row n of H, Q t over row n of Q: both rows'
+  // entries loaded first, then the gathers (R <= 128, R2 <= 256)
+  int kh[2], kq[4];
+  double gh[2], qv[4];
 #pragma unroll
-    for (int u = 0; u < 2; u++) {
-      const int r = r0 + 64 * u + lane;
-      k[u] = r < R ? nbr[(size_t)n * R + r] : n;
-      gv[u] = r < R ? G[(size_t)n * R + r] : 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < 2; u++) {
-      const int kk = k[u];
-      const double h = gv[u] * cds[kk];
-      const double a0 = t[3 * kk + 0], a1 = t[3 * kk + 1], a2 = t[3 * kk + 2];
-      const double t0 = UB ? u0 - a0 : a0, t1 = UB ? u1 - a1 : a1, t2 = UB ? u2 - a2 : a2;
-      const double w0 = (HW ? W[3 * kk + 0] : 0.0) + t0, w1 = (HW ? W[3 * kk + 1] : 0.0) + t1,
-                   w2 = (HW ? W[3 * kk + 2] : 0.0) + t2;
-      h0 += h * w0;
-      h1 += h * w1;
-      h2 += h * w2;
-      g0 += h * t0;
-      g1 += h * t1;
-      g2 += h * t2;
-    }
+  for (int u = 0; u < 2; u++) {
+    const int r = 64 * u + lane;
+    kh[u] = r < R ? nbr[(size_t)n * R + r] : n;
+    gh[u] = r < R ? G[(size_t)n * R + r] : 0.0;
   }
-  // Q t over row n of Q
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int r = 64 * u + lane;
+    kq[u] = r < R2 ? nbr2[(size_t)n * R2 + r] : n;
+    qv[u] = r < R2 ? Q[(size_t)n * R2 + r] : 0.0;
+  }
+  double h0 = 0, h1 = 0, h2 = 0, g0 = 0, g1 = 0, g2 = 0;
+#pragma unroll
+  for (int u = 0; u < 2; u++) {
+    const int kk = kh[u];
+    const double h = gh[u] * cds[kk];
+    const double a0 = t[3 * kk + 0], a1 = t[3 * kk + 1], a2 = t[3 * kk + 2];
+    const double t0 = UB ? u0 - a0 : a0, t1 = UB ? u1 - a1 : a1, t2 = UB ? u2 - a2 : a2;
+    const double w0 = (HW ? W[3 * kk + 0] : 0.0) + t0, w1 = (HW ? W[3 * kk + 1] : 0.0) + t1,
+                 w2 = (HW ? W[3 * kk + 2] : 0.0) + t2;
+    h0 += h * w0;
+    h1 += h * w1;
+    h2 += h * w2;
+    g0 += h * t0;
+    g1 += h * t1;
+    g2 += h * t2;
+  }
   double q0 = 0, q1 = 0, q2 = 0;
-  for (int r0 = 0; r0 < R2; r0 += 4 * 64) {
-    int k[4];
-    double qv[4];
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int r = r0 + 64 * u + lane;
-      k[u] = r < R2 ? nbr2[(size_t)n * R2 + r] : n;
-      qv[u] = r < R2 ? Q[(size_t)n * R2 + r] : 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int kk = k[u];
-      const double a0 = t[3 * kk + 0], a1 = t[3 * kk + 1], a2 = t[3 * kk + 2];
-      q0 += qv[u] * (UB ? u0 - a0 : a0);
-      q1 += qv[u] * (UB ? u1 - a1 : a1);
-      q2 += qv[u] * (UB ? u2 - a2 : a2);
-    }
+  for (int u = 0; u < 4; u++) {
+    const int kk = kq[u];
+    const double a0 = t[3 * kk + 0], a1 = t[3 * kk + 1], a2 = t[3 * kk + 2];
+    q0 += qv[u] * (UB ? u0 - a0 : a0);
+    q1 += qv[u] * (UB ? u1 - a1 : a1);
+    q2 += qv[u] * (UB ? u2 - a2 : a2);
   }
   for (int off = 32; off > 0; off >>= 1) {
     h0 += __shfl_xor(h0, off);
@@ -1139,7 +1134,7 @@ __global__ __launch_bounds__(64 * MK_WG) void k_fsi_interp_h(FsiArgs a, BoxArray
   __shared__ double term[MK_WG][3][64];
   __shared__ double qn[MK_WG][64];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int n = blockIdx.x * MK_WG + wv;
+  const int n = blockIdx.x * (blockDim.x >> 6) + wv;  // up to MK_WG wavefronts per workgroup
   const bool valid = n < a.nm;
   double p0 = 0, p1 = 0, p2 = 0, xpl = 0;
   if (valid) {
@@ -1841,11 +1836,13 @@ static int fsi_iterate(lbm_ctx *c) {
     const int s0 = m->m_max % 2;
     FsiArgs a0 = a;
     a0.ub = buf[s0];
-    const dim3 gm((unsigned)((n + MK_WG - 1) / MK_WG));
-    hipLaunchKernelGGL(k_fsi_interp_h, gm, dim3(64 * MK_WG), 0, c->stream, a0, o, m->R, m->H, m->rn, m->cds);
+    // one wavefront per workgroup: a filament's few markers spread over as
+    // many compute units (k_fsi_interp_h computes every row entry's weights)
+    const dim3 gm((unsigned)n);
+    hipLaunchKernelGGL(k_fsi_interp_h, gm, dim3(64), 0, c->stream, a0, o, m->R, m->H, m->rn, m->cds);
     const double zero[3] = {0, 0, 0};
     launch_passes<true>(c->stream, n, m->R, m->m_max, m->rn, m->cds, m->ds, m->nbr, m->H, buf, zero, m->ud, m->Fds,
-                        m->fsum, m->Fb);
+                        m->fsum, m->Fb, 1);
     hipLaunchKernelGGL(k_fsi_spread, fsi_spread_grid(m->B), dim3(256), 0, c->stream, a, o);
     HIPCHK(hipGetLastError());
     return LBM_OK;
