@@ -459,7 +459,7 @@ __global__ __launch_bounds__(64 * MK_WG) void k_ibm_interp_h(int nm, int R, cons
   __shared__ double term[MK_WG][3][64];
   __shared__ double qn[MK_WG][64];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int n = blockIdx.x * MK_WG + wv;
+  const int n = blockIdx.x * (blockDim.x >> 6) + wv;  // up to MK_WG wavefronts per workgroup
   const bool valid = n < nm;
   const int b = valid ? pidx[64 * n + lane] : -1;
   const int len = valid ? nlen[n] : 0;
@@ -723,7 +723,7 @@ __global__ __launch_bounds__(64 * MK_WG) void k_ibm_qpass(int nm, int R2, const 
                                                    double *__restrict__ tout, double *__restrict__ W, double u0,
                                                    double u1, double u2, int first) {
   const int lane = threadIdx.x & 63;
-  const int n = blockIdx.x * MK_WG + (threadIdx.x >> 6);
+  const int n = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (n >= nm) return;  // wave-uniform
   const int l3 = lane < 3 ? lane : 0;
   const double ul = lane == 0 ? u0 : lane == 1 ? u1 : u2;
@@ -774,7 +774,7 @@ __global__ __launch_bounds__(64 * MK_WG) void k_ibm_qfinal(int nm, int R, const 
                                                     double *__restrict__ A, double *__restrict__ dsum,
                                                     double *__restrict__ Fb, double *__restrict__ ub, int first) {
   const int lane = threadIdx.x & 63;
-  const int n = blockIdx.x * MK_WG + (threadIdx.x >> 6);
+  const int n = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (n >= nm) return;  // wave-uniform
   const int l3 = lane < 3 ? lane : 0;
   const double ul = lane == 0 ? u0 : lane == 1 ? u1 : u2;
