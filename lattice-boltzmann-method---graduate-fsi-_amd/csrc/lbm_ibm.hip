@@ -1429,6 +1429,7 @@ struct lbm_ibm {
   double *H = nullptr, *ub2 = nullptr;  // H: G_nk (times rho_k ds_k in the passes)
   double *rn = nullptr, *cds = nullptr;  // density point value rho_n per marker, rho_n ds_n
   int R = 0;
+  int rows_static = 0;  // rows by stencil bases (valid while the markers stay put)
   // paired passes (m_max >= 2): rows of Q = (I - H)^2, R2 entries each
   // (len2 real, the rest padding), the LDS window base of each row, Q per
   // step, the products' t buffers, W and the u_b of a leading single pass
@@ -1860,9 +1861,25 @@ static int fsi_iterate(lbm_ctx *c) {
 // Marker motion of one IBLBM call (lbm.cpp:1154-1166): the trapezoid rule
 // for the markers and, when the box follows the body, for its centre; then
 // the box and the geometry that depends on the marker positions.
+static int build_neighbours(lbm_ibm *m, const double *x, const double *y, const double *z, bool by_base);
+
 static int ibm_move(lbm_ctx *c, const double vcm[3], const double vb[3]) {
   lbm_ibm *m = c->ibm;
   const bool moved = vcm[0] + vb[0] != 0 || vcm[1] + vb[1] != 0 || vcm[2] + vb[2] != 0;
+  if (moved && m->rows_static) {
+    // a body set up at rest was given a velocity: its rows by stencil bases
+    // no longer cover every overlapping pair; the rows of a rigid motion
+    // (markers closer than 4 cells) replace them
+    m->nbr = nullptr;
+    m->nbr2 = nullptr;
+    m->rows_static = 0;
+    std::vector<double> x(m->nm), y(m->nm), z(m->nm);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(x.data(), m->mx, 8 * (size_t)m->nm, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(y.data(), m->my, 8 * (size_t)m->nm, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(z.data(), m->mz, 8 * (size_t)m->nm, hipMemcpyDeviceToHost));
+    if (int e = build_neighbours(m, x.data(), y.data(), z.data(), false)) return e;
+  }
   // (V_CM + V_CM_before) dt/2 = 0 leaves every marker where it is
   if (moved) {
     hipLaunchKernelGGL(k_ibm_move, blocks(m->nm), dim3(256), 0, c->stream, m->nm, m->mx, m->my, m->mz, vcm[0], vcm[1],
@@ -1938,9 +1955,11 @@ static int build_square(lbm_ibm *m, const std::vector<std::vector<int32_t>> &row
 // closer than 4 cells on every axis (itself included), sorted.  Stencils
 // (floor(x) - 1 .. floor(x) + 2 per axis) of markers 4 or more cells apart
 // on some axis never share a cell, and a rigid motion keeps the list.
+// by_base (a body that stays put): only markers whose stencil bases are at
+// most 3 apart on every axis, the pairs whose stencils do share cells.
 // Buckets of 4 cells; rows longer than 256 entries on average (markers far
 // denser than the lattice) keep the pass-by-pass path.
-static int build_neighbours(lbm_ibm *m, const double *x, const double *y, const double *z) {
+static int build_neighbours(lbm_ibm *m, const double *x, const double *y, const double *z, bool by_base) {
   const int nm = m->nm;
   std::vector<std::pair<uint64_t, int>> key(nm);
   auto bucket = [](double v) { return (int64_t)std::floor(v / 4.0); };
@@ -1961,7 +1980,10 @@ static int build_neighbours(lbm_ibm *m, const double *x, const double *y, const 
           auto it = std::lower_bound(key.begin(), key.end(), std::make_pair(kk, -1));
           for (; it != key.end() && it->first == kk; ++it) {
             const int q = it->second;
-            if (std::fabs(x[q] - x[n]) < 4.0 && std::fabs(y[q] - y[n]) < 4.0 && std::fabs(z[q] - z[n]) < 4.0)
+            if (std::fabs(x[q] - x[n]) < 4.0 && std::fabs(y[q] - y[n]) < 4.0 && std::fabs(z[q] - z[n]) < 4.0 &&
+                (!by_base || (std::fabs(std::floor(x[q]) - std::floor(x[n])) <= 3.0 &&
+                              std::fabs(std::floor(y[q]) - std::floor(y[n])) <= 3.0 &&
+                              std::fabs(std::floor(z[q]) - std::floor(z[n])) <= 3.0)))
               row.push_back(q);
           }
         }
@@ -2050,7 +2072,11 @@ static int ibm_setup(lbm_ctx *c, const lbm_ibm_desc *d, const double *x, const d
   HIPCHK(hipMemsetAsync(m->dsum, 0, 16 * nm, c->stream));
   // rigid bodies of the implicit scheme run the passes in marker space
   if (m->scheme == 0) {
-    e = build_neighbours(m, x, y, z);
+    // a body at rest keeps its stencil bases: only markers whose stencils
+    // overlap (bases at most 3 apart on every axis) enter a row
+    const bool by_base = !m->moving && !m->follow;
+    e = build_neighbours(m, x, y, z, by_base);
+    m->rows_static = by_base;
     if (e != LBM_OK) return e;
   }
   e = place_box(c, b);
