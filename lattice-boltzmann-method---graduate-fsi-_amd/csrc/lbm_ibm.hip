@@ -653,6 +653,7 @@ static void launch_passes(hipStream_t st, int nm, int R, int m_max, const double
 constexpr int SQ_WIN = 1024;
 constexpr int SQ_WAVES = 4;
 constexpr int SQ_UNROLL = 9;  // rows j loaded together per wavefront
+template <bool WIDE>  // rows of H longer than 64 entries (R <= 128)
 __global__ __launch_bounds__(64 * SQ_WAVES) void k_ibm_square(int R, const int32_t *__restrict__ nbr,
                                                               const double *__restrict__ G,
                                                               const double *__restrict__ cds, int R2,
@@ -667,8 +668,8 @@ __global__ __launch_bounds__(64 * SQ_WAVES) void k_ibm_square(int R, const int32
   // row n: lane l holds entries l and l + 64 (R <= 128); the thread's entry
   // of Q's row and its cds, loaded with them
   const size_t rn = (size_t)n * R;
-  const int ja = lane < R ? nbr[rn + lane] : n, jb = lane + 64 < R ? nbr[rn + lane + 64] : n;
-  const double ga = lane < R ? G[rn + lane] : 0.0, gb = lane + 64 < R ? G[rn + lane + 64] : 0.0;
+  const int ja = lane < R ? nbr[rn + lane] : n, jb = WIDE && lane + 64 < R ? nbr[rn + lane + 64] : n;
+  const double ga = lane < R ? G[rn + lane] : 0.0, gb = WIDE && lane + 64 < R ? G[rn + lane + 64] : 0.0;
   const int p = threadIdx.x;
   const size_t e2 = (size_t)n * R2 + p;
   const bool real = p < R2 && p < len2[n];
@@ -678,7 +679,7 @@ __global__ __launch_bounds__(64 * SQ_WAVES) void k_ibm_square(int R, const int32
   for (int i = lane; i < SQ_WIN; i += 64) win[wv][i] = 0.0;
   if (wv == 0) {
     if (ga != 0.0) atomicAdd(&win[0][ja - base], -2.0 * ga);
-    if (gb != 0.0) atomicAdd(&win[0][jb - base], -2.0 * gb);
+    if (WIDE && gb != 0.0) atomicAdd(&win[0][jb - base], -2.0 * gb);
   }
   for (int t0 = wv; t0 < R; t0 += SQ_WAVES * SQ_UNROLL) {
     int ka[SQ_UNROLL], kb[SQ_UNROLL];
@@ -687,20 +688,20 @@ __global__ __launch_bounds__(64 * SQ_WAVES) void k_ibm_square(int R, const int32
     for (int u = 0; u < SQ_UNROLL; u++) {
       const int t = t0 + SQ_WAVES * u;
       const int tl = t & 63;
-      const int j = t < 64 ? __shfl(ja, tl) : __shfl(jb, tl);
-      const double cj = t < 64 ? __shfl(ca, tl) : __shfl(cb, tl);
+      const int j = !WIDE || t < 64 ? __shfl(ja, tl) : __shfl(jb, tl);
+      const double cj = !WIDE || t < 64 ? __shfl(ca, tl) : __shfl(cb, tl);
       c[u] = t < R ? cj : 0.0;
       const size_t rj = (size_t)j * R;
       ka[u] = lane < R ? nbr[rj + lane] : j;
-      kb[u] = lane + 64 < R ? nbr[rj + lane + 64] : j;
+      kb[u] = WIDE && lane + 64 < R ? nbr[rj + lane + 64] : j;
       gka[u] = lane < R ? G[rj + lane] : 0.0;
-      gkb[u] = lane + 64 < R ? G[rj + lane + 64] : 0.0;
+      gkb[u] = WIDE && lane + 64 < R ? G[rj + lane + 64] : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < SQ_UNROLL; u++) {
       if (c[u] == 0.0) continue;  // uniform over the wavefront
       if (gka[u] != 0.0) atomicAdd(&win[wv][ka[u] - base], c[u] * gka[u]);
-      if (gkb[u] != 0.0) atomicAdd(&win[wv][kb[u] - base], c[u] * gkb[u]);
+      if (WIDE && gkb[u] != 0.0) atomicAdd(&win[wv][kb[u] - base], c[u] * gkb[u]);
     }
   }
   __syncthreads();
@@ -1760,8 +1761,12 @@ static int ibm_iterate(lbm_ctx *c, const double vcm[3], const double vb[3], doub
       // pass when m_max is odd, q - 1 products by Q, the closing launch
       hipLaunchKernelGGL(k_ibm_interp_h, gm, dim3(64 * MK_WG), 0, st, m->nm, m->R, m->pidx, m->st, o, m->near,
                          m->ds, m->nlen, m->nbr, m->H, m->rn, m->cds, m->ub2);
-      hipLaunchKernelGGL(k_ibm_square, dim3((unsigned)m->nm), dim3(64 * SQ_WAVES), 0, st, m->R, m->nbr, m->H, m->cds,
-                         m->R2, m->nbr2, m->len2, m->base2, m->Q2);
+      if (m->R > 64)
+        hipLaunchKernelGGL(k_ibm_square<true>, dim3((unsigned)m->nm), dim3(64 * SQ_WAVES), 0, st, m->R, m->nbr, m->H,
+                           m->cds, m->R2, m->nbr2, m->len2, m->base2, m->Q2);
+      else
+        hipLaunchKernelGGL(k_ibm_square<false>, dim3((unsigned)m->nm), dim3(64 * SQ_WAVES), 0, st, m->R, m->nbr, m->H,
+                           m->cds, m->R2, m->nbr2, m->len2, m->base2, m->Q2);
       const int lead = m->m_max % 2, q = (m->m_max - lead) / 2;
       double *src = m->ub2;
       if (lead) {
