@@ -765,7 +765,7 @@ __global__ __launch_bounds__(64 * MK_WG) void k_ibm_qpass(int nm, int R2, const 
 // none): s = 2 (W + t) - H (W + t), e_last = t - H t, e_after = Q t; then
 // A += 2 rho_n s ds_n, the drag/lift pairs -2 rho_n s ds_n, F_b = 2 rho_n
 // e_last, u_b = u_des - e_after.  `first`: A and the pairs start here.
-template <bool UB>
+template <bool UB, bool WIDE>  // WIDE: rows of H longer than 64 entries (R <= 128)
 __global__ __launch_bounds__(64 * MK_WG) void k_ibm_qfinal(int nm, int R, const int32_t *__restrict__ nbr,
                                                     const double *__restrict__ G, const double *__restrict__ cds,
                                                     int R2, const int32_t *__restrict__ nbr2,
@@ -789,10 +789,11 @@ __global__ __launch_bounds__(64 * MK_WG) void k_ibm_qfinal(int nm, int R, const 
   }
   // H (W + t) and H t over row n of H, Q t over row n of Q: both rows'
   // entries loaded first, then the gathers (R <= 128, R2 <= 256)
-  int kh[2], kq[4];
-  double gh[2], qv[4];
+  constexpr int NH = WIDE ? 2 : 1;
+  int kh[NH], kq[4];
+  double gh[NH], qv[4];
 #pragma unroll
-  for (int u = 0; u < 2; u++) {
+  for (int u = 0; u < NH; u++) {
     const int r = 64 * u + lane;
     kh[u] = r < R ? nbr[(size_t)n * R + r] : n;
     gh[u] = r < R ? G[(size_t)n * R + r] : 0.0;
@@ -805,7 +806,7 @@ __global__ __launch_bounds__(64 * MK_WG) void k_ibm_qfinal(int nm, int R, const 
   }
   double h0 = 0, h1 = 0, h2 = 0, g0 = 0, g1 = 0, g2 = 0;
 #pragma unroll
-  for (int u = 0; u < 2; u++) {
+  for (int u = 0; u < NH; u++) {
     const int kk = kh[u];
     const double h = gh[u] * cds[kk];
     const double a0 = t[3 * kk + 0], a1 = t[3 * kk + 1], a2 = t[3 * kk + 2];
@@ -1783,14 +1784,27 @@ static int ibm_iterate(lbm_ctx *c, const double vcm[3], const double vb[3], doub
           hipLaunchKernelGGL(k_ibm_qpass<false>, gm, dim3(64 * MK_WG), 0, st, m->nm, m->R2, m->nbr2, m->Q2,
                              m->sq_t[(i - 1) % 2], m->sq_t[i % 2], m->sq_w, vcm[0], vcm[1], vcm[2], 0);
       }
-      if (q == 1)
-        hipLaunchKernelGGL(k_ibm_qfinal<true>, gm, dim3(64 * MK_WG), 0, st, m->nm, m->R, m->nbr, m->H, m->cds, m->R2,
-                           m->nbr2, m->Q2, src, m->sq_w, 0, m->rn, m->ds, vcm[0], vcm[1], vcm[2], m->Fds, m->dsum,
-                           m->Fb, m->ub, lead ? 0 : 1);
-      else
-        hipLaunchKernelGGL(k_ibm_qfinal<false>, gm, dim3(64 * MK_WG), 0, st, m->nm, m->R, m->nbr, m->H, m->cds,
-                           m->R2, m->nbr2, m->Q2, m->sq_t[(q - 2) % 2], m->sq_w, 1, m->rn, m->ds, vcm[0], vcm[1],
-                           vcm[2], m->Fds, m->dsum, m->Fb, m->ub, lead ? 0 : 1);
+      const bool wide = m->R > 64;
+      if (q == 1) {
+        if (wide)
+          hipLaunchKernelGGL((k_ibm_qfinal<true, true>), gm, dim3(64 * MK_WG), 0, st, m->nm, m->R, m->nbr, m->H,
+                             m->cds, m->R2, m->nbr2, m->Q2, src, m->sq_w, 0, m->rn, m->ds, vcm[0], vcm[1], vcm[2],
+                             m->Fds, m->dsum, m->Fb, m->ub, lead ? 0 : 1);
+        else
+          hipLaunchKernelGGL((k_ibm_qfinal<true, false>), gm, dim3(64 * MK_WG), 0, st, m->nm, m->R, m->nbr, m->H,
+                             m->cds, m->R2, m->nbr2, m->Q2, src, m->sq_w, 0, m->rn, m->ds, vcm[0], vcm[1], vcm[2],
+                             m->Fds, m->dsum, m->Fb, m->ub, lead ? 0 : 1);
+      } else {
+        const double *tq = m->sq_t[(q - 2) % 2];
+        if (wide)
+          hipLaunchKernelGGL((k_ibm_qfinal<false, true>), gm, dim3(64 * MK_WG), 0, st, m->nm, m->R, m->nbr, m->H,
+                             m->cds, m->R2, m->nbr2, m->Q2, tq, m->sq_w, 1, m->rn, m->ds, vcm[0], vcm[1], vcm[2],
+                             m->Fds, m->dsum, m->Fb, m->ub, lead ? 0 : 1);
+        else
+          hipLaunchKernelGGL((k_ibm_qfinal<false, false>), gm, dim3(64 * MK_WG), 0, st, m->nm, m->R, m->nbr, m->H,
+                             m->cds, m->R2, m->nbr2, m->Q2, tq, m->sq_w, 1, m->rn, m->ds, vcm[0], vcm[1], vcm[2],
+                             m->Fds, m->dsum, m->Fb, m->ub, lead ? 0 : 1);
+      }
     } else {
       hipLaunchKernelGGL(k_ibm_interp_h, gm, dim3(64 * MK_WG), 0, st, m->nm, m->R, m->pidx, m->st, o, m->near,
                          m->ds, m->nlen, m->nbr, m->H, m->rn, m->cds, buf[s0]);

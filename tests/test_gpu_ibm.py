@@ -137,13 +137,20 @@ def test_ibm_fused_run_fp64(oracle_mod, name):
     lat.close()
 
 
-@pytest.mark.parametrize("m_max", [1, 2, 3, 4, 7])
-def test_ibm_iteration_counts_fp64(oracle_mod, m_max):
+@pytest.mark.parametrize("name,m_max", [("span", 1), ("span", 2), ("span", 3), ("span", 4), ("span", 7),
+                                         ("dense", 2), ("dense", 3), ("dense", 20)])
+def test_ibm_iteration_counts_fp64(oracle_mod, name, m_max):
     """The m_max passes run as a single pass (m_max odd) plus products by
     (I - H)^2 and a closing launch: every branch of that schedule (no
     product, one, several; with and without the single pass) against the
-    oracle's m_max spread/interpolate passes, fused and step-by-step."""
-    case, mk, box = IBM_CASES["span"]()
+    oracle's m_max spread/interpolate passes, fused and step-by-step.
+    "dense": 50 markers per ring of D = 8 (spacing 0.5), whose marker rows
+    are longer than one wavefront (65..128 entries)."""
+    if name == "dense":
+        case, mk, box = cases.ibm_cylinder(36, 24, 6, D=8.0, Re=20.0, perturb=0.01, center=(14.0, 12.0),
+                                           n_theta=50)
+    else:
+        case, mk, box = IBM_CASES[name]()
     nsteps = 6
     o = _oracle(oracle_mod, case)
     omk = {k: v.copy() for k, v in mk.items()}
