@@ -140,6 +140,24 @@ int lbm_reduce_fp64(lbm_ctx *ctx, int32_t what, double *out);
  * that array. */
 int lbm_reduce_fp64_noalloc(lbm_ctx *ctx, int32_t what, double *out);
 
+/* Checkpoint and resume (no counterpart in the reference, whose only resume
+ * is the interactive prompt that extends the step count, main.cpp:397-410).
+ * A checkpoint holds the state that stepping changes: both population
+ * buffers, the macroscopic arrays, the wall tables, the status
+ * word, a per-cell force field, the inlet/outlet slots, the thermal arrays,
+ * the running statistics and the context's state flags.  lbm_checkpoint_size
+ * gives its byte count; lbm_checkpoint_save writes it to caller memory
+ * (bytes at least that count); lbm_checkpoint_load restores it into a context
+ * built by the same set-up (lbm_create, lbm_upload_cells,
+ * lbm_upload_thermal, lbm_init), after which steps give the saved context's
+ * results bit for bit.  LBM_ERR_ARG when the checkpoint does not match the
+ * context (lattice, storage, thermal model, inlet/outlet cells, walls, force
+ * field); LBM_ERR_STATE before lbm_init; LBM_ERR_UNSUPPORTED for a context
+ * with an immersed boundary. */
+int lbm_checkpoint_size(lbm_ctx *ctx, uint64_t *bytes);
+int lbm_checkpoint_save(lbm_ctx *ctx, void *buf, uint64_t bytes);
+int lbm_checkpoint_load(lbm_ctx *ctx, const void *buf, uint64_t bytes);
+
 /* The streaming tables the device kernels use, for bit-exact comparison with
  * lbm.cpp:186-295: per cell and direction m, src = linear index of the
  * cell x - c_m (wrapped modulo N+2 per axis, lbm.cpp:197-199) and

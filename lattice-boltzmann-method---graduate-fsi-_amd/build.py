@@ -49,6 +49,7 @@ SOURCES = [
     os.path.join(HERE, "csrc", "lbm_faces.hip"),
     os.path.join(HERE, "csrc", "lbm_stats.hip"),
     os.path.join(HERE, "csrc", "lbm_noalloc.hip"),
+    os.path.join(HERE, "csrc", "lbm_checkpoint.hip"),
 ]
 HOST_SOURCES = [
     os.path.join(HERE, "host", "LBM.cpp"),

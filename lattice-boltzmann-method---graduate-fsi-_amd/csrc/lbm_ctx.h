@@ -12,6 +12,9 @@
 
 enum { ST_NONE = 0, ST_F = 1, ST_FSTAR = 2 };
 
+// FP64 planes of the running statistics (lbm_stats.hip)
+constexpr int LBM_STAT_PLANES = 26;
+
 struct lbm_ibm;  // immersed-boundary state (lbm_ibm.hip)
 
 struct lbm_ctx {
@@ -38,7 +41,7 @@ struct lbm_ctx {
   // the last buffer operation was a streaming that pulled from oth: oth holds
   // the f* that produced cur's f (what the momentum exchange reads)
   int fstar_ready = 0;
-  // running statistics of Sup_MacroProp (lbm_stats_update): STAT_PLANES
+  // running statistics of Sup_MacroProp (lbm_stats_update): LBM_STAT_PLANES
   // FP64 planes of ncell values, allocated on the first update
   double *stats = nullptr;
   int uploaded = 0;

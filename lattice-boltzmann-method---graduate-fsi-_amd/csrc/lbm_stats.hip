@@ -34,6 +34,7 @@ enum {
   ST_RMS_RS = 17, // 9: rms_ReynoldStress[a][b]
   STAT_PLANES = 26
 };
+static_assert(STAT_PLANES == LBM_STAT_PLANES, "the context's statistics planes");
 
 // MacroProp's MORE_DETAILS block (lbm.cpp:681-705): pressure = rho/3 and
 // sigma_ab = -(tau - 1/2)/tau sum_l f_l (c_la - u_a)(c_lb - u_b), for every

@@ -180,6 +180,9 @@ _SIGNATURES = {
     "lbm_batch_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _D]),
     "lbm_autotune": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "lbm_reduce_fp64_noalloc": (C.c_int, [_P, C.c_int32, _D]),
+    "lbm_checkpoint_size": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    "lbm_checkpoint_save": (C.c_int, [_P, C.c_void_p, C.c_uint64]),
+    "lbm_checkpoint_load": (C.c_int, [_P, C.c_void_p, C.c_uint64]),
     "lbm_fsi_setup": (C.c_int, [_P, _P, C.POINTER(_IbmDesc)]),
     "lbm_fsi_run": (C.c_int, [_P, C.c_int32, _D]),
     "lbm_momentum_exchange": (C.c_int, [_P, C.POINTER(C.c_int32), _D]),
@@ -409,6 +412,19 @@ class Lattice:
         out = np.zeros(3, np.float64)
         _check(self._lib.lbm_reduce_fp64(self._h, int(what), _dp(out)), "lbm_reduce_fp64")
         return out[:3] if what == REDUCE_MOMENTUM else float(out[0])
+
+    def checkpoint(self) -> np.ndarray:
+        """The evolving state (lbm_checkpoint_save) as a uint8 array."""
+        n = C.c_uint64(0)
+        _check(self._lib.lbm_checkpoint_size(self._h, C.byref(n)), "lbm_checkpoint_size")
+        buf = np.empty(int(n.value), dtype=np.uint8)
+        _check(self._lib.lbm_checkpoint_save(self._h, buf.ctypes.data, n.value), "lbm_checkpoint_save")
+        return buf
+
+    def restore(self, buf: np.ndarray):
+        """lbm_checkpoint_load of a checkpoint() of a lattice built by the same set-up."""
+        b = np.ascontiguousarray(buf, dtype=np.uint8)
+        _check(self._lib.lbm_checkpoint_load(self._h, b.ctypes.data, b.size), "lbm_checkpoint_load")
 
     def reduce_noalloc(self, what: int):
         """reduce() through lbm_reduce_fp64_noalloc (no device allocation)."""
