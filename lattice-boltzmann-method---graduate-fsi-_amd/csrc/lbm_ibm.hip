@@ -762,18 +762,19 @@ __global__ __launch_bounds__(64 * MK_WG) void k_ibm_qpass(int nm, int R2, const 
 
 // The closing launch of the paired passes, from t = t_{q-1} (UB: from
 // u_des - u_b of the u_b array `t`) and W = t_0 + ... + t_{q-2} (HW = 0:
-// none): s = 2 (W + t) - H (W + t), e_last = t - H t, e_after = Q t; then
-// A += 2 rho_n s ds_n, the drag/lift pairs -2 rho_n s ds_n, F_b = 2 rho_n
-// e_last, u_b = u_des - e_after.  `first`: A and the pairs start here.
+// none): s = 2 (W + t) - H (W + t), e_last = t - H t; then A += 2 rho_n s
+// ds_n, the drag/lift pairs -2 rho_n s ds_n, F_b = 2 rho_n e_last.
+// `first`: A and the pairs start here.  The markers' final u_b = u_des -
+// Q t is an output only (the next step interpolates afresh): k_ibm_ub_last
+// computes it when it is read.
 template <bool UB, bool WIDE>  // WIDE: rows of H longer than 64 entries (R <= 128)
 __global__ __launch_bounds__(64 * MK_WG) void k_ibm_qfinal(int nm, int R, const int32_t *__restrict__ nbr,
                                                     const double *__restrict__ G, const double *__restrict__ cds,
-                                                    int R2, const int32_t *__restrict__ nbr2,
-                                                    const double *__restrict__ Q, const double *__restrict__ t,
-                                                    const double *__restrict__ W, int HW, const double *__restrict__ rn,
+                                                    const double *__restrict__ t, const double *__restrict__ W,
+                                                    int HW, const double *__restrict__ rn,
                                                     const double *__restrict__ ds, double u0, double u1, double u2,
                                                     double *__restrict__ A, double *__restrict__ dsum,
-                                                    double *__restrict__ Fb, double *__restrict__ ub, int first) {
+                                                    double *__restrict__ Fb, int first) {
   const int lane = threadIdx.x & 63;
   const int n = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (n >= nm) return;  // wave-uniform
@@ -787,22 +788,16 @@ __global__ __launch_bounds__(64 * MK_WG) void k_ibm_qfinal(int nm, int R, const 
     Ao = A[3 * n + lane];
     if (lane < 2) Do = dsum[2 * n + lane];
   }
-  // H (W + t) and H t over row n of H, Q t over row n of Q: both rows'
-  // entries loaded first, then the gathers (R <= 128, R2 <= 256)
+  // H (W + t) and H t over row n of H: the row's entries first, then the
+  // gathers
   constexpr int NH = WIDE ? 2 : 1;
-  int kh[NH], kq[4];
-  double gh[NH], qv[4];
+  int kh[NH];
+  double gh[NH];
 #pragma unroll
   for (int u = 0; u < NH; u++) {
     const int r = 64 * u + lane;
     kh[u] = r < R ? nbr[(size_t)n * R + r] : n;
     gh[u] = r < R ? G[(size_t)n * R + r] : 0.0;
-  }
-#pragma unroll
-  for (int u = 0; u < 4; u++) {
-    const int r = 64 * u + lane;
-    kq[u] = r < R2 ? nbr2[(size_t)n * R2 + r] : n;
-    qv[u] = r < R2 ? Q[(size_t)n * R2 + r] : 0.0;
   }
   double h0 = 0, h1 = 0, h2 = 0, g0 = 0, g1 = 0, g2 = 0;
 #pragma unroll
@@ -820,15 +815,6 @@ __global__ __launch_bounds__(64 * MK_WG) void k_ibm_qfinal(int nm, int R, const 
     g1 += h * t1;
     g2 += h * t2;
   }
-  double q0 = 0, q1 = 0, q2 = 0;
-#pragma unroll
-  for (int u = 0; u < 4; u++) {
-    const int kk = kq[u];
-    const double a0 = t[3 * kk + 0], a1 = t[3 * kk + 1], a2 = t[3 * kk + 2];
-    q0 += qv[u] * (UB ? u0 - a0 : a0);
-    q1 += qv[u] * (UB ? u1 - a1 : a1);
-    q2 += qv[u] * (UB ? u2 - a2 : a2);
-  }
   for (int off = 32; off > 0; off >>= 1) {
     h0 += __shfl_xor(h0, off);
     h1 += __shfl_xor(h1, off);
@@ -836,14 +822,10 @@ __global__ __launch_bounds__(64 * MK_WG) void k_ibm_qfinal(int nm, int R, const 
     g0 += __shfl_xor(g0, off);
     g1 += __shfl_xor(g1, off);
     g2 += __shfl_xor(g2, off);
-    q0 += __shfl_xor(q0, off);
-    q1 += __shfl_xor(q1, off);
-    q2 += __shfl_xor(q2, off);
   }
   if (lane >= 3) return;
   const double hw = lane == 0 ? h0 : lane == 1 ? h1 : h2;
   const double ht = lane == 0 ? g0 : lane == 1 ? g1 : g2;
-  const double qt = lane == 0 ? q0 : lane == 1 ? q1 : q2;
   const double dt = 1.0, dx = 1.0, dy = 1.0;
   const double sm = 2.0 * Wn - hw;  // the 2q passes' summed e
   const double F = 2 * rho * sm / dt;
@@ -854,7 +836,34 @@ __global__ __launch_bounds__(64 * MK_WG) void k_ibm_qfinal(int nm, int R, const 
     dsum[2 * n + lane] = first ? dd : Do + dd;
   }
   Fb[3 * n + lane] = 2 * rho * (tn - ht) / dt;
-  ub[3 * n + lane] = ul - qt;
+}
+
+// The markers' u_b after the last of the paired passes: u_des - Q t_{q-1}
+// (UB: t from u_des - u_b of the u_b array `t`), one wavefront per marker.
+template <bool UB>
+__global__ __launch_bounds__(64 * MK_WG) void k_ibm_ub_last(int nm, int R2, const int32_t *__restrict__ nbr2,
+                                                     const double *__restrict__ Q, const double *__restrict__ t,
+                                                     double u0, double u1, double u2, double *__restrict__ ub) {
+  const int lane = threadIdx.x & 63;
+  const int n = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (n >= nm) return;  // wave-uniform
+  double q0 = 0, q1 = 0, q2 = 0;
+  for (int r = lane; r < R2; r += 64) {
+    const int kk = nbr2[(size_t)n * R2 + r];
+    const double qv = Q[(size_t)n * R2 + r];
+    const double a0 = t[3 * kk + 0], a1 = t[3 * kk + 1], a2 = t[3 * kk + 2];
+    q0 += qv * (UB ? u0 - a0 : a0);
+    q1 += qv * (UB ? u1 - a1 : a1);
+    q2 += qv * (UB ? u2 - a2 : a2);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    q0 += __shfl_xor(q0, off);
+    q1 += __shfl_xor(q1, off);
+    q2 += __shfl_xor(q2, off);
+  }
+  if (lane >= 3) return;
+  const double ul = lane == 0 ? u0 : lane == 1 ? u1 : u2;
+  ub[3 * n + lane] = ul - (lane == 0 ? q0 : lane == 1 ? q1 : q2);
 }
 
 template <int T = FIN_THREADS>
@@ -1438,6 +1447,11 @@ struct lbm_ibm {
   int R2 = 0;
   int32_t *nbr2 = nullptr, *len2 = nullptr, *base2 = nullptr;
   double *Q2 = nullptr, *sq_t[2] = {nullptr, nullptr}, *sq_w = nullptr, *sq_ub = nullptr;
+  // the last paired passes' u_b, computed when read: from t (UB: the u_b
+  // array t) and u_des
+  int ub_pending = 0, ub_from_ub = 0;
+  const double *ub_t = nullptr;
+  double ub_u[3] = {0, 0, 0};
   // a coupled filament's passes in marker space: R >= nm entries per row,
   // entry r = marker r (the nodes move, so G is recomputed every step)
   int fsi_rows = 0;
@@ -1718,6 +1732,7 @@ static int ibm_geometry(lbm_ctx *c) {
 // out[0..2].
 static int ibm_iterate(lbm_ctx *c, const double vcm[3], const double vb[3], double *out, hipStream_t st = nullptr) {
   lbm_ibm *m = c->ibm;
+  m->ub_pending = 0;  // every path below but the paired passes writes m->ub
   if (!st) st = c->stream;
   const Box &B = m->B;
   const BoxArrays o = box_arrays(m);
@@ -1785,26 +1800,24 @@ static int ibm_iterate(lbm_ctx *c, const double vcm[3], const double vb[3], doub
                              m->sq_t[(i - 1) % 2], m->sq_t[i % 2], m->sq_w, vcm[0], vcm[1], vcm[2], 0);
       }
       const bool wide = m->R > 64;
+      const double *tq = q == 1 ? src : m->sq_t[(q - 2) % 2];
+      const int hw = q == 1 ? 0 : 1, first = lead ? 0 : 1;
+#define LBM_QFINAL(UBV, WV)                                                                                        \
+  hipLaunchKernelGGL((k_ibm_qfinal<UBV, WV>), gm, dim3(64 * MK_WG), 0, st, m->nm, m->R, m->nbr, m->H, m->cds, tq, \
+                     m->sq_w, hw, m->rn, m->ds, vcm[0], vcm[1], vcm[2], m->Fds, m->dsum, m->Fb, first)
       if (q == 1) {
-        if (wide)
-          hipLaunchKernelGGL((k_ibm_qfinal<true, true>), gm, dim3(64 * MK_WG), 0, st, m->nm, m->R, m->nbr, m->H,
-                             m->cds, m->R2, m->nbr2, m->Q2, src, m->sq_w, 0, m->rn, m->ds, vcm[0], vcm[1], vcm[2],
-                             m->Fds, m->dsum, m->Fb, m->ub, lead ? 0 : 1);
-        else
-          hipLaunchKernelGGL((k_ibm_qfinal<true, false>), gm, dim3(64 * MK_WG), 0, st, m->nm, m->R, m->nbr, m->H,
-                             m->cds, m->R2, m->nbr2, m->Q2, src, m->sq_w, 0, m->rn, m->ds, vcm[0], vcm[1], vcm[2],
-                             m->Fds, m->dsum, m->Fb, m->ub, lead ? 0 : 1);
+        if (wide) LBM_QFINAL(true, true);
+        else LBM_QFINAL(true, false);
       } else {
-        const double *tq = m->sq_t[(q - 2) % 2];
-        if (wide)
-          hipLaunchKernelGGL((k_ibm_qfinal<false, true>), gm, dim3(64 * MK_WG), 0, st, m->nm, m->R, m->nbr, m->H,
-                             m->cds, m->R2, m->nbr2, m->Q2, tq, m->sq_w, 1, m->rn, m->ds, vcm[0], vcm[1], vcm[2],
-                             m->Fds, m->dsum, m->Fb, m->ub, lead ? 0 : 1);
-        else
-          hipLaunchKernelGGL((k_ibm_qfinal<false, false>), gm, dim3(64 * MK_WG), 0, st, m->nm, m->R, m->nbr, m->H,
-                             m->cds, m->R2, m->nbr2, m->Q2, tq, m->sq_w, 1, m->rn, m->ds, vcm[0], vcm[1], vcm[2],
-                             m->Fds, m->dsum, m->Fb, m->ub, lead ? 0 : 1);
+        if (wide) LBM_QFINAL(false, true);
+        else LBM_QFINAL(false, false);
       }
+#undef LBM_QFINAL
+      // the final u_b when it is read (lbm_ibm_markers)
+      m->ub_pending = 1;
+      m->ub_t = tq;
+      m->ub_from_ub = q == 1;
+      for (int c3 = 0; c3 < 3; c3++) m->ub_u[c3] = vcm[c3];
     } else {
       hipLaunchKernelGGL(k_ibm_interp_h, gm, dim3(64 * MK_WG), 0, st, m->nm, m->R, m->pidx, m->st, o, m->near,
                          m->ds, m->nlen, m->nbr, m->H, m->rn, m->cds, buf[s0]);
@@ -1843,6 +1856,7 @@ static int ibm_iterate(lbm_ctx *c, const double vcm[3], const double vb[3], doub
 // interpolate passes; the last interpolation computes no force.
 static int fsi_iterate(lbm_ctx *c) {
   lbm_ibm *m = c->ibm;
+  m->ub_pending = 0;
   double *fx, *fxp, fds;
   int n, fdev;
   filament_state(m->fil, &fx, &fxp, &n, &fds, &fdev);
@@ -2381,6 +2395,18 @@ int lbm_ibm_markers(lbm_ctx *c, double *x, double *y, double *z, double *ub, dou
   HIPCHK(hipStreamSynchronize(c->stream));
   lbm_ibm *m = c->ibm;
   const size_t nm = (size_t)m->nm;
+  if (m->ub_pending) {
+    const dim3 gm((unsigned)((m->nm + MK_WG - 1) / MK_WG));
+    if (m->ub_from_ub)
+      hipLaunchKernelGGL(k_ibm_ub_last<true>, gm, dim3(64 * MK_WG), 0, c->stream, m->nm, m->R2, m->nbr2, m->Q2,
+                         m->ub_t, m->ub_u[0], m->ub_u[1], m->ub_u[2], m->ub);
+    else
+      hipLaunchKernelGGL(k_ibm_ub_last<false>, gm, dim3(64 * MK_WG), 0, c->stream, m->nm, m->R2, m->nbr2, m->Q2,
+                         m->ub_t, m->ub_u[0], m->ub_u[1], m->ub_u[2], m->ub);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    m->ub_pending = 0;
+  }
   if (x) HIPCHK(hipMemcpy(x, m->mx, 8 * nm, hipMemcpyDeviceToHost));
   if (y) HIPCHK(hipMemcpy(y, m->my, 8 * nm, hipMemcpyDeviceToHost));
   if (z) HIPCHK(hipMemcpy(z, m->mz, 8 * nm, hipMemcpyDeviceToHost));
