@@ -42,6 +42,10 @@ struct FilArgs {
   double G[3], X0[3], e0[3];
   double *x, *xp, *T;   // 3n, 3n, n-1 (device)
   const double *F;      // 3n per unit length, or null
+  // a coupled step's output (or null): the total force sum_i F_i w_i with
+  // the nodes' weights w, and the free end's position after the step
+  const double *w;
+  double *out;
 };
 
 // parallel cyclic reduction of rows 0..m-1 (a x_{i-1} + b x_i + c x_{i+1} =
@@ -184,6 +188,30 @@ __global__ __launch_bounds__(FIL_MAX) void k_fil_step(FilArgs f, int ncalls) {
       f.xp[3 * i + d] = Xp[d];
     }
   if (i < m) f.T[i] = T;
+  if (f.out && f.F) {
+    // lane-strided partial sums over the nodes, then a wavefront tree
+    if (i < 64) {
+      double s0 = 0, s1 = 0, s2 = 0;
+      for (int k = i; k < n; k += 64) {
+        const double wk = f.w[k];
+        s0 += f.F[3 * k + 0] * wk;
+        s1 += f.F[3 * k + 1] * wk;
+        s2 += f.F[3 * k + 2] * wk;
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        s0 += __shfl_down(s0, o, 64);
+        s1 += __shfl_down(s1, o, 64);
+        s2 += __shfl_down(s2, o, 64);
+      }
+      if (i == 0) {
+        f.out[0] = s0;
+        f.out[1] = s1;
+        f.out[2] = s2;
+      }
+    }
+    if (i == n - 1)
+      for (int d = 0; d < 3; d++) f.out[3 + d] = X[d];
+  }
 }
 
 }  // namespace
@@ -222,7 +250,8 @@ void filament_state(lbm_filament *f, double **x, double **xp, int *n, double *ds
 }
 
 // device-pointer form for the coupled step: force already on the device
-int filament_step_dev(lbm_filament *f, const double *F_dev, double dt, int32_t ncalls, hipStream_t s) {
+int filament_step_dev(lbm_filament *f, const double *F_dev, double dt, int32_t ncalls, hipStream_t s,
+                      const double *w, double *out) {
   FilArgs a;
   a.n = f->d.nnodes;
   a.m = a.n - 1;
@@ -241,6 +270,8 @@ int filament_step_dev(lbm_filament *f, const double *F_dev, double dt, int32_t n
   a.xp = f->xp;
   a.T = f->T;
   a.F = F_dev;
+  a.w = w;
+  a.out = out;
   hipLaunchKernelGGL(k_fil_step, dim3(1), dim3(FIL_MAX), 0, s, a, (int)ncalls);
   return hipGetLastError() == hipSuccess ? LBM_OK : LBM_ERR_HIP;
 }
@@ -325,7 +356,7 @@ int lbm_filament_step(lbm_filament *f, const double *force, double dt, int32_t n
   if (nsteps == 0) return LBM_OK;
   FCHK(hipSetDevice(f->dev));
   if (force) FCHK(hipMemcpyAsync(f->F, force, 24 * (size_t)f->d.nnodes, hipMemcpyHostToDevice, f->stream));
-  if (int e = filament_step_dev(f, force ? f->F : nullptr, dt, nsteps, f->stream)) return e;
+  if (int e = filament_step_dev(f, force ? f->F : nullptr, dt, nsteps, f->stream, nullptr, nullptr)) return e;
   FCHK(hipStreamSynchronize(f->stream));
   return LBM_OK;
 }

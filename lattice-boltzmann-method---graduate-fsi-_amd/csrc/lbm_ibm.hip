@@ -49,7 +49,10 @@ using namespace lbm;
 
 // the filament's device state and step (lbm_filament.hip)
 void filament_state(lbm_filament *f, double **x, double **xp, int *n, double *ds, int *dev);
-int filament_step_dev(lbm_filament *f, const double *F_dev, double dt, int32_t ncalls, hipStream_t s);
+// w, out: a coupled step's output (total force with the node weights w, the
+// free end's position), or null
+int filament_step_dev(lbm_filament *f, const double *F_dev, double dt, int32_t ncalls, hipStream_t s,
+                      const double *w, double *out);
 
 namespace {
 
@@ -1361,32 +1364,6 @@ inline dim3 fsi_spread_grid(const Box &B) {
   return dim3((unsigned)((tiles + 3) / 4));
 }
 
-// Per coupled step: the total force on the filament, sum over nodes of
-// (-sum F_b) ds (lane-strided partial sums, then a wavefront tree), and the
-// free end's position.
-__global__ __launch_bounds__(64) void k_fsi_out(int nm, const double *__restrict__ fsum, const double *__restrict__ ds,
-                                                const double *__restrict__ x, double *__restrict__ out) {
-  const int l = threadIdx.x;
-  double s0 = 0, s1 = 0, s2 = 0;
-  for (int n = l; n < nm; n += 64) {
-    const double w = ds[n];
-    s0 += fsum[3 * n + 0] * w;
-    s1 += fsum[3 * n + 1] * w;
-    s2 += fsum[3 * n + 2] * w;
-  }
-  for (int o = 32; o > 0; o >>= 1) {
-    s0 += __shfl_down(s0, o, 64);
-    s1 += __shfl_down(s1, o, 64);
-    s2 += __shfl_down(s2, o, 64);
-  }
-  if (l == 0) {
-    out[0] = s0;
-    out[1] = s1;
-    out[2] = s2;
-  }
-  if (l < 3) out[3 + l] = x[3 * (nm - 1) + l];
-}
-
 __global__ void k_fill_i32(int32_t *p, int n, int32_t v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -2368,9 +2345,8 @@ int lbm_fsi_run(lbm_ctx *c, int32_t nsteps, double *out) {
       if ((e = after_stream(c)) != LBM_OK) return e;
     }
     c->state = ST_FSTAR;
-    if ((e = filament_step_dev(m->fil, m->fsum, 1.0, 1, c->stream)) != LBM_OK) return e;
-    hipLaunchKernelGGL(k_fsi_out, dim3(1), dim3(64), 0, c->stream, n, m->fsum, m->ds, fx, m->out + 6 * s);
-    HIPCHK(hipGetLastError());
+    // the filament step, which also writes the step's total force and tip
+    if ((e = filament_step_dev(m->fil, m->fsum, 1.0, 1, c->stream, m->ds, m->out + 6 * s)) != LBM_OK) return e;
   }
   {
     LaunchCtx L = launch_ctx(c);
