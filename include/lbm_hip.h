@@ -181,9 +181,10 @@ int lbm_export_links(lbm_ctx *ctx, int16_t *type, int32_t *src,
                               * counterpart): w0 = 1/4, w = 1/8, cs2 = 1/4,    *
                               * geq = w T (1 + c.u/cs2), alpha = (tau_g-1/2)/4; *
                               * walls: type_Energy TYPE_T Dirichlet at the     *
-                              * solid's T (anti-bounce-back), otherwise         *
-                              * adiabatic (bounce-back); nu, cp, j_dot_n unused *
-                              * (j_dot_n must be 0); Boussinesq adds            *
+                              * solid's T (anti-bounce-back), TYPE_Q bounce-   *
+                              * back plus j_dot_n per wall face (the heat flux *
+                              * into the fluid), otherwise adiabatic           *
+                              * (bounce-back); nu, cp unused; Boussinesq adds   *
                               * g_beta (T - T_ref) to F_y at T's time level      */
 
 typedef struct lbm_thermal_desc {

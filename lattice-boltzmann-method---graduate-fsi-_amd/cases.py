@@ -336,19 +336,20 @@ def q7_tau(alpha: float) -> float:
     return 0.5 + alpha / 0.25
 
 
-def q7_thermal(T, type_energy, tau_g, boussinesq=False, g_beta=0.0, T_ref=0.0, **params) -> Thermal:
-    return Thermal(T, type_energy, tau_g, nu=0.0, cp=1.0, boussinesq=boussinesq, g_beta=g_beta,
+def q7_thermal(T, type_energy, tau_g, boussinesq=False, g_beta=0.0, T_ref=0.0, j_dot_n=0.0, **params) -> Thermal:
+    return Thermal(T, type_energy, tau_g, nu=0.0, cp=1.0, j_dot_n=j_dot_n, boussinesq=boussinesq, g_beta=g_beta,
                    T_ref=T_ref, params=dict(model="d3q7", **params))
 
 
 def q7_conduction(Nx: int = 2, Ny: int = 8, Nz: int = 2, tau: float = 0.8, tau_g: float = 1.0,
                   Tb: float = 1.0, Tt: float = 0.0, bottom: int = TYPE_T, top: int = TYPE_T,
-                  T_fluid=None, seed: int = 4):
+                  T_fluid=None, seed: int = 4, j_dot_n: float = 0.0):
     """Fluid at rest between walls at j = 0 (T = Tb) and j = Ny+1 (T = Tt),
-    type_Energy `bottom` / `top` (TYPE_T Dirichlet, anything else adiabatic),
-    x and z periodic; the fluid starts at T_fluid (default: seeded values
-    between Tt and Tb).  With two Dirichlet walls the steady profile is
-    T_j = Tb + (Tt - Tb)(j - 1/2)/Ny (walls half-way between cells)."""
+    type_Energy `bottom` / `top` (TYPE_T Dirichlet, TYPE_Q the heat flux
+    j_dot_n into the fluid, anything else adiabatic), x and z periodic; the
+    fluid starts at T_fluid (default: seeded values between Tt and Tb).  With
+    two Dirichlet walls the steady profile is T_j = Tb + (Tt - Tb)(j - 1/2)/Ny
+    (walls half-way between cells)."""
     t, rho, ux, uy, uz = _blank(Nx, Ny, Nz)
     t[:, 0, :] = TYPE_S
     t[:, -1, :] = TYPE_S
@@ -363,7 +364,7 @@ def q7_conduction(Nx: int = 2, Ny: int = 8, Nz: int = 2, tau: float = 0.8, tau_g
     te[:, 0, :] = bottom
     te[:, -1, :] = top
     case = Case("q7_conduction", Nx, Ny, Nz, tau, t, rho, ux, uy, uz)
-    return case, q7_thermal(T, te, tau_g, Tb=Tb, Tt=Tt)
+    return case, q7_thermal(T, te, tau_g, j_dot_n=j_dot_n, Tb=Tb, Tt=Tt)
 
 
 def q7_sine(Np: int = 16, U: float = 0.0, A: float = 0.1, T0: float = 0.5, tau_g: float = 1.0,
@@ -382,17 +383,17 @@ def q7_sine(Np: int = 16, U: float = 0.0, A: float = 0.1, T0: float = 0.5, tau_g
 
 
 def q7_random_case(Nx: int, Ny: int, Nz: int, seed: int = 9, tau: float = 0.8, tau_g: float = 0.9,
-                   boussinesq: bool = True, g_beta: float = 2e-3):
+                   boussinesq: bool = True, g_beta: float = 2e-3, j_dot_n: float = 0.0):
     """random_mask_case with temperatures in [0.2, 0.8): half of the solids
-    Dirichlet (TYPE_T), half adiabatic (TYPE_Q), inlet cells freezing links,
-    and the Boussinesq force about T_ref = 0.5."""
+    Dirichlet (TYPE_T), half TYPE_Q (the heat flux j_dot_n, adiabatic at 0),
+    inlet cells freezing links, and the Boussinesq force about T_ref = 0.5."""
     case = random_mask_case(Nx, Ny, Nz, seed=seed, tau=tau)
     rng = np.random.default_rng(seed + 2000)
     T = 0.2 + 0.6 * rng.random(case.shape)
     te = np.zeros(case.shape, np.int16)
     solid = case.type == TYPE_S
     te[solid] = np.where(rng.random(case.shape) < 0.5, TYPE_T, TYPE_Q)[solid]
-    return case, q7_thermal(T, te, tau_g, boussinesq=boussinesq, g_beta=g_beta, T_ref=0.5)
+    return case, q7_thermal(T, te, tau_g, boussinesq=boussinesq, g_beta=g_beta, T_ref=0.5, j_dot_n=j_dot_n)
 
 
 def q7_rayleigh_benard(Nx: int = 48, Ny: int = 24, Nz: int = 1, Ra: float = 1e4, Pr: float = 0.71,

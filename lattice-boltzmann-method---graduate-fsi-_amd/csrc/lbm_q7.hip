@@ -13,6 +13,8 @@
 // Walls (half-way, like f): a D3Q7 link from fluid cell x to solid s = x - c_m
 // carries, by the solid's type_Energy,
 //   TYPE_T  g(x,m) = -g*(x, opp m) + 2 w_m T_s   (Dirichlet at T_s)
+//   TYPE_Q  g(x,m) =  g*(x, opp m) + j           (heat flux j into the fluid
+//                                                  through each wall face)
 //   other   g(x,m) =  g*(x, opp m)               (adiabatic)
 // written by x into the solid's slot m at its collision (the wall slots of
 // the flow populations).  Other cell types (IN, OUT, ...) freeze the link as
@@ -37,8 +39,11 @@ namespace {
 
 constexpr int Q7 = 7;
 constexpr uint32_t ALL19 = (1u << Q) - 1;
-// links bit 20 + m (fluid cells): the D3Q7 link m ends on a TYPE_T solid
+// links bit 20 + m (fluid cells): the D3Q7 link m ends on a TYPE_T solid;
+// neumann_bit(m): it ends on a TYPE_Q solid (bits 27..31, then 20, which no
+// Dirichlet link uses)
 constexpr int DIRICHLET_BIT = 20;
+__host__ __device__ constexpr uint32_t neumann_bit(int m) { return m < 6 ? 1u << (26 + m) : 1u << 20; }
 constexpr double W7[Q7] = {0.25, 0.125, 0.125, 0.125, 0.125, 0.125, 0.125};
 
 template <int N, class F>
@@ -56,6 +61,7 @@ struct G7 {
 template <class C>
 struct Q7Args {
   C omega_g, g_beta, T_ref, T0;
+  C jn;  // heat flux through a TYPE_Q wall face (lbm_thermal_desc.j_dot_n)
   C *T, *qx, *qy, *qz;
 };
 
@@ -114,6 +120,7 @@ __device__ __forceinline__ void g_wall_slots(const typename Store<S>::C (&g)[Q7]
         const uint32_t s = (mk & M_WRAP) ? wrap_src(a.g, x, y, z, CX[m], CY[m], CZ[m]) : idx - pull_off<m>(a.g);
         C v = g[OPP[m]];
         if (lk & (1u << (DIRICHLET_BIT + m))) v = C(2 * W7[m]) * (t.T[s] - t.T0) - v;
+        else if (lk & neumann_bit(m)) v = v + t.jn;
         Store<S>::st(at(out[m], s * (uint32_t)sizeof(S)), v);
       }
     }
@@ -348,6 +355,7 @@ __global__ __launch_bounds__(256) void k_q7_tables(const int16_t *__restrict__ t
   for (int m = 1; m < Q7; m++) {
     const uint32_t s = wrap_src(g, x, y, z, CX[m], CY[m], CZ[m]);
     if (type[s] == LBM_TYPE_S && typeE[s] == LBM_TYPE_T) lk |= 1u << (DIRICHLET_BIT + m);
+    if (type[s] == LBM_TYPE_S && typeE[s] == LBM_TYPE_Q) lk |= neumann_bit(m);
   }
   links[idx] = lk;
 }
@@ -424,6 +432,7 @@ Q7Args<C> q7args(const lbm_ctx *c) {
   t.g_beta = (C)(c->td.boussinesq ? c->td.g_beta : 0.0);
   t.T_ref = (C)c->td.T_ref;
   t.T0 = (C)c->T0;
+  t.jn = (C)c->td.j_dot_n;
   t.T = (C *)c->T;
   t.qx = (C *)c->qv[0];
   t.qy = (C *)c->qv[1];
@@ -537,7 +546,7 @@ int q7_download_g(lbm_ctx *c, double *g_dev) {
 
 // lbm_upload_thermal for LBM_THERMAL_D3Q7 (arguments checked by the caller)
 int q7_upload(lbm_ctx *c, const lbm_thermal_desc *d, const double *T, const int16_t *type_energy) {
-  if (!(d->tau_g > 0.5) || d->j_dot_n != 0.0) return LBM_ERR_ARG;
+  if (!(d->tau_g > 0.5) || !std::isfinite(d->j_dot_n)) return LBM_ERR_ARG;
   if ((uint64_t)c->g.ps * 26ull >= (1ull << 32)) return LBM_ERR_UNSUPPORTED;
   const size_t n = c->g.ncell;
   const uint32_t nx = c->g.nx, ny = c->g.ny, nz = c->g.nz, px = c->g.px;

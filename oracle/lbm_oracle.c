@@ -68,6 +68,7 @@ struct orc_lbm {
 /* D3Q7 temperature (declared in lbm_oracle.h with its specification) */
 typedef struct orc_q7 {
   double tau_g, g_beta, T_ref;
+  double j_dot_n; /* heat flux through a TYPE_Q wall face into the fluid */
   int boussinesq;
   double *g, *g_star; /* 7 per cell */
   double *T, *q;      /* 1 and 3 per cell */
@@ -353,6 +354,10 @@ int orc_q7_enable(orc_lbm *lb, double tau_g, int boussinesq, double g_beta, doub
   t->typeE = (int16_t *)calloc(n, sizeof(int16_t));
   lb->q7 = t;
   return 1;
+}
+
+void orc_q7_set_flux(orc_lbm *lb, double j_dot_n) {
+  if (lb->q7) lb->q7->j_dot_n = j_dot_n;
 }
 
 void orc_q7_set(orc_lbm *lb, const double *T, const int16_t *typeE) {
@@ -703,6 +708,8 @@ void orc_streaming(orc_lbm *lb) {
             } else if (nb->type == ORC_TYPE_S) {
               if (t->typeE[nn] == ORC_TYPE_T)
                 t->g[n * 7 + q7_opp[l]] = -t->g_star[n * 7 + l] + 2 * q7_w[l] * t->T[nn];
+              else if (t->typeE[nn] == ORC_TYPE_Q)
+                t->g[n * 7 + q7_opp[l]] = t->g_star[n * 7 + l] + t->j_dot_n;
               else
                 t->g[n * 7 + q7_opp[l]] = t->g_star[n * 7 + l];
             }

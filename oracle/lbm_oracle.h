@@ -204,12 +204,16 @@ double orc_geq(const orc_lbm *lb, int i, int j, int k, int l, int as_solid, doub
  *     cell's T and u of the last MacroProp;
  *   Streaming (TYPE_F x, neighbour x + c_i):  TYPE_F: g_i(nb) = g*_i(x);
  *     TYPE_S with type_Energy TYPE_T: g_opp(x) = -g*_i(x) + 2 w_i T(nb);
+ *     TYPE_S with type_Energy TYPE_Q: g_opp(x) = g*_i(x) + j_dot_n (the heat
+ *     flux into the fluid through that wall face, orc_q7_set_flux);
  *     other TYPE_S: g_opp(x) = g*_i(x);  any other type: nothing (frozen);
  *   MacroProp (TYPE_F):  T = sum g, q = sum g c, then (boussinesq)
  *     F[1] = F_y(set-up) + g_beta (T - T_ref), before u = (sum f c + F/2)/rho;
  *     Init sets F[1] the same way from the set-up T (for the first collision).
  * A solid's T stays its set-up value. */
 int orc_q7_enable(orc_lbm *lb, double tau_g, int boussinesq, double g_beta, double T_ref);
+/* the heat flux through TYPE_Q wall faces (default 0: adiabatic) */
+void orc_q7_set_flux(orc_lbm *lb, double j_dot_n);
 /* T and type_Energy of every cell, after orc_set_cells (the set-up F_y the
  * buoyancy adds to is read here) */
 void orc_q7_set(orc_lbm *lb, const double *T, const int16_t *typeE);

@@ -54,6 +54,7 @@ _SIG = {
     "orc_get_thermal": (None, [_P, _D, _D, _D, _D]),
     "orc_get_g": (None, [_P, _D]),
     "orc_q7_enable": (C.c_int, [_P, C.c_double, C.c_int, C.c_double, C.c_double]),
+    "orc_q7_set_flux": (None, [_P, C.c_double]),
     "orc_q7_set": (None, [_P, _D, C.POINTER(C.c_int16)]),
     "orc_q7_get": (None, [_P, _D, _D, _D, _D]),
     "orc_q7_get_g": (None, [_P, _D]),
@@ -212,8 +213,9 @@ class Oracle:
         self.lib.orc_set_g(self.h, _dp(g))
 
     # -- D3Q7 temperature (no reference counterpart) ----------------------------
-    def q7_enable(self, tau_g, boussinesq=False, g_beta=0.0, T_ref=0.0):
+    def q7_enable(self, tau_g, boussinesq=False, g_beta=0.0, T_ref=0.0, j_dot_n=0.0):
         assert self.lib.orc_q7_enable(self.h, float(tau_g), int(bool(boussinesq)), float(g_beta), float(T_ref))
+        self.lib.orc_q7_set_flux(self.h, float(j_dot_n))
 
     def set_q7(self, T=None, type_energy=None):
         """After set_cells (the buoyancy adds to the set-up F_y read here)."""

@@ -22,14 +22,14 @@ def _gpu(case, th, storage="f64", T0=float("nan")):
     lat = lbm_amd.Lattice(case.Nx, case.Ny, case.Nz, case.tau, storage=storage)
     lat.upload(**case.fields())
     lat.upload_thermal(th.T, th.type_energy, tau_g=th.tau_g, boussinesq=th.boussinesq, g_beta=th.g_beta,
-                       T_ref=th.T_ref, T0=T0, model="d3q7")
+                       T_ref=th.T_ref, T0=T0, j_dot_n=th.j_dot_n, model="d3q7")
     lat.init()
     return lat
 
 
 def _oracle(oracle_mod, case, th):
     o = oracle_mod.Oracle(case.Nx, case.Ny, case.Nz, case.tau)
-    o.q7_enable(th.tau_g, th.boussinesq, th.g_beta, th.T_ref)
+    o.q7_enable(th.tau_g, th.boussinesq, th.g_beta, th.T_ref, j_dot_n=th.j_dot_n)
     o.set_cells(**case.fields())
     o.set_q7(th.T, th.type_energy)
     o.init()
@@ -59,6 +59,9 @@ CASES = {
     "sine_advection": lambda: cases.q7_sine(Np=16, U=0.04, tau_g=0.8),
     "random": lambda: cases.q7_random_case(7, 6, 5, seed=9),
     "random_no_buoyancy": lambda: cases.q7_random_case(6, 5, 7, seed=13, boussinesq=False),
+    "conduction_flux_bottom": lambda: cases.q7_conduction(Nx=3, Ny=8, Nz=2, tau_g=1.1, bottom=cases.TYPE_Q,
+                                                          j_dot_n=2e-3),
+    "random_flux_walls": lambda: cases.q7_random_case(7, 6, 5, seed=17, j_dot_n=-5e-4),
     "rayleigh_benard": lambda: cases.q7_rayleigh_benard(Nx=16, Ny=8, Nz=1, Ra=5e3),
 }
 
@@ -80,7 +83,7 @@ def test_q7_fused_fp64(oracle_mod, name):
     lat.close()
 
 
-@pytest.mark.parametrize("name", ["conduction", "random", "rayleigh_benard"])
+@pytest.mark.parametrize("name", ["conduction", "random", "rayleigh_benard", "random_flux_walls"])
 def test_q7_reference_calls_fp64(oracle_mod, name):
     """Collision -> Streaming -> MacroProp as separate calls, against the
     oracle's same calls."""
@@ -174,7 +177,7 @@ def test_q7_temperature_range_error():
     lat.close()
 
 
-def test_q7_rejects_fp16_and_flux_walls():
+def test_q7_rejects_fp16():
     case, th = CASES["conduction"]()
     for st in ("f16s", "f16c"):
         lat = lbm_amd.Lattice(case.Nx, case.Ny, case.Nz, case.tau, storage=st)
@@ -183,9 +186,22 @@ def test_q7_rejects_fp16_and_flux_walls():
             lat.upload_thermal(th.T, th.type_energy, tau_g=th.tau_g, model="d3q7")
         assert e.value.code == lbm_amd.LBM_ERR_UNSUPPORTED
         lat.close()
-    lat = lbm_amd.Lattice(case.Nx, case.Ny, case.Nz, case.tau)
-    lat.upload(**case.fields())
-    with pytest.raises(lbm_amd.LBMError) as e:
-        lat.upload_thermal(th.T, th.type_energy, tau_g=th.tau_g, j_dot_n=1e-3, model="d3q7")
-    assert e.value.code == lbm_amd.LBM_ERR_ARG
+
+
+def test_q7_flux_wall_fp32_steady_profile():
+    """FP32 storage: a TYPE_Q wall letting q into the fluid against a
+    Dirichlet wall at Tt gives T_j = Tt + (q/alpha)(Ny + 1/2 - j), alpha =
+    (tau_g - 1/2)/4, to 1e-5 of the profile's span."""
+    q, tau_g = 2e-3, 1.0
+    case, th = cases.q7_conduction(Nx=3, Ny=12, Nz=2, tau_g=tau_g, Tb=0.5, Tt=0.25, bottom=cases.TYPE_Q,
+                                   j_dot_n=q)
+    lat = _gpu(case, th, storage="f32")
+    lat.step(12000)
+    assert lat.sync() == 0
+    T = lat.thermal()[0]
+    alpha = (tau_g - 0.5) / 4.0
+    j = np.arange(1, case.Ny + 1)
+    want = 0.25 + q / alpha * (case.Ny + 0.5 - j)
+    span = q / alpha * case.Ny
+    assert np.abs(T[:, 1:-1, :] - want[None, :, None]).max() < 1e-5 * span
     lat.close()

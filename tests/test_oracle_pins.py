@@ -715,7 +715,7 @@ def test_thermal_temperature_range_error(oracle_mod):
 
 def _q7_oracle(oracle_mod, case, th):
     o = oracle_mod.Oracle(case.Nx, case.Ny, case.Nz, case.tau)
-    o.q7_enable(th.tau_g, th.boussinesq, th.g_beta, th.T_ref)
+    o.q7_enable(th.tau_g, th.boussinesq, th.g_beta, th.T_ref, j_dot_n=th.j_dot_n)
     o.set_cells(**case.fields())
     o.set_q7(th.T, th.type_energy)
     o.init()
@@ -799,6 +799,58 @@ def test_q7_adiabatic_walls_conserve_heat(oracle_mod):
     assert abs(h1 - h0) < 1e-12 * h0
     # and the field did change
     assert np.abs(o.q7_thermal()[0][fl] - T[fl]).max() > 1e-3
+
+
+def test_q7_flux_wall_conduction_linear_profile(oracle_mod):
+    """A TYPE_Q wall at j = 0 letting the heat flux q into the fluid and a
+    Dirichlet wall at j = Ny+1 (T = Tt), fluid at rest: the steady profile
+    carries q through every plane, alpha dT/dy = -q, with the Dirichlet wall
+    temperature at its half-way position:
+    T_j = Tt + (q/alpha)(Ny + 1/2 - j), alpha = (tau_g - 1/2)/4."""
+    q = 2e-3
+    for tau_g in (0.8, 1.0, 1.6):
+        case, th = cases.q7_conduction(Ny=8, tau_g=tau_g, Tb=0.5, Tt=0.25, bottom=cases.TYPE_Q, j_dot_n=q)
+        o = _q7_oracle(oracle_mod, case, th)
+        assert o.run(12000) == 0
+        T = o.q7_thermal()[0]
+        alpha = (tau_g - 0.5) / 4.0
+        j = np.arange(1, case.Ny + 1)
+        want = 0.25 + q / alpha * (case.Ny + 0.5 - j)
+        assert np.abs(T[:, 1:-1, :] - want[None, :, None]).max() < 1e-10
+
+
+def test_q7_flux_walls_heat_balance(oracle_mod):
+    """A closed box of TYPE_Q walls with solid obstacles and an uneven flow:
+    every wall link of a fluid cell adds the flux j_dot_n to the heat each
+    step (the rest streams or bounces back), so the total fluid heat grows
+    by exactly n j_dot_n (wall links) over n steps; with the sign of the
+    flux reversed it falls by as much."""
+    case = cases.random_mask_case(8, 7, 6, seed=21, p_other=0.0)
+    t = case.type
+    for sl in ((0, slice(None), slice(None)), (-1, slice(None), slice(None)),
+               (slice(None), 0, slice(None)), (slice(None), -1, slice(None)),
+               (slice(None), slice(None), 0), (slice(None), slice(None), -1)):
+        t[sl] = cases.TYPE_S
+    case.ux[t == cases.TYPE_S] = 0.0
+    case.uy[t == cases.TYPE_S] = 0.0
+    case.uz[t == cases.TYPE_S] = 0.0
+    fl = t == cases.TYPE_F
+    # D3Q7 links from fluid cells to solids (the axis neighbours, periodic index)
+    links = 0
+    for ax in range(3):
+        for sgn in (1, -1):
+            links += int((fl & (np.roll(t, -sgn, axis=ax) == cases.TYPE_S)).sum())
+    rng = np.random.default_rng(3)
+    T = 0.3 + 0.4 * rng.random(case.shape)
+    te = np.full(case.shape, cases.TYPE_Q, np.int16)
+    for q in (1e-4, -1e-4):
+        th = cases.q7_thermal(T, te, 0.9, j_dot_n=q)
+        o = _q7_oracle(oracle_mod, case, th)
+        h0 = o.q7_g()[fl].sum()
+        n = 40
+        assert o.run(n) == 0
+        h1 = o.q7_thermal()[0][fl].sum()
+        assert abs(h1 - h0 - n * q * links) < 1e-12 * h0
 
 
 def test_q7_isothermal_dirichlet_box_is_a_fixed_point(oracle_mod):
