@@ -229,27 +229,36 @@ def fsi_thermal_leg(steps, cmp_steps=500, warmup=20):
     a 128-marker filament with K_b = 1e-3 and density ratio 1.5 clamped at
     its leading end, FP32 populations.  The whole coupled step per time step
     (filament geometry, IBM passes, fused D3Q19+D3Q7 step, FP64 structure
-    step), beside the fused D3Q19+D3Q7 step alone on the same lattice; then
-    FP32 against FP64 populations over `cmp_steps` steps."""
+    step), beside the fused D3Q19+D3Q7 step alone on the same lattice, and
+    the same loop with 16-bit (F16C) populations; then FP32 against FP64
+    populations over `cmp_steps` steps."""
     case, th, fil, box = cases.fsi_thermal_channel()
 
     def make(st):
         lat = lbm_amd.Lattice(case.Nx, case.Ny, case.Nz, case.tau, storage=st)
         lat.upload(**case.fields())
         lat.upload_thermal(th.T, th.type_energy, tau_g=th.tau_g, boussinesq=True, g_beta=th.g_beta,
-                           T_ref=th.T_ref, model="d3q7")
+                           T_ref=th.T_ref, T0=0.0, model="d3q7")
         lat.init()
         f = lbm_amd.Filament(**fil)
         lat.fsi_setup(f, box, m_max=5)
         return lat, f
 
-    lat, f = make("f32")
-    lat.fsi_run(warmup)
-    lat.sync()
-    t0 = time.perf_counter()
-    out = lat.fsi_run(steps)
-    code = lat.sync()
-    dt = time.perf_counter() - t0
+    def timed(st):
+        lat, f = make(st)
+        lat.fsi_run(warmup)
+        lat.sync()
+        t0 = time.perf_counter()
+        out = lat.fsi_run(steps)
+        code = lat.sync()
+        return lat, f, out, code, time.perf_counter() - t0
+
+    # the same coupled loop with 16-bit f and g (F16C)
+    l16, f16, _, code16, dt16 = timed("f16c")
+    x16, T16 = f16.get()[0], l16.thermal()[0]
+    l16.close()
+    f16.close()
+    lat, f, out, code, dt = timed("f32")
     x = f.get()[0]
     T = lat.thermal()[0]
     fl = case.type == cases.TYPE_F
@@ -260,7 +269,7 @@ def fsi_thermal_leg(steps, cmp_steps=500, warmup=20):
     plain = lbm_amd.Lattice(case.Nx, case.Ny, case.Nz, case.tau, storage="f32")
     plain.upload(**case.fields())
     plain.upload_thermal(th.T, th.type_energy, tau_g=th.tau_g, boussinesq=True, g_beta=th.g_beta,
-                         T_ref=th.T_ref, model="d3q7")
+                         T_ref=th.T_ref, T0=0.0, model="d3q7")
     plain.init()
     plain.step(warmup)
     plain_ms = plain.time_steps(200) / 200
@@ -281,6 +290,7 @@ def fsi_thermal_leg(steps, cmp_steps=500, warmup=20):
         cmp = dict(steps=cmp_steps, l2_u_f32_vs_f64=float(np.sqrt(num / den)),
                    max_dT_f32_vs_f64=float(np.abs(T32 - T64)[fl].max()),
                    max_dx_filament=float(np.abs(x32 - x64).max()))
+    ms16 = dt16 / steps * 1e3
     ms = dt / steps * 1e3
     cells = case.Nx * case.Ny * case.Nz
     return dict(
@@ -291,7 +301,11 @@ def fsi_thermal_leg(steps, cmp_steps=500, warmup=20):
         tip=[round(float(v), 4) for v in x[-1]], force_last=[float(v) for v in out[-1, :3]],
         nu_heated_wall=round(nu_wall, 3), T_mean=round(float(T[fl].mean()), 5),
         T_min=float(T[fl].min()), plain_status=plain_code,
-        finite=bool(np.isfinite(out).all()), f32_vs_f64=cmp)
+        finite=bool(np.isfinite(out).all()), f32_vs_f64=cmp,
+        f16c=dict(status=code16, ms_per_step=round(ms16, 4), mlups=round(cells * steps / dt16 / 1e6, 1),
+                  speedup_vs_f32=round(ms / ms16, 3),
+                  tip_vs_f32=float(np.abs(x16[-1] - x[-1]).max()),
+                  max_dT_vs_f32=float(np.abs(T16 - T)[fl].max())))
 
 
 def sphere_leg(steps, every=100, storage="f32", warmup=20):
@@ -510,6 +524,7 @@ def main():
     ap.add_argument("--no-cavity", action="store_true")
     ap.add_argument("--cavity-only", action="store_true", help="run only the 256^3 lid-driven cavity leg")
     ap.add_argument("--no-q7", action="store_true")
+    ap.add_argument("--q7-only", action="store_true", help="run only the D3Q7 temperature leg")
     ap.add_argument("--ibm-only", action="store_true", help="run only the IBM channel leg")
     ap.add_argument("--fsi-steps", type=int, default=10000)
     ap.add_argument("--sphere-steps", type=int, default=2000)
@@ -546,6 +561,11 @@ def main():
         return 0
     if a.fsi_only:
         print(json.dumps(fsi_thermal_leg(a.fsi_steps)), flush=True)
+        return 0
+    if a.q7_only:
+        q7 = q7_leg(a.warmup, a.steps)
+        q7["f16c"] = q7_leg(a.warmup, a.steps, storage="f16c")
+        print(json.dumps(q7), flush=True)
         return 0
     if a.sphere_only:
         print(json.dumps(sphere_leg(a.sphere_steps)), flush=True)
@@ -599,6 +619,7 @@ def main():
     if not a.no_q7:
         log("bench: D3Q7 temperature leg")
         q7 = q7_leg(a.warmup, a.steps)
+        q7["f16c"] = q7_leg(a.warmup, a.steps, storage="f16c")
         log(f"bench: d3q7 {json.dumps(q7)}")
 
     ibm = None

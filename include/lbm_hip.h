@@ -31,7 +31,9 @@ extern "C" {
 #define LBM_OK 0
 #define LBM_ERR_FEQ_NEGATIVE 1
 #define LBM_ERR_DENSITY_RANGE 2
-#define LBM_ERR_TEMPERATURE_RANGE 3
+#define LBM_ERR_TEMPERATURE_RANGE 3 /* ENERGY: T outside [0, 100]          *
+                                     * (lbm.cpp:794-797); D3Q7: |T| > 100  *
+                                     * or not finite (T relative)          */
 #define LBM_ERR_ARG (-1)
 #define LBM_ERR_STATE (-2)
 #define LBM_ERR_HIP (-3)
@@ -185,7 +187,8 @@ int lbm_export_links(lbm_ctx *ctx, int16_t *type, int32_t *src,
                               * back plus j_dot_n per wall face (the heat flux *
                               * into the fluid), otherwise adiabatic           *
                               * (bounce-back); nu, cp unused; Boussinesq adds   *
-                              * g_beta (T - T_ref) to F_y at T's time level      */
+                              * g_beta (T - T_ref) to F_y at T's time level;   *
+                              * every storage (16-bit: g - w T0 in f's units)  */
 
 typedef struct lbm_thermal_desc {
   double tau_g;     /* g relaxation time, 0.5 + alpha/cs2 (LBM.h:155)       */
@@ -195,8 +198,8 @@ typedef struct lbm_thermal_desc {
   double j_dot_n;   /* Neumann wall flux (LBM.h:161)                         */
   double g_beta;    /* Boussinesq: F_y = g_beta (T - T_ref) after MacroProp  */
   double T_ref;     /* (Tb + Tt)/2 in the reference                          */
-  double T0;        /* FP32 storage keeps g - w T0 (NaN: the mean set-up T   *
-                     * of the TYPE_F cells)                                  */
+  double T0;        /* FP32 and 16-bit storage keep g - w T0 (NaN: the mean  *
+                     * set-up T of the TYPE_F cells)                         */
   int32_t boussinesq;
   int32_t model;    /* LBM_THERMAL_ENERGY (0) or LBM_THERMAL_D3Q7            */
   int32_t reserved[2];

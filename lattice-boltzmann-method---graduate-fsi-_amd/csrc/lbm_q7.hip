@@ -20,9 +20,16 @@
 // the flow populations).  Other cell types (IN, OUT, ...) freeze the link as
 // they do for f: x keeps its own Init value in that direction.
 //
-// Buffers: 26 planes, f in 0..18, g in 19..25.  FP64 stores g; FP32 stores
-// g - w_i T0 (T0 near the mean temperature).  T and q = sum g c are written
-// for fluid cells only; a solid's T is its Dirichlet temperature.
+// T is a relative temperature here (it may be negative about T_ref): its
+// guard flags blow-up only, |T| > 100 or not finite, where the ENERGY
+// branch flags T outside [0, 100] as the reference does (lbm.cpp:794-797).
+//
+// Buffers: 26 planes, f in 0..18, g in 19..25.  FP64 stores g; the other
+// storages store g - w_i T0 (T0 near the mean temperature), the 16-bit ones
+// in the units of their f (scaled by Store<S>::K, a power of two: the scaling
+// is exact).  The kernels take g into temperature units after the loads and
+// back at the stores.  T and q = sum g c are written for fluid cells only; a
+// solid's T is its Dirichlet temperature.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
@@ -64,6 +71,23 @@ struct Q7Args {
   C jn;  // heat flux through a TYPE_Q wall face (lbm_thermal_desc.j_dot_n)
   C *T, *qx, *qy, *qz;
 };
+
+// Loaded g (storage units) into temperature units, in place.
+template <class S>
+__device__ __forceinline__ void g_units(typename Store<S>::C (&g)[Q7]) {
+  using C = typename Store<S>::C;
+  if constexpr (Store<S>::K != C(1)) {
+    constexpr C inv = C(1) / Store<S>::K;
+#pragma unroll
+    for (int m = 0; m < Q7; m++) g[m] = g[m] * inv;
+  }
+}
+
+// Store of one g value given in temperature units.
+template <class S>
+__device__ __forceinline__ void g_st(S *p, typename Store<S>::C v) {
+  Store<S>::st(p, v * Store<S>::K);
+}
 
 template <class C>
 __device__ __forceinline__ uint32_t after_loads7(uint32_t mk, const C (&g)[Q7]) {
@@ -121,7 +145,7 @@ __device__ __forceinline__ void g_wall_slots(const typename Store<S>::C (&g)[Q7]
         C v = g[OPP[m]];
         if (lk & (1u << (DIRICHLET_BIT + m))) v = C(2 * W7[m]) * (t.T[s] - t.T0) - v;
         else if (lk & neumann_bit(m)) v = v + t.jn;
-        Store<S>::st(at(out[m], s * (uint32_t)sizeof(S)), v);
+        g_st<S>(at(out[m], s * (uint32_t)sizeof(S)), v);
       }
     }
   });
@@ -146,6 +170,7 @@ __global__ __launch_bounds__(256) void k_q7(const Planes<S> P, const G7<S> G, co
   }
   mk = after_loads(mk, f);
   mk = after_loads7(mk, g);
+  g_units<S>(g);
   uint32_t st = ALL19;
   if (mk & M_FLUID) {
     if (mk & M_BACKEQ) patch_cell<S, true>(P, (uint8_t)mk, idx, a, f);
@@ -169,7 +194,7 @@ __global__ __launch_bounds__(256) void k_q7(const Planes<S> P, const G7<S> G, co
       if constexpr (FORCE != 0 || BQ) velocity(jx + C(0.5) * Fx, jy + C(0.5) * Fy, jz + C(0.5) * Fz, rho, ux, uy, uz);
       else velocity(jx, jy, jz, rho, ux, uy, uz);
       bad = (rho < C(0) || rho > C(100)) ? ST_RHO_RANGE : 0;
-      if (T < C(0) || T > C(100)) bad |= ST_T_RANGE;
+      if (!(fabs(T) <= C(100))) bad |= ST_T_RANGE;
     }
     if constexpr (EMIT) {
       a.rho[idx] = rho;
@@ -202,7 +227,7 @@ __global__ __launch_bounds__(256) void k_q7(const Planes<S> P, const G7<S> G, co
     });
     for_n<Q7>([&](auto M) {
       constexpr int m = decltype(M)::value;
-      Store<S>::st(at(G.out[m], boff), g[m]);
+      g_st<S>(at(G.out[m], boff), g[m]);
     });
   } else {
     for_dirs([&](auto M) {
@@ -211,7 +236,7 @@ __global__ __launch_bounds__(256) void k_q7(const Planes<S> P, const G7<S> G, co
     });
     for_n<Q7>([&](auto M) {
       constexpr int m = decltype(M)::value;
-      if (st & (1u << m)) Store<S>::st(at(G.out[m], boff), g[m]);
+      if (st & (1u << m)) g_st<S>(at(G.out[m], boff), g[m]);
     });
   }
   if constexpr (COLLIDE) {
@@ -237,6 +262,7 @@ __global__ __launch_bounds__(256) void k_q7_collide(S *__restrict__ buf, const u
   C f[Q], g[Q7];
   for (int m = 0; m < Q; m++) f[m] = Store<S>::ld(buf + (uint32_t)m * ps + idx);
   for (int m = 0; m < Q7; m++) g[m] = Store<S>::ld(buf + (uint32_t)(Q + m) * ps + idx);
+  g_units<S>(g);
   C rho, drho, ux, uy, uz, T, Fx, Fy, Fz;
   if constexpr (FORCE == 3) {
     // the IBM step: as k_collide's mode 3, with T from the cell's own g
@@ -268,7 +294,7 @@ __global__ __launch_bounds__(256) void k_q7_collide(S *__restrict__ buf, const u
   const bool neg = relax<S, (FORCE != 0 || BQ) ? 2 : 0>(f, rho, drho, ux, uy, uz, Fx, Fy, Fz, a.omega);
   g_relax(g, T, T - t.T0, ux, uy, uz, t.omega_g);
   for (int m = 0; m < Q; m++) Store<S>::st(buf + (uint32_t)m * ps + idx, f[m]);
-  for (int m = 0; m < Q7; m++) Store<S>::st(buf + (uint32_t)(Q + m) * ps + idx, g[m]);
+  for (int m = 0; m < Q7; m++) g_st<S>(buf + (uint32_t)(Q + m) * ps + idx, g[m]);
   if (mk & M_WALL) {
     const uint32_t lk = a.links[idx];
     S *fp[Q], *gp[Q7];
@@ -294,6 +320,7 @@ __global__ __launch_bounds__(256) void k_q7_macro(const S *__restrict__ buf, con
   C f[Q], g[Q7];
   for (int m = 0; m < Q; m++) f[m] = Store<S>::ld(buf + (uint32_t)m * ps + idx);
   for (int m = 0; m < Q7; m++) g[m] = Store<S>::ld(buf + (uint32_t)(Q + m) * ps + idx);
+  g_units<S>(g);
   const C T = t.T0 + g_sum(g);
   C rho, drho, jx, jy, jz, Fx, Fy, Fz, ux, uy, uz;
   moments<S>(f, rho, drho, jx, jy, jz);
@@ -308,7 +335,7 @@ __global__ __launch_bounds__(256) void k_q7_macro(const S *__restrict__ buf, con
   t.qy[idx] = g[3] - g[4];
   t.qz[idx] = g[5] - g[6];
   int bad = (rho < C(0) || rho > C(100)) ? ST_RHO_RANGE : 0;
-  if (T < C(0) || T > C(100)) bad |= ST_T_RANGE;
+  if (!(fabs(T) <= C(100))) bad |= ST_T_RANGE;
   if (bad) flag(a.status, a.first_bad, bad, idx);
 }
 
@@ -336,7 +363,7 @@ __global__ __launch_bounds__(256) void k_q7_init(S *__restrict__ A, S *__restric
     const C T = t.T[c];
     const C cu = C(CX[m]) * a.ux[c] + C(CY[m]) * a.uy[c] + C(CZ[m]) * a.uz[c];
     const C v = C(W7[m]) * (T - t.T0) + C(W7[m] * 4.0) * T * cu;
-    const S e = Store<S>::enc(v);
+    const S e = Store<S>::enc(v * Store<S>::K);
     A[(uint32_t)(Q + m) * ps + idx] = e;
     B[(uint32_t)(Q + m) * ps + idx] = e;
   });
@@ -365,8 +392,11 @@ template <class S>
 __global__ __launch_bounds__(256) void k_g7_out(const S *__restrict__ buf, Geo g, double T0, double *__restrict__ out) {
   const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= g.ncell) return;
-  for (int m = 0; m < Q7; m++)
-    out[(size_t)idx * Q7 + m] = (double)Store<S>::ld(buf + (uint32_t)(Q + m) * g.ps + idx) + W7[m] * T0;
+  using C = typename Store<S>::C;
+  C v[Q7];
+  for (int m = 0; m < Q7; m++) v[m] = Store<S>::ld(buf + (uint32_t)(Q + m) * g.ps + idx);
+  g_units<S>(v);
+  for (int m = 0; m < Q7; m++) out[(size_t)idx * Q7 + m] = (double)v[m] + W7[m] * T0;
 }
 
 // ---- host -----------------------------------------------------------------
@@ -491,6 +521,19 @@ void launch_s7(lbm_ctx *c, int mode, int fm) {
   else launch_b<S, false>(c, mode, fm);
 }
 
+template <class S>
+void init_s7(lbm_ctx *c) {
+  using C = typename Store<S>::C;
+  const LaunchCtx L = launch_ctx(c);
+  hipLaunchKernelGGL(k_q7_init<S>, grid_of(c), dim3(c->block), 0, c->stream, (S *)c->cur, (S *)c->oth, c->mask,
+                     kargs7<C>(L), q7args<C>(c));
+}
+
+template <class S>
+void g_out_s7(lbm_ctx *c, double *g_dev) {
+  hipLaunchKernelGGL(k_g7_out<S>, grid_of(c), dim3(c->block), 0, c->stream, (const S *)c->cur, c->g, c->T0, g_dev);
+}
+
 int upload_field(lbm_ctx *c, void *dst, const std::vector<double> &h) {
   const size_t n = c->g.ncell;
   if (c->csize == 8) {
@@ -506,8 +549,12 @@ int upload_field(lbm_ctx *c, void *dst, const std::vector<double> &h) {
 
 int q7_launch(lbm_ctx *c, int mode, int force_mode) {
   const int fm = force_mode < 0 ? c->force_mode : force_mode;
-  if (c->storage == LBM_STORE_F64) launch_s7<double>(c, mode, fm);
-  else launch_s7<float>(c, mode, fm);
+  switch (c->storage) {
+    case LBM_STORE_F64: launch_s7<double>(c, mode, fm); break;
+    case LBM_STORE_F16S: launch_s7<__half>(c, mode, fm); break;
+    case LBM_STORE_F16C: launch_s7<f16c_t>(c, mode, fm); break;
+    default: launch_s7<float>(c, mode, fm); break;
+  }
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }
@@ -522,24 +569,23 @@ int q7_tables(lbm_ctx *c) {
 
 int q7_init(lbm_ctx *c) {
   if (int e = q7_tables(c)) return e;
-  const LaunchCtx L = launch_ctx(c);
-  if (c->storage == LBM_STORE_F64)
-    hipLaunchKernelGGL(k_q7_init<double>, grid_of(c), dim3(c->block), 0, c->stream, (double *)c->cur,
-                       (double *)c->oth, c->mask, kargs7<double>(L), q7args<double>(c));
-  else
-    hipLaunchKernelGGL(k_q7_init<float>, grid_of(c), dim3(c->block), 0, c->stream, (float *)c->cur,
-                       (float *)c->oth, c->mask, kargs7<float>(L), q7args<float>(c));
+  switch (c->storage) {
+    case LBM_STORE_F64: init_s7<double>(c); break;
+    case LBM_STORE_F16S: init_s7<__half>(c); break;
+    case LBM_STORE_F16C: init_s7<f16c_t>(c); break;
+    default: init_s7<float>(c); break;
+  }
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }
 
 int q7_download_g(lbm_ctx *c, double *g_dev) {
-  if (c->storage == LBM_STORE_F64)
-    hipLaunchKernelGGL(k_g7_out<double>, grid_of(c), dim3(c->block), 0, c->stream, (const double *)c->cur, c->g,
-                       c->T0, g_dev);
-  else
-    hipLaunchKernelGGL(k_g7_out<float>, grid_of(c), dim3(c->block), 0, c->stream, (const float *)c->cur, c->g,
-                       c->T0, g_dev);
+  switch (c->storage) {
+    case LBM_STORE_F64: g_out_s7<double>(c, g_dev); break;
+    case LBM_STORE_F16S: g_out_s7<__half>(c, g_dev); break;
+    case LBM_STORE_F16C: g_out_s7<f16c_t>(c, g_dev); break;
+    default: g_out_s7<float>(c, g_dev); break;
+  }
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }

@@ -740,9 +740,11 @@ int lbm_upload_thermal(lbm_ctx *c, const lbm_thermal_desc *d, const double *T,
                        const int16_t *type_energy) {
   if (!c || !d) return LBM_ERR_ARG;
   if (!c->uploaded || c->state != ST_NONE) return LBM_ERR_STATE;
-  if (c->storage == LBM_STORE_F16S || c->storage == LBM_STORE_F16C) return LBM_ERR_UNSUPPORTED;
   if (c->ibm) return LBM_ERR_UNSUPPORTED;
   if (d->model != LBM_THERMAL_ENERGY && d->model != LBM_THERMAL_D3Q7) return LBM_ERR_ARG;
+  // the ENERGY branch keeps FP64 or FP32 populations; D3Q7 takes every storage
+  if (d->model == LBM_THERMAL_ENERGY && (c->storage == LBM_STORE_F16S || c->storage == LBM_STORE_F16C))
+    return LBM_ERR_UNSUPPORTED;
   if (d->model == LBM_THERMAL_D3Q7) {
     HIPCHK(hipSetDevice(c->dev));
     HIPCHK(hipStreamSynchronize(c->stream));

@@ -847,7 +847,9 @@ int orc_macroprop(orc_lbm *lb) {
           t->q[3 * n + 0] = qx;
           t->q[3 * n + 1] = qy;
           t->q[3 * n + 2] = qz;
-          if (T < 0 || T > 100) bad |= 2;
+          /* a D3Q7 T may be a relative temperature (negative about T_ref):
+             the guard catches blow-up only, |T| > 100 or not finite */
+          if (!(fabs(T) <= 100)) bad |= 2;
           if (t->boussinesq) c->F[1] = t->F0y[n] + t->g_beta * (T - t->T_ref);
         }
         c->ux = (rho_ux + 0.5 * ORC_DT * c->F[0]) / rho;

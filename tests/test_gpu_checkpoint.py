@@ -108,9 +108,10 @@ def test_resume_energy_thermal():
     _resume(lambda: _energy(case, th, "f64"), 15, 15, thermal=True)
 
 
-def test_resume_d3q7():
+@pytest.mark.parametrize("storage", ["f32", "f16c"])
+def test_resume_d3q7(storage):
     case, th = cases.q7_random_case(7, 6, 5, seed=9)
-    _resume(lambda: _q7(case, th, "f32"), 12, 12, thermal=True)
+    _resume(lambda: _q7(case, th, storage), 12, 12, thermal=True)
 
 
 def test_checkpoint_rejects_mismatch_and_ibm():

@@ -142,7 +142,7 @@ def _setup_c3(case, th, fil, box, storage):
     lat = lbm_amd.Lattice(case.Nx, case.Ny, case.Nz, case.tau, storage=storage)
     lat.upload(**case.fields())
     lat.upload_thermal(th.T, th.type_energy, tau_g=th.tau_g, boussinesq=th.boussinesq, g_beta=th.g_beta,
-                       T_ref=th.T_ref, model="d3q7")
+                       T_ref=th.T_ref, T0=0.0, model="d3q7")
     lat.init()
     f = lbm_amd.Filament(**fil)
     lat.fsi_setup(f, box, m_max=5)
@@ -194,7 +194,35 @@ def test_fsi_thermal_channel_3d_fp32_vs_fp64():
     # the filament bent downstream and the heated wall warmed the fluid
     assert np.abs(x64 - (np.array(fil["clamp"]) + np.outer(np.linspace(0, fil["length"], fil["n"]),
                                                             fil["tangent"]))).max() > 1e-2
-    assert T64[fl].max() > 0.1
+    assert T64[fl].mean() > th.T[fl].mean() + 1e-3
+    # the smooth start keeps T inside the walls' range
+    assert T64[fl].min() >= 0.0 and T32[fl].min() >= 0.0 and T64[fl].max() <= 1.0
+
+
+@pytest.mark.parametrize("storage,tol_x,tol_u,tol_T", [("f16s", 5e-3, 3e-3, 3e-3), ("f16c", 3e-3, 1e-3, 1.5e-3)])
+def test_fsi_thermal_channel_3d_fp16_vs_fp64(storage, tol_x, tol_u, tol_T):
+    """The coupled run over 400 steps with 16-bit f and g (the IBM, the
+    buoyancy and the filament in FP32/FP64 arithmetic) against FP64
+    populations: the filament, u and T within the stated bars."""
+    case, th, fil, box = cases.fsi_thermal_channel(40, 20, 20, Re=20.0, Ra=1e5, n_markers=32)
+    res = {}
+    for st in ("f64", storage):
+        lat, f = _setup_c3(case, th, fil, box, st)
+        out = lat.fsi_run(400)
+        assert lat.sync() == 0
+        res[st] = (f.get()[0], lat.macro(), lat.thermal()[0], out)
+        lat.close()
+        f.close()
+    (x64, m64, T64, _), (x16, m16, T16, o16) = res["f64"], res[storage]
+    fl = case.type == cases.TYPE_F
+    assert np.isfinite(o16).all()
+    ex = float(np.abs(x16 - x64).max())
+    eu = rel_l2_u(m16[1:], m64[1:], fl)
+    eT = float(np.abs(T16 - T64)[fl].max())
+    print(f"fsi heated channel {storage}: max|dx| {ex:.3e}, rel L2(u) {eu:.3e}, max|dT| {eT:.3e}")
+    assert ex < tol_x
+    assert eu < tol_u
+    assert eT < tol_T
 
 
 def test_fsi_long_run_stays_inextensible():

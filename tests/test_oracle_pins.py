@@ -774,6 +774,24 @@ def test_q7_dirichlet_conduction_linear_profile(oracle_mod):
         assert np.abs(T[:, 1:-1, :] - want[None, :, None]).max() < 1e-10
 
 
+def test_q7_relative_temperatures_and_blow_up_guard(oracle_mod):
+    """D3Q7 T is relative: walls at +0.5 and -0.5 run without a range error
+    to the linear profile through 0 (the ENERGY branch's T < 0 stop,
+    lbm.cpp:794-797, does not apply); |T| > 100 of either sign is flagged
+    at the first MacroProp."""
+    case, th = cases.q7_conduction(Ny=8, tau_g=1.0, Tb=0.5, Tt=-0.5)
+    o = _q7_oracle(oracle_mod, case, th)
+    assert o.run(6000) == 0
+    T = o.q7_thermal()[0]
+    j = np.arange(1, case.Ny + 1)
+    want = 0.5 - (j - 0.5) / case.Ny
+    assert np.abs(T[:, 1:-1, :] - want[None, :, None]).max() < 1e-10
+    for big in (150.0, -150.0):
+        case, th = cases.q7_conduction(Tb=big, Tt=big, T_fluid=big)
+        o = _q7_oracle(oracle_mod, case, th)
+        assert o.run(1) == 3  # ORC_ERR_TEMPERATURE_RANGE
+
+
 def test_q7_adiabatic_walls_conserve_heat(oracle_mod):
     """A closed box with adiabatic (bounce-back) walls and solid obstacles,
     a moving, uneven flow: the total fluid heat sum T is constant (every g
