@@ -175,7 +175,8 @@ int lbm_export_links(lbm_ctx *ctx, int16_t *type, int32_t *src,
  * and the optional Boussinesq force of Rayleigh_Benard (lbm.cpp:803-806).
  * The face loops of Streaming move g as they move f (outlet copies, inlet
  * g = rho w T (1 + ...), the k = 0 inflow write into cell (i,j,Nz+1),
- * lbm.cpp:309-313, 340-346, 613-619).  FP64 and FP32 storage. */
+ * lbm.cpp:309-313, 340-346, 613-619).  Every storage: FP32 keeps g - w T0,
+ * the 16-bit ones (g - w T0) in the units of their f. */
 /* lbm_thermal_desc.model */
 #define LBM_THERMAL_ENERGY 0 /* the reference's ENERGY branch: D3Q19 g on the *
                               * same lattice, viscous heating                  */

@@ -20,7 +20,8 @@
 // collision's Phi and u, which are kept per cell (4 values instead of 19;
 // the same expression on the same inputs gives the same F_vis).
 //
-// Storage: FP64 holds g; FP32 holds g - w T0 (T0 near the mean temperature)
+// Storage: FP64 holds g; FP32 holds g - w T0 (T0 near the mean temperature),
+// the 16-bit storages (g - w T0) K in the units of their f
 // so that small temperature differences keep their digits.
 //
 // The face loops' g (lbm_faces.hip) reach the fluid through the back-face
@@ -65,19 +66,22 @@ struct ThArgs {
   C *Fy;  // Boussinesq: per-cell F_y, rewritten after each MacroProp
 };
 
-// g storage: FP64 raw = g; FP32 raw = g - w T0
+// g storage: FP64 raw = g; FP32 raw = g - w T0; 16-bit raw = (g - w T0) K,
+// the units of f (K a power of two: exact both ways)
 template <class S>
 __device__ __forceinline__ typename Store<S>::C g_val(typename Store<S>::C raw, double w,
                                                       typename Store<S>::C T0) {
   using C = typename Store<S>::C;
-  if constexpr (Store<S>::shifted) return raw + C(w) * T0;
+  if constexpr (Store<S>::K != C(1)) return raw * C(1.0 / Store<S>::K) + C(w) * T0;
+  else if constexpr (Store<S>::shifted) return raw + C(w) * T0;
   else return raw;
 }
 template <class S>
 __device__ __forceinline__ typename Store<S>::C g_raw(typename Store<S>::C val, double w,
                                                       typename Store<S>::C T0) {
   using C = typename Store<S>::C;
-  if constexpr (Store<S>::shifted) return val - C(w) * T0;
+  if constexpr (Store<S>::K != C(1)) return (val - C(w) * T0) * C(Store<S>::K);
+  else if constexpr (Store<S>::shifted) return val - C(w) * T0;
   else return val;
 }
 
@@ -489,7 +493,7 @@ __global__ __launch_bounds__(256) void k_th_face_src(const Planes<S> Pf, const P
     double fv = (double)f[m], gv = (double)g[m];
     if constexpr (Store<S>::shifted) {
       fv = fv / (double)Store<S>::K + W[m];
-      gv = gv + W[m] * (double)t.T0;
+      gv = gv / (double)Store<S>::K + W[m] * (double)t.T0;
     }
     of[(size_t)m * ns + slot] = fv;
     of[(size_t)(Q + m) * ns + slot] = gv;
@@ -634,6 +638,21 @@ void launch_s(lbm_ctx *c, int mode, uint8_t sel) {
   }
 }
 
+// f(Tag<S>{}) with S the context's storage type
+template <class S>
+struct Tag {
+  using type = S;
+};
+template <class F>
+void with_storage(const lbm_ctx *c, F &&f) {
+  switch (c->storage) {
+    case LBM_STORE_F64: f(Tag<double>{}); break;
+    case LBM_STORE_F16S: f(Tag<__half>{}); break;
+    case LBM_STORE_F16C: f(Tag<f16c_t>{}); break;
+    default: f(Tag<float>{}); break;
+  }
+}
+
 int upload_cells_c(lbm_ctx *c, void *dst, const std::vector<double> &h) {
   const size_t n = c->g.ncell;
   if (c->csize == 8) {
@@ -677,30 +696,28 @@ int thermal_init(lbm_ctx *c) {
   int e = thermal_tables(c);
   if (e != LBM_OK) return e;
   const LaunchCtx L = launch_ctx(c);
-  if (c->storage == LBM_STORE_F64)
-    hipLaunchKernelGGL(k_th_init<double>, grid_of(c), dim3(c->block), 0, c->stream, (double *)c->cur, (double *)c->oth,
-                       c->mask, c->type, kargs<double>(L), thargs<double>(c));
-  else
-    hipLaunchKernelGGL(k_th_init<float>, grid_of(c), dim3(c->block), 0, c->stream, (float *)c->cur, (float *)c->oth,
-                       c->mask, c->type, kargs<float>(L), thargs<float>(c));
+  with_storage(c, [&](auto tag) {
+    using S = typename decltype(tag)::type;
+    hipLaunchKernelGGL(k_th_init<S>, grid_of(c), dim3(c->block), 0, c->stream, (S *)c->cur, (S *)c->oth, c->mask,
+                       c->type, kargs<S>(L), thargs<S>(c));
+  });
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }
 
 int thermal_launch(lbm_ctx *c, int mode, uint8_t sel) {
-  if (c->storage == LBM_STORE_F64) launch_s<double>(c, mode, sel);
-  else launch_s<float>(c, mode, sel);
+  with_storage(c, [&](auto tag) { launch_s<typename decltype(tag)::type>(c, mode, sel); });
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }
 
 int thermal_download_g(lbm_ctx *c, double *g_dev) {
-  if (c->storage == LBM_STORE_F64)
-    hipLaunchKernelGGL(k_g_out<double>, grid_of(c), dim3(c->block), 0, c->stream, (const double *)c->cur, c->g,
-                       (double)c->T0, g_dev);
-  else
-    hipLaunchKernelGGL(k_g_out<float>, grid_of(c), dim3(c->block), 0, c->stream, (const float *)c->cur, c->g,
-                       (float)c->T0, g_dev);
+  with_storage(c, [&](auto tag) {
+    using S = typename decltype(tag)::type;
+    using C = typename Store<S>::C;
+    hipLaunchKernelGGL(k_g_out<S>, grid_of(c), dim3(c->block), 0, c->stream, (const S *)c->cur, c->g, (C)c->T0,
+                       g_dev);
+  });
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }
@@ -715,8 +732,7 @@ void face_src_s(lbm_ctx *c) {
 
 int thermal_face_sources(lbm_ctx *c) {
   if (c->thermal != 1 || !c->fnsrc) return LBM_OK;
-  if (c->storage == LBM_STORE_F64) face_src_s<double>(c);
-  else face_src_s<float>(c);
+  with_storage(c, [&](auto tag) { face_src_s<typename decltype(tag)::type>(c); });
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }
@@ -724,12 +740,11 @@ int thermal_face_sources(lbm_ctx *c) {
 int thermal_backeq_solid(lbm_ctx *c) {
   if (c->thermal != 1) return LBM_OK;
   const LaunchCtx L = launch_ctx(c);
-  if (c->storage == LBM_STORE_F64)
-    hipLaunchKernelGGL(k_th_backeq_solid<double>, grid_of(c), dim3(c->block), 0, c->stream, (double *)c->cur,
-                       (double *)c->oth, c->mask, kargs<double>(L), thargs<double>(c));
-  else
-    hipLaunchKernelGGL(k_th_backeq_solid<float>, grid_of(c), dim3(c->block), 0, c->stream, (float *)c->cur,
-                       (float *)c->oth, c->mask, kargs<float>(L), thargs<float>(c));
+  with_storage(c, [&](auto tag) {
+    using S = typename decltype(tag)::type;
+    hipLaunchKernelGGL(k_th_backeq_solid<S>, grid_of(c), dim3(c->block), 0, c->stream, (S *)c->cur, (S *)c->oth,
+                       c->mask, kargs<S>(L), thargs<S>(c));
+  });
   HIPCHK(hipGetLastError());
   return LBM_OK;
 }
@@ -742,9 +757,6 @@ int lbm_upload_thermal(lbm_ctx *c, const lbm_thermal_desc *d, const double *T,
   if (!c->uploaded || c->state != ST_NONE) return LBM_ERR_STATE;
   if (c->ibm) return LBM_ERR_UNSUPPORTED;
   if (d->model != LBM_THERMAL_ENERGY && d->model != LBM_THERMAL_D3Q7) return LBM_ERR_ARG;
-  // the ENERGY branch keeps FP64 or FP32 populations; D3Q7 takes every storage
-  if (d->model == LBM_THERMAL_ENERGY && (c->storage == LBM_STORE_F16S || c->storage == LBM_STORE_F16C))
-    return LBM_ERR_UNSUPPORTED;
   if (d->model == LBM_THERMAL_D3Q7) {
     HIPCHK(hipSetDevice(c->dev));
     HIPCHK(hipStreamSynchronize(c->stream));

@@ -101,11 +101,12 @@ def test_resume_forced_periodic():
     _resume(lambda: _plain(case, "f64"), 25, 25)
 
 
-def test_resume_energy_thermal():
+@pytest.mark.parametrize("storage", ["f64", "f16c"])
+def test_resume_energy_thermal(storage):
     """The ENERGY branch: g, T, q and the previous collision's Phi and u
     (its dF/dt history) carried across."""
     case, th = cases.couette_thermal(Ny=12, Re=10.0)
-    _resume(lambda: _energy(case, th, "f64"), 15, 15, thermal=True)
+    _resume(lambda: _energy(case, th, storage), 15, 15, thermal=True)
 
 
 @pytest.mark.parametrize("storage", ["f32", "f16c"])
