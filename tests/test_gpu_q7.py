@@ -242,3 +242,39 @@ def test_q7_flux_wall_fp32_steady_profile():
     span = q / alpha * case.Ny
     assert np.abs(T[:, 1:-1, :] - want[None, :, None]).max() < 1e-5 * span
     lat.close()
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("storage,tol", [("f64", 1e-13), ("f32", 1e-9), ("f16s", 5e-6), ("f16c", 1e-6)])
+def test_q7_tgv256_heat_and_mass_conserved(storage, tol):
+    """At the benchmark's full size (Taylor-Green 256^3 carrying T = 0.5 +
+    0.1 sin sin sin, Boussinesq on, periodic): streaming moves every g and
+    f to another cell and the collisions keep each cell's T and rho, so
+    sum T and sum rho over the lattice are invariants of the exact scheme;
+    after 200 steps they hold to `tol` relative (the storage's rounding),
+    and the T field stays within its initial range."""
+    n = 256
+    case = cases.taylor_green(n, u0=0.04, Re=1600.0)
+    k = 2 * np.pi / n
+    x = np.arange(n, dtype=np.float64)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    T = 0.5 + 0.1 * np.sin(k * X) * np.sin(k * Y) * np.sin(k * Z)
+    lat = lbm_amd.Lattice(case.Nx, case.Ny, case.Nz, case.tau, storage=storage)
+    lat.upload(**case.fields())
+    lat.upload_thermal(T, None, tau_g=0.6, boussinesq=True, g_beta=1e-5, T_ref=0.5, model="d3q7")
+    lat.init()
+    lat.step(1)
+    assert lat.sync() == 0
+    rho0 = float(lat.macro()[0].sum(dtype=np.float64))
+    T0 = float(lat.thermal()[0].sum(dtype=np.float64))
+    lat.step(200)
+    assert lat.sync() == 0
+    rho1 = float(lat.macro()[0].sum(dtype=np.float64))
+    Tn = lat.thermal()[0]
+    T1 = float(Tn.sum(dtype=np.float64))
+    print(f"d3q7 256^3 {storage}: sum T drift {abs(T1 / T0 - 1):.3e}, sum rho drift {abs(rho1 / rho0 - 1):.3e}")
+    assert abs(T1 / T0 - 1) < tol
+    assert abs(rho1 / rho0 - 1) < tol
+    assert Tn.min() > 0.4 - 1e-3 and Tn.max() < 0.6 + 1e-3
+    lat.close()
+
