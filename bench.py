@@ -157,7 +157,8 @@ def ibm_channel(Nx=256, Ny=128, Nz=64, D=20.0, u_in=0.04, Re=100.0, n_theta=64):
 def ibm_leg(steps, cmp_steps, warmup=20):
     """The IBM channel: FP32 ms/step of the full IBM step (lbm_ibm_run)
     beside the plain fused step on the same lattice, C_D and C_L after
-    `steps` steps, and FP32 against FP64 storage after `cmp_steps`."""
+    `steps` steps, the same run with F16C populations, and FP32 against
+    FP64 storage after `cmp_steps`."""
     case, mk, box = ibm_channel()
     D, u_in, Nz = 20.0, 0.04, case.Nz
     q = 0.5 * 1.0 * u_in ** 2 * D * Nz  # C = F / (1/2 rho0 u^2 D Nz)
@@ -169,15 +170,20 @@ def ibm_leg(steps, cmp_steps, warmup=20):
         lat.ibm_setup(mk, box, m_max=20)
         return lat
 
-    lat = make("f32")
-    lat.ibm_run(warmup)
-    lat.sync()
-    t0 = time.perf_counter()
-    out = lat.ibm_run(steps)
-    code = lat.sync()
-    dt = time.perf_counter() - t0
-    rho, ux, uy, uz = lat.macro()
-    lat.close()
+    def timed(st):
+        lat = make(st)
+        lat.ibm_run(warmup)
+        lat.sync()
+        t0 = time.perf_counter()
+        out = lat.ibm_run(steps)
+        code = lat.sync()
+        dt = time.perf_counter() - t0
+        m = lat.macro()
+        lat.close()
+        return out, code, dt, m
+
+    out16, code16, dt16, _ = timed("f16c")
+    out, code, dt, (rho, ux, uy, uz) = timed("f32")
     plain = lbm_amd.Lattice(case.Nx, case.Ny, case.Nz, case.tau, storage="f32")
     plain.upload(**case.fields())
     plain.init()
@@ -219,7 +225,9 @@ def ibm_leg(steps, cmp_steps, warmup=20):
         cd_last=float(out[-1, 0] / q), cl_last=float(out[-1, 1] / q),
         cd_mean_last1000=float(tail[:, 0].mean() / q), cl_rms_last1000=float(np.sqrt((tail[:, 1] ** 2).mean()) / q),
         finite=bool(np.isfinite(out).all() and np.isfinite(ux).all()),
-        f32_vs_f64=cmp)
+        f32_vs_f64=cmp,
+        f16c=dict(status=code16, ms_per_step=round(dt16 / steps * 1e3, 4), speedup_vs_f32=round(dt / dt16, 3),
+                  cd_last=float(out16[-1, 0] / q), cl_last=float(out16[-1, 1] / q)))
 
 
 def fsi_thermal_leg(steps, cmp_steps=500, warmup=20):
