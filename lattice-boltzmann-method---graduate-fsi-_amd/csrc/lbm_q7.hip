@@ -35,6 +35,7 @@
 #include <string.h>
 
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "lbm_ctx.h"
@@ -71,6 +72,14 @@ struct Q7Args {
   C jn;  // heat flux through a TYPE_Q wall face (lbm_thermal_desc.j_dot_n)
   C *T, *qx, *qy, *qz;
 };
+
+// F16C holds |g - w T0| < 0.5, at the rest weight 1/4 |T - T0| < 2: a cell
+// past 1.9 flags the temperature range rather than let its g saturate.
+template <class S>
+__device__ __forceinline__ bool beyond_storage(typename Store<S>::C dT) {
+  if constexpr (std::is_same<S, f16c_t>::value) return !(fabs(dT) < 1.9f);
+  else return false;
+}
 
 // Loaded g (storage units) into temperature units, in place.
 template <class S>
@@ -194,7 +203,7 @@ __global__ __launch_bounds__(256) void k_q7(const Planes<S> P, const G7<S> G, co
       if constexpr (FORCE != 0 || BQ) velocity(jx + C(0.5) * Fx, jy + C(0.5) * Fy, jz + C(0.5) * Fz, rho, ux, uy, uz);
       else velocity(jx, jy, jz, rho, ux, uy, uz);
       bad = (rho < C(0) || rho > C(100)) ? ST_RHO_RANGE : 0;
-      if (!(fabs(T) <= C(100))) bad |= ST_T_RANGE;
+      if (!(fabs(T) <= C(100)) || beyond_storage<S>(dT)) bad |= ST_T_RANGE;
     }
     if constexpr (EMIT) {
       a.rho[idx] = rho;
@@ -335,7 +344,7 @@ __global__ __launch_bounds__(256) void k_q7_macro(const S *__restrict__ buf, con
   t.qy[idx] = g[3] - g[4];
   t.qz[idx] = g[5] - g[6];
   int bad = (rho < C(0) || rho > C(100)) ? ST_RHO_RANGE : 0;
-  if (!(fabs(T) <= C(100))) bad |= ST_T_RANGE;
+  if (!(fabs(T) <= C(100)) || beyond_storage<S>(T - t.T0)) bad |= ST_T_RANGE;
   if (bad) flag(a.status, a.first_bad, bad, idx);
 }
 

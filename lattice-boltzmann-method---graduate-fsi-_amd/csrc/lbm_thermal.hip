@@ -36,6 +36,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "lbm_ctx.h"
@@ -83,6 +84,15 @@ __device__ __forceinline__ typename Store<S>::C g_raw(typename Store<S>::C val, 
   if constexpr (Store<S>::K != C(1)) return (val - C(w) * T0) * C(Store<S>::K);
   else if constexpr (Store<S>::shifted) return val - C(w) * T0;
   else return val;
+}
+
+// F16C holds |g - w T0| < 0.5, at the rest weight 1/3 |rho T - T0| < 1.5: a
+// cell with |T - T0| past 1.4 flags the temperature range rather than let
+// its g saturate.
+template <class S>
+__device__ __forceinline__ bool beyond_storage(typename Store<S>::C T, typename Store<S>::C T0) {
+  if constexpr (std::is_same<S, f16c_t>::value) return !(fabs(T - T0) < 1.4f);
+  else return false;
 }
 
 // 1 + cubic Hermite terms of Calc_feq / Calc_geq (lbm.cpp:628-653), cs2 = 1/3
@@ -251,7 +261,7 @@ __global__ __launch_bounds__(256) void k_thermal(const Planes<S> Pf, const Plane
     });
     T = Ts;
     if (rho < C(0) || rho > C(100)) bad |= ST_RHO_RANGE;
-    if (T < C(0) || T > C(100)) bad |= ST_T_RANGE;
+    if (T < C(0) || T > C(100) || beyond_storage<S>(T, t.T0)) bad |= ST_T_RANGE;
     if constexpr (BQ) {
       Fy = t.g_beta * (T - t.T_ref);  // lbm.cpp:803-806
       t.Fy[idx] = Fy;

@@ -204,6 +204,19 @@ def test_q7_random_fp16(oracle_mod, storage, tol_u, tol_T):
     lat.close()
 
 
+def test_q7_f16c_flags_temperature_beyond_its_range():
+    """F16C holds |g - w T0| < 0.5, |T - T0| < 2 at the rest weight 1/4: a
+    cell past 1.9 flags LBM_ERR_TEMPERATURE_RANGE rather than let its g
+    saturate.  The fluid at 0 (T0 = 0) heated by a wall at 5 passes it
+    within a few steps; F16S (|g - w T0| < 16) runs the case."""
+    case, th = cases.q7_conduction(Nx=3, Ny=8, Nz=2, tau_g=0.9, Tb=5.0, Tt=0.0, T_fluid=0.0)
+    for st, want in (("f16c", lbm_amd.LBM_ERR_TEMPERATURE_RANGE), ("f16s", 0), ("f32", 0)):
+        lat = _gpu(case, th, storage=st)
+        lat.step(50)
+        assert lat.sync() == want, st
+        lat.close()
+
+
 @pytest.mark.parametrize("storage,tol", [("f16s", 1e-2), ("f16c", 5e-3)])
 def test_q7_conduction_fp16_steady_profile(storage, tol):
     """16-bit storage: the steady Dirichlet profile between walls at 1 and

@@ -140,8 +140,9 @@ def test_thermal_rayleigh_benard_fp16(oracle_mod, storage, steps, tol_T, tol_u):
     oracle: T to `tol_T`, u to `tol_u` relative L2.  This small case heats
     up past its walls (T = 2.0 at 400 steps, 5.6 at 1000, the oracle's as
     the device's), and F16C holds |g - w T0| < 0.5 only, |T - T0| < 1.5 at
-    the rest weight 1/3: beyond it its values saturate, so F16C is compared
-    over the first 300 steps and F16S (|g - w T0| < 16) over 1000.  (A
+    the rest weight 1/3 (past 1.4 it flags the temperature range, next
+    test), so F16C is compared over the first 300 steps and F16S
+    (|g - w T0| < 16) over 1000.  (A
     viscous-heating rise of 1e-3 of T, as in the Couette case, is below
     what 16-bit g resolve at a steady state: each step rounds the same
     populations the same way, and the error grows with the diffusion time,
@@ -162,6 +163,18 @@ def test_thermal_rayleigh_benard_fp16(oracle_mod, storage, steps, tol_T, tol_u):
     assert eT <= tol_T
     assert eu <= tol_u
     lat.close()
+
+
+def test_thermal_f16c_flags_temperature_beyond_its_range():
+    """The Rayleigh-Benard case past |T - T0| = 1.4 (T0 = 1, T = 2.4 by
+    about 500 steps): F16C reports LBM_ERR_TEMPERATURE_RANGE instead of
+    saturating its g; F16S holds the same run."""
+    case, th = CASES["rayleigh_benard"]()
+    for st, want in (("f16c", lbm_amd.LBM_ERR_TEMPERATURE_RANGE), ("f16s", 0)):
+        lat = _gpu(case, th, storage=st)
+        lat.step(1000)
+        assert lat.sync() == want, st
+        lat.close()
 
 
 @pytest.mark.parametrize("storage,tol_T,tol_u", [("f16s", 3e-4, 5e-4), ("f16c", 1e-4, 3e-4)])
@@ -281,12 +294,14 @@ def test_thermal_face_loops_fp32(oracle_mod):
     lat.close()
 
 
-@pytest.mark.parametrize("storage,tol_T,tol_u", [("f16s", 3e-5, 3e-4), ("f16c", 3e-5, 2e-4)])
-def test_thermal_face_loops_fp16(oracle_mod, storage, tol_T, tol_u):
+@pytest.mark.parametrize("storage,steps,tol_T,tol_u", [("f16s", 100, 3e-5, 3e-4), ("f16c", 80, 3e-5, 2e-4)])
+def test_thermal_face_loops_fp16(oracle_mod, storage, steps, tol_T, tol_u):
     """16-bit storage with the ENERGY face loops (their slots stay FP64)
-    against the FP64 oracle, 100 steps, every non-solid cell."""
+    against the FP64 oracle, every non-solid cell.  The inflow writes raise
+    T in this case (|T - T0| = 1.24 after 80 steps, 1.92 after 100, where
+    some g - w T0 reach 0.63): F16C, which holds |g - w T0| < 0.5 and flags
+    |T - T0| > 1.4, is compared after 80 steps."""
     case, th = cases.thermal_face_case(**FACE_TH["shell"])
-    steps = 100
     o = _oracle(oracle_mod, case, th)
     assert o.run(steps) == 0
     lat = _gpu(case, th, storage=storage)
