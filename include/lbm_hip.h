@@ -154,8 +154,11 @@ int lbm_reduce_fp64_noalloc(lbm_ctx *ctx, int32_t what, double *out);
  * lbm_upload_thermal, lbm_init), after which steps give the saved context's
  * results bit for bit.  LBM_ERR_ARG when the checkpoint does not match the
  * context (lattice, storage, thermal model, inlet/outlet cells, walls, force
- * field); LBM_ERR_STATE before lbm_init; LBM_ERR_UNSUPPORTED for a context
- * with an immersed boundary. */
+ * field); LBM_ERR_STATE before lbm_init.  An immersed rigid body that has
+ * never moved is covered between lbm_ibm_run/apply calls (its IBM steps
+ * depend on the populations only; the restoring context repeats the same
+ * lbm_ibm_setup); LBM_ERR_UNSUPPORTED for a body that moved, a pending
+ * IBLBM result and a coupled filament. */
 int lbm_checkpoint_size(lbm_ctx *ctx, uint64_t *bytes);
 int lbm_checkpoint_save(lbm_ctx *ctx, void *buf, uint64_t bytes);
 int lbm_checkpoint_load(lbm_ctx *ctx, const void *buf, uint64_t bytes);

@@ -10,9 +10,12 @@
 // arrays and the running statistics, with the context's state flags.  It is
 // restored into a context built by the same set-up (lbm_create,
 // lbm_upload_cells, lbm_upload_thermal, lbm_init): stepping it then gives the
-// same results, bit for bit, as stepping the saved context.  Contexts with an
-// immersed boundary are not covered (their markers, box and passes are state
-// of their own).
+// same results, bit for bit, as stepping the saved context.  A context with an
+// immersed boundary is covered while its body is rigid and has never moved,
+// between IBLBM calls: every IBM step then recomputes the markers' velocities
+// and forces from the populations, and the restoring context repeats the
+// same lbm_ibm_setup.  Moving bodies and filaments (their marker positions
+// and nodes are state of their own) are not.
 
 #include <hip/hip_runtime.h>
 
@@ -20,6 +23,8 @@
 #include <vector>
 
 #include "lbm_ctx.h"
+
+bool ibm_checkpointable(const lbm_ctx *c);
 
 namespace {
 
@@ -114,7 +119,7 @@ extern "C" {
 int lbm_checkpoint_size(lbm_ctx *c, uint64_t *bytes) {
   if (!c || !bytes) return LBM_ERR_ARG;
   if (c->state == ST_NONE) return LBM_ERR_STATE;
-  if (c->ibm) return LBM_ERR_UNSUPPORTED;
+  if (!ibm_checkpointable(c)) return LBM_ERR_UNSUPPORTED;
   *bytes = header_of(c).total;
   return LBM_OK;
 }
@@ -122,7 +127,7 @@ int lbm_checkpoint_size(lbm_ctx *c, uint64_t *bytes) {
 int lbm_checkpoint_save(lbm_ctx *c, void *buf, uint64_t bytes) {
   if (!c || !buf) return LBM_ERR_ARG;
   if (c->state == ST_NONE) return LBM_ERR_STATE;
-  if (c->ibm) return LBM_ERR_UNSUPPORTED;
+  if (!ibm_checkpointable(c)) return LBM_ERR_UNSUPPORTED;
   const CkptHeader h = header_of(c);
   if (bytes < h.total) return LBM_ERR_ARG;
   HIPCHK(hipSetDevice(c->dev));
@@ -140,7 +145,7 @@ int lbm_checkpoint_save(lbm_ctx *c, void *buf, uint64_t bytes) {
 int lbm_checkpoint_load(lbm_ctx *c, const void *buf, uint64_t bytes) {
   if (!c || !buf || bytes < sizeof(CkptHeader)) return LBM_ERR_ARG;
   if (c->state == ST_NONE) return LBM_ERR_STATE;  // the set-up, lbm_init included, comes first
-  if (c->ibm) return LBM_ERR_UNSUPPORTED;
+  if (!ibm_checkpointable(c)) return LBM_ERR_UNSUPPORTED;
   CkptHeader h;
   std::memcpy(&h, buf, sizeof h);
   const CkptHeader mine = header_of(c);

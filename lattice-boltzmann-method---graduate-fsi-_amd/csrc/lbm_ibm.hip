@@ -1403,6 +1403,7 @@ struct lbm_ibm {
   double *out = nullptr;  // 3 per step of a run
   int out_cap = 0;
   int ready = 0;          // IBLBM done, the next collision uses force mode 3
+  int displaced = 0;      // some step moved the markers (their positions are state)
   std::vector<void *> allocs;  // marker arrays, freed with the state
   // coupled filament (lbm_fsi_setup): its nodes are the markers, with their
   // own desired velocities and the density of the cell they sit in
@@ -1876,6 +1877,7 @@ static int build_neighbours(lbm_ibm *m, const double *x, const double *y, const 
 static int ibm_move(lbm_ctx *c, const double vcm[3], const double vb[3]) {
   lbm_ibm *m = c->ibm;
   const bool moved = vcm[0] + vb[0] != 0 || vcm[1] + vb[1] != 0 || vcm[2] + vb[2] != 0;
+  if (moved) m->displaced = 1;
   if (moved && m->rows_static) {
     // a body set up at rest was given a velocity: its rows by stencil bases
     // no longer cover every overlapping pair; the rows of a rigid motion
@@ -2421,4 +2423,12 @@ bool ibm_active(const lbm_ctx *c) { return c && c->ibm; }
 bool ibm_ready(const lbm_ctx *c) { return c && c->ibm && c->ibm->ready; }
 void ibm_consume(lbm_ctx *c) {
   if (c && c->ibm) c->ibm->ready = 0;
+}
+
+// Whether a checkpoint of the lattice alone resumes this context's IBM runs:
+// a rigid body that never moved, between IBLBM calls (every IBM step
+// recomputes its markers' velocities and forces from the populations).
+bool ibm_checkpointable(const lbm_ctx *c) {
+  const lbm_ibm *m = c->ibm;
+  return !m || (!m->fil && !m->displaced && !m->ready && !m->moving);
 }

@@ -140,10 +140,41 @@ def test_checkpoint_rejects_mismatch_and_ibm():
     with pytest.raises(lbm_amd.LBMError):
         c.checkpoint()
     c.close()
-    # a lattice with an immersed boundary
+    # an immersed body that moved: its marker positions are state of their own
     icase, mk, box = cases.ibm_cylinder(36, 24, 2, D=8.0, Re=20.0, center=(14.0, 12.0))
     d = _plain(icase, "f32")
     d.ibm_setup(mk, box, m_max=4)
+    d.ibm_run(2, vcm=np.tile([0.001, 0.0, 0.0], (3, 1)))
     with pytest.raises(lbm_amd.LBMError):
         d.checkpoint()
     d.close()
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_resume_ibm_cylinder_at_rest(storage):
+    """A rigid body at rest: every IBM step recomputes the markers' u_b and
+    F_b from the populations, so the lattice state resumes the IBM run; the
+    restoring lattice repeats the same lbm_ibm_setup.  Fields and the drag,
+    lift and interior density series bit for bit."""
+    icase, mk, box = cases.ibm_cylinder(36, 24, 2, D=8.0, Re=20.0, center=(14.0, 12.0))
+
+    def make():
+        lat = _plain(icase, storage)
+        lat.ibm_setup(mk, box, m_max=4)
+        return lat
+
+    a = make()
+    a.ibm_run(15)
+    assert a.sync() == 0
+    ck = a.checkpoint()
+    want_out = a.ibm_run(10)
+    assert a.sync() == 0
+    want = _state(a, False)
+    a.close()
+    b = make()
+    b.restore(ck)
+    got_out = b.ibm_run(10)
+    assert b.sync() == 0
+    np.testing.assert_array_equal(got_out, want_out)
+    _same(_state(b, False), want)
+    b.close()
